@@ -1,0 +1,168 @@
+/*
+ * neb_hip.h -- C ABI of libnebhip.so, the gfx950 (MI355X) backend for the
+ * neural eigenfluid network hot path of baiming-zhang/NEB.
+ *
+ * The reference has no compiled code: its hot path is CuPy array code whose
+ * matmuls/einsums go to cuBLAS and whose fused sine-jet chain is an NVRTC
+ * ElementwiseKernel.  Each entry point below replaces one of those operation
+ * groups; the reference interface it stands in for is cited (paths relative
+ * to the reference root):
+ *   HB2P  = experiments/2d/_backend/periodic_core/vortex_merger/src/eigenfluid/
+ *           gen/vortex_merger2d/hardcoded_backend.py
+ *   VJ2   = experiments/metaball/_backend/flow/cases/metaball/velocity_trace_jet2.py
+ *   NCC   = experiments/metaball/_backend/flow/cases/metaball/native_cupy_core.py
+ *   VCORE = experiments/metaball/_backend/flow/cases/metaball/velocity/core.py
+ *   FUSED = experiments/metaball/_backend/flow/cases/metaball/velocity/fused.py
+ *   FAST  = experiments/metaball/_backend/flow/cases/metaball/velocity/fast_path.py
+ *   PP    = experiments/metaball/_backend/exports/basis_projection/postprocessing/
+ *           paired_projection.py
+ *
+ * Conventions
+ *  - Every buffer is caller-owned device memory passed as a plain pointer;
+ *    the library allocates nothing.  Scratch space for split reductions is
+ *    a caller-provided workspace sized by the *_workspace_size functions.
+ *  - Calls are stream-ordered on `stream` (a hipStream_t, NULL = default
+ *    stream) and never synchronize the device.
+ *  - Return 0 on success or an NEB_ERR_* code; neb_last_error() returns a
+ *    thread-local message.  SHAPE/ARG map to ValueError, HIP to RuntimeError
+ *    (the reference's exception classes, HB2P:3299-3305, VCORE:60-61); no C++
+ *    exception crosses the ABI.
+ *  - Packed jets: a second-order jet of width f over n points is stored
+ *    channel-major [C][n][ld] with channel stride n*ld, channels ordered
+ *    value, d grads, d(d+1)/2 Hessian entries in the pair order
+ *    (0,0),(0,1),(1,1) [2D] or (0,0),(0,1),(0,2),(1,1),(1,2),(2,2) [3D]
+ *    (HB2P:154-166, VJ2:9,20-22).  C = 6 in 2D, 10 in 3D.
+ *  - prec selects the storage type of jets / staged weights and the MFMA
+ *    instruction: NEB_BF16 (bf16 operands, fp32 accumulate; fields, biases
+ *    and epilogue math in fp32), NEB_F32 (fp32), NEB_F64 (fp64).
+ *  - Leading dimensions are in elements and must keep 16-byte alignment of
+ *    every row (multiples of 8 for bf16, 4 for fp32, 2 for fp64).
+ */
+#ifndef NEB_HIP_H
+#define NEB_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NEB_ABI_VERSION 1
+
+enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
+enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
+enum { NEB_TANH = 0, NEB_SIN = 1 };                           /* activations */
+enum { NEB_CROSS_F32 = 0, NEB_CROSS_BF16 = 1, NEB_CROSS_BF16X3 = 2, NEB_CROSS_F64 = 3 };
+
+int neb_abi_version(void);
+const char* neb_last_error(void);
+
+/* Periodic sin/cos feature jets.  Replaces encoding_jet2 periodic branch
+ * (HB2P:566-589) / encoding_jet1 (HB2P:418-441) and features1 (NCC:76-82).
+ * x: [n][dim] fp32 (F32/BF16) or fp64 (F64).  out: packed [C][n][ldf], the
+ * 2*freqs*dim features followed by zero padding up to ldf. */
+int neb_feature_jets(int prec, int dim, int64_t n, const void* x, int freqs, const double* lo,
+                     const double* hi, void* out, int64_t ldf, void* stream);
+
+/* Hidden layer forward: z = omega*(x W^T + b) on all channels (bias on the
+ * value channel only), y = act-jet(z).  Replaces linear_forward_batch_jet2 +
+ * omega scaling + sine_forward_jet2 (HB2P:1005-1040, :1836-1837) and
+ * VJ2.linear/scaled/sine (VJ2:17,26-34) with the FUSED sine kernel
+ * (FUSED:3-8).  x: packed [C][n][ldx] (k valid columns), W: [width][ldw],
+ * bias: math precision [width] (fp32, fp64 for F64) or NULL.
+ * z, y: packed [C][n][ldy]. */
+int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* x,
+                          int64_t ldx, const void* w, int64_t ldw, const void* bias, double omega, void* z,
+                          void* y, int64_t ldy, void* stream);
+
+/* Output layer, raw jets: out = y W^T (+ bias on the value channel),
+ * packed [C][n][ldo] in fp32 (fp64 for F64).  Replaces the last
+ * linear_forward_batch_jet2 of forward_stream_jet2 (HB2P:1802-1880) /
+ * VJ2.forward (VJ2:65-76). */
+int neb_jet_output_jets(int prec, int dim, int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy,
+                        const void* w, int64_t ldw, const void* bias, void* out, int64_t ldo, void* stream);
+
+/* Output layer fused with the differential lift, writing field matrices in
+ * fp32 (fp64 for F64).  2D: w [K][ldw]; mass = (psi_y, -psi_x) as [2n][ldf]
+ * and stiff = omega = -(psi_xx+psi_yy) as [n][ldf] (ELC:35-38 =
+ * .../canonical_gauge/scripts/evaluate_large_comparison.py).  3D: w [3K][ldw]
+ * with rows 3k+c (vector potential); mass = curl A as [3n][ldf], stiff =
+ * grad u as [9n][ldf] rows 3a+j = d_j u_a (NCC:97-99, VJ2:93-101,
+ * VCORE:71-77).  layout 0 = point-major rows (p*comp+c, the reference's
+ * [N*comp, K] layout, VCORE:75-76); 1 = component-major (c*n+p). */
+int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy,
+                          const void* w, int64_t ldw, void* mass, void* stiff, int64_t ldf, int layout,
+                          void* stream);
+
+/* Output-layer reverse fused with the lift transpose and the transpose of
+ * the last hidden activation.  Given field matrices Phi (mass, stiff) and
+ * bm = 2w W^T dL/dM, bs = 2w W^T dL/dS ([width][ldb]; in 3D three
+ * consecutive blocks, one per component c, each [width][ldb] built from
+ * W rows 3k+c), computes the adjoint of the last pre-activation jet:
+ * adj = omega * act-jet^T(z, lift^T(2w Phi dL/dG) W).  Replaces
+ * VCORE:94-100 (gu = U Cu/n, lift) + VJ2.product_reverse/linear_reverse +
+ * FUSED:9-19 (and HB2P:1115-1174 in 2D).  z: tape packed [C][n][ldz];
+ * adj: packed [C][n][lda]. */
+int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width, int64_t K, const void* mass,
+                            const void* stiff, int64_t ldf, int layout, const void* bm, const void* bs,
+                            int64_t ldb, const void* z, int64_t ldz, double omega, void* adj, int64_t lda,
+                            void* stream);
+
+/* Hidden layer reverse: adj_out = omega_prev * act-jet^T(z_prev, adj_in W).
+ * wt = W^T staged as [k][ldw] (k = layer input width).  Replaces
+ * linear_backward_batch_jet2's input adjoint (HB2P:1115-1144), VJ2:28-30,
+ * then sine_backward_jet2 / FUSED:9-19 for the previous layer. */
+int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* adj_in,
+                           int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev, int64_t ldz,
+                           double omega_prev, void* adj_out, int64_t lda_out, void* stream);
+
+/* Split-R cross product  G = beta*G + scale * sum_s sign_s X_s^T diag(roww) Y_s
+ * over rows, FP64 result [m][ldg].  Segment s: X_s [rows_s][ldx_s] with m
+ * used columns, Y_s [rows_s][ldy_s] with n used columns; x_dtype / y_dtype
+ * are NEB_F32 / NEB_BF16 / NEB_F64.  symmetric != 0 requires X == Y and
+ * computes upper tiles only.  Replaces the FP64 Gram blocks of VCORE:40-46,
+ * experiments/metaball/run.py:99-107, experiments/periodic/paper_rollout.py:
+ * 178-179, the weight-gradient einsums HB2P:1087-1144 / VJ2:29 / FAST:27-29
+ * and the projection loads PP:148-154.  roww (fp64, per row) is allowed with
+ * a single segment only. */
+size_t neb_cross_workspace_size(int64_t m, int64_t n, int64_t rows, int symmetric);
+int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, const int64_t* ldx,
+              const int64_t* ldy, const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m,
+              int64_t n, const double* roww, int symmetric, double scale, double beta, double* G, int64_t ldg,
+              void* workspace, size_t ws_bytes, void* stream);
+
+/* Quadrature-weighted Gram  G = beta*G + w * Phi^T diag(roww) Phi  (FP64,
+ * full symmetric K x K).  Phi: [rows][ld], K columns, dtype phi_dtype.
+ * Replaces VCORE:40-46 `grams` and run.py:99-116 `fullmass`. */
+int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int phi_dtype, double w,
+             const double* roww, double beta, double* G, int64_t ldg, void* workspace, size_t ws_bytes,
+             void* stream);
+
+/* Column sums in FP64: out[c] = beta*out[c] + sum_r x[r][c] (bias
+ * gradients, HB2P:1101 `grad_bias`, VJ2:30).  Workspace from
+ * neb_colsum_workspace_size. */
+size_t neb_colsum_workspace_size(int64_t rows, int64_t cols);
+int neb_colsum(int dtype, const void* x, int64_t rows, int64_t cols, int64_t ld, double beta, double* out,
+               void* workspace, size_t ws_bytes, void* stream);
+
+/* Sum of squares in FP64 into device *out (gradient norm of the Adam clip,
+ * NCC:157).  Workspace: 256 doubles. */
+int neb_sumsq(int dtype, const void* x, int64_t n, double* out, void* workspace, size_t ws_bytes, void* stream);
+
+/* Adam step with bias correction on flat fp32/fp64 arrays, gradients scaled
+ * by `scale` (the clip factor min(1, clip/|g|)).  Replaces Model.adam
+ * (NCC:156-171) / adam_step_jet1_device (HB2P:2184-2278). */
+int neb_adam(int dtype, void* p, const void* g, void* m, void* v, int64_t n, double lr, double scale, double b1,
+             double b2, double eps, int64_t t, void* stream);
+
+/* Weight staging: dst[r][c] = scale * (transpose ? src[c][r] : src[r][c])
+ * converted from src_dtype to dst_dtype, zero outside the source extent
+ * (dst is drows x dcols).  Replaces the per-call `.astype(compute_dtype)`
+ * of the master weights (HB2P:1026-1030, VJ2:68). */
+int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst,
+              int64_t ldd, int64_t drows, int64_t dcols, int transpose, double scale, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEB_HIP_H */

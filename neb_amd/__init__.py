@@ -1,0 +1,15 @@
+"""neb_amd -- MI355X (gfx950) backend for the neural eigenfluid network hot path.
+
+Analytic second-order spatial jets of a shared tanh/sin trunk, the lifted
+velocity / vorticity / velocity-gradient fields, quadrature-weighted FP64
+Grams Phi^T W Phi, the spectral objective and the handwritten parameter
+reverse pass, as hand-written HIP kernels behind the C ABI of
+include/neb_hip.h (libnebhip.so).
+"""
+from .network import (EigenfluidNet, Fields, NetConfig, Tape, backward_fields, channels, cross, gram,
+                      output_fields, output_jets, trunk_forward)
+from .spectral import JetGram, compute, generalized_eigh, project, spectral_objective
+
+__all__ = ["EigenfluidNet", "Fields", "NetConfig", "Tape", "backward_fields", "channels", "cross", "gram",
+           "output_fields", "output_jets", "trunk_forward", "JetGram", "compute", "generalized_eigh", "project",
+           "spectral_objective"]

@@ -1,0 +1,544 @@
+// extern "C" entry points of libnebhip.so (declared in include/neb_hip.h).
+// Argument validation, template dispatch and kernel launches; no allocation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/neb_hip.h"
+#include "crossgemm.hpp"
+#include "jet_rowgemm.hpp"
+#include "misc_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(NEB_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+  return NEB_OK;
+}
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+size_t elem_size(int dtype) { return dtype == NEB_BF16 ? 2 : (dtype == NEB_F32 ? 4 : 8); }
+
+bool aligned(const void* p, int64_t ld, int dtype) {
+  const size_t es = elem_size(dtype);
+  return (reinterpret_cast<uintptr_t>(p) % 16 == 0) && ((ld * (int64_t)es) % 16 == 0);
+}
+
+template <typename K>
+int set_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return fail(NEB_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+  }
+  return NEB_OK;
+}
+
+template <int P, typename TIN, class Plan>
+int launch_rowgemm(const RowArgs& a, hipStream_t st, const char* what) {
+  constexpr int TN = Plan::WPN * Plan::FB * 16, TP = Plan::WPP * Plan::PB * 16;
+  const size_t lds = rowgemm_lds_bytes<P, Plan>();
+  auto kern = rowgemm_kernel<P, TIN, Plan>;
+  if (int rc = set_lds(kern, lds)) return rc;
+  const int64_t gx = (a.npts + TP - 1) / TP, gy = (a.nout + TN - 1) / TN;
+  if (gx > 0x7fffffff || gy > 65535) return fail(NEB_ERR_SHAPE, "%s: grid too large", what);
+  if (gx == 0 || gy == 0) return NEB_OK;
+  hipLaunchKernelGGL(kern, dim3((unsigned)gx, (unsigned)gy), dim3(256), lds, st, a);
+  return hip_status(what);
+}
+
+template <int P> struct FieldT { using type = float; };
+template <> struct FieldT<PREC_F64> { using type = double; };
+
+RowArgs row_args() {
+  RowArgs a;
+  memset(&a, 0, sizeof(a));
+  return a;
+}
+
+int check_common(int prec, int dim, int act) {
+  if (prec != NEB_F32 && prec != NEB_BF16 && prec != NEB_F64) return fail(NEB_ERR_ARG, "unknown prec %d", prec);
+  if (dim != 2 && dim != 3) return fail(NEB_ERR_ARG, "dim must be 2 or 3 (got %d)", dim);
+  if (act != NEB_TANH && act != NEB_SIN) return fail(NEB_ERR_ARG, "unknown activation %d", act);
+  return NEB_OK;
+}
+
+// --------------------------------------------------------------------------
+template <int P, int D, int ACT>
+int layer_forward(int64_t n, int64_t k, int64_t width, const void* x, int64_t ldx, const void* w, int64_t ldw,
+                  const void* bias, double omega, void* z, void* y, int64_t ldy, hipStream_t st) {
+  constexpr int C = Jet2Layout<D>::C;
+  using Sx = typename Prec<P>::store_t;
+  RowArgs a = row_args();
+  for (int c = 0; c < C; ++c) {
+    a.x[c] = reinterpret_cast<const Sx*>(x) + (int64_t)c * n * ldx;
+    a.ldx[c] = ldx;
+  }
+  a.w[0] = w; a.ldw[0] = ldw;
+  a.npts = n; a.nout = width; a.k = k;
+  a.bias = bias; a.omega = omega;
+  a.out0 = z; a.out1 = y; a.ldo = ldy; a.cstride = n * ldy;
+  return launch_rowgemm<P, Sx, PlanHiddenFwd<D, ACT>>(a, st, "neb_jet_layer_forward");
+}
+
+template <int P, int D>
+int output_jets(int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy, const void* w, int64_t ldw,
+                const void* bias, void* out, int64_t ldo, hipStream_t st) {
+  constexpr int C = Jet2Layout<D>::C;
+  using Sx = typename Prec<P>::store_t;
+  RowArgs a = row_args();
+  for (int c = 0; c < C; ++c) {
+    a.x[c] = reinterpret_cast<const Sx*>(y) + (int64_t)c * n * ldy;
+    a.ldx[c] = ldy;
+  }
+  a.w[0] = w; a.ldw[0] = ldw;
+  a.npts = n; a.nout = nout; a.k = k;
+  a.bias = bias;
+  a.out0 = out; a.ldo = ldo; a.cstride = n * ldo;
+  return launch_rowgemm<P, Sx, PlanOutJets<D>>(a, st, "neb_jet_output_jets");
+}
+
+template <int P>
+int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy, const void* w, int64_t ldw,
+                  void* mass, void* stiff, int64_t ldf, int layout, hipStream_t st) {
+  using Sx = typename Prec<P>::store_t;
+  const Sx* Y = reinterpret_cast<const Sx*>(y);
+  const int64_t cs = n * ldy;
+  RowArgs a = row_args();
+  a.npts = n; a.nout = K; a.k = k;
+  a.out0 = mass; a.out1 = stiff; a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
+  if (dim == 2) {
+    // channels: 0 v, 1 gx, 2 gy, 3 hxx, 4 hxy, 5 hyy
+    const int chans[4] = {1, 2, 3, 5};
+    for (int c = 0; c < 4; ++c) { a.x[c] = Y + chans[c] * cs; a.ldx[c] = ldy; }
+    a.w[0] = w; a.ldw[0] = ldw;
+    return launch_rowgemm<P, Sx, PlanOutFields2>(a, st, "neb_jet_output_fields");
+  }
+  for (int c = 0; c < 9; ++c) { a.x[c] = Y + (1 + c) * cs; a.ldx[c] = ldy; }
+  for (int s = 0; s < 3; ++s) {
+    a.w[s] = reinterpret_cast<const Sx*>(w) + s * ldw;
+    a.ldw[s] = 3 * ldw;
+  }
+  return launch_rowgemm<P, Sx, PlanOutFields3>(a, st, "neb_jet_output_fields");
+}
+
+template <int P, int ACT>
+int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* mass, const void* stiff, int64_t ldf,
+                    int layout, const void* bm, const void* bs, int64_t ldb, const void* z, int64_t ldz,
+                    double omega, void* adj, int64_t lda, hipStream_t st) {
+  using Sx = typename Prec<P>::store_t;
+  using F = typename FieldT<P>::type;
+  const F* Mf = reinterpret_cast<const F*>(mass);
+  const F* Sf = reinterpret_cast<const F*>(stiff);
+  RowArgs a = row_args();
+  a.npts = n; a.nout = width; a.k = K;
+  a.tape = z; a.ldt = ldz; a.tstride = n * ldz;
+  a.omega = omega;
+  a.out0 = adj; a.ldo = lda; a.cstride = n * lda;
+  if (dim == 2) {
+    if (layout == 0) {
+      a.x[0] = Mf; a.ldx[0] = 2 * ldf;
+      a.x[1] = Mf + ldf; a.ldx[1] = 2 * ldf;
+    } else {
+      a.x[0] = Mf; a.ldx[0] = ldf;
+      a.x[1] = Mf + n * ldf; a.ldx[1] = ldf;
+    }
+    a.x[2] = Sf; a.ldx[2] = ldf;
+    a.w[0] = bm; a.ldw[0] = ldb;
+    a.w[1] = bs; a.ldw[1] = ldb;
+    return launch_rowgemm<P, F, PlanOutBwd2A<ACT>>(a, st, "neb_jet_output_backward");
+  }
+  for (int c = 0; c < 3; ++c) {
+    a.x[c] = layout == 0 ? Mf + c * ldf : Mf + c * n * ldf;
+    a.ldx[c] = layout == 0 ? 3 * ldf : ldf;
+  }
+  for (int c = 0; c < 9; ++c) {
+    a.x[3 + c] = layout == 0 ? Sf + c * ldf : Sf + c * n * ldf;
+    a.ldx[3 + c] = layout == 0 ? 9 * ldf : ldf;
+  }
+  for (int s = 0; s < 3; ++s) {
+    a.w[s] = reinterpret_cast<const Sx*>(bm) + s * width * ldb;
+    a.w[3 + s] = reinterpret_cast<const Sx*>(bs) + s * width * ldb;
+    a.ldw[s] = ldb;
+    a.ldw[3 + s] = ldb;
+  }
+  return launch_rowgemm<P, F, PlanOutBwd3<ACT>>(a, st, "neb_jet_output_backward");
+}
+
+template <int P, int D, int ACT>
+int layer_backward(int64_t n, int64_t k, int64_t width, const void* adj_in, int64_t lda_in, const void* wt,
+                   int64_t ldw, const void* z_prev, int64_t ldz, double omega_prev, void* adj_out,
+                   int64_t lda_out, hipStream_t st) {
+  constexpr int C = Jet2Layout<D>::C;
+  using Sx = typename Prec<P>::store_t;
+  RowArgs a = row_args();
+  for (int c = 0; c < C; ++c) {
+    a.x[c] = reinterpret_cast<const Sx*>(adj_in) + (int64_t)c * n * lda_in;
+    a.ldx[c] = lda_in;
+  }
+  a.w[0] = wt; a.ldw[0] = ldw;
+  a.npts = n; a.nout = k; a.k = width;
+  a.tape = z_prev; a.ldt = ldz; a.tstride = n * ldz;
+  a.omega = omega_prev;
+  a.out0 = adj_out; a.ldo = lda_out; a.cstride = n * lda_out;
+  return launch_rowgemm<P, Sx, PlanHiddenBwd<D, ACT>>(a, st, "neb_jet_layer_backward");
+}
+
+// --------------------------------------------------------------------------
+int cross_tiles(int64_t m, int64_t n, int symmetric, int* tm, int* tn) {
+  *tm = (int)((m + CR_TM - 1) / CR_TM);
+  *tn = (int)((n + CR_TN - 1) / CR_TN);
+  return symmetric ? (*tm) * (*tm + 1) / 2 : (*tm) * (*tn);
+}
+
+int cross_splits(int ntiles, int64_t rows) {
+  // aim for >= 4 workgroups per CU over 256 CUs, each split >= 1024 rows
+  int s = (1024 + ntiles - 1) / ntiles;
+  int64_t maxs = (rows + 1023) / 1024;
+  if (s > maxs) s = (int)maxs;
+  if (s < 1) s = 1;
+  if (s > 4096) s = 4096;
+  return s;
+}
+
+template <int MODE, typename TX, typename TY>
+int launch_cross(CrossArgs& a, double scale, double beta, double* G, int64_t ldg, hipStream_t st) {
+  const size_t lds = cross_lds_bytes<MODE>();
+  auto kern = cross_kernel<MODE, TX, TY>;
+  if (int rc = set_lds(kern, lds)) return rc;
+  hipLaunchKernelGGL(kern, dim3(a.ntiles, a.splits), dim3(256), lds, st, a);
+  if (int rc = hip_status("neb_cross")) return rc;
+  const int64_t tot = a.m * a.n;
+  if (tot > 0) {
+    hipLaunchKernelGGL(cross_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a.partial,
+                       a.splits, a.ntiles, a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
+  }
+  return hip_status("neb_cross reduce");
+}
+
+template <int MODE>
+int cross_dispatch_types(int xd, int yd, CrossArgs& a, double scale, double beta, double* G, int64_t ldg,
+                         hipStream_t st) {
+  if constexpr (MODE == CR_F64) {
+    if (xd == NEB_F64 && yd == NEB_F64) return launch_cross<MODE, double, double>(a, scale, beta, G, ldg, st);
+    return fail(NEB_ERR_ARG, "NEB_CROSS_F64 needs fp64 operands");
+  } else if constexpr (MODE == CR_F32) {
+    if (xd == NEB_F32 && yd == NEB_F32) return launch_cross<MODE, float, float>(a, scale, beta, G, ldg, st);
+    return fail(NEB_ERR_ARG, "NEB_CROSS_F32 needs fp32 operands");
+  } else {
+    if (xd == NEB_F32 && yd == NEB_F32) return launch_cross<MODE, float, float>(a, scale, beta, G, ldg, st);
+    if (xd == NEB_BF16 && yd == NEB_BF16) return launch_cross<MODE, __bf16, __bf16>(a, scale, beta, G, ldg, st);
+    if (xd == NEB_F32 && yd == NEB_BF16) return launch_cross<MODE, float, __bf16>(a, scale, beta, G, ldg, st);
+    if (xd == NEB_BF16 && yd == NEB_F32) return launch_cross<MODE, __bf16, float>(a, scale, beta, G, ldg, st);
+    return fail(NEB_ERR_ARG, "bf16 cross modes need fp32/bf16 operands");
+  }
+}
+
+template <typename T> int colsum_launch(const void* x, int64_t rows, int64_t cols, int64_t ld, double beta,
+                                        double* out, double* part, int splits, hipStream_t st) {
+  const int64_t chunk = (rows + splits - 1) / splits;
+  hipLaunchKernelGGL(colsum_kernel<T>, dim3((unsigned)((cols + 63) / 64), splits), dim3(256), 0, st,
+                     reinterpret_cast<const T*>(x), ld, rows, cols, chunk, part);
+  if (int rc = hip_status("neb_colsum")) return rc;
+  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, st, part, splits, cols,
+                     out, beta);
+  return hip_status("neb_colsum reduce");
+}
+
+int colsum_splits(int64_t rows) {
+  int64_t s = (rows + 2047) / 2048;
+  if (s > 256) s = 256;
+  if (s < 1) s = 1;
+  return (int)s;
+}
+
+template <typename TS, typename TD>
+int stage_launch(const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst, int64_t ldd, int64_t drows,
+                 int64_t dcols, int transpose, double scale, hipStream_t st) {
+  const int64_t tot = drows * dcols;
+  if (tot == 0) return NEB_OK;
+  hipLaunchKernelGGL((stage_kernel<TS, TD>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<const TS*>(src), rows, cols, lds, reinterpret_cast<TD*>(dst), ldd, drows, dcols,
+                     transpose, scale);
+  return hip_status("neb_stage");
+}
+
+template <typename TS>
+int stage_dst(int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst, int64_t ldd,
+              int64_t drows, int64_t dcols, int transpose, double scale, hipStream_t st) {
+  switch (dst_dtype) {
+    case NEB_F32: return stage_launch<TS, float>(src, rows, cols, lds, dst, ldd, drows, dcols, transpose, scale, st);
+    case NEB_BF16: return stage_launch<TS, __bf16>(src, rows, cols, lds, dst, ldd, drows, dcols, transpose, scale, st);
+    case NEB_F64: return stage_launch<TS, double>(src, rows, cols, lds, dst, ldd, drows, dcols, transpose, scale, st);
+  }
+  return fail(NEB_ERR_ARG, "neb_stage: unknown dst dtype %d", dst_dtype);
+}
+
+}  // namespace
+
+// ==========================================================================
+extern "C" {
+
+int neb_abi_version(void) { return NEB_ABI_VERSION; }
+const char* neb_last_error(void) { return g_err.c_str(); }
+
+int neb_feature_jets(int prec, int dim, int64_t n, const void* x, int freqs, const double* lo, const double* hi,
+                     void* out, int64_t ldf, void* stream) {
+  if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
+  if (n < 0 || freqs < 1 || ldf < 2 * freqs * dim) return fail(NEB_ERR_SHAPE, "neb_feature_jets: bad sizes");
+  if (!x || !out || !lo || !hi) return fail(NEB_ERR_ARG, "neb_feature_jets: null pointer");
+  if (n == 0) return NEB_OK;
+  const int64_t tot = n * ldf;
+  const dim3 grid((unsigned)((tot + 255) / 256));
+  const double l2 = dim == 3 ? lo[2] : 0.0, h2 = dim == 3 ? hi[2] : 1.0;
+#define FEAT(D, TX, SS)                                                                                          \
+  hipLaunchKernelGGL((feature_kernel<D, TX, SS>), grid, dim3(256), 0, S(stream), reinterpret_cast<const TX*>(x), \
+                     n, freqs, lo[0], lo[1], l2, hi[0], hi[1], h2, reinterpret_cast<SS*>(out), ldf)
+  if (prec == NEB_F64) {
+    if (dim == 2) FEAT(2, double, double); else FEAT(3, double, double);
+  } else if (prec == NEB_F32) {
+    if (dim == 2) FEAT(2, float, float); else FEAT(3, float, float);
+  } else {
+    if (dim == 2) FEAT(2, float, __bf16); else FEAT(3, float, __bf16);
+  }
+#undef FEAT
+  return hip_status("neb_feature_jets");
+}
+
+#define DISPATCH_PDA(CALL)                                                        \
+  switch (prec) {                                                                 \
+    case NEB_BF16:                                                                \
+      if (dim == 2) { if (act == NEB_TANH) return CALL(PREC_BF16, 2, ACT_TANH); return CALL(PREC_BF16, 2, ACT_SIN); } \
+      if (act == NEB_TANH) return CALL(PREC_BF16, 3, ACT_TANH); return CALL(PREC_BF16, 3, ACT_SIN);                  \
+    case NEB_F32:                                                                 \
+      if (dim == 2) { if (act == NEB_TANH) return CALL(PREC_F32, 2, ACT_TANH); return CALL(PREC_F32, 2, ACT_SIN); }   \
+      if (act == NEB_TANH) return CALL(PREC_F32, 3, ACT_TANH); return CALL(PREC_F32, 3, ACT_SIN);                    \
+    default:                                                                      \
+      if (dim == 2) { if (act == NEB_TANH) return CALL(PREC_F64, 2, ACT_TANH); return CALL(PREC_F64, 2, ACT_SIN); }   \
+      if (act == NEB_TANH) return CALL(PREC_F64, 3, ACT_TANH); return CALL(PREC_F64, 3, ACT_SIN);                    \
+  }
+
+int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* x,
+                          int64_t ldx, const void* w, int64_t ldw, const void* bias, double omega, void* z, void* y,
+                          int64_t ldy, void* stream) {
+  if (int rc = check_common(prec, dim, act)) return rc;
+  if (n < 0 || k < 1 || width < 1 || ldx < k || ldw < k || ldy < width)
+    return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward: bad sizes");
+  if (!x || !w || !z || !y) return fail(NEB_ERR_ARG, "neb_jet_layer_forward: null pointer");
+  if (!aligned(x, ldx, prec) || !aligned(w, ldw, prec) || !aligned(y, ldy, prec) || !aligned(z, ldy, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_layer_forward: rows must be 16-byte aligned");
+#define CALL(P, D, A) layer_forward<P, D, A>(n, k, width, x, ldx, w, ldw, bias, omega, z, y, ldy, S(stream))
+  DISPATCH_PDA(CALL)
+#undef CALL
+}
+
+int neb_jet_output_jets(int prec, int dim, int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy,
+                        const void* w, int64_t ldw, const void* bias, void* out, int64_t ldo, void* stream) {
+  if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
+  if (n < 0 || k < 1 || nout < 1 || ldy < k || ldw < k || ldo < nout)
+    return fail(NEB_ERR_SHAPE, "neb_jet_output_jets: bad sizes");
+  if (!y || !w || !out) return fail(NEB_ERR_ARG, "neb_jet_output_jets: null pointer");
+  if (!aligned(y, ldy, prec) || !aligned(w, ldw, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_output_jets: rows must be 16-byte aligned");
+  hipStream_t st = S(stream);
+  if (prec == NEB_BF16) return dim == 2 ? output_jets<PREC_BF16, 2>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st)
+                                        : output_jets<PREC_BF16, 3>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st);
+  if (prec == NEB_F32) return dim == 2 ? output_jets<PREC_F32, 2>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st)
+                                       : output_jets<PREC_F32, 3>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st);
+  return dim == 2 ? output_jets<PREC_F64, 2>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st)
+                  : output_jets<PREC_F64, 3>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st);
+}
+
+int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy,
+                          const void* w, int64_t ldw, void* mass, void* stiff, int64_t ldf, int layout,
+                          void* stream) {
+  if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
+  if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
+    return fail(NEB_ERR_SHAPE, "neb_jet_output_fields: bad sizes");
+  if (!y || !w || !mass || !stiff) return fail(NEB_ERR_ARG, "neb_jet_output_fields: null pointer");
+  if (!aligned(y, ldy, prec) || !aligned(w, ldw, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_output_fields: rows must be 16-byte aligned");
+  hipStream_t st = S(stream);
+  if (prec == NEB_BF16) return output_fields<PREC_BF16>(dim, n, k, K, y, ldy, w, ldw, mass, stiff, ldf, layout, st);
+  if (prec == NEB_F32) return output_fields<PREC_F32>(dim, n, k, K, y, ldy, w, ldw, mass, stiff, ldf, layout, st);
+  return output_fields<PREC_F64>(dim, n, k, K, y, ldy, w, ldw, mass, stiff, ldf, layout, st);
+}
+
+int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width, int64_t K, const void* mass,
+                            const void* stiff, int64_t ldf, int layout, const void* bm, const void* bs, int64_t ldb,
+                            const void* z, int64_t ldz, double omega, void* adj, int64_t lda, void* stream) {
+  if (int rc = check_common(prec, dim, act)) return rc;
+  if (n < 0 || width < 1 || K < 1 || ldf < K || ldb < K || ldz < width || lda < width ||
+      (layout != 0 && layout != 1))
+    return fail(NEB_ERR_SHAPE, "neb_jet_output_backward: bad sizes");
+  if (!mass || !stiff || !bm || !bs || !z || !adj) return fail(NEB_ERR_ARG, "neb_jet_output_backward: null pointer");
+  const int fdt = prec == NEB_F64 ? NEB_F64 : NEB_F32;
+  if (!aligned(mass, ldf, fdt) || !aligned(stiff, ldf, fdt) || !aligned(bm, ldb, prec) || !aligned(bs, ldb, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_output_backward: rows must be 16-byte aligned");
+  hipStream_t st = S(stream);
+#define OB(P, A) output_backward<P, A>(dim, n, width, K, mass, stiff, ldf, layout, bm, bs, ldb, z, ldz, omega, adj, lda, st)
+  if (prec == NEB_BF16) return act == NEB_TANH ? OB(PREC_BF16, ACT_TANH) : OB(PREC_BF16, ACT_SIN);
+  if (prec == NEB_F32) return act == NEB_TANH ? OB(PREC_F32, ACT_TANH) : OB(PREC_F32, ACT_SIN);
+  return act == NEB_TANH ? OB(PREC_F64, ACT_TANH) : OB(PREC_F64, ACT_SIN);
+#undef OB
+}
+
+int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* adj_in,
+                           int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev, int64_t ldz,
+                           double omega_prev, void* adj_out, int64_t lda_out, void* stream) {
+  if (int rc = check_common(prec, dim, act)) return rc;
+  if (n < 0 || k < 1 || width < 1 || lda_in < width || ldw < width || ldz < k || lda_out < k)
+    return fail(NEB_ERR_SHAPE, "neb_jet_layer_backward: bad sizes");
+  if (!adj_in || !wt || !z_prev || !adj_out) return fail(NEB_ERR_ARG, "neb_jet_layer_backward: null pointer");
+  if (!aligned(adj_in, lda_in, prec) || !aligned(wt, ldw, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_layer_backward: rows must be 16-byte aligned");
+#define CALL(P, D, A) \
+  layer_backward<P, D, A>(n, k, width, adj_in, lda_in, wt, ldw, z_prev, ldz, omega_prev, adj_out, lda_out, S(stream))
+  DISPATCH_PDA(CALL)
+#undef CALL
+}
+
+size_t neb_cross_workspace_size(int64_t m, int64_t n, int64_t rows, int symmetric) {
+  int tm, tn;
+  const int nt = cross_tiles(m, n, symmetric, &tm, &tn);
+  const int sp = cross_splits(nt, rows);
+  return (size_t)sp * nt * CR_TM * CR_TN * sizeof(double);
+}
+
+int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, const int64_t* ldx, const int64_t* ldy,
+              const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m, int64_t n,
+              const double* roww, int symmetric, double scale, double beta, double* G, int64_t ldg, void* workspace,
+              size_t ws_bytes, void* stream) {
+  if (nseg < 1 || nseg > CR_MAXSEG) return fail(NEB_ERR_ARG, "neb_cross: 1..%d segments", CR_MAXSEG);
+  if (m < 1 || n < 1 || ldg < n) return fail(NEB_ERR_SHAPE, "neb_cross: bad output shape");
+  if (roww && nseg != 1) return fail(NEB_ERR_ARG, "neb_cross: row weights need a single segment");
+  if (symmetric && (m != n || nseg != 1 || x[0] != y[0] || ldx[0] != ldy[0] || x_dtype != y_dtype))
+    return fail(NEB_ERR_ARG, "neb_cross: symmetric mode needs X == Y");
+  if (!G || !workspace) return fail(NEB_ERR_ARG, "neb_cross: null output or workspace");
+  CrossArgs a;
+  memset(&a, 0, sizeof(a));
+  int64_t maxrows = 0;
+  for (int s = 0; s < nseg; ++s) {
+    if (!x[s] || !y[s] || rows[s] < 0 || ldx[s] < m || ldy[s] < n)
+      return fail(NEB_ERR_SHAPE, "neb_cross: bad segment %d", s);
+    a.seg[s].x = x[s]; a.seg[s].y = y[s];
+    a.seg[s].ldx = ldx[s]; a.seg[s].ldy = ldy[s];
+    a.seg[s].rows = rows[s]; a.seg[s].sign = sign ? sign[s] : 1.0;
+    if (rows[s] > maxrows) maxrows = rows[s];
+  }
+  a.nseg = nseg; a.m = m; a.n = n; a.roww = roww; a.symmetric = symmetric;
+  a.ntiles = cross_tiles(m, n, symmetric, &a.tiles_m, &a.tiles_n);
+  a.splits = cross_splits(a.ntiles, maxrows);
+  const size_t need = (size_t)a.splits * a.ntiles * CR_TM * CR_TN * sizeof(double);
+  if (ws_bytes < need) return fail(NEB_ERR_ARG, "neb_cross: workspace %zu < %zu bytes", ws_bytes, need);
+  a.partial = reinterpret_cast<double*>(workspace);
+  hipStream_t st = S(stream);
+  switch (mode) {
+    case NEB_CROSS_F32: return cross_dispatch_types<CR_F32>(x_dtype, y_dtype, a, scale, beta, G, ldg, st);
+    case NEB_CROSS_BF16: return cross_dispatch_types<CR_BF16>(x_dtype, y_dtype, a, scale, beta, G, ldg, st);
+    case NEB_CROSS_BF16X3: return cross_dispatch_types<CR_BF16X3>(x_dtype, y_dtype, a, scale, beta, G, ldg, st);
+    case NEB_CROSS_F64: return cross_dispatch_types<CR_F64>(x_dtype, y_dtype, a, scale, beta, G, ldg, st);
+  }
+  return fail(NEB_ERR_ARG, "neb_cross: unknown mode %d", mode);
+}
+
+int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int phi_dtype, double w,
+             const double* roww, double beta, double* G, int64_t ldg, void* workspace, size_t ws_bytes,
+             void* stream) {
+  const void* xs[1] = {phi};
+  const int64_t lds[1] = {ld};
+  const int64_t rs[1] = {rows};
+  const double sg[1] = {1.0};
+  return neb_cross(mode, 1, xs, xs, lds, lds, rs, sg, phi_dtype, phi_dtype, K, K, roww, 1, w, beta, G, ldg,
+                   workspace, ws_bytes, stream);
+}
+
+size_t neb_colsum_workspace_size(int64_t rows, int64_t cols) {
+  return (size_t)colsum_splits(rows) * (size_t)cols * sizeof(double);
+}
+
+int neb_colsum(int dtype, const void* x, int64_t rows, int64_t cols, int64_t ld, double beta, double* out,
+               void* workspace, size_t ws_bytes, void* stream) {
+  if (rows < 0 || cols < 1 || ld < cols) return fail(NEB_ERR_SHAPE, "neb_colsum: bad sizes");
+  if (!x || !out || !workspace) return fail(NEB_ERR_ARG, "neb_colsum: null pointer");
+  const int sp = colsum_splits(rows);
+  if (ws_bytes < (size_t)sp * cols * sizeof(double)) return fail(NEB_ERR_ARG, "neb_colsum: workspace too small");
+  double* part = reinterpret_cast<double*>(workspace);
+  hipStream_t st = S(stream);
+  switch (dtype) {
+    case NEB_F32: return colsum_launch<float>(x, rows, cols, ld, beta, out, part, sp, st);
+    case NEB_BF16: return colsum_launch<__bf16>(x, rows, cols, ld, beta, out, part, sp, st);
+    case NEB_F64: return colsum_launch<double>(x, rows, cols, ld, beta, out, part, sp, st);
+  }
+  return fail(NEB_ERR_ARG, "neb_colsum: unknown dtype %d", dtype);
+}
+
+int neb_sumsq(int dtype, const void* x, int64_t n, double* out, void* workspace, size_t ws_bytes, void* stream) {
+  if (n < 0 || !x || !out || !workspace) return fail(NEB_ERR_ARG, "neb_sumsq: bad arguments");
+  if (ws_bytes < 256 * sizeof(double)) return fail(NEB_ERR_ARG, "neb_sumsq: workspace too small");
+  double* part = reinterpret_cast<double*>(workspace);
+  hipStream_t st = S(stream);
+  switch (dtype) {
+    case NEB_F32: hipLaunchKernelGGL(sumsq_kernel<float>, dim3(256), dim3(256), 0, st, reinterpret_cast<const float*>(x), n, part); break;
+    case NEB_F64: hipLaunchKernelGGL(sumsq_kernel<double>, dim3(256), dim3(256), 0, st, reinterpret_cast<const double*>(x), n, part); break;
+    case NEB_BF16: hipLaunchKernelGGL(sumsq_kernel<__bf16>, dim3(256), dim3(256), 0, st, reinterpret_cast<const __bf16*>(x), n, part); break;
+    default: return fail(NEB_ERR_ARG, "neb_sumsq: unknown dtype %d", dtype);
+  }
+  if (int rc = hip_status("neb_sumsq")) return rc;
+  hipLaunchKernelGGL(sum_splits_kernel, dim3(1), dim3(256), 0, st, part, 256, (int64_t)1, out, 0.0);
+  return hip_status("neb_sumsq reduce");
+}
+
+int neb_adam(int dtype, void* p, const void* g, void* m, void* v, int64_t n, double lr, double scale, double b1,
+             double b2, double eps, int64_t t, void* stream) {
+  if (n < 0 || !p || !g || !m || !v || t < 1) return fail(NEB_ERR_ARG, "neb_adam: bad arguments");
+  if (n == 0) return NEB_OK;
+  const double c1 = 1.0 - pow(b1, (double)t), c2 = 1.0 - pow(b2, (double)t);
+  const dim3 grid((unsigned)((n + 255) / 256));
+  hipStream_t st = S(stream);
+  if (dtype == NEB_F32)
+    hipLaunchKernelGGL(adam_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<float*>(p),
+                       reinterpret_cast<const float*>(g), reinterpret_cast<float*>(m), reinterpret_cast<float*>(v), n,
+                       lr, scale, b1, b2, eps, c1, c2);
+  else if (dtype == NEB_F64)
+    hipLaunchKernelGGL(adam_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<double*>(p),
+                       reinterpret_cast<const double*>(g), reinterpret_cast<double*>(m), reinterpret_cast<double*>(v),
+                       n, lr, scale, b1, b2, eps, c1, c2);
+  else
+    return fail(NEB_ERR_ARG, "neb_adam: fp32 or fp64 master weights only");
+  return hip_status("neb_adam");
+}
+
+int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst,
+              int64_t ldd, int64_t drows, int64_t dcols, int transpose, double scale, void* stream) {
+  if (!src || !dst || rows < 0 || cols < 0 || drows < 0 || dcols < 0 || ldd < dcols)
+    return fail(NEB_ERR_ARG, "neb_stage: bad arguments");
+  hipStream_t st = S(stream);
+  switch (src_dtype) {
+    case NEB_F32: return stage_dst<float>(dst_dtype, src, rows, cols, lds, dst, ldd, drows, dcols, transpose, scale, st);
+    case NEB_F64: return stage_dst<double>(dst_dtype, src, rows, cols, lds, dst, ldd, drows, dcols, transpose, scale, st);
+    case NEB_BF16: return stage_dst<__bf16>(dst_dtype, src, rows, cols, lds, dst, ldd, drows, dcols, transpose, scale, st);
+  }
+  return fail(NEB_ERR_ARG, "neb_stage: unknown src dtype %d", src_dtype);
+}
+
+}  // extern "C"

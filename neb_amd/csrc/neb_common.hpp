@@ -1,0 +1,172 @@
+// Common types for the gfx950 (CDNA4) neural-eigenfluid kernels.
+//
+// Precision policies select the MFMA instruction used by every contraction:
+//   PREC_BF16 : bf16 operands, v_mfma_f32_16x16x32_bf16, fp32 accumulate
+//   PREC_F32  : fp32 operands, v_mfma_f32_16x16x4f32
+//   PREC_F64  : fp64 operands, v_mfma_f64_16x16x4f64 (finite-difference checks)
+//
+// All MFMA tiles are 16x16 with a 64-wide wavefront.  Operand fragments are
+// read from LDS tiles stored [row][k] (k contiguous): lane l reads row l%16 and
+// the KE consecutive k values starting at (l/16)*KE.  A and B fragments use the
+// same lane->k map, so the contraction is correct for any hardware k order.
+// The 16x16 result is held as: lane l, element r -> row 4*(l/16)+r, col l%16.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+enum { PREC_F32 = 0, PREC_BF16 = 1, PREC_F64 = 2 };
+enum { ACT_TANH = 0, ACT_SIN = 1 };
+
+template <int P> struct Prec;
+
+template <> struct Prec<PREC_BF16> {
+  using store_t = __bf16;  // activations / tape in HBM and MFMA operands in LDS
+  using math_t = float;    // epilogue arithmetic
+  using frag_t = bf16x8;
+  using acc_t = f32x4;
+  using accel_t = float;
+  static constexpr int KI = 32;  // k per MFMA
+  static constexpr int KE = 8;   // k elements per lane per fragment
+  static constexpr int RK = 32;  // k per LDS stage
+  static constexpr int PAD = 8;  // LDS row padding (elements)
+  __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <> struct Prec<PREC_F32> {
+  using store_t = float;
+  using math_t = float;
+  using frag_t = float;
+  using acc_t = f32x4;
+  using accel_t = float;
+  static constexpr int KI = 4;
+  static constexpr int KE = 1;
+  static constexpr int RK = 32;
+  static constexpr int PAD = 4;
+  __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <> struct Prec<PREC_F64> {
+  using store_t = double;
+  using math_t = double;
+  using frag_t = double;
+  using acc_t = f64x4;
+  using accel_t = double;
+  static constexpr int KI = 4;
+  static constexpr int KE = 1;
+  static constexpr int RK = 16;
+  static constexpr int PAD = 2;
+  __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// scalar conversions
+// ---------------------------------------------------------------------------
+template <typename T> __device__ inline T cvt(float v);
+template <typename T> __device__ inline T cvt(double v);
+template <> __device__ inline float cvt<float>(float v) { return v; }
+template <> __device__ inline float cvt<float>(double v) { return (float)v; }
+template <> __device__ inline double cvt<double>(float v) { return (double)v; }
+template <> __device__ inline double cvt<double>(double v) { return v; }
+template <> __device__ inline __bf16 cvt<__bf16>(float v) { return (__bf16)v; }
+template <> __device__ inline __bf16 cvt<__bf16>(double v) { return (__bf16)(float)v; }
+
+template <typename T> __device__ inline double todbl_any(T v) { return (double)v; }
+template <> __device__ inline double todbl_any<__bf16>(__bf16 v) { return (double)(float)v; }
+
+template <typename M, typename T> __device__ inline M tomath(T v) { return (M)v; }
+template <> __device__ inline float tomath<float, __bf16>(__bf16 v) { return (float)v; }
+template <> __device__ inline double tomath<double, __bf16>(__bf16 v) { return (double)(float)v; }
+
+// ---------------------------------------------------------------------------
+// activations: value and first three derivatives of f(u)
+// ---------------------------------------------------------------------------
+template <int ACT, typename M> struct Act;
+
+template <typename M> struct Act<ACT_TANH, M> {
+  __device__ static inline void eval(M u, M& f0, M& f1, M& f2) {
+    M t = tanh(u);
+    M d = (M)1 - t * t;
+    f0 = t; f1 = d; f2 = (M)-2 * t * d;
+  }
+  __device__ static inline void eval3(M u, M& f1, M& f2, M& f3) {
+    M t = tanh(u);
+    M d = (M)1 - t * t;
+    f1 = d; f2 = (M)-2 * t * d; f3 = (M)-2 * d * ((M)1 - (M)3 * t * t);
+  }
+};
+
+template <typename M> struct Act<ACT_SIN, M> {
+  __device__ static inline void eval(M u, M& f0, M& f1, M& f2) {
+    M s = sin(u), c = cos(u);
+    f0 = s; f1 = c; f2 = -s;
+  }
+  __device__ static inline void eval3(M u, M& f1, M& f2, M& f3) {
+    M s = sin(u), c = cos(u);
+    f1 = c; f2 = -s; f3 = -c;
+  }
+};
+
+// Hessian pair tables in the reference order: 2D (0,0),(0,1),(1,1);
+// 3D (0,0),(0,1),(0,2),(1,1),(1,2),(2,2).
+template <int D> struct Jet2Layout;
+template <> struct Jet2Layout<2> {
+  static constexpr int C = 6, P = 3;
+  __device__ static constexpr int pi(int p) { return p == 2 ? 1 : 0; }
+  __device__ static constexpr int pj(int p) { return p == 0 ? 0 : 1; }
+};
+template <> struct Jet2Layout<3> {
+  static constexpr int C = 10, P = 6;
+  __device__ static constexpr int pi(int p) { return p < 3 ? 0 : (p < 5 ? 1 : 2); }
+  __device__ static constexpr int pj(int p) { return p < 3 ? p : (p < 5 ? p - 2 : 2); }
+};
+
+// Forward second-order chain rule for y = f(u) on one (point, feature) pair.
+// in/out: [C] packed channels (value, grads, hessian pairs).
+template <int D, int ACT, typename M>
+__device__ inline void act_jet_forward(const M* u, M* y) {
+  using L = Jet2Layout<D>;
+  M f0, f1, f2;
+  Act<ACT, M>::eval(u[0], f0, f1, f2);
+  y[0] = f0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) y[1 + a] = f1 * u[1 + a];
+#pragma unroll
+  for (int p = 0; p < L::P; ++p)
+    y[1 + D + p] = f1 * u[1 + D + p] + f2 * u[1 + L::pi(p)] * u[1 + L::pj(p)];
+}
+
+// Transpose of act_jet_forward w.r.t. the pre-activation jet u given the
+// output adjoint a; result in b.  Off-diagonal pairs contribute to both grads.
+template <int D, int ACT, typename M>
+__device__ inline void act_jet_reverse(const M* u, const M* a, M* b) {
+  using L = Jet2Layout<D>;
+  M f1, f2, f3;
+  Act<ACT, M>::eval3(u[0], f1, f2, f3);
+  M bv = a[0] * f1;
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    bv += f2 * a[1 + c] * u[1 + c];
+    b[1 + c] = f1 * a[1 + c];
+  }
+#pragma unroll
+  for (int p = 0; p < L::P; ++p) {
+    const int i = L::pi(p), j = L::pj(p);
+    const M ah = a[1 + D + p];
+    bv += ah * (f2 * u[1 + D + p] + f3 * u[1 + i] * u[1 + j]);
+    b[1 + i] += f2 * ah * u[1 + j];
+    b[1 + j] += f2 * ah * u[1 + i];
+    b[1 + D + p] = f1 * ah;
+  }
+  b[0] = bv;
+}

@@ -1,0 +1,476 @@
+"""Shared-trunk eigenfluid network on MI355X: parameters, analytic-jet
+forward, Gram assembly and the handwritten reverse pass.
+
+Host-side mirror of the reference backend's model surface
+(experiments/metaball/_backend/flow/cases/metaball/native_cupy_core.py:111-185
+``Model`` with ``W``/``b`` lists, ``forward``/``backward``/``adam``/
+``save``/``load``; velocity_trace_jet2.py ``forward``/``backward``/
+``velocity``/``velocity_grad``; velocity/core.py ``fields``/``grams``).  All
+arithmetic on the path runs in libnebhip.so kernels; torch only owns device
+memory and streams, plus the small FP64 K x K products and factorizations
+(rocBLAS / rocSOLVER).
+
+Network: features (periodic sin/cos jets) -> L hidden layers
+act(omega_l (x W_l^T + b_l)) -> linear output with K (2D stream function)
+or 3K (3D vector potential, rows 3k+c) outputs.  Jets are second order;
+channel order value, grads, Hessian pairs (0,0),(0,1),(1,1) [2D] /
+(0,0),(0,1),(0,2),(1,1),(1,2),(2,2) [3D].
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _hip as H
+
+PRECISIONS = {"bf16": H.BF16, "f32": H.F32, "f64": H.F64}
+_STORE = {H.BF16: torch.bfloat16, H.F32: torch.float32, H.F64: torch.float64}
+_FIELD = {H.BF16: torch.float32, H.F32: torch.float32, H.F64: torch.float64}
+_CROSS = {H.BF16: H.CROSS_BF16, H.F32: H.CROSS_F32, H.F64: H.CROSS_F64}
+GRAM_MODES = {"f32": H.CROSS_F32, "bf16": H.CROSS_BF16, "bf16x3": H.CROSS_BF16X3, "f64": H.CROSS_F64}
+_DT = {torch.float32: H.F32, torch.bfloat16: H.BF16, torch.float64: H.F64}
+
+
+def _ld(k: int) -> int:
+    return (k + 7) // 8 * 8
+
+
+def _stream():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def channels(dim: int) -> int:
+    return 1 + dim + dim * (dim + 1) // 2
+
+
+@dataclass
+class NetConfig:
+    dim: int = 2
+    frequencies: int = 8
+    lo: tuple = (0.0, 0.0)
+    hi: tuple = (2 * math.pi, 2 * math.pi)
+    widths: tuple = (128, 128, 128)
+    modes: int = 64
+    activation: str = "tanh"
+    omegas: tuple | None = None
+
+    @property
+    def features(self) -> int:
+        return 2 * self.frequencies * self.dim
+
+    @property
+    def ncomp(self) -> int:
+        return 1 if self.dim == 2 else 3
+
+    @property
+    def sizes(self) -> list[int]:
+        return [self.features, *self.widths, self.ncomp * self.modes]
+
+    @property
+    def act_code(self) -> int:
+        return {"tanh": H.TANH, "sin": H.SIN}[self.activation]
+
+    def omega(self, l: int) -> float:
+        return 1.0 if self.omegas is None else float(self.omegas[l])
+
+
+class Workspace:
+    """Grow-only scratch buffer for split reductions (the library allocates
+    nothing itself)."""
+
+    def __init__(self, device):
+        self.device = device
+        self.buf = torch.empty(0, dtype=torch.uint8, device=device)
+
+    def get(self, nbytes: int) -> torch.Tensor:
+        nbytes = max(int(nbytes), 256 * 8)
+        if self.buf.numel() < nbytes:
+            self.buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        return self.buf
+
+
+_WS: dict = {}
+
+
+def workspace(device) -> Workspace:
+    key = str(device)
+    if key not in _WS:
+        _WS[key] = Workspace(device)
+    return _WS[key]
+
+
+class EigenfluidNet:
+    """Mode-shared trunk with K (2D) or 3K (3D) linear outputs.
+
+    Parameters live in one flat master tensor (fp32, or fp64 for FP64 runs)
+    with per-layer views ``W[l]`` [out, in] and ``b[l]`` [out]; Adam moments
+    ``m``/``v`` mirror it (the reference's ``mW``/``vW``/``mb``/``vb``,
+    NCC:125-128).  Initialization: Xavier-normal, std sqrt(2/(fan_in+fan_out)),
+    drawn layer by layer as ``default_rng(seed).normal(size=(out, in))``, zero
+    biases.
+    """
+
+    def __init__(self, cfg: NetConfig, seed: int = 0, device="cuda", master_dtype=torch.float32):
+        self.cfg = cfg
+        self.device = torch.device(device)
+        self.master_dtype = master_dtype
+        sizes = cfg.sizes
+        rng = np.random.default_rng(seed)
+        host = []
+        self._shapes = []
+        for fi, fo in zip(sizes[:-1], sizes[1:]):
+            host.append((rng.normal(size=(fo, fi)) * math.sqrt(2.0 / (fi + fo))).ravel())
+            host.append(np.zeros(fo))
+            self._shapes.append((fo, fi))
+        flat = np.concatenate(host)
+        self.params = torch.tensor(flat, dtype=master_dtype, device=self.device)
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.epoch = 0
+        self._views()
+        self._staged = {}
+        self.version = 0
+
+    # -- parameter views -------------------------------------------------
+    def _views(self):
+        self.W, self.b, self._offsets = [], [], []
+        off = 0
+        for fo, fi in self._shapes:
+            self.W.append(self.params[off:off + fo * fi].view(fo, fi))
+            wo = off
+            off += fo * fi
+            self.b.append(self.params[off:off + fo])
+            self._offsets.append((wo, off))
+            off += fo
+
+    def split(self, flat: torch.Tensor):
+        """Views of a flat parameter-shaped tensor as ([W...], [b...])."""
+        Ws, bs = [], []
+        for (fo, fi), (wo, bo) in zip(self._shapes, self._offsets):
+            Ws.append(flat[wo:wo + fo * fi].view(fo, fi))
+            bs.append(flat[bo:bo + fo])
+        return Ws, bs
+
+    def load_arrays(self, W, b):
+        with torch.no_grad():
+            for dst, src in zip(self.W + self.b, list(W) + list(b)):
+                dst.copy_(torch.as_tensor(np.asarray(src), dtype=self.master_dtype))
+        self.touch()
+
+    def touch(self):
+        """Mark master weights as changed (drops staged copies)."""
+        self.version += 1
+        self._staged.clear()
+
+    # -- staging to the MFMA operand precision ----------------------------
+    def staged(self, prec: int):
+        key = (prec, self.version)
+        if key in self._staged:
+            return self._staged[key]
+        st = _STORE[prec]
+        mdt = _DT[self.master_dtype]
+        math_dt = torch.float64 if prec == H.F64 else torch.float32
+        out = {"W": [], "WT": [], "b": []}
+        s = _stream()
+        for l, (W, b) in enumerate(zip(self.W, self.b)):
+            fo, fi = W.shape
+            ws = torch.empty((fo, _ld(fi)), dtype=st, device=self.device)
+            H.call("neb_stage", mdt, prec, W.data_ptr(), fo, fi, fi, ws.data_ptr(), ws.shape[1], fo, ws.shape[1],
+                   0, 1.0, s)
+            out["W"].append(ws)
+            if l < len(self.W) - 1:
+                wt = torch.empty((fi, _ld(fo)), dtype=st, device=self.device)
+                H.call("neb_stage", mdt, prec, W.data_ptr(), fo, fi, fi, wt.data_ptr(), wt.shape[1], fi,
+                       wt.shape[1], 1, 1.0, s)
+                out["WT"].append(wt)
+            out["b"].append(b.to(math_dt).contiguous())
+        self._staged = {key: out}
+        return out
+
+    # -- optimizer ---------------------------------------------------------
+    def adam(self, grads: torch.Tensor, lr: float, clip: float = 8.0):
+        """Adam with global-norm clip (NCC:156-171; one parameter group for
+        the shared trunk).  Nonfinite or zero gradients raise
+        FloatingPointError before any update."""
+        dt = _DT[self.master_dtype]
+        ws = workspace(self.device).get(256 * 8)
+        norm2 = torch.empty(1, dtype=torch.float64, device=self.device)
+        s = _stream()
+        H.call("neb_sumsq", dt, grads.data_ptr(), grads.numel(), norm2.data_ptr(), ws.data_ptr(), ws.numel(), s)
+        norm = math.sqrt(float(norm2.item()))
+        if not math.isfinite(norm) or norm <= 0:
+            raise FloatingPointError("Invalid parameter gradients; no optimizer update")
+        scale = min(1.0, clip / max(norm, 1e-30))
+        t = self.epoch + 1
+        H.call("neb_adam", dt, self.params.data_ptr(), grads.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+               self.params.numel(), lr, scale, 0.9, 0.999, 1e-8, t, s)
+        self.epoch = t
+        self.touch()
+        return norm
+
+    # -- checkpoint (NCC:173-185 key layout) ---------------------------------
+    def save(self, path):
+        arrays = {}
+        mW, mb = self.split(self.m)
+        vW, vb = self.split(self.v)
+        for name, lst in (("W", self.W), ("b", self.b), ("mW", mW), ("vW", vW), ("mb", mb), ("vb", vb)):
+            for j, a in enumerate(lst):
+                arrays[f"{name}_{j}"] = a.detach().cpu().numpy()
+        arrays["epoch"] = np.asarray(self.epoch)
+        tmp = str(path) + ".tmp.npz"
+        np.savez(tmp, **arrays)
+        import os
+        os.replace(tmp, path)
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.epoch = int(z["epoch"])
+            mW, mb = self.split(self.m)
+            vW, vb = self.split(self.v)
+            with torch.no_grad():
+                for name, lst in (("W", self.W), ("b", self.b), ("mW", mW), ("vW", vW), ("mb", mb), ("vb", vb)):
+                    for j, a in enumerate(lst):
+                        src = z[f"{name}_{j}"]
+                        if src.shape != tuple(a.shape):
+                            raise ValueError(f"{name}_{j}: shape {src.shape} != {tuple(a.shape)}")
+                        a.copy_(torch.as_tensor(src, dtype=self.master_dtype))
+        self.touch()
+
+
+# ---------------------------------------------------------------------------
+# forward
+# ---------------------------------------------------------------------------
+@dataclass
+class Tape:
+    prec: int
+    n: int
+    inputs: list = field(default_factory=list)   # packed layer inputs [C, n, ld] (X0, Y1, ..., Y_L)
+    pre: list = field(default_factory=list)      # packed scaled pre-activations Z_l
+
+
+def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
+    """Feature jets and all hidden layers; keeps the jet tape."""
+    cfg = net.cfg
+    if x.ndim != 2 or x.shape[1] != cfg.dim:
+        raise ValueError(f"points must be [n, {cfg.dim}]")
+    xdt = torch.float64 if prec == H.F64 else torch.float32
+    x = x.to(device=net.device, dtype=xdt).contiguous()
+    n = x.shape[0]
+    C = channels(cfg.dim)
+    st = _STORE[prec]
+    stg = net.staged(prec)
+    s = _stream()
+    lo = H.dbl_array([float(v) for v in cfg.lo])
+    hi = H.dbl_array([float(v) for v in cfg.hi])
+    F = cfg.features
+    X0 = torch.empty((C, n, _ld(F)), dtype=st, device=net.device)
+    import ctypes
+    H.call("neb_feature_jets", prec, cfg.dim, n, x.data_ptr(), cfg.frequencies, ctypes.addressof(lo),
+           ctypes.addressof(hi), X0.data_ptr(), X0.shape[2], s)
+    tape = Tape(prec, n, [X0], [])
+    cur, k = X0, F
+    for l, width in enumerate(cfg.widths):
+        Z = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
+        Y = torch.empty_like(Z)
+        H.call("neb_jet_layer_forward", prec, cfg.dim, cfg.act_code, n, k, width, cur.data_ptr(), cur.shape[2],
+               stg["W"][l].data_ptr(), stg["W"][l].shape[1], stg["b"][l].data_ptr(), cfg.omega(l), Z.data_ptr(),
+               Y.data_ptr(), Y.shape[2], s)
+        tape.pre.append(Z)
+        tape.inputs.append(Y)
+        cur, k = Y, width
+    return tape
+
+
+def output_jets(net: EigenfluidNet, tape: Tape):
+    """Raw output jets (value [n, nout], grad [n, nout, d], hess [n, nout, P])
+    -- the analogue of the per-mode field jets of forward_stream_jet2
+    (HB2P:1802-1880) / VJ2.forward (VJ2:65-76)."""
+    cfg = net.cfg
+    prec = tape.prec
+    stg = net.staged(prec)
+    Y = tape.inputs[-1]
+    n, k = tape.n, cfg.widths[-1]
+    nout = cfg.ncomp * cfg.modes
+    C = channels(cfg.dim)
+    out = torch.empty((C, n, _ld(nout)), dtype=_FIELD[prec], device=net.device)
+    H.call("neb_jet_output_jets", prec, cfg.dim, n, k, nout, Y.data_ptr(), Y.shape[2], stg["W"][-1].data_ptr(),
+           stg["W"][-1].shape[1], stg["b"][-1].data_ptr(), out.data_ptr(), out.shape[2], _stream())
+    out = out[:, :, :nout]
+    d = cfg.dim
+    return out[0], out[1:1 + d].permute(1, 2, 0), out[1 + d:].permute(1, 2, 0)
+
+
+@dataclass
+class Fields:
+    mass: torch.Tensor    # [n*cm, ldK] (view [:, :K]); 2D (u, v), 3D curl A
+    stiff: torch.Tensor   # [n*cs, ldK]; 2D omega, 3D grad u (rows 3a+j)
+    K: int
+    layout: int = 0
+
+    @property
+    def U(self):
+        return self.mass[:, :self.K]
+
+    @property
+    def D(self):
+        return self.stiff[:, :self.K]
+
+
+def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0) -> Fields:
+    """Output layer fused with the lift: the field matrices the Gram and the
+    spectral objective consume (VCORE:71-77 ``fields``; ELC:35-38)."""
+    cfg = net.cfg
+    prec = tape.prec
+    stg = net.staged(prec)
+    Y = tape.inputs[-1]
+    n, k, K = tape.n, cfg.widths[-1], cfg.modes
+    cm, cs = (2, 1) if cfg.dim == 2 else (3, 9)
+    ft = _FIELD[prec]
+    mass = torch.empty((cm * n, _ld(K)), dtype=ft, device=net.device)
+    stiff = torch.empty((cs * n, _ld(K)), dtype=ft, device=net.device)
+    H.call("neb_jet_output_fields", prec, cfg.dim, n, k, K, Y.data_ptr(), Y.shape[2], stg["W"][-1].data_ptr(),
+           stg["W"][-1].shape[1], mass.data_ptr(), stiff.data_ptr(), mass.shape[1], layout, _stream())
+    return Fields(mass, stiff, K, layout)
+
+
+# ---------------------------------------------------------------------------
+# Gram
+# ---------------------------------------------------------------------------
+def gram(phi: torch.Tensor, K: int, w: float, mode: str = "bf16x3", out: torch.Tensor | None = None,
+         beta: float = 0.0, roww: torch.Tensor | None = None) -> torch.Tensor:
+    """G = beta*G + w * Phi[:, :K]^T diag(roww) Phi[:, :K]  in FP64
+    (VCORE:40-46 ``grams``; experiments/metaball/run.py:99-116)."""
+    if phi.ndim != 2 or phi.stride(1) != 1:
+        raise ValueError("Phi must be a row-major 2-D tensor")
+    if out is None:
+        out = torch.empty((K, K), dtype=torch.float64, device=phi.device)
+        beta = 0.0
+    m = GRAM_MODES[mode]
+    rows = phi.shape[0]
+    need = H.load().neb_cross_workspace_size(K, K, rows, 1)
+    ws = workspace(phi.device).get(need)
+    rw = 0 if roww is None else roww.to(torch.float64).contiguous().data_ptr()
+    H.call("neb_gram", m, phi.data_ptr(), rows, K, phi.stride(0), _DT[phi.dtype], float(w), rw, float(beta),
+           out.data_ptr(), out.stride(0), ws.data_ptr(), ws.numel(), _stream())
+    return out
+
+
+def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, beta: float = 0.0):
+    """sum_s sign_s X_s^T Y_s over rows; segments = [(X, Y, rows, sign)] with
+    X/Y row-major 2-D tensors (views allowed)."""
+    xs = [s[0] for s in segments]
+    ys = [s[1] for s in segments]
+    if out is None:
+        out = torch.empty((m, n), dtype=torch.float64, device=xs[0].device)
+        beta = 0.0
+    rows = [int(s[2]) for s in segments]
+    need = H.load().neb_cross_workspace_size(m, n, max(rows), 0)
+    ws = workspace(xs[0].device).get(need)
+    H.call("neb_cross", mode, len(segments), H.ptr_array([t.data_ptr() for t in xs]),
+           H.ptr_array([t.data_ptr() for t in ys]), H.i64_array([t.stride(0) for t in xs]),
+           H.i64_array([t.stride(0) for t in ys]), H.i64_array(rows), H.dbl_array([float(s[3]) for s in segments]),
+           _DT[xs[0].dtype], _DT[ys[0].dtype], m, n, 0, 0, float(scale), float(beta), out.data_ptr(),
+           out.stride(0), ws.data_ptr(), ws.numel(), _stream())
+    return out
+
+
+def _rows_view(t: torch.Tensor, start_row: int, step: int) -> torch.Tensor:
+    """Every ``step``-th row of a row-major 2-D tensor starting at start_row."""
+    return t.as_strided((t.shape[0] // step, t.shape[1]), (t.stride(0) * step, 1),
+                        t.storage_offset() + start_row * t.stride(0))
+
+
+# ---------------------------------------------------------------------------
+# reverse pass
+# ---------------------------------------------------------------------------
+def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Tensor, dS: torch.Tensor,
+                    w: float, grads: torch.Tensor | None = None) -> torch.Tensor:
+    """Parameter gradient of L(M, S) with M = w U^T U, S = w D^T D, given
+    dL/dM and dL/dS (symmetric, FP64 K x K).  Fuses the field adjoint
+    2w Phi dL/dG (VCORE:94-95) into the output-layer reverse, then runs the
+    handwritten jet reverse through every layer (HB2P:2087-2139, VJ2:77-82).
+    Returns the flat gradient in the master dtype."""
+    cfg = net.cfg
+    prec = tape.prec
+    stg = net.staged(prec)
+    d, C, n, K = cfg.dim, channels(cfg.dim), tape.n, cfg.modes
+    L = len(cfg.widths)
+    width = cfg.widths[-1]
+    st = _STORE[prec]
+    s = _stream()
+    dM = 0.5 * (dM + dM.T)
+    dS = 0.5 * (dS + dS.T)
+    Wout = net.W[-1].to(torch.float64)               # [ncomp*K, width]
+    ldK = fields.mass.shape[1]
+    if d == 2:
+        Bm = (2.0 * w) * (Wout.T @ dM)               # [width, K]
+        Bs = (2.0 * w) * (Wout.T @ dS)
+    else:
+        Bm = torch.cat([(2.0 * w) * (Wout[c::3].T @ dM) for c in range(3)], 0)  # [3*width, K]
+        Bs = torch.cat([(2.0 * w) * (Wout[c::3].T @ dS) for c in range(3)], 0)
+    bm = torch.zeros((Bm.shape[0], _ld(K)), dtype=st, device=net.device)
+    bs = torch.zeros_like(bm)
+    bm[:, :K] = Bm.to(st)
+    bs[:, :K] = Bs.to(st)
+    A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
+    Z = tape.pre[-1]
+    H.call("neb_jet_output_backward", prec, d, cfg.act_code, n, width, K, fields.mass.data_ptr(),
+           fields.stiff.data_ptr(), ldK, fields.layout, bm.data_ptr(), bs.data_ptr(), bm.shape[1], Z.data_ptr(),
+           Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
+
+    if grads is None:
+        grads = torch.empty_like(net.params)
+    gW, gb = net.split(grads)
+    mode = _CROSS[prec]
+    Ylast = tape.inputs[-1]
+    ch = lambda t, c: t[c]  # noqa: E731  packed channel c as [n, ld]
+    U, Dg = fields.mass, fields.stiff
+    if fields.layout != 0:
+        raise ValueError("backward_fields expects point-major field matrices")
+    if d == 2:
+        u, v = _rows_view(U, 0, 2), _rows_view(U, 1, 2)
+        Pm = cross([(u, ch(Ylast, 2), n, 1.0), (v, ch(Ylast, 1), n, -1.0)], K, width, mode)
+        Ps = cross([(Dg, ch(Ylast, 3), n, -1.0), (Dg, ch(Ylast, 5), n, -1.0)], K, width, mode)
+        gW[-1].copy_(((2.0 * w) * (dM @ Pm + dS @ Ps)).to(grads.dtype))
+    else:
+        from .lifts import curl_terms
+        comps = [_rows_view(U, a, 3) for a in range(3)]
+        grads3 = [_rows_view(Dg, c, 9) for c in range(9)]
+        for sl in range(3):
+            segm = [(comps[a], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a < 3 and w_s == sl]
+            segs = [(grads3[a - 3], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms()
+                    if a >= 3 and w_s == sl]
+            Pm = cross(segm, K, width, mode)
+            Pd = torch.zeros_like(Pm)
+            for i in range(0, len(segs), 8):
+                cross(segs[i:i + 8], K, width, mode, out=Pd, beta=1.0 if i else 0.0)
+            gW[-1][sl::3].copy_(((2.0 * w) * (dM @ Pm + dS @ Pd)).to(grads.dtype))
+    gb[-1].zero_()
+
+    cw = workspace(net.device)
+    for l in range(L - 1, -1, -1):
+        X = tape.inputs[l]
+        k = X.shape[2] if l > 0 else cfg.features
+        wl = cfg.widths[l]
+        A2 = A.view(C * n, A.shape[2])
+        X2 = X.view(C * n, X.shape[2])
+        G = cross([(A2, X2, C * n, 1.0)], wl, k, mode)
+        gW[l].copy_(G.to(grads.dtype))
+        need = H.load().neb_colsum_workspace_size(n, wl)
+        ws = cw.get(need)
+        db = torch.empty(wl, dtype=torch.float64, device=net.device)
+        H.call("neb_colsum", _DT[st], A[0].data_ptr(), n, wl, A.shape[2], 0.0, db.data_ptr(), ws.data_ptr(),
+               ws.numel(), s)
+        gb[l].copy_(db.to(grads.dtype))
+        if l > 0:
+            Zp = tape.pre[l - 1]
+            An = torch.empty((C, n, _ld(k)), dtype=st, device=net.device)
+            WT = stg["WT"][l]
+            H.call("neb_jet_layer_backward", prec, d, cfg.act_code, n, k, wl, A.data_ptr(), A.shape[2],
+                   WT.data_ptr(), WT.shape[1], Zp.data_ptr(), Zp.shape[2], cfg.omega(l - 1), An.data_ptr(),
+                   An.shape[2], s)
+            A = An
+    return grads

@@ -1,0 +1,444 @@
+"""FP64 NumPy oracle for the neural eigenfluid network hot path.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker / CPU baseline.  The product path (``neb_amd``) never imports it.
+
+It restates, in plain FP64 array code, the algorithms of the reference backend
+(baiming-zhang/NEB; paths relative to the reference root, ``HB2P`` =
+experiments/2d/_backend/periodic_core/vortex_merger/src/eigenfluid/gen/
+vortex_merger2d/hardcoded_backend.py, ``VJ2`` = experiments/metaball/_backend/
+flow/cases/metaball/velocity_trace_jet2.py, ``VCORE`` = .../metaball/velocity/
+core.py, ``NCC`` = .../metaball/native_cupy_core.py, ``PP`` =
+experiments/metaball/_backend/exports/basis_projection/postprocessing/
+paired_projection.py), applied to the shared-trunk tanh network of the
+benchmark configurations:
+
+    features (periodic sin/cos jets)  ->  L hidden layers act(omega*(x W^T + b))
+    ->  linear output layer  ->  differential lift (2D: u=(psi_y,-psi_x),
+    omega=-lap psi;  3D: u = curl A, grad u)  ->  Gram  Phi^T diag(w) Phi
+    ->  spectral loss  tr((M+eps I)^-1 S) + lam ||M-I||_F^2  ->  reverse pass.
+
+Jets are compact second-order Taylor jets: value [n, f], grad [n, f, d],
+hess [n, f, P] with P = d(d+1)/2 unique Hessian entries in PAIRS order
+((0,0),(0,1),(1,1)) in 2D and ((0,0),(0,1),(0,2),(1,1),(1,2),(2,2)) in 3D
+(HB2P:154-166, VJ2:9).
+
+Pinning: every function here is checked in tests/test_oracle.py against
+closed forms (exact Fourier-basis Gram = I, manufactured curl/Laplacian
+fields), central finite differences (spatial jets and the parameter reverse
+pass) and library routines (scipy.linalg.eigh, dense solves).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+def pairs(d: int) -> list[tuple[int, int]]:
+    """Unique Hessian index pairs in the reference order (VJ2:9, HB2P:159)."""
+    return [(i, j) for i in range(d) for j in range(i, d)]
+
+
+def channels(d: int) -> int:
+    """Packed Jet2 channel count: value + d grads + d(d+1)/2 Hessian entries."""
+    return 1 + d + d * (d + 1) // 2
+
+
+@dataclass
+class Jet:
+    value: np.ndarray  # [n, f]
+    grad: np.ndarray   # [n, f, d]
+    hess: np.ndarray   # [n, f, P]
+
+    def pack(self) -> np.ndarray:
+        """Channel-major packed operand [C, n, f] (VJ2:20-22, HB2P:1014-1025)."""
+        d = self.grad.shape[-1]
+        return np.stack([self.value]
+                        + [self.grad[..., a] for a in range(d)]
+                        + [self.hess[..., p] for p in range(self.hess.shape[-1])])
+
+    @staticmethod
+    def unpack(packed: np.ndarray, d: int) -> "Jet":
+        P = d * (d + 1) // 2
+        return Jet(packed[0], np.stack(list(packed[1:1 + d]), -1),
+                   np.stack(list(packed[1 + d:1 + d + P]), -1))
+
+
+# --------------------------------------------------------------------------
+# a1: periodic sin/cos feature jets (HB2P:576-589 / :428-441)
+# --------------------------------------------------------------------------
+def feature_jets(x: np.ndarray, frequencies: int, lo, hi) -> Jet:
+    """Features ordered [for coord: for h in 1..F: (sin, cos)] of
+    phase = pi*h*(xn+1), xn = 2(x-lo)/(hi-lo)-1  (HB2P:570-589)."""
+    x = np.asarray(x, dtype=np.float64)
+    n, d = x.shape
+    lo = np.broadcast_to(np.asarray(lo, dtype=np.float64), (d,))
+    hi = np.broadcast_to(np.asarray(hi, dtype=np.float64), (d,))
+    F = 2 * frequencies * d
+    P = d * (d + 1) // 2
+    value = np.zeros((n, F))
+    grad = np.zeros((n, F, d))
+    hess = np.zeros((n, F, P))
+    pl = pairs(d)
+    col = 0
+    for a in range(d):
+        scale = 2.0 / (hi[a] - lo[a])
+        xn = scale * (x[:, a] - lo[a]) - 1.0
+        diag = pl.index((a, a))
+        for h in range(1, frequencies + 1):
+            k = math.pi * h
+            phase = k * (xn + 1.0)
+            dp = k * scale
+            s, c = np.sin(phase), np.cos(phase)
+            value[:, col], grad[:, col, a], hess[:, col, diag] = s, c * dp, -s * dp * dp
+            value[:, col + 1], grad[:, col + 1, a], hess[:, col + 1, diag] = c, -s * dp, -c * dp * dp
+            col += 2
+    return Jet(value, grad, hess)
+
+
+# --------------------------------------------------------------------------
+# activations: f, f', f'', f''' (tanh for the benchmark net; sin = SIREN,
+# HB2P:1038-1040, FUSED:3-19)
+# --------------------------------------------------------------------------
+def activation_derivs(kind: str, u: np.ndarray):
+    if kind == "tanh":
+        t = np.tanh(u)
+        d1 = 1.0 - t * t
+        return t, d1, -2.0 * t * d1, -2.0 * d1 * (1.0 - 3.0 * t * t)
+    if kind == "sin":
+        s, c = np.sin(u), np.cos(u)
+        return s, c, -s, -c
+    raise ValueError(f"unknown activation {kind!r}")
+
+
+def linear_jet(x: Jet, W: np.ndarray, b: np.ndarray | None) -> Jet:
+    """Packed-channel linear layer: all channels times W^T, bias on value rows
+    only (HB2P:1005-1036, VJ2:26-27)."""
+    v = x.value @ W.T
+    if b is not None:
+        v = v + b
+    return Jet(v, np.einsum("nid,oi->nod", x.grad, W), np.einsum("nip,oi->nop", x.hess, W))
+
+
+def activation_jet(kind: str, pre: Jet) -> Jet:
+    """y=f(u), grad y = f' grad u, hess y = f' hess u + f'' grad u grad u^T
+    (HB2P:252-272 unary2, VJ2:31-34)."""
+    f0, f1, f2, _ = activation_derivs(kind, pre.value)
+    d = pre.grad.shape[-1]
+    hess = np.stack([f1 * pre.hess[..., p] + f2 * pre.grad[..., i] * pre.grad[..., j]
+                     for p, (i, j) in enumerate(pairs(d))], -1)
+    return Jet(f0, f1[..., None] * pre.grad, hess)
+
+
+def activation_jet_reverse(kind: str, pre: Jet, adj: Jet) -> Jet:
+    """Transpose of activation_jet w.r.t. the pre-activation jet
+    (HB2P:1147-1174 unary_backward_jet2, VJ2:35-41, FUSED:9-19)."""
+    _, f1, f2, f3 = activation_derivs(kind, pre.value)
+    d = pre.grad.shape[-1]
+    g = pre.grad
+    av = adj.value * f1 + f2 * np.sum(adj.grad * g, -1)
+    ag = f1[..., None] * adj.grad
+    for p, (i, j) in enumerate(pairs(d)):
+        ah = adj.hess[..., p]
+        av = av + ah * (f2 * pre.hess[..., p] + f3 * g[..., i] * g[..., j])
+        ag[..., i] += f2 * ah * g[..., j]
+        ag[..., j] += f2 * ah * g[..., i]
+    return Jet(av, ag, f1[..., None] * adj.hess)
+
+
+# --------------------------------------------------------------------------
+# network: shared trunk, Xavier-normal init
+# --------------------------------------------------------------------------
+@dataclass
+class Net:
+    W: list  # [out, in] per layer (hidden..., output)
+    b: list  # [out]
+    activation: str = "tanh"
+    omegas: list = field(default_factory=list)  # per hidden layer
+
+    @property
+    def hidden_layers(self) -> int:
+        return len(self.W) - 1
+
+
+def init_net(sizes: list[int], seed: int, activation: str = "tanh", omegas=None) -> Net:
+    """Xavier-normal weights, std = sqrt(2/(fan_in+fan_out)), drawn layer by
+    layer from numpy.random.default_rng(seed) as rng.normal(size=(out, in));
+    zero biases."""
+    rng = np.random.default_rng(seed)
+    W, b = [], []
+    for fi, fo in zip(sizes[:-1], sizes[1:]):
+        W.append(rng.normal(size=(fo, fi)) * math.sqrt(2.0 / (fi + fo)))
+        b.append(np.zeros(fo))
+    L = len(sizes) - 2
+    om = list(omegas) if omegas is not None else [1.0] * L
+    return Net(W, b, activation, om)
+
+
+def forward(net: Net, x0: Jet, record: bool = False):
+    """Trunk + linear output layer.  Returns (output jet, tape) where the tape
+    holds (layer input jet, scaled pre-activation jet) per hidden layer and
+    the last hidden output (HB2P:1802-1880, VJ2:65-76)."""
+    x = x0
+    tape = []
+    for l in range(net.hidden_layers):
+        z = linear_jet(x, net.W[l], net.b[l])
+        om = net.omegas[l]
+        pre = Jet(om * z.value, om * z.grad, om * z.hess)
+        if record:
+            tape.append((x, pre))
+        x = activation_jet(net.activation, pre)
+    out = linear_jet(x, net.W[-1], net.b[-1])
+    return out, (tape, x)
+
+
+def backward(net: Net, tape, adj_out: Jet):
+    """Handwritten parameter reverse pass through the jets (HB2P:2087-2139,
+    VJ2:77-82, linear transpose HB2P:1115-1144)."""
+    layers, last = tape
+    gW = [None] * len(net.W)
+    gb = [None] * len(net.W)
+    a = adj_out
+    gW[-1] = np.einsum("cno,cni->oi", a.pack(), last.pack())
+    gb[-1] = a.value.sum(0)
+    W = net.W[-1]
+    a = Jet(a.value @ W, np.einsum("nod,oi->nid", a.grad, W), np.einsum("nop,oi->nip", a.hess, W))
+    for l in range(net.hidden_layers - 1, -1, -1):
+        src, pre = layers[l]
+        a = activation_jet_reverse(net.activation, pre, a)
+        om = net.omegas[l]
+        a = Jet(om * a.value, om * a.grad, om * a.hess)
+        gW[l] = np.einsum("cno,cni->oi", a.pack(), src.pack())
+        gb[l] = a.value.sum(0)
+        W = net.W[l]
+        a = Jet(a.value @ W, np.einsum("nod,oi->nid", a.grad, W), np.einsum("nop,oi->nip", a.hess, W))
+    return gW, gb
+
+
+# --------------------------------------------------------------------------
+# a5: differential lifts and their transposes
+# --------------------------------------------------------------------------
+def lift2d(out: Jet):
+    """psi jets [n, K] -> (mass fields [n, 2, K] = (psi_y, -psi_x),
+    stiffness field [n, K] = omega = -(psi_xx + psi_yy))  (ELC:35-38,
+    HB2P:2530-2546)."""
+    u = np.stack((out.grad[..., 1], -out.grad[..., 0]), 1)
+    w = -(out.hess[..., 0] + out.hess[..., 2])
+    return u, w
+
+
+def lift2d_reverse(gu: np.ndarray, gw: np.ndarray) -> Jet:
+    n, _, K = gu.shape
+    grad = np.stack((-gu[:, 1], gu[:, 0]), -1)
+    hess = np.stack((-gw, np.zeros_like(gw), -gw), -1)
+    return Jet(np.zeros((n, K)), grad, hess)
+
+
+def _hid(a: int, b: int) -> int:
+    return pairs(3).index(tuple(sorted((a, b))))
+
+
+def lift3d(out: Jet):
+    """Vector potential jets with columns 3k+c -> (u [n, 3, K] = curl A,
+    du [n, 3, 3, K] with du[:, a, j] = d_j u_a)  (NCC:97-99, VJ2:93-101)."""
+    n, KC = out.value.shape
+    K = KC // 3
+    g = out.grad.reshape(n, K, 3, 3)   # [n, k, comp, axis]
+    h = out.hess.reshape(n, K, 3, 6)   # [n, k, comp, pair]
+    u = np.stack((g[:, :, 2, 1] - g[:, :, 1, 2], g[:, :, 0, 2] - g[:, :, 2, 0],
+                  g[:, :, 1, 0] - g[:, :, 0, 1]), 1)
+    du = np.stack([np.stack([h[:, :, (a + 2) % 3, _hid(j, (a + 1) % 3)]
+                             - h[:, :, (a + 1) % 3, _hid(j, (a + 2) % 3)]
+                             for j in range(3)], 1) for a in range(3)], 1)
+    return u, du
+
+
+def lift3d_reverse(gu: np.ndarray, gd: np.ndarray) -> Jet:
+    """Transpose of lift3d (NCC:101-109 curl_adjoint, VCORE:32-38 lift)."""
+    n, _, K = gu.shape
+    g = np.zeros((n, K, 3, 3))
+    g[:, :, 2, 1] += gu[:, 0]; g[:, :, 1, 2] -= gu[:, 0]
+    g[:, :, 0, 2] += gu[:, 1]; g[:, :, 2, 0] -= gu[:, 1]
+    g[:, :, 1, 0] += gu[:, 2]; g[:, :, 0, 1] -= gu[:, 2]
+    h = np.zeros((n, K, 3, 6))
+    for a in range(3):
+        for j in range(3):
+            h[:, :, (a + 2) % 3, _hid(j, (a + 1) % 3)] += gd[:, a, j]
+            h[:, :, (a + 1) % 3, _hid(j, (a + 2) % 3)] -= gd[:, a, j]
+    return Jet(np.zeros((n, 3 * K)), g.reshape(n, 3 * K, 3), h.reshape(n, 3 * K, 6))
+
+
+def field_rows(f: np.ndarray) -> np.ndarray:
+    """[n, comp..., K] -> [n*comp, K], point-major, component-minor (VCORE:75-76)."""
+    return f.reshape(f.shape[0] * int(np.prod(f.shape[1:-1], dtype=np.int64)), f.shape[-1])
+
+
+# --------------------------------------------------------------------------
+# a6: Gram / stiffness assembly
+# --------------------------------------------------------------------------
+def gram(Phi: np.ndarray, w, block: int = 3072) -> np.ndarray:
+    """Blocked FP64 G = sum_r w_r Phi_r^T Phi_r (VCORE:40-46, metaball/run.py:99-107).
+    ``w`` is a scalar or a per-row weight vector."""
+    Phi = np.asarray(Phi)
+    w = np.broadcast_to(np.asarray(w, dtype=np.float64), (Phi.shape[0],))
+    G = np.zeros((Phi.shape[1], Phi.shape[1]))
+    for s in range(0, Phi.shape[0], block):
+        B = Phi[s:s + block].astype(np.float64)
+        G += B.T @ (w[s:s + block, None] * B)
+    return G
+
+
+# --------------------------------------------------------------------------
+# a7/a8: spectral objective and its adjoint
+# --------------------------------------------------------------------------
+def spectral_loss(M: np.ndarray, S: np.ndarray, lam: float = 1e-2, ridge: float = 0.0,
+                  cond_limit: float = 1e12):
+    """L = tr((M + ridge I)^-1 S) + lam ||M - I||_F^2 with the reference's
+    rank/condition gate (VJ2:107-117: nonfinite, nonpositive or condition
+    > 1e12 raises FloatingPointError).  Returns (L, dL/dM, dL/dS)."""
+    K = len(M)
+    Mr = 0.5 * (M + M.T) + ridge * np.eye(K)
+    e = np.linalg.eigvalsh(Mr)
+    if not np.all(np.isfinite(e)) or e[-1] <= 0 or e[0] / e[-1] < 1.0 / cond_limit:
+        raise FloatingPointError("mass matrix nonfinite, rank deficient or condition > limit")
+    L = np.linalg.cholesky(Mr)
+    inv = np.linalg.solve(L.T, np.linalg.solve(L, np.eye(K)))
+    inv = 0.5 * (inv + inv.T)
+    Ss = 0.5 * (S + S.T)
+    D = M - np.eye(K)
+    loss = float(np.trace(inv @ Ss) + lam * np.sum(D * D))
+    cross = inv @ Ss @ inv
+    dM = -0.5 * (cross + cross.T) + 2.0 * lam * D
+    dS = inv
+    return loss, dM, dS
+
+
+def field_adjoint(Phi: np.ndarray, w, dG: np.ndarray) -> np.ndarray:
+    """d/dPhi of <dG, Phi^T W Phi> for symmetric dG: 2 W Phi dG (VCORE:94-95)."""
+    w = np.asarray(w, dtype=np.float64)
+    if w.ndim:
+        w = w[:, None]
+    return 2.0 * w * (Phi @ dG)
+
+
+# --------------------------------------------------------------------------
+# a7: FP64 generalized symmetric eigendecomposition with the rank gate
+# (experiments/periodic/paper_rollout.py:180-188)
+# --------------------------------------------------------------------------
+def generalized_eigh(S: np.ndarray, M: np.ndarray, rank_tol: float = 1e-12):
+    import scipy.linalg as la
+    Ms = 0.5 * (M + M.T)
+    Ss = 0.5 * (S + S.T)
+    ev = np.linalg.eigvalsh(Ms)
+    rank = int(np.sum(ev > ev[-1] * rank_tol))
+    if rank != len(M):
+        raise ArithmeticError(f"Full-K rank gate: rank {rank} < K={len(M)}")
+    return la.eigh(Ss, Ms)
+
+
+# --------------------------------------------------------------------------
+# a12: full-mass projection (PP:141-177, experiments/2d/check_cpu.py:27-33)
+# --------------------------------------------------------------------------
+def project(M: np.ndarray, B: np.ndarray, T: np.ndarray, w, block: int = 2048):
+    """Two-pass streamed projection of targets T [R, m] onto basis B [R, K]:
+    loads = w B^T T, c = M^-1 loads, normal residual gate, then the direct
+    residual pass and the Gram identity check."""
+    K = M.shape[0]
+    R = B.shape[0]
+    w = np.broadcast_to(np.asarray(w, dtype=np.float64), (R,))
+    loads = np.zeros((K, T.shape[1]))
+    norms = np.zeros(T.shape[1])
+    for s in range(0, R, block):
+        b = B[s:s + block].astype(np.float64)
+        t = T[s:s + block].astype(np.float64)
+        ww = w[s:s + block, None]
+        loads += b.T @ (ww * t)
+        norms += np.sum(ww * t * t, 0)
+    c = np.linalg.solve(M, loads)
+    normal = np.linalg.norm(M @ c - loads, axis=0) / np.linalg.norm(loads, axis=0)
+    gram_square = norms - 2 * np.sum(c * loads, 0) + np.sum(c * (M @ c), 0)
+    direct = np.zeros(T.shape[1])
+    for s in range(0, R, block):
+        diff = B[s:s + block].astype(np.float64) @ c - T[s:s + block]
+        direct += np.sum(w[s:s + block, None] * diff * diff, 0)
+    return dict(loads=loads, target_norms=norms, coefficients=c, normal_residuals=normal,
+                direct_relative_L2=np.sqrt(direct / norms),
+                Gram_identity_errors=np.sqrt(np.maximum(gram_square, 0) / norms))
+
+
+# --------------------------------------------------------------------------
+# a11: Adam with global-norm clip (NCC:156-171; the shared trunk has one
+# parameter group, so the per-mode clip becomes one global clip)
+# --------------------------------------------------------------------------
+def adam(params, grads, m, v, t: int, lr: float, clip: float = 8.0,
+         b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+    norm = math.sqrt(sum(float(np.sum(np.asarray(g, dtype=np.float64) ** 2)) for g in grads))
+    if not math.isfinite(norm) or norm <= 0:
+        raise FloatingPointError("Invalid parameter gradients; no optimizer update")
+    scale = min(1.0, clip / max(norm, 1e-30))
+    out = []
+    for p, g, mm, vv in zip(params, grads, m, v):
+        g = g * scale
+        mm = b1 * mm + (1 - b1) * g
+        vv = b2 * vv + (1 - b2) * g * g
+        p = p - lr * (mm / (1 - b1 ** t)) / (np.sqrt(vv / (1 - b2 ** t)) + eps)
+        out.append((p, mm, vv))
+    return out, norm
+
+
+# --------------------------------------------------------------------------
+# synthetic inputs of the benchmark configurations
+# --------------------------------------------------------------------------
+def periodic_grid(n: int) -> np.ndarray:
+    """n x n grid on [0,2pi)^2, meshgrid indexing='xy' (ELC:26-28)."""
+    axis = np.arange(n) * 2 * np.pi / n
+    x, y = np.meshgrid(axis, axis, indexing="xy")
+    return np.column_stack((x.ravel(), y.ravel()))
+
+
+def blob_domain_points(count: int, seed: int, blobs: int = 6, sigma: float = 0.35,
+                       level: float = 0.5, batch: int = 1 << 20):
+    """Rejection-sample ``count`` points of [-1,1]^3 where
+    sum_b exp(-|x-c_b|^2/(2 sigma^2)) > level; centres uniform in [-0.6,0.6]^3."""
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(-0.6, 0.6, (blobs, 3))
+    out = []
+    have = 0
+    while have < count:
+        p = rng.uniform(-1.0, 1.0, (batch, 3))
+        s = np.zeros(batch)
+        for c in centres:
+            s += np.exp(-np.sum((p - c) ** 2, 1) / (2 * sigma * sigma))
+        keep = p[s > level]
+        out.append(keep)
+        have += len(keep)
+    return np.concatenate(out)[:count], centres
+
+
+def fourier_potential_targets(points: np.ndarray, count: int, seed: int, kmax: int = 6,
+                              waves: int = 16):
+    """Divergence-free targets u = curl A with A a Gaussian-weighted random
+    Fourier series (|k| <= kmax), analytic curl as in
+    experiments/metaball/large_eddy.py:23-30.  Returns [n, 3, count]."""
+    rng = np.random.default_rng(seed)
+    n = len(points)
+    out = np.zeros((n, 3, count))
+    for t in range(count):
+        ks = []
+        while len(ks) < waves:
+            k = rng.integers(-kmax, kmax + 1, 3)
+            if 0 < k @ k <= kmax * kmax:
+                ks.append(k)
+        ks = np.array(ks, dtype=np.float64) * (np.pi / 2)
+        amp = rng.normal(size=(waves, 3)) * np.exp(-np.sum(ks * ks, 1) / (2 * (np.pi * kmax / 4) ** 2))[:, None]
+        ph = rng.uniform(0, 2 * np.pi, waves)
+        arg = points @ ks.T + ph  # [n, waves]
+        c = np.cos(arg)
+        # A = sum_w amp_w sin(k.x+ph): dA_c/dx_j = sum_w amp_wc k_wj cos(.)
+        J = np.einsum("nw,wc,wj->ncj", c, amp, ks)
+        out[:, :, t] = np.stack((J[:, 2, 1] - J[:, 1, 2], J[:, 0, 2] - J[:, 2, 0],
+                                 J[:, 1, 0] - J[:, 0, 1]), 1)
+    return out

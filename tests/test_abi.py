@@ -1,0 +1,46 @@
+"""CPU checks of the C ABI: the built library loads and exports every entry
+point declared in include/neb_hip.h, with ctypes signatures for each."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from neb_amd import _hip as H
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "neb_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(neb_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_bound_surface():
+    decl = declared_symbols()
+    assert decl, "no declarations parsed"
+    assert sorted(H.SIGNATURES) == decl
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(H.LIB_PATH):
+        pytest.fail("libnebhip.so is not built (run python -m neb_amd.build)")
+    lib = ctypes.CDLL(H.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    L = H.load()
+    assert L.neb_abi_version() == 1
+
+
+def test_argument_errors_raise_without_touching_the_gpu():
+    H.load()
+    # unknown precision / dimension are rejected before any launch
+    with pytest.raises(ValueError):
+        H.call("neb_jet_layer_forward", 7, 2, 0, 8, 8, 8, 16, 8, 16, 8, 0, 1.0, 16, 16, 8, None)
+    with pytest.raises(ValueError):
+        H.call("neb_jet_layer_forward", 0, 4, 0, 8, 8, 8, 16, 8, 16, 8, 0, 1.0, 16, 16, 8, None)
+    with pytest.raises(ValueError):
+        H.call("neb_cross", 0, 0, None, None, None, None, None, None, 0, 0, 4, 4, None, 0, 1.0, 0.0, 16, 4, 16,
+               0, None)
+    assert H.load().neb_cross_workspace_size(1024, 1024, 131072, 1) > 0

@@ -1,0 +1,289 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Sizes are small enough for the oracle to finish in seconds.  Tolerances:
+FP64 kernels agree with the oracle to ~1e-10 relative; FP32 to ~1e-5; BF16
+(8-bit significand operands, fp32 accumulation) to ~3e-2; the split
+BF16 hi+lo Gram to <= 1e-5 relative Frobenius error against the FP64 Gram of
+the same fields.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import neb_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # CPU-only host: the gpu tier runs on the MI355X box
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import neb_amd as nb  # noqa: E402
+from neb_amd import network as nw  # noqa: E402
+
+DEV = "cuda"
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make(dim, K, widths, F, act="tanh", seed=0, master=torch.float64, omegas=None, bias=True):
+    lo, hi = ((0.0, 0.0), (2 * math.pi, 2 * math.pi)) if dim == 2 else ((-1.0,) * 3, (1.0,) * 3)
+    cfg = nb.NetConfig(dim=dim, frequencies=F, lo=lo, hi=hi, widths=tuple(widths), modes=K, activation=act,
+                       omegas=omegas)
+    net = nb.EigenfluidNet(cfg, seed=seed, device=DEV, master_dtype=master)
+    if bias:
+        rng = np.random.default_rng(seed + 100)
+        with torch.no_grad():
+            for b in net.b:
+                b.copy_(torch.as_tensor(rng.normal(size=b.shape) * 0.2, dtype=master))
+        net.touch()
+    onet = o.Net([w.detach().cpu().double().numpy() for w in net.W], [b.detach().cpu().double().numpy() for b in net.b],
+                 act, list(omegas) if omegas else [1.0] * len(widths))
+    return cfg, net, onet
+
+
+def points(dim, n, seed=1):
+    rng = np.random.default_rng(seed)
+    if dim == 2:
+        return rng.uniform(0, 2 * np.pi, (n, 2))
+    return rng.uniform(-1, 1, (n, 3))
+
+
+TOL = {"f64": 1e-10, "f32": 2e-5, "bf16": 3e-2}
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_feature_jets(prec, dim):
+    n, F = 77, 3
+    cfg, net, _ = make(dim, 4, [16], F)
+    x = points(dim, n)
+    tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS[prec])
+    X0 = tape.inputs[0].double().cpu().numpy()[:, :, :cfg.features]
+    ref = o.feature_jets(x, F, cfg.lo, cfg.hi).pack()
+    tol = {"f64": 1e-13, "f32": 1e-6, "bf16": 5e-3}[prec]
+    assert rel(X0, ref) < tol
+    assert np.all(tape.inputs[0].double().cpu().numpy()[:, :, cfg.features:] == 0)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("dim,act", [(2, "tanh"), (3, "tanh"), (2, "sin")])
+def test_forward_output_jets(prec, dim, act):
+    n, K, F = 203, 40 if dim == 2 else 12, 3
+    omegas = (1.5, 1.2, 1.1) if act == "sin" else None
+    cfg, net, onet = make(dim, K, [64, 48, 32], F, act=act, omegas=omegas)
+    x = points(dim, n)
+    tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS[prec])
+    v, g, h = nw.output_jets(net, tape)
+    out, _ = o.forward(onet, o.feature_jets(x, F, cfg.lo, cfg.hi))
+    tol = TOL[prec]
+    assert rel(v.cpu(), out.value) < tol
+    assert rel(g.cpu(), out.grad) < tol
+    assert rel(h.cpu(), out.hess) < tol * (3 if prec == "bf16" else 1)
+    # tape: pre-activation of layer 0
+    z0 = tape.pre[0].double().cpu().numpy()[:, :, :64]
+    ref_z0 = o.linear_jet(o.feature_jets(x, F, cfg.lo, cfg.hi), onet.W[0], onet.b[0])
+    om = onet.omegas[0]
+    ref_z0 = o.Jet(om * ref_z0.value, om * ref_z0.grad, om * ref_z0.hess).pack()
+    assert rel(z0, ref_z0) < tol
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_output_fields_lift(prec, dim):
+    n, F = 150, 2
+    K = 36 if dim == 2 else 20
+    cfg, net, onet = make(dim, K, [32, 32], F)
+    x = points(dim, n, seed=4)
+    tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS[prec])
+    f = nw.output_fields(net, tape)
+    out, _ = o.forward(onet, o.feature_jets(x, F, cfg.lo, cfg.hi))
+    if dim == 2:
+        u, w = o.lift2d(out)
+        Um, Ds = o.field_rows(u), w
+    else:
+        u, du = o.lift3d(out)
+        Um, Ds = o.field_rows(u), o.field_rows(du)
+    tol = TOL[prec]
+    assert rel(f.U.cpu(), Um) < tol
+    assert rel(f.D.cpu(), Ds) < tol * (2 if prec == "bf16" else 1)
+    # component-major layout holds the same rows, permuted
+    f1 = nw.output_fields(net, tape, layout=1)
+    cm = 2 if dim == 2 else 3
+    perm = f1.U.cpu().numpy().reshape(cm, n, K).transpose(1, 0, 2).reshape(n * cm, K)
+    assert np.array_equal(perm, f.U.cpu().numpy())
+
+
+@pytest.mark.parametrize("mode,tol", [("f64", 1e-13), ("f32", 2e-6), ("bf16x3", 1e-5), ("bf16", 1e-2)])
+@pytest.mark.parametrize("R,K", [(1000, 40), (4099, 130), (20000, 256)])
+def test_gram_modes_against_fp64(mode, tol, R, K):
+    rng = np.random.default_rng(R + K)
+    phi = rng.normal(size=(R, K)) * np.exp(rng.normal(size=K))[None, :]
+    dt = torch.float64 if mode == "f64" else torch.float32
+    t = torch.tensor(phi, dtype=dt, device=DEV)
+    G = nw.gram(t, K, 0.37, mode).cpu().numpy()
+    ref = o.gram(t.double().cpu().numpy(), 0.37)
+    assert rel(G, ref) < tol, rel(G, ref)
+    assert np.array_equal(G, G.T)
+
+
+def test_gram_accumulate_and_row_weights():
+    rng = np.random.default_rng(3)
+    phi = rng.normal(size=(3000, 64))
+    ww = rng.uniform(0.1, 1.0, 3000)
+    t = torch.tensor(phi, device=DEV)
+    G = nw.gram(t, 64, 2.0, "f64", roww=torch.tensor(ww, device=DEV))
+    assert rel(G.cpu(), 2.0 * o.gram(phi, ww)) < 1e-13
+    nw.gram(t, 64, 1.0, "f64", out=G, beta=1.0)
+    assert rel(G.cpu(), 2.0 * o.gram(phi, ww) + o.gram(phi, 1.0)) < 1e-13
+
+
+@pytest.mark.parametrize("mode", ["f64", "f32", "bf16"])
+def test_cross_segments_signs(mode):
+    rng = np.random.default_rng(9)
+    X1, Y1 = rng.normal(size=(513, 70)), rng.normal(size=(513, 33))
+    X2, Y2 = rng.normal(size=(1200, 70)), rng.normal(size=(1200, 33))
+    dt = torch.float64 if mode == "f64" else torch.float32
+    T = lambda a: torch.tensor(a, dtype=dt, device=DEV)  # noqa: E731
+    code = nw.GRAM_MODES[mode]
+    G = nw.cross([(T(X1), T(Y1), 513, 1.0), (T(X2), T(Y2), 1200, -0.5)], 70, 33, code, scale=3.0).cpu().numpy()
+    ref = 3.0 * (X1.T @ Y1 - 0.5 * X2.T @ Y2)
+    tol = {"f64": 1e-13, "f32": 2e-6, "bf16": 1e-2}[mode]
+    assert rel(G, ref) < tol
+
+
+def _oracle_step(onet, x, F, cfg, w, lam, ridge):
+    x0 = o.feature_jets(x, F, cfg.lo, cfg.hi)
+    out, tape = o.forward(onet, x0, record=True)
+    n = len(x)
+    K = cfg.modes
+    if cfg.dim == 2:
+        u, om = o.lift2d(out)
+        Pm, Ps = o.field_rows(u), om
+    else:
+        u, du = o.lift3d(out)
+        Pm, Ps = o.field_rows(u), o.field_rows(du)
+    M, S = o.gram(Pm, w), o.gram(Ps, w)
+    L, dM, dS = o.spectral_loss(M, S, lam, ridge)
+    gPm, gPs = o.field_adjoint(Pm, w, dM), o.field_adjoint(Ps, w, dS)
+    if cfg.dim == 2:
+        adj = o.lift2d_reverse(gPm.reshape(n, 2, K), gPs)
+    else:
+        adj = o.lift3d_reverse(gPm.reshape(n, 3, K), gPs.reshape(n, 3, 3, K))
+    gW, gb = o.backward(onet, tape, adj)
+    return L, M, S, gW, gb
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-9), ("f32", 1e-4), ("bf16", 6e-2)])
+@pytest.mark.parametrize("dim,act", [(2, "tanh"), (3, "tanh"), (2, "sin")])
+def test_spectral_step_gradients(prec, tol, dim, act):
+    F = 2
+    K = 24 if dim == 2 else 8
+    widths = [48, 40, 32]
+    omegas = (1.4, 1.2, 1.1) if act == "sin" else None
+    master = torch.float64 if prec == "f64" else torch.float32
+    cfg, net, onet = make(dim, K, widths, F, act=act, master=master, omegas=omegas)
+    x = points(dim, 300, seed=7)
+    w = 1.0 / 300
+    L, M, S, gW, gb = _oracle_step(onet, x, F, cfg, w, 1e-2, 0.0)
+    gm = "f64" if prec == "f64" else ("f32" if prec == "f32" else "bf16x3")
+    res = nb.compute(net, torch.tensor(x), w, prec=prec, gram_mode=gm, gate="cholesky")
+    assert rel(res["M"].cpu(), M) < tol
+    assert rel(res["S"].cpu(), S) < tol
+    assert abs(float(res["loss"]) - L) < tol * abs(L)
+    GW, Gb = net.split(res["grads"])
+    for a, b in zip(GW, gW):
+        assert rel(a.detach().cpu(), b) < tol * 3
+    for a, b in zip(Gb[:-1], gb[:-1]):
+        assert rel(a.detach().cpu(), b) < tol * 3
+    assert float(Gb[-1].abs().max()) == 0.0 and np.abs(gb[-1]).max() == 0.0
+
+
+def test_fd_directional_fp64_gpu():
+    """Config-C style check on the GPU path itself: FP64 handwritten gradient
+    vs central FD (h=1e-3) of the GPU FP64 loss along random unit directions."""
+    F, K = 2, 16
+    cfg, net, _ = make(2, K, [32, 32, 32], F, master=torch.float64)
+    x = torch.tensor(points(2, 1024, seed=3))
+    w = 4 * math.pi ** 2 / 1024
+    res = nb.compute(net, x, w, prec="f64", gram_mode="f64", gate="cholesky")
+    g = res["grads"].clone()
+    base = net.params.detach().clone()
+    rng = np.random.default_rng(5)
+    h = 1e-3
+    for _ in range(6):
+        d = torch.tensor(rng.normal(size=base.numel()), device=DEV)
+        d /= d.norm()
+        with torch.no_grad():
+            net.params.copy_(base + h * d)
+        net.touch()
+        Lp = float(nb.compute(net, x, w, prec="f64", gram_mode="f64", training=False, gate="cholesky")["loss"])
+        with torch.no_grad():
+            net.params.copy_(base - h * d)
+        net.touch()
+        Lm = float(nb.compute(net, x, w, prec="f64", gram_mode="f64", training=False, gate="cholesky")["loss"])
+        fd = (Lp - Lm) / (2 * h)
+        exact = float(g @ d)
+        assert abs(fd - exact) <= 1e-4 * max(abs(exact), 1e-3 * abs(float(res["loss"]))), (fd, exact)
+    with torch.no_grad():
+        net.params.copy_(base)
+    net.touch()
+
+
+def test_autograd_function_matches_closed_form():
+    F, K = 2, 12
+    cfg, net, _ = make(2, K, [32, 32], F, master=torch.float64)
+    x = torch.tensor(points(2, 256, seed=2))
+    w = 1.0 / 256
+    M, S = nb.JetGram.apply(net.params, x, net, w, "f64", "f64")
+    Mi = torch.linalg.inv(M)
+    loss = torch.trace(Mi @ S) + 1e-2 * torch.sum((M - torch.eye(K, dtype=M.dtype, device=DEV)) ** 2)
+    g, = torch.autograd.grad(loss, net.params)
+    res = nb.compute(net, x, w, prec="f64", gram_mode="f64", gate="cholesky")
+    assert rel(g.cpu(), res["grads"].cpu()) < 1e-9
+
+
+def test_adam_first_step_closed_form():
+    cfg, net, _ = make(2, 8, [16], 2, master=torch.float32)
+    rng = np.random.default_rng(82)
+    g = torch.tensor(rng.normal(size=net.params.numel()), dtype=torch.float32, device=DEV)
+    before = net.params.detach().clone()
+    norm = net.adam(g, 1e-4, clip=1e6)
+    assert abs(norm - float(g.double().norm())) < 1e-6 * norm
+    expect = before - 1e-4 * g / (g.abs() + 1e-8)
+    assert torch.allclose(net.params, expect, rtol=2e-6, atol=2e-7)
+    with pytest.raises(FloatingPointError):
+        net.adam(g * float("nan"), 1e-4)
+
+
+def test_projection_against_oracle():
+    rng = np.random.default_rng(77)
+    B = rng.normal(size=(3001, 24))
+    T = rng.normal(size=(3001, 20))
+    w = 0.125
+    M = o.gram(B, w)
+    ref = o.project(M, B, T, w)
+    Bt, Tt = torch.tensor(B, device=DEV), torch.tensor(T, device=DEV)
+    Mt = nw.gram(Bt, 24, w, "f64")
+    res = nb.project(Mt, Bt, Tt, w)
+    assert rel(res["coefficients"].cpu(), ref["coefficients"]) < 1e-12
+    assert rel(res["direct_relative_L2"].cpu(), ref["direct_relative_L2"]) < 1e-12
+    assert float((res["direct_relative_L2"] - res["Gram_identity_errors"]).abs().max()) < 1e-8
+    assert float(res["normal_residuals"].max()) < 1e-8
+
+
+def test_checkpoint_roundtrip(tmp_path):
+    cfg, net, _ = make(3, 8, [16, 16], 2, master=torch.float32)
+    net.epoch = 3
+    p = tmp_path / "ck.npz"
+    net.save(p)
+    cfg2, net2, _ = make(3, 8, [16, 16], 2, seed=9, master=torch.float32)
+    net2.load(p)
+    assert net2.epoch == 3 and torch.equal(net2.params, net.params)
+    z = np.load(p)
+    assert {"W_0", "b_2", "mW_1", "vb_0", "epoch"} <= set(z.files)

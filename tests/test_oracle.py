@@ -1,0 +1,337 @@
+"""Pins for the FP64 oracle: closed forms, finite differences, library routines.
+
+Each test is chosen so that a plausible slip in the oracle (a dropped term, a
+wrong sign or index, a transposed operand) fails it.  The reference has no
+stored arrays: its known-answer tests are restated here --
+experiments/2d/_backend/periodic_core/vortex_merger/canonical_gauge/scripts/
+verify_analytic.py:48-66,89-97 (Fourier basis Gram = I), tests/
+test_metaball_operators.py:22 (blocked Gram = dense), experiments/2d/
+check_cpu.py:27-33 (projection identity), tests/test_network_backends.py:74-95
+(reverse vs directional FD, first Adam step), experiments/metaball/_backend/
+flow/cases/metaball/potential_scalar/frozen/native_cupy_verify.py:49-60
+(manufactured curl / vorticity / divergence).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import neb_oracle as o
+
+
+def _net(d, F, widths, K, seed=0, act="tanh", omegas=None):
+    ncomp = 1 if d == 2 else 3
+    sizes = [2 * F * d] + list(widths) + [ncomp * K]
+    net = o.init_net(sizes, seed, act, omegas)
+    rng = np.random.default_rng(seed + 1)
+    for b in net.b:  # nonzero biases exercise the bias-on-value-rows rule
+        b[:] = rng.normal(size=b.shape) * 0.3
+    return net
+
+
+def _eval(net, x, F, lo, hi):
+    out, _ = o.forward(net, o.feature_jets(x, F, lo, hi))
+    return out
+
+
+def test_pairs_and_channels():
+    assert o.pairs(2) == [(0, 0), (0, 1), (1, 1)]
+    assert o.pairs(3) == [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    assert o.channels(2) == 6 and o.channels(3) == 10
+
+
+def test_feature_jets_closed_form_and_fd():
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 2 * np.pi, (7, 2))
+    j = o.feature_jets(x, 3, 0.0, 2 * np.pi)
+    # on [0, 2pi) the phase is exactly h*x (HB2P:576-589)
+    for a in range(2):
+        for h in range(1, 4):
+            col = a * 6 + 2 * (h - 1)
+            np.testing.assert_allclose(j.value[:, col], np.sin(h * x[:, a]), atol=1e-14)
+            np.testing.assert_allclose(j.value[:, col + 1], np.cos(h * x[:, a]), atol=1e-14)
+            np.testing.assert_allclose(j.grad[:, col, a], h * np.cos(h * x[:, a]), atol=1e-13)
+            np.testing.assert_allclose(j.hess[:, col + 1, 0 if a == 0 else 2],
+                                       -h * h * np.cos(h * x[:, a]), atol=1e-12)
+    # 3D on [-1,1]: FD of value -> grad and grad -> hess
+    x3 = rng.uniform(-1, 1, (5, 3))
+    j3 = o.feature_jets(x3, 2, -1.0, 1.0)
+    eps = 1e-6
+    for a in range(3):
+        e = np.zeros(3); e[a] = eps
+        jp, jm = o.feature_jets(x3 + e, 2, -1.0, 1.0), o.feature_jets(x3 - e, 2, -1.0, 1.0)
+        np.testing.assert_allclose((jp.value - jm.value) / (2 * eps), j3.grad[..., a], atol=2e-7)
+        for p, (i, k) in enumerate(o.pairs(3)):
+            if k == a:
+                np.testing.assert_allclose((jp.grad[..., i] - jm.grad[..., i]) / (2 * eps),
+                                           j3.hess[..., p], atol=5e-6)
+
+
+@pytest.mark.parametrize("d,act", [(2, "tanh"), (3, "tanh"), (2, "sin")])
+def test_forward_jets_match_spatial_fd(d, act):
+    F = 2
+    lo, hi = (0.0, 2 * np.pi) if d == 2 else (-1.0, 1.0)
+    omegas = [1.7, 1.3] if act == "sin" else None
+    net = _net(d, F, [9, 7], 3, seed=5, act=act, omegas=omegas)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(lo + 0.1, hi - 0.1, (6, d))
+    out = _eval(net, x, F, lo, hi)
+    eps = 1e-5
+    for a in range(d):
+        e = np.zeros(d); e[a] = eps
+        op, om = _eval(net, x + e, F, lo, hi), _eval(net, x - e, F, lo, hi)
+        np.testing.assert_allclose((op.value - om.value) / (2 * eps), out.grad[..., a], rtol=1e-7, atol=1e-8)
+        for p, (i, k) in enumerate(o.pairs(d)):
+            if k == a:
+                np.testing.assert_allclose((op.grad[..., i] - om.grad[..., i]) / (2 * eps),
+                                           out.hess[..., p], rtol=1e-6, atol=1e-7)
+
+
+def test_independent_scalar_forward():
+    """Independent brute-force forward (no jets) for the value channel
+    (tests/test_network_backends.py:56-71 pattern)."""
+    net = _net(2, 3, [8, 8], 4, seed=2)
+    x = np.random.default_rng(0).uniform(0, 2 * np.pi, (5, 2))
+    f = np.concatenate([np.stack((np.sin(h * x[:, a]), np.cos(h * x[:, a])), 1)
+                        for a in range(2) for h in range(1, 4)], 1)
+    h = f
+    for W, b in zip(net.W[:-1], net.b[:-1]):
+        h = np.tanh(h @ W.T + b)
+    ref = h @ net.W[-1].T + net.b[-1]
+    np.testing.assert_allclose(_eval(net, x, 3, 0.0, 2 * np.pi).value, ref, rtol=1e-13, atol=1e-13)
+
+
+def test_activation_reverse_is_transpose():
+    rng = np.random.default_rng(4)
+    for act in ("tanh", "sin"):
+        for d in (2, 3):
+            P = d * (d + 1) // 2
+            pre = o.Jet(rng.normal(size=(5, 4)), rng.normal(size=(5, 4, d)), rng.normal(size=(5, 4, P)))
+            dp = o.Jet(rng.normal(size=(5, 4)), rng.normal(size=(5, 4, d)), rng.normal(size=(5, 4, P)))
+            adj = o.Jet(rng.normal(size=(5, 4)), rng.normal(size=(5, 4, d)), rng.normal(size=(5, 4, P)))
+            eps = 1e-6
+            yp = o.activation_jet(act, o.Jet(pre.value + eps * dp.value, pre.grad + eps * dp.grad, pre.hess + eps * dp.hess))
+            ym = o.activation_jet(act, o.Jet(pre.value - eps * dp.value, pre.grad - eps * dp.grad, pre.hess - eps * dp.hess))
+            lhs = sum(np.sum(a * (p - m)) for a, p, m in zip(adj.pack(), yp.pack(), ym.pack())) / (2 * eps)
+            back = o.activation_jet_reverse(act, pre, adj)
+            rhs = sum(np.sum(a * b) for a, b in zip(back.pack(), dp.pack()))
+            assert abs(lhs - rhs) < 1e-7 * max(1, abs(rhs))
+
+
+def test_lift2d_fourier_closed_form():
+    """psi = sin(ax+by): u = (psi_y, -psi_x), omega = (a^2+b^2) psi."""
+    x = np.random.default_rng(1).uniform(0, 6, (9, 2))
+    a, b = 2.0, -3.0
+    ph = a * x[:, 0] + b * x[:, 1]
+    s, c = np.sin(ph), np.cos(ph)
+    jet = o.Jet(s[:, None], np.stack((a * c, b * c), -1)[:, None],
+                np.stack((-a * a * s, -a * b * s, -b * b * s), -1)[:, None])
+    u, w = o.lift2d(jet)
+    np.testing.assert_allclose(u[:, 0, 0], b * c, atol=1e-14)
+    np.testing.assert_allclose(u[:, 1, 0], -a * c, atol=1e-14)
+    np.testing.assert_allclose(w[:, 0], (a * a + b * b) * s, atol=1e-13)
+
+
+def test_lift3d_manufactured_curl():
+    """A = (sin 2y, sin 3z, sin 5x): exact curl, grad u, div u = 0
+    (native_cupy_verify.py:49-60)."""
+    rng = np.random.default_rng(2)
+    p = rng.uniform(-1, 1, (11, 3))
+    x, y, z = p.T
+    n = len(p)
+    val = np.stack((np.sin(2 * y), np.sin(3 * z), np.sin(5 * x)), 1)
+    g = np.zeros((n, 3, 3)); h = np.zeros((n, 3, 6))
+    g[:, 0, 1] = 2 * np.cos(2 * y); h[:, 0, 3] = -4 * np.sin(2 * y)
+    g[:, 1, 2] = 3 * np.cos(3 * z); h[:, 1, 5] = -9 * np.sin(3 * z)
+    g[:, 2, 0] = 5 * np.cos(5 * x); h[:, 2, 0] = -25 * np.sin(5 * x)
+    u, du = o.lift3d(o.Jet(val, g, h))
+    exact = np.stack((-3 * np.cos(3 * z), -5 * np.cos(5 * x), -2 * np.cos(2 * y)), 1)
+    np.testing.assert_allclose(u[:, :, 0], exact, atol=1e-13)
+    grad_exact = np.zeros((n, 3, 3))
+    grad_exact[:, 0, 2] = 9 * np.sin(3 * z)
+    grad_exact[:, 1, 0] = 25 * np.sin(5 * x)
+    grad_exact[:, 2, 1] = 4 * np.sin(2 * y)
+    np.testing.assert_allclose(du[..., 0], grad_exact, atol=1e-12)
+    np.testing.assert_allclose(np.trace(du[..., 0], axis1=1, axis2=2), 0, atol=1e-12)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_lift_reverse_adjoint_identity(d):
+    rng = np.random.default_rng(6)
+    n, K = 5, 4
+    P = d * (d + 1) // 2
+    KC = K if d == 2 else 3 * K
+    jet = o.Jet(rng.normal(size=(n, KC)), rng.normal(size=(n, KC, d)), rng.normal(size=(n, KC, P)))
+    if d == 2:
+        u, w = o.lift2d(jet)
+        gu, gw = rng.normal(size=u.shape), rng.normal(size=w.shape)
+        back = o.lift2d_reverse(gu, gw)
+        lhs = np.sum(u * gu) + np.sum(w * gw)
+    else:
+        u, du = o.lift3d(jet)
+        gu, gd = rng.normal(size=u.shape), rng.normal(size=du.shape)
+        back = o.lift3d_reverse(gu, gd)
+        lhs = np.sum(u * gu) + np.sum(du * gd)
+    rhs = sum(np.sum(a * b) for a, b in zip(back.pack(), jet.pack()))
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-12)
+
+
+def test_fourier_basis_gram_identity_and_eigenvalues():
+    """verify_analytic.py:48-66,89-97: the normalised Fourier basis on the
+    n x n grid has velocity Gram = I and stiffness Gram = diag(kx^2+ky^2)."""
+    n, k = 32, 16
+    extent = int(np.ceil(np.sqrt(k)))
+    wp = [(x, y) for x in range(extent + 1) for y in range(-extent, extent + 1) if x > 0 or (x == 0 and y > 0)]
+    wp = np.array(sorted(wp, key=lambda p: (p[0] ** 2 + p[1] ** 2, p[0], p[1]))[:k // 2], dtype=float)
+    pts = o.periodic_grid(n)
+    ph = pts @ wp.T
+    lam = np.repeat(np.sum(wp * wp, 1), 2)
+    scale = np.sqrt(2) / (2 * np.pi)
+    # psi_k = phi_k / sqrt(lam_k), phi = (cos, sin) pairs
+    val = np.stack((np.cos(ph), np.sin(ph)), -1).reshape(n * n, k) * scale
+    d1 = np.stack((-np.sin(ph), np.cos(ph)), -1).reshape(n * n, k) * scale
+    kx, ky = np.repeat(wp[:, 0], 2), np.repeat(wp[:, 1], 2)
+    grad = np.stack((d1 * kx, d1 * ky), -1)
+    hess = np.stack((-val * kx * kx, -val * kx * ky, -val * ky * ky), -1)
+    s = 1 / np.sqrt(lam)
+    jet = o.Jet(val * s, grad * s[:, None], hess * s[:, None])
+    u, w = o.lift2d(jet)
+    weight = (2 * np.pi / n) ** 2
+    M = o.gram(o.field_rows(u), weight, block=97)
+    S = o.gram(w, weight)
+    np.testing.assert_allclose(M, np.eye(k), atol=1e-12)
+    np.testing.assert_allclose(S, np.diag(lam), atol=1e-11)
+    vals, _ = o.generalized_eigh(S, M)
+    np.testing.assert_allclose(vals, np.sort(lam), rtol=1e-12)
+
+
+def test_gram_blocked_equals_dense_and_weights():
+    rng = np.random.default_rng(17)
+    B = rng.normal(size=(137, 5))
+    w = rng.uniform(0.1, 1, 137)
+    np.testing.assert_allclose(o.gram(B, w, block=31), B.T @ (w[:, None] * B), rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(o.gram(B, 0.25, block=7), 0.25 * np.einsum("ri,rj->ij", B, B), rtol=1e-13, atol=1e-13)
+
+
+def test_spectral_loss_gradient_fd_and_gate():
+    rng = np.random.default_rng(8)
+    K = 6
+    A = rng.normal(size=(20, K)); M = A.T @ A / 20
+    B = rng.normal(size=(30, K)); S = B.T @ B / 30
+    L, dM, dS = o.spectral_loss(M, S, lam=1e-2)
+    np.testing.assert_allclose(L, np.trace(np.linalg.solve(M, S)) + 1e-2 * np.sum((M - np.eye(K)) ** 2), rtol=1e-12)
+    eps = 1e-6
+    for _ in range(4):
+        E = rng.normal(size=(K, K)); E = E + E.T
+        Lp = o.spectral_loss(M + eps * E, S)[0]; Lm = o.spectral_loss(M - eps * E, S)[0]
+        np.testing.assert_allclose((Lp - Lm) / (2 * eps), np.sum(dM * E), rtol=1e-6)
+        Lp = o.spectral_loss(M, S + eps * E)[0]; Lm = o.spectral_loss(M, S - eps * E)[0]
+        np.testing.assert_allclose((Lp - Lm) / (2 * eps), np.sum(dS * E), rtol=1e-6)
+    sing = A[:, :3] @ rng.normal(size=(3, K))
+    with pytest.raises(FloatingPointError):
+        o.spectral_loss(sing.T @ sing, S)
+    # the ridge makes the rank-deficient case well posed
+    o.spectral_loss(sing.T @ sing, S, ridge=1e-3)
+
+
+def _step_loss(net, x0, w, d, lam, ridge):
+    out, tape = o.forward(net, x0, record=True)
+    if d == 2:
+        u, om = o.lift2d(out)
+        Pm, Ps = o.field_rows(u), om
+    else:
+        u, du = o.lift3d(out)
+        Pm, Ps = o.field_rows(u), o.field_rows(du)
+    M, S = o.gram(Pm, w), o.gram(Ps, w)
+    L, dM, dS = o.spectral_loss(M, S, lam, ridge)
+    return L, (out, tape, Pm, Ps, dM, dS, u)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_spectral_step_reverse_matches_central_fd(d):
+    """Config-C analogue at oracle size: handwritten reverse through the
+    spectral loss, lifts and second-order jets vs central FD (h=1e-3) along
+    random unit parameter directions (tests/test_network_backends.py:74-90)."""
+    F, K = 2, 3
+    lo, hi = (0.0, 2 * np.pi) if d == 2 else (-1.0, 1.0)
+    net = _net(d, F, [10, 8], K, seed=9)
+    rng = np.random.default_rng(12)
+    x = rng.uniform(lo, hi, (40, d))
+    w = 1.0 / len(x)
+    x0 = o.feature_jets(x, F, lo, hi)
+    lam, ridge = 1e-2, 0.0
+    L, (out, tape, Pm, Ps, dM, dS, u) = _step_loss(net, x0, w, d, lam, ridge)
+    gPm = o.field_adjoint(Pm, w, dM)
+    gPs = o.field_adjoint(Ps, w, dS)
+    n = len(x)
+    if d == 2:
+        adj = o.lift2d_reverse(gPm.reshape(n, 2, K), gPs)
+    else:
+        adj = o.lift3d_reverse(gPm.reshape(n, 3, K), gPs.reshape(n, 3, 3, K))
+    gW, gb = o.backward(net, tape, adj)
+    params = net.W + net.b
+    grads = gW + gb
+    h = 1e-3
+    for _ in range(8):
+        dirs = [rng.normal(size=p.shape) for p in params]
+        nrm = math.sqrt(sum(np.sum(q * q) for q in dirs))
+        dirs = [q / nrm for q in dirs]
+        exact = sum(np.sum(g * q) for g, q in zip(grads, dirs))
+        saved = [p.copy() for p in params]
+        for p, q in zip(params, dirs):
+            p += h * q
+        Lp = _step_loss(net, x0, w, d, lam, ridge)[0]
+        for p, s0, q in zip(params, saved, dirs):
+            p[...] = s0 - h * q
+        Lm = _step_loss(net, x0, w, d, lam, ridge)[0]
+        for p, s0 in zip(params, saved):
+            p[...] = s0
+        fd = (Lp - Lm) / (2 * h)
+        assert abs(fd - exact) <= 1e-4 * max(abs(exact), 1e-3 * abs(L)), (fd, exact)
+
+
+def test_projection_identity_and_dense_solve():
+    """experiments/2d/check_cpu.py:27-33 and PP:179-208."""
+    rng = np.random.default_rng(77)
+    B = rng.normal(size=(137, 8)); T = rng.normal(size=(137, 20)); w = rng.uniform(.1, 1, 137)
+    M = o.gram(B, w)
+    r = o.project(M, B, T, w, block=31)
+    C = np.linalg.solve(M, B.T @ (w[:, None] * T))
+    np.testing.assert_allclose(r["coefficients"], C, rtol=1e-12)
+    d = np.sum(w[:, None] * (B @ C - T) ** 2, 0)
+    np.testing.assert_allclose(d, np.sum(w[:, None] * T * T, 0) - np.sum(C * r["loads"], 0), rtol=1e-12)
+    np.testing.assert_allclose(r["direct_relative_L2"], r["Gram_identity_errors"], atol=1e-12)
+    assert np.all(r["normal_residuals"] < 1e-12)
+
+
+def test_adam_first_step_closed_form():
+    rng = np.random.default_rng(82)
+    ps = [rng.normal(size=(3, 4)), rng.normal(size=4)]
+    gs = [rng.normal(size=(3, 4)), rng.normal(size=4)]
+    out, norm = o.adam(ps, gs, [np.zeros_like(p) for p in ps], [np.zeros_like(p) for p in ps], 1, 1e-4, clip=1e6)
+    for (p1, _, _), p0, g in zip(out, ps, gs):
+        np.testing.assert_allclose(p1, p0 - 1e-4 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-14)
+    with pytest.raises(FloatingPointError):
+        o.adam(ps, [g * np.nan for g in gs], [np.zeros_like(p) for p in ps], [np.zeros_like(p) for p in ps], 1, 1e-4)
+
+
+def test_divergence_free_targets_fd():
+    """Analytic curl targets vs FD of the same construction; div = 0."""
+    rng = np.random.default_rng(5)
+    p = rng.uniform(-1, 1, (6, 3))
+    u = o.fourier_potential_targets(p, 2, seed=3)
+    eps = 1e-5
+    div = np.zeros((6, 2))
+    for a in range(3):
+        e = np.zeros(3); e[a] = eps
+        up = o.fourier_potential_targets(p + e, 2, seed=3)
+        um = o.fourier_potential_targets(p - e, 2, seed=3)
+        div += (up[:, a] - um[:, a]) / (2 * eps)
+    np.testing.assert_allclose(div, 0, atol=1e-6 * np.abs(u).max())
+
+
+def test_blob_domain_points():
+    pts, centres = o.blob_domain_points(1000, seed=0, batch=4096)
+    assert pts.shape == (1000, 3) and np.all(np.abs(pts) <= 1)
+    s = sum(np.exp(-np.sum((pts - c) ** 2, 1) / (2 * 0.35 ** 2)) for c in centres)
+    assert np.all(s > 0.5)
