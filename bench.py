@@ -1,0 +1,251 @@
+"""Benchmark: 2D periodic operator grid, K=1024 spectral training step.
+
+Workload (one step): 256^2 uniform grid on [0,2pi)^2 (65,536 points), sin/cos
+features at frequencies 1..8 (32 features), 4 hidden tanh layers of width
+256, K=1024 stream-function modes, Xavier-normal weights (seed 0):
+second-order jet forward -> lifted fields u=(psi_y,-psi_x) [131,072 x 1024]
+and omega=-lap psi [65,536 x 1024] -> FP64 mass and stiffness Grams
+Phi^T W Phi (BF16 hi+lo split MFMA) -> FP64 spectral objective
+tr((M+eps I)^-1 S) + 1e-2 ||M-I||_F^2 with its closed-form adjoint ->
+handwritten reverse pass to all parameters -> Adam.
+
+metric: point*modes/s = N*K / step time (whole job; inputs resident in HBM).
+Also reported: Gram TFLOP/s (algorithmic R*K*(K+1)) for the BF16 hi+lo and
+FP32 MFMA modes with their relative Frobenius error against the FP64 Gram of
+the same fields, the FP32 step, per-stage times, the roofline of the
+dominant kernel and the FP64 NumPy oracle timed on the host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_GRID = 256
+K_MODES = 1024
+WIDTHS = (256, 256, 256, 256)
+FREQS = 8
+PEAK = {"bf16": 2.5e15, "f32": 157.3e12, "f64": 78.6e12}
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host oracle baseline")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 comparison step")
+    return ap.parse_args()
+
+
+class StageTimer:
+    """HIP events on the launch stream (torch's current stream, which every
+    libnebhip call uses)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.events = {}
+
+    def start(self, name):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.events.setdefault(name, []).append([e, None])
+
+    def stop(self, name):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.events[name][-1][1] = e
+
+    def ms(self):
+        return {k: [a.elapsed_time(b) for a, b in v] for k, v in self.events.items()}
+
+
+def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None):
+    K = net.cfg.modes
+
+    def mark(fn, name):
+        if timer is None:
+            return fn()
+        timer.start(name)
+        r = fn()
+        timer.stop(name)
+        return r
+
+    def step():
+        p = nw.PRECISIONS[prec]
+        tape = mark(lambda: nw.trunk_forward(net, x, p), "trunk_forward")
+        fields = mark(lambda: nw.output_fields(net, tape), "output_fields")
+        M = mark(lambda: nw.gram(fields.mass, K, w, gram_mode), "gram_mass")
+        S = mark(lambda: nw.gram(fields.stiff, K, w, gram_mode), "gram_stiff")
+        loss, dM, dS = mark(lambda: sp.spectral_objective(M, S, 1e-2, ridge, gate="cholesky"), "objective_fp64")
+        g = mark(lambda: nw.backward_fields(net, tape, fields, dM, dS, w), "reverse")
+        mark(lambda: net.adam(g, 1e-4), "adam")
+        return loss
+
+    return step
+
+
+def cpu_baseline(seconds_target=15.0):
+    """FP64 NumPy oracle of the same step on a bounded sample of the grid
+    (first n_s points), timed on this host; scaled to point*modes/s."""
+    import numpy as np
+    from oracle import neb_oracle as o
+    pts = o.periodic_grid(N_GRID)
+    sizes = [4 * FREQS] + list(WIDTHS) + [K_MODES]
+    net = o.init_net(sizes, 0)
+    w = (2 * math.pi / N_GRID) ** 2
+    n_s = 128
+    t_total, n_done = 0.0, 0
+    while True:
+        x = pts[n_done % len(pts): n_done % len(pts) + n_s]
+        t0 = time.perf_counter()
+        x0 = o.feature_jets(x, FREQS, (0.0, 0.0), (2 * math.pi, 2 * math.pi))
+        out, tape = o.forward(net, x0, record=True)
+        u, om = o.lift2d(out)
+        Pm, Ps = o.field_rows(u), om
+        M, S = o.gram(Pm, w), o.gram(Ps, w)
+        dM = np.eye(K_MODES) * 1e-3  # the objective's K x K algebra is size-independent; fixed cotangents
+        gPm, gPs = o.field_adjoint(Pm, w, dM), o.field_adjoint(Ps, w, dM)
+        adj = o.lift2d_reverse(gPm.reshape(len(x), 2, K_MODES), gPs)
+        o.backward(net, tape, adj)
+        t_total += time.perf_counter() - t0
+        n_done += n_s
+        if t_total >= seconds_target:
+            break
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    return dict(value=n_done * K_MODES / t_total, unit="point*modes/s", cores=min(threads, cores), kind="port",
+                sample=f"{n_done} of the 65,536 grid points in {n_s}-point batches, full jet fwd + Grams + "
+                       f"reverse at K=1024 (FP64 NumPy oracle; K x K objective excluded), {t_total:.1f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build: the "
+              "point-sharded multi-GPU design is documented, not launched)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import neb_amd as nb
+    from neb_amd import network as nw
+    from neb_amd import spectral as sp
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    axis = np.arange(N_GRID) * 2 * np.pi / N_GRID
+    gx, gy = np.meshgrid(axis, axis, indexing="xy")
+    x = torch.tensor(np.column_stack((gx.ravel(), gy.ravel())), dtype=torch.float32, device=dev)
+    N = x.shape[0]
+    w = (2 * math.pi / N_GRID) ** 2
+    cfg = nb.NetConfig(dim=2, frequencies=FREQS, widths=WIDTHS, modes=K_MODES)
+    net = nb.EigenfluidNet(cfg, seed=0, device=dev)
+    K = K_MODES
+
+    # ridge: the K=1024 / width-256 mass is rank <= 256 by construction; a fixed
+    # ridge of 1e-4 * mean diag(M_0) makes the FP64 Cholesky well posed.
+    tape = nw.trunk_forward(net, x, nw.PRECISIONS["bf16"])
+    fields = nw.output_fields(net, tape)
+    M0 = nw.gram(fields.mass, K, w, "bf16x3")
+    ridge = 1e-4 * float(torch.diagonal(M0).mean())
+
+    # ---- Gram modes on the same fields: TFLOP/s and error vs FP64 ----------
+    phi64 = fields.mass[:, :K].double().contiguous()
+    G64 = nw.gram(phi64, K, w, "f64")
+    del phi64
+    gram_stats = {}
+    Rm = fields.mass.shape[0]
+    flops = Rm * K * (K + 1)
+    for mode in ("bf16x3", "f32"):
+        G = nw.gram(fields.mass, K, w, mode)
+        err = float(torch.linalg.norm(G - G64) / torch.linalg.norm(G64))
+        torch.cuda.synchronize()
+        reps = 5
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            nw.gram(fields.mass, K, w, mode, out=G)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / reps * 1e-3
+        gram_stats[mode] = dict(tflops=flops / t / 1e12, ms=t * 1e3, rel_err_vs_fp64=err)
+    del G64, tape, fields
+
+    # ---- FP32 comparison step ------------------------------------------------
+    f32_rate = None
+    if not args.no_f32:
+        step32 = make_step(nb, nw, sp, net, x, w, "f32", "f32", ridge)
+        for _ in range(1):
+            step32()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n32 = max(1, min(3, args.steps))
+        for _ in range(n32):
+            step32()
+        torch.cuda.synchronize()
+        f32_rate = N * K / ((time.perf_counter() - t0) / n32)
+
+    # ---- headline BF16 step --------------------------------------------------
+    step = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge)
+    for _ in range(args.warmup):
+        step()
+    timer = StageTimer(torch)
+    step_t = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, timer)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step_t()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    stages = {k: float(np.mean(v)) for k, v in timer.ms().items()}
+
+    # roofline: dominant kernel = the BF16 hi+lo mass+stiffness Gram
+    gram_ms = stages["gram_mass"] + stages["gram_stiff"]
+    gram_flops = (2 * N + N) * K * (K + 1)
+    achieved = gram_flops / (gram_ms * 1e-3)
+    roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK["bf16"] / 1e12, unit="TFLOP/s",
+                    frac=achieved / PEAK["bf16"], traffic=None,
+                    kernel="cross_kernel<CR_BF16X3> (mass R=131072 + stiffness R=65536, K=1024)")
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "point*modes/s, jet fwd+bwd + Gram spectral step, 2D K=1024",
+        "value": N * K / dt,
+        "unit": "point*modes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic: 256^2 periodic grid, Xavier-normal weights seed 0",
+        "config": {"workload": "2D periodic operator grid: 65,536 pts, sin/cos freqs 1..8, 4x256 tanh, K=1024, "
+                               "mass+stiffness Gram, spectral loss + reverse + Adam",
+                   "points": N, "modes": K, "widths": list(WIDTHS), "ridge": ridge},
+        "stages_ms": stages,
+        "gram": gram_stats,
+        "fp32_step_point_modes_per_s": f32_rate,
+        "loss": float(loss),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -49,6 +49,7 @@ template <> struct CrossPol<CR_F32> {
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 template <> struct CrossPol<CR_BF16> {
   using S = __bf16; using frag_t = bf16x8; using acc_t = f32x4; using outer_t = float;
@@ -56,6 +57,7 @@ template <> struct CrossPol<CR_BF16> {
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 template <> struct CrossPol<CR_BF16X3> : CrossPol<CR_BF16> {
   static constexpr int NPROD = 3, FLUSH = 256;
@@ -66,6 +68,7 @@ template <> struct CrossPol<CR_F64> {
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
 template <typename T> __device__ inline double todbl(T v) { return (double)v; }
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(256) void cross_kernel(CrossArgs a) {
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int li = wi + i * 16 + 4 * (lane >> 4) + r;
+        const int li = wi + i * 16 + Pol::drow(lane, r);
         const int lj = wj + j * 16 + fr;
         out[li * CR_TN + lj] = (double)outer[i][j][r];
       }
@@ -250,12 +253,11 @@ __global__ __launch_bounds__(256) void cross_reduce_kernel(const double* __restr
                                                             double* __restrict__ G, int64_t ldg) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= m * n) return;
-  const int64_t i = idx / n, j = idx % n;
-  int64_t ti = i / CR_TM, tj = j / CR_TN, li = i % CR_TM, lj = j % CR_TN;
-  if (symmetric && ti > tj) {
-    int64_t t = ti; ti = tj; tj = t;
-    t = li; li = lj; lj = t;
+  int64_t i = idx / n, j = idx % n;
+  if (symmetric && i > j) {  // read the upper triangle: exact symmetry
+    const int64_t t = i; i = j; j = t;
   }
+  const int64_t ti = i / CR_TM, tj = j / CR_TN, li = i % CR_TM, lj = j % CR_TN;
   int64_t tile;
   if (symmetric) tile = ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti);
   else tile = ti * tiles_n + tj;
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(256) void cross_reduce_kernel(const double* __restr
   const double* p = partial + tile * (CR_TM * CR_TN) + li * CR_TN + lj;
   const int64_t stride = (int64_t)ntiles * CR_TM * CR_TN;
   for (int q = 0; q < splits; ++q) s += p[q * stride];
-  double* g = G + i * ldg + j;
+  double* g = G + (idx / n) * ldg + (idx % n);
   *g = (beta == 0.0 ? 0.0 : beta * *g) + scale * s;
 }
 

@@ -10,8 +10,8 @@
 // round trip for the unactivated GEMM output.
 //
 // MFMA orientation: A = weight fragment (rows = features), B = input
-// fragment (cols = points), so lane l holds point l%16 and the four
-// consecutive features 4*(l/16)+r of each 16x16 block.
+// fragment (cols = points), so lane l holds point l%16 and four features
+// (rows drow(l, r), r = 0..3) of each 16x16 block.
 //
 // Reference operations replaced (paths relative to the reference root):
 //   packed Jet2 linear layer   HB2P:1005-1036, VJ2:20-27   (forward plans)
@@ -160,11 +160,11 @@ __global__ __launch_bounds__(256) void rowgemm_kernel(RowArgs a) {
 #pragma unroll
     for (int j = 0; j < PB; ++j) {
       const int64_t p = p0 + wp * PB * 16 + j * 16 + fr;
-      const int64_t fb = f0 + wn * FB * 16 + i * 16 + 4 * (lane >> 4);
+      const int64_t fb = f0 + wn * FB * 16 + i * 16;
       if (p >= a.npts) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t f = fb + r;
+        const int64_t f = fb + PR::drow(lane, r);
         if (f >= a.nout) continue;
         M v[NA];
 #pragma unroll

@@ -9,7 +9,9 @@
 // read from LDS tiles stored [row][k] (k contiguous): lane l reads row l%16 and
 // the KE consecutive k values starting at (l/16)*KE.  A and B fragments use the
 // same lane->k map, so the contraction is correct for any hardware k order.
-// The 16x16 result is held as: lane l, element r -> row 4*(l/16)+r, col l%16.
+// The 16x16 result is held as lane l, element r -> column l%16 and row
+// drow(l, r): 4*(l/16)+r for the fp32-accumulating instructions, (l/16)+4*r
+// for v_mfma_f64_16x16x4f64 (measured on gfx950).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -37,6 +39,7 @@ template <> struct Prec<PREC_BF16> {
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 template <> struct Prec<PREC_F32> {
@@ -52,6 +55,7 @@ template <> struct Prec<PREC_F32> {
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 template <> struct Prec<PREC_F64> {
@@ -67,6 +71,7 @@ template <> struct Prec<PREC_F64> {
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ static inline int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
 // ---------------------------------------------------------------------------
