@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host oracle baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 comparison step")
+    ap.add_argument("--config", default="B", choices=["B", "E"],
+                    help="B: 2D 256^2 grid K=1024 (default); E: 3D blob domain, 1,599,754 points, K=1024")
     return ap.parse_args()
 
 
@@ -69,8 +71,9 @@ class StageTimer:
         return {k: [a.elapsed_time(b) for a, b in v] for k, v in self.events.items()}
 
 
-def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None):
+def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=None):
     K = net.cfg.modes
+    solver = solver if solver is not None else sp.SpectralSolver(1e-2, ridge)
 
     def mark(fn, name):
         if timer is None:
@@ -86,7 +89,7 @@ def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None):
         fields = mark(lambda: nw.output_fields(net, tape), "output_fields")
         M = mark(lambda: nw.gram(fields.mass, K, w, gram_mode), "gram_mass")
         S = mark(lambda: nw.gram(fields.stiff, K, w, gram_mode), "gram_stiff")
-        loss, dM, dS = mark(lambda: sp.spectral_objective(M, S, 1e-2, ridge, gate="cholesky"), "objective_fp64")
+        loss, dM, dS = mark(lambda: solver(M, S), "objective_fp64")
         g = mark(lambda: nw.backward_fields(net, tape, fields, dM, dS, w), "reverse")
         mark(lambda: net.adam(g, 1e-4), "adam")
         return loss
@@ -131,12 +134,105 @@ def cpu_baseline(seconds_target=15.0):
                        f"reverse at K=1024 (FP64 NumPy oracle; K x K objective excluded), {t_total:.1f} s")
 
 
+N3D = 1_599_754
+CHUNK3D = 131_072
+
+
+def bench_3d(args):
+    """3D blob domain, K=1024 vector potential: two-pass chunked step
+    (pass 1: jets + lifted fields + FP64 Grams per 131,072-point chunk;
+    FP64 objective; pass 2: per-chunk recomputation + reverse) + Adam."""
+    import numpy as np
+    import torch
+
+    import neb_amd as nb
+    from neb_amd import network as nw
+    from neb_amd import spectral as sp
+    from neb_amd import synthetic as syn
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.perf_counter()
+    pts, _, vol = syn.blob_domain(N3D, seed=0)
+    print(f"bench E: sampled {N3D} blob-domain points in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    x = torch.tensor(pts, dtype=torch.float32, device=dev)
+    N, K = N3D, K_MODES
+    w = 1.0 / N
+    cfg = nb.NetConfig(dim=3, frequencies=6, lo=(-1.0,) * 3, hi=(1.0,) * 3, widths=WIDTHS, modes=K)
+    net = nb.EigenfluidNet(cfg, seed=0, device=dev)
+    # the 3K=3072-output net spans curls of 3*256 fields: rank(M) <= 768 < K,
+    # so the objective runs with a fixed ridge of 1e-4 * mean diag(M_0)
+    f0 = nw.output_fields(net, nw.trunk_forward(net, x[:CHUNK3D], nw.PRECISIONS["bf16"]))
+    M0 = nw.gram(f0.mass, K, w * N / CHUNK3D, "bf16x3")
+    ridge = 1e-4 * float(torch.diagonal(M0).mean())
+    # Gram rate on one chunk's fields (mass R=393,216, grad-u R=1,179,648)
+    gram_stats = {}
+    for mode in ("bf16x3", "f32"):
+        G = nw.gram(f0.stiff, K, w, mode)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        nw.gram(f0.mass, K, w, mode, out=G)
+        nw.gram(f0.stiff, K, w, mode, out=G)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) * 1e-3
+        gram_stats[mode] = dict(tflops=(12 * CHUNK3D) * K * (K + 1) / t / 1e12, ms=t * 1e3)
+    del f0, G
+    solver = sp.SpectralSolver(1e-2, ridge)
+
+    def step(prec, gmode, solv):
+        tt = {}
+        res = sp.chunked_step(net, x, w, prec, gmode, CHUNK3D, solv, True, tt)
+        net.adam(res["grads"], 1e-4)
+        return res, tt
+
+    f32_rate = None
+    if not args.no_f32:
+        s32 = sp.SpectralSolver(1e-2, ridge)
+        step("f32", "f32", s32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step("f32", "f32", s32)
+        torch.cuda.synchronize()
+        f32_rate = N * K / (time.perf_counter() - t0)
+    for _ in range(max(1, min(args.warmup, 2))):
+        step("bf16", "bf16x3", solver)
+    steps = max(1, min(args.steps, 3))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stages = {}
+    for _ in range(steps):
+        res, tt = step("bf16", "bf16x3", solver)
+        for k, v in tt.items():
+            stages[k] = stages.get(k, 0.0) + v / steps * 1e3
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out = {
+        "metric": "point*modes/s, jet fwd+bwd + Gram spectral step, 3D K=1024",
+        "value": N * K / dt, "unit": "point*modes/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16", "data": "synthetic: 6-blob domain in [-1,1]^3 (seed 0), Xavier-normal weights seed 0",
+        "config": {"workload": "3D blob domain: 1,599,754 pts, sin/cos freqs 1..6, 4x256 tanh, K=1024 vector "
+                               "potential, curl lift, mass+grad-u Gram, 131,072-pt chunks, spectral loss + "
+                               "reverse + Adam", "points": N, "modes": K, "chunk": CHUNK3D, "ridge": ridge,
+                   "domain_volume_estimate": vol},
+        "stages_ms": stages, "gram_one_chunk": gram_stats, "fp32_step_point_modes_per_s": f32_rate,
+        "loss": float(res["loss"]), "objective_path": solver.last_path,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build: the "
               "point-sharded multi-GPU design is documented, not launched)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "E":
+        bench_3d(args)
+        return
     import numpy as np
     import torch
 
@@ -199,11 +295,12 @@ def main():
         f32_rate = N * K / ((time.perf_counter() - t0) / n32)
 
     # ---- headline BF16 step --------------------------------------------------
-    step = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge)
+    solver = sp.SpectralSolver(1e-2, ridge)
+    step = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, solver=solver)
     for _ in range(args.warmup):
         step()
     timer = StageTimer(torch)
-    step_t = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, timer)
+    step_t = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, timer, solver=solver)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -241,6 +338,7 @@ def main():
         "gram": gram_stats,
         "fp32_step_point_modes_per_s": f32_rate,
         "loss": float(loss),
+        "objective_path": solver.last_path,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -8,8 +8,9 @@ include/neb_hip.h (libnebhip.so).
 """
 from .network import (EigenfluidNet, Fields, NetConfig, Tape, backward_fields, channels, cross, gram,
                       output_fields, output_jets, trunk_forward)
-from .spectral import JetGram, compute, generalized_eigh, project, spectral_objective
+from .spectral import (JetGram, SpectralSolver, basis_stream, chunked_step, compute, fullmass, generalized_eigh,
+                       project, project_streamed, spectral_objective)
 
 __all__ = ["EigenfluidNet", "Fields", "NetConfig", "Tape", "backward_fields", "channels", "cross", "gram",
-           "output_fields", "output_jets", "trunk_forward", "JetGram", "compute", "generalized_eigh", "project",
+           "output_fields", "output_jets", "trunk_forward", "JetGram", "SpectralSolver", "basis_stream", "chunked_step", "compute", "fullmass", "generalized_eigh", "project", "project_streamed",
            "spectral_objective"]

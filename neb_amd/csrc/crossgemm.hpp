@@ -19,6 +19,8 @@
 // register set every FLUSH rows, which bounds the length of any fp32
 // rounding chain; partials are then combined in FP64.
 #pragma once
+#include <type_traits>
+
 #include "neb_common.hpp"
 
 enum { CR_F32 = 0, CR_BF16 = 1, CR_BF16X3 = 2, CR_F64 = 3 };
@@ -53,7 +55,7 @@ template <> struct CrossPol<CR_F32> {
 };
 template <> struct CrossPol<CR_BF16> {
   using S = __bf16; using frag_t = bf16x8; using acc_t = f32x4; using outer_t = float;
-  static constexpr int KI = 32, KE = 8, RK = 32, PAD = 8, NPROD = 1, FLUSH = 512;
+  static constexpr int KI = 32, KE = 8, RK = 32, PAD = 8, NPROD = 1, FLUSH = 1 << 30;
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
@@ -76,60 +78,86 @@ template <> __device__ inline double todbl<__bf16>(__bf16 v) { return (double)(f
 template <typename T> __device__ inline float toflt(T v) { return (float)v; }
 template <> __device__ inline float toflt<__bf16>(__bf16 v) { return (float)v; }
 
-// Transposing tile load: rows [r0, r0+RK) x cols [c0, c0+TM) of a row-major
-// source -> LDS hi[col][row] (and lo[col][row] for the split mode).  Thread t
-// owns column t % TM and RK/2 consecutive rows, so each global load
-// instruction reads TM consecutive elements of one row.
+// Transposing tile loader: rows [r0, r0+RK) x cols [c0, c0+TM) of a
+// row-major source -> LDS hi[col][row] (and lo[col][row] for the split mode).
+// Thread t owns column t % TM and RK/2 consecutive rows, so each global load
+// instruction reads TM consecutive elements of one row.  fetch() fills
+// registers (issued one stage ahead of the MFMAs); commit() converts / splits
+// and writes LDS with 16-byte stores.
 template <int MODE, typename TIN>
-__device__ inline void cross_load(typename CrossPol<MODE>::S* hi, typename CrossPol<MODE>::S* lo,
-                                  const TIN* __restrict__ src, int64_t ld, int64_t r0, int64_t rows,
-                                  int64_t c0, int64_t cols, const double* roww) {
+struct CrossLoad {
   using Pol = CrossPol<MODE>;
   using S = typename Pol::S;
-  constexpr int RK = Pol::RK, LDK = RK + Pol::PAD, HR = RK / 2;
-  const int col = threadIdx.x % CR_TM;
-  const int rh = (threadIdx.x / CR_TM) * HR;
-  const int64_t gc = c0 + col;
-  const bool cok = gc < cols;
-  S* dh = hi + col * LDK + rh;
-  if constexpr (MODE == CR_F64) {
+  using V = typename std::conditional<MODE == CR_F64, double, float>::type;
+  static constexpr int RK = Pol::RK, LDK = RK + Pol::PAD, HR = RK / 2;
+  TIN v[HR];  // raw loads: conversion happens in commit(), after the MFMAs
+  int64_t r0_, rows_;
+
+  __device__ inline void fetch(const TIN* __restrict__ src, int64_t ld, int64_t r0, int64_t rows, int64_t c0,
+                               int64_t cols) {
+    const int col = threadIdx.x % CR_TM;
+    const int rh = (threadIdx.x / CR_TM) * HR;
+    const int64_t gc = c0 + col;
+    const bool cok = gc < cols;
 #pragma unroll
     for (int e = 0; e < HR; ++e) {
       const int64_t gr = r0 + rh + e;
-      double v = (cok && gr < rows) ? todbl(src[gr * ld + gc]) : 0.0;
-      if (roww && gr < rows) v *= roww[gr];
-      dh[e] = v;
+      v[e] = (cok && gr < rows) ? src[gr * ld + gc] : (TIN)0;
     }
-  } else {
-    float v[HR];
+    r0_ = r0;
+    rows_ = rows;
+  }
+
+  // scale: segment sign (applied before rounding to the MFMA operand type);
+  // roww: optional per-row weights
+  __device__ inline void commit(S* hi, S* lo, V scale, const double* roww) const {
+    const int col = threadIdx.x % CR_TM;
+    const int rh = (threadIdx.x / CR_TM) * HR;
+    V x[HR];
 #pragma unroll
-    for (int e = 0; e < HR; ++e) {
-      const int64_t gr = r0 + rh + e;
-      v[e] = (cok && gr < rows) ? toflt(src[gr * ld + gc]) : 0.0f;
-      if (roww && gr < rows) v[e] = (float)((double)v[e] * roww[gr]);
+    for (int e = 0; e < HR; ++e) x[e] = scale * (V)todbl(v[e]);
+    if (roww) {
+#pragma unroll
+      for (int e = 0; e < HR; ++e) {
+        const int64_t gr = r0_ + rh + e;
+        if (gr < rows_) x[e] = (V)((double)x[e] * roww[gr]);
+      }
     }
-    if constexpr (MODE == CR_F32) {
+    S* dh = hi + col * LDK + rh;
+    if constexpr (MODE == CR_F64) {
 #pragma unroll
-      for (int e = 0; e < HR; e += 4) *reinterpret_cast<f32x4*>(dh + e) = f32x4{v[e], v[e + 1], v[e + 2], v[e + 3]};
+      for (int e = 0; e < HR; e += 2) *reinterpret_cast<f64x2*>(dh + e) = f64x2{x[e], x[e + 1]};
+    } else if constexpr (MODE == CR_F32) {
+#pragma unroll
+      for (int e = 0; e < HR; e += 4) *reinterpret_cast<f32x4*>(dh + e) = f32x4{x[e], x[e + 1], x[e + 2], x[e + 3]};
     } else {
 #pragma unroll
       for (int e = 0; e < HR; e += 8) {
         bf16x8 h, l;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          __bf16 hq = (__bf16)v[e + q];
+          const __bf16 hq = (__bf16)x[e + q];
           h[q] = hq;
-          if constexpr (MODE == CR_BF16X3) l[q] = (__bf16)(v[e + q] - (float)hq);
+          if constexpr (MODE == CR_BF16X3) l[q] = (__bf16)(x[e + q] - (float)hq);
         }
         *reinterpret_cast<bf16x8*>(dh + e) = h;
         if constexpr (MODE == CR_BF16X3) *reinterpret_cast<bf16x8*>(lo + col * LDK + rh + e) = l;
       }
     }
   }
+};
+
+// Row range of split `split` within a segment of `rows` rows.
+template <int RK>
+__device__ inline void split_range(int64_t rows, int splits, int split, int64_t& rbeg, int64_t& rend) {
+  int64_t chunk = (rows + splits - 1) / splits;
+  chunk = (chunk + RK - 1) / RK * RK;
+  rbeg = (int64_t)split * chunk;
+  rend = rbeg + chunk < rows ? rbeg + chunk : rows;
 }
 
 template <int MODE, typename TX, typename TY>
-__global__ __launch_bounds__(256) void cross_kernel(CrossArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void cross_kernel(CrossArgs a) {
   using Pol = CrossPol<MODE>;
   using S = typename Pol::S;
   using frag_t = typename Pol::frag_t;
@@ -138,102 +166,151 @@ __global__ __launch_bounds__(256) void cross_kernel(CrossArgs a) {
   constexpr int RK = Pol::RK, LDK = RK + Pol::PAD, KI = Pol::KI, KE = Pol::KE;
   constexpr int NL = (MODE == CR_BF16X3) ? 2 : 1;  // hi (+ lo) tiles
   constexpr int TILE = CR_TM * LDK;
+  constexpr int STAGE = 2 * NL * TILE;
 
   extern __shared__ __align__(16) unsigned char smem[];
-  S* xs = reinterpret_cast<S*>(smem);  // [NL][TM][LDK]
-  S* ys = xs + NL * TILE;              // [NL][TN][LDK]
+  S* const buf0 = reinterpret_cast<S*>(smem);
+  S* const buf1 = buf0 + STAGE;
 
-  // tile coordinates (upper-triangular enumeration when symmetric)
-  int ti, tj;
+  // XCD-aware map: workgroup L runs on XCD L % 8.  All tiles of one row split
+  // go to the same XCD (consecutive L/8), so the rows they share are read from
+  // HBM once and then served from that XCD's L2.
+  int split, tile;
   {
-    int t = blockIdx.x;
-    if (a.symmetric) {
-      ti = 0;
-      int rowlen = a.tiles_n;
-      while (t >= rowlen) { t -= rowlen; ++ti; --rowlen; }
-      tj = ti + t;
+    const int L = blockIdx.x;
+    if (a.splits % 8 == 0) {
+      const int q = L >> 3;
+      split = (L & 7) + 8 * (q / a.ntiles);
+      tile = q % a.ntiles;
     } else {
-      ti = t / a.tiles_n;
-      tj = t % a.tiles_n;
+      split = L / a.ntiles;
+      tile = L % a.ntiles;
     }
   }
-  const int split = blockIdx.y;
+  int ti, tj;
+  if (a.symmetric) {
+    int t = tile, rowlen = a.tiles_n;
+    ti = 0;
+    while (t >= rowlen) { t -= rowlen; ++ti; --rowlen; }
+    tj = ti + t;
+  } else {
+    ti = tile / a.tiles_n;
+    tj = tile % a.tiles_n;
+  }
   const int64_t i0 = (int64_t)ti * CR_TM, j0 = (int64_t)tj * CR_TN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
   const int fr = lane & 15, kg = (lane >> 4) * KE;
 
+  // Two-level accumulation (MFMA registers flushed into `outer` every FLUSH
+  // rows) only where the fp32 rounding chain matters: the fp32 and split
+  // bf16 Gram modes.  Single bf16 products (weight gradients) and fp64 use
+  // the MFMA accumulators alone.
+  constexpr bool TWO = Pol::FLUSH < (1 << 20);
   acc_t inner[4][4];
-  outer_t outer[4][4][4];
+  outer_t outer[TWO ? 4 : 1][TWO ? 4 : 1][TWO ? 4 : 1];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      inner[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < 4; ++j) inner[i][j] = acc_t{0, 0, 0, 0};
+  if constexpr (TWO) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) outer[i][j][r] = 0;
-    }
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) outer[i][j][r] = 0;
+  }
 
-  for (int s = 0; s < a.nseg; ++s) {
-    const CrossSeg sg = a.seg[s];
-    int64_t chunk = (sg.rows + a.splits - 1) / a.splits;
-    chunk = (chunk + RK - 1) / RK * RK;
-    const int64_t rbeg = (int64_t)split * chunk;
-    const int64_t rend = rbeg + chunk < sg.rows ? rbeg + chunk : sg.rows;
-    if (rbeg >= rend) continue;
-    const TX* X = reinterpret_cast<const TX*>(sg.x);
-    const TY* Y = reinterpret_cast<const TY*>(sg.y);
-    int since = 0;
-    for (int64_t r0 = rbeg; r0 < rend; r0 += RK) {
-      cross_load<MODE, TX>(xs, xs + TILE, X, sg.ldx, r0, rend, i0, a.m, a.roww);
-      cross_load<MODE, TY>(ys, ys + TILE, Y, sg.ldy, r0, rend, j0, a.n, nullptr);
-      __syncthreads();
+  CrossLoad<MODE, TX> lx;
+  CrossLoad<MODE, TY> ly;
+  // flattened (segment, k-step) iteration with one-stage prefetch
+  int s = 0;
+  int64_t rbeg = 0, rend = 0, r0 = 0;
+  auto next_seg = [&](int from) {
+    for (s = from; s < a.nseg; ++s) {
+      split_range<RK>(a.seg[s].rows, a.splits, split, rbeg, rend);
+      if (rbeg < rend) { r0 = rbeg; return; }
+    }
+  };
+  next_seg(0);
+  if (s >= a.nseg) goto write_out;
+  lx.fetch(reinterpret_cast<const TX*>(a.seg[s].x), a.seg[s].ldx, r0, rend, i0, a.m);
+  ly.fetch(reinterpret_cast<const TY*>(a.seg[s].y), a.seg[s].ldy, r0, rend, j0, a.n);
+  lx.commit(buf0, buf0 + TILE, (typename CrossLoad<MODE, TX>::V)a.seg[s].sign, a.roww);
+  ly.commit(buf0 + NL * TILE, buf0 + NL * TILE + TILE, 1, nullptr);
+  __syncthreads();
+  {
+    int stage = 0, since = 0;
+    (void)since;
+    while (true) {
+      // position of the next k-step
+      const bool seg_last = r0 + RK >= rend;
+      int ns = s;
+      int64_t nr0 = r0 + RK;
+      if (seg_last) {
+        next_seg(s + 1);
+        ns = s;
+        nr0 = r0;
+      }
+      const bool more = ns < a.nseg;
+      if (more) {
+        lx.fetch(reinterpret_cast<const TX*>(a.seg[ns].x), a.seg[ns].ldx, nr0, rend, i0, a.m);
+        ly.fetch(reinterpret_cast<const TY*>(a.seg[ns].y), a.seg[ns].ldy, nr0, rend, j0, a.n);
+      }
+      const S* xs = stage ? buf1 : buf0;
+      const S* ys = xs + NL * TILE;
 #pragma unroll
       for (int kk = 0; kk < RK; kk += KI) {
-        frag_t xh[4], yh[4];
+        frag_t yh[4], yl[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) xh[i] = *reinterpret_cast<const frag_t*>(xs + (wi + i * 16 + fr) * LDK + kk + kg);
+        for (int j = 0; j < 4; ++j) {
+          yh[j] = *reinterpret_cast<const frag_t*>(ys + (wj + j * 16 + fr) * LDK + kk + kg);
+          if constexpr (MODE == CR_BF16X3) yl[j] = *reinterpret_cast<const frag_t*>(ys + TILE + (wj + j * 16 + fr) * LDK + kk + kg);
+        }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) yh[j] = *reinterpret_cast<const frag_t*>(ys + (wj + j * 16 + fr) * LDK + kk + kg);
-        if constexpr (MODE == CR_BF16X3) {
-          frag_t xl[4], yl[4];
+        for (int i = 0; i < 4; ++i) {
+          const frag_t xh = *reinterpret_cast<const frag_t*>(xs + (wi + i * 16 + fr) * LDK + kk + kg);
+          if constexpr (MODE == CR_BF16X3) {
+            const frag_t xl = *reinterpret_cast<const frag_t*>(xs + TILE + (wi + i * 16 + fr) * LDK + kk + kg);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) xl[i] = *reinterpret_cast<const frag_t*>(xs + TILE + (wi + i * 16 + fr) * LDK + kk + kg);
+            for (int j = 0; j < 4; ++j) {
+              inner[i][j] = Pol::mfma(xl, yh[j], inner[i][j]);
+              inner[i][j] = Pol::mfma(xh, yl[j], inner[i][j]);
+              inner[i][j] = Pol::mfma(xh, yh[j], inner[i][j]);
+            }
+          } else {
 #pragma unroll
-          for (int j = 0; j < 4; ++j) yl[j] = *reinterpret_cast<const frag_t*>(ys + TILE + (wj + j * 16 + fr) * LDK + kk + kg);
+            for (int j = 0; j < 4; ++j) inner[i][j] = Pol::mfma(xh, yh[j], inner[i][j]);
+          }
+        }
+      }
+      if constexpr (TWO) {
+        since += RK;
+        if (since >= Pol::FLUSH || !more) {
 #pragma unroll
           for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-              inner[i][j] = Pol::mfma(xl[i], yh[j], inner[i][j]);
-              inner[i][j] = Pol::mfma(xh[i], yl[j], inner[i][j]);
-              inner[i][j] = Pol::mfma(xh[i], yh[j], inner[i][j]);
+#pragma unroll
+              for (int r = 0; r < 4; ++r) outer[i][j][r] += (outer_t)inner[i][j][r];
+              inner[i][j] = acc_t{0, 0, 0, 0};
             }
-        } else {
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) inner[i][j] = Pol::mfma(xh[i], yh[j], inner[i][j]);
+          since = 0;
         }
       }
+      if (!more) break;
+      stage ^= 1;
+      S* nb = stage ? buf1 : buf0;
+      lx.commit(nb, nb + TILE, (typename CrossLoad<MODE, TX>::V)a.seg[ns].sign, a.roww);
+      ly.commit(nb + NL * TILE, nb + NL * TILE + TILE, 1, nullptr);
       __syncthreads();
-      since += RK;
-      if (since >= Pol::FLUSH || r0 + RK >= rend) {
-        const outer_t sgn = (outer_t)sg.sign;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) outer[i][j][r] += sgn * (outer_t)inner[i][j][r];
-            inner[i][j] = acc_t{0, 0, 0, 0};
-          }
-        since = 0;
-      }
+      r0 = nr0;
     }
   }
 
-  double* out = a.partial + ((int64_t)split * a.ntiles + blockIdx.x) * (CR_TM * CR_TN);
+write_out:
+  double* out = a.partial + ((int64_t)split * a.ntiles + tile) * (CR_TM * CR_TN);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -242,36 +319,92 @@ __global__ __launch_bounds__(256) void cross_kernel(CrossArgs a) {
       for (int r = 0; r < 4; ++r) {
         const int li = wi + i * 16 + Pol::drow(lane, r);
         const int lj = wj + j * 16 + fr;
-        out[li * CR_TN + lj] = (double)outer[i][j][r];
+        if constexpr (TWO) out[li * CR_TN + lj] = (double)outer[i][j][r];
+        else out[li * CR_TN + lj] = (double)inner[i][j][r];
       }
 }
 
-// G[i,j] = beta*G[i,j] + scale * sum_split partial  (mirrored when symmetric)
+// G = beta*G + scale * sum_split partial, over 32x32 output blocks.  For the
+// symmetric mode only blocks with bi <= bj run; each writes (i,j) and the
+// mirrored (j,i) through an LDS transpose, and diagonal blocks take every
+// value from the upper triangle, so G is exactly symmetric.  All global
+// reads and writes are row-contiguous.
 __global__ __launch_bounds__(256) void cross_reduce_kernel(const double* __restrict__ partial, int splits,
                                                             int ntiles, int tiles_n, int symmetric, int64_t m,
                                                             int64_t n, double scale, double beta,
                                                             double* __restrict__ G, int64_t ldg) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= m * n) return;
-  int64_t i = idx / n, j = idx % n;
-  if (symmetric && i > j) {  // read the upper triangle: exact symmetry
-    const int64_t t = i; i = j; j = t;
+  __shared__ double tr[32][33];
+  const int64_t nbj = (n + 31) / 32;
+  int64_t bi, bj;
+  if (symmetric) {
+    int64_t t = blockIdx.x, rowlen = nbj;
+    bi = 0;
+    while (t >= rowlen) { t -= rowlen; ++bi; --rowlen; }
+    bj = bi + t;
+  } else {
+    bi = blockIdx.x / nbj;
+    bj = blockIdx.x % nbj;
   }
-  const int64_t ti = i / CR_TM, tj = j / CR_TN, li = i % CR_TM, lj = j % CR_TN;
-  int64_t tile;
-  if (symmetric) tile = ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti);
-  else tile = ti * tiles_n + tj;
-  double s = 0.0;
-  const double* p = partial + tile * (CR_TM * CR_TN) + li * CR_TN + lj;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int64_t stride = (int64_t)ntiles * CR_TM * CR_TN;
-  for (int q = 0; q < splits; ++q) s += p[q * stride];
-  double* g = G + (idx / n) * ldg + (idx % n);
-  *g = (beta == 0.0 ? 0.0 : beta * *g) + scale * s;
+  double val[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = ty + 8 * q;
+    const int64_t i = bi * 32 + row, j = bj * 32 + tx;
+    double sacc = 0.0;
+    if (i < m && j < n) {
+      const int64_t ti = i / CR_TM, tj = j / CR_TN;
+      const int64_t tile = symmetric ? ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti) : ti * tiles_n + tj;
+      const double* p = partial + tile * (CR_TM * CR_TN) + (i % CR_TM) * CR_TN + (j % CR_TN);
+      int sp = 0;
+      for (; sp + 8 <= splits; sp += 8) {  // issue 8 independent loads, then add in order
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = p[(sp + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sacc += t[u];
+      }
+      for (; sp < splits; ++sp) sacc += p[sp * stride];
+    }
+    val[q] = sacc;
+    tr[row][tx] = sacc;
+  }
+  if (!symmetric) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t i = bi * 32 + ty + 8 * q, j = bj * 32 + tx;
+      if (i < m && j < n) {
+        double* g = G + i * ldg + j;
+        *g = (beta == 0.0 ? 0.0 : beta * *g) + scale * val[q];
+      }
+    }
+    return;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int row = ty + 8 * q;
+    const int64_t i = bi * 32 + row, j = bj * 32 + tx;
+    if (i < m && j < n) {
+      const double v = (bi == bj && row > tx) ? tr[tx][row] : val[q];
+      double* g = G + i * ldg + j;
+      *g = (beta == 0.0 ? 0.0 : beta * *g) + scale * v;
+    }
+    if (bi != bj) {
+      // mirrored block: G[bj*32 + row][bi*32 + tx] = value at (bi*32+tx, bj*32+row)
+      const int64_t mi = bj * 32 + row, mj = bi * 32 + tx;
+      if (mi < n && mj < m) {
+        double* g = G + mi * ldg + mj;
+        *g = (beta == 0.0 ? 0.0 : beta * *g) + scale * tr[tx][row];
+      }
+    }
+  }
 }
 
 template <int MODE>
-constexpr size_t cross_lds_bytes() {
+constexpr size_t cross_lds_bytes() {  // two stages of X and Y tiles (hi [+ lo])
   using Pol = CrossPol<MODE>;
   constexpr int NL = (MODE == CR_BF16X3) ? 2 : 1;
-  return sizeof(typename Pol::S) * (size_t)(Pol::RK + Pol::PAD) * CR_TM * NL * 2;
+  return 2 * sizeof(typename Pol::S) * (size_t)(Pol::RK + Pol::PAD) * CR_TM * NL * 2;
 }

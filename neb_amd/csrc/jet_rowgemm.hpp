@@ -57,39 +57,130 @@ template <> __device__ inline bf16x8 fneg<bf16x8>(bf16x8 v) {
   return __builtin_bit_cast(bf16x8, u);
 }
 
-// Cooperative load of an R x RK tile (rows r0.., k0..) of a row-major source
-// into LDS [R][LDK], converting TIN -> S.  Out-of-range rows / k are zero.
-template <int NTHR, int R, int RK, int LDK, typename TIN, typename S>
-__device__ inline void load_rows(S* __restrict__ lds, const TIN* __restrict__ src, int64_t ld,
-                                 int64_t r0, int64_t nrows, int64_t k0, int64_t kmax) {
-  constexpr int VE = 16 / sizeof(TIN);
+// Joint loader of NSRC tiles (channels or weight slices), each R rows x RK
+// k-values of a row-major source, into LDS [NSRC][R][LDK].  fetch() issues
+// the 16-byte global loads into registers (so they overlap the MFMAs of the
+// previous stage); commit() converts TIN -> S and writes LDS with one vector
+// store per load.  Rows >= nrows and k >= kmax read as zero.
+template <int NSRC, int R, int RK, int LDK, typename TIN, typename S>
+struct TileLoad {
+  static constexpr int VE = 16 / sizeof(TIN);
+  static constexpr int VPR = RK / VE;
+  static constexpr int PER = R * VPR;
+  static constexpr int NV = NSRC * PER;
+  static constexpr int NVT = (NV + 255) / 256;
   static_assert(RK % VE == 0, "tile k must be a multiple of the vector width");
-  constexpr int VPR = RK / VE;
-  constexpr int NV = R * VPR;
-  for (int v = threadIdx.x; v < NV; v += NTHR) {
-    const int row = v / VPR;
-    const int col = (v % VPR) * VE;
-    const int64_t gr = r0 + row;
-    const int64_t gk = k0 + col;
-    TIN vals[VE];
-    if (gr < nrows && gk + VE <= kmax) {
-      typedef TIN vec_t __attribute__((ext_vector_type(VE)));
-      vec_t t = *reinterpret_cast<const vec_t*>(src + gr * ld + gk);
+  typedef TIN vin __attribute__((ext_vector_type(VE)));
+  typedef S vout __attribute__((ext_vector_type(VE)));
+  typedef int i32x4 __attribute__((ext_vector_type(4)));
+  vin v[NVT];
+
+  __device__ inline void fetch(const void* const* src, const int64_t* ld, int64_t r0, int64_t nrows, int64_t k0,
+                               int64_t kmax) {
 #pragma unroll
-      for (int e = 0; e < VE; ++e) vals[e] = t[e];
-    } else {
+    for (int t = 0; t < NVT; ++t) {
+      const int idx = threadIdx.x + 256 * t;
+      vin val = __builtin_bit_cast(vin, i32x4{0, 0, 0, 0});
+      if (NV % 256 == 0 || idx < NV) {
+        const int s = idx / PER, rem = idx % PER;
+        const int row = rem / VPR, col = (rem % VPR) * VE;
+        const int64_t gr = r0 + row, gk = k0 + col;
+        // select the source with compile-time indices only (a dynamic index
+        // into the kernel-argument arrays would force a private copy)
+        const void* sp = src[0];
+        int64_t sld = ld[0];
 #pragma unroll
-      for (int e = 0; e < VE; ++e)
-        vals[e] = (gr < nrows && gk + e < kmax) ? src[gr * ld + gk + e] : (TIN)0;
+        for (int q = 1; q < NSRC; ++q)
+          if (s == q) { sp = src[q]; sld = ld[q]; }
+        if (gr < nrows) {
+          const TIN* p = reinterpret_cast<const TIN*>(sp) + gr * sld + gk;
+          if (gk + VE <= kmax) {
+            val = *reinterpret_cast<const vin*>(p);
+          } else {
+#pragma unroll
+            for (int e = 0; e < VE; ++e)
+              if (gk + e < kmax) val[e] = p[e];
+          }
+        }
+      }
+      v[t] = val;
     }
-    S* dst = lds + row * LDK + col;
+  }
+
+  __device__ inline void commit(S* lds) const {
 #pragma unroll
-    for (int e = 0; e < VE; ++e) dst[e] = (S)vals[e];
+    for (int t = 0; t < NVT; ++t) {
+      const int idx = threadIdx.x + 256 * t;
+      if (NV % 256 == 0 || idx < NV) {
+        const int s = idx / PER, rem = idx % PER;
+        const int row = rem / VPR, col = (rem % VPR) * VE;
+        vout o;
+#pragma unroll
+        for (int e = 0; e < VE; ++e) o[e] = (S)v[t][e];
+        *reinterpret_cast<vout*>(lds + (s * R + row) * LDK + col) = o;
+      }
+    }
+  }
+};
+
+// Group stores / loads of GE consecutive features (16-byte vectors).
+template <typename T, int GE, typename M>
+__device__ inline void st_group(T* dst, const M* v, int nvalid) {
+  constexpr int PV = (int)(16 / sizeof(T)) < GE ? (int)(16 / sizeof(T)) : GE;
+  typedef T pv __attribute__((ext_vector_type(PV)));
+  if (nvalid >= GE) {
+#pragma unroll
+    for (int c = 0; c < GE; c += PV) {
+      pv t;
+#pragma unroll
+      for (int e = 0; e < PV; ++e) t[e] = cvt<T>(v[c + e]);
+      *reinterpret_cast<pv*>(dst + c) = t;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < GE; ++e)
+      if (e < nvalid) dst[e] = cvt<T>(v[e]);
   }
 }
 
+template <typename T, int GE, typename M>
+__device__ inline void ld_group(const T* src, M* v, int nvalid) {
+  constexpr int PV = (int)(16 / sizeof(T)) < GE ? (int)(16 / sizeof(T)) : GE;
+  typedef T pv __attribute__((ext_vector_type(PV)));
+  if (nvalid >= GE) {
+#pragma unroll
+    for (int c = 0; c < GE; c += PV) {
+      const pv t = *reinterpret_cast<const pv*>(src + c);
+#pragma unroll
+      for (int e = 0; e < PV; ++e) v[c + e] = tomath<M>(t[e]);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < GE; ++e) v[e] = e < nvalid ? tomath<M>(src[e]) : (M)0;
+  }
+}
+
+template <int P, class Plan>
+constexpr size_t rowgemm_stage_elems() {
+  using PR = Prec<P>;
+  constexpr int LDK = PR::RK + PR::PAD;
+  constexpr int TN = Plan::WPN * Plan::FB * 16, TP = Plan::WPP * Plan::PB * 16;
+  return (size_t)LDK * (Plan::NW * TN + Plan::CI * TP);
+}
+
+template <int P, class Plan>
+constexpr bool rowgemm_double_buffered() {
+  return 2 * rowgemm_stage_elems<P, Plan>() * sizeof(typename Prec<P>::store_t) <= 96 * 1024;
+}
+
+template <int P, class Plan>
+constexpr size_t rowgemm_lds_bytes() {
+  return (rowgemm_double_buffered<P, Plan>() ? 2 : 1) * rowgemm_stage_elems<P, Plan>() *
+         sizeof(typename Prec<P>::store_t);
+}
+
 template <int P, typename TIN, class Plan>
-__global__ __launch_bounds__(256) void rowgemm_kernel(RowArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void rowgemm_kernel(RowArgs a) {
   using PR = Prec<P>;
   using S = typename PR::store_t;
   using M = typename PR::math_t;
@@ -99,16 +190,37 @@ __global__ __launch_bounds__(256) void rowgemm_kernel(RowArgs a) {
   constexpr int CI = Plan::CI, NW = Plan::NW, NA = Plan::NA, NT = Plan::NT;
   constexpr int FB = Plan::FB, PB = Plan::PB, WPN = Plan::WPN, WPP = Plan::WPP;
   static_assert(WPN * WPP == 4, "four waves per workgroup");
+  static_assert(FB % 2 == 0, "epilogue groups pair feature blocks");
   constexpr int TN = WPN * FB * 16, TP = WPP * PB * 16;
+  constexpr int WT = TN * LDK, XT = TP * LDK;
+  constexpr int STAGE = NW * WT + CI * XT;
+  constexpr bool DB = rowgemm_double_buffered<P, Plan>();
+  constexpr int GE = 8;  // epilogue group: 8 consecutive features
 
   extern __shared__ __align__(16) unsigned char smem[];
-  S* ws = reinterpret_cast<S*>(smem);    // [NW][TN][LDK]
-  S* xs = ws + NW * TN * LDK;            // [CI][TP][LDK]
+  S* buf0 = reinterpret_cast<S*>(smem);
+  S* buf1 = DB ? buf0 + STAGE : buf0;
+
+  // XCD-aware tile map: workgroup L runs on XCD L % 8; all feature tiles of
+  // one point block go to the same XCD so its input rows are read from HBM
+  // once and served from that XCD's L2 afterwards.
+  const int64_t npb = (a.npts + TP - 1) / TP, nfb = (a.nout + TN - 1) / TN;
+  int64_t pb, fbk;
+  {
+    const int64_t L = blockIdx.x;
+    if (npb % 8 == 0) {
+      const int64_t q = L >> 3;
+      pb = (L & 7) + 8 * (q / nfb);
+      fbk = q % nfb;
+    } else {
+      pb = L / nfb;
+      fbk = L % nfb;
+    }
+  }
+  const int64_t p0 = pb * TP, f0 = fbk * TN;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wn = wave % WPN, wp = wave / WPN;
-  const int64_t p0 = (int64_t)blockIdx.x * TP;
-  const int64_t f0 = (int64_t)blockIdx.y * TN;
 
   acc_t acc[NA][FB][PB];
 #pragma unroll
@@ -118,17 +230,29 @@ __global__ __launch_bounds__(256) void rowgemm_kernel(RowArgs a) {
 #pragma unroll
       for (int j = 0; j < PB; ++j) acc[s][i][j] = acc_t{0, 0, 0, 0};
 
+  TileLoad<NW, TN, RK, LDK, S, S> lw;
+  TileLoad<CI, TP, RK, LDK, TIN, S> lx;
   const int fr = lane & 15, kg = (lane >> 4) * KE;
-  for (int64_t k0 = 0; k0 < a.k; k0 += RK) {
-#pragma unroll
-    for (int s = 0; s < NW; ++s)
-      load_rows<256, TN, RK, LDK, S, S>(ws + s * TN * LDK, reinterpret_cast<const S*>(a.w[s]), a.ldw[s],
-                                        f0, a.nout, k0, a.k);
-#pragma unroll
-    for (int c = 0; c < CI; ++c)
-      load_rows<256, TP, RK, LDK, TIN, S>(xs + c * TP * LDK, reinterpret_cast<const TIN*>(a.x[c]), a.ldx[c],
-                                          p0, a.npts, k0, a.k);
-    __syncthreads();
+  // A-operand row fr of block i holds feature 4*FB*g(fr) + 4*i + r(fr), so the
+  // result lane (fr', g) owns 4*FB consecutive features of one point.
+  const int wrow = wn * FB * 16 + 4 * FB * PR::rowg(fr) + PR::rowr(fr);
+  const int nk = (int)((a.k + RK - 1) / RK);
+
+  lw.fetch(a.w, a.ldw, f0, a.nout, 0, a.k);
+  lx.fetch(a.x, a.ldx, p0, a.npts, 0, a.k);
+  lw.commit(buf0);
+  lx.commit(buf0 + NW * WT);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    S* cur = (DB && (kt & 1)) ? buf1 : buf0;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      const int64_t k1 = (int64_t)(kt + 1) * RK;
+      lw.fetch(a.w, a.ldw, f0, a.nout, k1, a.k);
+      lx.fetch(a.x, a.ldx, p0, a.npts, k1, a.k);
+    }
+    const S* ws = cur;
+    const S* xs = cur + NW * WT;
 #pragma unroll
     for (int kk = 0; kk < RK; kk += KI) {
       frag_t wf[NW][FB], xf[CI][PB];
@@ -136,12 +260,12 @@ __global__ __launch_bounds__(256) void rowgemm_kernel(RowArgs a) {
       for (int s = 0; s < NW; ++s)
 #pragma unroll
         for (int i = 0; i < FB; ++i)
-          wf[s][i] = *reinterpret_cast<const frag_t*>(ws + (s * TN + wn * FB * 16 + i * 16 + fr) * LDK + kk + kg);
+          wf[s][i] = *reinterpret_cast<const frag_t*>(ws + s * WT + (wrow + 4 * i) * LDK + kk + kg);
 #pragma unroll
       for (int c = 0; c < CI; ++c)
 #pragma unroll
         for (int j = 0; j < PB; ++j)
-          xf[c][j] = *reinterpret_cast<const frag_t*>(xs + (c * TP + wp * PB * 16 + j * 16 + fr) * LDK + kk + kg);
+          xf[c][j] = *reinterpret_cast<const frag_t*>(xs + c * XT + (wp * PB * 16 + j * 16 + fr) * LDK + kk + kg);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -152,61 +276,91 @@ __global__ __launch_bounds__(256) void rowgemm_kernel(RowArgs a) {
             acc[Plan::ta[t]][i][j] = PR::mfma(wv, xf[Plan::tc[t]][j], acc[Plan::ta[t]][i][j]);
         }
     }
+    if (more) {
+      if (!DB) __syncthreads();
+      S* nxt = (DB && !(kt & 1)) ? buf1 : buf0;
+      lw.commit(nxt);
+      lx.commit(nxt + NW * WT);
+    }
     __syncthreads();
   }
 
+  const int g = lane >> 4;
 #pragma unroll
-  for (int i = 0; i < FB; ++i)
+  for (int j = 0; j < PB; ++j) {
+    const int64_t p = p0 + wp * PB * 16 + j * 16 + fr;
 #pragma unroll
-    for (int j = 0; j < PB; ++j) {
-      const int64_t p = p0 + wp * PB * 16 + j * 16 + fr;
-      const int64_t fb = f0 + wn * FB * 16 + i * 16;
-      if (p >= a.npts) continue;
+    for (int q = 0; q < FB / 2; ++q) {
+      const int64_t f = f0 + wn * FB * 16 + 4 * FB * g + 8 * q;
+      const int64_t rem = a.nout - f;
+      if (p < a.npts && rem > 0) {
+        const int nvalid = rem >= GE ? GE : (int)rem;
+        M v[NA][GE];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int64_t f = fb + PR::drow(lane, r);
-        if (f >= a.nout) continue;
-        M v[NA];
+        for (int s = 0; s < NA; ++s)
 #pragma unroll
-        for (int s = 0; s < NA; ++s) v[s] = (M)acc[s][i][j][r];
-        Plan::template epilogue<P>(a, p, f, v);
+          for (int e = 0; e < GE; ++e) v[s][e] = (M)acc[s][2 * q + (e >> 2)][j][e & 3];
+        Plan::template epilogue<P, GE>(a, p, f, nvalid, v);
       }
     }
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Plans
+// Plans.  epilogue<P, GE>(a, p, f, nvalid, v): v[s][e] is accumulator set s
+// at point p, feature f+e (e < nvalid <= GE).
 // ---------------------------------------------------------------------------
 template <int D> struct TileFor;
 template <> struct TileFor<2> { static constexpr int FB = 4, PB = 1, WPN = 2, WPP = 2; };
 template <> struct TileFor<3> { static constexpr int FB = 2, PB = 1, WPN = 2, WPP = 2; };
+
+#define IDENTITY_TERMS                                             \
+  static constexpr int ta[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};    \
+  static constexpr int tc[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};    \
+  static constexpr int tw[10] = {0};                               \
+  static constexpr bool tneg[10] = {false};
 
 // Hidden layer forward: Z = omega*(X W^T + b) (all channels), Y = f-jet(Z).
 template <int D, int ACT> struct PlanHiddenFwd {
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
-  static constexpr int ta[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-  static constexpr int tc[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-  static constexpr int tw[10] = {0};
-  static constexpr bool tneg[10] = {false};
-  template <int P> __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f,
-                                                         typename Prec<P>::math_t* v) {
-    using M = typename Prec<P>::math_t;
+  IDENTITY_TERMS
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     using S = typename Prec<P>::store_t;
-    M u[C], y[C];
+    using L = Jet2Layout<D>;
     const M om = (M)a.omega;
-    if (a.bias) v[0] += reinterpret_cast<const M*>(a.bias)[f];
+    const M* bias = reinterpret_cast<const M*>(a.bias);
+    M u[C][GE];
 #pragma unroll
-    for (int c = 0; c < C; ++c) u[c] = om * v[c];
-    act_jet_forward<D, ACT, M>(u, y);
-    S* z = reinterpret_cast<S*>(a.out0);
-    S* o = reinterpret_cast<S*>(a.out1);
+    for (int e = 0; e < GE; ++e) {
+      const M b = (bias && e < nv) ? bias[f + e] : (M)0;
+      u[0][e] = om * (v[0][e] + b);
+#pragma unroll
+      for (int c = 1; c < C; ++c) u[c][e] = om * v[c][e];
+    }
     const int64_t off = p * a.ldo + f;
+    S* z = reinterpret_cast<S*>(a.out0) + off;
+    S* y = reinterpret_cast<S*>(a.out1) + off;
 #pragma unroll
-    for (int c = 0; c < C; ++c) {
-      z[c * a.cstride + off] = cvt<S>(u[c]);
-      o[c * a.cstride + off] = cvt<S>(y[c]);
+    for (int c = 0; c < C; ++c) st_group<S, GE>(z + c * a.cstride, u[c], nv);
+    M f1[GE], f2[GE], t[GE];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) Act<ACT, M>::eval(u[0][e], t[e], f1[e], f2[e]);
+    st_group<S, GE>(y, t, nv);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+#pragma unroll
+      for (int e = 0; e < GE; ++e) t[e] = f1[e] * u[1 + c][e];
+      st_group<S, GE>(y + (1 + c) * a.cstride, t, nv);
+    }
+#pragma unroll
+    for (int q = 0; q < L::P; ++q) {
+#pragma unroll
+      for (int e = 0; e < GE; ++e)
+        t[e] = f1[e] * u[1 + D + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
+      st_group<S, GE>(y + (1 + D + q) * a.cstride, t, nv);
     }
   }
 };
@@ -216,18 +370,16 @@ template <int D> struct PlanOutJets {
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
-  static constexpr int ta[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-  static constexpr int tc[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-  static constexpr int tw[10] = {0};
-  static constexpr bool tneg[10] = {false};
-  template <int P> __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f,
-                                                         typename Prec<P>::math_t* v) {
-    using M = typename Prec<P>::math_t;
-    if (a.bias) v[0] += reinterpret_cast<const M*>(a.bias)[f];
-    M* o = reinterpret_cast<M*>(a.out0);
-    const int64_t off = p * a.ldo + f;
+  IDENTITY_TERMS
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    const M* bias = reinterpret_cast<const M*>(a.bias);
+    if (bias)
 #pragma unroll
-    for (int c = 0; c < C; ++c) o[c * a.cstride + off] = v[c];
+      for (int e = 0; e < GE; ++e) v[0][e] += e < nv ? bias[f + e] : (M)0;
+    M* o = reinterpret_cast<M*>(a.out0) + p * a.ldo + f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) st_group<M, GE>(o + c * a.cstride, v[c], nv);
   }
 };
 
@@ -241,19 +393,15 @@ struct PlanOutFields2 {
   static constexpr int tc[NT] = {1, 0, 2, 3};
   static constexpr int tw[NT] = {0, 0, 0, 0};
   static constexpr bool tneg[NT] = {false, true, true, true};
-  template <int P> __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f,
-                                                         typename Prec<P>::math_t* v) {
-    using M = typename Prec<P>::math_t;
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     M* mass = reinterpret_cast<M*>(a.out0);
     M* stiff = reinterpret_cast<M*>(a.out1);
-    if (a.layout == 0) {
-      mass[(2 * p) * a.ldo + f] = v[0];
-      mass[(2 * p + 1) * a.ldo + f] = v[1];
-    } else {
-      mass[p * a.ldo + f] = v[0];
-      mass[(a.npts + p) * a.ldo + f] = v[1];
-    }
-    stiff[p * a.ldo2 + f] = v[2];
+    const int64_t r0 = a.layout == 0 ? 2 * p : p;
+    const int64_t r1 = a.layout == 0 ? 2 * p + 1 : a.npts + p;
+    st_group<M, GE>(mass + r0 * a.ldo + f, v[0], nv);
+    st_group<M, GE>(mass + r1 * a.ldo + f, v[1], nv);
+    st_group<M, GE>(stiff + p * a.ldo2 + f, v[2], nv);
   }
 };
 
@@ -288,24 +436,49 @@ struct PlanOutFields3 {
 #undef A2
   static constexpr bool tneg[NT] = {false, true, false, true, false, true, false, true, false, true, false, true,
                                     false, true, false, true, false, true, false, true, false, true, false, true};
-  template <int P> __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f,
-                                                         typename Prec<P>::math_t* v) {
-    using M = typename Prec<P>::math_t;
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     M* U = reinterpret_cast<M*>(a.out0);
     M* Dg = reinterpret_cast<M*>(a.out1);
-    if (a.layout == 0) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) U[(3 * p + c) * a.ldo + f] = v[c];
+    for (int c = 0; c < 3; ++c) {
+      const int64_t r = a.layout == 0 ? 3 * p + c : c * a.npts + p;
+      st_group<M, GE>(U + r * a.ldo + f, v[c], nv);
+    }
 #pragma unroll
-      for (int c = 0; c < 9; ++c) Dg[(9 * p + c) * a.ldo2 + f] = v[3 + c];
-    } else {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) U[(c * a.npts + p) * a.ldo + f] = v[c];
-#pragma unroll
-      for (int c = 0; c < 9; ++c) Dg[(c * a.npts + p) * a.ldo2 + f] = v[3 + c];
+    for (int c = 0; c < 9; ++c) {
+      const int64_t r = a.layout == 0 ? 9 * p + c : c * a.npts + p;
+      st_group<M, GE>(Dg + r * a.ldo2 + f, v[3 + c], nv);
     }
   }
 };
+
+// Shared epilogue of the reverse plans: given the adjoint of the activation
+// output (adj[C][GE]), apply the transposed jet chain with the tape Z and
+// the omega scale, and store the packed pre-activation adjoint.
+template <int P, int D, int ACT, int GE, typename M>
+__device__ inline void reverse_epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*adj)[GE]) {
+  using S = typename Prec<P>::store_t;
+  constexpr int C = Jet2Layout<D>::C;
+  const S* z = reinterpret_cast<const S*>(a.tape) + p * a.ldt + f;
+  M u[C][GE];
+#pragma unroll
+  for (int c = 0; c < C; ++c) ld_group<S, GE>(z + c * a.tstride, u[c], nv);
+  M b[C][GE];
+  const M om = (M)a.omega;
+#pragma unroll
+  for (int e = 0; e < GE; ++e) {
+    M uc[C], ac[C], bc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) { uc[c] = u[c][e]; ac[c] = adj[c][e]; }
+    act_jet_reverse<D, ACT, M>(uc, ac, bc);
+#pragma unroll
+    for (int c = 0; c < C; ++c) b[c][e] = om * bc[c];
+  }
+  S* o = reinterpret_cast<S*>(a.out0) + p * a.ldo + f;
+#pragma unroll
+  for (int c = 0; c < C; ++c) st_group<S, GE>(o + c * a.cstride, b[c], nv);
+}
 
 // Hidden layer reverse: A = A'_{l} W_l (all channels) then the transposed
 // activation chain of layer l-1 with its tape Z:  out = omega * f^T(Z, A).
@@ -314,25 +487,10 @@ template <int D, int ACT> struct PlanHiddenBwd {
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
-  static constexpr int ta[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-  static constexpr int tc[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-  static constexpr int tw[10] = {0};
-  static constexpr bool tneg[10] = {false};
-  template <int P> __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f,
-                                                         typename Prec<P>::math_t* v) {
-    using M = typename Prec<P>::math_t;
-    using S = typename Prec<P>::store_t;
-    const S* z = reinterpret_cast<const S*>(a.tape);
-    M u[C], b[C];
-    const int64_t toff = p * a.ldt + f;
-#pragma unroll
-    for (int c = 0; c < C; ++c) u[c] = tomath<M>(z[c * a.tstride + toff]);
-    act_jet_reverse<D, ACT, M>(u, v, b);
-    S* o = reinterpret_cast<S*>(a.out0);
-    const M om = (M)a.omega;
-    const int64_t off = p * a.ldo + f;
-#pragma unroll
-    for (int c = 0; c < C; ++c) o[c * a.cstride + off] = cvt<S>(om * b[c]);
+  IDENTITY_TERMS
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    reverse_epilogue<P, D, ACT, GE, M>(a, p, f, nv, v);
   }
 };
 
@@ -341,32 +499,22 @@ template <int D, int ACT> struct PlanHiddenBwd {
 // the mass fields, x[2]=omega rows of the stiffness fields.  Weights:
 // w[0] = Bm = 2w W^T dL/dM, w[1] = Bs = 2w W^T dL/dS  ([width][K]).
 //   adj gx = -(v . Bm), adj gy = u . Bm, adj hxx = adj hyy = -(omega . Bs)
-struct PlanOutBwd2 {
+template <int ACT> struct PlanOutBwd2 {
   static constexpr int CI = 3, NW = 2, NA = 3, NT = 3;
   static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
   static constexpr int ta[NT] = {1, 0, 2};
   static constexpr int tc[NT] = {0, 1, 2};
   static constexpr int tw[NT] = {0, 0, 1};
   static constexpr bool tneg[NT] = {false, true, true};
-};
-
-template <int ACT> struct PlanOutBwd2A : PlanOutBwd2 {
-  template <int P> __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f,
-                                                         typename Prec<P>::math_t* v) {
-    using M = typename Prec<P>::math_t;
-    using S = typename Prec<P>::store_t;
-    M adj[6] = {(M)0, v[0], v[1], v[2], (M)0, v[2]};
-    const S* z = reinterpret_cast<const S*>(a.tape);
-    M u[6], b[6];
-    const int64_t toff = p * a.ldt + f;
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    M adj[6][GE];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) u[c] = tomath<M>(z[c * a.tstride + toff]);
-    act_jet_reverse<2, ACT, M>(u, adj, b);
-    S* o = reinterpret_cast<S*>(a.out0);
-    const M om = (M)a.omega;
-    const int64_t off = p * a.ldo + f;
-#pragma unroll
-    for (int c = 0; c < 6; ++c) o[c * a.cstride + off] = cvt<S>(om * b[c]);
+    for (int e = 0; e < GE; ++e) {
+      adj[0][e] = (M)0; adj[1][e] = v[0][e]; adj[2][e] = v[1][e];
+      adj[3][e] = v[2][e]; adj[4][e] = (M)0; adj[5][e] = v[2][e];
+    }
+    reverse_epilogue<P, 2, ACT, GE, M>(a, p, f, nv, adj);
   }
 };
 
@@ -377,59 +525,32 @@ template <int ACT> struct PlanOutBwd2A : PlanOutBwd2 {
 template <int ACT> struct PlanOutBwd3 {
   static constexpr int CI = 12, NW = 6, NA = 9, NT = 24;
   static constexpr int FB = 2, PB = 1, WPN = 2, WPP = 2;
-  static constexpr int ta[NT] = {PlanOutFields3::tc[0], PlanOutFields3::tc[1], PlanOutFields3::tc[2],
-                                 PlanOutFields3::tc[3], PlanOutFields3::tc[4], PlanOutFields3::tc[5],
-                                 PlanOutFields3::tc[6], PlanOutFields3::tc[7], PlanOutFields3::tc[8],
-                                 PlanOutFields3::tc[9], PlanOutFields3::tc[10], PlanOutFields3::tc[11],
-                                 PlanOutFields3::tc[12], PlanOutFields3::tc[13], PlanOutFields3::tc[14],
-                                 PlanOutFields3::tc[15], PlanOutFields3::tc[16], PlanOutFields3::tc[17],
-                                 PlanOutFields3::tc[18], PlanOutFields3::tc[19], PlanOutFields3::tc[20],
-                                 PlanOutFields3::tc[21], PlanOutFields3::tc[22], PlanOutFields3::tc[23]};
-  static constexpr int tc[NT] = {PlanOutFields3::ta[0], PlanOutFields3::ta[1], PlanOutFields3::ta[2],
-                                 PlanOutFields3::ta[3], PlanOutFields3::ta[4], PlanOutFields3::ta[5],
-                                 PlanOutFields3::ta[6], PlanOutFields3::ta[7], PlanOutFields3::ta[8],
-                                 PlanOutFields3::ta[9], PlanOutFields3::ta[10], PlanOutFields3::ta[11],
-                                 PlanOutFields3::ta[12], PlanOutFields3::ta[13], PlanOutFields3::ta[14],
-                                 PlanOutFields3::ta[15], PlanOutFields3::ta[16], PlanOutFields3::ta[17],
-                                 PlanOutFields3::ta[18], PlanOutFields3::ta[19], PlanOutFields3::ta[20],
-                                 PlanOutFields3::ta[21], PlanOutFields3::ta[22], PlanOutFields3::ta[23]};
+  using F = PlanOutFields3;
+  static constexpr int ta[NT] = {F::tc[0], F::tc[1], F::tc[2], F::tc[3], F::tc[4], F::tc[5],
+                                 F::tc[6], F::tc[7], F::tc[8], F::tc[9], F::tc[10], F::tc[11],
+                                 F::tc[12], F::tc[13], F::tc[14], F::tc[15], F::tc[16], F::tc[17],
+                                 F::tc[18], F::tc[19], F::tc[20], F::tc[21], F::tc[22], F::tc[23]};
+  static constexpr int tc[NT] = {F::ta[0], F::ta[1], F::ta[2], F::ta[3], F::ta[4], F::ta[5],
+                                 F::ta[6], F::ta[7], F::ta[8], F::ta[9], F::ta[10], F::ta[11],
+                                 F::ta[12], F::ta[13], F::ta[14], F::ta[15], F::ta[16], F::ta[17],
+                                 F::ta[18], F::ta[19], F::ta[20], F::ta[21], F::ta[22], F::ta[23]};
   // mass fields (terms 0..5) use w[c], stiffness fields (terms 6..23) w[3+c]
-  static constexpr int tw[NT] = {PlanOutFields3::tw[0], PlanOutFields3::tw[1], PlanOutFields3::tw[2],
-                                 PlanOutFields3::tw[3], PlanOutFields3::tw[4], PlanOutFields3::tw[5],
-                                 3 + PlanOutFields3::tw[6], 3 + PlanOutFields3::tw[7], 3 + PlanOutFields3::tw[8],
-                                 3 + PlanOutFields3::tw[9], 3 + PlanOutFields3::tw[10], 3 + PlanOutFields3::tw[11],
-                                 3 + PlanOutFields3::tw[12], 3 + PlanOutFields3::tw[13], 3 + PlanOutFields3::tw[14],
-                                 3 + PlanOutFields3::tw[15], 3 + PlanOutFields3::tw[16], 3 + PlanOutFields3::tw[17],
-                                 3 + PlanOutFields3::tw[18], 3 + PlanOutFields3::tw[19], 3 + PlanOutFields3::tw[20],
-                                 3 + PlanOutFields3::tw[21], 3 + PlanOutFields3::tw[22], 3 + PlanOutFields3::tw[23]};
+  static constexpr int tw[NT] = {F::tw[0], F::tw[1], F::tw[2], F::tw[3], F::tw[4], F::tw[5],
+                                 3 + F::tw[6], 3 + F::tw[7], 3 + F::tw[8], 3 + F::tw[9], 3 + F::tw[10],
+                                 3 + F::tw[11], 3 + F::tw[12], 3 + F::tw[13], 3 + F::tw[14], 3 + F::tw[15],
+                                 3 + F::tw[16], 3 + F::tw[17], 3 + F::tw[18], 3 + F::tw[19], 3 + F::tw[20],
+                                 3 + F::tw[21], 3 + F::tw[22], 3 + F::tw[23]};
   static constexpr bool tneg[NT] = {false, true, false, true, false, true, false, true, false, true, false, true,
                                     false, true, false, true, false, true, false, true, false, true, false, true};
-  template <int P> __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f,
-                                                         typename Prec<P>::math_t* v) {
-    using M = typename Prec<P>::math_t;
-    using S = typename Prec<P>::store_t;
-    M adj[10];
-    adj[0] = (M)0;
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    M adj[10][GE];
 #pragma unroll
-    for (int c = 0; c < 9; ++c) adj[1 + c] = v[c];
-    const S* z = reinterpret_cast<const S*>(a.tape);
-    M u[10], b[10];
-    const int64_t toff = p * a.ldt + f;
+    for (int e = 0; e < GE; ++e) {
+      adj[0][e] = (M)0;
 #pragma unroll
-    for (int c = 0; c < 10; ++c) u[c] = tomath<M>(z[c * a.tstride + toff]);
-    act_jet_reverse<3, ACT, M>(u, adj, b);
-    S* o = reinterpret_cast<S*>(a.out0);
-    const M om = (M)a.omega;
-    const int64_t off = p * a.ldo + f;
-#pragma unroll
-    for (int c = 0; c < 10; ++c) o[c * a.cstride + off] = cvt<S>(om * b[c]);
+      for (int c = 0; c < 9; ++c) adj[1 + c][e] = v[c][e];
+    }
+    reverse_epilogue<P, 3, ACT, GE, M>(a, p, f, nv, adj);
   }
 };
-
-template <int P, class Plan>
-constexpr size_t rowgemm_lds_bytes() {
-  using PR = Prec<P>;
-  constexpr int LDK = PR::RK + PR::PAD;
-  constexpr int TN = Plan::WPN * Plan::FB * 16, TP = Plan::WPP * Plan::PB * 16;
-  return sizeof(typename PR::store_t) * (size_t)LDK * (Plan::NW * TN + Plan::CI * TP);
-}

@@ -45,24 +45,52 @@ __global__ __launch_bounds__(256) void feature_kernel(const TX* __restrict__ x, 
   for (int c = 0; c < L::C; ++c) out[c * cs + p * ldf + f] = cvt<S>(ch[c]);
 }
 
-// Column sums in FP64: partial[s][col] over a row range per split.
+// Column sums in FP64: partial[split][col] over the split's row range.
+// Each thread sums 16-byte column vectors down a strided set of rows; row
+// lanes are then combined through LDS.
 template <typename S>
 __global__ __launch_bounds__(256) void colsum_kernel(const S* __restrict__ x, int64_t ld, int64_t rows,
                                                      int64_t cols, int64_t chunk, double* __restrict__ partial) {
-  __shared__ double red[4][64];
-  const int64_t col = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
-  const int rl = threadIdx.x >> 6;
-  const int64_t rbeg = (int64_t)blockIdx.y * chunk;
-  int64_t rend = rbeg + chunk;
-  if (rend > rows) rend = rows;
-  double s = 0.0;
-  if (col < cols)
-    for (int64_t r = rbeg + rl; r < rend; r += 4) s += todbl_any(x[r * ld + col]);
-  red[rl][threadIdx.x & 63] = s;
-  __syncthreads();
-  if (rl == 0 && col < cols)
-    partial[(int64_t)blockIdx.y * cols + col] = red[0][threadIdx.x] + red[1][threadIdx.x] +
-                                                red[2][threadIdx.x] + red[3][threadIdx.x];
+  constexpr int VE = 16 / sizeof(S);
+  typedef S vs __attribute__((ext_vector_type(VE)));
+  __shared__ double red[256 * VE];
+  const int64_t rbeg = (int64_t)blockIdx.x * chunk;
+  const int64_t rend = rbeg + chunk < rows ? rbeg + chunk : rows;
+  const int64_t ngroups = (cols + VE - 1) / VE;
+  const int CG = ngroups < 256 ? (int)ngroups : 256;
+  const int RL = 256 / CG;
+  const int cgl = threadIdx.x % CG, rl = threadIdx.x / CG;
+  for (int64_t g0 = 0; g0 < ngroups; g0 += CG) {
+    double acc[VE];
+#pragma unroll
+    for (int e = 0; e < VE; ++e) acc[e] = 0.0;
+    const int64_t col = (g0 + cgl) * VE;
+    if (rl < RL && g0 + cgl < ngroups) {
+      for (int64_t r = rbeg + rl; r < rend; r += RL) {
+        const S* p = x + r * ld + col;
+        if (col + VE <= cols) {
+          const vs v = *reinterpret_cast<const vs*>(p);
+#pragma unroll
+          for (int e = 0; e < VE; ++e) acc[e] += todbl_any(v[e]);
+        } else {
+#pragma unroll
+          for (int e = 0; e < VE; ++e)
+            if (col + e < cols) acc[e] += todbl_any(p[e]);
+        }
+      }
+    }
+    if (rl < RL)
+#pragma unroll
+      for (int e = 0; e < VE; ++e) red[(rl * CG + cgl) * VE + e] = acc[e];
+    __syncthreads();
+    for (int t = threadIdx.x; t < CG * VE; t += 256) {
+      double sacc = 0.0;
+      for (int q = 0; q < RL; ++q) sacc += red[q * CG * VE + t];
+      const int64_t c = g0 * VE + t;
+      if (c < cols) partial[(int64_t)blockIdx.x * cols + c] = sacc;
+    }
+    __syncthreads();
+  }
 }
 
 // out[c] = sum_s partial[s][c] in fixed order

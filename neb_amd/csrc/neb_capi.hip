@@ -60,9 +60,9 @@ int launch_rowgemm(const RowArgs& a, hipStream_t st, const char* what) {
   auto kern = rowgemm_kernel<P, TIN, Plan>;
   if (int rc = set_lds(kern, lds)) return rc;
   const int64_t gx = (a.npts + TP - 1) / TP, gy = (a.nout + TN - 1) / TN;
-  if (gx > 0x7fffffff || gy > 65535) return fail(NEB_ERR_SHAPE, "%s: grid too large", what);
+  if (gx * gy > 0x7fffffff) return fail(NEB_ERR_SHAPE, "%s: grid too large", what);
   if (gx == 0 || gy == 0) return NEB_OK;
-  hipLaunchKernelGGL(kern, dim3((unsigned)gx, (unsigned)gy), dim3(256), lds, st, a);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(gx * gy)), dim3(256), lds, st, a);
   return hip_status(what);
 }
 
@@ -165,7 +165,7 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
     a.x[2] = Sf; a.ldx[2] = ldf;
     a.w[0] = bm; a.ldw[0] = ldb;
     a.w[1] = bs; a.ldw[1] = ldb;
-    return launch_rowgemm<P, F, PlanOutBwd2A<ACT>>(a, st, "neb_jet_output_backward");
+    return launch_rowgemm<P, F, PlanOutBwd2<ACT>>(a, st, "neb_jet_output_backward");
   }
   for (int c = 0; c < 3; ++c) {
     a.x[c] = layout == 0 ? Mf + c * ldf : Mf + c * n * ldf;
@@ -211,9 +211,11 @@ int cross_tiles(int64_t m, int64_t n, int symmetric, int* tm, int* tn) {
 }
 
 int cross_splits(int ntiles, int64_t rows) {
-  // aim for >= 4 workgroups per CU over 256 CUs, each split >= 1024 rows
-  int s = (1024 + ntiles - 1) / ntiles;
-  int64_t maxs = (rows + 1023) / 1024;
+  // about two workgroups per CU (256 CUs), a multiple of 8 so every XCD gets
+  // whole splits, and at least 1024 rows per split
+  int s = (512 + ntiles - 1) / ntiles;
+  s = (s + 7) / 8 * 8;
+  const int64_t maxs = (rows + 1023) / 1024;
   if (s > maxs) s = (int)maxs;
   if (s < 1) s = 1;
   if (s > 4096) s = 4096;
@@ -225,13 +227,12 @@ int launch_cross(CrossArgs& a, double scale, double beta, double* G, int64_t ldg
   const size_t lds = cross_lds_bytes<MODE>();
   auto kern = cross_kernel<MODE, TX, TY>;
   if (int rc = set_lds(kern, lds)) return rc;
-  hipLaunchKernelGGL(kern, dim3(a.ntiles, a.splits), dim3(256), lds, st, a);
+  hipLaunchKernelGGL(kern, dim3(a.ntiles * a.splits), dim3(256), lds, st, a);
   if (int rc = hip_status("neb_cross")) return rc;
-  const int64_t tot = a.m * a.n;
-  if (tot > 0) {
-    hipLaunchKernelGGL(cross_reduce_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a.partial,
-                       a.splits, a.ntiles, a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
-  }
+  const int64_t nbi = (a.m + 31) / 32, nbj = (a.n + 31) / 32;
+  const int64_t nblk = a.symmetric ? nbi * (nbi + 1) / 2 : nbi * nbj;
+  hipLaunchKernelGGL(cross_reduce_kernel, dim3((unsigned)nblk), dim3(256), 0, st, a.partial, a.splits, a.ntiles,
+                     a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
   return hip_status("neb_cross reduce");
 }
 
@@ -256,8 +257,8 @@ int cross_dispatch_types(int xd, int yd, CrossArgs& a, double scale, double beta
 template <typename T> int colsum_launch(const void* x, int64_t rows, int64_t cols, int64_t ld, double beta,
                                         double* out, double* part, int splits, hipStream_t st) {
   const int64_t chunk = (rows + splits - 1) / splits;
-  hipLaunchKernelGGL(colsum_kernel<T>, dim3((unsigned)((cols + 63) / 64), splits), dim3(256), 0, st,
-                     reinterpret_cast<const T*>(x), ld, rows, cols, chunk, part);
+  hipLaunchKernelGGL(colsum_kernel<T>, dim3(splits), dim3(256), 0, st, reinterpret_cast<const T*>(x), ld, rows,
+                     cols, chunk, part);
   if (int rc = hip_status("neb_colsum")) return rc;
   hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, st, part, splits, cols,
                      out, beta);
@@ -265,8 +266,8 @@ template <typename T> int colsum_launch(const void* x, int64_t rows, int64_t col
 }
 
 int colsum_splits(int64_t rows) {
-  int64_t s = (rows + 2047) / 2048;
-  if (s > 256) s = 256;
+  int64_t s = (rows + 255) / 256;
+  if (s > 1024) s = 1024;
   if (s < 1) s = 1;
   return (int)s;
 }

@@ -20,6 +20,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 enum { PREC_F32 = 0, PREC_BF16 = 1, PREC_F64 = 2 };
 enum { ACT_TANH = 0, ACT_SIN = 1 };
@@ -40,6 +41,8 @@ template <> struct Prec<PREC_BF16> {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
   __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+  __device__ static inline int rowg(int rho) { return rho >> 2; }  // inverse of drow
+  __device__ static inline int rowr(int rho) { return rho & 3; }
 };
 
 template <> struct Prec<PREC_F32> {
@@ -56,6 +59,8 @@ template <> struct Prec<PREC_F32> {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   __device__ static inline int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+  __device__ static inline int rowg(int rho) { return rho >> 2; }  // inverse of drow
+  __device__ static inline int rowr(int rho) { return rho & 3; }
 };
 
 template <> struct Prec<PREC_F64> {
@@ -72,6 +77,8 @@ template <> struct Prec<PREC_F64> {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   __device__ static inline int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+  __device__ static inline int rowg(int rho) { return rho & 3; }
+  __device__ static inline int rowr(int rho) { return rho >> 2; }
 };
 
 // ---------------------------------------------------------------------------

@@ -15,6 +15,7 @@ MB = experiments/metaball/_backend/flow/cases/metaball):
 """
 from __future__ import annotations
 
+import math
 import time
 from contextlib import contextmanager
 
@@ -55,6 +56,68 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
     cross = inv @ Ss @ inv
     dM = -_sym(cross) + 2.0 * lam * D
     return loss, dM, inv
+
+
+class SpectralSolver:
+    """Stateful FP64 evaluator of the spectral objective for a training loop.
+
+    Step 1 (or whenever the check below fails) factors M_r = M + ridge I by
+    FP64 Cholesky, as spectral_objective does.  Later steps start from the
+    previous step's inverse X and refine it with Newton-Schulz iterations
+    X <- X (2I - M_r X) (two FP64 GEMMs each, quadratic convergence), which
+    is valid while the residual ||I - M_r X||_F < 1; the iteration stops at a
+    relative residual <= tol, else the Cholesky path runs.  Either way the
+    returned inverse satisfies ||I - M_r X||_F <= tol, and a condition
+    estimate ||M_r||_F ||X||_F above cond_limit raises FloatingPointError
+    (the gate of MB/velocity_trace_jet2.py:109-111)."""
+
+    def __init__(self, lam: float = 1e-2, ridge: float = 0.0, cond_limit: float = 1e12, tol: float = 1e-10,
+                 max_iter: int = 6):
+        self.lam, self.ridge, self.cond_limit, self.tol, self.max_iter = lam, ridge, cond_limit, tol, max_iter
+        self.X = None
+        self.last_path = None
+
+    def _residual(self, Mr, X, eye):
+        return float(torch.linalg.norm(eye - Mr @ X))
+
+    def __call__(self, M: torch.Tensor, S: torch.Tensor):
+        K = M.shape[0]
+        eye = torch.eye(K, dtype=torch.float64, device=M.device)
+        Mr = _sym(M) + self.ridge * eye
+        X = None
+        if self.X is not None and self.X.shape == Mr.shape:
+            X = self.X
+            ok = False
+            for _ in range(self.max_iter):
+                R = eye - Mr @ X
+                r = float(torch.linalg.norm(R))
+                if not r < 0.5:          # also catches NaN
+                    break
+                if r <= self.tol:
+                    ok = True
+                    break
+                X = X + X @ R            # = X (2I - Mr X)
+            if ok:
+                self.last_path = "newton-schulz"
+            else:
+                X = None
+        if X is None:
+            L, info = torch.linalg.cholesky_ex(Mr)
+            if int(info.item()) != 0:
+                raise FloatingPointError("mass Cholesky failed; no update")
+            X = torch.cholesky_inverse(L)
+            self.last_path = "cholesky"
+        X = _sym(X)
+        cond = float(torch.linalg.norm(Mr) * torch.linalg.norm(X))
+        if not math.isfinite(cond) or cond > self.cond_limit:
+            raise FloatingPointError("mass condition estimate exceeds limit; no update")
+        self.X = X
+        Ss = _sym(S)
+        D = M - eye
+        loss = torch.sum(X * Ss) + self.lam * torch.sum(D * D)
+        cross = X @ Ss @ X
+        dM = -_sym(cross) + 2.0 * self.lam * D
+        return loss, dM, X
 
 
 def generalized_eigh(S: torch.Tensor, M: torch.Tensor, rank_tol: float = 1e-12):
@@ -108,6 +171,113 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
         with timed(tt, "analytic_backward"):
             out["grads"] = nw.backward_fields(net, tape, fields, dM, dS, weight)
     return out
+
+
+def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
+                 gram_mode: str = "bf16x3", chunk: int = 131072, solver=None, training: bool = True,
+                 timings: dict | None = None):
+    """Spectral step over point chunks, memory-bounded for large point sets
+    (the structure of MB/velocity/core.py:79-105: fields -> grams ->
+    objective -> recompute + reverse).  Pass 1 evaluates each chunk's jets
+    and fields and accumulates the FP64 Grams; pass 2 recomputes each
+    chunk's jet tape (per-layer recomputation: no tape outlives its chunk)
+    and accumulates the parameter gradient.  Returns dict(loss, M, S, grads)."""
+    p = nw.PRECISIONS[prec]
+    tt = {} if timings is None else timings
+    K = net.cfg.modes
+    n = points.shape[0]
+    solver = solver if solver is not None else SpectralSolver()
+    M = torch.zeros((K, K), dtype=torch.float64, device=net.device)
+    S = torch.zeros_like(M)
+    with timed(tt, "pass1_forward_and_grams"):
+        for c0 in range(0, n, chunk):
+            tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
+            f = nw.output_fields(net, tape)
+            del tape
+            nw.gram(f.mass, K, weight, gram_mode, out=M, beta=1.0)
+            nw.gram(f.stiff, K, weight, gram_mode, out=S, beta=1.0)
+            del f
+    with timed(tt, "spectral_objective"):
+        loss, dM, dS = solver(M, S)
+    out = dict(loss=loss, M=M, S=S, timings=tt)
+    if training:
+        grads = torch.zeros_like(net.params)
+        with timed(tt, "pass2_recompute_and_reverse"):
+            for c0 in range(0, n, chunk):
+                tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
+                f = nw.output_fields(net, tape)
+                grads += nw.backward_fields(net, tape, f, dM, dS, weight)
+                del tape, f
+        out["grads"] = grads
+    return out
+
+
+def basis_stream(net: nw.EigenfluidNet, points: torch.Tensor, prec: str = "bf16", chunk: int = 131072):
+    """Yield (first point, Fields) chunk by chunk: the full-grid basis
+    evaluation of experiments/metaball/evaluate.py:130-190 without the disk
+    round trip."""
+    p = nw.PRECISIONS[prec]
+    for c0 in range(0, points.shape[0], chunk):
+        tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
+        f = nw.output_fields(net, tape)
+        del tape
+        yield c0, f
+
+
+def fullmass(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
+             gram_mode: str = "bf16x3", chunk: int = 131072, rank_tol: float = 1e-12):
+    """Full-point FP64 mass Gram with the rank gate of
+    experiments/metaball/run.py:99-116 (ValueError when rank deficient)."""
+    K = net.cfg.modes
+    M = torch.zeros((K, K), dtype=torch.float64, device=net.device)
+    for _, f in basis_stream(net, points, prec, chunk):
+        nw.gram(f.mass, K, weight, gram_mode, out=M, beta=1.0)
+    ev = torch.linalg.eigvalsh(_sym(M))
+    if float(ev[0]) <= rank_tol * float(ev[-1]):
+        raise ValueError("Fullmass basis is rank deficient")
+    return M, ev
+
+
+def project_streamed(net: nw.EigenfluidNet, points: torch.Tensor, targets: torch.Tensor, mass: torch.Tensor,
+                     weight: float, prec: str = "bf16", chunk: int = 131072, ridge: float = 0.0,
+                     normal_tol: float = 1e-8, identity_tol: float = 1e-8):
+    """Two-pass full-mass projection of targets onto the network basis
+    (PP:141-177): pass 1 streams the basis chunk by chunk and accumulates
+    loads = w B^T T and the target norms (FP64 cross kernel), the FP64 solve
+    (with optional ridge) is gated on the normal residual, pass 2 re-streams
+    the basis for the direct residual sum w |B c - T|^2, which must agree with
+    the Gram identity.  targets: [n*comp, m] point-major rows, FP64."""
+    K = net.cfg.modes
+    m = targets.shape[1]
+    comp = 2 if net.cfg.dim == 2 else 3
+    loads = torch.zeros((K, m), dtype=torch.float64, device=net.device)
+    norms2 = torch.zeros((m, m), dtype=torch.float64, device=net.device)
+    for c0, f in basis_stream(net, points, prec, chunk):
+        rows = f.mass.shape[0]
+        B = f.mass[:, :K].to(torch.float64).contiguous()
+        T = targets[comp * c0: comp * c0 + rows]
+        nw.cross([(B, T, rows, 1.0)], K, m, nw.GRAM_MODES["f64"], scale=weight, out=loads, beta=1.0)
+        nw.cross([(T, T, rows, 1.0)], m, m, nw.GRAM_MODES["f64"], scale=weight, out=norms2, beta=1.0)
+        del B, f
+    norms = torch.diagonal(norms2).clone()
+    Mr = _sym(mass) + ridge * torch.eye(K, dtype=torch.float64, device=mass.device)
+    coeff = torch.linalg.solve(Mr, loads)
+    normal = torch.linalg.norm(Mr @ coeff - loads, dim=0) / torch.linalg.norm(loads, dim=0)
+    if not bool(torch.isfinite(coeff).all()) or not bool((normal < normal_tol).all()):
+        raise ValueError("fullmass normal-residual gate failed")
+    gram_square = norms - 2 * torch.sum(coeff * loads, 0) + torch.sum(coeff * (mass @ coeff), 0)
+    direct = torch.zeros(m, dtype=torch.float64, device=net.device)
+    for c0, f in basis_stream(net, points, prec, chunk):
+        rows = f.mass.shape[0]
+        diff = f.mass[:, :K].to(torch.float64) @ coeff - targets[comp * c0: comp * c0 + rows]
+        direct += weight * torch.sum(diff * diff, 0)
+        del f, diff
+    errors = torch.sqrt(direct / norms)
+    identity = torch.sqrt(torch.clamp(gram_square, min=0) / norms)
+    if not bool((torch.abs(errors - identity) < identity_tol).all()):
+        raise ValueError("Gram/direct identity tolerance failed")
+    return dict(loads=loads, target_norms=norms, coefficients=coeff, normal_residuals=normal,
+                direct_relative_L2=errors, Gram_identity_errors=identity)
 
 
 class JetGram(torch.autograd.Function):

@@ -335,3 +335,69 @@ def test_checkpoint_roundtrip(tmp_path):
     assert net2.epoch == 3 and torch.equal(net2.params, net.params)
     z = np.load(p)
     assert {"W_0", "b_2", "mW_1", "vb_0", "epoch"} <= set(z.files)
+
+
+def test_chunked_step_equals_single_pass():
+    """Two-pass chunked step (Gram accumulation, then recompute + reverse)
+    equals the single-pass step; FP64, 3D."""
+    F, K = 2, 8
+    cfg, net, _ = make(3, K, [32, 32], F, master=torch.float64)
+    x = torch.tensor(points(3, 1000, seed=8))
+    w = 1.0 / 1000
+    ref = nb.compute(net, x, w, prec="f64", gram_mode="f64", gate="cholesky")
+    res = nb.chunked_step(net, x, w, prec="f64", gram_mode="f64", chunk=256)
+    r = rel(res["M"].cpu(), ref["M"].cpu()); assert r < 1e-13, r
+    r = rel(res["S"].cpu(), ref["S"].cpu()); assert r < 1e-13, r
+    r = rel(res["grads"].cpu(), ref["grads"].cpu()); assert r < 1e-11, r
+
+
+def test_project_streamed_against_oracle():
+    """Streamed two-pass full-mass projection of 20 divergence-free targets
+    onto the 3D network basis vs the oracle's dense projection (PP:141-177)."""
+    from neb_amd import synthetic as syn
+    F, K = 2, 12
+    cfg, net, onet = make(3, K, [32, 32], F, master=torch.float64)
+    pts, _, _ = syn.blob_domain(2000, seed=3, batch=1 << 14)
+    T = syn.curl_targets(pts, 20, seed=4).reshape(-1, 20)       # [3n, 20] point-major
+    w = 1.0 / len(pts)
+    x = torch.tensor(pts)
+    M, ev = nb.fullmass(net, x, w, prec="f64", gram_mode="f64", chunk=512)
+    res = nb.project_streamed(net, x, torch.tensor(T, device=DEV), M, w, prec="f64", chunk=512)
+    out, _ = o.forward(onet, o.feature_jets(pts, F, cfg.lo, cfg.hi))
+    u, _ = o.lift3d(out)
+    B = o.field_rows(u)
+    Mo = o.gram(B, w)
+    r = rel(M.cpu(), Mo); assert r < 1e-12, r
+    ref = o.project(Mo, B, T, w)
+    r = rel(res["coefficients"].cpu(), ref["coefficients"]); assert r < 1e-9, r
+    r = rel(res["direct_relative_L2"].cpu(), ref["direct_relative_L2"]); assert r < 1e-9, r
+    assert float(res["normal_residuals"].max()) < 1e-8
+
+
+def test_fullmass_rank_gate_rejects_rank_deficient_basis():
+    """K modes from a width-4 trunk in 2D: rank(M) <= 8 < K, so the full-mass
+    rank gate (experiments/metaball/run.py:108-110) raises."""
+    cfg, net, _ = make(2, 16, [4], 2, master=torch.float64)
+    x = torch.tensor(points(2, 500))
+    with pytest.raises(ValueError):
+        nb.fullmass(net, x, 1.0 / 500, prec="f64", gram_mode="f64")
+
+
+def test_spectral_solver_paths_agree():
+    """Newton-Schulz refinement of the previous inverse and the Cholesky path
+    give the same objective and adjoints."""
+    rng = np.random.default_rng(0)
+    K = 64
+    A = rng.normal(size=(400, K)); M0 = A.T @ A / 400
+    B = rng.normal(size=(300, K)); S = B.T @ B / 300
+    Mt, St = torch.tensor(M0, device=DEV), torch.tensor(S, device=DEV)
+    solver = nb.SpectralSolver(1e-2)
+    solver(Mt, St)
+    assert solver.last_path == "cholesky"
+    M1 = Mt + 1e-4 * torch.tensor(np.eye(K), device=DEV)
+    L1, dM1, dS1 = solver(M1, St)
+    assert solver.last_path == "newton-schulz"
+    L2, dM2, dS2 = nb.spectral_objective(M1, St, 1e-2, gate="cholesky")
+    assert abs(float(L1) - float(L2)) < 1e-10 * abs(float(L2))
+    r = rel(dM1.cpu(), dM2.cpu()); assert r < 1e-9, r
+    r = rel(dS1.cpu(), dS2.cpu()); assert r < 1e-9, r
