@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host oracle baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 comparison step")
+    ap.add_argument("--kernel", default=None,
+                    choices=["hidden_fwd", "hidden_bwd", "out_fields", "out_bwd", "gram", "cross_dw", "copy"],
+                    help="run one kernel of the config-B step --steps times (for counter profiling)")
     ap.add_argument("--config", default="B", choices=["B", "E"],
                     help="B: 2D 256^2 grid K=1024 (default); E: 3D blob domain, 1,599,754 points, K=1024")
     return ap.parse_args()
@@ -132,6 +135,73 @@ def cpu_baseline(seconds_target=15.0):
     return dict(value=n_done * K_MODES / t_total, unit="point*modes/s", cores=min(threads, cores), kind="port",
                 sample=f"{n_done} of the 65,536 grid points in {n_s}-point batches, full jet fwd + Grams + "
                        f"reverse at K=1024 (FP64 NumPy oracle; K x K objective excluded), {t_total:.1f} s")
+
+
+def bench_kernel(args):
+    """Launch one kernel of the config-B step `--steps` times on the real
+    shapes (65,536 points, 4x256, K=1024), timed with HIP events; for
+    rocprofv3 counter collection.  `copy` is a plain device copy of the mass
+    field matrix (512 MiB read + 512 MiB written), the calibration kernel for
+    FETCH_SIZE / WRITE_SIZE."""
+    import numpy as np
+    import torch
+
+    import neb_amd as nb
+    from neb_amd import _hip as H
+    from neb_amd import network as nw
+    from neb_amd import synthetic as syn
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    x = torch.tensor(syn.periodic_grid(N_GRID), dtype=torch.float32, device=dev)
+    N, K = x.shape[0], K_MODES
+    w = syn.grid_weight(N_GRID)
+    cfg = nb.NetConfig(dim=2, frequencies=FREQS, widths=WIDTHS, modes=K)
+    net = nb.EigenfluidNet(cfg, seed=0, device=dev)
+    p = nw.PRECISIONS["bf16"]
+    tape = nw.trunk_forward(net, x, p)
+    fields = nw.output_fields(net, tape)
+    stg = net.staged(p)
+    s = torch.cuda.current_stream().cuda_stream
+    C = 6
+    Y, Z = tape.inputs[2], tape.pre[2]
+    out = torch.empty_like(Y)
+    dM = torch.eye(K, dtype=torch.float64, device=dev) * 1e-3
+    Bm = torch.zeros((256, K), dtype=torch.bfloat16, device=dev)
+    A = torch.empty_like(Y)
+    copy_dst = torch.empty_like(fields.mass)
+
+    def run():
+        k = args.kernel
+        if k == "hidden_fwd":
+            H.call("neb_jet_layer_forward", p, 2, H.TANH, N, 256, 256, Y.data_ptr(), 256, stg["W"][2].data_ptr(),
+                   256, stg["b"][2].data_ptr(), 1.0, out.data_ptr(), A.data_ptr(), 256, s)
+        elif k == "hidden_bwd":
+            H.call("neb_jet_layer_backward", p, 2, H.TANH, N, 256, 256, Y.data_ptr(), 256, stg["WT"][2].data_ptr(),
+                   256, Z.data_ptr(), 256, 1.0, out.data_ptr(), 256, s)
+        elif k == "out_fields":
+            nw.output_fields(net, tape)
+        elif k == "out_bwd":
+            H.call("neb_jet_output_backward", p, 2, H.TANH, N, 256, K, fields.mass.data_ptr(),
+                   fields.stiff.data_ptr(), K, 0, Bm.data_ptr(), Bm.data_ptr(), K, Z.data_ptr(), 256, 1.0,
+                   out.data_ptr(), 256, s)
+        elif k == "gram":
+            nw.gram(fields.mass, K, w, "bf16x3")
+        elif k == "cross_dw":
+            nw.cross([(Y.view(C * N, 256), Z.view(C * N, 256), C * N, 1.0)], 256, 256, nw._CROSS[p])
+        elif k == "copy":
+            copy_dst.copy_(fields.mass)
+
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel": args.kernel, "launches": args.steps, "ms_per_launch": e0.elapsed_time(e1) / args.steps,
+                      "copy_bytes": 2 * fields.mass.numel() * 4 if args.kernel == "copy" else None}), flush=True)
 
 
 N3D = 1_599_754
@@ -232,6 +302,9 @@ def main():
         sys.exit(2)
     if args.config == "E":
         bench_3d(args)
+        return
+    if args.kernel:
+        bench_kernel(args)
         return
     import numpy as np
     import torch

@@ -155,6 +155,18 @@ int neb_sumsq(int dtype, const void* x, int64_t n, double* out, void* workspace,
 int neb_adam(int dtype, void* p, const void* g, void* m, void* v, int64_t n, double lr, double scale, double b1,
              double b2, double eps, int64_t t, void* stream);
 
+/* FP64 inverse of a symmetric positive definite K x K matrix (blocked
+ * Gauss-Jordan sweeps, 64-column pivot blocks).  X must not alias A.  *info
+ * (device int) is set to 0, or to the 1-based index of the first
+ * nonpositive pivot, i.e. the matrix is not positive definite (the
+ * FloatingPointError gate of velocity_trace_jet2.py:109-111); X is then
+ * meaningless.  Replaces cholesky + solve / solve_triangular of
+ * velocity_trace_jet2.py:112 and velocity/core.py:52-56.  Workspace from
+ * neb_spd_inverse_workspace_size. */
+size_t neb_spd_inverse_workspace_size(int64_t K);
+int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
+                    size_t ws_bytes, void* stream);
+
 /* Weight staging: dst[r][c] = scale * (transpose ? src[c][r] : src[r][c])
  * converted from src_dtype to dst_dtype, zero outside the source extent
  * (dst is drows x dcols).  Replaces the per-call `.astype(compute_dtype)`

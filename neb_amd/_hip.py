@@ -44,6 +44,8 @@ SIGNATURES = {
     "neb_sumsq": (_int, [_int, _vp, _i64, _vp, _vp, _sz, _vp]),
     "neb_adam": (_int, [_int, _vp, _vp, _vp, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _i64, _vp]),
     "neb_stage": (_int, [_int, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _int, _dbl, _vp]),
+    "neb_spd_inverse_workspace_size": (_sz, [_i64]),
+    "neb_spd_inverse": (_int, [_i64, _vp, _i64, _vp, _i64, _vp, _vp, _sz, _vp]),
 }
 
 _lib = None

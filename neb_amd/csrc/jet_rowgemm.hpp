@@ -180,7 +180,7 @@ constexpr size_t rowgemm_lds_bytes() {
 }
 
 template <int P, typename TIN, class Plan>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void rowgemm_kernel(RowArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F64 ? 1 : 2))) void rowgemm_kernel(RowArgs a) {
   using PR = Prec<P>;
   using S = typename PR::store_t;
   using M = typename PR::math_t;
@@ -190,12 +190,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   constexpr int CI = Plan::CI, NW = Plan::NW, NA = Plan::NA, NT = Plan::NT;
   constexpr int FB = Plan::FB, PB = Plan::PB, WPN = Plan::WPN, WPP = Plan::WPP;
   static_assert(WPN * WPP == 4, "four waves per workgroup");
-  static_assert(FB % 2 == 0, "epilogue groups pair feature blocks");
+  static_assert(FB % 2 == 0, "8-feature epilogue groups pair feature blocks");
   constexpr int TN = WPN * FB * 16, TP = WPP * PB * 16;
   constexpr int WT = TN * LDK, XT = TP * LDK;
   constexpr int STAGE = NW * WT + CI * XT;
   constexpr bool DB = rowgemm_double_buffered<P, Plan>();
-  constexpr int GE = 8;  // epilogue group: 8 consecutive features
 
   extern __shared__ __align__(16) unsigned char smem[];
   S* buf0 = reinterpret_cast<S*>(smem);
@@ -230,26 +229,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
 #pragma unroll
       for (int j = 0; j < PB; ++j) acc[s][i][j] = acc_t{0, 0, 0, 0};
 
-  TileLoad<NW, TN, RK, LDK, S, S> lw;
-  TileLoad<CI, TP, RK, LDK, TIN, S> lx;
+  // Two register stages of prefetch: while stage kt computes from LDS, the
+  // loads of stages kt+1 and kt+2 are both in flight.
+  TileLoad<NW, TN, RK, LDK, S, S> lw[2];
+  TileLoad<CI, TP, RK, LDK, TIN, S> lx[2];
   const int fr = lane & 15, kg = (lane >> 4) * KE;
-  // A-operand row fr of block i holds feature 4*FB*g(fr) + 4*i + r(fr), so the
-  // result lane (fr', g) owns 4*FB consecutive features of one point.
-  const int wrow = wn * FB * 16 + 4 * FB * PR::rowg(fr) + PR::rowr(fr);
+  // Feature permutation: result element (block i, row drow(lane, r)) of lane
+  // group g = lane/16 is feature phi(g, i, r) of the wave's 16*FB features,
+  //   GE = 8 (2-byte outputs): phi = 32*(i/2) + 8*g + 4*(i%2) + r
+  //   GE = 4 (wider outputs):  phi = 16*i + 4*g + r
+  // so that in every epilogue store instruction the four lane groups of a
+  // point write adjacent 16-byte chunks (whole 32-byte sectors).
+  constexpr int GE = Plan::template out_bytes<P>() == 2 ? 8 : 4;
+  const int rg = PR::rowg(fr), rr = PR::rowr(fr);
   const int nk = (int)((a.k + RK - 1) / RK);
 
-  lw.fetch(a.w, a.ldw, f0, a.nout, 0, a.k);
-  lx.fetch(a.x, a.ldx, p0, a.npts, 0, a.k);
-  lw.commit(buf0);
-  lx.commit(buf0 + NW * WT);
+  // the second in-flight stage only where its registers fit beside the
+  // accumulators (raw operands no wider than the LDS type)
+  constexpr int NVEC = TileLoad<NW, TN, RK, LDK, S, S>::NVT + TileLoad<CI, TP, RK, LDK, TIN, S>::NVT;
+  constexpr bool DEEP = sizeof(S) == 2 && sizeof(TIN) == 2 && NA * FB * PB <= 24 && NVEC <= 6;
+  lw[0].fetch(a.w, a.ldw, f0, a.nout, 0, a.k);
+  lx[0].fetch(a.x, a.ldx, p0, a.npts, 0, a.k);
+  if (DEEP && nk > 1) {
+    lw[1].fetch(a.w, a.ldw, f0, a.nout, RK, a.k);
+    lx[1].fetch(a.x, a.ldx, p0, a.npts, RK, a.k);
+  }
+  lw[0].commit(buf0);
+  lx[0].commit(buf0 + NW * WT);
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
+    const int cur_set = DEEP ? (kt & 1) : 1;
     S* cur = (DB && (kt & 1)) ? buf1 : buf0;
-    const bool more = kt + 1 < nk;
-    if (more) {
+    if (!DEEP && kt + 1 < nk) {  // single in-flight stage: fetch kt+1 into set 0
       const int64_t k1 = (int64_t)(kt + 1) * RK;
-      lw.fetch(a.w, a.ldw, f0, a.nout, k1, a.k);
-      lx.fetch(a.x, a.ldx, p0, a.npts, k1, a.k);
+      lw[0].fetch(a.w, a.ldw, f0, a.nout, k1, a.k);
+      lx[0].fetch(a.x, a.ldx, p0, a.npts, k1, a.k);
+    }
+    if (DEEP && kt + 2 < nk) {
+      const int64_t k2 = (int64_t)(kt + 2) * RK;
+      if (cur_set == 0) {
+        lw[0].fetch(a.w, a.ldw, f0, a.nout, k2, a.k);
+        lx[0].fetch(a.x, a.ldx, p0, a.npts, k2, a.k);
+      } else {
+        lw[1].fetch(a.w, a.ldw, f0, a.nout, k2, a.k);
+        lx[1].fetch(a.x, a.ldx, p0, a.npts, k2, a.k);
+      }
     }
     const S* ws = cur;
     const S* xs = cur + NW * WT;
@@ -260,7 +284,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
       for (int s = 0; s < NW; ++s)
 #pragma unroll
         for (int i = 0; i < FB; ++i)
-          wf[s][i] = *reinterpret_cast<const frag_t*>(ws + s * WT + (wrow + 4 * i) * LDK + kk + kg);
+          wf[s][i] = *reinterpret_cast<const frag_t*>(
+              ws + s * WT + (wn * FB * 16 + (GE == 8 ? 32 * (i >> 1) + 8 * rg + 4 * (i & 1) : 16 * i + 4 * rg) + rr) * LDK +
+              kk + kg);
 #pragma unroll
       for (int c = 0; c < CI; ++c)
 #pragma unroll
@@ -276,11 +302,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
             acc[Plan::ta[t]][i][j] = PR::mfma(wv, xf[Plan::tc[t]][j], acc[Plan::ta[t]][i][j]);
         }
     }
-    if (more) {
+    if (kt + 1 < nk) {
       if (!DB) __syncthreads();
       S* nxt = (DB && !(kt & 1)) ? buf1 : buf0;
-      lw.commit(nxt);
-      lx.commit(nxt + NW * WT);
+      if (DEEP && cur_set == 0) {  // stage kt+1 lives in register set (kt+1)&1 (set 0 when not DEEP)
+        lw[1].commit(nxt);
+        lx[1].commit(nxt + NW * WT);
+      } else {
+        lw[0].commit(nxt);
+        lx[0].commit(nxt + NW * WT);
+      }
     }
     __syncthreads();
   }
@@ -290,8 +321,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   for (int j = 0; j < PB; ++j) {
     const int64_t p = p0 + wp * PB * 16 + j * 16 + fr;
 #pragma unroll
-    for (int q = 0; q < FB / 2; ++q) {
-      const int64_t f = f0 + wn * FB * 16 + 4 * FB * g + 8 * q;
+    for (int q = 0; q < FB * 4 / GE; ++q) {
+      const int64_t f = f0 + wn * FB * 16 + (GE == 8 ? 32 * q + 8 * g : 16 * q + 4 * g);
       const int64_t rem = a.nout - f;
       if (p < a.npts && rem > 0) {
         const int nvalid = rem >= GE ? GE : (int)rem;
@@ -299,7 +330,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
 #pragma unroll
         for (int s = 0; s < NA; ++s)
 #pragma unroll
-          for (int e = 0; e < GE; ++e) v[s][e] = (M)acc[s][2 * q + (e >> 2)][j][e & 3];
+          for (int e = 0; e < GE; ++e)
+            v[s][e] = GE == 8 ? (M)acc[s][2 * q + (e >> 2)][j][e & 3] : (M)acc[s][q][j][e];
         Plan::template epilogue<P, GE>(a, p, f, nvalid, v);
       }
     }
@@ -322,6 +354,7 @@ template <> struct TileFor<3> { static constexpr int FB = 2, PB = 1, WPN = 2, WP
 
 // Hidden layer forward: Z = omega*(X W^T + b) (all channels), Y = f-jet(Z).
 template <int D, int ACT> struct PlanHiddenFwd {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::store_t); }
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
@@ -367,6 +400,7 @@ template <int D, int ACT> struct PlanHiddenFwd {
 
 // Output layer, raw jets: O = X W^T (+ b on the value channel), field_t out.
 template <int D> struct PlanOutJets {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::math_t); }
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
@@ -387,6 +421,7 @@ template <int D> struct PlanOutJets {
 //   u = psi_y, v = -psi_x  (mass fields),  omega = -(psi_xx + psi_yy).
 // Inputs: x[0]=gx, x[1]=gy, x[2]=hxx, x[3]=hyy of the last hidden jet.
 struct PlanOutFields2 {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::math_t); }
   static constexpr int CI = 4, NW = 1, NA = 3, NT = 4;
   static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
   static constexpr int ta[NT] = {0, 1, 2, 2};
@@ -416,6 +451,7 @@ struct PairIdx3 {
   }
 };
 struct PlanOutFields3 {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::math_t); }
   static constexpr int CI = 9, NW = 3, NA = 12, NT = 24;
   static constexpr int FB = 2, PB = 1, WPN = 2, WPP = 2;
 #define A1(a) ((a + 1) % 3)
@@ -484,6 +520,7 @@ __device__ inline void reverse_epilogue(const RowArgs& a, int64_t p, int64_t f, 
 // activation chain of layer l-1 with its tape Z:  out = omega * f^T(Z, A).
 // The host passes W_l^T so the kernel still contracts X . W^T.
 template <int D, int ACT> struct PlanHiddenBwd {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::store_t); }
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
@@ -500,6 +537,7 @@ template <int D, int ACT> struct PlanHiddenBwd {
 // w[0] = Bm = 2w W^T dL/dM, w[1] = Bs = 2w W^T dL/dS  ([width][K]).
 //   adj gx = -(v . Bm), adj gy = u . Bm, adj hxx = adj hyy = -(omega . Bs)
 template <int ACT> struct PlanOutBwd2 {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::store_t); }
   static constexpr int CI = 3, NW = 2, NA = 3, NT = 3;
   static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
   static constexpr int ta[NT] = {1, 0, 2};
@@ -523,6 +561,7 @@ template <int ACT> struct PlanOutBwd2 {
 // Weights w[0..2] = 2w W_c^T dL/dM, w[3..5] = 2w W_c^T dL/dS.
 // Accumulators = adjoints of the last hidden jet channels g0..g2, h0..h5.
 template <int ACT> struct PlanOutBwd3 {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::store_t); }
   static constexpr int CI = 12, NW = 6, NA = 9, NT = 24;
   static constexpr int FB = 2, PB = 1, WPN = 2, WPP = 2;
   using F = PlanOutFields3;

@@ -99,7 +99,15 @@ __global__ __launch_bounds__(256) void sum_splits_kernel(const double* __restric
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= cols) return;
   double s = 0.0;
-  for (int q = 0; q < splits; ++q) s += partial[(int64_t)q * cols + c];
+  int q = 0;
+  for (; q + 8 <= splits; q += 8) {  // 8 independent loads in flight, added in order
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t[u] = partial[(int64_t)(q + u) * cols + c];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += t[u];
+  }
+  for (; q < splits; ++q) s += partial[(int64_t)q * cols + c];
   out[c] = (beta == 0.0 ? 0.0 : beta * out[c]) + s;
 }
 

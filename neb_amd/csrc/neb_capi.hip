@@ -12,6 +12,7 @@
 #include "crossgemm.hpp"
 #include "jet_rowgemm.hpp"
 #include "misc_kernels.hpp"
+#include "spd_inverse.hpp"
 
 namespace {
 
@@ -527,6 +528,41 @@ int neb_adam(int dtype, void* p, const void* g, void* m, void* v, int64_t n, dou
   else
     return fail(NEB_ERR_ARG, "neb_adam: fp32 or fp64 master weights only");
   return hip_status("neb_adam");
+}
+
+size_t neb_spd_inverse_workspace_size(int64_t K) {
+  return (size_t)(2 * K * K + SPD_NB * SPD_NB) * sizeof(double);
+}
+
+int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
+                    size_t ws_bytes, void* stream) {
+  if (K < 1 || lda < K || ldx < K) return fail(NEB_ERR_SHAPE, "neb_spd_inverse: bad sizes");
+  if (!A || !X || !info || !workspace) return fail(NEB_ERR_ARG, "neb_spd_inverse: null pointer");
+  if (A == X) return fail(NEB_ERR_ARG, "neb_spd_inverse: X must not alias A");
+  if (ws_bytes < neb_spd_inverse_workspace_size(K)) return fail(NEB_ERR_ARG, "neb_spd_inverse: workspace too small");
+  double* buf[2] = {reinterpret_cast<double*>(workspace), reinterpret_cast<double*>(workspace) + K * K};
+  double* Pi = reinterpret_cast<double*>(workspace) + 2 * K * K;
+  hipStream_t st = S(stream);
+  if (int rc = set_lds(spd_update_kernel, spd_update_lds_bytes())) return rc;
+  const int64_t nsteps = (K + SPD_NB - 1) / SPD_NB;
+  const unsigned nt = (unsigned)nsteps;
+  const double* src = A;
+  int64_t lds = lda;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    const int64_t k0 = s * SPD_NB;
+    const int nb = (int)(K - k0 < SPD_NB ? K - k0 : SPD_NB);
+    hipLaunchKernelGGL(spd_pivot_kernel, dim3(1), dim3(256), 0, st, src, lds, k0, nb, Pi, info, s == 0 ? 1 : 0);
+    if (int rc = hip_status("neb_spd_inverse pivot")) return rc;
+    const bool last = s == nsteps - 1;
+    double* dst = last ? X : buf[s & 1];
+    const int64_t ldd = last ? ldx : K;
+    hipLaunchKernelGGL(spd_update_kernel, dim3(nt, nt), dim3(256), spd_update_lds_bytes(), st, src, lds, K, k0, nb,
+                       Pi, dst, ldd, last ? -1.0 : 1.0);
+    if (int rc = hip_status("neb_spd_inverse update")) return rc;
+    src = dst;
+    lds = ldd;
+  }
+  return NEB_OK;
 }
 
 int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst,

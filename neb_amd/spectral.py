@@ -28,6 +28,25 @@ def _sym(A):
     return 0.5 * (A + A.T)
 
 
+def spd_inverse(M: torch.Tensor) -> torch.Tensor:
+    """FP64 inverse of a symmetric positive definite matrix on the GPU
+    (libnebhip blocked Gauss-Jordan sweeps).  Raises FloatingPointError when
+    a pivot is not positive, i.e. the matrix is not positive definite (the
+    gate of MB/velocity_trace_jet2.py:109-111); no update is made then."""
+    K = M.shape[0]
+    A = M.to(torch.float64).contiguous()
+    X = torch.empty_like(A)
+    info = torch.empty(1, dtype=torch.int32, device=A.device)
+    need = nw.H.load().neb_spd_inverse_workspace_size(K)
+    ws = nw.workspace(A.device).get(need)
+    nw.H.call("neb_spd_inverse", K, A.data_ptr(), A.stride(0), X.data_ptr(), X.stride(0), info.data_ptr(),
+              ws.data_ptr(), ws.numel(), nw._stream())
+    bad = int(info.item())
+    if bad != 0:
+        raise FloatingPointError(f"mass matrix is not positive definite (pivot {bad}); no update")
+    return _sym(X)
+
+
 def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridge: float = 0.0,
                        cond_limit: float = 1e12, gate: str = "eig"):
     """L = tr((M + ridge I)^-1 S) + lam ||M - I||_F^2 and its closed-form
@@ -36,7 +55,8 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
     The gate follows MB/velocity_trace_jet2.py:109-111: a nonfinite,
     nonpositive or too ill-conditioned mass raises FloatingPointError before
     anything is updated.  gate="eig" checks eigenvalues (the reference's
-    rule); gate="cholesky" only requires the FP64 Cholesky to succeed.
+    rule); gate="cholesky" only requires positive definiteness (every pivot
+    of the FP64 inversion positive, as a Cholesky factorization requires).
     Returns (loss: float, dM, dS) with FP64 tensors on M's device."""
     K = M.shape[0]
     eye = torch.eye(K, dtype=torch.float64, device=M.device)
@@ -46,10 +66,7 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
         ok = bool(torch.isfinite(e).all()) and float(e[-1]) > 0 and float(e[0] / e[-1]) >= 1.0 / cond_limit
         if not ok:
             raise FloatingPointError("Physical mass zero/rank deficient/condition exceeds limit; no update")
-    L, info = torch.linalg.cholesky_ex(Mr)
-    if gate != "eig" and int(info.item()) != 0:
-        raise FloatingPointError("mass Cholesky failed; no update")
-    inv = _sym(torch.cholesky_inverse(L))
+    inv = spd_inverse(Mr)
     Ss = _sym(S)
     D = M - eye
     loss = torch.sum(inv * Ss) + lam * torch.sum(D * D)
@@ -61,12 +78,12 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
 class SpectralSolver:
     """Stateful FP64 evaluator of the spectral objective for a training loop.
 
-    Step 1 (or whenever the check below fails) factors M_r = M + ridge I by
-    FP64 Cholesky, as spectral_objective does.  Later steps start from the
+    Step 1 (or whenever the check below fails) inverts M_r = M + ridge I
+    directly (FP64, positive-definiteness gate), as spectral_objective does.  Later steps start from the
     previous step's inverse X and refine it with Newton-Schulz iterations
     X <- X (2I - M_r X) (two FP64 GEMMs each, quadratic convergence), which
     is valid while the residual ||I - M_r X||_F < 1; the iteration stops at a
-    relative residual <= tol, else the Cholesky path runs.  Either way the
+    relative residual <= tol, else the direct inverse runs.  Either way the
     returned inverse satisfies ||I - M_r X||_F <= tol, and a condition
     estimate ||M_r||_F ||X||_F above cond_limit raises FloatingPointError
     (the gate of MB/velocity_trace_jet2.py:109-111)."""
@@ -102,11 +119,8 @@ class SpectralSolver:
             else:
                 X = None
         if X is None:
-            L, info = torch.linalg.cholesky_ex(Mr)
-            if int(info.item()) != 0:
-                raise FloatingPointError("mass Cholesky failed; no update")
-            X = torch.cholesky_inverse(L)
-            self.last_path = "cholesky"
+            X = spd_inverse(Mr)
+            self.last_path = "spd-inverse"
         X = _sym(X)
         cond = float(torch.linalg.norm(Mr) * torch.linalg.norm(X))
         if not math.isfinite(cond) or cond > self.cond_limit:

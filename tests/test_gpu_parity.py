@@ -393,7 +393,7 @@ def test_spectral_solver_paths_agree():
     Mt, St = torch.tensor(M0, device=DEV), torch.tensor(S, device=DEV)
     solver = nb.SpectralSolver(1e-2)
     solver(Mt, St)
-    assert solver.last_path == "cholesky"
+    assert solver.last_path == "spd-inverse"
     M1 = Mt + 1e-4 * torch.tensor(np.eye(K), device=DEV)
     L1, dM1, dS1 = solver(M1, St)
     assert solver.last_path == "newton-schulz"
@@ -401,3 +401,29 @@ def test_spectral_solver_paths_agree():
     assert abs(float(L1) - float(L2)) < 1e-10 * abs(float(L2))
     r = rel(dM1.cpu(), dM2.cpu()); assert r < 1e-9, r
     r = rel(dS1.cpu(), dS2.cpu()); assert r < 1e-9, r
+
+
+@pytest.mark.parametrize("K", [5, 64, 100, 300, 1024])
+def test_spd_inverse_against_lapack(K):
+    from neb_amd import spectral as sp
+    rng = np.random.default_rng(K)
+    B = rng.normal(size=(K + 40, K))
+    A = B.T @ B / K + 1e-3 * np.eye(K)
+    X = sp.spd_inverse(torch.tensor(A, device=DEV)).cpu().numpy()
+    ref = np.linalg.inv(A)
+    r = rel(X, ref)
+    assert r < 1e-10 * max(1.0, np.linalg.cond(A) / 1e4), r
+    assert np.array_equal(X, X.T)
+
+
+def test_spd_inverse_rejects_indefinite():
+    from neb_amd import spectral as sp
+    rng = np.random.default_rng(1)
+    B = rng.normal(size=(80, 70))
+    A = B.T @ B / 70
+    A[40, 40] = -1.0  # breaks positive definiteness in the second pivot block
+    with pytest.raises(FloatingPointError):
+        sp.spd_inverse(torch.tensor(A, device=DEV))
+    # the flag is cleared by the next call
+    ok = B.T @ B / 70 + np.eye(70)
+    sp.spd_inverse(torch.tensor(ok, device=DEV))
