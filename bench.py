@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the host oracle baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 comparison step")
     ap.add_argument("--kernel", default=None,
-                    choices=["hidden_fwd", "hidden_bwd", "out_fields", "out_bwd", "gram", "cross_dw", "copy"],
+                    choices=["hidden_fwd", "hidden_bwd", "out_fields", "out_bwd", "gram", "gram_f32in", "cross_dw",
+                             "copy"],
                     help="run one kernel of the config-B step --steps times (for counter profiling)")
     ap.add_argument("--config", default="B", choices=["B", "E"],
                     help="B: 2D 256^2 grid K=1024 (default); E: 3D blob domain, 1,599,754 points, K=1024")
@@ -88,10 +89,11 @@ def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=
 
     def step():
         p = nw.PRECISIONS[prec]
+        split = nw.split_fields_for(prec, gram_mode)
         tape = mark(lambda: nw.trunk_forward(net, x, p), "trunk_forward")
-        fields = mark(lambda: nw.output_fields(net, tape), "output_fields")
-        M = mark(lambda: nw.gram(fields.mass, K, w, gram_mode), "gram_mass")
-        S = mark(lambda: nw.gram(fields.stiff, K, w, gram_mode), "gram_stiff")
+        fields = mark(lambda: nw.output_fields(net, tape, split=split), "output_fields")
+        M = mark(lambda: nw.field_gram(fields, "mass", w, gram_mode), "gram_mass")
+        S = mark(lambda: nw.field_gram(fields, "stiff", w, gram_mode), "gram_stiff")
         loss, dM, dS = mark(lambda: solver(M, S), "objective_fp64")
         g = mark(lambda: nw.backward_fields(net, tape, fields, dM, dS, w), "reverse")
         mark(lambda: net.adam(g, 1e-4), "adam")
@@ -160,7 +162,7 @@ def bench_kernel(args):
     net = nb.EigenfluidNet(cfg, seed=0, device=dev)
     p = nw.PRECISIONS["bf16"]
     tape = nw.trunk_forward(net, x, p)
-    fields = nw.output_fields(net, tape)
+    fields = nw.output_fields(net, tape, split=True)
     stg = net.staged(p)
     s = torch.cuda.current_stream().cuda_stream
     C = 6
@@ -169,7 +171,9 @@ def bench_kernel(args):
     dM = torch.eye(K, dtype=torch.float64, device=dev) * 1e-3
     Bm = torch.zeros((256, K), dtype=torch.bfloat16, device=dev)
     A = torch.empty_like(Y)
-    copy_dst = torch.empty_like(fields.mass)
+    f32in = nw.output_fields(net, tape) if args.kernel == "gram_f32in" else None
+    copy_src = torch.empty(fields.mass.shape, dtype=torch.float32, device=dev) if args.kernel == "copy" else None
+    copy_dst = torch.empty_like(copy_src) if copy_src is not None else None
 
     def run():
         k = args.kernel
@@ -180,17 +184,19 @@ def bench_kernel(args):
             H.call("neb_jet_layer_backward", p, 2, H.TANH, N, 256, 256, Y.data_ptr(), 256, stg["WT"][2].data_ptr(),
                    256, Z.data_ptr(), 256, 1.0, out.data_ptr(), 256, s)
         elif k == "out_fields":
-            nw.output_fields(net, tape)
+            nw.output_fields(net, tape, split=True)
         elif k == "out_bwd":
             H.call("neb_jet_output_backward", p, 2, H.TANH, N, 256, K, fields.mass.data_ptr(),
-                   fields.stiff.data_ptr(), K, 0, Bm.data_ptr(), Bm.data_ptr(), K, Z.data_ptr(), 256, 1.0,
+                   fields.stiff.data_ptr(), K, H.BF16, 0, Bm.data_ptr(), Bm.data_ptr(), K, Z.data_ptr(), 256, 1.0,
                    out.data_ptr(), 256, s)
         elif k == "gram":
-            nw.gram(fields.mass, K, w, "bf16x3")
+            nw.field_gram(fields, "mass", w, "bf16x3")
+        elif k == "gram_f32in":
+            nw.gram(f32in.mass, K, w, "bf16x3")
         elif k == "cross_dw":
             nw.cross([(Y.view(C * N, 256), Z.view(C * N, 256), C * N, 1.0)], 256, 256, nw._CROSS[p])
         elif k == "copy":
-            copy_dst.copy_(fields.mass)
+            copy_dst.copy_(copy_src)
 
     run()
     torch.cuda.synchronize()
@@ -201,7 +207,7 @@ def bench_kernel(args):
     e1.record()
     torch.cuda.synchronize()
     print(json.dumps({"kernel": args.kernel, "launches": args.steps, "ms_per_launch": e0.elapsed_time(e1) / args.steps,
-                      "copy_bytes": 2 * fields.mass.numel() * 4 if args.kernel == "copy" else None}), flush=True)
+                      "copy_bytes": 2 * copy_src.numel() * 4 if copy_src is not None else None}), flush=True)
 
 
 N3D = 1_599_754
@@ -233,23 +239,26 @@ def bench_3d(args):
     net = nb.EigenfluidNet(cfg, seed=0, device=dev)
     # the 3K=3072-output net spans curls of 3*256 fields: rank(M) <= 768 < K,
     # so the objective runs with a fixed ridge of 1e-4 * mean diag(M_0)
-    f0 = nw.output_fields(net, nw.trunk_forward(net, x[:CHUNK3D], nw.PRECISIONS["bf16"]))
+    tape0 = nw.trunk_forward(net, x[:CHUNK3D], nw.PRECISIONS["bf16"])
+    f0 = nw.output_fields(net, tape0)
+    s0 = nw.output_fields(net, tape0, split=True)
+    del tape0
     M0 = nw.gram(f0.mass, K, w * N / CHUNK3D, "bf16x3")
     ridge = 1e-4 * float(torch.diagonal(M0).mean())
     # Gram rate on one chunk's fields (mass R=393,216, grad-u R=1,179,648)
     gram_stats = {}
-    for mode in ("bf16x3", "f32"):
-        G = nw.gram(f0.stiff, K, w, mode)
+    for mode, src, gm in (("bf16x3-split", s0, "bf16x3"), ("bf16x3", f0, "bf16x3"), ("f32", f0, "f32")):
+        G = nw.field_gram(src, "stiff", w, gm)
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        nw.gram(f0.mass, K, w, mode, out=G)
-        nw.gram(f0.stiff, K, w, mode, out=G)
+        nw.field_gram(src, "mass", w, gm, out=G)
+        nw.field_gram(src, "stiff", w, gm, out=G)
         e1.record()
         torch.cuda.synchronize()
         t = e0.elapsed_time(e1) * 1e-3
         gram_stats[mode] = dict(tflops=(12 * CHUNK3D) * K * (K + 1) / t / 1e12, ms=t * 1e3)
-    del f0, G
+    del f0, s0, G
     solver = sp.SpectralSolver(1e-2, ridge)
 
     def step(prec, gmode, solv):
@@ -328,30 +337,39 @@ def main():
     # ridge of 1e-4 * mean diag(M_0) makes the FP64 Cholesky well posed.
     tape = nw.trunk_forward(net, x, nw.PRECISIONS["bf16"])
     fields = nw.output_fields(net, tape)
+    split = nw.output_fields(net, tape, split=True)
     M0 = nw.gram(fields.mass, K, w, "bf16x3")
     ridge = 1e-4 * float(torch.diagonal(M0).mean())
 
     # ---- Gram modes on the same fields: TFLOP/s and error vs FP64 ----------
+    # "bf16x3-split": the step's path (pre-split bf16 planes); "bf16x3" splits
+    # the fp32 fields per tile; both give the same bits.
     phi64 = fields.mass[:, :K].double().contiguous()
     G64 = nw.gram(phi64, K, w, "f64")
     del phi64
     gram_stats = {}
     Rm = fields.mass.shape[0]
     flops = Rm * K * (K + 1)
-    for mode in ("bf16x3", "f32"):
-        G = nw.gram(fields.mass, K, w, mode)
+    runs = {"bf16x3-split": lambda out: nw.field_gram(split, "mass", w, "bf16x3", out=out),
+            "bf16x3": lambda out: nw.gram(fields.mass, K, w, "bf16x3", out=out),
+            "f32": lambda out: nw.gram(fields.mass, K, w, "f32", out=out)}
+    results = {}
+    for mode, run in runs.items():
+        G = run(None)
+        results[mode] = G
         err = float(torch.linalg.norm(G - G64) / torch.linalg.norm(G64))
         torch.cuda.synchronize()
         reps = 5
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            nw.gram(fields.mass, K, w, mode, out=G)
+            run(G)
         e1.record()
         torch.cuda.synchronize()
         t = e0.elapsed_time(e1) / reps * 1e-3
         gram_stats[mode] = dict(tflops=flops / t / 1e12, ms=t * 1e3, rel_err_vs_fp64=err)
-    del G64, tape, fields
+    gram_stats["split_bit_identical"] = bool(torch.equal(results["bf16x3-split"], results["bf16x3"]))
+    del G64, tape, fields, split, results
 
     # ---- FP32 comparison step ------------------------------------------------
     f32_rate = None

@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 1
+#define NEB_ABI_VERSION 2
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -94,6 +94,17 @@ int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, co
                           const void* w, int64_t ldw, void* mass, void* stiff, int64_t ldf, int layout,
                           void* stream);
 
+/* The same fields (prec NEB_BF16 only) written as two bf16 planes each:
+ * hi = bf16(x) and lo = bf16(x - hi) of the fp32 value x, in the layout of
+ * neb_jet_output_fields.  neb_gram_split consumes both planes (the split
+ * Gram without re-splitting per tile); the hi plane alone is the bf16
+ * operand of neb_jet_output_backward and of the weight-gradient crosses.
+ * Replaces the same call sites as neb_jet_output_fields (VCORE:71-77)
+ * where the Gram runs in split precision. */
+int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy, const void* w,
+                                int64_t ldw, void* mass_hi, void* mass_lo, void* stiff_hi, void* stiff_lo,
+                                int64_t ldf, int layout, void* stream);
+
 /* Output-layer reverse fused with the lift transpose and the transpose of
  * the last hidden activation.  Given field matrices Phi (mass, stiff) and
  * bm = 2w W^T dL/dM, bs = 2w W^T dL/dS ([width][ldb]; in 3D three
@@ -102,11 +113,13 @@ int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, co
  * adj = omega * act-jet^T(z, lift^T(2w Phi dL/dG) W).  Replaces
  * VCORE:94-100 (gu = U Cu/n, lift) + VJ2.product_reverse/linear_reverse +
  * FUSED:9-19 (and HB2P:1115-1174 in 2D).  z: tape packed [C][n][ldz];
- * adj: packed [C][n][lda]. */
+ * adj: packed [C][n][lda].  fields_dtype: NEB_F32 (or the bf16 hi plane,
+ * NEB_BF16) for prec NEB_BF16, NEB_F32 for NEB_F32, NEB_F64 for NEB_F64;
+ * fp32 fields meet the bf16 MFMA rounded to bf16, i.e. as the hi plane. */
 int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width, int64_t K, const void* mass,
-                            const void* stiff, int64_t ldf, int layout, const void* bm, const void* bs,
-                            int64_t ldb, const void* z, int64_t ldz, double omega, void* adj, int64_t lda,
-                            void* stream);
+                            const void* stiff, int64_t ldf, int fields_dtype, int layout, const void* bm,
+                            const void* bs, int64_t ldb, const void* z, int64_t ldz, double omega, void* adj,
+                            int64_t lda, void* stream);
 
 /* Hidden layer reverse: adj_out = omega_prev * act-jet^T(z_prev, adj_in W).
  * wt = W^T staged as [k][ldw] (k = layer input width).  Replaces
@@ -137,6 +150,16 @@ int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, co
 int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int phi_dtype, double w,
              const double* roww, double beta, double* G, int64_t ldg, void* workspace, size_t ws_bytes,
              void* stream);
+
+/* Split-precision Gram  G = beta*G + w * Phi^T Phi  (FP64, full symmetric)
+ * from pre-split planes (neb_jet_output_fields_split): three bf16 MFMA
+ * products hi.hi + hi.lo + lo.hi per pair, fp32 accumulation flushed into
+ * fp32 partial sums every 256 rows, FP64 split reduction.  Bit-identical to
+ * neb_gram(NEB_CROSS_BF16X3) on the fp32 matrix the planes came from.
+ * Workspace: neb_cross_workspace_size(K, K, rows, 1).  Replaces VCORE:40-46
+ * `grams` on the split-precision path. */
+int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w, double beta,
+                   double* G, int64_t ldg, void* workspace, size_t ws_bytes, void* stream);
 
 /* Column sums in FP64: out[c] = beta*out[c] + sum_r x[r][c] (bias
  * gradients, HB2P:1101 `grad_bias`, VJ2:30).  Workspace from

@@ -15,6 +15,8 @@
 //   CR_BF16X3 fp32 operands split on load into hi = bf16(x), lo = bf16(x-hi);
 //             x.y ~ hi.hi + hi.lo + lo.hi  (three bf16 MFMAs, fp32 accumulate)
 //   CR_F64    fp64 operands, v_mfma_f64_16x16x4f64
+// CR_BF16X3 also takes operands already split into hi / lo bf16 planes
+// (SplitBF16), which skips the split in the loader.
 // MFMA accumulators are flushed into a second fp32 (fp64 for CR_F64)
 // register set every FLUSH rows, which bounds the length of any fp32
 // rounding chain; partials are then combined in FP64.
@@ -32,7 +34,13 @@ struct CrossSeg {
   const void* y;
   int64_t ldx, ldy, rows;
   double sign;
+  const void* xlo;  // lo planes of pre-split operands (SplitBF16), else null
+  const void* ylo;
 };
+
+// Operand tag: an fp32 matrix given as two bf16 planes hi = bf16(x) and
+// lo = bf16(x - hi) with the same leading dimension (CR_BF16X3 only).
+struct SplitBF16 {};
 
 struct CrossArgs {
   CrossSeg seg[CR_MAXSEG];
@@ -93,10 +101,20 @@ struct CrossLoad {
   TIN v[HR];  // raw loads: conversion happens in commit(), after the MFMAs
   int64_t r0_, rows_;
 
-  __device__ inline void fetch(const TIN* __restrict__ src, int64_t ld, int64_t r0, int64_t rows, int64_t c0,
-                               int64_t cols) {
+  __device__ inline void fetch(const TIN* __restrict__ src, const void*, int64_t ld, int64_t r0, int64_t rows,
+                               int64_t c0, int64_t cols) {
     const int col = threadIdx.x % CR_TM;
     const int rh = (threadIdx.x / CR_TM) * HR;
+    r0_ = r0;
+    rows_ = rows;
+    if (r0 + RK <= rows && c0 + CR_TM <= cols) {
+      // interior tile (workgroup-uniform test): uniform base, 32-bit offsets
+      const TIN* base = src + r0 * ld + c0;
+      const int ld32 = (int)ld;
+#pragma unroll
+      for (int e = 0; e < HR; ++e) v[e] = base[(rh + e) * ld32 + col];
+      return;
+    }
     const int64_t gc = c0 + col;
     const bool cok = gc < cols;
 #pragma unroll
@@ -104,8 +122,6 @@ struct CrossLoad {
       const int64_t gr = r0 + rh + e;
       v[e] = (cok && gr < rows) ? src[gr * ld + gc] : (TIN)0;
     }
-    r0_ = r0;
-    rows_ = rows;
   }
 
   // scale: segment sign (applied before rounding to the MFMA operand type);
@@ -147,6 +163,84 @@ struct CrossLoad {
   }
 };
 
+// Loader of pre-split operands: thread t owns the column pair 2*(t%64),
+// +1 and rows 8*(t/64) .. +7 of the k-step, so each global load is one
+// 4-byte pair per lane and one 256-byte row segment per wave.  commit()
+// transposes the pairs into two 8-row column vectors per plane (16-bit
+// lane shuffles only) and applies a -1 sign by flipping sign bits, which
+// is exact for both planes.
+template <>
+struct CrossLoad<CR_BF16X3, SplitBF16> {
+  using S = __bf16;
+  using V = float;
+  static constexpr int RK = CrossPol<CR_BF16X3>::RK, LDK = RK + CrossPol<CR_BF16X3>::PAD, RG = 8;
+  static_assert(RK == 4 * RG, "four waves cover the k-step");
+  uint32_t h[RG], l[RG];
+  uint32_t cmask;  // columns of the pair inside the matrix (per 16-bit half)
+  int nrow;        // rows of this thread's 8 inside the segment
+
+  // Every load is unconditional from an in-bounds address (rows past the
+  // end re-read the last row, a pair past the last column re-reads column
+  // 0; a pair straddling the last column stays inside the padded row, whose
+  // length is even); commit() zeroes what lies outside.  No branch sits
+  // between the loads, so they all stay in flight together.
+  __device__ inline void fetch(const SplitBF16* src, const void* src_lo, int64_t ld, int64_t r0, int64_t rows,
+                               int64_t c0, int64_t cols) {
+    const __bf16* H = reinterpret_cast<const __bf16*>(src);
+    const __bf16* L = reinterpret_cast<const __bf16*>(src_lo);
+    const int cp = threadIdx.x & 63, rg = (threadIdx.x >> 6) * RG;
+    if (r0 + RK <= rows && c0 + CR_TM <= cols) {
+      // interior tile (workgroup-uniform test): uniform bases, 32-bit offsets
+      const int64_t b = r0 * ld + c0;
+      const int ld32 = (int)ld;
+      cmask = 0xffffffffu;
+      nrow = RG;
+#pragma unroll
+      for (int e = 0; e < RG; ++e) {
+        const int off = (rg + e) * ld32 + 2 * cp;
+        h[e] = *reinterpret_cast<const uint32_t*>(H + b + off);
+        l[e] = *reinterpret_cast<const uint32_t*>(L + b + off);
+      }
+      return;
+    }
+    const int64_t gc = c0 + 2 * cp;
+    cmask = gc + 1 < cols ? 0xffffffffu : (gc < cols ? 0x0000ffffu : 0u);
+    const int64_t cc = gc < cols ? gc : 0;
+    const int64_t rleft = rows - (r0 + rg);
+    nrow = rleft >= RG ? RG : (rleft > 0 ? (int)rleft : 0);
+#pragma unroll
+    for (int e = 0; e < RG; ++e) {
+      int64_t gr = r0 + rg + e;
+      gr = gr < rows ? gr : rows - 1;
+      const int64_t off = gr * ld + cc;
+      h[e] = *reinterpret_cast<const uint32_t*>(H + off);
+      l[e] = *reinterpret_cast<const uint32_t*>(L + off);
+    }
+  }
+
+  __device__ inline void commit(S* hi, S* lo, V scale, const double*) const {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const int cp = threadIdx.x & 63, rg = (threadIdx.x >> 6) * RG;
+    const uint32_t flip = scale < 0 ? 0x80008000u : 0u;
+    u32x4 h0, h1, l0, l1;
+#pragma unroll
+    for (int q = 0; q < RG / 2; ++q) {
+      const uint32_t ma = 2 * q < nrow ? cmask : 0u, mb = 2 * q + 1 < nrow ? cmask : 0u;
+      const uint32_t a = (h[2 * q] ^ flip) & ma, b = (h[2 * q + 1] ^ flip) & mb;
+      h0[q] = (a & 0xffffu) | (b << 16);
+      h1[q] = (a >> 16) | (b & 0xffff0000u);
+      const uint32_t c = (l[2 * q] ^ flip) & ma, d = (l[2 * q + 1] ^ flip) & mb;
+      l0[q] = (c & 0xffffu) | (d << 16);
+      l1[q] = (c >> 16) | (d & 0xffff0000u);
+    }
+    const int o = 2 * cp * LDK + rg;
+    *reinterpret_cast<u32x4*>(hi + o) = h0;
+    *reinterpret_cast<u32x4*>(hi + o + LDK) = h1;
+    *reinterpret_cast<u32x4*>(lo + o) = l0;
+    *reinterpret_cast<u32x4*>(lo + o + LDK) = l1;
+  }
+};
+
 // Row range of split `split` within a segment of `rows` rows.
 template <int RK>
 __device__ inline void split_range(int64_t rows, int splits, int split, int64_t& rbeg, int64_t& rend) {
@@ -172,20 +266,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   S* const buf0 = reinterpret_cast<S*>(smem);
   S* const buf1 = buf0 + STAGE;
 
-  // XCD-aware map: workgroup L runs on XCD L % 8.  All tiles of one row split
-  // go to the same XCD (consecutive L/8), so the rows they share are read from
-  // HBM once and then served from that XCD's L2.
+  // XCD-aware map: workgroup L runs on XCD L % 8.  The work items
+  // u = split * ntiles + tile are dealt to the XCDs in contiguous blocks of
+  // N/8, so the tiles one XCD runs at a time share their row splits, which
+  // are read from HBM once and then served from that XCD's L2.
   int split, tile;
   {
     const int L = blockIdx.x;
-    if (a.splits % 8 == 0) {
-      const int q = L >> 3;
-      split = (L & 7) + 8 * (q / a.ntiles);
-      tile = q % a.ntiles;
-    } else {
-      split = L / a.ntiles;
-      tile = L % a.ntiles;
-    }
+    const int N = a.ntiles * a.splits;
+    const int u = (N % 8 == 0) ? (L & 7) * (N >> 3) + (L >> 3) : L;
+    split = u / a.ntiles;
+    tile = u % a.ntiles;
   }
   int ti, tj;
   if (a.symmetric) {
@@ -235,8 +326,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   };
   next_seg(0);
   if (s >= a.nseg) goto write_out;
-  lx.fetch(reinterpret_cast<const TX*>(a.seg[s].x), a.seg[s].ldx, r0, rend, i0, a.m);
-  ly.fetch(reinterpret_cast<const TY*>(a.seg[s].y), a.seg[s].ldy, r0, rend, j0, a.n);
+  lx.fetch(reinterpret_cast<const TX*>(a.seg[s].x), a.seg[s].xlo, a.seg[s].ldx, r0, rend, i0, a.m);
+  ly.fetch(reinterpret_cast<const TY*>(a.seg[s].y), a.seg[s].ylo, a.seg[s].ldy, r0, rend, j0, a.n);
   lx.commit(buf0, buf0 + TILE, (typename CrossLoad<MODE, TX>::V)a.seg[s].sign, a.roww);
   ly.commit(buf0 + NL * TILE, buf0 + NL * TILE + TILE, 1, nullptr);
   __syncthreads();
@@ -255,8 +346,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
       }
       const bool more = ns < a.nseg;
       if (more) {
-        lx.fetch(reinterpret_cast<const TX*>(a.seg[ns].x), a.seg[ns].ldx, nr0, rend, i0, a.m);
-        ly.fetch(reinterpret_cast<const TY*>(a.seg[ns].y), a.seg[ns].ldy, nr0, rend, j0, a.n);
+        lx.fetch(reinterpret_cast<const TX*>(a.seg[ns].x), a.seg[ns].xlo, a.seg[ns].ldx, nr0, rend, i0, a.m);
+        ly.fetch(reinterpret_cast<const TY*>(a.seg[ns].y), a.seg[ns].ylo, a.seg[ns].ldy, nr0, rend, j0, a.n);
       }
       const S* xs = stage ? buf1 : buf0;
       const S* ys = xs + NL * TILE;

@@ -47,6 +47,8 @@ struct RowArgs {
   const void* tape;         // pre-activation jet tape (packed, store_t)
   int64_t ldt, tstride;
   int layout;               // fields: 0 = point-major rows, 1 = component-major
+  void* out0lo;             // split fields: bf16 lo planes beside out0 / out1
+  void* out1lo;
 };
 
 template <typename T> __device__ inline T fneg(T v) { return -v; }
@@ -140,6 +142,25 @@ __device__ inline void st_group(T* dst, const M* v, int nvalid) {
 #pragma unroll
     for (int e = 0; e < GE; ++e)
       if (e < nvalid) dst[e] = cvt<T>(v[e]);
+  }
+}
+
+// Field row store at element offset `off`: the math type as is, or (SPLIT)
+// hi = bf16(x) into `out` and lo = bf16(x - hi) into `out_lo`.  x - hi is
+// exact in fp32, so hi + lo carries 16 significant bits of x.
+template <bool SPLIT, int GE, typename M>
+__device__ inline void st_field(void* out, void* out_lo, int64_t off, const M* v, int nv) {
+  if constexpr (SPLIT) {
+    float h[GE], l[GE];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      h[e] = (float)(__bf16)(float)v[e];
+      l[e] = (float)v[e] - h[e];
+    }
+    st_group<__bf16, GE>(reinterpret_cast<__bf16*>(out) + off, h, nv);
+    st_group<__bf16, GE>(reinterpret_cast<__bf16*>(out_lo) + off, l, nv);
+  } else {
+    st_group<M, GE>(reinterpret_cast<M*>(out) + off, v, nv);
   }
 }
 
@@ -420,8 +441,10 @@ template <int D> struct PlanOutJets {
 // 2D output layer with the velocity / vorticity lift (ELC:35-38):
 //   u = psi_y, v = -psi_x  (mass fields),  omega = -(psi_xx + psi_yy).
 // Inputs: x[0]=gx, x[1]=gy, x[2]=hxx, x[3]=hyy of the last hidden jet.
-struct PlanOutFields2 {
-  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::math_t); }
+template <bool SPLIT> struct PlanOutFields2T {
+  template <int P> __host__ __device__ static constexpr int out_bytes() {
+    return SPLIT ? 2 : sizeof(typename Prec<P>::math_t);
+  }
   static constexpr int CI = 4, NW = 1, NA = 3, NT = 4;
   static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
   static constexpr int ta[NT] = {0, 1, 2, 2};
@@ -430,15 +453,14 @@ struct PlanOutFields2 {
   static constexpr bool tneg[NT] = {false, true, true, true};
   template <int P, int GE, typename M>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
-    M* mass = reinterpret_cast<M*>(a.out0);
-    M* stiff = reinterpret_cast<M*>(a.out1);
     const int64_t r0 = a.layout == 0 ? 2 * p : p;
     const int64_t r1 = a.layout == 0 ? 2 * p + 1 : a.npts + p;
-    st_group<M, GE>(mass + r0 * a.ldo + f, v[0], nv);
-    st_group<M, GE>(mass + r1 * a.ldo + f, v[1], nv);
-    st_group<M, GE>(stiff + p * a.ldo2 + f, v[2], nv);
+    st_field<SPLIT, GE, M>(a.out0, a.out0lo, r0 * a.ldo + f, v[0], nv);
+    st_field<SPLIT, GE, M>(a.out0, a.out0lo, r1 * a.ldo + f, v[1], nv);
+    st_field<SPLIT, GE, M>(a.out1, a.out1lo, p * a.ldo2 + f, v[2], nv);
   }
 };
+using PlanOutFields2 = PlanOutFields2T<false>;
 
 // 3D output layer with the curl lift (NCC:97-99, VJ2:93-101).  Vector
 // potential A has output rows 3k+c, passed as three weight slices W_c.
@@ -450,8 +472,10 @@ struct PairIdx3 {
     return a > b ? hid(b, a) : (a == 0 ? b : (a == 1 ? 2 + b : 5));
   }
 };
-struct PlanOutFields3 {
-  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::math_t); }
+template <bool SPLIT> struct PlanOutFields3T {
+  template <int P> __host__ __device__ static constexpr int out_bytes() {
+    return SPLIT ? 2 : sizeof(typename Prec<P>::math_t);
+  }
   static constexpr int CI = 9, NW = 3, NA = 12, NT = 24;
   static constexpr int FB = 2, PB = 1, WPN = 2, WPP = 2;
 #define A1(a) ((a + 1) % 3)
@@ -474,20 +498,19 @@ struct PlanOutFields3 {
                                     false, true, false, true, false, true, false, true, false, true, false, true};
   template <int P, int GE, typename M>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
-    M* U = reinterpret_cast<M*>(a.out0);
-    M* Dg = reinterpret_cast<M*>(a.out1);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const int64_t r = a.layout == 0 ? 3 * p + c : c * a.npts + p;
-      st_group<M, GE>(U + r * a.ldo + f, v[c], nv);
+      st_field<SPLIT, GE, M>(a.out0, a.out0lo, r * a.ldo + f, v[c], nv);
     }
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
       const int64_t r = a.layout == 0 ? 9 * p + c : c * a.npts + p;
-      st_group<M, GE>(Dg + r * a.ldo2 + f, v[3 + c], nv);
+      st_field<SPLIT, GE, M>(a.out1, a.out1lo, r * a.ldo2 + f, v[3 + c], nv);
     }
   }
 };
+using PlanOutFields3 = PlanOutFields3T<false>;
 
 // Shared epilogue of the reverse plans: given the adjoint of the activation
 // output (adj[C][GE]), apply the transposed jet chain with the tape Z and

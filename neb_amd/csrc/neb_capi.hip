@@ -118,36 +118,38 @@ int output_jets(int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy, 
   return launch_rowgemm<P, Sx, PlanOutJets<D>>(a, st, "neb_jet_output_jets");
 }
 
-template <int P>
+template <int P, bool SPLIT = false>
 int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy, const void* w, int64_t ldw,
-                  void* mass, void* stiff, int64_t ldf, int layout, hipStream_t st) {
+                  void* mass, void* stiff, int64_t ldf, int layout, hipStream_t st, void* mass_lo = nullptr,
+                  void* stiff_lo = nullptr) {
   using Sx = typename Prec<P>::store_t;
   const Sx* Y = reinterpret_cast<const Sx*>(y);
   const int64_t cs = n * ldy;
   RowArgs a = row_args();
   a.npts = n; a.nout = K; a.k = k;
   a.out0 = mass; a.out1 = stiff; a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
+  a.out0lo = mass_lo; a.out1lo = stiff_lo;
+  const char* what = SPLIT ? "neb_jet_output_fields_split" : "neb_jet_output_fields";
   if (dim == 2) {
     // channels: 0 v, 1 gx, 2 gy, 3 hxx, 4 hxy, 5 hyy
     const int chans[4] = {1, 2, 3, 5};
     for (int c = 0; c < 4; ++c) { a.x[c] = Y + chans[c] * cs; a.ldx[c] = ldy; }
     a.w[0] = w; a.ldw[0] = ldw;
-    return launch_rowgemm<P, Sx, PlanOutFields2>(a, st, "neb_jet_output_fields");
+    return launch_rowgemm<P, Sx, PlanOutFields2T<SPLIT>>(a, st, what);
   }
   for (int c = 0; c < 9; ++c) { a.x[c] = Y + (1 + c) * cs; a.ldx[c] = ldy; }
   for (int s = 0; s < 3; ++s) {
     a.w[s] = reinterpret_cast<const Sx*>(w) + s * ldw;
     a.ldw[s] = 3 * ldw;
   }
-  return launch_rowgemm<P, Sx, PlanOutFields3>(a, st, "neb_jet_output_fields");
+  return launch_rowgemm<P, Sx, PlanOutFields3T<SPLIT>>(a, st, what);
 }
 
-template <int P, int ACT>
+template <int P, int ACT, typename F>
 int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* mass, const void* stiff, int64_t ldf,
                     int layout, const void* bm, const void* bs, int64_t ldb, const void* z, int64_t ldz,
                     double omega, void* adj, int64_t lda, hipStream_t st) {
   using Sx = typename Prec<P>::store_t;
-  using F = typename FieldT<P>::type;
   const F* Mf = reinterpret_cast<const F*>(mass);
   const F* Sf = reinterpret_cast<const F*>(stiff);
   RowArgs a = row_args();
@@ -211,16 +213,42 @@ int cross_tiles(int64_t m, int64_t n, int symmetric, int* tm, int* tn) {
   return symmetric ? (*tm) * (*tm + 1) / 2 : (*tm) * (*tn);
 }
 
-int cross_splits(int ntiles, int64_t rows) {
-  // about two workgroups per CU (256 CUs), a multiple of 8 so every XCD gets
-  // whole splits, and at least 1024 rows per split
-  int s = (512 + ntiles - 1) / ntiles;
-  s = (s + 7) / 8 * 8;
+// Row splits of a cross product: as many as fill ONE round of resident
+// workgroups (`slots` = CUs x resident workgroups per CU, at most 2 per CU,
+// which bounds the FP64 partials), with ntiles * splits a multiple of 8 so
+// every XCD gets the same number of work items, and >= 1024 rows per split.
+// A second, partly filled round would idle most of the chip for as long as
+// the first.
+constexpr int CR_MAX_SLOTS = 512;
+
+int cross_splits(int ntiles, int64_t rows, int slots = CR_MAX_SLOTS) {
+  if (slots > CR_MAX_SLOTS) slots = CR_MAX_SLOTS;
+  int s = slots / ntiles;
+  while (s > 1 && (s * ntiles) % 8 != 0) --s;
   const int64_t maxs = (rows + 1023) / 1024;
   if (s > maxs) s = (int)maxs;
   if (s < 1) s = 1;
-  if (s > 4096) s = 4096;
   return s;
+}
+
+// Resident workgroups of a kernel on the current device times the CU count
+// (at most 2 per CU), queried once per kernel and device into `cache`.
+template <typename Kern>
+int resident_slots(Kern kern, size_t lds, int (&cache)[16]) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return CR_MAX_SLOTS;
+  if (cache[dev] == 0) {
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 256, lds) !=
+            hipSuccess ||
+        cus < 1 || per_cu < 1) {
+      (void)hipGetLastError();
+      return CR_MAX_SLOTS;
+    }
+    cache[dev] = cus * (per_cu < 2 ? per_cu : 2);
+  }
+  return cache[dev];
 }
 
 template <int MODE, typename TX, typename TY>
@@ -228,6 +256,12 @@ int launch_cross(CrossArgs& a, double scale, double beta, double* G, int64_t ldg
   const size_t lds = cross_lds_bytes<MODE>();
   auto kern = cross_kernel<MODE, TX, TY>;
   if (int rc = set_lds(kern, lds)) return rc;
+  // splits were sized for CR_MAX_SLOTS (the workspace bound); fit one round
+  int64_t maxrows = 0;
+  for (int s = 0; s < a.nseg; ++s) maxrows = a.seg[s].rows > maxrows ? a.seg[s].rows : maxrows;
+  static int slots_cache[16] = {0};  // one per kernel instantiation
+  const int fit = cross_splits(a.ntiles, maxrows, resident_slots(kern, lds, slots_cache));
+  if (fit < a.splits) a.splits = fit;
   hipLaunchKernelGGL(kern, dim3(a.ntiles * a.splits), dim3(256), lds, st, a);
   if (int rc = hip_status("neb_cross")) return rc;
   const int64_t nbi = (a.m + 31) / 32, nbj = (a.n + 31) / 32;
@@ -385,22 +419,46 @@ int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, co
   return output_fields<PREC_F64>(dim, n, k, K, y, ldy, w, ldw, mass, stiff, ldf, layout, st);
 }
 
+int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy, const void* w,
+                                int64_t ldw, void* mass_hi, void* mass_lo, void* stiff_hi, void* stiff_lo,
+                                int64_t ldf, int layout, void* stream) {
+  if (int rc = check_common(NEB_BF16, dim, NEB_TANH)) return rc;
+  if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
+    return fail(NEB_ERR_SHAPE, "neb_jet_output_fields_split: bad sizes");
+  if (!y || !w || !mass_hi || !mass_lo || !stiff_hi || !stiff_lo)
+    return fail(NEB_ERR_ARG, "neb_jet_output_fields_split: null pointer");
+  if (!aligned(y, ldy, NEB_BF16) || !aligned(w, ldw, NEB_BF16) || !aligned(mass_hi, ldf, NEB_BF16) ||
+      !aligned(mass_lo, ldf, NEB_BF16) || !aligned(stiff_hi, ldf, NEB_BF16) || !aligned(stiff_lo, ldf, NEB_BF16))
+    return fail(NEB_ERR_ARG, "neb_jet_output_fields_split: rows must be 16-byte aligned");
+  return output_fields<PREC_BF16, true>(dim, n, k, K, y, ldy, w, ldw, mass_hi, stiff_hi, ldf, layout, S(stream),
+                                        mass_lo, stiff_lo);
+}
+
 int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width, int64_t K, const void* mass,
-                            const void* stiff, int64_t ldf, int layout, const void* bm, const void* bs, int64_t ldb,
-                            const void* z, int64_t ldz, double omega, void* adj, int64_t lda, void* stream) {
+                            const void* stiff, int64_t ldf, int fields_dtype, int layout, const void* bm,
+                            const void* bs, int64_t ldb, const void* z, int64_t ldz, double omega, void* adj,
+                            int64_t lda, void* stream) {
   if (int rc = check_common(prec, dim, act)) return rc;
   if (n < 0 || width < 1 || K < 1 || ldf < K || ldb < K || ldz < width || lda < width ||
       (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_backward: bad sizes");
   if (!mass || !stiff || !bm || !bs || !z || !adj) return fail(NEB_ERR_ARG, "neb_jet_output_backward: null pointer");
-  const int fdt = prec == NEB_F64 ? NEB_F64 : NEB_F32;
-  if (!aligned(mass, ldf, fdt) || !aligned(stiff, ldf, fdt) || !aligned(bm, ldb, prec) || !aligned(bs, ldb, prec))
+  const bool fok = prec == NEB_F64 ? fields_dtype == NEB_F64
+                                   : (fields_dtype == NEB_F32 || (prec == NEB_BF16 && fields_dtype == NEB_BF16));
+  if (!fok) return fail(NEB_ERR_ARG, "neb_jet_output_backward: fields dtype %d does not fit prec %d", fields_dtype, prec);
+  if (!aligned(mass, ldf, fields_dtype) || !aligned(stiff, ldf, fields_dtype) || !aligned(bm, ldb, prec) ||
+      !aligned(bs, ldb, prec))
     return fail(NEB_ERR_ARG, "neb_jet_output_backward: rows must be 16-byte aligned");
   hipStream_t st = S(stream);
-#define OB(P, A) output_backward<P, A>(dim, n, width, K, mass, stiff, ldf, layout, bm, bs, ldb, z, ldz, omega, adj, lda, st)
-  if (prec == NEB_BF16) return act == NEB_TANH ? OB(PREC_BF16, ACT_TANH) : OB(PREC_BF16, ACT_SIN);
-  if (prec == NEB_F32) return act == NEB_TANH ? OB(PREC_F32, ACT_TANH) : OB(PREC_F32, ACT_SIN);
-  return act == NEB_TANH ? OB(PREC_F64, ACT_TANH) : OB(PREC_F64, ACT_SIN);
+#define OB(P, A, F) \
+  output_backward<P, A, F>(dim, n, width, K, mass, stiff, ldf, layout, bm, bs, ldb, z, ldz, omega, adj, lda, st)
+  if (prec == NEB_BF16) {
+    if (fields_dtype == NEB_BF16)
+      return act == NEB_TANH ? OB(PREC_BF16, ACT_TANH, __bf16) : OB(PREC_BF16, ACT_SIN, __bf16);
+    return act == NEB_TANH ? OB(PREC_BF16, ACT_TANH, float) : OB(PREC_BF16, ACT_SIN, float);
+  }
+  if (prec == NEB_F32) return act == NEB_TANH ? OB(PREC_F32, ACT_TANH, float) : OB(PREC_F32, ACT_SIN, float);
+  return act == NEB_TANH ? OB(PREC_F64, ACT_TANH, double) : OB(PREC_F64, ACT_SIN, double);
 #undef OB
 }
 
@@ -440,7 +498,7 @@ int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, co
   memset(&a, 0, sizeof(a));
   int64_t maxrows = 0;
   for (int s = 0; s < nseg; ++s) {
-    if (!x[s] || !y[s] || rows[s] < 0 || ldx[s] < m || ldy[s] < n)
+    if (!x[s] || !y[s] || rows[s] < 0 || ldx[s] < m || ldy[s] < n || ldx[s] >= (1 << 26) || ldy[s] >= (1 << 26))
       return fail(NEB_ERR_SHAPE, "neb_cross: bad segment %d", s);
     a.seg[s].x = x[s]; a.seg[s].y = y[s];
     a.seg[s].ldx = ldx[s]; a.seg[s].ldy = ldy[s];
@@ -472,6 +530,28 @@ int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int
   const double sg[1] = {1.0};
   return neb_cross(mode, 1, xs, xs, lds, lds, rs, sg, phi_dtype, phi_dtype, K, K, roww, 1, w, beta, G, ldg,
                    workspace, ws_bytes, stream);
+}
+
+int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w, double beta,
+                   double* G, int64_t ldg, void* workspace, size_t ws_bytes, void* stream) {
+  if (rows < 0 || K < 1 || ld < K || ld >= (1 << 26) || ldg < K)
+    return fail(NEB_ERR_SHAPE, "neb_gram_split: bad sizes");
+  if (!phi_hi || !phi_lo || !G || !workspace) return fail(NEB_ERR_ARG, "neb_gram_split: null pointer");
+  if (!aligned(phi_hi, ld, NEB_BF16) || !aligned(phi_lo, ld, NEB_BF16))
+    return fail(NEB_ERR_ARG, "neb_gram_split: rows must be 16-byte aligned");
+  CrossArgs a;
+  memset(&a, 0, sizeof(a));
+  a.seg[0].x = phi_hi; a.seg[0].y = phi_hi;
+  a.seg[0].xlo = phi_lo; a.seg[0].ylo = phi_lo;
+  a.seg[0].ldx = ld; a.seg[0].ldy = ld;
+  a.seg[0].rows = rows; a.seg[0].sign = 1.0;
+  a.nseg = 1; a.m = K; a.n = K; a.roww = nullptr; a.symmetric = 1;
+  a.ntiles = cross_tiles(K, K, 1, &a.tiles_m, &a.tiles_n);
+  a.splits = cross_splits(a.ntiles, rows);
+  const size_t need = (size_t)a.splits * a.ntiles * CR_TM * CR_TN * sizeof(double);
+  if (ws_bytes < need) return fail(NEB_ERR_ARG, "neb_gram_split: workspace %zu < %zu bytes", ws_bytes, need);
+  a.partial = reinterpret_cast<double*>(workspace);
+  return launch_cross<CR_BF16X3, SplitBF16, SplitBF16>(a, w, beta, G, ldg, S(stream));
 }
 
 size_t neb_colsum_workspace_size(int64_t rows, int64_t cols) {

@@ -19,19 +19,28 @@
 
 constexpr int SPD_NB = 64;
 
-// Pivot block inversion: Pi = (A[k0:k0+nb, k0:k0+nb])^-1 by scalar sweeps.
-// The 64 x 64 block lives in registers (thread t owns row t/4, columns
-// 16*(t%4) .. +15); each column step exchanges only the pivot row and the
-// pivot column through LDS.  Rows/columns >= nb are identity padding and are
-// never swept.  info: 0 on success, else the 1-based global index of the
-// first nonpositive / nonfinite pivot (first writer wins).
+// Pivot block inversion: Pi = (A[k0:k0+nb, k0:k0+nb])^-1 by scalar sweeps,
+// lazily scaled.  The sweep's rank-1 updates commute with scaling a row or a
+// column, so a swept row/column is left as it is (instead of multiplied by
+// 1/p) and the pivot entry is only negated (stored -p for the true -1/p);
+// afterwards every entry (i, j) carries the pending factor d_i d_j with
+// d_c = 1/p_c.  Per column a thread then does one FMA per entry it holds:
+//   v_ij -= x_i r_j,  x_i = v_ic / p (x_c = 0),  r_j = v_cj (r_c = 0).
+// Pivots p are entries no earlier step has scaled, so the positivity test is
+// the one of the plain sweep.  The 64 x 64 block lives in registers (thread t
+// owns row t/4, columns 16*(t%4) .. +15); each column step exchanges the
+// pivot row and column through LDS (double-buffered: one barrier per column).
+// Rows/columns >= nb are identity padding and are never swept.  info: 0 on
+// success, else the 1-based global index of the first nonpositive /
+// nonfinite pivot (first writer wins).
 __global__ __launch_bounds__(256) void spd_pivot_kernel(const double* __restrict__ A, int64_t lda, int64_t k0,
                                                         int nb, double* __restrict__ Pi, int* __restrict__ info,
                                                         int reset) {
-  __shared__ double rowbuf[2][SPD_NB];
+  __shared__ __align__(16) double rowbuf[2][SPD_NB];
   __shared__ double colbuf[2][SPD_NB];
+  __shared__ double dvec[SPD_NB];
   const int t = threadIdx.x;
-  const int i = t >> 2, jg = (t & 3) * 16;
+  const int i = t >> 2, jb = t & 3, jg = jb * 16;
   if (reset && t == 0) *info = 0;  // first sweep clears the flag (no memset node)
   double v[16];
 #pragma unroll
@@ -40,47 +49,45 @@ __global__ __launch_bounds__(256) void spd_pivot_kernel(const double* __restrict
     v[q] = (i < nb && j < nb) ? A[(k0 + i) * lda + k0 + j] : (i == j ? 1.0 : 0.0);
   }
   int bad = 0;
-  for (int c = 0; c < nb; ++c) {
-    const int b = c & 1;  // alternate buffers: one barrier per column
-    if (i == c) {
+  for (int c0 = 0; c0 < nb; c0 += 16) {
+    const bool own = (c0 >> 4) == jb;  // this thread holds columns c0 .. c0+15
 #pragma unroll
-      for (int q = 0; q < 16; q += 2) *reinterpret_cast<f64x2*>(&rowbuf[b][jg + q]) = f64x2{v[q], v[q + 1]};
-    }
-    if ((c >> 4) == (t & 3)) {
-      const int qc = c & 15;
-      double cv = v[0];
+    for (int u = 0; u < 16; ++u) {
+      const int c = c0 + u, b = u & 1;
+      if (c < nb) {  // uniform over the workgroup
+        if (i == c) {
 #pragma unroll
-      for (int q = 1; q < 16; ++q) cv = (q == qc) ? v[q] : cv;
-      colbuf[b][i] = cv;
-    }
-    __syncthreads();
-    double r[16];
+          for (int q = 0; q < 16; q += 2)
+            *reinterpret_cast<f64x2*>(&rowbuf[b][jg + q]) =
+                f64x2{(own && q == u) ? 0.0 : v[q], (own && q + 1 == u) ? 0.0 : v[q + 1]};
+        }
+        if (own) colbuf[b][i] = v[u];
+        __syncthreads();
+        double r[16];
 #pragma unroll
-    for (int q = 0; q < 16; q += 2) {
-      const f64x2 rv = *reinterpret_cast<const f64x2*>(&rowbuf[b][jg + q]);
-      r[q] = rv[0];
-      r[q + 1] = rv[1];
-    }
-    double p = rowbuf[b][c];
-    const double cc = colbuf[b][i];
-    const bool okp = (p > 0.0) && isfinite(p);
-    if (!okp && bad == 0) bad = (int)(k0 + c + 1);
-    p = okp ? p : 1.0;  // keep the arithmetic finite; the caller raises on info
-    const double ip = 1.0 / p;
-    const double ci = cc * ip;
-    const bool prow = i == c;
+        for (int q = 0; q < 16; q += 2) {
+          const f64x2 rv = *reinterpret_cast<const f64x2*>(&rowbuf[b][jg + q]);
+          r[q] = rv[0];
+          r[q + 1] = rv[1];
+        }
+        const double p = colbuf[b][c];
+        const bool okp = (p > 0.0) && isfinite(p);
+        if (!okp && bad == 0) bad = (int)(k0 + c + 1);
+        const double ip = 1.0 / (okp ? p : 1.0);  // keep the arithmetic finite; the caller raises on info
+        const double x = (i == c) ? 0.0 : colbuf[b][i] * ip;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const bool pcol = (jg + q) == c;
-      const double gen = pcol ? ci : v[q] - ci * r[q];
-      const double rowv = pcol ? -ip : r[q] * ip;
-      v[q] = prow ? rowv : gen;
+        for (int q = 0; q < 16; ++q) v[q] = fma(-x, r[q], v[q]);
+        v[u] = (own && i == c) ? -v[u] : v[u];
+        if (t == 0) dvec[c] = ip;
+      }
     }
   }
+  __syncthreads();
+  const double di = i < nb ? dvec[i] : 0.0;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int j = jg + q;
-    Pi[i * SPD_NB + j] = (i < nb && j < nb) ? -v[q] : 0.0;  // sweeps gave -P^-1
+    Pi[i * SPD_NB + j] = (i < nb && j < nb) ? -v[q] * di * dvec[j] : 0.0;  // sweeps gave -P^-1
   }
   if (t == 0 && bad != 0) atomicCAS(info, 0, bad);
 }

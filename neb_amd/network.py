@@ -309,6 +309,12 @@ class Fields:
     stiff: torch.Tensor   # [n*cs, ldK]; 2D omega, 3D grad u (rows 3a+j)
     K: int
     layout: int = 0
+    mass_lo: torch.Tensor | None = None   # split fields: mass/stiff hold the bf16 hi
+    stiff_lo: torch.Tensor | None = None  # planes, these the lo planes
+
+    @property
+    def split(self) -> bool:
+        return self.mass_lo is not None
 
     @property
     def U(self):
@@ -319,15 +325,28 @@ class Fields:
         return self.stiff[:, :self.K]
 
 
-def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0) -> Fields:
+def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0, split: bool = False) -> Fields:
     """Output layer fused with the lift: the field matrices the Gram and the
-    spectral objective consume (VCORE:71-77 ``fields``; ELC:35-38)."""
+    spectral objective consume (VCORE:71-77 ``fields``; ELC:35-38).
+
+    split=True (bf16 tapes only) writes each fp32 field value x as two bf16
+    planes hi = bf16(x), lo = bf16(x - hi): the operand of the split Gram
+    (gram(..., lo=...)), and hi alone is the bf16 operand of the reverse."""
     cfg = net.cfg
     prec = tape.prec
     stg = net.staged(prec)
     Y = tape.inputs[-1]
     n, k, K = tape.n, cfg.widths[-1], cfg.modes
     cm, cs = (2, 1) if cfg.dim == 2 else (3, 9)
+    if split:
+        if prec != H.BF16:
+            raise ValueError("split fields need a bf16 tape")
+        planes = [torch.empty((r * n, _ld(K)), dtype=torch.bfloat16, device=net.device)
+                  for r in (cm, cm, cs, cs)]
+        H.call("neb_jet_output_fields_split", cfg.dim, n, k, K, Y.data_ptr(), Y.shape[2],
+               stg["W"][-1].data_ptr(), stg["W"][-1].shape[1], *[t.data_ptr() for t in planes],
+               planes[0].shape[1], layout, _stream())
+        return Fields(planes[0], planes[2], K, layout, planes[1], planes[3])
     ft = _FIELD[prec]
     mass = torch.empty((cm * n, _ld(K)), dtype=ft, device=net.device)
     stiff = torch.empty((cs * n, _ld(K)), dtype=ft, device=net.device)
@@ -340,9 +359,12 @@ def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0) -> Fields:
 # Gram
 # ---------------------------------------------------------------------------
 def gram(phi: torch.Tensor, K: int, w: float, mode: str = "bf16x3", out: torch.Tensor | None = None,
-         beta: float = 0.0, roww: torch.Tensor | None = None) -> torch.Tensor:
+         beta: float = 0.0, roww: torch.Tensor | None = None, lo: torch.Tensor | None = None) -> torch.Tensor:
     """G = beta*G + w * Phi[:, :K]^T diag(roww) Phi[:, :K]  in FP64
-    (VCORE:40-46 ``grams``; experiments/metaball/run.py:99-116)."""
+    (VCORE:40-46 ``grams``; experiments/metaball/run.py:99-116).
+
+    lo: the lo plane of split fields (phi is then the hi plane); only the
+    "bf16x3" mode without row weights applies to split operands."""
     if phi.ndim != 2 or phi.stride(1) != 1:
         raise ValueError("Phi must be a row-major 2-D tensor")
     if out is None:
@@ -352,10 +374,32 @@ def gram(phi: torch.Tensor, K: int, w: float, mode: str = "bf16x3", out: torch.T
     rows = phi.shape[0]
     need = H.load().neb_cross_workspace_size(K, K, rows, 1)
     ws = workspace(phi.device).get(need)
+    if lo is not None:
+        if mode != "bf16x3" or roww is not None:
+            raise ValueError("split operands take the bf16x3 mode without row weights")
+        if lo.shape != phi.shape or lo.stride() != phi.stride() or phi.dtype != torch.bfloat16 \
+                or lo.dtype != torch.bfloat16:
+            raise ValueError("hi and lo planes must be bf16 tensors of one shape and stride")
+        H.call("neb_gram_split", phi.data_ptr(), lo.data_ptr(), rows, K, phi.stride(0), float(w), float(beta),
+               out.data_ptr(), out.stride(0), ws.data_ptr(), ws.numel(), _stream())
+        return out
     rw = 0 if roww is None else roww.to(torch.float64).contiguous().data_ptr()
     H.call("neb_gram", m, phi.data_ptr(), rows, K, phi.stride(0), _DT[phi.dtype], float(w), rw, float(beta),
            out.data_ptr(), out.stride(0), ws.data_ptr(), ws.numel(), _stream())
     return out
+
+
+def split_fields_for(prec: str, gram_mode: str) -> bool:
+    """Whether output_fields should write split planes: the bf16 tape with
+    the split-precision Gram (results are bit-identical to fp32 fields)."""
+    return prec == "bf16" and gram_mode == "bf16x3"
+
+
+def field_gram(fields: Fields, which: str, w: float, mode: str = "bf16x3", out: torch.Tensor | None = None,
+               beta: float = 0.0) -> torch.Tensor:
+    """FP64 Gram of the mass ("mass") or stiffness ("stiff") fields, split or not."""
+    phi, lo = (fields.mass, fields.mass_lo) if which == "mass" else (fields.stiff, fields.stiff_lo)
+    return gram(phi, fields.K, w, mode, out=out, beta=beta, lo=lo)
 
 
 def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, beta: float = 0.0):
@@ -418,8 +462,8 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
     Z = tape.pre[-1]
     H.call("neb_jet_output_backward", prec, d, cfg.act_code, n, width, K, fields.mass.data_ptr(),
-           fields.stiff.data_ptr(), ldK, fields.layout, bm.data_ptr(), bs.data_ptr(), bm.shape[1], Z.data_ptr(),
-           Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
+           fields.stiff.data_ptr(), ldK, _DT[fields.mass.dtype], fields.layout, bm.data_ptr(), bs.data_ptr(),
+           bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
 
     if grads is None:
         grads = torch.empty_like(net.params)

@@ -28,11 +28,9 @@ def _sym(A):
     return 0.5 * (A + A.T)
 
 
-def spd_inverse(M: torch.Tensor) -> torch.Tensor:
-    """FP64 inverse of a symmetric positive definite matrix on the GPU
-    (libnebhip blocked Gauss-Jordan sweeps).  Raises FloatingPointError when
-    a pivot is not positive, i.e. the matrix is not positive definite (the
-    gate of MB/velocity_trace_jet2.py:109-111); no update is made then."""
+def _spd_inverse_async(M: torch.Tensor):
+    """Enqueue the FP64 SPD inverse; returns (X, info) without synchronising
+    (info[0] != 0 names the first nonpositive pivot, 1-based)."""
     K = M.shape[0]
     A = M.to(torch.float64).contiguous()
     X = torch.empty_like(A)
@@ -41,9 +39,22 @@ def spd_inverse(M: torch.Tensor) -> torch.Tensor:
     ws = nw.workspace(A.device).get(need)
     nw.H.call("neb_spd_inverse", K, A.data_ptr(), A.stride(0), X.data_ptr(), X.stride(0), info.data_ptr(),
               ws.data_ptr(), ws.numel(), nw._stream())
+    return X, info
+
+
+def _raise_not_pd(bad: int):
+    raise FloatingPointError(f"mass matrix is not positive definite (pivot {bad}); no update")
+
+
+def spd_inverse(M: torch.Tensor) -> torch.Tensor:
+    """FP64 inverse of a symmetric positive definite matrix on the GPU
+    (libnebhip blocked Gauss-Jordan sweeps).  Raises FloatingPointError when
+    a pivot is not positive, i.e. the matrix is not positive definite (the
+    gate of MB/velocity_trace_jet2.py:109-111); no update is made then."""
+    X, info = _spd_inverse_async(M)
     bad = int(info.item())
     if bad != 0:
-        raise FloatingPointError(f"mass matrix is not positive definite (pivot {bad}); no update")
+        _raise_not_pd(bad)
     return _sym(X)
 
 
@@ -83,16 +94,20 @@ class SpectralSolver:
     previous step's inverse X and refine it with Newton-Schulz iterations
     X <- X (2I - M_r X) (two FP64 GEMMs each, quadratic convergence), which
     is valid while the residual ||I - M_r X||_F < 1; the iteration stops at a
-    relative residual <= tol, else the direct inverse runs.  Either way the
+    relative residual <= tol, else the direct inverse runs and the next
+    `backoff` steps go straight to it (a rank-deficient mass held up by a
+    ridge changes too much per step for the warm start to converge).  Either way the
     returned inverse satisfies ||I - M_r X||_F <= tol, and a condition
     estimate ||M_r||_F ||X||_F above cond_limit raises FloatingPointError
     (the gate of MB/velocity_trace_jet2.py:109-111)."""
 
     def __init__(self, lam: float = 1e-2, ridge: float = 0.0, cond_limit: float = 1e12, tol: float = 1e-10,
-                 max_iter: int = 6):
+                 max_iter: int = 6, backoff: int = 16):
         self.lam, self.ridge, self.cond_limit, self.tol, self.max_iter = lam, ridge, cond_limit, tol, max_iter
+        self.backoff = backoff
         self.X = None
         self.last_path = None
+        self._skip = 0       # warm starts left to skip after one that failed
 
     def _residual(self, Mr, X, eye):
         return float(torch.linalg.norm(eye - Mr @ X))
@@ -102,7 +117,9 @@ class SpectralSolver:
         eye = torch.eye(K, dtype=torch.float64, device=M.device)
         Mr = _sym(M) + self.ridge * eye
         X = None
-        if self.X is not None and self.X.shape == Mr.shape:
+        if self._skip > 0:
+            self._skip -= 1
+        elif self.X is not None and self.X.shape == Mr.shape:
             X = self.X
             ok = False
             for _ in range(self.max_iter):
@@ -118,11 +135,20 @@ class SpectralSolver:
                 self.last_path = "newton-schulz"
             else:
                 X = None
+                self._skip = self.backoff
+        info = None
         if X is None:
-            X = spd_inverse(Mr)
+            X, info = _spd_inverse_async(Mr)
             self.last_path = "spd-inverse"
         X = _sym(X)
-        cond = float(torch.linalg.norm(Mr) * torch.linalg.norm(X))
+        # one host synchronisation for both gates
+        cond_t = (torch.linalg.norm(Mr) * torch.linalg.norm(X)).reshape(1)
+        if info is not None:
+            bad, cond = torch.cat((info.to(torch.float64), cond_t)).tolist()
+            if bad != 0:
+                _raise_not_pd(int(bad))
+        else:
+            cond = float(cond_t)
         if not math.isfinite(cond) or cond > self.cond_limit:
             raise FloatingPointError("mass condition estimate exceeds limit; no update")
         self.X = X
@@ -172,12 +198,13 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     p = nw.PRECISIONS[prec]
     tt = {} if timings is None else timings
     K = net.cfg.modes
+    split = nw.split_fields_for(prec, gram_mode)
     with timed(tt, "neural_forward_analytic_derivatives"):
         tape = nw.trunk_forward(net, points, p)
-        fields = nw.output_fields(net, tape)
+        fields = nw.output_fields(net, tape, split=split)
     with timed(tt, "gram_assembly"):
-        M = nw.gram(fields.mass, K, weight, gram_mode)
-        S = nw.gram(fields.stiff, K, weight, gram_mode)
+        M = nw.field_gram(fields, "mass", weight, gram_mode)
+        S = nw.field_gram(fields, "stiff", weight, gram_mode)
     with timed(tt, "spectral_objective"):
         loss, dM, dS = spectral_objective(M, S, lam, ridge, gate=gate)
     out = dict(loss=loss, M=M, S=S, fields=fields, timings=tt)
@@ -201,15 +228,16 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
     K = net.cfg.modes
     n = points.shape[0]
     solver = solver if solver is not None else SpectralSolver()
+    split = nw.split_fields_for(prec, gram_mode)
     M = torch.zeros((K, K), dtype=torch.float64, device=net.device)
     S = torch.zeros_like(M)
     with timed(tt, "pass1_forward_and_grams"):
         for c0 in range(0, n, chunk):
             tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
-            f = nw.output_fields(net, tape)
+            f = nw.output_fields(net, tape, split=split)
             del tape
-            nw.gram(f.mass, K, weight, gram_mode, out=M, beta=1.0)
-            nw.gram(f.stiff, K, weight, gram_mode, out=S, beta=1.0)
+            nw.field_gram(f, "mass", weight, gram_mode, out=M, beta=1.0)
+            nw.field_gram(f, "stiff", weight, gram_mode, out=S, beta=1.0)
             del f
     with timed(tt, "spectral_objective"):
         loss, dM, dS = solver(M, S)
@@ -219,21 +247,22 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
         with timed(tt, "pass2_recompute_and_reverse"):
             for c0 in range(0, n, chunk):
                 tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
-                f = nw.output_fields(net, tape)
+                f = nw.output_fields(net, tape, split=split)
                 grads += nw.backward_fields(net, tape, f, dM, dS, weight)
                 del tape, f
         out["grads"] = grads
     return out
 
 
-def basis_stream(net: nw.EigenfluidNet, points: torch.Tensor, prec: str = "bf16", chunk: int = 131072):
+def basis_stream(net: nw.EigenfluidNet, points: torch.Tensor, prec: str = "bf16", chunk: int = 131072,
+                 split: bool = False):
     """Yield (first point, Fields) chunk by chunk: the full-grid basis
     evaluation of experiments/metaball/evaluate.py:130-190 without the disk
     round trip."""
     p = nw.PRECISIONS[prec]
     for c0 in range(0, points.shape[0], chunk):
         tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
-        f = nw.output_fields(net, tape)
+        f = nw.output_fields(net, tape, split=split)
         del tape
         yield c0, f
 
@@ -244,8 +273,8 @@ def fullmass(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: s
     experiments/metaball/run.py:99-116 (ValueError when rank deficient)."""
     K = net.cfg.modes
     M = torch.zeros((K, K), dtype=torch.float64, device=net.device)
-    for _, f in basis_stream(net, points, prec, chunk):
-        nw.gram(f.mass, K, weight, gram_mode, out=M, beta=1.0)
+    for _, f in basis_stream(net, points, prec, chunk, split=nw.split_fields_for(prec, gram_mode)):
+        nw.field_gram(f, "mass", weight, gram_mode, out=M, beta=1.0)
     ev = torch.linalg.eigvalsh(_sym(M))
     if float(ev[0]) <= rank_tol * float(ev[-1]):
         raise ValueError("Fullmass basis is rank deficient")
@@ -303,10 +332,9 @@ class JetGram(torch.autograd.Function):
     def forward(ctx, params, points, net, weight, prec, gram_mode):
         p = nw.PRECISIONS[prec]
         tape = nw.trunk_forward(net, points, p)
-        fields = nw.output_fields(net, tape)
-        K = net.cfg.modes
-        M = nw.gram(fields.mass, K, weight, gram_mode)
-        S = nw.gram(fields.stiff, K, weight, gram_mode)
+        fields = nw.output_fields(net, tape, split=nw.split_fields_for(prec, gram_mode))
+        M = nw.field_gram(fields, "mass", weight, gram_mode)
+        S = nw.field_gram(fields, "stiff", weight, gram_mode)
         ctx.state = (net, tape, fields, weight)
         return M, S
 

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared_symbols():
         assert hasattr(lib, name), name
     L = H.load()
-    assert L.neb_abi_version() == 1
+    assert L.neb_abi_version() == H.ABI_VERSION
 
 
 def test_argument_errors_raise_without_touching_the_gpu():
@@ -43,4 +43,13 @@ def test_argument_errors_raise_without_touching_the_gpu():
     with pytest.raises(ValueError):
         H.call("neb_cross", 0, 0, None, None, None, None, None, None, 0, 0, 4, 4, None, 0, 1.0, 0.0, 16, 4, 16,
                0, None)
+    # split fields: bad dimension, split Gram: null lo plane, reverse: fields
+    # dtype that does not fit the precision
+    with pytest.raises(ValueError):
+        H.call("neb_jet_output_fields_split", 4, 8, 8, 8, 16, 8, 16, 8, 16, 16, 16, 16, 8, 0, None)
+    with pytest.raises(ValueError):
+        H.call("neb_gram_split", 16, None, 8, 8, 8, 1.0, 0.0, 16, 8, 16, 1 << 20, None)
+    with pytest.raises(ValueError):
+        H.call("neb_jet_output_backward", H.F32, 2, 0, 8, 8, 8, 16, 16, 8, H.BF16, 0, 16, 16, 8, 16, 8, 1.0, 16, 8,
+               None)
     assert H.load().neb_cross_workspace_size(1024, 1024, 131072, 1) > 0

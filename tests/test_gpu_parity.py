@@ -121,6 +121,68 @@ def test_output_fields_lift(prec, dim):
     assert np.array_equal(perm, f.U.cpu().numpy())
 
 
+@pytest.mark.parametrize("dim,layout", [(2, 0), (2, 1), (3, 0)])
+def test_split_fields_are_the_rounded_fp32_fields(dim, layout):
+    """Split planes: hi = bf16(x), lo = bf16(x - hi) of the fp32 field x,
+    bit for bit, including ragged K (not a multiple of the 64-feature tile)."""
+    n, F = 333, 2
+    K = 100 if dim == 2 else 37
+    cfg, net, _ = make(dim, K, [32, 32], F)
+    tape = nw.trunk_forward(net, torch.tensor(points(dim, n, seed=5)), nw.PRECISIONS["bf16"])
+    f = nw.output_fields(net, tape, layout=layout)
+    sf = nw.output_fields(net, tape, layout=layout, split=True)
+    assert sf.split and not f.split
+    for x, hi, lo in ((f.U, sf.U, sf.mass_lo[:, :K]), (f.D, sf.D, sf.stiff_lo[:, :K])):
+        h = x.to(torch.bfloat16)
+        assert torch.equal(hi, h)
+        assert torch.equal(lo, (x - h.float()).to(torch.bfloat16))
+
+
+@pytest.mark.parametrize("R,K", [(1000, 40), (4099, 130), (20000, 256), (2 * 4096 + 7, 1024)])
+def test_split_gram_is_bit_identical(R, K):
+    """The Gram over pre-split planes equals the split-on-load bf16x3 Gram of
+    the fp32 matrix bit for bit (same operands, same order of sums), and so
+    inherits its error bound against FP64."""
+    rng = np.random.default_rng(R + K)
+    phi = rng.normal(size=(R, K)) * np.exp(rng.normal(size=K))[None, :]
+    ld = nw._ld(K)
+    t = torch.zeros((R, ld), dtype=torch.float32, device=DEV)
+    t[:, :K] = torch.tensor(phi, dtype=torch.float32)
+    hi = t.to(torch.bfloat16)
+    lo = (t - hi.float()).to(torch.bfloat16)
+    G1 = nw.gram(t, K, 0.37, "bf16x3")
+    G2 = nw.gram(hi, K, 0.37, "bf16x3", lo=lo)
+    assert torch.equal(G1, G2)
+    r = rel(G2.cpu(), o.gram(t[:, :K].double().cpu().numpy(), 0.37)); assert r < 1e-5, r
+    G3 = nw.gram(hi, K, 1.0, "bf16x3", lo=lo, out=G2.clone(), beta=2.0)
+    r = rel(G3.cpu(), (2.0 + 1.0 / 0.37) * G2.cpu()); assert r < 1e-12, r
+
+
+def test_split_gram_rejects_other_modes():
+    t = torch.zeros((64, 8), dtype=torch.bfloat16, device=DEV)
+    with pytest.raises(ValueError):
+        nw.gram(t, 8, 1.0, "f32", lo=t)
+    with pytest.raises(ValueError):
+        nw.gram(t, 8, 1.0, "bf16x3", lo=t, roww=torch.ones(64, device=DEV))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reverse_with_split_fields_is_bit_identical(dim):
+    """The reverse pass reads the hi plane where it rounded the fp32 fields
+    to bf16: the gradients agree bit for bit."""
+    F = 2
+    K = 40 if dim == 2 else 12
+    cfg, net, _ = make(dim, K, [48, 32], F, master=torch.float32)
+    x = torch.tensor(points(dim, 500, seed=8))
+    tape = nw.trunk_forward(net, x, nw.PRECISIONS["bf16"])
+    rng = np.random.default_rng(2)
+    dM = rng.normal(size=(K, K)); dM = torch.tensor(dM + dM.T, device=DEV)
+    dS = rng.normal(size=(K, K)); dS = torch.tensor(dS + dS.T, device=DEV)
+    g1 = nw.backward_fields(net, tape, nw.output_fields(net, tape), dM, dS, 1e-3)
+    g2 = nw.backward_fields(net, tape, nw.output_fields(net, tape, split=True), dM, dS, 1e-3)
+    assert torch.equal(g1, g2)
+
+
 @pytest.mark.parametrize("mode,tol", [("f64", 1e-13), ("f32", 2e-6), ("bf16x3", 1e-5), ("bf16", 1e-2)])
 @pytest.mark.parametrize("R,K", [(1000, 40), (4099, 130), (20000, 256)])
 def test_gram_modes_against_fp64(mode, tol, R, K):
@@ -401,6 +463,30 @@ def test_spectral_solver_paths_agree():
     assert abs(float(L1) - float(L2)) < 1e-10 * abs(float(L2))
     r = rel(dM1.cpu(), dM2.cpu()); assert r < 1e-9, r
     r = rel(dS1.cpu(), dS2.cpu()); assert r < 1e-9, r
+
+
+def test_spectral_solver_backoff_after_failed_warm_start():
+    """A warm start that does not converge falls back to the direct inverse
+    and the next `backoff` calls skip the warm start; results are unchanged."""
+    rng = np.random.default_rng(1)
+    K = 48
+    A = rng.normal(size=(300, K)); M0 = A.T @ A / 300
+    B = rng.normal(size=(200, K)); S = B.T @ B / 200
+    Mt, St = torch.tensor(M0, device=DEV), torch.tensor(S, device=DEV)
+    solver = nb.SpectralSolver(1e-2, backoff=2)
+    solver(Mt, St)
+    far = 3.0 * Mt                                   # residual ||I - 3M X|| >> 1/2
+    L1, dM1, _ = solver(far, St)
+    assert solver.last_path == "spd-inverse" and solver._skip == 2
+    near = far + 1e-6 * torch.eye(K, dtype=torch.float64, device=DEV)
+    for left in (1, 0):
+        L, dM, _ = solver(near, St)
+        assert solver.last_path == "spd-inverse" and solver._skip == left
+    L, dM, _ = solver(near, St)
+    assert solver.last_path == "newton-schulz"
+    L2, dM2, _ = nb.spectral_objective(near, St, 1e-2, gate="cholesky")
+    assert abs(float(L) - float(L2)) < 1e-10 * abs(float(L2))
+    r = rel(dM.cpu(), dM2.cpu()); assert r < 1e-9, r
 
 
 @pytest.mark.parametrize("K", [5, 64, 100, 300, 1024])
