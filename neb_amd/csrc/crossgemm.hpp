@@ -63,7 +63,7 @@ template <> struct CrossPol<CR_F32> {
 };
 template <> struct CrossPol<CR_BF16> {
   using S = __bf16; using frag_t = bf16x8; using acc_t = f32x4; using outer_t = float;
-  static constexpr int KI = 32, KE = 8, RK = 32, PAD = 8, NPROD = 1, FLUSH = 1 << 30;
+  static constexpr int KI = 32, KE = 8, RK = 32, PAD = 0, NPROD = 1, FLUSH = 1 << 30;
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
@@ -87,11 +87,12 @@ template <typename T> __device__ inline float toflt(T v) { return (float)v; }
 template <> __device__ inline float toflt<__bf16>(__bf16 v) { return (float)v; }
 
 // Transposing tile loader: rows [r0, r0+RK) x cols [c0, c0+TM) of a
-// row-major source -> LDS hi[col][row] (and lo[col][row] for the split mode).
-// Thread t owns column t % TM and RK/2 consecutive rows, so each global load
-// instruction reads TM consecutive elements of one row.  fetch() fills
-// registers (issued one stage ahead of the MFMAs); commit() converts / splits
-// and writes LDS with 16-byte stores.
+// row-major source -> LDS hi[col][row] (and lo[col][row] for the split
+// mode).  Thread t owns column t % TM and RK/2 consecutive rows, so each
+// global load instruction reads TM consecutive elements of one row.  fetch()
+// fills registers (issued one stage ahead of the MFMAs); commit() converts /
+// splits and writes LDS with 16-byte stores, 8 consecutive lanes writing one
+// chunk of 8 consecutive tile rows (bank-conflict free under lds_k<S, 1>).
 template <int MODE, typename TIN>
 struct CrossLoad {
   using Pol = CrossPol<MODE>;
@@ -139,13 +140,14 @@ struct CrossLoad {
         if (gr < rows_) x[e] = (V)((double)x[e] * roww[gr]);
       }
     }
-    S* dh = hi + col * LDK + rh;
+    S* dh = hi + col * LDK;
     if constexpr (MODE == CR_F64) {
 #pragma unroll
-      for (int e = 0; e < HR; e += 2) *reinterpret_cast<f64x2*>(dh + e) = f64x2{x[e], x[e + 1]};
+      for (int e = 0; e < HR; e += 2) *reinterpret_cast<f64x2*>(dh + rh + e) = f64x2{x[e], x[e + 1]};
     } else if constexpr (MODE == CR_F32) {
 #pragma unroll
-      for (int e = 0; e < HR; e += 4) *reinterpret_cast<f32x4*>(dh + e) = f32x4{x[e], x[e + 1], x[e + 2], x[e + 3]};
+      for (int e = 0; e < HR; e += 4)
+        *reinterpret_cast<f32x4*>(dh + rh + e) = f32x4{x[e], x[e + 1], x[e + 2], x[e + 3]};
     } else {
 #pragma unroll
       for (int e = 0; e < HR; e += 8) {
@@ -156,8 +158,9 @@ struct CrossLoad {
           h[q] = hq;
           if constexpr (MODE == CR_BF16X3) l[q] = (__bf16)(x[e + q] - (float)hq);
         }
-        *reinterpret_cast<bf16x8*>(dh + e) = h;
-        if constexpr (MODE == CR_BF16X3) *reinterpret_cast<bf16x8*>(lo + col * LDK + rh + e) = l;
+        const int pos = col * LDK + lds_k<S, 1>(col, rh + e);
+        *reinterpret_cast<bf16x8*>(hi + pos) = h;
+        if constexpr (MODE == CR_BF16X3) *reinterpret_cast<bf16x8*>(lo + pos) = l;
       }
     }
   }
@@ -166,31 +169,33 @@ struct CrossLoad {
 // Loader of pre-split operands: thread t owns the column pair 2*(t%64),
 // +1 and rows 8*(t/64) .. +7 of the k-step, so each global load is one
 // 4-byte pair per lane and one 256-byte row segment per wave.  commit()
-// transposes the pairs into two 8-row column vectors per plane (16-bit
-// lane shuffles only) and applies a -1 sign by flipping sign bits, which
-// is exact for both planes.
+// transposes the pairs into two 8-row column vectors per plane (byte
+// permutes only) and writes them in an order that keeps every 8 lanes on 4
+// even and 4 odd tile rows (lanes with (t/4) odd write their odd column
+// first), which is bank-conflict free under lds_k<S, 1>.  A -1 sign flips
+// sign bits, exact in both planes.
 template <>
 struct CrossLoad<CR_BF16X3, SplitBF16> {
   using S = __bf16;
   using V = float;
   static constexpr int RK = CrossPol<CR_BF16X3>::RK, LDK = RK + CrossPol<CR_BF16X3>::PAD, RG = 8;
-  static_assert(RK == 4 * RG, "four waves cover the k-step");
+  static_assert(RK == 4 * RG && CR_TM == 128, "four 8-row chunks x 64 column pairs");
   uint32_t h[RG], l[RG];
   uint32_t cmask;  // columns of the pair inside the matrix (per 16-bit half)
   int nrow;        // rows of this thread's 8 inside the segment
 
-  // Every load is unconditional from an in-bounds address (rows past the
-  // end re-read the last row, a pair past the last column re-reads column
-  // 0; a pair straddling the last column stays inside the padded row, whose
-  // length is even); commit() zeroes what lies outside.  No branch sits
-  // between the loads, so they all stay in flight together.
+  // Interior tiles load from a uniform base with 32-bit offsets.  Elsewhere
+  // every load is still unconditional from an in-bounds address (rows past
+  // the end re-read the last row, a pair past the last column re-reads
+  // column 0; a pair straddling the last column stays inside the padded row,
+  // whose length is even) and commit() zeroes what lies outside, so no
+  // branch sits between the loads.
   __device__ inline void fetch(const SplitBF16* src, const void* src_lo, int64_t ld, int64_t r0, int64_t rows,
                                int64_t c0, int64_t cols) {
     const __bf16* H = reinterpret_cast<const __bf16*>(src);
     const __bf16* L = reinterpret_cast<const __bf16*>(src_lo);
     const int cp = threadIdx.x & 63, rg = (threadIdx.x >> 6) * RG;
     if (r0 + RK <= rows && c0 + CR_TM <= cols) {
-      // interior tile (workgroup-uniform test): uniform bases, 32-bit offsets
       const int64_t b = r0 * ld + c0;
       const int ld32 = (int)ld;
       cmask = 0xffffffffu;
@@ -222,22 +227,27 @@ struct CrossLoad<CR_BF16X3, SplitBF16> {
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const int cp = threadIdx.x & 63, rg = (threadIdx.x >> 6) * RG;
     const uint32_t flip = scale < 0 ? 0x80008000u : 0u;
-    u32x4 h0, h1, l0, l1;
+    const bool oddf = (cp >> 2) & 1;  // write the odd column first
+    // byte selectors of v_perm over {b : a}: low halves (column 2p) are
+    // bytes 0,1,4,5; high halves (column 2p+1) bytes 2,3,6,7
+    const uint32_t sa = oddf ? 0x07060302u : 0x05040100u, sb = oddf ? 0x05040100u : 0x07060302u;
+    u32x4 ha, hb, la, lb;
 #pragma unroll
-    for (int q = 0; q < RG / 2; ++q) {
-      const uint32_t ma = 2 * q < nrow ? cmask : 0u, mb = 2 * q + 1 < nrow ? cmask : 0u;
-      const uint32_t a = (h[2 * q] ^ flip) & ma, b = (h[2 * q + 1] ^ flip) & mb;
-      h0[q] = (a & 0xffffu) | (b << 16);
-      h1[q] = (a >> 16) | (b & 0xffff0000u);
-      const uint32_t c = (l[2 * q] ^ flip) & ma, d = (l[2 * q + 1] ^ flip) & mb;
-      l0[q] = (c & 0xffffu) | (d << 16);
-      l1[q] = (c >> 16) | (d & 0xffff0000u);
+    for (int k = 0; k < RG / 2; ++k) {
+      const uint32_t ma = 2 * k < nrow ? cmask : 0u, mb = 2 * k + 1 < nrow ? cmask : 0u;
+      const uint32_t a = (h[2 * k] ^ flip) & ma, b = (h[2 * k + 1] ^ flip) & mb;
+      ha[k] = __builtin_amdgcn_perm(b, a, sa);
+      hb[k] = __builtin_amdgcn_perm(b, a, sb);
+      const uint32_t c = (l[2 * k] ^ flip) & ma, d = (l[2 * k + 1] ^ flip) & mb;
+      la[k] = __builtin_amdgcn_perm(d, c, sa);
+      lb[k] = __builtin_amdgcn_perm(d, c, sb);
     }
-    const int o = 2 * cp * LDK + rg;
-    *reinterpret_cast<u32x4*>(hi + o) = h0;
-    *reinterpret_cast<u32x4*>(hi + o + LDK) = h1;
-    *reinterpret_cast<u32x4*>(lo + o) = l0;
-    *reinterpret_cast<u32x4*>(lo + o + LDK) = l1;
+    const int ca = 2 * cp + (oddf ? 1 : 0), cb = 2 * cp + (oddf ? 0 : 1);
+    const int pa = ca * LDK + lds_k<S, 1>(ca, rg), pb = cb * LDK + lds_k<S, 1>(cb, rg);
+    *reinterpret_cast<u32x4*>(hi + pa) = ha;
+    *reinterpret_cast<u32x4*>(lo + pa) = la;
+    *reinterpret_cast<u32x4*>(hi + pb) = hb;
+    *reinterpret_cast<u32x4*>(lo + pb) = lb;
   }
 };
 
@@ -292,6 +302,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
   const int fr = lane & 15, kg = (lane >> 4) * KE;
+  // fragment rows are 16m + fr, so their chunk swizzle depends on fr only
+  // (for bf16 KI == RK: kk is 0 and the swizzle is exact; fp32/fp64 k is unswizzled)
+  const int koff = lds_k<S, 1>(fr, kg);
 
   // Two-level accumulation (MFMA registers flushed into `outer` every FLUSH
   // rows) only where the fp32 rounding chain matters: the fp32 and split
@@ -356,14 +369,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
         frag_t yh[4], yl[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          yh[j] = *reinterpret_cast<const frag_t*>(ys + (wj + j * 16 + fr) * LDK + kk + kg);
-          if constexpr (MODE == CR_BF16X3) yl[j] = *reinterpret_cast<const frag_t*>(ys + TILE + (wj + j * 16 + fr) * LDK + kk + kg);
+          const int off = (wj + j * 16 + fr) * LDK + koff + kk;
+          yh[j] = *reinterpret_cast<const frag_t*>(ys + off);
+          if constexpr (MODE == CR_BF16X3) yl[j] = *reinterpret_cast<const frag_t*>(ys + TILE + off);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const frag_t xh = *reinterpret_cast<const frag_t*>(xs + (wi + i * 16 + fr) * LDK + kk + kg);
+          const int off = (wi + i * 16 + fr) * LDK + koff + kk;
+          const frag_t xh = *reinterpret_cast<const frag_t*>(xs + off);
           if constexpr (MODE == CR_BF16X3) {
-            const frag_t xl = *reinterpret_cast<const frag_t*>(xs + TILE + (wi + i * 16 + fr) * LDK + kk + kg);
+            const frag_t xl = *reinterpret_cast<const frag_t*>(xs + TILE + off);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
               inner[i][j] = Pol::mfma(xl, yh[j], inner[i][j]);

@@ -119,7 +119,7 @@ struct TileLoad {
         vout o;
 #pragma unroll
         for (int e = 0; e < VE; ++e) o[e] = (S)v[t][e];
-        *reinterpret_cast<vout*>(lds + (s * R + row) * LDK + col) = o;
+        *reinterpret_cast<vout*>(lds + (s * R + row) * LDK + lds_k<S, 2>(row, col)) = o;
       }
     }
   }
@@ -302,17 +302,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F
     for (int kk = 0; kk < RK; kk += KI) {
       frag_t wf[NW][FB], xf[CI][PB];
 #pragma unroll
-      for (int s = 0; s < NW; ++s)
+      for (int i = 0; i < FB; ++i) {
+        const int row = wn * FB * 16 + (GE == 8 ? 32 * (i >> 1) + 8 * rg + 4 * (i & 1) : 16 * i + 4 * rg) + rr;
+        const int off = row * LDK + lds_k<S, 2>(row, kk + kg);
 #pragma unroll
-        for (int i = 0; i < FB; ++i)
-          wf[s][i] = *reinterpret_cast<const frag_t*>(
-              ws + s * WT + (wn * FB * 16 + (GE == 8 ? 32 * (i >> 1) + 8 * rg + 4 * (i & 1) : 16 * i + 4 * rg) + rr) * LDK +
-              kk + kg);
+        for (int s = 0; s < NW; ++s) wf[s][i] = *reinterpret_cast<const frag_t*>(ws + s * WT + off);
+      }
 #pragma unroll
-      for (int c = 0; c < CI; ++c)
+      for (int j = 0; j < PB; ++j) {
+        const int row = wp * PB * 16 + j * 16 + fr;
+        const int off = row * LDK + lds_k<S, 2>(row, kk + kg);
 #pragma unroll
-        for (int j = 0; j < PB; ++j)
-          xf[c][j] = *reinterpret_cast<const frag_t*>(xs + c * XT + (wp * PB * 16 + j * 16 + fr) * LDK + kk + kg);
+        for (int c = 0; c < CI; ++c) xf[c][j] = *reinterpret_cast<const frag_t*>(xs + c * XT + off);
+      }
 #pragma unroll
       for (int t = 0; t < NT; ++t)
 #pragma unroll

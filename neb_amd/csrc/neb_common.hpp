@@ -36,7 +36,7 @@ template <> struct Prec<PREC_BF16> {
   static constexpr int KI = 32;  // k per MFMA
   static constexpr int KE = 8;   // k elements per lane per fragment
   static constexpr int RK = 32;  // k per LDS stage
-  static constexpr int PAD = 8;  // LDS row padding (elements)
+  static constexpr int PAD = 0;  // unpadded rows, chunk-swizzled (lds_k)
   __device__ static inline acc_t mfma(frag_t a, frag_t b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
   }
@@ -181,4 +181,22 @@ __device__ inline void act_jet_reverse(const M* u, const M* a, M* b) {
     b[1 + D + p] = f1 * ah;
   }
   b[0] = bv;
+}
+
+// LDS position of k-offset k (a multiple of 8 for bf16) inside tile row
+// `row`.  bf16 tiles hold RK = 32 k-values per row, four 16-byte chunks,
+// unpadded; chunk c of row r sits at slot c ^ ((r >> SH) & 3).  On gfx950 a
+// ds_read_b128 serves 16 lanes per cycle as the lane blocks {8a..8a+7,
+// 8b..8b+7} with (a, b) in {(0,3), (1,2), (4,7), (5,6)} over 64 banks, and a
+// ds_write_b128 serves 8 consecutive lanes over 32 banks (measured with
+// SQ_LDS_BANK_CONFLICT).  The row-GEMM (SH = 2: its loader writes 2 rows x
+// 4 chunks per 8 lanes, its weight fragments read rows 0-3, 8-11, 16-19,
+// 24-27) and the cross GEMM (SH = 1: its loader writes 8 consecutive rows x
+// one chunk per 8 lanes) are then bank-conflict free for both the MFMA
+// fragment reads and the loader writes, where padded rows made one or the
+// other 2-way.  fp32 / fp64 tiles keep padded rows (k unchanged).
+template <typename S, int SH>
+__device__ inline int lds_k(int row, int k) {
+  if constexpr (sizeof(S) == 2) return ((((k >> 3) ^ (row >> SH)) & 3) << 3) | (k & 7);
+  return k;
 }
