@@ -1,0 +1,299 @@
+// Split-R cross GEMM for operands stored in bf16 (NL = 1: one plane, the
+// CR_BF16 product; NL = 2: hi and lo planes, the CR_BF16X3 product
+// hi.hi + hi.lo + lo.hi), fed by asynchronous global -> LDS copies.
+//
+// Same contraction, partial-tile layout and reduction as cross_kernel
+// (crossgemm.hpp); what differs is the data path:
+//   * global_load_lds_dwordx4 moves 16 bytes per lane straight into LDS, so
+//     no VGPRs stage the tiles and no VALU converts them; a ring of NS LDS
+//     stages keeps NS-1 k-steps of loads in flight per workgroup;
+//   * tiles stay k-major in LDS (the row-major source layout, 32 rows x 128
+//     columns per plane) and the MFMA fragments (8 consecutive k per lane)
+//     are read with ds_read_b64_tr_b16: within each 16-lane group, lane i
+//     receives element i % 4 of the 8-byte blocks addressed by lanes
+//     i/4 + 4e (e = 0..3).  Pointing lane m at row 8kg + m/4 (+4 for the
+//     second read), columns c + 4 (m % 4) .. +3 hands lane i column c + i,
+//     rows 8kg .. 8kg+3 (then +4 .. +7): the bf16 16x16x32 MFMA operand.
+//   * 16-byte column units of k-row r sit at unit u ^ swz(r),
+//     swz(r) = 2 (r % 4) + 8 ((r / 8) % 2), which spreads the 32 units one
+//     transposing read touches over all 16 bank groups twice (the 512-byte
+//     minimum of two cycles).
+// WM x WN waves each own a (128/WM) x (128/WN) block of the 128 x 128 tile.
+// Rows past a segment's end are zeroed in LDS before use; columns past the
+// matrix (inside the padded rows) only reach output rows / columns that are
+// never stored.  Segment signs must be +-1 (applied by flipping the sign
+// bits of the X fragments); per-row weights are not supported here.
+#pragma once
+#include "crossgemm.hpp"
+
+// NL planes per operand, WM x WN waves over the 128 x 128 tile (each wave
+// (128 / WM) x (128 / WN)), NS ring slots.
+template <int NL, int WM, int WN, int NS_>
+struct AsyncPol {
+  static constexpr int NS = NS_;               // LDS ring slots
+  static constexpr int NWAVE = WM * WN;
+  static constexpr int BI = 128 / WM / 16, BJ = 128 / WN / 16;  // 16x16 blocks per wave
+  static constexpr int RK = 32;                // k-rows per stage
+  static constexpr int PLANE = RK * CR_TM;     // bf16 elements per plane tile
+  static constexpr int STAGE = 2 * NL * PLANE; // X planes then Y planes
+  static constexpr int LPW = 16 * NL / NWAVE;  // 1 KB copies per wave per stage
+  static constexpr int FLUSH = NL == 2 ? 256 : (1 << 30);
+  static_assert(16 * NL % NWAVE == 0, "copy pieces split evenly over the waves");
+};
+
+
+__device__ inline int async_swz(int kr) { return 2 * (kr & 3) + 8 * ((kr >> 3) & 1); }
+
+__device__ inline bf16x8 async_neg(bf16x8 v) {  // exact negation: flip the sign bits
+  typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+  u16x8 u = __builtin_bit_cast(u16x8, v);
+  u ^= (unsigned short)0x8000;
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+// s_waitcnt vmcnt(n) with expcnt / lgkmcnt left at their maxima (vmcnt is
+// split over bits 3:0 and 15:14 of the immediate)
+constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 15) | ((n >> 4) << 14); }
+// this wave waits for all but the last `later` stages of copies (LPW each)
+template <int LPW, int NS>
+__device__ inline void wait_copies(int later) {
+  static_assert(LPW * (NS - 2) <= 63, "vmcnt range");
+  if (later >= NS - 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(LPW * (NS - 2)));
+  else if (NS > 3 && later == 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(LPW));
+  else if (later == 0) __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // deeper rings: conservative
+}
+
+// k-step cursor over the (segment, row) sequence of one row split
+struct AsyncCursor {
+  int s;
+  int64_t r0, rend;
+  __device__ inline void seek(const CrossArgs& a, int split, int from) {
+    for (s = from; s < a.nseg; ++s) {
+      int64_t rb, re;
+      split_range<32>(a.seg[s].rows, a.splits, split, rb, re);
+      if (rb < re) { r0 = rb; rend = re; return; }
+    }
+  }
+  __device__ inline void next(const CrossArgs& a, int split) {
+    r0 += 32;
+    if (r0 >= rend) seek(a, split, s + 1);
+  }
+};
+
+template <int NL, int WM, int WN, int NS_>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(NL == 1 ? 4 : 2))) void
+cross_async_kernel(CrossArgs a) {
+  using Pol = AsyncPol<NL, WM, WN, NS_>;
+  constexpr int NS = Pol::NS, RK = Pol::RK, PLANE = Pol::PLANE, STAGE = Pol::STAGE, LPW = Pol::LPW;
+  constexpr int BI = Pol::BI, BJ = Pol::BJ, NT = 64 * Pol::NWAVE;
+  typedef short v4i16 __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(3))) v4i16 lds_v4;
+  typedef __attribute__((address_space(3))) void lds_void;
+  // one LDS object per ring slot: the compiler's wait insertion tracks the
+  // asynchronous copies per object, so a read of slot k waits for slot k's
+  // copies only and the copies into the other slots stay in flight
+  static_assert(NS >= 2 && NS <= 4, "the k-loop below is unrolled over a ring of 2..4 slots");
+  __shared__ __align__(16) __bf16 ring0[STAGE];
+  __shared__ __align__(16) __bf16 ring1[STAGE];
+  __shared__ __align__(16) __bf16 ring2[NS > 2 ? STAGE : 8];
+  __shared__ __align__(16) __bf16 ring3[NS > 3 ? STAGE : 8];
+
+  int split, tile;
+  {
+    const int L = blockIdx.x;
+    const int N = a.ntiles * a.splits;
+    const int u = (N % 8 == 0) ? (L & 7) * (N >> 3) + (L >> 3) : L;
+    split = u / a.ntiles;
+    tile = u % a.ntiles;
+  }
+  int ti, tj;
+  if (a.symmetric) {
+    int t = tile, rowlen = a.tiles_n;
+    ti = 0;
+    while (t >= rowlen) { t -= rowlen; ++ti; --rowlen; }
+    tj = ti + t;
+  } else {
+    ti = tile / a.tiles_n;
+    tj = tile % a.tiles_n;
+  }
+  const int64_t i0 = (int64_t)ti * CR_TM, j0 = (int64_t)tj * CR_TN;
+  // wave index made provably uniform, so the segment descriptors behind the
+  // per-wave copy pieces are read with scalar loads (a vector load would be
+  // counted by vmcnt, and waiting for it would also wait for every copy)
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = wave / WN, wn = wave % WN;  // this wave's (16 BI) x (16 BJ) block of the tile
+
+  // ---- load side: this wave's 1 KB pieces of a stage -----------------------
+  // piece q = wave * LPW + t: operand op = q / (8 NL), plane q / 8 % NL, rows
+  // 4 (q % 8) .. +3.  Lane p fills LDS unit p of the piece: k-row 4 (q % 8)
+  // + p / 16, unit p % 16, holding global column unit (p % 16) ^ swz(row).
+  const int prow = lane >> 4, punit = lane & 15;
+  auto issue = [&](const AsyncCursor& c, __bf16* slot) {
+    const CrossSeg& sg = a.seg[c.s];
+#pragma unroll
+    for (int t = 0; t < LPW; ++t) {
+      const int q = wave * LPW + t;
+      const int op = q / (8 * NL), pl = (q / 8) % NL, kb = q % 8;
+      const int kr = 4 * kb + prow;
+      const __bf16* src = reinterpret_cast<const __bf16*>(op == 0 ? (pl == 0 ? sg.x : sg.xlo) : (pl == 0 ? sg.y : sg.ylo));
+      const int64_t ld = op == 0 ? sg.ldx : sg.ldy;
+      const int64_t c0 = op == 0 ? i0 : j0;
+      int64_t gr = c.r0 + kr;
+      gr = gr < c.rend ? gr : c.rend - 1;
+      int64_t gc = c0 + 8 * (punit ^ async_swz(kr));
+      gc = gc + 8 <= ld ? gc : ld - 8;
+      __bf16* dst = slot + (op * NL + pl) * PLANE + kb * 4 * CR_TM;
+      __builtin_amdgcn_global_load_lds(src + gr * ld + gc, (lds_void*)dst, 16, 0, 0);
+    }
+  };
+
+  // ---- compute side: per-lane byte offsets of the transposing reads --------
+  // lane = 16 kg + m reads k-row 8 kg + m / 4 (+4 for the second half),
+  // columns cb + 4 (m % 4): unit cb / 8 + (m % 4) / 2, half (m % 2).
+  const int kg = lane >> 4, m = lane & 15;
+  const int kr0 = 8 * kg + (m >> 2);
+  const int sw = async_swz(kr0);  // equal for kr0 + 4
+  auto rd_off = [&](int cb) {
+    const int unit = (cb >> 3) + ((m & 3) >> 1);
+    return kr0 * (CR_TM * 2) + 16 * (unit ^ sw) + 8 * (m & 1);  // bytes within a plane tile
+  };
+  int xoff[BI], yoff[BJ];
+#pragma unroll
+  for (int i = 0; i < BI; ++i) xoff[i] = rd_off(wm * 16 * BI + 16 * i);
+#pragma unroll
+  for (int j = 0; j < BJ; ++j) yoff[j] = rd_off(wn * 16 * BJ + 16 * j);
+
+  auto frag = [&](const __bf16* slot, int byte_off) -> bf16x8 {
+    typedef __attribute__((address_space(3))) unsigned char lds_u8;
+    lds_v4* p0 = (lds_v4*)((lds_u8*)(slot) + byte_off);
+    lds_v4* p1 = (lds_v4*)((lds_u8*)(slot) + byte_off + 4 * CR_TM * 2);
+    const v4i16 lo4 = __builtin_amdgcn_ds_read_tr16_b64_v4i16(p0);
+    const v4i16 hi4 = __builtin_amdgcn_ds_read_tr16_b64_v4i16(p1);
+    typedef short v8i16 __attribute__((ext_vector_type(8)));
+    const v8i16 v = {lo4[0], lo4[1], lo4[2], lo4[3], hi4[0], hi4[1], hi4[2], hi4[3]};
+    return __builtin_bit_cast(bf16x8, v);
+  };
+
+  constexpr bool TWO = Pol::FLUSH < (1 << 20);
+  f32x4 inner[BI][BJ];
+  float outer[TWO ? BI : 1][TWO ? BJ : 1][TWO ? 4 : 1];
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) inner[i][j] = f32x4{0, 0, 0, 0};
+  if constexpr (TWO) {
+#pragma unroll
+    for (int i = 0; i < BI; ++i)
+#pragma unroll
+      for (int j = 0; j < BJ; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) outer[i][j][r] = 0.f;
+  }
+
+  // ---- pipeline ------------------------------------------------------------
+  AsyncCursor cl, cc;  // load cursor (runs NS-1 steps ahead), compute cursor
+  cl.seek(a, split, 0);
+  cc = cl;
+  int nissued = 0, k = 0, since = 0;
+  __bf16* const slots[4] = {ring0, ring1, ring2, ring3};
+#pragma unroll
+  for (int t = 0; t < NS - 1; ++t)
+    if (cl.s < a.nseg) {
+      issue(cl, slots[t]);
+      ++nissued;
+      cl.next(a, split);
+    }
+  // step k computes from slot k % NS and refills slot (k - 1) % NS
+  auto step = [&](__bf16* cur, __bf16* refill) -> bool {
+    if (cc.s >= a.nseg) return false;
+    // this wave's copies of step k are done once at most (nissued - k - 1)
+    // later steps' copies are outstanding; the barrier then covers every
+    // wave's, and every wave is past step k-1, whose slot is refilled below
+    wait_copies<LPW, NS>(nissued - k - 1);
+    __syncthreads();
+    const int64_t valid = cc.rend - cc.r0;
+    if (valid < RK) {  // last k-step of a segment: zero the rows past its end
+      const int nz = (RK - (int)valid) * 16;
+      for (int e = threadIdx.x; e < 2 * NL * nz; e += NT) {
+        const int p = e / nz, rem = e % nz;
+        const int kr = (int)valid + rem / 16, un = rem % 16;
+        *reinterpret_cast<uint4*>(cur + p * PLANE + kr * CR_TM + 8 * un) = uint4{0, 0, 0, 0};
+      }
+      __syncthreads();
+    }
+    if (cl.s < a.nseg) {
+      issue(cl, refill);
+      ++nissued;
+      cl.next(a, split);
+    }
+    const bool neg = a.seg[cc.s].sign < 0;
+    bf16x8 xh[BI], xl[BI];
+#pragma unroll
+    for (int i = 0; i < BI; ++i) {
+      xh[i] = frag(cur, xoff[i]);
+      if constexpr (NL == 2) xl[i] = frag(cur, PLANE * 2 + xoff[i]);
+      if (neg) {
+        xh[i] = async_neg(xh[i]);
+        if constexpr (NL == 2) xl[i] = async_neg(xl[i]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < BJ; ++j) {
+      const bf16x8 yh = frag(cur, NL * PLANE * 2 + yoff[j]);
+      if constexpr (NL == 2) {
+        const bf16x8 yl = frag(cur, NL * PLANE * 2 + PLANE * 2 + yoff[j]);
+#pragma unroll
+        for (int i = 0; i < BI; ++i) {
+          inner[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[i], yh, inner[i][j], 0, 0, 0);
+          inner[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[i], yl, inner[i][j], 0, 0, 0);
+          inner[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[i], yh, inner[i][j], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < BI; ++i)
+          inner[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[i], yh, inner[i][j], 0, 0, 0);
+      }
+    }
+    cc.next(a, split);
+    ++k;
+    if constexpr (TWO) {
+      since += RK;
+      if (since >= Pol::FLUSH || cc.s >= a.nseg) {
+#pragma unroll
+        for (int i = 0; i < BI; ++i)
+#pragma unroll
+          for (int j = 0; j < BJ; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) outer[i][j][r] += inner[i][j][r];
+            inner[i][j] = f32x4{0, 0, 0, 0};
+          }
+        since = 0;
+      }
+    }
+    return true;
+  };
+  if constexpr (NS == 4) {
+    while (step(ring0, ring3) && step(ring1, ring0) && step(ring2, ring1) && step(ring3, ring2)) {
+    }
+  } else if constexpr (NS == 3) {
+    while (step(ring0, ring2) && step(ring1, ring0) && step(ring2, ring1)) {
+    }
+  } else {
+    while (step(ring0, ring1) && step(ring1, ring0)) {
+    }
+  }
+
+  double* out = a.partial + ((int64_t)split * a.ntiles + tile) * (CR_TM * CR_TN);
+#pragma unroll
+  for (int i = 0; i < BI; ++i)
+#pragma unroll
+    for (int j = 0; j < BJ; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int li = wm * 16 * BI + i * 16 + 4 * (lane >> 4) + r;
+        const int lj = wn * 16 * BJ + j * 16 + (lane & 15);
+        if constexpr (TWO) out[li * CR_TN + lj] = (double)outer[i][j][r];
+        else out[li * CR_TN + lj] = (double)inner[i][j][r];
+      }
+}

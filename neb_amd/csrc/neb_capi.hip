@@ -9,6 +9,7 @@
 #include <string>
 
 #include "../../include/neb_hip.h"
+#include "cross_async.hpp"
 #include "crossgemm.hpp"
 #include "jet_rowgemm.hpp"
 #include "misc_kernels.hpp"
@@ -234,13 +235,13 @@ int cross_splits(int ntiles, int64_t rows, int slots = CR_MAX_SLOTS) {
 // Resident workgroups of a kernel on the current device times the CU count
 // (at most 2 per CU), queried once per kernel and device into `cache`.
 template <typename Kern>
-int resident_slots(Kern kern, size_t lds, int (&cache)[16]) {
+int resident_slots(Kern kern, size_t lds, int (&cache)[16], int threads = 256) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return CR_MAX_SLOTS;
   if (cache[dev] == 0) {
     int cus = 0, per_cu = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 256, lds) !=
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), threads, lds) !=
             hipSuccess ||
         cus < 1 || per_cu < 1) {
       (void)hipGetLastError();
@@ -271,6 +272,49 @@ int launch_cross(CrossArgs& a, double scale, double beta, double* G, int64_t ldg
   return hip_status("neb_cross reduce");
 }
 
+// bf16-stored operands (one plane, or hi + lo planes) through the
+// asynchronous global -> LDS pipeline (cross_async.hpp)
+// ring / wave configuration per plane count (measured on the K = 1024
+// workloads): one plane: 8 waves of 32 x 64, 4 slots (3 k-steps of copies in
+// flight), two workgroups per CU; two planes: 4 waves of 64 x 64 (fewer
+// fragment reads per MFMA), 2 slots, two workgroups per CU (160 KB LDS).
+template <int NL> struct AsyncCfg;
+template <> struct AsyncCfg<1> { static constexpr int WM = 4, WN = 2, NS = 4; };
+template <> struct AsyncCfg<2> { static constexpr int WM = 2, WN = 2, NS = 2; };
+
+template <int NL>
+int launch_cross_async(CrossArgs& a, double scale, double beta, double* G, int64_t ldg, hipStream_t st) {
+  using C = AsyncCfg<NL>;
+  constexpr int NT = 64 * C::WM * C::WN;
+  const size_t lds = 0;  // the ring is static LDS
+  auto kern = cross_async_kernel<NL, C::WM, C::WN, C::NS>;
+  int64_t maxrows = 0;
+  for (int s = 0; s < a.nseg; ++s) maxrows = a.seg[s].rows > maxrows ? a.seg[s].rows : maxrows;
+  static int slots_cache[16] = {0};  // one per kernel instantiation
+  const int fit = cross_splits(a.ntiles, maxrows, resident_slots(kern, lds, slots_cache, NT));
+  if (fit < a.splits) a.splits = fit;
+  hipLaunchKernelGGL(kern, dim3(a.ntiles * a.splits), dim3(NT), lds, st, a);
+  if (int rc = hip_status("neb_cross (async)")) return rc;
+  const int64_t nbi = (a.m + 31) / 32, nbj = (a.n + 31) / 32;
+  const int64_t nblk = a.symmetric ? nbi * (nbi + 1) / 2 : nbi * nbj;
+  hipLaunchKernelGGL(cross_reduce_kernel, dim3((unsigned)nblk), dim3(256), 0, st, a.partial, a.splits, a.ntiles,
+                     a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
+  return hip_status("neb_cross reduce");
+}
+
+// The asynchronous path takes 16-byte aligned bf16 rows, no row weights and
+// segment signs of +-1.
+bool async_ok(const CrossArgs& a) {
+  if (a.roww) return false;
+  for (int s = 0; s < a.nseg; ++s) {
+    const CrossSeg& g = a.seg[s];
+    if (g.sign != 1.0 && g.sign != -1.0) return false;
+    if (!aligned(g.x, g.ldx, NEB_BF16) || !aligned(g.y, g.ldy, NEB_BF16)) return false;
+    if ((g.xlo && !aligned(g.xlo, g.ldx, NEB_BF16)) || (g.ylo && !aligned(g.ylo, g.ldy, NEB_BF16))) return false;
+  }
+  return true;
+}
+
 template <int MODE>
 int cross_dispatch_types(int xd, int yd, CrossArgs& a, double scale, double beta, double* G, int64_t ldg,
                          hipStream_t st) {
@@ -282,6 +326,8 @@ int cross_dispatch_types(int xd, int yd, CrossArgs& a, double scale, double beta
     return fail(NEB_ERR_ARG, "NEB_CROSS_F32 needs fp32 operands");
   } else {
     if (xd == NEB_F32 && yd == NEB_F32) return launch_cross<MODE, float, float>(a, scale, beta, G, ldg, st);
+    if (MODE == CR_BF16 && xd == NEB_BF16 && yd == NEB_BF16 && async_ok(a))
+      return launch_cross_async<1>(a, scale, beta, G, ldg, st);
     if (xd == NEB_BF16 && yd == NEB_BF16) return launch_cross<MODE, __bf16, __bf16>(a, scale, beta, G, ldg, st);
     if (xd == NEB_F32 && yd == NEB_BF16) return launch_cross<MODE, float, __bf16>(a, scale, beta, G, ldg, st);
     if (xd == NEB_BF16 && yd == NEB_F32) return launch_cross<MODE, __bf16, float>(a, scale, beta, G, ldg, st);
@@ -551,7 +597,7 @@ int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t
   const size_t need = (size_t)a.splits * a.ntiles * CR_TM * CR_TN * sizeof(double);
   if (ws_bytes < need) return fail(NEB_ERR_ARG, "neb_gram_split: workspace %zu < %zu bytes", ws_bytes, need);
   a.partial = reinterpret_cast<double*>(workspace);
-  return launch_cross<CR_BF16X3, SplitBF16, SplitBF16>(a, w, beta, G, ldg, S(stream));
+  return launch_cross_async<2>(a, w, beta, G, ldg, S(stream));
 }
 
 size_t neb_colsum_workspace_size(int64_t rows, int64_t cols) {
