@@ -19,26 +19,65 @@
 
 constexpr int SPD_NB = 64;
 
-// Pivot block inversion: Pi = (A[k0:k0+nb, k0:k0+nb])^-1 by scalar sweeps,
-// lazily scaled.  The sweep's rank-1 updates commute with scaling a row or a
-// column, so a swept row/column is left as it is (instead of multiplied by
-// 1/p) and the pivot entry is only negated (stored -p for the true -1/p);
-// afterwards every entry (i, j) carries the pending factor d_i d_j with
-// d_c = 1/p_c.  Per column a thread then does one FMA per entry it holds:
-//   v_ij -= x_i r_j,  x_i = v_ic / p (x_c = 0),  r_j = v_cj (r_c = 0).
-// Pivots p are entries no earlier step has scaled, so the positivity test is
-// the one of the plain sweep.  The 64 x 64 block lives in registers (thread t
-// owns row t/4, columns 16*(t%4) .. +15); each column step exchanges the
-// pivot row and column through LDS (double-buffered: one barrier per column).
-// Rows/columns >= nb are identity padding and are never swept.  info: 0 on
-// success, else the 1-based global index of the first nonpositive /
+// 4 x 4 inverse of a symmetric positive definite P by cofactors (one
+// division), and its leading principal minors m1..m4 (Sylvester: P is SPD
+// exactly when all four are positive; the first nonpositive one names the
+// first nonpositive pivot of the scalar sweep, pivot_k = m_k / m_(k-1)).
+__device__ inline void spd_inv4(const double (&a)[4][4], double (&inv)[4][4], double (&mn)[4]) {
+  const double s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1], s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+  const double s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3], s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+  const double s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3], s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+  const double c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3], c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+  const double c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2], c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+  const double c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2], c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+  const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  mn[0] = a[0][0];
+  mn[1] = s0;
+  mn[2] = a[2][2] * s0 - a[2][1] * s1 + a[2][0] * s3;
+  mn[3] = det;
+  const double id = 1.0 / det;
+  inv[0][0] = (a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3) * id;
+  inv[0][1] = (-a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3) * id;
+  inv[0][2] = (a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3) * id;
+  inv[0][3] = (-a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3) * id;
+  inv[1][0] = (-a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1) * id;
+  inv[1][1] = (a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1) * id;
+  inv[1][2] = (-a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1) * id;
+  inv[1][3] = (a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1) * id;
+  inv[2][0] = (a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0) * id;
+  inv[2][1] = (-a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0) * id;
+  inv[2][2] = (a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0) * id;
+  inv[2][3] = (-a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0) * id;
+  inv[3][0] = (-a[1][0] * c3 + a[1][1] * c1 - a[1][2] * c0) * id;
+  inv[3][1] = (a[0][0] * c3 - a[0][1] * c1 + a[0][2] * c0) * id;
+  inv[3][2] = (-a[3][0] * s3 + a[3][1] * s1 - a[3][2] * s0) * id;
+  inv[3][3] = (a[2][0] * s3 - a[2][1] * s1 + a[2][2] * s0) * id;
+}
+
+// Pivot block inversion: Pi = (A[k0:k0+nb, k0:k0+nb])^-1 by lazily scaled
+// sweeps over 4 x 4 pivot blocks.  Sweeping block b with pivot block P
+// (current values) is the rank-4 update
+//   v_ij -= x_i r_j,  x_i = v_(i,b) P^-1 (0 for rows of b),  r_j = v_(b,j)
+//   (0 for columns of b),
+// after which the diagonal block is negated; rows and columns of b are left
+// as they are.  The update commutes with left-multiplying a row block or
+// right-multiplying a column block by a 4 x 4 matrix, so the pending
+// factors P_b^-1 of every swept row / column block are applied once at the
+// end: -A^-1 = diag(P^-1) * v * diag(P^-1).  Each step's P is a block no
+// earlier step has transformed, so its leading minors are those of the
+// plain sweep (the positive-definiteness gate).  The 64 x 64 block lives in
+// registers (thread t owns row t/4, columns 16*(t%4) .. +15); each step
+// exchanges the pivot rows and columns through LDS (double-buffered: one
+// barrier per 4 columns).  Rows/columns >= nb are identity padding.  info:
+// 0 on success, else the 1-based global index of the first nonpositive /
 // nonfinite pivot (first writer wins).
 __global__ __launch_bounds__(256) void spd_pivot_kernel(const double* __restrict__ A, int64_t lda, int64_t k0,
                                                         int nb, double* __restrict__ Pi, int* __restrict__ info,
                                                         int reset) {
-  __shared__ __align__(16) double rowbuf[2][SPD_NB];
-  __shared__ double colbuf[2][SPD_NB];
-  __shared__ double dvec[SPD_NB];
+  __shared__ __align__(16) double rowbuf[2][4][SPD_NB];
+  __shared__ __align__(16) double colbuf[2][SPD_NB][4];
+  __shared__ __align__(16) double pivs[SPD_NB / 4][16];
+  __shared__ __align__(16) double M[SPD_NB][SPD_NB + 2];
   const int t = threadIdx.x;
   const int i = t >> 2, jb = t & 3, jg = jb * 16;
   if (reset && t == 0) *info = 0;  // first sweep clears the flag (no memset node)
@@ -52,42 +91,110 @@ __global__ __launch_bounds__(256) void spd_pivot_kernel(const double* __restrict
   for (int c0 = 0; c0 < nb; c0 += 16) {
     const bool own = (c0 >> 4) == jb;  // this thread holds columns c0 .. c0+15
 #pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int c = c0 + u, b = u & 1;
+    for (int s = 0; s < 4; ++s) {
+      const int c = c0 + 4 * s, u = 4 * s, b = s & 1;
       if (c < nb) {  // uniform over the workgroup
-        if (i == c) {
+        const bool inblk = i >= c && i < c + 4;
+        if (inblk) {
 #pragma unroll
           for (int q = 0; q < 16; q += 2)
-            *reinterpret_cast<f64x2*>(&rowbuf[b][jg + q]) =
-                f64x2{(own && q == u) ? 0.0 : v[q], (own && q + 1 == u) ? 0.0 : v[q + 1]};
+            *reinterpret_cast<f64x2*>(&rowbuf[b][i - c][jg + q]) =
+                f64x2{(own && q >= u && q < u + 4) ? 0.0 : v[q], (own && q + 1 >= u && q + 1 < u + 4) ? 0.0 : v[q + 1]};
         }
-        if (own) colbuf[b][i] = v[u];
+        if (own) {
+          *reinterpret_cast<f64x2*>(&colbuf[b][i][0]) = f64x2{v[u], v[u + 1]};
+          *reinterpret_cast<f64x2*>(&colbuf[b][i][2]) = f64x2{v[u + 2], v[u + 3]};
+        }
         __syncthreads();
-        double r[16];
+        double P[4][4], Q[4][4], mn[4];
 #pragma unroll
-        for (int q = 0; q < 16; q += 2) {
-          const f64x2 rv = *reinterpret_cast<const f64x2*>(&rowbuf[b][jg + q]);
-          r[q] = rv[0];
-          r[q + 1] = rv[1];
+        for (int r = 0; r < 4; ++r) {
+          const f64x2 p01 = *reinterpret_cast<const f64x2*>(&colbuf[b][c + r][0]);
+          const f64x2 p23 = *reinterpret_cast<const f64x2*>(&colbuf[b][c + r][2]);
+          P[r][0] = p01[0]; P[r][1] = p01[1]; P[r][2] = p23[0]; P[r][3] = p23[1];
         }
-        const double p = colbuf[b][c];
-        const bool okp = (p > 0.0) && isfinite(p);
-        if (!okp && bad == 0) bad = (int)(k0 + c + 1);
-        const double ip = 1.0 / (okp ? p : 1.0);  // keep the arithmetic finite; the caller raises on info
-        const double x = (i == c) ? 0.0 : colbuf[b][i] * ip;
+        spd_inv4(P, Q, mn);
+        bool ok = true;
+        int first = 4;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = fma(-x, r[q], v[q]);
-        v[u] = (own && i == c) ? -v[u] : v[u];
-        if (t == 0) dvec[c] = ip;
+        for (int r = 3; r >= 0; --r)
+          if (!(mn[r] > 0.0) || !isfinite(mn[r])) { ok = false; first = r; }
+        if (!ok) {
+          if (bad == 0) bad = (int)(k0 + c + first + 1);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Q[r][q] = r == q ? 1.0 : 0.0;  // keep the arithmetic finite
+        }
+        const f64x2 a01 = *reinterpret_cast<const f64x2*>(&colbuf[b][i][0]);
+        const f64x2 a23 = *reinterpret_cast<const f64x2*>(&colbuf[b][i][2]);
+        const double ai[4] = {a01[0], a01[1], a23[0], a23[1]};
+        double x[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const double xm = ai[0] * Q[0][m] + ai[1] * Q[1][m] + ai[2] * Q[2][m] + ai[3] * Q[3][m];
+          x[m] = inblk ? 0.0 : xm;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+          for (int q = 0; q < 16; q += 2) {
+            const f64x2 rv = *reinterpret_cast<const f64x2*>(&rowbuf[b][m][jg + q]);
+            v[q] = fma(-x[m], rv[0], v[q]);
+            v[q + 1] = fma(-x[m], rv[1], v[q + 1]);
+          }
+        }
+        if (own && inblk) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) v[u + k] = -v[u + k];
+        }
+        if (t == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) pivs[c >> 2][4 * r + q] = Q[r][q];
+        }
       }
     }
   }
-  __syncthreads();
-  const double di = i < nb ? dvec[i] : 0.0;
+  // pending factors: T = diag(P^-1) v (rows) through LDS, then T diag(P^-1)
+  // (columns) in registers; Pi = -T
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int j = jg + q;
-    Pi[i * SPD_NB + j] = (i < nb && j < nb) ? -v[q] * di * dvec[j] : 0.0;  // sweeps gave -P^-1
+  for (int q = 0; q < 16; q += 2) *reinterpret_cast<f64x2*>(&M[i][jg + q]) = f64x2{v[q], v[q + 1]};
+  __syncthreads();
+  const int nbk = (nb + 3) >> 2;
+  const int I = i >> 2, ir = i & 3;
+  double tr[16];
+  if (I < nbk) {
+    double li[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) li[a] = pivs[I][4 * ir + a];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      double acc = 0.0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc = fma(li[a], M[4 * I + a][jg + q], acc);
+      tr[q] = acc;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) tr[q] = 0.0;
+  }
+#pragma unroll
+  for (int J = 0; J < 4; ++J) {
+    const int Jg = (jg >> 2) + J;  // global column block
+    double o[4] = {0.0, 0.0, 0.0, 0.0};
+    if (Jg < nbk) {
+#pragma unroll
+      for (int bq = 0; bq < 4; ++bq)
+#pragma unroll
+        for (int cq = 0; cq < 4; ++cq) o[cq] = fma(tr[4 * J + bq], pivs[Jg][4 * bq + cq], o[cq]);
+    }
+#pragma unroll
+    for (int cq = 0; cq < 4; ++cq) {
+      const int j = jg + 4 * J + cq;
+      Pi[i * SPD_NB + j] = (i < nb && j < nb) ? -o[cq] : 0.0;
+    }
   }
   if (t == 0 && bad != 0) atomicCAS(info, 0, bad);
 }

@@ -518,7 +518,7 @@ def test_spectral_solver_backoff_after_failed_warm_start():
     r = rel(dM.cpu(), dM2.cpu()); assert r < 1e-9, r
 
 
-@pytest.mark.parametrize("K", [5, 64, 100, 300, 1024])
+@pytest.mark.parametrize("K", [5, 64, 70, 100, 130, 300, 1024])
 def test_spd_inverse_against_lapack(K):
     from neb_amd import spectral as sp
     rng = np.random.default_rng(K)
@@ -536,8 +536,8 @@ def test_spd_inverse_rejects_indefinite():
     rng = np.random.default_rng(1)
     B = rng.normal(size=(80, 70))
     A = B.T @ B / 70
-    A[40, 40] = -1.0  # breaks positive definiteness in the second pivot block
-    with pytest.raises(FloatingPointError):
+    A[40, 40] = -1.0  # breaks positive definiteness at pivot 41 (1-based), inside the second 64-block
+    with pytest.raises(FloatingPointError, match=r"pivot 41\)"):
         sp.spd_inverse(torch.tensor(A, device=DEV))
     # the flag is cleared by the next call
     ok = B.T @ B / 70 + np.eye(70)
