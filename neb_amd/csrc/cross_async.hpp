@@ -51,19 +51,6 @@ __device__ inline bf16x8 async_neg(bf16x8 v) {  // exact negation: flip the sign
   return __builtin_bit_cast(bf16x8, u);
 }
 
-// s_waitcnt vmcnt(n) with expcnt / lgkmcnt left at their maxima (vmcnt is
-// split over bits 3:0 and 15:14 of the immediate)
-constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 15) | ((n >> 4) << 14); }
-// this wave waits for all but the last `later` stages of copies (LPW each)
-template <int LPW, int NS>
-__device__ inline void wait_copies(int later) {
-  static_assert(LPW * (NS - 2) <= 63, "vmcnt range");
-  if (later >= NS - 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(LPW * (NS - 2)));
-  else if (NS > 3 && later == 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(LPW));
-  else if (later == 0) __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-  else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // deeper rings: conservative
-}
-
 // k-step cursor over the (segment, row) sequence of one row split
 struct AsyncCursor {
   int s;

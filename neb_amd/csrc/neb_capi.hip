@@ -7,12 +7,14 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/neb_hip.h"
 #include "cross_async.hpp"
 #include "crossgemm.hpp"
 #include "jet_rowgemm.hpp"
 #include "misc_kernels.hpp"
+#include "rowgemm_async.hpp"
 #include "spd_inverse.hpp"
 
 namespace {
@@ -55,9 +57,25 @@ int set_lds(K kernel, size_t bytes) {
   return NEB_OK;
 }
 
+template <typename Kern>
+int resident_slots(Kern kern, size_t lds, int (&cache)[16], int threads);
+
 template <int P, typename TIN, class Plan>
 int launch_rowgemm(const RowArgs& a, hipStream_t st, const char* what) {
   constexpr int TN = Plan::WPN * Plan::FB * 16, TP = Plan::WPP * Plan::PB * 16;
+  if constexpr (P == PREC_BF16 && std::is_same<TIN, __bf16>::value && row_async_ok<Plan>()) {
+    // bf16 operands: persistent workgroups on the asynchronous-copy ring,
+    // one round of resident workgroups, each a contiguous range of tiles
+    const int64_t ntiles = ((a.npts + TP - 1) / TP) * ((a.nout + TN - 1) / TN);
+    if (ntiles == 0) return NEB_OK;
+    auto kern = rowgemm_async_kernel<Plan>;
+    static int slots_cache[16] = {0};
+    const int64_t slots = resident_slots(kern, 0, slots_cache, 256);
+    const int64_t per_wg = (ntiles + slots - 1) / slots;
+    const int64_t grid = (ntiles + per_wg - 1) / per_wg;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, a, per_wg);
+    return hip_status(what);
+  }
   const size_t lds = rowgemm_lds_bytes<P, Plan>();
   auto kern = rowgemm_kernel<P, TIN, Plan>;
   if (int rc = set_lds(kern, lds)) return rc;
@@ -235,7 +253,7 @@ int cross_splits(int ntiles, int64_t rows, int slots = CR_MAX_SLOTS) {
 // Resident workgroups of a kernel on the current device times the CU count
 // (at most 2 per CU), queried once per kernel and device into `cache`.
 template <typename Kern>
-int resident_slots(Kern kern, size_t lds, int (&cache)[16], int threads = 256) {
+int resident_slots(Kern kern, size_t lds, int (&cache)[16], int threads) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return CR_MAX_SLOTS;
   if (cache[dev] == 0) {
@@ -261,7 +279,7 @@ int launch_cross(CrossArgs& a, double scale, double beta, double* G, int64_t ldg
   int64_t maxrows = 0;
   for (int s = 0; s < a.nseg; ++s) maxrows = a.seg[s].rows > maxrows ? a.seg[s].rows : maxrows;
   static int slots_cache[16] = {0};  // one per kernel instantiation
-  const int fit = cross_splits(a.ntiles, maxrows, resident_slots(kern, lds, slots_cache));
+  const int fit = cross_splits(a.ntiles, maxrows, resident_slots(kern, lds, slots_cache, 256));
   if (fit < a.splits) a.splits = fit;
   hipLaunchKernelGGL(kern, dim3(a.ntiles * a.splits), dim3(256), lds, st, a);
   if (int rc = hip_status("neb_cross")) return rc;

@@ -200,3 +200,16 @@ __device__ inline int lds_k(int row, int k) {
   if constexpr (sizeof(S) == 2) return ((((k >> 3) ^ (row >> SH)) & 3) << 3) | (k & 7);
   return k;
 }
+
+// s_waitcnt vmcnt(n) with expcnt / lgkmcnt left at their maxima (vmcnt is
+// split over bits 3:0 and 15:14 of the immediate)
+constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 15) | ((n >> 4) << 14); }
+// this wave waits for all but the last `later` stages of copies (LPW each)
+template <int LPW, int NS>
+__device__ inline void wait_copies(int later) {
+  static_assert(LPW * (NS - 2) <= 63, "vmcnt range");
+  if (later >= NS - 2) __builtin_amdgcn_s_waitcnt(vmcnt_imm(LPW * (NS - 2)));
+  else if (NS > 3 && later == 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(LPW));
+  else if (later == 0) __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // deeper rings: conservative
+}

@@ -1,0 +1,236 @@
+// Persistent row GEMM for bf16 operands fed by asynchronous global -> LDS
+// copies: the same plans, accumulator sets, fragment permutation and
+// epilogues as rowgemm_kernel (jet_rowgemm.hpp), with a different data path.
+//
+//   * global_load_lds_dwordx4 copies 16 bytes per lane straight into LDS
+//     (no staging VGPRs); one copy instruction fills 16 tile rows x 32 k
+//     (1 KB).  The row-major sources are k-contiguous, which is the MFMA
+//     operand order, so the tiles keep the chunk-swizzled layout of
+//     lds_k<S, 2> and the fragments are plain ds_read_b128.
+//   * A ring of NS LDS slots (one LDS object each, so the compiler's wait
+//     insertion tracks the copies per slot) keeps NS-1 k-steps in flight.
+//   * Each workgroup walks a contiguous range of (point block, feature block)
+//     tiles; the copy cursor runs ahead across tile boundaries, so the next
+//     tile's first k-steps land while the current tile's epilogue (the jet
+//     chain and stores) runs.
+// Rows past the point / feature count re-read the last row (their results
+// are never stored); k past the reduction length is zeroed in LDS on the
+// last k-step of a tile.
+#pragma once
+#include "jet_rowgemm.hpp"
+
+template <class Plan>
+struct RowAsyncPol {
+  static constexpr int RK = 32, LDK = 32;
+  static constexpr int TN = Plan::WPN * Plan::FB * 16, TP = Plan::WPP * Plan::PB * 16;
+  static constexpr int WT = TN * LDK, XT = TP * LDK;
+  static constexpr int STAGE = Plan::NW * WT + Plan::CI * XT;  // bf16 elements per slot
+  static constexpr int WPIECE = Plan::NW * TN / 16, XPIECE = Plan::CI * TP / 16;
+  static constexpr int NPIECE = WPIECE + XPIECE;  // 1 KB copies per slot
+  static constexpr int PPW = (NPIECE + 3) / 4;    // per wave (the last round padded with repeats)
+  static constexpr int SLOT_BYTES = STAGE * 2;
+  // ring depth: as many slots as keep two workgroups per CU (<= 80 KB each)
+  static constexpr int NS = SLOT_BYTES * 4 <= 80 * 1024 ? 4 : (SLOT_BYTES * 3 <= 80 * 1024 ? 3 : 2);
+};
+
+template <class Plan>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void rowgemm_async_kernel(
+    RowArgs a, int64_t per_wg) {
+  constexpr int P = PREC_BF16;
+  using PR = Prec<P>;
+  using S = __bf16;
+  using M = float;
+  using frag_t = typename PR::frag_t;
+  using acc_t = typename PR::acc_t;
+  using Q = RowAsyncPol<Plan>;
+  constexpr int RK = Q::RK, LDK = Q::LDK, KE = PR::KE;
+  constexpr int CI = Plan::CI, NW = Plan::NW, NA = Plan::NA, NT = Plan::NT;
+  constexpr int FB = Plan::FB, PB = Plan::PB, WPN = Plan::WPN, WPP = Plan::WPP;
+  constexpr int TN = Q::TN, TP = Q::TP, WT = Q::WT, XT = Q::XT, STAGE = Q::STAGE;
+  constexpr int NS = Q::NS, NPIECE = Q::NPIECE, PPW = Q::PPW, WPIECE = Q::WPIECE;
+  static_assert(WPN * WPP == 4, "four waves per workgroup");
+  static_assert(FB % 2 == 0, "8-feature epilogue groups pair feature blocks");
+  static_assert(NS >= 2 && NS <= 4, "ring of 2..4 slots");
+
+  // LDS pointers stay in address space 3 from the arrays on (a generic
+  // pointer cast back to LDS would need a null check per use)
+  typedef __attribute__((address_space(3))) S lds_S;
+  typedef __attribute__((address_space(3))) const frag_t lds_frag;
+  __shared__ __align__(16) S ring0[STAGE];
+  __shared__ __align__(16) S ring1[STAGE];
+  __shared__ __align__(16) S ring2[NS > 2 ? STAGE : 8];
+  __shared__ __align__(16) S ring3[NS > 3 ? STAGE : 8];
+  lds_S* const r0 = (lds_S*)ring0;
+  lds_S* const r1 = (lds_S*)ring1;
+  lds_S* const r2 = (lds_S*)ring2;
+  lds_S* const r3 = (lds_S*)ring3;
+  lds_S* const slots[4] = {r0, r1, r2, r3};
+
+  const int64_t nfb = (a.nout + TN - 1) / TN, npb = (a.npts + TP - 1) / TP;
+  const int64_t ntiles = npb * nfb;
+  const int64_t t_beg = (int64_t)blockIdx.x * per_wg;
+  const int64_t t_end = t_beg + per_wg < ntiles ? t_beg + per_wg : ntiles;
+  const int nk = (int)((a.k + RK - 1) / RK);
+  const int kpart = (int)(a.k - (int64_t)(nk - 1) * RK);  // valid k of the last k-step
+
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wn = wave % WPN, wp = wave / WPN;
+  const int fr = lane & 15, kg = (lane >> 4) * KE;
+  constexpr int GE = Plan::template out_bytes<P>() == 2 ? 8 : 4;
+  const int rg = PR::rowg(fr), rr = PR::rowr(fr);
+
+  // ---- copies: piece q of a slot (static), issued by wave q % 4 -----------
+  const int cr = lane >> 2, cu = lane & 3;   // row in the 16-row piece, LDS chunk slot
+  const int cch = cu ^ ((cr >> 2) & 3);      // global chunk held by that slot (lds_k<S, 2>)
+  auto issue = [&](int64_t t, int kt, lds_S* slot) __attribute__((always_inline)) {
+    const int64_t p0 = (t / nfb) * TP, f0 = (t % nfb) * TN, k0 = (int64_t)kt * RK;
+#pragma unroll
+    for (int q = 0; q < 4 * PPW; ++q) {
+      if ((q & 3) == wave) {
+        const int qq = q < NPIECE ? q : q - NPIECE;  // padding repeats an earlier piece
+        const S* src;
+        int64_t ld, row, nrows;
+        lds_S* dst;
+        if (qq < WPIECE) {
+          const int s = qq / (TN / 16), rb = qq % (TN / 16);
+          src = reinterpret_cast<const S*>(a.w[s]);
+          ld = a.ldw[s];
+          row = f0 + 16 * rb + cr;
+          nrows = a.nout;
+          dst = slot + s * WT + 16 * rb * LDK;
+        } else {
+          const int c = (qq - WPIECE) / (TP / 16), rb = (qq - WPIECE) % (TP / 16);
+          src = reinterpret_cast<const S*>(a.x[c]);
+          ld = a.ldx[c];
+          row = p0 + 16 * rb + cr;
+          nrows = a.npts;
+          dst = slot + NW * WT + c * XT + 16 * rb * LDK;
+        }
+        row = row < nrows ? row : nrows - 1;
+        int64_t kk = k0 + 8 * cch;
+        kk = kk + 8 <= ld ? kk : ld - 8;
+        __builtin_amdgcn_global_load_lds(src + row * ld + kk, (__attribute__((address_space(3))) void*)dst, 16, 0,
+                                         0);
+      }
+    }
+  };
+
+  int64_t lt = t_beg;  // copy cursor (tile, k-step)
+  int lk = 0;
+  int nissued = 0;
+#pragma unroll
+  for (int s = 0; s < NS - 1; ++s)
+    if (lt < t_end) {
+      issue(lt, lk, slots[s]);
+      ++nissued;
+      if (++lk == nk) { lk = 0; ++lt; }
+    }
+
+  acc_t acc[NA][FB][PB];
+  auto zero_acc = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int s = 0; s < NA; ++s)
+#pragma unroll
+      for (int i = 0; i < FB; ++i)
+#pragma unroll
+        for (int j = 0; j < PB; ++j) acc[s][i][j] = acc_t{0, 0, 0, 0};
+  };
+  zero_acc();
+
+  int64_t ct = t_beg;  // compute cursor
+  int ck = 0, kdone = 0;
+  // always inlined: the accumulators must stay in registers
+  auto step = [&](lds_S* cur, lds_S* refill) __attribute__((always_inline)) -> bool {
+    if (ct >= t_end) return false;
+    wait_copies<PPW, NS>(nissued - kdone - 1);
+    __syncthreads();
+    if (ck == nk - 1 && kpart < RK) {  // zero k >= the reduction length
+      constexpr int ROWS = NW * TN + CI * TP;
+      const int nz = RK - kpart;
+      for (int e = threadIdx.x; e < ROWS * nz; e += 256) {
+        const int row = e / nz, k = kpart + e % nz;
+        cur[row * LDK + lds_k<S, 2>(row, k)] = (S)0.0f;
+      }
+      __syncthreads();
+    }
+    if (lt < t_end) {  // refill the slot the previous step used
+      issue(lt, lk, refill);
+      ++nissued;
+      if (++lk == nk) { lk = 0; ++lt; }
+    }
+    const lds_S* ws = cur;
+    const lds_S* xs = cur + NW * WT;
+    frag_t wf[NW][FB], xf[CI][PB];
+#pragma unroll
+    for (int i = 0; i < FB; ++i) {
+      const int row = wn * FB * 16 + (GE == 8 ? 32 * (i >> 1) + 8 * rg + 4 * (i & 1) : 16 * i + 4 * rg) + rr;
+      const int off = row * LDK + lds_k<S, 2>(row, kg);
+#pragma unroll
+      for (int s = 0; s < NW; ++s) wf[s][i] = *reinterpret_cast<lds_frag*>(ws + s * WT + off);
+    }
+#pragma unroll
+    for (int j = 0; j < PB; ++j) {
+      const int row = wp * PB * 16 + j * 16 + fr;
+      const int off = row * LDK + lds_k<S, 2>(row, kg);
+#pragma unroll
+      for (int c = 0; c < CI; ++c) xf[c][j] = *reinterpret_cast<lds_frag*>(xs + c * XT + off);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int i = 0; i < FB; ++i) {
+        const frag_t wv = Plan::tneg[t] ? fneg(wf[Plan::tw[t]][i]) : wf[Plan::tw[t]][i];
+#pragma unroll
+        for (int j = 0; j < PB; ++j)
+          acc[Plan::ta[t]][i][j] = PR::mfma(wv, xf[Plan::tc[t]][j], acc[Plan::ta[t]][i][j]);
+      }
+    ++kdone;
+    if (++ck == nk) {  // tile done: epilogue while the next tile's copies fly
+      const int64_t p0 = (ct / nfb) * TP, f0 = (ct % nfb) * TN;
+      const int g = lane >> 4;
+#pragma unroll
+      for (int j = 0; j < PB; ++j) {
+        const int64_t p = p0 + wp * PB * 16 + j * 16 + fr;
+#pragma unroll
+        for (int q = 0; q < FB * 4 / GE; ++q) {
+          const int64_t f = f0 + wn * FB * 16 + (GE == 8 ? 32 * q + 8 * g : 16 * q + 4 * g);
+          const int64_t rem = a.nout - f;
+          if (p < a.npts && rem > 0) {
+            const int nvalid = rem >= GE ? GE : (int)rem;
+            M v[NA][GE];
+#pragma unroll
+            for (int s = 0; s < NA; ++s)
+#pragma unroll
+              for (int e = 0; e < GE; ++e)
+                v[s][e] = GE == 8 ? (M)acc[s][2 * q + (e >> 2)][j][e & 3] : (M)acc[s][q][j][e];
+            Plan::template epilogue<P, GE>(a, p, f, nvalid, v);
+          }
+        }
+      }
+      zero_acc();
+      ck = 0;
+      ++ct;
+    }
+    return true;
+  };
+  if constexpr (NS == 4) {
+    while (step(r0, r3) && step(r1, r0) && step(r2, r1) && step(r3, r2)) {
+    }
+  } else if constexpr (NS == 3) {
+    while (step(r0, r2) && step(r1, r0) && step(r2, r1)) {
+    }
+  } else {
+    while (step(r0, r1) && step(r1, r0)) {
+    }
+  }
+}
+
+// Plans whose slot leaves a ring of fewer than 3 slots stay on the
+// register-staged rowgemm_kernel, whose two register stages in flight did
+// better there (measured on the 2D output-layer reverse; halving its point
+// tile to fit 3 slots doubled the weight traffic per point and was slower
+// still).
+template <class Plan>
+constexpr bool row_async_ok() {
+  return RowAsyncPol<Plan>::NS >= 3;
+}
