@@ -48,8 +48,10 @@ def parse():
                     choices=["hidden_fwd", "hidden_bwd", "out_fields", "out_bwd", "gram", "gram_f32in", "cross_dw",
                              "copy"],
                     help="run one kernel of the config-B step --steps times (for counter profiling)")
-    ap.add_argument("--config", default="B", choices=["B", "E"],
-                    help="B: 2D 256^2 grid K=1024 (default); E: 3D blob domain, 1,599,754 points, K=1024")
+    ap.add_argument("--config", default="B", choices=["B", "D", "E"],
+                    help="B: 2D 256^2 grid K=1024 (default); D: 3D blob domain, 1,599,754 points, K=512, "
+                         "forward + Grams + FP64 generalized eigendecomposition; E: the same points, K=1024, "
+                         "training step + 20-target full-mass projection")
     return ap.parse_args()
 
 
@@ -118,9 +120,11 @@ def profile_traffic(name):
     return None
 
 
-def cpu_baseline(seconds_target=15.0):
+def cpu_baseline(seconds_target=12.0, single_seconds=8.0):
     """FP64 NumPy oracle of the same step on a bounded sample of the grid
-    (first n_s points), timed on this host; scaled to point*modes/s."""
+    (consecutive 128-point batches), timed on this host with all of its
+    cores and again with BLAS pinned to one thread (the reference's own
+    pinning); scaled to point*modes/s."""
     import numpy as np
     from oracle import neb_oracle as o
     pts = o.periodic_grid(N_GRID)
@@ -128,31 +132,50 @@ def cpu_baseline(seconds_target=15.0):
     net = o.init_net(sizes, 0)
     w = (2 * math.pi / N_GRID) ** 2
     n_s = 128
-    t_total, n_done = 0.0, 0
-    while True:
-        x = pts[n_done % len(pts): n_done % len(pts) + n_s]
-        t0 = time.perf_counter()
-        x0 = o.feature_jets(x, FREQS, (0.0, 0.0), (2 * math.pi, 2 * math.pi))
-        out, tape = o.forward(net, x0, record=True)
-        u, om = o.lift2d(out)
-        Pm, Ps = o.field_rows(u), om
-        M, S = o.gram(Pm, w), o.gram(Ps, w)
-        dM = np.eye(K_MODES) * 1e-3  # the objective's K x K algebra is size-independent; fixed cotangents
-        gPm, gPs = o.field_adjoint(Pm, w, dM), o.field_adjoint(Ps, w, dM)
-        adj = o.lift2d_reverse(gPm.reshape(len(x), 2, K_MODES), gPs)
-        o.backward(net, tape, adj)
-        t_total += time.perf_counter() - t0
-        n_done += n_s
-        if t_total >= seconds_target:
-            break
+
+    def run(seconds):
+        t_total, n_done = 0.0, 0
+        while True:
+            x = pts[n_done % len(pts): n_done % len(pts) + n_s]
+            t0 = time.perf_counter()
+            x0 = o.feature_jets(x, FREQS, (0.0, 0.0), (2 * math.pi, 2 * math.pi))
+            out, tape = o.forward(net, x0, record=True)
+            u, om = o.lift2d(out)
+            Pm, Ps = o.field_rows(u), om
+            M, S = o.gram(Pm, w), o.gram(Ps, w)
+            dM = np.eye(K_MODES) * 1e-3  # the objective's K x K algebra is size-independent; fixed cotangents
+            gPm, gPs = o.field_adjoint(Pm, w, dM), o.field_adjoint(Ps, w, dM)
+            adj = o.lift2d_reverse(gPm.reshape(len(x), 2, K_MODES), gPs)
+            o.backward(net, tape, adj)
+            t_total += time.perf_counter() - t0
+            n_done += n_s
+            if t_total >= seconds:
+                return n_done, t_total
+
+    n_all, t_all = run(seconds_target)
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count()
     threads = int(os.environ.get("OMP_NUM_THREADS", cores))
-    return dict(value=n_done * K_MODES / t_total, unit="point*modes/s", cores=min(threads, cores), kind="port",
-                sample=f"{n_done} of the 65,536 grid points in {n_s}-point batches, full jet fwd + Grams + "
-                       f"reverse at K=1024 (FP64 NumPy oracle; K x K objective excluded), {t_total:.1f} s")
+    single = None
+    try:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1):
+            n_1, t_1 = run(single_seconds)
+        single = n_1 * K_MODES / t_1
+    except ImportError:
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return dict(value=n_all * K_MODES / t_all, unit="point*modes/s", cores=min(threads, cores), kind="port",
+                sample=f"{n_all} of the 65,536 grid points in {n_s}-point batches, full jet fwd + Grams + "
+                       f"reverse at K=1024 (FP64 NumPy oracle; K x K objective excluded), {t_all:.1f} s",
+                single_thread_value=single, affinity_cores=cores, cpu_model=model)
 
 
 def bench_kernel(args):
@@ -304,6 +327,34 @@ def bench_3d(args):
             stages[k] = stages.get(k, 0.0) + v / steps * 1e3
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
+
+    # 20 divergence-free targets projected onto the basis by the FP64 mass
+    # solve (two streamed passes: loads, then the direct residual)
+    t1 = time.perf_counter()
+    tgt = syn.curl_targets(pts, 20, seed=1)
+    targets = torch.tensor(tgt.reshape(N * 3, 20), dtype=torch.float64, device=dev)
+    del tgt
+    gen_s = time.perf_counter() - t1
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # FP64 mass of the fp32 basis of the current network (fp32 widened on
+    # load), as the projection's Gram identity needs; the K=1024 basis is
+    # rank deficient, so the solve uses the step's ridge
+    M64 = torch.zeros((K, K), dtype=torch.float64, device=dev)
+    for _, f in sp.basis_stream(net, x, "bf16", CHUNK3D):
+        nw.field_gram(f, "mass", w, "f64", out=M64, beta=1.0)
+        del f
+    torch.cuda.synchronize()
+    mass_s = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    proj = sp.project_streamed(net, x, targets, M64, w, "bf16", CHUNK3D, ridge=ridge)
+    torch.cuda.synchronize()
+    proj_s = time.perf_counter() - t1
+    projection = dict(targets=20, ms=proj_s * 1e3, fp64_mass_ms=mass_s * 1e3, target_generation_host_s=gen_s,
+                      normal_residual_max=float(proj["normal_residuals"].max()),
+                      direct_relative_L2_median=float(proj["direct_relative_L2"].median()),
+                      identity_gap_max=float((proj["direct_relative_L2"] - proj["Gram_identity_errors"]).abs().max()))
+    del targets, proj, M64
     out = {
         "metric": "point*modes/s, jet fwd+bwd + Gram spectral step, 3D K=1024",
         "value": N * K / dt, "unit": "point*modes/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -314,7 +365,72 @@ def bench_3d(args):
                                "reverse + Adam", "points": N, "modes": K, "chunk": CHUNK3D, "ridge": ridge,
                    "domain_volume_estimate": vol},
         "stages_ms": stages, "gram_one_chunk": gram_stats, "fp32_step_point_modes_per_s": f32_rate,
-        "loss": float(res["loss"]), "objective_path": solver.last_path,
+        "loss": float(res["loss"]), "objective_path": solver.last_path, "projection": projection,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_3d_eig(args):
+    """3D blob domain, K=512 vector potential (1536 outputs, full-rank mass):
+    chunked jets + curl-lifted fields + split FP64 Grams over all 1,599,754
+    points, then the FP64 generalized eigendecomposition S x = lambda M x."""
+    import torch
+    import neb_amd as nb
+    from neb_amd import network as nw, spectral as sp, synthetic as syn
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    pts, _, vol = syn.blob_domain(N3D, seed=0)
+    x = torch.tensor(pts, dtype=torch.float32, device=dev)
+    N, K = N3D, 512
+    w = 1.0 / N
+    cfg = nb.NetConfig(dim=3, frequencies=6, lo=(-1.0,) * 3, hi=(1.0,) * 3, widths=WIDTHS, modes=K)
+    net = nb.EigenfluidNet(cfg, seed=0, device=dev)
+
+    def grams(prec, gmode):
+        return sp.chunked_step(net, x, w, prec, gmode, CHUNK3D, solver=lambda M, S: (0.0, None, None),
+                               training=False)
+
+    for _ in range(max(1, min(args.warmup, 2))):
+        warm = grams("bf16", "bf16x3")
+        sp.generalized_eigh(warm["S"], warm["M"])  # first-call library set-up stays out of the timing
+    del warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = grams("bf16", "bf16x3")
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vals, _ = sp.generalized_eigh(res["S"], res["M"])
+    torch.cuda.synchronize()
+    t_eig = time.perf_counter() - t0
+    ref = grams("f64", "f64")  # FP64 jets, fields and Grams
+    v64, _ = sp.generalized_eigh(ref["S"], ref["M"])
+    gram_err = float(torch.linalg.norm(res["M"] - ref["M"]) / torch.linalg.norm(ref["M"]))
+    eig_err = float(((vals - v64).abs() / v64.abs()).max())
+    del ref
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grams("f32", "f32")
+    torch.cuda.synchronize()
+    t_f32 = time.perf_counter() - t0
+    R_m, R_s = 3 * N, 9 * N
+    out = {
+        "metric": "point*modes/s, jet forward + split-precision Gram assembly + FP64 generalized eigh, 3D K=512",
+        "value": N * K / (t_fwd + t_eig), "unit": "point*modes/s", "n_gpus": 1, "steps": 1,
+        "warmup": args.warmup, "ms_per_step": (t_fwd + t_eig) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic: 6-blob domain in [-1,1]^3 (seed 0), Xavier-normal weights seed 0",
+        "config": {"workload": "3D blob domain: 1,599,754 pts, sin/cos freqs 1..6, 4x256 tanh, K=512 vector "
+                               "potential, curl lift, mass + grad-u split Grams in 131,072-pt chunks, FP64 "
+                               "generalized eigendecomposition", "points": N, "modes": K, "chunk": CHUNK3D,
+                   "domain_volume_estimate": vol},
+        "stages_ms": {"forward_fields_grams": t_fwd * 1e3, "generalized_eigh": t_eig * 1e3},
+        "gram_useful_tflops": (R_m + R_s) * K * (K + 1) / t_fwd / 1e12,
+        "fp32_forward_grams_ms": t_f32 * 1e3,
+        "mass_gram_rel_err_vs_fp64_path": gram_err,
+        "eigenvalue_max_rel_err_vs_fp64_path": eig_err,
+        "eigenvalues_min_max": [float(vals[0]), float(vals[-1])],
     }
     print(json.dumps(out), flush=True)
 
@@ -327,6 +443,9 @@ def main():
         sys.exit(2)
     if args.config == "E":
         bench_3d(args)
+        return
+    if args.config == "D":
+        bench_3d_eig(args)
         return
     if args.kernel:
         bench_kernel(args)
@@ -386,7 +505,38 @@ def main():
         gram_stats[mode] = dict(tflops=flops / t / 1e12, ms=t * 1e3, rel_err_vs_fp64=err)
     gram_stats["split_vs_split_on_load_rel_diff"] = float(
         torch.linalg.norm(results["bf16x3-split"] - results["bf16x3"]) / torch.linalg.norm(results["bf16x3"]))
-    del G64, tape, fields, split, results
+    del G64, results
+
+    # ---- end-to-end errors against the FP64 path (FP64 jets, fields, Grams) --
+    tape64 = nw.trunk_forward(net, x, nw.PRECISIONS["f64"])
+    f64f = nw.output_fields(net, tape64)
+    del tape64
+    M64 = nw.gram(f64f.mass, K, w, "f64")
+    S64 = nw.gram(f64f.stiff, K, w, "f64")
+    eye = torch.eye(K, dtype=torch.float64, device=dev)
+    lam64, _ = sp.generalized_eigh(S64, M64 + ridge * eye)
+    top = WIDTHS[-1]  # rank of M: the physical eigenvalues are the top `top`; the rest are ridge-dominated
+    accuracy = {}
+    for prec in ("bf16", "f32"):
+        tp = nw.trunk_forward(net, x, nw.PRECISIONS[prec])
+        fp = nw.output_fields(net, tp, split=(prec == "bf16"))
+        del tp
+        gm = "bf16x3" if prec == "bf16" else "f32"
+        Mp = nw.field_gram(fp, "mass", w, gm)
+        Sp = nw.field_gram(fp, "stiff", w, gm)
+        U = fp.mass[:, :K].float() + (fp.mass_lo[:, :K].float() if fp.split else 0.0)
+        vel = float(torch.linalg.norm(U.double() - f64f.mass[:, :K]) / torch.linalg.norm(f64f.mass[:, :K]))
+        lam, _ = sp.generalized_eigh(Sp, Mp + ridge * eye)
+        accuracy[prec] = dict(velocity_rel_l2=vel,
+                              mass_gram_rel_err=float(torch.linalg.norm(Mp - M64) / torch.linalg.norm(M64)),
+                              stiffness_gram_rel_err=float(torch.linalg.norm(Sp - S64) / torch.linalg.norm(S64)),
+                              eigenvalue_max_rel_err_top=float(((lam[-top:] - lam64[-top:]).abs()
+                                                                / lam64[-top:].abs()).max()))
+        del fp, U
+    accuracy["note"] = (f"vs the FP64 path on the same grid and weights; eigenvalues of S x = lambda (M + ridge I) x, "
+                        f"the top {top} (= rank M; the others are set by the ridge); bf16 velocities are the hi + lo "
+                        f"planes of the fp32 field values")
+    del f64f, M64, S64, tape, fields, split
 
     # ---- FP32 comparison step ------------------------------------------------
     f32_rate = None
@@ -449,6 +599,7 @@ def main():
         "stages_ms": stages,
         "gram": gram_stats,
         "fp32_step_point_modes_per_s": f32_rate,
+        "accuracy_vs_fp64": accuracy,
         "loss": float(loss),
         "objective_path": solver.last_path,
         "roofline": roofline,

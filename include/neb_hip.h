@@ -132,7 +132,8 @@ int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int
 /* Split-R cross product  G = beta*G + scale * sum_s sign_s X_s^T diag(roww) Y_s
  * over rows, FP64 result [m][ldg].  Segment s: X_s [rows_s][ldx_s] with m
  * used columns, Y_s [rows_s][ldy_s] with n used columns; x_dtype / y_dtype
- * are NEB_F32 / NEB_BF16 / NEB_F64.  symmetric != 0 requires X == Y and
+ * are NEB_F32 / NEB_BF16 / NEB_F64 (NEB_CROSS_F64: fp64 operands, or fp32
+ * X widened exactly to fp64 against fp64 or fp32 Y).  symmetric != 0 requires X == Y and
  * computes upper tiles only.  Replaces the FP64 Gram blocks of VCORE:40-46,
  * experiments/metaball/run.py:99-107, experiments/periodic/paper_rollout.py:
  * 178-179, the weight-gradient einsums HB2P:1087-1144 / VJ2:29 / FAST:27-29

@@ -338,7 +338,11 @@ int cross_dispatch_types(int xd, int yd, CrossArgs& a, double scale, double beta
                          hipStream_t st) {
   if constexpr (MODE == CR_F64) {
     if (xd == NEB_F64 && yd == NEB_F64) return launch_cross<MODE, double, double>(a, scale, beta, G, ldg, st);
-    return fail(NEB_ERR_ARG, "NEB_CROSS_F64 needs fp64 operands");
+    // fp32 operands (network fields) widened exactly to fp64 on load: the
+    // FP64 mass and projection loads of an fp32 basis
+    if (xd == NEB_F32 && yd == NEB_F64) return launch_cross<MODE, float, double>(a, scale, beta, G, ldg, st);
+    if (xd == NEB_F32 && yd == NEB_F32) return launch_cross<MODE, float, float>(a, scale, beta, G, ldg, st);
+    return fail(NEB_ERR_ARG, "NEB_CROSS_F64 needs fp64 or fp32 X with fp64 / fp32 Y");
   } else if constexpr (MODE == CR_F32) {
     if (xd == NEB_F32 && yd == NEB_F32) return launch_cross<MODE, float, float>(a, scale, beta, G, ldg, st);
     return fail(NEB_ERR_ARG, "NEB_CROSS_F32 needs fp32 operands");

@@ -297,11 +297,11 @@ def project_streamed(net: nw.EigenfluidNet, points: torch.Tensor, targets: torch
     norms2 = torch.zeros((m, m), dtype=torch.float64, device=net.device)
     for c0, f in basis_stream(net, points, prec, chunk):
         rows = f.mass.shape[0]
-        B = f.mass[:, :K].to(torch.float64).contiguous()
         T = targets[comp * c0: comp * c0 + rows]
-        nw.cross([(B, T, rows, 1.0)], K, m, nw.GRAM_MODES["f64"], scale=weight, out=loads, beta=1.0)
+        # fp32 basis widened to fp64 on load (fp64 basis in the FP64 mode)
+        nw.cross([(f.mass, T, rows, 1.0)], K, m, nw.GRAM_MODES["f64"], scale=weight, out=loads, beta=1.0)
         nw.cross([(T, T, rows, 1.0)], m, m, nw.GRAM_MODES["f64"], scale=weight, out=norms2, beta=1.0)
-        del B, f
+        del f
     norms = torch.diagonal(norms2).clone()
     Mr = _sym(mass) + ridge * torch.eye(K, dtype=torch.float64, device=mass.device)
     coeff = torch.linalg.solve(Mr, loads)
