@@ -198,7 +198,7 @@ cross_async_kernel(CrossArgs a) {
     // later steps' copies are outstanding; the barrier then covers every
     // wave's, and every wave is past step k-1, whose slot is refilled below
     wait_copies<LPW, NS>(nissued - k - 1);
-    __syncthreads();
+    ring_barrier();
     const int64_t valid = cc.rend - cc.r0;
     if (valid < RK) {  // last k-step of a segment: zero the rows past its end
       const int nz = (RK - (int)valid) * 16;
@@ -207,7 +207,7 @@ cross_async_kernel(CrossArgs a) {
         const int kr = (int)valid + rem / 16, un = rem % 16;
         *reinterpret_cast<uint4*>(cur + p * PLANE + kr * CR_TM + 8 * un) = uint4{0, 0, 0, 0};
       }
-      __syncthreads();
+      ring_barrier();
     }
     if (cl.s < a.nseg) {
       issue(cl, refill);

@@ -213,3 +213,17 @@ __device__ inline void wait_copies(int later) {
   else if (later == 0) __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
   else __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // deeper rings: conservative
 }
+
+// Workgroup barrier for the asynchronous-copy rings.  __syncthreads() is a
+// release/acquire fence plus s_barrier, and the compiler implements the
+// fence as s_waitcnt vmcnt(0): every copy still in flight (global -> LDS
+// copies count against vmcnt) would drain at each barrier.  The rings order
+// their copies with explicit vmcnt waits instead, so this barrier only
+// completes the wave's own LDS accesses (lgkmcnt(0)); the empty asm
+// statements keep the compiler from moving memory accesses across it.
+__device__ inline void ring_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt / expcnt untouched
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}

@@ -82,8 +82,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   // ---- copies: piece q of a slot (static), issued by wave q % 4 -----------
   const int cr = lane >> 2, cu = lane & 3;   // row in the 16-row piece, LDS chunk slot
   const int cch = cu ^ ((cr >> 2) & 3);      // global chunk held by that slot (lds_k<S, 2>)
-  auto issue = [&](int64_t t, int kt, lds_S* slot) __attribute__((always_inline)) {
-    const int64_t p0 = (t / nfb) * TP, f0 = (t % nfb) * TN, k0 = (int64_t)kt * RK;
+  auto issue = [&](int64_t tp, int64_t tf, int kt, lds_S* slot) __attribute__((always_inline)) {
+    const int64_t p0 = tp * TP, f0 = tf * TN, k0 = (int64_t)kt * RK;
 #pragma unroll
     for (int q = 0; q < 4 * PPW; ++q) {
       if ((q & 3) == wave) {
@@ -115,15 +115,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
     }
   };
 
-  int64_t lt = t_beg;  // copy cursor (tile, k-step)
+  int64_t lt = t_beg;  // copy cursor (tile, its point / feature block, k-step)
+  int64_t ltp = t_beg / nfb, ltf = t_beg % nfb;
   int lk = 0;
   int nissued = 0;
+  auto advance = [&]() __attribute__((always_inline)) {
+    if (++lk == nk) {
+      lk = 0;
+      ++lt;
+      if (++ltf == nfb) { ltf = 0; ++ltp; }
+    }
+  };
 #pragma unroll
   for (int s = 0; s < NS - 1; ++s)
     if (lt < t_end) {
-      issue(lt, lk, slots[s]);
+      issue(ltp, ltf, lk, slots[s]);
       ++nissued;
-      if (++lk == nk) { lk = 0; ++lt; }
+      advance();
     }
 
   acc_t acc[NA][FB][PB];
@@ -138,12 +146,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   zero_acc();
 
   int64_t ct = t_beg;  // compute cursor
+  int64_t ctp = t_beg / nfb, ctf = t_beg % nfb;
   int ck = 0, kdone = 0;
   // always inlined: the accumulators must stay in registers
   auto step = [&](lds_S* cur, lds_S* refill) __attribute__((always_inline)) -> bool {
     if (ct >= t_end) return false;
     wait_copies<PPW, NS>(nissued - kdone - 1);
-    __syncthreads();
+    ring_barrier();
     if (ck == nk - 1 && kpart < RK) {  // zero k >= the reduction length
       constexpr int ROWS = NW * TN + CI * TP;
       const int nz = RK - kpart;
@@ -151,12 +160,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
         const int row = e / nz, k = kpart + e % nz;
         cur[row * LDK + lds_k<S, 2>(row, k)] = (S)0.0f;
       }
-      __syncthreads();
+      ring_barrier();
     }
     if (lt < t_end) {  // refill the slot the previous step used
-      issue(lt, lk, refill);
+      issue(ltp, ltf, lk, refill);
       ++nissued;
-      if (++lk == nk) { lk = 0; ++lt; }
+      advance();
     }
     const lds_S* ws = cur;
     const lds_S* xs = cur + NW * WT;
@@ -186,7 +195,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
       }
     ++kdone;
     if (++ck == nk) {  // tile done: epilogue while the next tile's copies fly
-      const int64_t p0 = (ct / nfb) * TP, f0 = (ct % nfb) * TN;
+      const int64_t p0 = ctp * TP, f0 = ctf * TN;
       const int g = lane >> 4;
 #pragma unroll
       for (int j = 0; j < PB; ++j) {
@@ -210,6 +219,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
       zero_acc();
       ck = 0;
       ++ct;
+      if (++ctf == nfb) { ctf = 0; ++ctp; }
     }
     return true;
   };
