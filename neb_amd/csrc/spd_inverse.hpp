@@ -6,14 +6,20 @@
 // Replaces the FP64 Cholesky + triangular inverse of the spectral objective
 // (velocity_trace_jet2.py:109-112 `cholesky` / `solve`, velocity/core.py:
 // 52-56) for K up to a few thousand: the K^3 work is tiny, so the latency of
-// the factorization's sequential steps is what matters; here there are
-// 2 launches per 64 columns and each launch is fully parallel except the
-// 64 x 64 pivot inversion, which one workgroup does in LDS.
+// the sequential steps is what matters.  There is one launch per 64 columns
+// (plus one for the first pivot): launch k applies sweep k to every 64 x 64
+// tile, and the workgroup that produces tile (k+1, k+1) inverts it in the
+// same launch, so the next launch finds its pivot inverse ready.  Products
+// run on v_mfma_f64_16x16x4f64 from LDS tiles; the pivot inversion keeps its
+// block in MFMA registers and applies its rank-4 updates as MFMAs.
 //
 // Sweep of pivot block k (P = A_kk, Pi = P^-1):
 //   A_kk <- -Pi,  A_ik <- A_ik Pi,  A_kj <- Pi A_kj,  A_ij <- A_ij - A_ik Pi A_kj
-// Sweeping every block yields -A^-1.  The scalar version of the same sweep
-// inverts the pivot block inside the workgroup.
+// Sweeping every block yields -A^-1.  The matrix is swept padded to
+// Kp = 64 ceil(K / 64) with an identity block beyond K (its sweep leaves the
+// identity in place and the K x K part untouched), so every tile is full.
+// The sweeps keep the matrix symmetric, so A_kj is read as A_jk^T: every
+// operand is a tile of block column k, stored [row][k] in LDS.
 #pragma once
 #include "neb_common.hpp"
 
@@ -54,256 +60,332 @@ __device__ inline void spd_inv4(const double (&a)[4][4], double (&inv)[4][4], do
   inv[3][3] = (a[2][0] * s3 - a[2][1] * s1 + a[2][2] * s0) * id;
 }
 
-// Pivot block inversion: Pi = (A[k0:k0+nb, k0:k0+nb])^-1 by lazily scaled
-// sweeps over 4 x 4 pivot blocks.  Sweeping block b with pivot block P
-// (current values) is the rank-4 update
-//   v_ij -= x_i r_j,  x_i = v_(i,b) P^-1 (0 for rows of b),  r_j = v_(b,j)
-//   (0 for columns of b),
-// after which the diagonal block is negated; rows and columns of b are left
-// as they are.  The update commutes with left-multiplying a row block or
-// right-multiplying a column block by a 4 x 4 matrix, so the pending
-// factors P_b^-1 of every swept row / column block are applied once at the
-// end: -A^-1 = diag(P^-1) * v * diag(P^-1).  Each step's P is a block no
-// earlier step has transformed, so its leading minors are those of the
-// plain sweep (the positive-definiteness gate).  The 64 x 64 block lives in
-// registers (thread t owns row t/4, columns 16*(t%4) .. +15); each step
-// exchanges the pivot rows and columns through LDS (double-buffered: one
-// barrier per 4 columns).  Rows/columns >= nb are identity padding.  info:
-// 0 on success, else the 1-based global index of the first nonpositive /
-// nonfinite pivot (first writer wins).
-__global__ __launch_bounds__(256) void spd_pivot_kernel(const double* __restrict__ A, int64_t lda, int64_t k0,
-                                                        int nb, double* __restrict__ Pi, int* __restrict__ info,
-                                                        int reset) {
-  __shared__ __align__(16) double rowbuf[2][4][SPD_NB];
-  __shared__ __align__(16) double colbuf[2][SPD_NB][4];
-  __shared__ __align__(16) double pivs[SPD_NB / 4][16];
-  __shared__ __align__(16) double M[SPD_NB][SPD_NB + 2];
-  const int t = threadIdx.x;
-  const int i = t >> 2, jb = t & 3, jg = jb * 16;
-  if (reset && t == 0) *info = 0;  // first sweep clears the flag (no memset node)
-  double v[16];
+
+constexpr int SPD_LDP = SPD_NB + 4;         // LDS row stride (doubles): conflict-free 16-byte operand reads
+constexpr int SPD_TILE = SPD_NB * SPD_LDP;  // doubles per LDS tile
+// pivot inversion scratch: pivot rows [2][4][64], pivot columns [2][64][4],
+// the 16 block inverses [16][16] and the final 64 x 64 tile
+constexpr int SPD_PIVOT_LDS = 2 * 4 * SPD_NB + 2 * SPD_NB * 4 + 16 * 16 + SPD_TILE;
+
+// Register tile in MFMA layout: wave w owns the 32 x 32 quadrant at rows
+// 32 (w / 2), columns 32 (w % 2) as 2 x 2 tiles of 16 x 16; element e of
+// tile (tr, tc) in lane l is quadrant row 16 tr + l / 16 + 4 e, column
+// 16 tc + l % 16 (the v_mfma_f64_16x16x4f64 result layout).
+__device__ inline int spd_row(int R0, int tr, int lane, int e) { return R0 + 16 * tr + (lane >> 4) + 4 * e; }
+__device__ inline int spd_col(int C0, int tc, int lane) { return C0 + 16 * tc + (lane & 15); }
+
+// Thread t's share of a 64 x 64 tile (rows i0.., columns j0..): pair u holds
+// row (t + 256 u) / 32, columns 2 ((t + 256 u) % 32) + {0, 1}.  bounded: the
+// source is the caller's K x K matrix, read as identity beyond K; otherwise a
+// padded buffer with full tiles.
+struct SpdPart {
+  f64x2 v[8];
+};
+
+__device__ inline SpdPart spd_fetch(const double* __restrict__ src, int64_t ld, int64_t K, bool bounded, int64_t i0,
+                                    int64_t j0, int t) {
+  SpdPart p;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int j = jg + q;
-    v[q] = (i < nb && j < nb) ? A[(k0 + i) * lda + k0 + j] : (i == j ? 1.0 : 0.0);
+  for (int u = 0; u < 8; ++u) {
+    const int q = t + 256 * u;
+    const int64_t gi = i0 + (q >> 5), gj = j0 + 2 * (q & 31);
+    if (bounded) {
+      p.v[u][0] = (gi < K && gj < K) ? src[gi * ld + gj] : (gi == gj ? 1.0 : 0.0);
+      p.v[u][1] = (gi < K && gj + 1 < K) ? src[gi * ld + gj + 1] : (gi == gj + 1 ? 1.0 : 0.0);
+    } else {
+      p.v[u] = *reinterpret_cast<const f64x2*>(src + gi * ld + gj);
+    }
   }
+  return p;
+}
+
+__device__ inline void spd_put(double* L, const SpdPart& p, int t) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const int q = t + 256 * u;
+    *reinterpret_cast<f64x2*>(L + (q >> 5) * SPD_LDP + 2 * (q & 31)) = p.v[u];
+  }
+}
+
+// acc[tr][tc] += L[quadrant rows][0:64] * RT[quadrant columns][0:64]^T with
+// L and RT stored [row][k] in LDS.  MFMA pair p takes k = 8p + 2g + e in its
+// e-th instruction (lane group g = lane / 16) for both operands, so each lane
+// reads 16 bytes per operand per pair.
+__device__ inline void spd_mma(const double* L, const double* RT, int R0, int C0, int lane, f64x4 (&acc)[2][2]) {
+  const double* la = L + (R0 + (lane & 15)) * SPD_LDP + 2 * (lane >> 4);
+  const double* lb = RT + (C0 + (lane & 15)) * SPD_LDP + 2 * (lane >> 4);
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const f64x2 a0 = *reinterpret_cast<const f64x2*>(la + 8 * p);
+    const f64x2 a1 = *reinterpret_cast<const f64x2*>(la + 16 * SPD_LDP + 8 * p);
+    const f64x2 b0 = *reinterpret_cast<const f64x2*>(lb + 8 * p);
+    const f64x2 b1 = *reinterpret_cast<const f64x2*>(lb + 16 * SPD_LDP + 8 * p);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[e], b0[e], acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[e], b1[e], acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[e], b0[e], acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[e], b1[e], acc[1][1], 0, 0, 0);
+    }
+  }
+}
+
+__device__ inline void spd_acc_store(double* M, const f64x4 (&v)[2][2], int R0, int C0, int lane) {
+#pragma unroll
+  for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) M[spd_row(R0, tr, lane, e) * SPD_LDP + spd_col(C0, tc, lane)] = v[tr][tc][e];
+}
+
+// Inversion of the 64 x 64 pivot block held in MFMA layout (v) by lazily
+// scaled sweeps over 4 x 4 pivot blocks.  Sweeping block b (columns
+// c = 4b .. c+3) with pivot block P (current values) is the rank-4 update
+//   v_ij -= x_i r_j,  x_i = v_(i,b) P^-1 (0 for rows of b),
+//   r_j = v_(b,j) (0 for columns of b),
+// one MFMA per 16 x 16 tile with the four pivot columns as its k, after
+// which the diagonal block is negated; rows and columns of b are left as
+// they are.  The update commutes with left-multiplying a row block or
+// right-multiplying a column block by a 4 x 4 matrix, so the pending factors
+// P_b^-1 of every swept row / column block are applied once at the end:
+// -A^-1 = diag(P^-1) v diag(P^-1).  Each step's P is a block no earlier step
+// has transformed, so its leading minors are those of the plain sweep (the
+// positive-definiteness gate).  Pivot rows and columns pass through LDS,
+// double-buffered: one barrier per 4 columns.  lds: SPD_PIVOT_LDS doubles,
+// not in use by the caller.  Writes Pi = P^-1 (64 x 64, row stride 64).
+// info: the 1-based index k0 + i + 1 of the first nonpositive / nonfinite
+// pivot (first writer wins).
+__device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int lane, int t, int64_t k0,
+                                      double* __restrict__ Pi, int* __restrict__ info) {
+  double* rowbuf = lds;
+  double* colbuf = lds + 2 * 4 * SPD_NB;
+  double* pivs = colbuf + 2 * SPD_NB * 4;
+  double* M = pivs + 16 * 16;
+  const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+  const int g = lane >> 4, r16 = lane & 15;
   int bad = 0;
-  for (int c0 = 0; c0 < nb; c0 += 16) {
-    const bool own = (c0 >> 4) == jb;  // this thread holds columns c0 .. c0+15
+  for (int b4 = 0; b4 < 4; ++b4) {
+    const int hq = b4 >> 1;  // quadrant half holding these 16 rows / columns
+    const bool tq = b4 & 1;  // 16 x 16 tile inside that half
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-      const int c = c0 + 4 * s, u = 4 * s, b = s & 1;
-      if (c < nb) {  // uniform over the workgroup
-        const bool inblk = i >= c && i < c + 4;
-        if (inblk) {
+      const int b = 4 * b4 + s, c = 4 * b;
+      double* rb = rowbuf + (b & 1) * 4 * SPD_NB;
+      double* cb = colbuf + (b & 1) * SPD_NB * 4;
+      if ((w >> 1) == hq) {  // pivot row c + g is element s of tile row tq
 #pragma unroll
-          for (int q = 0; q < 16; q += 2)
-            *reinterpret_cast<f64x2*>(&rowbuf[b][i - c][jg + q]) =
-                f64x2{(own && q >= u && q < u + 4) ? 0.0 : v[q], (own && q + 1 >= u && q + 1 < u + 4) ? 0.0 : v[q + 1]};
-        }
-        if (own) {
-          *reinterpret_cast<f64x2*>(&colbuf[b][i][0]) = f64x2{v[u], v[u + 1]};
-          *reinterpret_cast<f64x2*>(&colbuf[b][i][2]) = f64x2{v[u + 2], v[u + 3]};
-        }
-        __syncthreads();
-        double P[4][4], Q[4][4], mn[4];
+        for (int tc = 0; tc < 2; ++tc) rb[g * SPD_NB + spd_col(C0, tc, lane)] = tq ? v[1][tc][s] : v[0][tc][s];
+      }
+      if ((w & 1) == hq && (r16 >> 2) == s) {  // pivot column c + r16 % 4 is lane column r16 of tile column tq
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const f64x2 p01 = *reinterpret_cast<const f64x2*>(&colbuf[b][c + r][0]);
-          const f64x2 p23 = *reinterpret_cast<const f64x2*>(&colbuf[b][c + r][2]);
-          P[r][0] = p01[0]; P[r][1] = p01[1]; P[r][2] = p23[0]; P[r][3] = p23[1];
-        }
-        spd_inv4(P, Q, mn);
-        bool ok = true;
-        int first = 4;
+        for (int tr = 0; tr < 2; ++tr)
 #pragma unroll
-        for (int r = 3; r >= 0; --r)
-          if (!(mn[r] > 0.0) || !isfinite(mn[r])) { ok = false; first = r; }
-        if (!ok) {
-          if (bad == 0) bad = (int)(k0 + c + first + 1);
+          for (int e = 0; e < 4; ++e) cb[spd_row(R0, tr, lane, e) * 4 + (r16 & 3)] = tq ? v[tr][1][e] : v[tr][0][e];
+      }
+      __syncthreads();
+      double P[4][4], Q[4][4], mn[4];
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+      for (int r = 0; r < 4; ++r) {
+        const f64x2 p01 = *reinterpret_cast<const f64x2*>(&cb[(c + r) * 4]);
+        const f64x2 p23 = *reinterpret_cast<const f64x2*>(&cb[(c + r) * 4 + 2]);
+        P[r][0] = p01[0]; P[r][1] = p01[1]; P[r][2] = p23[0]; P[r][3] = p23[1];
+      }
+      spd_inv4(P, Q, mn);
+      bool ok = true;
+      int first = 4;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) Q[r][q] = r == q ? 1.0 : 0.0;  // keep the arithmetic finite
-        }
-        const f64x2 a01 = *reinterpret_cast<const f64x2*>(&colbuf[b][i][0]);
-        const f64x2 a23 = *reinterpret_cast<const f64x2*>(&colbuf[b][i][2]);
-        const double ai[4] = {a01[0], a01[1], a23[0], a23[1]};
-        double x[4];
+      for (int r = 3; r >= 0; --r)
+        if (!(mn[r] > 0.0) || !isfinite(mn[r])) { ok = false; first = r; }
+      if (!ok) {
+        if (bad == 0) bad = (int)(k0 + c + first + 1);
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const double xm = ai[0] * Q[0][m] + ai[1] * Q[1][m] + ai[2] * Q[2][m] + ai[3] * Q[3][m];
-          x[m] = inblk ? 0.0 : xm;
-        }
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
+          for (int q = 0; q < 4; ++q) Q[r][q] = r == q ? 1.0 : 0.0;  // keep the arithmetic finite
+      }
+      // k of the MFMA = lane group g: x_i[g] = v_(i,b) P^-1[:, g], -r_g(j)
+      double qc[4];
 #pragma unroll
-          for (int q = 0; q < 16; q += 2) {
-            const f64x2 rv = *reinterpret_cast<const f64x2*>(&rowbuf[b][m][jg + q]);
-            v[q] = fma(-x[m], rv[0], v[q]);
-            v[q + 1] = fma(-x[m], rv[1], v[q + 1]);
-          }
-        }
-        if (own && inblk) {
+      for (int a = 0; a < 4; ++a) qc[a] = g == 0 ? Q[a][0] : g == 1 ? Q[a][1] : g == 2 ? Q[a][2] : Q[a][3];
+      double x[2], y[2];
 #pragma unroll
-          for (int k = 0; k < 4; ++k) v[u + k] = -v[u + k];
-        }
-        if (t == 0) {
+      for (int tr = 0; tr < 2; ++tr) {
+        const int i = R0 + 16 * tr + r16;
+        const f64x2 a01 = *reinterpret_cast<const f64x2*>(&cb[i * 4]);
+        const f64x2 a23 = *reinterpret_cast<const f64x2*>(&cb[i * 4 + 2]);
+        const double xv = a01[0] * qc[0] + a01[1] * qc[1] + a23[0] * qc[2] + a23[1] * qc[3];
+        x[tr] = (i >> 2) == b ? 0.0 : xv;
+      }
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+      for (int tc = 0; tc < 2; ++tc) {
+        const int j = spd_col(C0, tc, lane);
+        y[tc] = (j >> 2) == b ? 0.0 : -rb[g * SPD_NB + j];
+      }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) pivs[c >> 2][4 * r + q] = Q[r][q];
-        }
+      for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) v[tr][tc] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[tr], y[tc], v[tr][tc], 0, 0, 0);
+      if ((w >> 1) == hq && (w & 1) == hq && (r16 >> 2) == s) {  // negate the diagonal pivot block
+        if (tq) v[1][1][s] = -v[1][1][s];
+        else v[0][0][s] = -v[0][0][s];
+      }
+      if (t == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pivs[b * 16 + 4 * r + q] = Q[r][q];
       }
     }
   }
   // pending factors: T = diag(P^-1) v (rows) through LDS, then T diag(P^-1)
   // (columns) in registers; Pi = -T
-#pragma unroll
-  for (int q = 0; q < 16; q += 2) *reinterpret_cast<f64x2*>(&M[i][jg + q]) = f64x2{v[q], v[q + 1]};
+  spd_acc_store(M, v, R0, C0, lane);
   __syncthreads();
-  const int nbk = (nb + 3) >> 2;
+  const int i = t >> 2, jg = (t & 3) * 16;
   const int I = i >> 2, ir = i & 3;
-  double tr[16];
-  if (I < nbk) {
-    double li[4];
+  double li[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) li[a] = pivs[I][4 * ir + a];
+  for (int a = 0; a < 4; ++a) li[a] = pivs[I * 16 + 4 * ir + a];
+  double tr_[16];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      double acc = 0.0;
+  for (int q = 0; q < 16; ++q) {
+    double acc = 0.0;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) acc = fma(li[a], M[4 * I + a][jg + q], acc);
-      tr[q] = acc;
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) tr[q] = 0.0;
+    for (int a = 0; a < 4; ++a) acc = fma(li[a], M[(4 * I + a) * SPD_LDP + jg + q], acc);
+    tr_[q] = acc;
   }
 #pragma unroll
   for (int J = 0; J < 4; ++J) {
-    const int Jg = (jg >> 2) + J;  // global column block
+    const int Jg = (jg >> 2) + J;  // column block
     double o[4] = {0.0, 0.0, 0.0, 0.0};
-    if (Jg < nbk) {
 #pragma unroll
-      for (int bq = 0; bq < 4; ++bq)
+    for (int bq = 0; bq < 4; ++bq)
 #pragma unroll
-        for (int cq = 0; cq < 4; ++cq) o[cq] = fma(tr[4 * J + bq], pivs[Jg][4 * bq + cq], o[cq]);
-    }
-#pragma unroll
-    for (int cq = 0; cq < 4; ++cq) {
-      const int j = jg + 4 * J + cq;
-      Pi[i * SPD_NB + j] = (i < nb && j < nb) ? -o[cq] : 0.0;
-    }
+      for (int cq = 0; cq < 4; ++cq) o[cq] = fma(tr_[4 * J + bq], pivs[Jg * 16 + 4 * bq + cq], o[cq]);
+    *reinterpret_cast<f64x2*>(&Pi[i * SPD_NB + jg + 4 * J]) = f64x2{-o[0], -o[1]};
+    *reinterpret_cast<f64x2*>(&Pi[i * SPD_NB + jg + 4 * J + 2]) = f64x2{-o[2], -o[3]};
   }
   if (t == 0 && bad != 0) atomicCAS(info, 0, bad);
 }
 
-// acc[a][b] += sum_q LT[q][ti+a] * R[q][tj+b]  (LT = left operand transposed)
-__device__ inline void spd_mm(const double (*LT)[SPD_NB + 2], const double (*R)[SPD_NB + 2], int ti, int tj, int nb,
-                              double (&acc)[4][4]) {
-  for (int q = 0; q < nb; ++q) {
-    const f64x2 l01 = *reinterpret_cast<const f64x2*>(&LT[q][ti]);
-    const f64x2 l23 = *reinterpret_cast<const f64x2*>(&LT[q][ti + 2]);
-    const f64x2 r01 = *reinterpret_cast<const f64x2*>(&R[q][tj]);
-    const f64x2 r23 = *reinterpret_cast<const f64x2*>(&R[q][tj + 2]);
-    const double l[4] = {l01[0], l01[1], l23[0], l23[1]};
-    const double r[4] = {r01[0], r01[1], r23[0], r23[1]};
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] += l[a] * r[b];
-  }
-}
-
-// One block sweep over the whole matrix: B = sweep_k(A) (ping-pong buffers),
-// scaled by `out_scale` (-1 on the last sweep turns -A^-1 into A^-1).
-// 64 x 64 output tile per workgroup, 4 x 4 outputs per thread; LDS operands
-// are stored [q][row] so every inner-product step reads 16-byte vectors.
-__global__ __launch_bounds__(256) void spd_update_kernel(const double* __restrict__ A, int64_t lda, int64_t K,
-                                                         int64_t k0, int nb, const double* __restrict__ Pi,
-                                                         double* __restrict__ B, int64_t ldb, double out_scale) {
+// First pivot: Pi = (A[0:64, 0:64], identity beyond K)^-1; clears info.
+__global__ __launch_bounds__(256) void spd_pivot0_kernel(const double* __restrict__ A, int64_t lda, int64_t K,
+                                                         double* __restrict__ Pi, int* __restrict__ info) {
   extern __shared__ __align__(16) double spd_smem[];
-  double(*L0)[SPD_NB + 2] = reinterpret_cast<double(*)[SPD_NB + 2]>(spd_smem);
-  double(*L1)[SPD_NB + 2] = reinterpret_cast<double(*)[SPD_NB + 2]>(spd_smem + SPD_NB * (SPD_NB + 2));
-  const int64_t i0 = (int64_t)blockIdx.y * SPD_NB, j0 = (int64_t)blockIdx.x * SPD_NB;
-  const int t = threadIdx.x;
-  const int ti = (t / 16) * 4, tj = (t % 16) * 4;  // this thread's 4x4 outputs
-  const bool ipiv = i0 == k0, jpiv = j0 == k0;
-  double acc[4][4];
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+  if (t == 0) *info = 0;  // the first launch of an inversion clears the flag (no memset node)
+  spd_put(spd_smem, spd_fetch(A, lda, K, true, 0, 0, t), t);
+  __syncthreads();
+  f64x4 v[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int tr = 0; tr < 2; ++tr)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-
-  if (ipiv && jpiv) {
+    for (int tc = 0; tc < 2; ++tc)
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[a][b] = -Pi[(ti + a) * SPD_NB + tj + b];
-  } else if (jpiv) {
-    // A_ik Pi : LT[q][r] = A[i0+r][k0+q], R = Pi
-    for (int e = t; e < SPD_NB * SPD_NB; e += 256) {
-      const int r = e / SPD_NB, c = e % SPD_NB;
-      L0[c][r] = (i0 + r < K && c < nb) ? A[(i0 + r) * lda + k0 + c] : 0.0;
-      L1[r][c] = Pi[e];
-    }
-    __syncthreads();
-    spd_mm(L0, L1, ti, tj, nb, acc);
-  } else if (ipiv) {
-    // Pi A_kj : LT = Pi^T = Pi (symmetric), R[q][c] = A[k0+q][j0+c]
-    for (int e = t; e < SPD_NB * SPD_NB; e += 256) {
-      const int r = e / SPD_NB, c = e % SPD_NB;
-      L0[r][c] = Pi[e];
-      L1[r][c] = (r < nb && j0 + c < K) ? A[(k0 + r) * lda + j0 + c] : 0.0;
-    }
-    __syncthreads();
-    spd_mm(L0, L1, ti, tj, nb, acc);
-  } else {
-    // A_ij - (A_ik Pi) A_kj
-    for (int e = t; e < SPD_NB * SPD_NB; e += 256) {
-      const int r = e / SPD_NB, c = e % SPD_NB;
-      L0[c][r] = (i0 + r < K && c < nb) ? A[(i0 + r) * lda + k0 + c] : 0.0;
-      L1[r][c] = Pi[e];
-    }
-    __syncthreads();
-    double tt[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) tt[a][b] = 0.0;
-    spd_mm(L0, L1, ti, tj, nb, tt);  // T[ti+a][tj+b] = (A_ik Pi)
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) L0[tj + b][ti + a] = tt[a][b];  // LT[q][r] = T[r][q]
-    for (int e = t; e < SPD_NB * SPD_NB; e += 256) {
-      const int r = e / SPD_NB, c = e % SPD_NB;
-      L1[r][c] = (r < nb && j0 + c < K) ? A[(k0 + r) * lda + j0 + c] : 0.0;
-    }
-    __syncthreads();
-    double upd[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) upd[a][b] = 0.0;
-    spd_mm(L0, L1, ti, tj, nb, upd);
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int64_t gi = i0 + ti + a, gj = j0 + tj + b;
-        acc[a][b] = ((gi < K && gj < K) ? A[gi * lda + gj] : 0.0) - upd[a][b];
-      }
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int64_t gi = i0 + ti + a, gj = j0 + tj + b;
-      if (gi < K && gj < K) B[gi * ldb + gj] = out_scale * acc[a][b];
-    }
+      for (int e = 0; e < 4; ++e)
+        v[tr][tc][e] = spd_smem[spd_row(R0, tr, lane, e) * SPD_LDP + spd_col(C0, tc, lane)];
+  __syncthreads();
+  spd_pivot_regs(v, spd_smem, w, lane, t, 0, Pi, info);
 }
 
-constexpr size_t spd_update_lds_bytes() { return 2 * sizeof(double) * SPD_NB * (SPD_NB + 2); }
+// Sweep k on tile (blockIdx.y, blockIdx.x) of the padded matrix, src -> dst;
+// on the last sweep dst is X: the tile is negated (-A^-1 -> A^-1) and
+// clipped to K x K.  The workgroup of tile (k+1, k+1) inverts its new value
+// as the next pivot (Pi_next) instead of storing it: sweep k+1 replaces that
+// tile by -Pi_next without reading it.
+__global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict__ src, int64_t lds, int bounded,
+                                                       int64_t K, int k, const double* __restrict__ Pi,
+                                                       double* __restrict__ Pi_next, double* __restrict__ dst,
+                                                       int64_t ldd, int last, int* __restrict__ info) {
+  extern __shared__ __align__(16) double spd_smem[];
+  double* L0 = spd_smem;
+  double* L1 = L0 + SPD_TILE;
+  double* L2 = L1 + SPD_TILE;
+  const int t = threadIdx.x, lane = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+  const int ti = blockIdx.y, tj = blockIdx.x;
+  const int64_t i0 = (int64_t)ti * SPD_NB, j0 = (int64_t)tj * SPD_NB, k0 = (int64_t)k * SPD_NB;
+  const bool bnd = bounded != 0;
+  f64x4 acc[2][2];
+#pragma unroll
+  for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc) acc[tr][tc] = f64x4{0.0, 0.0, 0.0, 0.0};
+  if (ti == k && tj == k) {  // A_kk <- -Pi
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+      for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[tr][tc][e] = -Pi[spd_row(R0, tr, lane, e) * SPD_NB + spd_col(C0, tc, lane)];
+  } else if (tj == k) {  // A_ik Pi  (RT = Pi^T = Pi)
+    const SpdPart pa = spd_fetch(src, lds, K, bnd, i0, k0, t);
+    const SpdPart pp = spd_fetch(Pi, SPD_NB, SPD_NB, false, 0, 0, t);
+    spd_put(L0, pa, t);
+    spd_put(L1, pp, t);
+    __syncthreads();
+    spd_mma(L0, L1, R0, C0, lane, acc);
+  } else if (ti == k) {  // Pi A_kj = Pi A_jk^T
+    const SpdPart pp = spd_fetch(Pi, SPD_NB, SPD_NB, false, 0, 0, t);
+    const SpdPart pb = spd_fetch(src, lds, K, bnd, j0, k0, t);
+    spd_put(L0, pp, t);
+    spd_put(L1, pb, t);
+    __syncthreads();
+    spd_mma(L0, L1, R0, C0, lane, acc);
+  } else {  // A_ij - (A_ik Pi) A_jk^T
+    const SpdPart pa = spd_fetch(src, lds, K, bnd, i0, k0, t);
+    const SpdPart pp = spd_fetch(Pi, SPD_NB, SPD_NB, false, 0, 0, t);
+    const SpdPart pb = spd_fetch(src, lds, K, bnd, j0, k0, t);
+    f64x4 aij[2][2];
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+      for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t gi = i0 + spd_row(R0, tr, lane, e), gj = j0 + spd_col(C0, tc, lane);
+          aij[tr][tc][e] = !bnd ? src[gi * lds + gj] : (gi < K && gj < K) ? src[gi * lds + gj] : (gi == gj ? 1.0 : 0.0);
+        }
+    spd_put(L0, pa, t);
+    spd_put(L1, pp, t);
+    spd_put(L2, pb, t);
+    __syncthreads();
+    f64x4 tt[2][2];
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+      for (int tc = 0; tc < 2; ++tc) tt[tr][tc] = f64x4{0.0, 0.0, 0.0, 0.0};
+    spd_mma(L0, L1, R0, C0, lane, tt);  // T = A_ik Pi
+    __syncthreads();                     // every wave is done reading L0
+    spd_acc_store(L0, tt, R0, C0, lane);
+    __syncthreads();
+    spd_mma(L0, L2, R0, C0, lane, acc);  // T A_jk^T
+#pragma unroll
+    for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+      for (int tc = 0; tc < 2; ++tc) acc[tr][tc] = aij[tr][tc] - acc[tr][tc];
+    if (ti == k + 1 && tj == k + 1) {
+      __syncthreads();  // the pivot inversion reuses the LDS tiles
+      spd_pivot_regs(acc, spd_smem, w, lane, t, (int64_t)(k + 1) * SPD_NB, Pi_next, info);
+      return;
+    }
+  }
+#pragma unroll
+  for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t gi = i0 + spd_row(R0, tr, lane, e), gj = j0 + spd_col(C0, tc, lane);
+        if (!last) dst[gi * ldd + gj] = acc[tr][tc][e];
+        else if (gi < K && gj < K) dst[gi * ldd + gj] = -acc[tr][tc][e];
+      }
+}
+
+constexpr size_t spd_step_lds_bytes() { return 3 * sizeof(double) * SPD_TILE; }
+constexpr size_t spd_pivot0_lds_bytes() { return sizeof(double) * SPD_PIVOT_LDS; }
