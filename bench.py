@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host oracle baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 comparison step")
+    ap.add_argument("--step-only", action="store_true",
+                    help="skip the Gram-mode statistics and the FP64 accuracy legs (profiler runs)")
     ap.add_argument("--kernel", default=None,
                     choices=["hidden_fwd", "hidden_bwd", "out_fields", "out_bwd", "gram", "gram_f32in", "cross_dw",
                              "copy"],
@@ -79,7 +81,7 @@ class StageTimer:
 
 def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=None):
     K = net.cfg.modes
-    solver = solver if solver is not None else sp.SpectralSolver(1e-2, ridge)
+    solver = solver if solver is not None else sp.SpectralSolver(1e-2, ridge, deferred=True)
 
     def mark(fn, name):
         if timer is None:
@@ -97,8 +99,10 @@ def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=
         M = mark(lambda: nw.field_gram(fields, "mass", w, gram_mode), "gram_mass")
         S = mark(lambda: nw.field_gram(fields, "stiff", w, gram_mode), "gram_stiff")
         loss, dM, dS = mark(lambda: solver(M, S), "objective_fp64")
-        g = mark(lambda: nw.backward_fields(net, tape, fields, dM, dS, w), "reverse")
-        mark(lambda: net.adam(g, 1e-4), "adam")
+        g = mark(lambda: nw.backward_fields(net, tape, fields, dM, dS, w, symmetric=True), "reverse")
+        # the positive-definiteness / condition gates and the gradient gate are
+        # evaluated on the device; adam synchronises once, after queueing
+        mark(lambda: net.adam(g, 1e-4, gate=solver.gate, on_gate=solver.raise_gate), "adam")
         return loss
 
     return step
@@ -298,17 +302,17 @@ def bench_3d(args):
         t = e0.elapsed_time(e1) * 1e-3
         gram_stats[mode] = dict(tflops=(12 * CHUNK3D) * K * (K + 1) / t / 1e12, ms=t * 1e3)
     del f0, s0, G
-    solver = sp.SpectralSolver(1e-2, ridge)
+    solver = sp.SpectralSolver(1e-2, ridge, deferred=True)
 
     def step(prec, gmode, solv):
         tt = {}
         res = sp.chunked_step(net, x, w, prec, gmode, CHUNK3D, solv, True, tt)
-        net.adam(res["grads"], 1e-4)
+        net.adam(res["grads"], 1e-4, gate=solv.gate, on_gate=solv.raise_gate)
         return res, tt
 
     f32_rate = None
     if not args.no_f32:
-        s32 = sp.SpectralSolver(1e-2, ridge)
+        s32 = sp.SpectralSolver(1e-2, ridge, deferred=True)
         step("f32", "f32", s32)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -435,6 +439,76 @@ def bench_3d_eig(args):
     print(json.dumps(out), flush=True)
 
 
+def gram_mode_stats(nw, fields, split, K, w):
+    """TFLOP/s and error vs FP64 of the Gram modes on the same fields:
+    "bf16x3-split" is the step's path (pre-split bf16 planes), "bf16x3" splits
+    the fp32 fields per tile (same bits), "f32" is the FP32 MFMA Gram."""
+    import torch
+    phi64 = fields.mass[:, :K].double().contiguous()
+    G64 = nw.gram(phi64, K, w, "f64")
+    del phi64
+    gram_stats = {}
+    Rm = fields.mass.shape[0]
+    flops = Rm * K * (K + 1)
+    runs = {"bf16x3-split": lambda out: nw.field_gram(split, "mass", w, "bf16x3", out=out),
+            "bf16x3": lambda out: nw.gram(fields.mass, K, w, "bf16x3", out=out),
+            "f32": lambda out: nw.gram(fields.mass, K, w, "f32", out=out)}
+    results = {}
+    for mode, run in runs.items():
+        G = run(None)
+        results[mode] = G
+        err = float(torch.linalg.norm(G - G64) / torch.linalg.norm(G64))
+        torch.cuda.synchronize()
+        reps = 5
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run(G)
+        e1.record()
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / reps * 1e-3
+        gram_stats[mode] = dict(tflops=flops / t / 1e12, ms=t * 1e3, rel_err_vs_fp64=err)
+    gram_stats["split_vs_split_on_load_rel_diff"] = float(
+        torch.linalg.norm(results["bf16x3-split"] - results["bf16x3"]) / torch.linalg.norm(results["bf16x3"]))
+    return gram_stats
+
+
+def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
+    """End-to-end errors of the bf16 and fp32 paths against the FP64 path
+    (FP64 jets, fields and Grams) on the same grid and weights."""
+    import torch
+    dev = x.device
+    tape64 = nw.trunk_forward(net, x, nw.PRECISIONS["f64"])
+    f64f = nw.output_fields(net, tape64)
+    del tape64
+    M64 = nw.gram(f64f.mass, K, w, "f64")
+    S64 = nw.gram(f64f.stiff, K, w, "f64")
+    eye = torch.eye(K, dtype=torch.float64, device=dev)
+    lam64, _ = sp.generalized_eigh(S64, M64 + ridge * eye)
+    top = WIDTHS[-1]  # rank of M: the physical eigenvalues are the top `top`; the rest are ridge-dominated
+    accuracy = {}
+    for prec in ("bf16", "f32"):
+        tp = nw.trunk_forward(net, x, nw.PRECISIONS[prec])
+        fp = nw.output_fields(net, tp, split=(prec == "bf16"))
+        del tp
+        gm = "bf16x3" if prec == "bf16" else "f32"
+        Mp = nw.field_gram(fp, "mass", w, gm)
+        Sp = nw.field_gram(fp, "stiff", w, gm)
+        U = fp.mass[:, :K].float() + (fp.mass_lo[:, :K].float() if fp.split else 0.0)
+        vel = float(torch.linalg.norm(U.double() - f64f.mass[:, :K]) / torch.linalg.norm(f64f.mass[:, :K]))
+        lam, _ = sp.generalized_eigh(Sp, Mp + ridge * eye)
+        accuracy[prec] = dict(velocity_rel_l2=vel,
+                              mass_gram_rel_err=float(torch.linalg.norm(Mp - M64) / torch.linalg.norm(M64)),
+                              stiffness_gram_rel_err=float(torch.linalg.norm(Sp - S64) / torch.linalg.norm(S64)),
+                              eigenvalue_max_rel_err_top=float(((lam[-top:] - lam64[-top:]).abs()
+                                                                / lam64[-top:].abs()).max()))
+        del fp, U
+    accuracy["note"] = (f"vs the FP64 path on the same grid and weights; eigenvalues of S x = lambda (M + ridge I) x, "
+                        f"the top {top} (= rank M; the others are set by the ridge); bf16 velocities are the hi + lo "
+                        f"planes of the fp32 field values")
+    return accuracy
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -479,64 +553,11 @@ def main():
     # ---- Gram modes on the same fields: TFLOP/s and error vs FP64 ----------
     # "bf16x3-split": the step's path (pre-split bf16 planes); "bf16x3" splits
     # the fp32 fields per tile; both give the same bits.
-    phi64 = fields.mass[:, :K].double().contiguous()
-    G64 = nw.gram(phi64, K, w, "f64")
-    del phi64
-    gram_stats = {}
-    Rm = fields.mass.shape[0]
-    flops = Rm * K * (K + 1)
-    runs = {"bf16x3-split": lambda out: nw.field_gram(split, "mass", w, "bf16x3", out=out),
-            "bf16x3": lambda out: nw.gram(fields.mass, K, w, "bf16x3", out=out),
-            "f32": lambda out: nw.gram(fields.mass, K, w, "f32", out=out)}
-    results = {}
-    for mode, run in runs.items():
-        G = run(None)
-        results[mode] = G
-        err = float(torch.linalg.norm(G - G64) / torch.linalg.norm(G64))
-        torch.cuda.synchronize()
-        reps = 5
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            run(G)
-        e1.record()
-        torch.cuda.synchronize()
-        t = e0.elapsed_time(e1) / reps * 1e-3
-        gram_stats[mode] = dict(tflops=flops / t / 1e12, ms=t * 1e3, rel_err_vs_fp64=err)
-    gram_stats["split_vs_split_on_load_rel_diff"] = float(
-        torch.linalg.norm(results["bf16x3-split"] - results["bf16x3"]) / torch.linalg.norm(results["bf16x3"]))
-    del G64, results
-
-    # ---- end-to-end errors against the FP64 path (FP64 jets, fields, Grams) --
-    tape64 = nw.trunk_forward(net, x, nw.PRECISIONS["f64"])
-    f64f = nw.output_fields(net, tape64)
-    del tape64
-    M64 = nw.gram(f64f.mass, K, w, "f64")
-    S64 = nw.gram(f64f.stiff, K, w, "f64")
-    eye = torch.eye(K, dtype=torch.float64, device=dev)
-    lam64, _ = sp.generalized_eigh(S64, M64 + ridge * eye)
-    top = WIDTHS[-1]  # rank of M: the physical eigenvalues are the top `top`; the rest are ridge-dominated
-    accuracy = {}
-    for prec in ("bf16", "f32"):
-        tp = nw.trunk_forward(net, x, nw.PRECISIONS[prec])
-        fp = nw.output_fields(net, tp, split=(prec == "bf16"))
-        del tp
-        gm = "bf16x3" if prec == "bf16" else "f32"
-        Mp = nw.field_gram(fp, "mass", w, gm)
-        Sp = nw.field_gram(fp, "stiff", w, gm)
-        U = fp.mass[:, :K].float() + (fp.mass_lo[:, :K].float() if fp.split else 0.0)
-        vel = float(torch.linalg.norm(U.double() - f64f.mass[:, :K]) / torch.linalg.norm(f64f.mass[:, :K]))
-        lam, _ = sp.generalized_eigh(Sp, Mp + ridge * eye)
-        accuracy[prec] = dict(velocity_rel_l2=vel,
-                              mass_gram_rel_err=float(torch.linalg.norm(Mp - M64) / torch.linalg.norm(M64)),
-                              stiffness_gram_rel_err=float(torch.linalg.norm(Sp - S64) / torch.linalg.norm(S64)),
-                              eigenvalue_max_rel_err_top=float(((lam[-top:] - lam64[-top:]).abs()
-                                                                / lam64[-top:].abs()).max()))
-        del fp, U
-    accuracy["note"] = (f"vs the FP64 path on the same grid and weights; eigenvalues of S x = lambda (M + ridge I) x, "
-                        f"the top {top} (= rank M; the others are set by the ridge); bf16 velocities are the hi + lo "
-                        f"planes of the fp32 field values")
-    del f64f, M64, S64, tape, fields, split
+    gram_stats, accuracy = {}, {}
+    if not args.step_only:
+        gram_stats = gram_mode_stats(nw, fields, split, K, w)
+        accuracy = accuracy_vs_fp64(nw, sp, net, x, K, w, ridge)
+    del tape, fields, split
 
     # ---- FP32 comparison step ------------------------------------------------
     f32_rate = None
@@ -553,7 +574,7 @@ def main():
         f32_rate = N * K / ((time.perf_counter() - t0) / n32)
 
     # ---- headline BF16 step --------------------------------------------------
-    solver = sp.SpectralSolver(1e-2, ridge)
+    solver = sp.SpectralSolver(1e-2, ridge, deferred=True)
     step = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, solver=solver)
     for _ in range(args.warmup):
         step()

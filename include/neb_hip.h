@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 2
+#define NEB_ABI_VERSION 3
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -178,6 +178,41 @@ int neb_sumsq(int dtype, const void* x, int64_t n, double* out, void* workspace,
  * (NCC:156-171) / adam_step_jet1_device (HB2P:2184-2278). */
 int neb_adam(int dtype, void* p, const void* g, void* m, void* v, int64_t n, double lr, double scale, double b1,
              double b2, double eps, int64_t t, void* stream);
+
+/* Adam step as neb_adam with the step's gates evaluated on the device, so a
+ * training step needs no host synchronisation before the update is queued:
+ * the clip factor min(1, clip/|g|) is formed from the device squared norm
+ * *norm2 (neb_sumsq), and p, m, v are left untouched when |g| is zero or
+ * nonfinite or when *gate (device int, nullable: an earlier gate of the same
+ * step, e.g. the mass positive-definiteness flag) is nonzero; the host reads
+ * *norm2 and *gate afterwards to raise.  Replaces Model.adam (NCC:156-171:
+ * clip_grad_norm_ + the FloatingPointError gate + Adam). */
+int neb_adam_gated(int dtype, void* p, const void* g, void* m, void* v, int64_t n, double lr, double clip,
+                   double b1, double b2, double eps, int64_t t, const double* norm2, const int* gate, void* stream);
+
+/* B = (A + A^T)/2 + shift I for K x K FP64 matrices (B may alias A): the
+ * ridge-shifted symmetric mass M_r of the spectral objective
+ * (MB/velocity_trace_jet2.py:107-109, `0.5 * (M + M.T)` + ridge). */
+int neb_sym_shift(int64_t K, const double* A, int64_t lda, double shift, double* B, int64_t ldb, void* stream);
+
+/* Terms of the spectral objective L = <X, sym(S)> + lam ||M - I||_F^2 with
+ * X = M_r^-1 (MB/velocity_trace_jet2.py:107-117; the trace tr(M_r^-1 S) as
+ * the Frobenius product): symmetrises X in place, writes Ss = sym(S), and
+ * writes the device scalars *loss and *gate.  *gate = *info when info is
+ * non-null and nonzero (first nonpositive pivot of neb_spd_inverse), else -1
+ * when the condition estimate ||M_r||_F ||X||_F (M_r = sym(M) + ridge I) is
+ * nonfinite or above cond_limit, else 0: the gates of
+ * MB/velocity_trace_jet2.py:109-111 as a device flag for neb_adam_gated.
+ * Deterministic.  Workspace: neb_spectral_workspace_size(K). */
+size_t neb_spectral_workspace_size(int64_t K);
+int neb_spectral_terms(int64_t K, const double* M, int64_t ldm, const double* S, int64_t lds, double* X,
+                       int64_t ldx, const int* info, double lam, double ridge, double cond_limit, double* Ss,
+                       int64_t ldss, double* loss, int* gate, void* workspace, size_t ws_bytes, void* stream);
+
+/* Mass adjoint of the spectral objective, dL/dM = -sym(C) + 2 lam (M - I)
+ * with C = X Ss X (MB/velocity_trace_jet2.py:113-117).  dM may alias C, not M. */
+int neb_spectral_adjoint(int64_t K, const double* C, int64_t ldc, const double* M, int64_t ldm, double lam, double* dM,
+                         int64_t ldd, void* stream);
 
 /* FP64 inverse of a symmetric positive definite K x K matrix (blocked
  * Gauss-Jordan sweeps, 64-column pivot blocks).  X must not alias A.  *info

@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
@@ -46,7 +46,13 @@ SIGNATURES = {
     "neb_colsum": (_int, [_int, _vp, _i64, _i64, _i64, _dbl, _vp, _vp, _sz, _vp]),
     "neb_sumsq": (_int, [_int, _vp, _i64, _vp, _vp, _sz, _vp]),
     "neb_adam": (_int, [_int, _vp, _vp, _vp, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _i64, _vp]),
+    "neb_adam_gated": (_int, [_int, _vp, _vp, _vp, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _i64, _vp, _vp, _vp]),
     "neb_stage": (_int, [_int, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _int, _dbl, _vp]),
+    "neb_sym_shift": (_int, [_i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
+    "neb_spectral_workspace_size": (_sz, [_i64]),
+    "neb_spectral_terms": (_int, [_i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _dbl, _dbl, _dbl, _vp, _i64, _vp, _vp,
+                                  _vp, _sz, _vp]),
+    "neb_spectral_adjoint": (_int, [_i64, _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
     "neb_spd_inverse_workspace_size": (_sz, [_i64]),
     "neb_spd_inverse": (_int, [_i64, _vp, _i64, _vp, _i64, _vp, _vp, _sz, _vp]),
 }

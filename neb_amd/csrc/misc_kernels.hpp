@@ -144,6 +144,29 @@ __global__ __launch_bounds__(256) void adam_kernel(T* __restrict__ p, const T* _
   p[i] = (T)((double)p[i] - lr * (mi / c1) / (sqrt(vi / c2) + eps));
 }
 
+// Adam with the gates of the step evaluated on the device: the clip scale
+// min(1, clip / |g|) comes from the squared gradient norm *norm2, and nothing
+// is written when |g| is zero or nonfinite or *gate (an earlier gate of the
+// step, nullable) is nonzero (NCC:156-171 clip and update).
+template <typename T>
+__global__ __launch_bounds__(256) void adam_gated_kernel(T* __restrict__ p, const T* __restrict__ g, T* __restrict__ m,
+                                                         T* __restrict__ v, int64_t n, double lr, double clip,
+                                                         double b1, double b2, double eps, double c1, double c2,
+                                                         const double* __restrict__ norm2,
+                                                         const int* __restrict__ gate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double nrm = sqrt(*norm2);
+  if ((gate && *gate != 0) || !(nrm > 0.0) || !isfinite(nrm)) return;
+  const double scale = fmin(1.0, clip / fmax(nrm, 1e-30));
+  const double gi = (double)g[i] * scale;
+  const double mi = b1 * (double)m[i] + (1.0 - b1) * gi;
+  const double vi = b2 * (double)v[i] + (1.0 - b2) * gi * gi;
+  m[i] = (T)mi;
+  v[i] = (T)vi;
+  p[i] = (T)((double)p[i] - lr * (mi / c1) / (sqrt(vi / c2) + eps));
+}
+
 // dst[r][c] (dst_rows x dst_cols, zero outside the source) = src[r][c] or
 // src[c][r] when transposing; converts the master precision to the MFMA
 // operand precision.

@@ -16,6 +16,7 @@
 #include "misc_kernels.hpp"
 #include "rowgemm_async.hpp"
 #include "spd_inverse.hpp"
+#include "spectral_terms.hpp"
 
 namespace {
 
@@ -678,8 +679,30 @@ int neb_adam(int dtype, void* p, const void* g, void* m, void* v, int64_t n, dou
   return hip_status("neb_adam");
 }
 
+int neb_adam_gated(int dtype, void* p, const void* g, void* m, void* v, int64_t n, double lr, double clip,
+                   double b1, double b2, double eps, int64_t t, const double* norm2, const int* gate, void* stream) {
+  if (n < 0 || !p || !g || !m || !v || !norm2 || t < 1 || !(clip > 0.0))
+    return fail(NEB_ERR_ARG, "neb_adam_gated: bad arguments");
+  if (n == 0) return NEB_OK;
+  const double c1 = 1.0 - pow(b1, (double)t), c2 = 1.0 - pow(b2, (double)t);
+  const dim3 grid((unsigned)((n + 255) / 256));
+  hipStream_t st = S(stream);
+  if (dtype == NEB_F32)
+    hipLaunchKernelGGL(adam_gated_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<float*>(p),
+                       reinterpret_cast<const float*>(g), reinterpret_cast<float*>(m), reinterpret_cast<float*>(v), n,
+                       lr, clip, b1, b2, eps, c1, c2, norm2, gate);
+  else if (dtype == NEB_F64)
+    hipLaunchKernelGGL(adam_gated_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<double*>(p),
+                       reinterpret_cast<const double*>(g), reinterpret_cast<double*>(m), reinterpret_cast<double*>(v),
+                       n, lr, clip, b1, b2, eps, c1, c2, norm2, gate);
+  else
+    return fail(NEB_ERR_ARG, "neb_adam_gated: fp32 or fp64 master weights only");
+  return hip_status("neb_adam_gated");
+}
+
 size_t neb_spd_inverse_workspace_size(int64_t K) {
-  return (size_t)(2 * K * K + SPD_NB * SPD_NB) * sizeof(double);
+  const int64_t Kp = (K + SPD_NB - 1) / SPD_NB * SPD_NB;
+  return (size_t)(2 * Kp * Kp + 2 * SPD_NB * SPD_NB) * sizeof(double);
 }
 
 int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
@@ -688,29 +711,73 @@ int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t 
   if (!A || !X || !info || !workspace) return fail(NEB_ERR_ARG, "neb_spd_inverse: null pointer");
   if (A == X) return fail(NEB_ERR_ARG, "neb_spd_inverse: X must not alias A");
   if (ws_bytes < neb_spd_inverse_workspace_size(K)) return fail(NEB_ERR_ARG, "neb_spd_inverse: workspace too small");
-  double* buf[2] = {reinterpret_cast<double*>(workspace), reinterpret_cast<double*>(workspace) + K * K};
-  double* Pi = reinterpret_cast<double*>(workspace) + 2 * K * K;
+  const int64_t nt = (K + SPD_NB - 1) / SPD_NB, Kp = nt * SPD_NB;
+  if (nt > 65535) return fail(NEB_ERR_SHAPE, "neb_spd_inverse: K too large");
+  double* base = reinterpret_cast<double*>(workspace);
+  double* buf[2] = {base, base + Kp * Kp};  // padded Kp x Kp ping-pong matrices
+  double* Pi[2] = {base + 2 * Kp * Kp, base + 2 * Kp * Kp + SPD_NB * SPD_NB};
   hipStream_t st = S(stream);
-  if (int rc = set_lds(spd_update_kernel, spd_update_lds_bytes())) return rc;
-  const int64_t nsteps = (K + SPD_NB - 1) / SPD_NB;
-  const unsigned nt = (unsigned)nsteps;
+  if (int rc = set_lds(spd_pivot0_kernel, spd_pivot0_lds_bytes())) return rc;
+  if (int rc = set_lds(spd_step_kernel, spd_step_lds_bytes())) return rc;
+  hipLaunchKernelGGL(spd_pivot0_kernel, dim3(1), dim3(256), spd_pivot0_lds_bytes(), st, A, lda, K, Pi[0], info);
+  if (int rc = hip_status("neb_spd_inverse pivot")) return rc;
   const double* src = A;
   int64_t lds = lda;
-  for (int64_t s = 0; s < nsteps; ++s) {
-    const int64_t k0 = s * SPD_NB;
-    const int nb = (int)(K - k0 < SPD_NB ? K - k0 : SPD_NB);
-    hipLaunchKernelGGL(spd_pivot_kernel, dim3(1), dim3(256), 0, st, src, lds, k0, nb, Pi, info, s == 0 ? 1 : 0);
-    if (int rc = hip_status("neb_spd_inverse pivot")) return rc;
-    const bool last = s == nsteps - 1;
-    double* dst = last ? X : buf[s & 1];
-    const int64_t ldd = last ? ldx : K;
-    hipLaunchKernelGGL(spd_update_kernel, dim3(nt, nt), dim3(256), spd_update_lds_bytes(), st, src, lds, K, k0, nb,
-                       Pi, dst, ldd, last ? -1.0 : 1.0);
-    if (int rc = hip_status("neb_spd_inverse update")) return rc;
+  for (int64_t k = 0; k < nt; ++k) {
+    const bool last = k == nt - 1;
+    double* dst = last ? X : buf[k & 1];
+    const int64_t ldd = last ? ldx : Kp;
+    hipLaunchKernelGGL(spd_step_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), spd_step_lds_bytes(), st, src,
+                       lds, k == 0 ? 1 : 0, K, (int)k, Pi[k & 1], Pi[(k + 1) & 1], dst, ldd, last ? 1 : 0, info);
+    if (int rc = hip_status("neb_spd_inverse sweep")) return rc;
     src = dst;
     lds = ldd;
   }
   return NEB_OK;
+}
+
+int neb_sym_shift(int64_t K, const double* A, int64_t lda, double shift, double* B, int64_t ldb, void* stream) {
+  if (K < 1 || lda < K || ldb < K || !A || !B) return fail(NEB_ERR_ARG, "neb_sym_shift: bad arguments");
+  const int64_t nt = (K + SYM_T - 1) / SYM_T;
+  if (nt > 65535) return fail(NEB_ERR_SHAPE, "neb_sym_shift: K too large");
+  hipLaunchKernelGGL(sym_shift_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, S(stream), K, A, lda, shift,
+                     B, ldb);
+  return hip_status("neb_sym_shift");
+}
+
+size_t neb_spectral_workspace_size(int64_t K) {
+  const int64_t nt = (K + SYM_T - 1) / SYM_T;
+  return (size_t)(nt * nt * 4) * sizeof(double);
+}
+
+int neb_spectral_terms(int64_t K, const double* M, int64_t ldm, const double* Sm, int64_t lds, double* X,
+                       int64_t ldx, const int* info, double lam, double ridge, double cond_limit, double* Ss,
+                       int64_t ldss, double* loss, int* gate, void* workspace, size_t ws_bytes, void* stream) {
+  if (K < 1 || ldm < K || lds < K || ldx < K || ldss < K) return fail(NEB_ERR_SHAPE, "neb_spectral_terms: bad sizes");
+  if (!M || !Sm || !X || !Ss || !loss || !gate || !workspace)
+    return fail(NEB_ERR_ARG, "neb_spectral_terms: null pointer");
+  if (ws_bytes < neb_spectral_workspace_size(K)) return fail(NEB_ERR_ARG, "neb_spectral_terms: workspace too small");
+  const int64_t nt = (K + SYM_T - 1) / SYM_T;
+  if (nt > 65535) return fail(NEB_ERR_SHAPE, "neb_spectral_terms: K too large");
+  double* part = reinterpret_cast<double*>(workspace);
+  hipStream_t st = S(stream);
+  hipLaunchKernelGGL(spectral_terms_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, st, K, M, ldm, Sm, lds,
+                     X, ldx, Ss, ldss, ridge, part);
+  if (int rc = hip_status("neb_spectral_terms")) return rc;
+  hipLaunchKernelGGL(spectral_final_kernel, dim3(1), dim3(256), 0, st, part, nt * nt, info, lam, cond_limit, loss,
+                     gate);
+  return hip_status("neb_spectral_terms final");
+}
+
+int neb_spectral_adjoint(int64_t K, const double* C, int64_t ldc, const double* M, int64_t ldm, double lam, double* dM,
+                         int64_t ldd, void* stream) {
+  if (K < 1 || ldc < K || ldm < K || ldd < K || !C || !M || !dM) return fail(NEB_ERR_ARG, "neb_spectral_adjoint: bad arguments");
+  if (dM == M) return fail(NEB_ERR_ARG, "neb_spectral_adjoint: dM must not alias M");
+  const int64_t nt = (K + SYM_T - 1) / SYM_T;
+  if (nt > 65535) return fail(NEB_ERR_SHAPE, "neb_spectral_adjoint: K too large");
+  hipLaunchKernelGGL(spectral_adjoint_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, S(stream), K, C, ldc,
+                     M, ldm, lam, dM, ldd);
+  return hip_status("neb_spectral_adjoint");
 }
 
 int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst,

@@ -191,22 +191,42 @@ class EigenfluidNet:
         return out
 
     # -- optimizer ---------------------------------------------------------
-    def adam(self, grads: torch.Tensor, lr: float, clip: float = 8.0):
+    def adam(self, grads: torch.Tensor, lr: float, clip: float = 8.0, gate: torch.Tensor | None = None,
+             on_gate=None):
         """Adam with global-norm clip (NCC:156-171; one parameter group for
         the shared trunk).  Nonfinite or zero gradients raise
-        FloatingPointError before any update."""
+        FloatingPointError before any update.
+
+        The gates are evaluated on the device (neb_adam_gated): the update is
+        queued without a host synchronisation and writes nothing when a gate
+        fails; one synchronisation afterwards reads the gradient norm and
+        `gate` (an optional device int32 flag of an earlier gate of the same
+        step, nonzero = failed, e.g. SpectralSolver.gate) and raises.  A
+        failed `gate` calls on_gate(value) first, which raises the gate's own
+        error.  Returns the gradient norm."""
         dt = _DT[self.master_dtype]
         ws = workspace(self.device).get(256 * 8)
         norm2 = torch.empty(1, dtype=torch.float64, device=self.device)
         s = _stream()
         H.call("neb_sumsq", dt, grads.data_ptr(), grads.numel(), norm2.data_ptr(), ws.data_ptr(), ws.numel(), s)
-        norm = math.sqrt(float(norm2.item()))
+        t = self.epoch + 1
+        if gate is not None and (gate.dtype != torch.int32 or gate.numel() != 1 or gate.device != norm2.device):
+            raise ValueError("gate must be a one-element int32 tensor on the parameters' device")
+        H.call("neb_adam_gated", dt, self.params.data_ptr(), grads.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
+               self.params.numel(), lr, clip, 0.9, 0.999, 1e-8, t, norm2.data_ptr(),
+               gate.data_ptr() if gate is not None else None, s)
+        if gate is None:
+            n2, gv = float(norm2.item()), 0
+        else:
+            n2, gv = torch.cat((norm2, gate.to(torch.float64))).tolist()
+            gv = int(gv)
+        if gv != 0:
+            if on_gate is not None:
+                on_gate(gv)
+            raise FloatingPointError(f"step gate {gv} failed; no optimizer update")
+        norm = math.sqrt(n2) if n2 >= 0 else float("nan")
         if not math.isfinite(norm) or norm <= 0:
             raise FloatingPointError("Invalid parameter gradients; no optimizer update")
-        scale = min(1.0, clip / max(norm, 1e-30))
-        t = self.epoch + 1
-        H.call("neb_adam", dt, self.params.data_ptr(), grads.data_ptr(), self.m.data_ptr(), self.v.data_ptr(),
-               self.params.numel(), lr, scale, 0.9, 0.999, 1e-8, t, s)
         self.epoch = t
         self.touch()
         return norm
@@ -421,6 +441,50 @@ def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, bet
     return out
 
 
+def _f64_rows(A: torch.Tensor) -> torch.Tensor:
+    """A as an FP64 matrix with unit column stride (no copy when it is one)."""
+    A = A.to(torch.float64)
+    return A if A.stride(1) == 1 else A.contiguous()
+
+
+def sym_shift(A: torch.Tensor, shift: float = 0.0, out: torch.Tensor | None = None) -> torch.Tensor:
+    """(A + A^T) / 2 + shift I in FP64 (one launch; out may be A)."""
+    A = _f64_rows(A)
+    K = A.shape[0]
+    out = torch.empty((K, K), dtype=torch.float64, device=A.device) if out is None else out
+    H.call("neb_sym_shift", K, A.data_ptr(), A.stride(0), float(shift), out.data_ptr(), out.stride(0), _stream())
+    return out
+
+
+def spectral_terms(M: torch.Tensor, S: torch.Tensor, X: torch.Tensor, info: torch.Tensor | None, lam: float,
+                   ridge: float, cond_limit: float):
+    """Objective terms around the inverse X of M_r = sym(M) + ridge I
+    (neb_spectral_terms): symmetrises X in place and returns
+    (loss, X, sym(S), gate) with the device scalar loss = <X, sym(S)> +
+    lam ||M - I||^2 and the device int32 gate (0, first bad pivot, or -1 for
+    the condition limit)."""
+    M, S = _f64_rows(M), _f64_rows(S)
+    K = M.shape[0]
+    Ss = torch.empty((K, K), dtype=torch.float64, device=M.device)
+    loss = torch.empty((), dtype=torch.float64, device=M.device)
+    gate = torch.empty(1, dtype=torch.int32, device=M.device)
+    ws = workspace(M.device).get(H.load().neb_spectral_workspace_size(K))
+    H.call("neb_spectral_terms", K, M.data_ptr(), M.stride(0), S.data_ptr(), S.stride(0), X.data_ptr(), X.stride(0),
+           info.data_ptr() if info is not None else None, float(lam), float(ridge), float(cond_limit),
+           Ss.data_ptr(), K, loss.data_ptr(), gate.data_ptr(), ws.data_ptr(), ws.numel(), _stream())
+    return loss, X, Ss, gate
+
+
+def mass_adjoint(X: torch.Tensor, Ss: torch.Tensor, M: torch.Tensor, lam: float) -> torch.Tensor:
+    """dL/dM = -sym(X Ss X) + 2 lam (M - I) (two FP64 GEMMs + one launch)."""
+    M = _f64_rows(M)
+    K = M.shape[0]
+    C = torch.mm(torch.mm(X, Ss), X)
+    H.call("neb_spectral_adjoint", K, C.data_ptr(), K, M.data_ptr(), M.stride(0), float(lam), C.data_ptr(), K,
+           _stream())
+    return C
+
+
 def _rows_view(t: torch.Tensor, start_row: int, step: int) -> torch.Tensor:
     """Every ``step``-th row of a row-major 2-D tensor starting at start_row."""
     return t.as_strided((t.shape[0] // step, t.shape[1]), (t.stride(0) * step, 1),
@@ -431,12 +495,14 @@ def _rows_view(t: torch.Tensor, start_row: int, step: int) -> torch.Tensor:
 # reverse pass
 # ---------------------------------------------------------------------------
 def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Tensor, dS: torch.Tensor,
-                    w: float, grads: torch.Tensor | None = None) -> torch.Tensor:
+                    w: float, grads: torch.Tensor | None = None, symmetric: bool = False) -> torch.Tensor:
     """Parameter gradient of L(M, S) with M = w U^T U, S = w D^T D, given
     dL/dM and dL/dS (symmetric, FP64 K x K).  Fuses the field adjoint
     2w Phi dL/dG (VCORE:94-95) into the output-layer reverse, then runs the
     handwritten jet reverse through every layer (HB2P:2087-2139, VJ2:77-82).
-    Returns the flat gradient in the master dtype."""
+    symmetric=True skips the symmetrisation of dM and dS for adjoints that
+    are symmetric by construction (SpectralSolver's).  Returns the flat
+    gradient in the master dtype."""
     cfg = net.cfg
     prec = tape.prec
     stg = net.staged(prec)
@@ -445,20 +511,24 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     width = cfg.widths[-1]
     st = _STORE[prec]
     s = _stream()
-    dM = 0.5 * (dM + dM.T)
-    dS = 0.5 * (dS + dS.T)
+    if symmetric:
+        dM, dS = _f64_rows(dM), _f64_rows(dS)
+    else:
+        dM, dS = sym_shift(dM), sym_shift(dS)
     Wout = net.W[-1].to(torch.float64)               # [ncomp*K, width]
     ldK = fields.mass.shape[1]
     if d == 2:
-        Bm = (2.0 * w) * (Wout.T @ dM)               # [width, K]
-        Bs = (2.0 * w) * (Wout.T @ dS)
+        Bm = torch.mm(Wout.T, dM)                    # [width, K]
+        Bs = torch.mm(Wout.T, dS)
     else:
-        Bm = torch.cat([(2.0 * w) * (Wout[c::3].T @ dM) for c in range(3)], 0)  # [3*width, K]
-        Bs = torch.cat([(2.0 * w) * (Wout[c::3].T @ dS) for c in range(3)], 0)
-    bm = torch.zeros((Bm.shape[0], _ld(K)), dtype=st, device=net.device)
-    bs = torch.zeros_like(bm)
-    bm[:, :K] = Bm.to(st)
-    bs[:, :K] = Bs.to(st)
+        Bm = torch.cat([Wout[c::3].T @ dM for c in range(3)], 0)  # [3*width, K]
+        Bs = torch.cat([Wout[c::3].T @ dS for c in range(3)], 0)
+    # 2w B in the operand precision, zero-padded to the row stride (one launch each)
+    bm = torch.empty((Bm.shape[0], _ld(K)), dtype=st, device=net.device)
+    bs = torch.empty_like(bm)
+    for src, dst in ((Bm, bm), (Bs, bs)):
+        H.call("neb_stage", H.F64, prec, src.data_ptr(), src.shape[0], K, src.stride(0), dst.data_ptr(), dst.shape[1],
+               dst.shape[0], dst.shape[1], 0, 2.0 * w, s)
     A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
     Z = tape.pre[-1]
     H.call("neb_jet_output_backward", prec, d, cfg.act_code, n, width, K, fields.mass.data_ptr(),
@@ -478,7 +548,8 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
         u, v = _rows_view(U, 0, 2), _rows_view(U, 1, 2)
         Pm = cross([(u, ch(Ylast, 2), n, 1.0), (v, ch(Ylast, 1), n, -1.0)], K, width, mode)
         Ps = cross([(Dg, ch(Ylast, 3), n, -1.0), (Dg, ch(Ylast, 5), n, -1.0)], K, width, mode)
-        gW[-1].copy_(((2.0 * w) * (dM @ Pm + dS @ Ps)).to(grads.dtype))
+        gout = torch.addmm(Pm, dM, Pm, beta=0.0, alpha=2.0 * w)
+        gW[-1].copy_(gout.addmm_(dS, Ps, alpha=2.0 * w))
     else:
         from .lifts import curl_terms
         comps = [_rows_view(U, a, 3) for a in range(3)]
@@ -502,13 +573,13 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
         A2 = A.view(C * n, A.shape[2])
         X2 = X.view(C * n, X.shape[2])
         G = cross([(A2, X2, C * n, 1.0)], wl, k, mode)
-        gW[l].copy_(G.to(grads.dtype))
+        gW[l].copy_(G)
         need = H.load().neb_colsum_workspace_size(n, wl)
         ws = cw.get(need)
         db = torch.empty(wl, dtype=torch.float64, device=net.device)
         H.call("neb_colsum", _DT[st], A[0].data_ptr(), n, wl, A.shape[2], 0.0, db.data_ptr(), ws.data_ptr(),
                ws.numel(), s)
-        gb[l].copy_(db.to(grads.dtype))
+        gb[l].copy_(db)
         if l > 0:
             Zp = tape.pre[l - 1]
             An = torch.empty((C, n, _ld(k)), dtype=st, device=net.device)

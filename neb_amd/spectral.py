@@ -58,6 +58,26 @@ def spd_inverse(M: torch.Tensor) -> torch.Tensor:
     return _sym(X)
 
 
+def _eye(K: int, device) -> torch.Tensor:
+    key = (K, str(device))
+    e = _EYES.get(key)
+    if e is None:
+        e = _EYES[key] = torch.eye(K, dtype=torch.float64, device=device)
+    return e
+
+
+_EYES: dict = {}
+
+
+def _raise_gate(value: int):
+    """FloatingPointError of a failed objective gate value (0 = passed,
+    k > 0 = first nonpositive pivot k, -1 = condition limit / nonfinite)."""
+    if value > 0:
+        _raise_not_pd(value)
+    if value != 0:
+        raise FloatingPointError("mass condition estimate exceeds limit; no update")
+
+
 def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridge: float = 0.0,
                        cond_limit: float = 1e12, gate: str = "eig"):
     """L = tr((M + ridge I)^-1 S) + lam ||M - I||_F^2 and its closed-form
@@ -67,23 +87,20 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
     nonpositive or too ill-conditioned mass raises FloatingPointError before
     anything is updated.  gate="eig" checks eigenvalues (the reference's
     rule); gate="cholesky" only requires positive definiteness (every pivot
-    of the FP64 inversion positive, as a Cholesky factorization requires).
-    Returns (loss: float, dM, dS) with FP64 tensors on M's device."""
-    K = M.shape[0]
-    eye = torch.eye(K, dtype=torch.float64, device=M.device)
-    Mr = _sym(M) + ridge * eye
+    of the FP64 inversion positive, as a Cholesky factorization requires) and
+    a finite result.  Returns (loss, dM, dS) with FP64 tensors on M's device
+    (loss 0-dim)."""
+    Mr = nw.sym_shift(M, ridge)
     if gate == "eig":
         e = torch.linalg.eigvalsh(Mr)
         ok = bool(torch.isfinite(e).all()) and float(e[-1]) > 0 and float(e[0] / e[-1]) >= 1.0 / cond_limit
         if not ok:
             raise FloatingPointError("Physical mass zero/rank deficient/condition exceeds limit; no update")
-    inv = spd_inverse(Mr)
-    Ss = _sym(S)
-    D = M - eye
-    loss = torch.sum(inv * Ss) + lam * torch.sum(D * D)
-    cross = inv @ Ss @ inv
-    dM = -_sym(cross) + 2.0 * lam * D
-    return loss, dM, inv
+    X, info = _spd_inverse_async(Mr)
+    loss, X, Ss, g = nw.spectral_terms(M, S, X, info, lam, ridge, math.inf)
+    _raise_gate(int(g.item()))
+    dM = nw.mass_adjoint(X, Ss, M, lam)
+    return loss, dM, X
 
 
 class SpectralSolver:
@@ -99,27 +116,45 @@ class SpectralSolver:
     ridge changes too much per step for the warm start to converge).  Either way the
     returned inverse satisfies ||I - M_r X||_F <= tol, and a condition
     estimate ||M_r||_F ||X||_F above cond_limit raises FloatingPointError
-    (the gate of MB/velocity_trace_jet2.py:109-111)."""
+    (the gate of MB/velocity_trace_jet2.py:109-111).
+
+    deferred=True keeps both gates on the device: the call queues its work
+    without a host synchronisation and leaves `gate` (one-element int32:
+    0 = passed, k > 0 = first nonpositive pivot k, -1 = condition limit or
+    nonfinite); the caller hands it to EigenfluidNet.adam(gate=...,
+    on_gate=solver.raise_gate), which skips the update on the device and
+    raises, or calls check() before using the results."""
 
     def __init__(self, lam: float = 1e-2, ridge: float = 0.0, cond_limit: float = 1e12, tol: float = 1e-10,
-                 max_iter: int = 6, backoff: int = 16):
+                 max_iter: int = 6, backoff: int = 16, deferred: bool = False):
         self.lam, self.ridge, self.cond_limit, self.tol, self.max_iter = lam, ridge, cond_limit, tol, max_iter
         self.backoff = backoff
+        self.deferred = deferred
         self.X = None
+        self.gate = None
         self.last_path = None
         self._skip = 0       # warm starts left to skip after one that failed
 
     def _residual(self, Mr, X, eye):
         return float(torch.linalg.norm(eye - Mr @ X))
 
+    def raise_gate(self, value: int):
+        """Raise the FloatingPointError of a failed gate value (see `gate`)."""
+        _raise_gate(value)
+
+    def check(self):
+        """Synchronise and raise if the last deferred call's gate failed."""
+        if self.gate is not None:
+            self.raise_gate(int(self.gate.item()))
+
     def __call__(self, M: torch.Tensor, S: torch.Tensor):
         K = M.shape[0]
-        eye = torch.eye(K, dtype=torch.float64, device=M.device)
-        Mr = _sym(M) + self.ridge * eye
+        Mr = nw.sym_shift(M, self.ridge)
         X = None
         if self._skip > 0:
             self._skip -= 1
         elif self.X is not None and self.X.shape == Mr.shape:
+            eye = _eye(K, M.device)
             X = self.X
             ok = False
             for _ in range(self.max_iter):
@@ -140,23 +175,15 @@ class SpectralSolver:
         if X is None:
             X, info = _spd_inverse_async(Mr)
             self.last_path = "spd-inverse"
-        X = _sym(X)
-        # one host synchronisation for both gates
-        cond_t = (torch.linalg.norm(Mr) * torch.linalg.norm(X)).reshape(1)
-        if info is not None:
-            bad, cond = torch.cat((info.to(torch.float64), cond_t)).tolist()
-            if bad != 0:
-                _raise_not_pd(int(bad))
+        # X <- sym(X), loss, and the positive-definiteness / condition gates
+        # on the device (one fused launch + one reduction block)
+        loss, X, Ss, gate = nw.spectral_terms(M, S, X, info, self.lam, self.ridge, self.cond_limit)
+        if self.deferred:
+            self.gate = gate
         else:
-            cond = float(cond_t)
-        if not math.isfinite(cond) or cond > self.cond_limit:
-            raise FloatingPointError("mass condition estimate exceeds limit; no update")
+            self.raise_gate(int(gate.item()))  # one host synchronisation for both gates
         self.X = X
-        Ss = _sym(S)
-        D = M - eye
-        loss = torch.sum(X * Ss) + self.lam * torch.sum(D * D)
-        cross = X @ Ss @ X
-        dM = -_sym(cross) + 2.0 * self.lam * D
+        dM = nw.mass_adjoint(X, Ss, M, self.lam)
         return loss, dM, X
 
 
@@ -210,7 +237,7 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     out = dict(loss=loss, M=M, S=S, fields=fields, timings=tt)
     if training:
         with timed(tt, "analytic_backward"):
-            out["grads"] = nw.backward_fields(net, tape, fields, dM, dS, weight)
+            out["grads"] = nw.backward_fields(net, tape, fields, dM, dS, weight, symmetric=True)
     return out
 
 
@@ -248,7 +275,7 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
             for c0 in range(0, n, chunk):
                 tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
                 f = nw.output_fields(net, tape, split=split)
-                grads += nw.backward_fields(net, tape, f, dM, dS, weight)
+                grads += nw.backward_fields(net, tape, f, dM, dS, weight, symmetric=True)
                 del tape, f
         out["grads"] = grads
     return out

@@ -396,8 +396,104 @@ def test_adam_first_step_closed_form():
     assert abs(norm - float(g.double().norm())) < 1e-6 * norm
     expect = before - 1e-4 * g / (g.abs() + 1e-8)
     assert torch.allclose(net.params, expect, rtol=2e-6, atol=2e-7)
+    after = net.params.detach().clone()
     with pytest.raises(FloatingPointError):
         net.adam(g * float("nan"), 1e-4)
+    with pytest.raises(FloatingPointError):
+        net.adam(torch.zeros_like(g), 1e-4)
+    assert torch.equal(net.params, after) and net.epoch == 1  # the device gate skipped both updates
+
+
+def test_adam_device_gate_skips_update():
+    """A failed step gate handed to adam leaves parameters, moments and the
+    epoch untouched and raises through on_gate; a passing gate updates
+    exactly as without one."""
+    cfg, net, _ = make(2, 8, [16], 2, master=torch.float32)
+    rng = np.random.default_rng(83)
+    g = torch.tensor(rng.normal(size=net.params.numel()), dtype=torch.float32, device=DEV)
+    p0, m0, v0 = net.params.clone(), net.m.clone(), net.v.clone()
+    calls = []
+
+    def on_gate(v):
+        calls.append(v)
+        raise FloatingPointError(f"gate {v}")
+
+    bad = torch.tensor([7], dtype=torch.int32, device=DEV)
+    with pytest.raises(FloatingPointError, match="gate 7"):
+        net.adam(g, 1e-3, gate=bad, on_gate=on_gate)
+    assert calls == [7] and net.epoch == 0
+    assert torch.equal(net.params, p0) and torch.equal(net.m, m0) and torch.equal(net.v, v0)
+    good = torch.zeros(1, dtype=torch.int32, device=DEV)
+    n1 = net.adam(g, 1e-3, clip=0.5, gate=good, on_gate=on_gate)
+    _, net2, _ = make(2, 8, [16], 2, master=torch.float32)
+    n2 = net2.adam(g, 1e-3, clip=0.5)
+    assert n1 == n2 and torch.equal(net.params, net2.params) and torch.equal(net.m, net2.m)
+
+
+def test_deferred_solver_gates():
+    """deferred=True gives the same objective and adjoints without a host
+    synchronisation and reports the positive-definiteness and condition
+    gates through `gate` (raised by raise_gate / check)."""
+    rng = np.random.default_rng(5)
+    K = 100
+    A = rng.normal(size=(400, K)); M0 = A.T @ A / 400
+    B = rng.normal(size=(300, K)); S0 = B.T @ B / 300
+    Mt, St = torch.tensor(M0, device=DEV), torch.tensor(S0, device=DEV)
+    ref = nb.SpectralSolver(1e-2)(Mt, St)
+    sd = nb.SpectralSolver(1e-2, deferred=True)
+    out = sd(Mt, St)
+    assert int(sd.gate.item()) == 0
+    sd.check()
+    for a, b in zip(out, ref):
+        assert torch.equal(a, b)
+    Mbad = M0.copy(); Mbad[70, 70] = -1.0
+    sd2 = nb.SpectralSolver(1e-2, deferred=True)
+    sd2(torch.tensor(Mbad, device=DEV), St)
+    assert int(sd2.gate.item()) == 71
+    with pytest.raises(FloatingPointError, match=r"pivot 71\)"):
+        sd2.check()
+    sd3 = nb.SpectralSolver(1e-2, cond_limit=1.0, deferred=True)
+    sd3(Mt, St)
+    assert int(sd3.gate.item()) == -1
+    with pytest.raises(FloatingPointError, match="condition"):
+        sd3.check()
+
+
+@pytest.mark.parametrize("K", [1, 33, 100, 1024])
+def test_spectral_term_kernels(K):
+    """sym_shift / spectral_terms / mass_adjoint against plain FP64 numpy on
+    non-symmetric inputs (ragged K, in-place outputs): symmetrised matrices
+    bit for bit, the loss to rounding, the gate values."""
+    rng = np.random.default_rng(K)
+    M, S, X, C = (rng.normal(size=(K, K)) for _ in range(4))
+    lam, ridge = 0.03, 0.25
+    I = np.eye(K)
+    Mt = torch.tensor(M, device=DEV)
+    Mr = nw.sym_shift(Mt, ridge)
+    assert np.array_equal(Mr.cpu().numpy(), 0.5 * (M + M.T) + ridge * I)
+    A = torch.tensor(S, device=DEV)
+    nw.sym_shift(A, 0.0, out=A)  # in place
+    assert np.array_equal(A.cpu().numpy(), 0.5 * (S + S.T))
+    Xs, Ss = 0.5 * (X + X.T), 0.5 * (S + S.T)
+    loss_ref = np.sum(Xs * Ss) + lam * np.sum((M - I) ** 2)
+    cond = np.linalg.norm(0.5 * (M + M.T) + ridge * I) * np.linalg.norm(Xs)
+    for limit, info, expect in ((cond * 2, None, 0), (cond / 2, None, -1), (cond * 2, 17, 17)):
+        Xt = torch.tensor(X, device=DEV)
+        it = None if info is None else torch.tensor([info], dtype=torch.int32, device=DEV)
+        loss, Xo, Sso, gate = nw.spectral_terms(Mt, torch.tensor(S, device=DEV), Xt, it, lam, ridge, limit)
+        assert Xo.data_ptr() == Xt.data_ptr() and np.array_equal(Xt.cpu().numpy(), Xs)
+        assert np.array_equal(Sso.cpu().numpy(), Ss)
+        assert abs(float(loss) - loss_ref) <= 1e-12 * max(1.0, abs(loss_ref))
+        assert int(gate.item()) == expect
+    Ct = torch.tensor(C, device=DEV)
+    H = nw.H
+    H.call("neb_spectral_adjoint", K, Ct.data_ptr(), K, Mt.data_ptr(), K, lam, Ct.data_ptr(), K, nw._stream())
+    ref = -0.5 * (C + C.T) + 2.0 * lam * (M - I)
+    r = rel(Ct.cpu(), ref); assert r < 1e-15, r
+    Xt = torch.tensor(Xs, device=DEV); St = torch.tensor(Ss, device=DEV)
+    dM = nw.mass_adjoint(Xt, St, Mt, lam)
+    ref = -0.5 * ((Xs @ Ss @ Xs) + (Xs @ Ss @ Xs).T) + 2.0 * lam * (M - I)
+    r = rel(dM.cpu(), ref); assert r < 1e-12, r
 
 
 def test_projection_against_oracle():
@@ -536,9 +632,21 @@ def test_spd_inverse_rejects_indefinite():
     rng = np.random.default_rng(1)
     B = rng.normal(size=(80, 70))
     A = B.T @ B / 70
-    A[40, 40] = -1.0  # breaks positive definiteness at pivot 41 (1-based), inside the second 64-block
+    A[40, 40] = -1.0  # breaks positive definiteness at pivot 41 (1-based), inside the first 64-block
     with pytest.raises(FloatingPointError, match=r"pivot 41\)"):
         sp.spd_inverse(torch.tensor(A, device=DEV))
     # the flag is cleared by the next call
     ok = B.T @ B / 70 + np.eye(70)
     sp.spd_inverse(torch.tensor(ok, device=DEV))
+
+
+def test_spd_inverse_rejects_indefinite_later_block():
+    """A nonpositive pivot in the third 64-block: that block's inverse is
+    formed inside the sweep of the second block."""
+    from neb_amd import spectral as sp
+    rng = np.random.default_rng(2)
+    B = rng.normal(size=(220, 200))
+    A = B.T @ B / 200
+    A[150, 150] = -1.0
+    with pytest.raises(FloatingPointError, match=r"pivot 151\)"):
+        sp.spd_inverse(torch.tensor(A, device=DEV))
