@@ -155,8 +155,9 @@ int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int
 /* Split-precision Gram  G = beta*G + w * Phi^T Phi  (FP64, full symmetric)
  * from pre-split planes (neb_jet_output_fields_split): three bf16 MFMA
  * products hi.hi + hi.lo + lo.hi per pair, fp32 accumulation flushed into
- * fp32 partial sums every 256 rows, FP64 split reduction.  Bit-identical to
- * neb_gram(NEB_CROSS_BF16X3) on the fp32 matrix the planes came from.
+ * fp32 partial sums every 256 rows, FP64 split reduction: the products of
+ * neb_gram(NEB_CROSS_BF16X3) on the fp32 matrix the planes came from, equal
+ * up to the rounding of differently split partial sums.  ld < 2^25.
  * Workspace: neb_cross_workspace_size(K, K, rows, 1).  Replaces VCORE:40-46
  * `grams` on the split-precision path. */
 int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w, double beta,

@@ -68,7 +68,7 @@ struct AsyncCursor {
   }
 };
 
-template <int NL, int WM, int WN, int NS_>
+template <int NL, int WM, int WN, int NS_, bool NEG>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(NL == 1 ? 4 : 2))) void
 cross_async_kernel(CrossArgs a) {
   using Pol = AsyncPol<NL, WM, WN, NS_>;
@@ -115,23 +115,47 @@ cross_async_kernel(CrossArgs a) {
   // piece q = wave * LPW + t: operand op = q / (8 NL), plane q / 8 % NL, rows
   // 4 (q % 8) .. +3.  Lane p fills LDS unit p of the piece: k-row 4 (q % 8)
   // + p / 16, unit p % 16, holding global column unit (p % 16) ^ swz(row).
+  // A full k-step loads from the uniform address of its first row plus a
+  // per-lane byte offset that is fixed for the whole segment (set when the
+  // load cursor enters it); only a segment's last, partial k-step clamps
+  // rows (and pays the 64-bit address arithmetic).  Offsets fit 32 bits:
+  // the host keeps 32 ld bytes below 2^31.
   const int prow = lane >> 4, punit = lane & 15;
-  auto issue = [&](const AsyncCursor& c, __bf16* slot) {
+  int lseg = -1;
+  uint32_t poff[LPW];
+  auto issue = [&](const AsyncCursor& c, __bf16* slot) __attribute__((always_inline)) {
     const CrossSeg& sg = a.seg[c.s];
+    if (c.s != lseg) {
+#pragma unroll
+      for (int t = 0; t < LPW; ++t) {
+        const int q = wave * LPW + t;
+        const int op = q / (8 * NL), kr = 4 * (q % 8) + prow;
+        const int64_t ld = op == 0 ? sg.ldx : sg.ldy;
+        int64_t gc = (op == 0 ? i0 : j0) + 8 * (punit ^ async_swz(kr));
+        gc = gc + 8 <= ld ? gc : ld - 8;
+        poff[t] = (uint32_t)((kr * ld + gc) * 2);
+      }
+      lseg = c.s;
+    }
+    const bool full = c.r0 + RK <= c.rend;
 #pragma unroll
     for (int t = 0; t < LPW; ++t) {
       const int q = wave * LPW + t;
       const int op = q / (8 * NL), pl = (q / 8) % NL, kb = q % 8;
-      const int kr = 4 * kb + prow;
       const __bf16* src = reinterpret_cast<const __bf16*>(op == 0 ? (pl == 0 ? sg.x : sg.xlo) : (pl == 0 ? sg.y : sg.ylo));
       const int64_t ld = op == 0 ? sg.ldx : sg.ldy;
-      const int64_t c0 = op == 0 ? i0 : j0;
-      int64_t gr = c.r0 + kr;
-      gr = gr < c.rend ? gr : c.rend - 1;
-      int64_t gc = c0 + 8 * (punit ^ async_swz(kr));
-      gc = gc + 8 <= ld ? gc : ld - 8;
       __bf16* dst = slot + (op * NL + pl) * PLANE + kb * 4 * CR_TM;
-      __builtin_amdgcn_global_load_lds(src + gr * ld + gc, (lds_void*)dst, 16, 0, 0);
+      if (full) {
+        const char* base = reinterpret_cast<const char*>(src + c.r0 * ld);
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const __bf16*>(base + poff[t]), (lds_void*)dst, 16, 0, 0);
+      } else {
+        const int kr = 4 * kb + prow;
+        int64_t gr = c.r0 + kr;
+        gr = gr < c.rend ? gr : c.rend - 1;
+        int64_t gc = (op == 0 ? i0 : j0) + 8 * (punit ^ async_swz(kr));
+        gc = gc + 8 <= ld ? gc : ld - 8;
+        __builtin_amdgcn_global_load_lds(src + gr * ld + gc, (lds_void*)dst, 16, 0, 0);
+      }
     }
   };
 
@@ -141,7 +165,7 @@ cross_async_kernel(CrossArgs a) {
   const int kg = lane >> 4, m = lane & 15;
   const int kr0 = 8 * kg + (m >> 2);
   const int sw = async_swz(kr0);  // equal for kr0 + 4
-  auto rd_off = [&](int cb) {
+  auto rd_off = [&](int cb) __attribute__((always_inline)) {
     const int unit = (cb >> 3) + ((m & 3) >> 1);
     return kr0 * (CR_TM * 2) + 16 * (unit ^ sw) + 8 * (m & 1);  // bytes within a plane tile
   };
@@ -151,7 +175,7 @@ cross_async_kernel(CrossArgs a) {
 #pragma unroll
   for (int j = 0; j < BJ; ++j) yoff[j] = rd_off(wn * 16 * BJ + 16 * j);
 
-  auto frag = [&](const __bf16* slot, int byte_off) -> bf16x8 {
+  auto frag = [&](const __bf16* slot, int byte_off) __attribute__((always_inline)) -> bf16x8 {
     typedef __attribute__((address_space(3))) unsigned char lds_u8;
     lds_v4* p0 = (lds_v4*)((lds_u8*)(slot) + byte_off);
     lds_v4* p1 = (lds_v4*)((lds_u8*)(slot) + byte_off + 4 * CR_TM * 2);
@@ -192,7 +216,7 @@ cross_async_kernel(CrossArgs a) {
       cl.next(a, split);
     }
   // step k computes from slot k % NS and refills slot (k - 1) % NS
-  auto step = [&](__bf16* cur, __bf16* refill) -> bool {
+  auto step = [&](__bf16* cur, __bf16* refill) __attribute__((always_inline)) -> bool {
     if (cc.s >= a.nseg) return false;
     // this wave's copies of step k are done once at most (nissued - k - 1)
     // later steps' copies are outstanding; the barrier then covers every
@@ -214,15 +238,19 @@ cross_async_kernel(CrossArgs a) {
       ++nissued;
       cl.next(a, split);
     }
-    const bool neg = a.seg[cc.s].sign < 0;
     bf16x8 xh[BI], xl[BI];
 #pragma unroll
     for (int i = 0; i < BI; ++i) {
       xh[i] = frag(cur, xoff[i]);
       if constexpr (NL == 2) xl[i] = frag(cur, PLANE * 2 + xoff[i]);
-      if (neg) {
-        xh[i] = async_neg(xh[i]);
-        if constexpr (NL == 2) xl[i] = async_neg(xl[i]);
+    }
+    if constexpr (NEG) {  // some segment has sign -1 (uniform per k-step)
+      if (a.seg[cc.s].sign < 0) {
+#pragma unroll
+        for (int i = 0; i < BI; ++i) {
+          xh[i] = async_neg(xh[i]);
+          if constexpr (NL == 2) xl[i] = async_neg(xl[i]);
+        }
       }
     }
 #pragma unroll

@@ -301,12 +301,12 @@ template <int NL> struct AsyncCfg;
 template <> struct AsyncCfg<1> { static constexpr int WM = 4, WN = 2, NS = 4; };
 template <> struct AsyncCfg<2> { static constexpr int WM = 2, WN = 2, NS = 2; };
 
-template <int NL>
-int launch_cross_async(CrossArgs& a, double scale, double beta, double* G, int64_t ldg, hipStream_t st) {
+template <int NL, bool NEG>
+int launch_cross_async_as(CrossArgs& a, double scale, double beta, double* G, int64_t ldg, hipStream_t st) {
   using C = AsyncCfg<NL>;
   constexpr int NT = 64 * C::WM * C::WN;
   const size_t lds = 0;  // the ring is static LDS
-  auto kern = cross_async_kernel<NL, C::WM, C::WN, C::NS>;
+  auto kern = cross_async_kernel<NL, C::WM, C::WN, C::NS, NEG>;
   int64_t maxrows = 0;
   for (int s = 0; s < a.nseg; ++s) maxrows = a.seg[s].rows > maxrows ? a.seg[s].rows : maxrows;
   static int slots_cache[16] = {0};  // one per kernel instantiation
@@ -321,13 +321,24 @@ int launch_cross_async(CrossArgs& a, double scale, double beta, double* G, int64
   return hip_status("neb_cross reduce");
 }
 
-// The asynchronous path takes 16-byte aligned bf16 rows, no row weights and
-// segment signs of +-1.
+// sign flips compiled in only when some segment has sign -1
+template <int NL>
+int launch_cross_async(CrossArgs& a, double scale, double beta, double* G, int64_t ldg, hipStream_t st) {
+  bool neg = false;
+  for (int s = 0; s < a.nseg; ++s) neg = neg || a.seg[s].sign < 0;
+  return neg ? launch_cross_async_as<NL, true>(a, scale, beta, G, ldg, st)
+             : launch_cross_async_as<NL, false>(a, scale, beta, G, ldg, st);
+}
+
+// The asynchronous path takes 16-byte aligned bf16 rows, no row weights,
+// segment signs of +-1 and row strides below 2^25 elements (the per-lane
+// copy offsets of a 32-row k-step are 32-bit byte offsets).
 bool async_ok(const CrossArgs& a) {
   if (a.roww) return false;
   for (int s = 0; s < a.nseg; ++s) {
     const CrossSeg& g = a.seg[s];
     if (g.sign != 1.0 && g.sign != -1.0) return false;
+    if (g.ldx >= ((int64_t)1 << 25) || g.ldy >= ((int64_t)1 << 25)) return false;
     if (!aligned(g.x, g.ldx, NEB_BF16) || !aligned(g.y, g.ldy, NEB_BF16)) return false;
     if ((g.xlo && !aligned(g.xlo, g.ldx, NEB_BF16)) || (g.ylo && !aligned(g.ylo, g.ldy, NEB_BF16))) return false;
   }
@@ -603,8 +614,8 @@ int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int
 
 int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w, double beta,
                    double* G, int64_t ldg, void* workspace, size_t ws_bytes, void* stream) {
-  if (rows < 0 || K < 1 || ld < K || ld >= (1 << 26) || ldg < K)
-    return fail(NEB_ERR_SHAPE, "neb_gram_split: bad sizes");
+  if (rows < 0 || K < 1 || ld < K || ld >= (1 << 25) || ldg < K)
+    return fail(NEB_ERR_SHAPE, "neb_gram_split: bad sizes (ld must be below 2^25)");
   if (!phi_hi || !phi_lo || !G || !workspace) return fail(NEB_ERR_ARG, "neb_gram_split: null pointer");
   if (!aligned(phi_hi, ld, NEB_BF16) || !aligned(phi_lo, ld, NEB_BF16))
     return fail(NEB_ERR_ARG, "neb_gram_split: rows must be 16-byte aligned");

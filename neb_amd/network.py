@@ -19,6 +19,7 @@ channel order value, grads, Hessian pairs (0,0),(0,1),(1,1) [2D] /
 from __future__ import annotations
 
 import math
+from contextlib import nullcontext
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -93,13 +94,25 @@ class Workspace:
 
 
 _WS: dict = {}
+_SIDE: dict = {}
 
 
 def workspace(device) -> Workspace:
-    key = str(device)
+    """Scratch buffer of `device` for the current stream: kernels queued on
+    different streams never share one."""
+    key = (str(device), torch.cuda.current_stream(device).cuda_stream)
     if key not in _WS:
         _WS[key] = Workspace(device)
     return _WS[key]
+
+
+def side_stream(device) -> torch.cuda.Stream:
+    """Second stream of `device` (default priority) for work that overlaps
+    the current stream: the weight gradients beside the jet reverse."""
+    key = str(device)
+    if key not in _SIDE:
+        _SIDE[key] = torch.cuda.Stream(device=device)
+    return _SIDE[key]
 
 
 class EigenfluidNet:
@@ -422,6 +435,21 @@ def field_gram(fields: Fields, which: str, w: float, mode: str = "bf16x3", out: 
     return gram(phi, fields.K, w, mode, out=out, beta=beta, lo=lo)
 
 
+def field_grams(fields: Fields, w: float, mode: str = "bf16x3", M: torch.Tensor | None = None,
+                S: torch.Tensor | None = None, beta: float = 0.0, mass_events=None):
+    """(mass Gram, stiffness Gram) as field_gram, one after the other on the
+    current stream (run side by side they contend for the same memory path
+    and finish later).  mass_events: optional (start, end) events recorded
+    around the mass Gram."""
+    if mass_events is not None:
+        mass_events[0].record()
+    M = field_gram(fields, "mass", w, mode, out=M, beta=beta if M is not None else 0.0)
+    if mass_events is not None:
+        mass_events[1].record()
+    S = field_gram(fields, "stiff", w, mode, out=S, beta=beta if S is not None else 0.0)
+    return M, S
+
+
 def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, beta: float = 0.0):
     """sum_s sign_s X_s^T Y_s over rows; segments = [(X, Y, rows, sign)] with
     X/Y row-major 2-D tensors (views allowed)."""
@@ -529,12 +557,6 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     for src, dst in ((Bm, bm), (Bs, bs)):
         H.call("neb_stage", H.F64, prec, src.data_ptr(), src.shape[0], K, src.stride(0), dst.data_ptr(), dst.shape[1],
                dst.shape[0], dst.shape[1], 0, 2.0 * w, s)
-    A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
-    Z = tape.pre[-1]
-    H.call("neb_jet_output_backward", prec, d, cfg.act_code, n, width, K, fields.mass.data_ptr(),
-           fields.stiff.data_ptr(), ldK, _DT[fields.mass.dtype], fields.layout, bm.data_ptr(), bs.data_ptr(),
-           bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
-
     if grads is None:
         grads = torch.empty_like(net.params)
     gW, gb = net.split(grads)
@@ -544,6 +566,58 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     U, Dg = fields.mass, fields.stiff
     if fields.layout != 0:
         raise ValueError("backward_fields expects point-major field matrices")
+    # The weight gradients (cross products over points) only read the fields,
+    # the tape and each layer's adjoint once it exists: they run on the side
+    # stream, overlapping the jet reverse on the current stream, which waits
+    # for them before returning.
+    on_gpu = net.device.type == "cuda"
+    main = torch.cuda.current_stream(net.device) if on_gpu else None
+    side = side_stream(net.device) if on_gpu else None
+    if on_gpu:
+        side.wait_stream(main)
+    A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
+    Z = tape.pre[-1]
+    H.call("neb_jet_output_backward", prec, d, cfg.act_code, n, width, K, fields.mass.data_ptr(),
+           fields.stiff.data_ptr(), ldK, _DT[fields.mass.dtype], fields.layout, bm.data_ptr(), bs.data_ptr(),
+           bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
+    ctx = torch.cuda.stream(side) if on_gpu else nullcontext()
+    with ctx:
+        _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, dM, dS, w, gW, gb)
+
+    for l in range(L - 1, -1, -1):
+        X = tape.inputs[l]
+        k = X.shape[2] if l > 0 else cfg.features
+        wl = cfg.widths[l]
+        if on_gpu:
+            side.wait_stream(main)       # A (layer l's adjoint) is complete
+            A.record_stream(side)
+        with (torch.cuda.stream(side) if on_gpu else nullcontext()):
+            A2 = A.view(C * n, A.shape[2])
+            X2 = X.view(C * n, X.shape[2])
+            G = cross([(A2, X2, C * n, 1.0)], wl, k, mode)
+            gW[l].copy_(G)
+            cw = workspace(net.device)
+            ws = cw.get(H.load().neb_colsum_workspace_size(n, wl))
+            db = torch.empty(wl, dtype=torch.float64, device=net.device)
+            H.call("neb_colsum", _DT[st], A[0].data_ptr(), n, wl, A.shape[2], 0.0, db.data_ptr(), ws.data_ptr(),
+                   ws.numel(), _stream())
+            gb[l].copy_(db)
+        if l > 0:
+            Zp = tape.pre[l - 1]
+            An = torch.empty((C, n, _ld(k)), dtype=st, device=net.device)
+            WT = stg["WT"][l]
+            H.call("neb_jet_layer_backward", prec, d, cfg.act_code, n, k, wl, A.data_ptr(), A.shape[2],
+                   WT.data_ptr(), WT.shape[1], Zp.data_ptr(), Zp.shape[2], cfg.omega(l - 1), An.data_ptr(),
+                   An.shape[2], s)
+            A = An
+    if on_gpu:
+        main.wait_stream(side)
+    return grads
+
+
+def _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, dM, dS, w, gW, gb):
+    """Output-layer weight gradient 2w (dL/dM P_m + dL/dS P_s) with P the
+    field x last-hidden-jet cross products (VCORE:94-95 through the lift)."""
     if d == 2:
         u, v = _rows_view(U, 0, 2), _rows_view(U, 1, 2)
         Pm = cross([(u, ch(Ylast, 2), n, 1.0), (v, ch(Ylast, 1), n, -1.0)], K, width, mode)
@@ -562,30 +636,5 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
             Pd = torch.zeros_like(Pm)
             for i in range(0, len(segs), 8):
                 cross(segs[i:i + 8], K, width, mode, out=Pd, beta=1.0 if i else 0.0)
-            gW[-1][sl::3].copy_(((2.0 * w) * (dM @ Pm + dS @ Pd)).to(grads.dtype))
+            gW[-1][sl::3].copy_(((2.0 * w) * (dM @ Pm + dS @ Pd)))
     gb[-1].zero_()
-
-    cw = workspace(net.device)
-    for l in range(L - 1, -1, -1):
-        X = tape.inputs[l]
-        k = X.shape[2] if l > 0 else cfg.features
-        wl = cfg.widths[l]
-        A2 = A.view(C * n, A.shape[2])
-        X2 = X.view(C * n, X.shape[2])
-        G = cross([(A2, X2, C * n, 1.0)], wl, k, mode)
-        gW[l].copy_(G)
-        need = H.load().neb_colsum_workspace_size(n, wl)
-        ws = cw.get(need)
-        db = torch.empty(wl, dtype=torch.float64, device=net.device)
-        H.call("neb_colsum", _DT[st], A[0].data_ptr(), n, wl, A.shape[2], 0.0, db.data_ptr(), ws.data_ptr(),
-               ws.numel(), s)
-        gb[l].copy_(db)
-        if l > 0:
-            Zp = tape.pre[l - 1]
-            An = torch.empty((C, n, _ld(k)), dtype=st, device=net.device)
-            WT = stg["WT"][l]
-            H.call("neb_jet_layer_backward", prec, d, cfg.act_code, n, k, wl, A.data_ptr(), A.shape[2],
-                   WT.data_ptr(), WT.shape[1], Zp.data_ptr(), Zp.shape[2], cfg.omega(l - 1), An.data_ptr(),
-                   An.shape[2], s)
-            A = An
-    return grads

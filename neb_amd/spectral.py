@@ -230,8 +230,7 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
         tape = nw.trunk_forward(net, points, p)
         fields = nw.output_fields(net, tape, split=split)
     with timed(tt, "gram_assembly"):
-        M = nw.field_gram(fields, "mass", weight, gram_mode)
-        S = nw.field_gram(fields, "stiff", weight, gram_mode)
+        M, S = nw.field_grams(fields, weight, gram_mode)
     with timed(tt, "spectral_objective"):
         loss, dM, dS = spectral_objective(M, S, lam, ridge, gate=gate)
     out = dict(loss=loss, M=M, S=S, fields=fields, timings=tt)
@@ -263,8 +262,7 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
             tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
             f = nw.output_fields(net, tape, split=split)
             del tape
-            nw.field_gram(f, "mass", weight, gram_mode, out=M, beta=1.0)
-            nw.field_gram(f, "stiff", weight, gram_mode, out=S, beta=1.0)
+            nw.field_grams(f, weight, gram_mode, M=M, S=S, beta=1.0)
             del f
     with timed(tt, "spectral_objective"):
         loss, dM, dS = solver(M, S)
@@ -360,8 +358,7 @@ class JetGram(torch.autograd.Function):
         p = nw.PRECISIONS[prec]
         tape = nw.trunk_forward(net, points, p)
         fields = nw.output_fields(net, tape, split=nw.split_fields_for(prec, gram_mode))
-        M = nw.field_gram(fields, "mass", weight, gram_mode)
-        S = nw.field_gram(fields, "stiff", weight, gram_mode)
+        M, S = nw.field_grams(fields, weight, gram_mode)
         ctx.state = (net, tape, fields, weight)
         return M, S
 
