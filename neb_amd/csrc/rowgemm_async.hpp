@@ -115,15 +115,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
     }
   };
 
+  // Tiles are numbered feature-block-major (t = tf * npb + tp) and dealt in
+  // contiguous ranges: the workgroups that read the same point block at the
+  // same time are npb / per_wg apart, i.e. on one XCD (a multiple of 8 in the
+  // launches here), so each point block comes from HBM once and each
+  // workgroup's weight slice stays in L2.
   int64_t lt = t_beg;  // copy cursor (tile, its point / feature block, k-step)
-  int64_t ltp = t_beg / nfb, ltf = t_beg % nfb;
+  int64_t ltp = t_beg % npb, ltf = t_beg / npb;
   int lk = 0;
   int nissued = 0;
   auto advance = [&]() __attribute__((always_inline)) {
     if (++lk == nk) {
       lk = 0;
       ++lt;
-      if (++ltf == nfb) { ltf = 0; ++ltp; }
+      if (++ltp == npb) { ltp = 0; ++ltf; }
     }
   };
 #pragma unroll
@@ -146,7 +151,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   zero_acc();
 
   int64_t ct = t_beg;  // compute cursor
-  int64_t ctp = t_beg / nfb, ctf = t_beg % nfb;
+  int64_t ctp = t_beg % npb, ctf = t_beg / npb;
   int ck = 0, kdone = 0;
   // always inlined: the accumulators must stay in registers
   auto step = [&](lds_S* cur, lds_S* refill) __attribute__((always_inline)) -> bool {
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
       zero_acc();
       ck = 0;
       ++ct;
-      if (++ctf == nfb) { ctf = 0; ++ctp; }
+      if (++ctp == npb) { ctp = 0; ++ctf; }
     }
     return true;
   };

@@ -204,10 +204,14 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
 #pragma unroll
           for (int q = 0; q < 4; ++q) Q[r][q] = r == q ? 1.0 : 0.0;  // keep the arithmetic finite
       }
-      // k of the MFMA = lane group g: x_i[g] = v_(i,b) P^-1[:, g], -r_g(j)
+      // k of the MFMA = lane group g: x_i[g] = v_(i,b) P^-1[:, g], -r_g(j).
+      // Lane-dependent choices are exact arithmetic masks (x 1.0 / x 0.0 of
+      // finite values), not selects: the compiler turns those into divergent
+      // branches with an LDS wait inside.
+      const double e0 = g == 0 ? 1.0 : 0.0, e1 = g == 1 ? 1.0 : 0.0, e2 = g == 2 ? 1.0 : 0.0, e3 = g == 3 ? 1.0 : 0.0;
       double qc[4];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) qc[a] = g == 0 ? Q[a][0] : g == 1 ? Q[a][1] : g == 2 ? Q[a][2] : Q[a][3];
+      for (int a = 0; a < 4; ++a) qc[a] = Q[a][0] * e0 + Q[a][1] * e1 + Q[a][2] * e2 + Q[a][3] * e3;
       double x[2], y[2];
 #pragma unroll
       for (int tr = 0; tr < 2; ++tr) {
@@ -215,12 +219,12 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
         const f64x2 a01 = *reinterpret_cast<const f64x2*>(&cb[i * 4]);
         const f64x2 a23 = *reinterpret_cast<const f64x2*>(&cb[i * 4 + 2]);
         const double xv = a01[0] * qc[0] + a01[1] * qc[1] + a23[0] * qc[2] + a23[1] * qc[3];
-        x[tr] = (i >> 2) == b ? 0.0 : xv;
+        x[tr] = xv * ((i >> 2) == b ? 0.0 : 1.0);
       }
 #pragma unroll
       for (int tc = 0; tc < 2; ++tc) {
         const int j = spd_col(C0, tc, lane);
-        y[tc] = (j >> 2) == b ? 0.0 : -rb[g * SPD_NB + j];
+        y[tc] = -rb[g * SPD_NB + j] * ((j >> 2) == b ? 0.0 : 1.0);
       }
 #pragma unroll
       for (int tr = 0; tr < 2; ++tr)
@@ -230,11 +234,9 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
         if (tq) v[1][1][s] = -v[1][1][s];
         else v[0][0][s] = -v[0][0][s];
       }
-      if (t == 0) {
+      if (w == 0 && r16 == 0) {  // lane 16 g stores column g of P^-1
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) pivs[b * 16 + 4 * r + q] = Q[r][q];
+        for (int r = 0; r < 4; ++r) pivs[b * 16 + 4 * r + g] = qc[r];
       }
     }
   }
