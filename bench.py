@@ -601,7 +601,7 @@ def main():
     achieved = gram_flops / (gram_ms * 1e-3)
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK["bf16"] / 1e12, unit="TFLOP/s",
                     frac=achieved / PEAK["bf16"], traffic=profile_traffic("gram_split_k1024_pmc.md"),
-                    kernel="neb_gram_split mass Gram: cross_kernel<CR_BF16X3, SplitBF16> + cross_reduce_kernel, "
+                    kernel="neb_gram_split mass Gram: cross_async_kernel<2, 2, 2, 2, false> + cross_reduce_kernel, "
                            "R=131072, K=1024", launch_ms=gram_ms, flops_per_launch=gram_flops)
 
     cpu = None if args.no_cpu else cpu_baseline()
