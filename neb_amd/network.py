@@ -503,14 +503,33 @@ def spectral_terms(M: torch.Tensor, S: torch.Tensor, X: torch.Tensor, info: torc
     return loss, X, Ss, gate
 
 
-def mass_adjoint(X: torch.Tensor, Ss: torch.Tensor, M: torch.Tensor, lam: float) -> torch.Tensor:
-    """dL/dM = -sym(X Ss X) + 2 lam (M - I) (two FP64 GEMMs + one launch)."""
+def mass_adjoint(X: torch.Tensor, Ss: torch.Tensor, M: torch.Tensor, lam: float,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """dL/dM = -sym(X Ss X) + 2 lam (M - I) (two FP64 GEMMs + one launch),
+    into `out` (a K x K view with unit column stride) when given."""
     M = _f64_rows(M)
     K = M.shape[0]
     C = torch.mm(torch.mm(X, Ss), X)
-    H.call("neb_spectral_adjoint", K, C.data_ptr(), K, M.data_ptr(), M.stride(0), float(lam), C.data_ptr(), K,
-           _stream())
-    return C
+    out = C if out is None else out
+    H.call("neb_spectral_adjoint", K, C.data_ptr(), K, M.data_ptr(), M.stride(0), float(lam), out.data_ptr(),
+           out.stride(0), _stream())
+    return out
+
+
+def adjoint_pair(K: int, device) -> torch.Tensor:
+    """[K, 2K] FP64 buffer holding dL/dM (columns :K) and dL/dS (columns K:)
+    side by side: backward_fields then forms B = W^T [dL/dM | dL/dS] and the
+    output-layer gradient [dL/dM | dL/dS] [P_m; P_s] as one GEMM each."""
+    return torch.empty((K, 2 * K), dtype=torch.float64, device=device)
+
+
+def _as_pair(dM: torch.Tensor, dS: torch.Tensor):
+    """The [K, 2K] buffer of adjoint_pair when dM and dS are its two halves."""
+    K = dM.shape[0]
+    if (dM.dtype == dS.dtype == torch.float64 and dM.shape == dS.shape == (K, K) and dM.stride() == (2 * K, 1)
+            and dS.stride() == (2 * K, 1) and dS.data_ptr() == dM.data_ptr() + 8 * K):
+        return dM.as_strided((K, 2 * K), (2 * K, 1))
+    return None
 
 
 def _rows_view(t: torch.Tensor, start_row: int, step: int) -> torch.Tensor:
@@ -539,24 +558,28 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     width = cfg.widths[-1]
     st = _STORE[prec]
     s = _stream()
-    if symmetric:
-        dM, dS = _f64_rows(dM), _f64_rows(dS)
-    else:
-        dM, dS = sym_shift(dM), sym_shift(dS)
+    pair = _as_pair(dM, dS) if symmetric else None
+    if pair is None:
+        pair = adjoint_pair(K, net.device)
+        if symmetric:
+            pair[:, :K].copy_(dM)
+            pair[:, K:].copy_(dS)
+        else:
+            sym_shift(dM, out=pair[:, :K])
+            sym_shift(dS, out=pair[:, K:])
+    dM, dS = pair[:, :K], pair[:, K:]
     Wout = net.W[-1].to(torch.float64)               # [ncomp*K, width]
     ldK = fields.mass.shape[1]
     if d == 2:
-        Bm = torch.mm(Wout.T, dM)                    # [width, K]
-        Bs = torch.mm(Wout.T, dS)
+        Bp = torch.mm(Wout.T, pair)                  # [width, 2K] = [W^T dL/dM | W^T dL/dS]
     else:
-        Bm = torch.cat([Wout[c::3].T @ dM for c in range(3)], 0)  # [3*width, K]
-        Bs = torch.cat([Wout[c::3].T @ dS for c in range(3)], 0)
+        Bp = torch.cat([torch.mm(Wout[c::3].T, pair) for c in range(3)], 0)  # [3*width, 2K]
     # 2w B in the operand precision, zero-padded to the row stride (one launch each)
-    bm = torch.empty((Bm.shape[0], _ld(K)), dtype=st, device=net.device)
+    bm = torch.empty((Bp.shape[0], _ld(K)), dtype=st, device=net.device)
     bs = torch.empty_like(bm)
-    for src, dst in ((Bm, bm), (Bs, bs)):
-        H.call("neb_stage", H.F64, prec, src.data_ptr(), src.shape[0], K, src.stride(0), dst.data_ptr(), dst.shape[1],
-               dst.shape[0], dst.shape[1], 0, 2.0 * w, s)
+    for half, dst in ((0, bm), (1, bs)):
+        H.call("neb_stage", H.F64, prec, Bp.data_ptr() + 8 * K * half, Bp.shape[0], K, Bp.stride(0), dst.data_ptr(),
+               dst.shape[1], dst.shape[0], dst.shape[1], 0, 2.0 * w, s)
     if grads is None:
         grads = torch.empty_like(net.params)
     gW, gb = net.split(grads)
@@ -582,7 +605,7 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
            bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
     ctx = torch.cuda.stream(side) if on_gpu else nullcontext()
     with ctx:
-        _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, dM, dS, w, gW, gb)
+        _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, pair, w, gW, gb)
 
     for l in range(L - 1, -1, -1):
         X = tape.inputs[l]
@@ -615,15 +638,16 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     return grads
 
 
-def _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, dM, dS, w, gW, gb):
+def _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, pair, w, gW, gb):
     """Output-layer weight gradient 2w (dL/dM P_m + dL/dS P_s) with P the
-    field x last-hidden-jet cross products (VCORE:94-95 through the lift)."""
+    field x last-hidden-jet cross products (VCORE:94-95 through the lift),
+    as one GEMM [dL/dM | dL/dS] [P_m; P_s] (the adjoints are symmetric)."""
     if d == 2:
         u, v = _rows_view(U, 0, 2), _rows_view(U, 1, 2)
-        Pm = cross([(u, ch(Ylast, 2), n, 1.0), (v, ch(Ylast, 1), n, -1.0)], K, width, mode)
-        Ps = cross([(Dg, ch(Ylast, 3), n, -1.0), (Dg, ch(Ylast, 5), n, -1.0)], K, width, mode)
-        gout = torch.addmm(Pm, dM, Pm, beta=0.0, alpha=2.0 * w)
-        gW[-1].copy_(gout.addmm_(dS, Ps, alpha=2.0 * w))
+        P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
+        cross([(u, ch(Ylast, 2), n, 1.0), (v, ch(Ylast, 1), n, -1.0)], K, width, mode, out=P[:K])
+        cross([(Dg, ch(Ylast, 3), n, -1.0), (Dg, ch(Ylast, 5), n, -1.0)], K, width, mode, out=P[K:])
+        gW[-1].copy_(torch.addmm(P[:K], pair, P, beta=0.0, alpha=2.0 * w))
     else:
         from .lifts import curl_terms
         comps = [_rows_view(U, a, 3) for a in range(3)]
@@ -632,9 +656,9 @@ def _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, dM, dS, w, 
             segm = [(comps[a], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a < 3 and w_s == sl]
             segs = [(grads3[a - 3], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms()
                     if a >= 3 and w_s == sl]
-            Pm = cross(segm, K, width, mode)
-            Pd = torch.zeros_like(Pm)
+            P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
+            cross(segm, K, width, mode, out=P[:K])
             for i in range(0, len(segs), 8):
-                cross(segs[i:i + 8], K, width, mode, out=Pd, beta=1.0 if i else 0.0)
-            gW[-1][sl::3].copy_(((2.0 * w) * (dM @ Pm + dS @ Pd)))
+                cross(segs[i:i + 8], K, width, mode, out=P[K:], beta=1.0 if i else 0.0)
+            gW[-1][sl::3].copy_(torch.addmm(P[:K], pair, P, beta=0.0, alpha=2.0 * w))
     gb[-1].zero_()

@@ -28,12 +28,13 @@ def _sym(A):
     return 0.5 * (A + A.T)
 
 
-def _spd_inverse_async(M: torch.Tensor):
-    """Enqueue the FP64 SPD inverse; returns (X, info) without synchronising
+def _spd_inverse_async(M: torch.Tensor, out: torch.Tensor | None = None):
+    """Enqueue the FP64 SPD inverse (into `out`, a K x K view with unit
+    column stride, when given); returns (X, info) without synchronising
     (info[0] != 0 names the first nonpositive pivot, 1-based)."""
     K = M.shape[0]
     A = M.to(torch.float64).contiguous()
-    X = torch.empty_like(A)
+    X = torch.empty_like(A) if out is None else out
     info = torch.empty(1, dtype=torch.int32, device=A.device)
     need = nw.H.load().neb_spd_inverse_workspace_size(K)
     ws = nw.workspace(A.device).get(need)
@@ -96,10 +97,12 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
         ok = bool(torch.isfinite(e).all()) and float(e[-1]) > 0 and float(e[0] / e[-1]) >= 1.0 / cond_limit
         if not ok:
             raise FloatingPointError("Physical mass zero/rank deficient/condition exceeds limit; no update")
-    X, info = _spd_inverse_async(Mr)
+    K = M.shape[0]
+    pair = nw.adjoint_pair(K, M.device)      # [dL/dM | dL/dS]
+    X, info = _spd_inverse_async(Mr, out=pair[:, K:])
     loss, X, Ss, g = nw.spectral_terms(M, S, X, info, lam, ridge, math.inf)
     _raise_gate(int(g.item()))
-    dM = nw.mass_adjoint(X, Ss, M, lam)
+    dM = nw.mass_adjoint(X, Ss, M, lam, out=pair[:, :K])
     return loss, dM, X
 
 
@@ -172,9 +175,12 @@ class SpectralSolver:
                 X = None
                 self._skip = self.backoff
         info = None
+        pair = nw.adjoint_pair(K, M.device)  # [dL/dM | dL/dS] side by side for the reverse
         if X is None:
-            X, info = _spd_inverse_async(Mr)
+            X, info = _spd_inverse_async(Mr, out=pair[:, K:])
             self.last_path = "spd-inverse"
+        else:
+            X = pair[:, K:].copy_(X)
         # X <- sym(X), loss, and the positive-definiteness / condition gates
         # on the device (one fused launch + one reduction block)
         loss, X, Ss, gate = nw.spectral_terms(M, S, X, info, self.lam, self.ridge, self.cond_limit)
@@ -183,7 +189,7 @@ class SpectralSolver:
         else:
             self.raise_gate(int(gate.item()))  # one host synchronisation for both gates
         self.X = X
-        dM = nw.mass_adjoint(X, Ss, M, self.lam)
+        dM = nw.mass_adjoint(X, Ss, M, self.lam, out=pair[:, :K])
         return loss, dM, X
 
 
