@@ -2,7 +2,8 @@
 // CR_BF16 product; NL = 2: hi and lo planes, the CR_BF16X3 product
 // hi.hi + hi.lo + lo.hi), fed by asynchronous global -> LDS copies.
 //
-// Same contraction, partial-tile layout and reduction as cross_kernel
+// Same contraction, partial-tile layout and reduction as cross_kernel (with
+// fp32 partial tiles)
 // (crossgemm.hpp); what differs is the data path:
 //   * global_load_lds_dwordx4 moves 16 bytes per lane straight into LDS, so
 //     no VGPRs stage the tiles and no VALU converts them; a ring of NS LDS
@@ -299,7 +300,9 @@ cross_async_kernel(CrossArgs a) {
     }
   }
 
-  double* out = a.partial + ((int64_t)split * a.ntiles + tile) * (CR_TM * CR_TN);
+  // fp32 partial tile: the sums above are fp32 already (exactly the values
+  // an FP64 tile would hold, at half the bytes); the reduction adds in FP64
+  float* out = reinterpret_cast<float*>(a.partial) + ((int64_t)split * a.ntiles + tile) * (CR_TM * CR_TN);
 #pragma unroll
   for (int i = 0; i < BI; ++i)
 #pragma unroll
@@ -308,7 +311,7 @@ cross_async_kernel(CrossArgs a) {
       for (int r = 0; r < 4; ++r) {
         const int li = wm * 16 * BI + i * 16 + 4 * (lane >> 4) + r;
         const int lj = wn * 16 * BJ + j * 16 + (lane & 15);
-        if constexpr (TWO) out[li * CR_TN + lj] = (double)outer[i][j][r];
-        else out[li * CR_TN + lj] = (double)inner[i][j][r];
+        if constexpr (TWO) out[li * CR_TN + lj] = outer[i][j][r];
+        else out[li * CR_TN + lj] = inner[i][j][r];
       }
 }

@@ -430,12 +430,14 @@ write_out:
       }
 }
 
-// G = beta*G + scale * sum_split partial, over 32x32 output blocks.  For the
+// G = beta*G + scale * sum_split partial (FP64 sums of FP64 or fp32 partial
+// tiles), over 32x32 output blocks.  For the
 // symmetric mode only blocks with bi <= bj run; each writes (i,j) and the
 // mirrored (j,i) through an LDS transpose, and diagonal blocks take every
 // value from the upper triangle, so G is exactly symmetric.  All global
 // reads and writes are row-contiguous.
-__global__ __launch_bounds__(256) void cross_reduce_kernel(const double* __restrict__ partial, int splits,
+template <typename TP>
+__global__ __launch_bounds__(256) void cross_reduce_kernel(const TP* __restrict__ partial, int splits,
                                                             int ntiles, int tiles_n, int symmetric, int64_t m,
                                                             int64_t n, double scale, double beta,
                                                             double* __restrict__ G, int64_t ldg) {
@@ -462,16 +464,16 @@ __global__ __launch_bounds__(256) void cross_reduce_kernel(const double* __restr
     if (i < m && j < n) {
       const int64_t ti = i / CR_TM, tj = j / CR_TN;
       const int64_t tile = symmetric ? ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti) : ti * tiles_n + tj;
-      const double* p = partial + tile * (CR_TM * CR_TN) + (i % CR_TM) * CR_TN + (j % CR_TN);
+      const TP* p = partial + tile * (CR_TM * CR_TN) + (i % CR_TM) * CR_TN + (j % CR_TN);
       int sp = 0;
       for (; sp + 8 <= splits; sp += 8) {  // issue 8 independent loads, then add in order
-        double t[8];
+        TP t[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) t[u] = p[(sp + u) * stride];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) sacc += t[u];
+        for (int u = 0; u < 8; ++u) sacc += (double)t[u];
       }
-      for (; sp < splits; ++sp) sacc += p[sp * stride];
+      for (; sp < splits; ++sp) sacc += (double)p[sp * stride];
     }
     val[q] = sacc;
     tr[row][tx] = sacc;

@@ -286,7 +286,7 @@ int launch_cross(CrossArgs& a, double scale, double beta, double* G, int64_t ldg
   if (int rc = hip_status("neb_cross")) return rc;
   const int64_t nbi = (a.m + 31) / 32, nbj = (a.n + 31) / 32;
   const int64_t nblk = a.symmetric ? nbi * (nbi + 1) / 2 : nbi * nbj;
-  hipLaunchKernelGGL(cross_reduce_kernel, dim3((unsigned)nblk), dim3(256), 0, st, a.partial, a.splits, a.ntiles,
+  hipLaunchKernelGGL(cross_reduce_kernel<double>, dim3((unsigned)nblk), dim3(256), 0, st, a.partial, a.splits, a.ntiles,
                      a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
   return hip_status("neb_cross reduce");
 }
@@ -316,7 +316,8 @@ int launch_cross_async_as(CrossArgs& a, double scale, double beta, double* G, in
   if (int rc = hip_status("neb_cross (async)")) return rc;
   const int64_t nbi = (a.m + 31) / 32, nbj = (a.n + 31) / 32;
   const int64_t nblk = a.symmetric ? nbi * (nbi + 1) / 2 : nbi * nbj;
-  hipLaunchKernelGGL(cross_reduce_kernel, dim3((unsigned)nblk), dim3(256), 0, st, a.partial, a.splits, a.ntiles,
+  hipLaunchKernelGGL(cross_reduce_kernel<float>, dim3((unsigned)nblk), dim3(256), 0, st,
+                     reinterpret_cast<const float*>(a.partial), a.splits, a.ntiles,
                      a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
   return hip_status("neb_cross reduce");
 }
