@@ -102,8 +102,10 @@ def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             timer.events.setdefault("gram_mass_launch", []).append(list(ev))
         M, S = mark(lambda: nw.field_grams(fields, w, gram_mode, mass_events=ev), "grams")
+        # output-layer products on the side stream, overlapping the FP64 objective
+        prod = nw.output_products(net, tape, fields)
         loss, dM, dS = mark(lambda: solver(M, S), "objective_fp64")
-        g = mark(lambda: nw.backward_fields(net, tape, fields, dM, dS, w, symmetric=True), "reverse")
+        g = mark(lambda: nw.backward_fields(net, tape, fields, dM, dS, w, symmetric=True, products=prod), "reverse")
         # the positive-definiteness / condition gates and the gradient gate are
         # evaluated on the device; adam synchronises once, after queueing
         mark(lambda: net.adam(g, 1e-4, gate=solver.gate, on_gate=solver.raise_gate), "adam")

@@ -251,6 +251,27 @@ int cross_splits(int ntiles, int64_t rows, int slots = CR_MAX_SLOTS) {
   return s;
 }
 
+// Workgroups per CU that the asynchronous cross launches of the calling
+// thread may occupy (0: as many as fit, at most 2); see
+// neb_set_cross_occupancy.
+thread_local int g_cross_per_cu = 0;
+
+// CU count of the current device (0 if unknown)
+int device_cus() {
+  static int cache[16] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 0;
+  if (cache[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return 0;
+    }
+    cache[dev] = cus;
+  }
+  return cache[dev];
+}
+
 // Resident workgroups of a kernel on the current device times the CU count
 // (at most 2 per CU), queried once per kernel and device into `cache`.
 template <typename Kern>
@@ -310,7 +331,10 @@ int launch_cross_async_as(CrossArgs& a, double scale, double beta, double* G, in
   int64_t maxrows = 0;
   for (int s = 0; s < a.nseg; ++s) maxrows = a.seg[s].rows > maxrows ? a.seg[s].rows : maxrows;
   static int slots_cache[16] = {0};  // one per kernel instantiation
-  const int fit = cross_splits(a.ntiles, maxrows, resident_slots(kern, lds, slots_cache, NT));
+  int slots = resident_slots(kern, lds, slots_cache, NT);
+  const int cus = device_cus();
+  if (g_cross_per_cu > 0 && cus > 0 && slots > cus * g_cross_per_cu) slots = cus * g_cross_per_cu;
+  const int fit = cross_splits(a.ntiles, maxrows, slots);
   if (fit < a.splits) a.splits = fit;
   hipLaunchKernelGGL(kern, dim3(a.ntiles * a.splits), dim3(NT), lds, st, a);
   if (int rc = hip_status("neb_cross (async)")) return rc;
@@ -414,6 +438,12 @@ int stage_dst(int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_
 
 // ==========================================================================
 extern "C" {
+
+int neb_set_cross_occupancy(int per_cu) {
+  if (per_cu < 0 || per_cu > 2) return fail(NEB_ERR_ARG, "neb_set_cross_occupancy: 0, 1 or 2");
+  g_cross_per_cu = per_cu;
+  return NEB_OK;
+}
 
 int neb_abi_version(void) { return NEB_ABI_VERSION; }
 const char* neb_last_error(void) { return g_err.c_str(); }

@@ -302,10 +302,12 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
                                                        int64_t K, int k, const double* __restrict__ Pi,
                                                        double* __restrict__ Pi_next, double* __restrict__ dst,
                                                        int64_t ldd, int last, int* __restrict__ info) {
+  // two LDS tiles (70 KB): a workgroup fits beside a 64 KB cross-GEMM
+  // workgroup on a CU, so the sweeps share the GPU with the output-layer
+  // products queued on another stream
   extern __shared__ __align__(16) double spd_smem[];
   double* L0 = spd_smem;
   double* L1 = L0 + SPD_TILE;
-  double* L2 = L1 + SPD_TILE;
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
@@ -355,7 +357,6 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
         }
     spd_put(L0, pa, t);
     spd_put(L1, pp, t);
-    spd_put(L2, pb, t);
     __syncthreads();
     f64x4 tt[2][2];
 #pragma unroll
@@ -363,10 +364,11 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
 #pragma unroll
       for (int tc = 0; tc < 2; ++tc) tt[tr][tc] = f64x4{0.0, 0.0, 0.0, 0.0};
     spd_mma(L0, L1, R0, C0, lane, tt);  // T = A_ik Pi
-    __syncthreads();                     // every wave is done reading L0
+    __syncthreads();                     // every wave is done reading L0 and L1
     spd_acc_store(L0, tt, R0, C0, lane);
+    spd_put(L1, pb, t);                  // A_jk (held in registers) replaces Pi
     __syncthreads();
-    spd_mma(L0, L2, R0, C0, lane, acc);  // T A_jk^T
+    spd_mma(L0, L1, R0, C0, lane, acc);  // T A_jk^T
 #pragma unroll
     for (int tr = 0; tr < 2; ++tr)
 #pragma unroll
@@ -389,5 +391,6 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
       }
 }
 
-constexpr size_t spd_step_lds_bytes() { return 3 * sizeof(double) * SPD_TILE; }
+constexpr size_t spd_step_lds_bytes() { return 2 * sizeof(double) * SPD_TILE; }
+static_assert(SPD_PIVOT_LDS <= 2 * SPD_TILE, "the pivot inversion reuses the sweep's LDS tiles");
 constexpr size_t spd_pivot0_lds_bytes() { return sizeof(double) * SPD_PIVOT_LDS; }

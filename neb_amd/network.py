@@ -542,14 +542,16 @@ def _rows_view(t: torch.Tensor, start_row: int, step: int) -> torch.Tensor:
 # reverse pass
 # ---------------------------------------------------------------------------
 def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Tensor, dS: torch.Tensor,
-                    w: float, grads: torch.Tensor | None = None, symmetric: bool = False) -> torch.Tensor:
+                    w: float, grads: torch.Tensor | None = None, symmetric: bool = False,
+                    products: "OutputProducts | None" = None) -> torch.Tensor:
     """Parameter gradient of L(M, S) with M = w U^T U, S = w D^T D, given
     dL/dM and dL/dS (symmetric, FP64 K x K).  Fuses the field adjoint
     2w Phi dL/dG (VCORE:94-95) into the output-layer reverse, then runs the
     handwritten jet reverse through every layer (HB2P:2087-2139, VJ2:77-82).
     symmetric=True skips the symmetrisation of dM and dS for adjoints that
-    are symmetric by construction (SpectralSolver's).  Returns the flat
-    gradient in the master dtype."""
+    are symmetric by construction (SpectralSolver's); `products` are the
+    output-layer products queued earlier by output_products (computed here
+    when None).  Returns the flat gradient in the master dtype."""
     cfg = net.cfg
     prec = tape.prec
     stg = net.staged(prec)
@@ -584,9 +586,6 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
         grads = torch.empty_like(net.params)
     gW, gb = net.split(grads)
     mode = _CROSS[prec]
-    Ylast = tape.inputs[-1]
-    ch = lambda t, c: t[c]  # noqa: E731  packed channel c as [n, ld]
-    U, Dg = fields.mass, fields.stiff
     if fields.layout != 0:
         raise ValueError("backward_fields expects point-major field matrices")
     # The weight gradients (cross products over points) only read the fields,
@@ -605,7 +604,13 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
            bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
     ctx = torch.cuda.stream(side) if on_gpu else nullcontext()
     with ctx:
-        _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, pair, w, gW, gb)
+        if products is None:
+            P = _products(net, tape, fields)
+        else:
+            if on_gpu and products.event is not None:
+                side.wait_event(products.event)
+            P = products.P
+        _output_layer_grads(d, K, P, pair, w, gW, gb)
 
     for l in range(L - 1, -1, -1):
         X = tape.inputs[l]
@@ -638,27 +643,80 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     return grads
 
 
-def _output_layer_grads(net, d, n, K, width, mode, U, Dg, Ylast, ch, pair, w, gW, gb):
-    """Output-layer weight gradient 2w (dL/dM P_m + dL/dS P_s) with P the
-    field x last-hidden-jet cross products (VCORE:94-95 through the lift),
-    as one GEMM [dL/dM | dL/dS] [P_m; P_s] (the adjoints are symmetric)."""
+def _products(net: EigenfluidNet, tape: Tape, fields: Fields) -> list:
+    """[P_m; P_s] ([2K, width] FP64; one per output component slice in 3D):
+    the field x last-hidden-jet cross products over points of the
+    output-layer gradient (VCORE:94-95 through the lift)."""
+    cfg = net.cfg
+    d, n, K = cfg.dim, tape.n, cfg.modes
+    width = cfg.widths[-1]
+    mode = _CROSS[tape.prec]
+    Ylast = tape.inputs[-1]
+    ch = lambda t, c: t[c]  # noqa: E731  packed channel c as [n, ld]
+    U, Dg = fields.mass, fields.stiff
+    if fields.layout != 0:
+        raise ValueError("the output-layer products expect point-major field matrices")
     if d == 2:
         u, v = _rows_view(U, 0, 2), _rows_view(U, 1, 2)
         P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
         cross([(u, ch(Ylast, 2), n, 1.0), (v, ch(Ylast, 1), n, -1.0)], K, width, mode, out=P[:K])
         cross([(Dg, ch(Ylast, 3), n, -1.0), (Dg, ch(Ylast, 5), n, -1.0)], K, width, mode, out=P[K:])
-        gW[-1].copy_(torch.addmm(P[:K], pair, P, beta=0.0, alpha=2.0 * w))
+        return [P]
+    from .lifts import curl_terms
+    comps = [_rows_view(U, a, 3) for a in range(3)]
+    grads3 = [_rows_view(Dg, c, 9) for c in range(9)]
+    out = []
+    for sl in range(3):
+        segm = [(comps[a], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a < 3 and w_s == sl]
+        segs = [(grads3[a - 3], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms()
+                if a >= 3 and w_s == sl]
+        P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
+        cross(segm, K, width, mode, out=P[:K])
+        for i in range(0, len(segs), 8):
+            cross(segs[i:i + 8], K, width, mode, out=P[K:], beta=1.0 if i else 0.0)
+        out.append(P)
+    return out
+
+
+class OutputProducts:
+    """Output-layer cross products queued on the side stream (see
+    output_products); `event` marks their completion on that stream."""
+
+    def __init__(self, P: list, event):
+        self.P, self.event = P, event
+
+
+def output_products(net: EigenfluidNet, tape: Tape, fields: Fields) -> OutputProducts:
+    """Queue the output-layer products [P_m; P_s] on the side stream.  They
+    depend on the fields and the last hidden jets only, not on the objective,
+    so they run while the FP64 objective (a chain of small, latency-bound
+    launches) holds the current stream; pass the result to backward_fields."""
+    dev = fields.mass.device
+    if dev.type != "cuda":
+        return OutputProducts(_products(net, tape, fields), None)
+    main, side = torch.cuda.current_stream(dev), side_stream(dev)
+    side.wait_stream(main)
+    # one cross workgroup per CU: the 64 KB of LDS it leaves free per CU is
+    # room for the objective's sweep workgroups beside it
+    H.call("neb_set_cross_occupancy", 1)
+    try:
+        with torch.cuda.stream(side):
+            P = _products(net, tape, fields)
+            ev = torch.cuda.Event()
+            ev.record(side)
+    finally:
+        H.call("neb_set_cross_occupancy", 0)
+    for t in (fields.mass, fields.stiff, tape.inputs[-1]):
+        t.record_stream(side)
+    return OutputProducts(P, ev)
+
+
+def _output_layer_grads(d, K, P, pair, w, gW, gb):
+    """Output-layer weight gradient 2w (dL/dM P_m + dL/dS P_s) as one GEMM
+    [dL/dM | dL/dS] [P_m; P_s] (the adjoints are symmetric)."""
+    if d == 2:
+        gW[-1].copy_(torch.addmm(P[0][:K], pair, P[0], beta=0.0, alpha=2.0 * w))
     else:
-        from .lifts import curl_terms
-        comps = [_rows_view(U, a, 3) for a in range(3)]
-        grads3 = [_rows_view(Dg, c, 9) for c in range(9)]
         for sl in range(3):
-            segm = [(comps[a], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a < 3 and w_s == sl]
-            segs = [(grads3[a - 3], ch(Ylast, 1 + c), n, sg) for (a, c, w_s, sg) in curl_terms()
-                    if a >= 3 and w_s == sl]
-            P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
-            cross(segm, K, width, mode, out=P[:K])
-            for i in range(0, len(segs), 8):
-                cross(segs[i:i + 8], K, width, mode, out=P[K:], beta=1.0 if i else 0.0)
-            gW[-1][sl::3].copy_(torch.addmm(P[:K], pair, P, beta=0.0, alpha=2.0 * w))
+            gW[-1][sl::3].copy_(torch.addmm(P[sl][:K], pair, P[sl], beta=0.0, alpha=2.0 * w))
     gb[-1].zero_()

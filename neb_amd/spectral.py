@@ -237,12 +237,15 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
         fields = nw.output_fields(net, tape, split=split)
     with timed(tt, "gram_assembly"):
         M, S = nw.field_grams(fields, weight, gram_mode)
+    # the output-layer products need only the fields and the tape: they run
+    # on the side stream while the FP64 objective holds this one
+    prod = nw.output_products(net, tape, fields) if training else None
     with timed(tt, "spectral_objective"):
         loss, dM, dS = spectral_objective(M, S, lam, ridge, gate=gate)
     out = dict(loss=loss, M=M, S=S, fields=fields, timings=tt)
     if training:
         with timed(tt, "analytic_backward"):
-            out["grads"] = nw.backward_fields(net, tape, fields, dM, dS, weight, symmetric=True)
+            out["grads"] = nw.backward_fields(net, tape, fields, dM, dS, weight, symmetric=True, products=prod)
     return out
 
 

@@ -53,3 +53,13 @@ def test_argument_errors_raise_without_touching_the_gpu():
         H.call("neb_jet_output_backward", H.F32, 2, 0, 8, 8, 8, 16, 16, 8, H.BF16, 0, 16, 16, 8, 16, 8, 1.0, 16, 8,
                None)
     assert H.load().neb_cross_workspace_size(1024, 1024, 131072, 1) > 0
+
+
+def test_cross_occupancy_knob_arguments():
+    """neb_set_cross_occupancy accepts 0, 1, 2 (host-side setting only)."""
+    from neb_amd import _hip as H
+    lib = H.load()
+    assert lib.neb_set_cross_occupancy(3) == 4  # NEB_ERR_ARG
+    assert lib.neb_set_cross_occupancy(-1) == 4
+    for v in (1, 2, 0):
+        assert lib.neb_set_cross_occupancy(v) == 0

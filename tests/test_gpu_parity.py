@@ -511,7 +511,9 @@ def test_overlapped_streams_match_sequential():
     g1 = nw.backward_fields(net, tape, f, dM, dS, 0.01, symmetric=True)
     g2 = nw.backward_fields(net, tape, f, dM, dS, 0.01, symmetric=True)
     g3 = nw.backward_fields(net, tape, f, dM.clone(), dS.clone(), 0.01)
-    assert torch.equal(g1, g2) and torch.equal(g1, g3)
+    prod = nw.output_products(net, tape, f)  # queued on the side stream ahead of the reverse
+    g4 = nw.backward_fields(net, tape, f, dM, dS, 0.01, symmetric=True, products=prod)
+    assert torch.equal(g1, g2) and torch.equal(g1, g3) and torch.equal(g1, g4)
     assert torch.isfinite(g1).all() and float(g1.abs().max()) > 0
 
 
