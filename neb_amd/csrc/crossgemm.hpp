@@ -455,29 +455,39 @@ __global__ __launch_bounds__(256) void cross_reduce_kernel(const TP* __restrict_
   }
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int64_t stride = (int64_t)ntiles * CR_TM * CR_TN;
-  double val[4];
+  // the thread's 4 rows share one loop over the splits, 8 splits at a time:
+  // 32 independent loads in flight per thread, each row summed in split order
+  // (rows past the output read the tile's first element and are not stored:
+  // conditional loads would compile to a branch and a wait per load)
+  const TP* p[4];
+  bool ok[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const int row = ty + 8 * q;
-    const int64_t i = bi * 32 + row, j = bj * 32 + tx;
-    double sacc = 0.0;
-    if (i < m && j < n) {
-      const int64_t ti = i / CR_TM, tj = j / CR_TN;
-      const int64_t tile = symmetric ? ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti) : ti * tiles_n + tj;
-      const TP* p = partial + tile * (CR_TM * CR_TN) + (i % CR_TM) * CR_TN + (j % CR_TN);
-      int sp = 0;
-      for (; sp + 8 <= splits; sp += 8) {  // issue 8 independent loads, then add in order
-        TP t[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) t[u] = p[(sp + u) * stride];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) sacc += (double)t[u];
-      }
-      for (; sp < splits; ++sp) sacc += (double)p[sp * stride];
-    }
-    val[q] = sacc;
-    tr[row][tx] = sacc;
+    const int64_t i = bi * 32 + ty + 8 * q, j = bj * 32 + tx;
+    ok[q] = i < m && j < n;
+    const int64_t ti = ok[q] ? i / CR_TM : 0, tj = ok[q] ? j / CR_TN : 0;
+    const int64_t tile = symmetric ? ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti) : ti * tiles_n + tj;
+    p[q] = partial + tile * (CR_TM * CR_TN) + (ok[q] ? (i % CR_TM) * CR_TN + (j % CR_TN) : 0);
   }
+  double val[4] = {0.0, 0.0, 0.0, 0.0};
+  int sp = 0;
+  for (; sp + 8 <= splits; sp += 8) {
+    TP t[4][8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[q][u] = p[q][(sp + u) * stride];  // valid even for rows past the output
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) val[q] += (double)t[q][u];
+  }
+  for (; sp < splits; ++sp) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) val[q] += (double)p[q][sp * stride];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) tr[ty + 8 * q][tx] = val[q];
   if (!symmetric) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {

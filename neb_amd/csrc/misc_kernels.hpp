@@ -66,13 +66,25 @@ __global__ __launch_bounds__(256) void colsum_kernel(const S* __restrict__ x, in
     for (int e = 0; e < VE; ++e) acc[e] = 0.0;
     const int64_t col = (g0 + cgl) * VE;
     if (rl < RL && g0 + cgl < ngroups) {
-      for (int64_t r = rbeg + rl; r < rend; r += RL) {
-        const S* p = x + r * ld + col;
-        if (col + VE <= cols) {
-          const vs v = *reinterpret_cast<const vs*>(p);
+      if (col + VE <= cols) {  // 8 rows of 16-byte loads in flight, added in row order
+        int64_t r = rbeg + rl;
+        for (; r + 7 * RL < rend; r += 8 * RL) {
+          vs v[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const vs*>(x + (r + u * RL) * ld + col);
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int e = 0; e < VE; ++e) acc[e] += todbl_any(v[u][e]);
+        }
+        for (; r < rend; r += RL) {
+          const vs v = *reinterpret_cast<const vs*>(x + r * ld + col);
 #pragma unroll
           for (int e = 0; e < VE; ++e) acc[e] += todbl_any(v[e]);
-        } else {
+        }
+      } else {
+        for (int64_t r = rbeg + rl; r < rend; r += RL) {
+          const S* p = x + r * ld + col;
 #pragma unroll
           for (int e = 0; e < VE; ++e)
             if (col + e < cols) acc[e] += todbl_any(p[e]);
