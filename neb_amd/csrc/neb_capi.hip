@@ -754,7 +754,7 @@ int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t 
   if (A == X) return fail(NEB_ERR_ARG, "neb_spd_inverse: X must not alias A");
   if (ws_bytes < neb_spd_inverse_workspace_size(K)) return fail(NEB_ERR_ARG, "neb_spd_inverse: workspace too small");
   const int64_t nt = (K + SPD_NB - 1) / SPD_NB, Kp = nt * SPD_NB;
-  if (nt > 65535) return fail(NEB_ERR_SHAPE, "neb_spd_inverse: K too large");
+  if (nt > 2048) return fail(NEB_ERR_SHAPE, "neb_spd_inverse: K too large");
   double* base = reinterpret_cast<double*>(workspace);
   double* buf[2] = {base, base + Kp * Kp};  // padded Kp x Kp ping-pong matrices
   double* Pi[2] = {base + 2 * Kp * Kp, base + 2 * Kp * Kp + SPD_NB * SPD_NB};
@@ -769,8 +769,9 @@ int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t 
     const bool last = k == nt - 1;
     double* dst = last ? X : buf[k & 1];
     const int64_t ldd = last ? ldx : Kp;
-    hipLaunchKernelGGL(spd_step_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), spd_step_lds_bytes(), st, src,
-                       lds, k == 0 ? 1 : 0, K, (int)k, Pi[k & 1], Pi[(k + 1) & 1], dst, ldd, last ? 1 : 0, info);
+    hipLaunchKernelGGL(spd_step_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), spd_step_lds_bytes(), st,
+                       src, lds, k == 0 ? 1 : 0, K, (int)k, (int)nt, Pi[k & 1], Pi[(k + 1) & 1], dst, ldd,
+                       last ? 1 : 0, info);
     if (int rc = hip_status("neb_spd_inverse sweep")) return rc;
     src = dst;
     lds = ldd;

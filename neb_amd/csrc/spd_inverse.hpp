@@ -293,13 +293,16 @@ __global__ __launch_bounds__(256) void spd_pivot0_kernel(const double* __restric
   spd_pivot_regs(v, spd_smem, w, lane, t, 0, Pi, info);
 }
 
-// Sweep k on tile (blockIdx.y, blockIdx.x) of the padded matrix, src -> dst;
-// on the last sweep dst is X: the tile is negated (-A^-1 -> A^-1) and
-// clipped to K x K.  The workgroup of tile (k+1, k+1) inverts its new value
-// as the next pivot (Pi_next) instead of storing it: sweep k+1 replaces that
-// tile by -Pi_next without reading it.
+// Sweep k on tile pair (ti, tj), ti <= tj (blockIdx.x counts the upper
+// triangle of the nt x nt tile grid row by row) of the padded matrix,
+// src -> dst: the sweeps keep the matrix symmetric, so the workgroup stores
+// its tile and, through an LDS transpose, the mirrored tile (tj, ti).  On
+// the last sweep dst is X: tiles are negated (-A^-1 -> A^-1) and clipped to
+// K x K.  The workgroup of tile (k+1, k+1) inverts its new value as the next
+// pivot (Pi_next) instead of storing it: sweep k+1 replaces that tile by
+// -Pi_next without reading it.
 __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict__ src, int64_t lds, int bounded,
-                                                       int64_t K, int k, const double* __restrict__ Pi,
+                                                       int64_t K, int k, int nt, const double* __restrict__ Pi,
                                                        double* __restrict__ Pi_next, double* __restrict__ dst,
                                                        int64_t ldd, int last, int* __restrict__ info) {
   // two LDS tiles (70 KB): a workgroup fits beside a 64 KB cross-GEMM
@@ -311,7 +314,12 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
-  const int ti = blockIdx.y, tj = blockIdx.x;
+  int ti = 0, tj;
+  {
+    int u = blockIdx.x, rowlen = nt;
+    while (u >= rowlen) { u -= rowlen; ++ti; --rowlen; }
+    tj = ti + u;
+  }
   const int64_t i0 = (int64_t)ti * SPD_NB, j0 = (int64_t)tj * SPD_NB, k0 = (int64_t)k * SPD_NB;
   const bool bnd = bounded != 0;
   f64x4 acc[2][2];
@@ -379,6 +387,7 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
       return;
     }
   }
+  const double sgn = last ? -1.0 : 1.0;
 #pragma unroll
   for (int tr = 0; tr < 2; ++tr)
 #pragma unroll
@@ -386,9 +395,19 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int64_t gi = i0 + spd_row(R0, tr, lane, e), gj = j0 + spd_col(C0, tc, lane);
-        if (!last) dst[gi * ldd + gj] = acc[tr][tc][e];
-        else if (gi < K && gj < K) dst[gi * ldd + gj] = -acc[tr][tc][e];
+        if (!last || (gi < K && gj < K)) dst[gi * ldd + gj] = sgn * acc[tr][tc][e];
       }
+  if (ti != tj) {  // the mirrored tile (tj, ti), row-contiguous through LDS
+    __syncthreads();  // every wave is done with the operand tiles
+    spd_acc_store(L0, acc, R0, C0, lane);
+    __syncthreads();
+#pragma unroll 4
+    for (int u = 0; u < 16; ++u) {
+      const int q = t + 256 * u, r = q >> 6, c = q & 63;  // row r of the transpose = column r of the tile
+      const int64_t gi = j0 + r, gj = i0 + c;
+      if (!last || (gi < K && gj < K)) dst[gi * ldd + gj] = sgn * L0[c * SPD_LDP + r];
+    }
+  }
 }
 
 constexpr size_t spd_step_lds_bytes() { return 2 * sizeof(double) * SPD_TILE; }
