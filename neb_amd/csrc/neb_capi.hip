@@ -67,6 +67,7 @@ int launch_rowgemm(const RowArgs& a, hipStream_t st, const char* what) {
   if constexpr (P == PREC_BF16 && std::is_same<TIN, __bf16>::value && row_async_ok<Plan>()) {
     // bf16 operands: persistent workgroups on the asynchronous-copy ring,
     // one round of resident workgroups, each a contiguous range of tiles
+    constexpr int TN = RowAsyncPol<Plan>::TN, TP = RowAsyncPol<Plan>::TP;
     const int64_t ntiles = ((a.npts + TP - 1) / TP) * ((a.nout + TN - 1) / TN);
     if (ntiles == 0) return NEB_OK;
     auto kern = rowgemm_async_kernel<Plan>;

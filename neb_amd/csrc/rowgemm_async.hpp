@@ -19,10 +19,30 @@
 #pragma once
 #include "jet_rowgemm.hpp"
 
+// Tile shape of the async launch: the plan's own, except for the hidden
+// layers, whose 256-feature width fits one workgroup's tile (2D: 4 waves x 64
+// features x 16 points; 3D: 4 waves x 32 features x 16 points, half the
+// width).  A point block split over feature tiles is read once per tile, and
+// the workgroups holding the other tiles of a block drift further apart than
+// an XCD's L2 holds (the hidden layers read their input jets twice from HBM
+// with two 128-feature tiles); one tile across the width reads it once.  The
+// accumulators per wave stay what they were (the same 64 or 32 features x 16
+// points of every channel).
+template <class Plan>
+struct RowAsyncShape {
+  static constexpr int FB = Plan::FB, PB = Plan::PB, WPN = Plan::WPN, WPP = Plan::WPP;
+};
+template <int D> struct WideHiddenShape {
+  static constexpr int FB = D == 2 ? 4 : 2, PB = 1, WPN = 4, WPP = 1;
+};
+template <int D, int ACT> struct RowAsyncShape<PlanHiddenFwd<D, ACT>> : WideHiddenShape<D> {};
+template <int D, int ACT> struct RowAsyncShape<PlanHiddenBwd<D, ACT>> : WideHiddenShape<D> {};
+
 template <class Plan>
 struct RowAsyncPol {
+  using SH = RowAsyncShape<Plan>;
   static constexpr int RK = 32, LDK = 32;
-  static constexpr int TN = Plan::WPN * Plan::FB * 16, TP = Plan::WPP * Plan::PB * 16;
+  static constexpr int TN = SH::WPN * SH::FB * 16, TP = SH::WPP * SH::PB * 16;
   static constexpr int WT = TN * LDK, XT = TP * LDK;
   static constexpr int STAGE = Plan::NW * WT + Plan::CI * XT;  // bf16 elements per slot
   static constexpr int WPIECE = Plan::NW * TN / 16, XPIECE = Plan::CI * TP / 16;
@@ -45,7 +65,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   using Q = RowAsyncPol<Plan>;
   constexpr int RK = Q::RK, LDK = Q::LDK, KE = PR::KE;
   constexpr int CI = Plan::CI, NW = Plan::NW, NA = Plan::NA, NT = Plan::NT;
-  constexpr int FB = Plan::FB, PB = Plan::PB, WPN = Plan::WPN, WPP = Plan::WPP;
+  constexpr int FB = Q::SH::FB, PB = Q::SH::PB, WPN = Q::SH::WPN, WPP = Q::SH::WPP;
   constexpr int TN = Q::TN, TP = Q::TP, WT = Q::WT, XT = Q::XT, STAGE = Q::STAGE;
   constexpr int NS = Q::NS, NPIECE = Q::NPIECE, PPW = Q::PPW, WPIECE = Q::WPIECE;
   static_assert(WPN * WPP == 4, "four waves per workgroup");
