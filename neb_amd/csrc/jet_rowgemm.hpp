@@ -382,7 +382,10 @@ template <int D, int ACT> struct PlanHiddenFwd {
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
   IDENTITY_TERMS
-  template <int P, int GE, typename M>
+  // The value channel takes the bias (a.bias, fp32 [nout] or null); a
+  // caller that has added it to v[0] already passes BIAS_ADDED.
+  static constexpr bool BIAS_CH0 = true;
+  template <int P, int GE, typename M, bool BIAS_ADDED = false>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     using S = typename Prec<P>::store_t;
     using L = Jet2Layout<D>;
@@ -391,8 +394,8 @@ template <int D, int ACT> struct PlanHiddenFwd {
     M u[C][GE];
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
-      const M b = (bias && e < nv) ? bias[f + e] : (M)0;
-      u[0][e] = om * (v[0][e] + b);
+      const M b = (!BIAS_ADDED && bias && e < nv) ? bias[f + e] : (M)0;
+      u[0][e] = BIAS_ADDED ? om * v[0][e] : om * (v[0][e] + b);
 #pragma unroll
       for (int c = 1; c < C; ++c) u[c][e] = om * v[c][e];
     }
@@ -518,13 +521,10 @@ using PlanOutFields3 = PlanOutFields3T<false>;
 // output (adj[C][GE]), apply the transposed jet chain with the tape Z and
 // the omega scale, and store the packed pre-activation adjoint.
 template <int P, int D, int ACT, int GE, typename M>
-__device__ inline void reverse_epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*adj)[GE]) {
+__device__ inline void reverse_epilogue_u(const RowArgs& a, int64_t p, int64_t f, int nv, M (*adj)[GE],
+                                          M (*u)[GE]) {
   using S = typename Prec<P>::store_t;
   constexpr int C = Jet2Layout<D>::C;
-  const S* z = reinterpret_cast<const S*>(a.tape) + p * a.ldt + f;
-  M u[C][GE];
-#pragma unroll
-  for (int c = 0; c < C; ++c) ld_group<S, GE>(z + c * a.tstride, u[c], nv);
   M b[C][GE];
   const M om = (M)a.omega;
 #pragma unroll
@@ -541,6 +541,17 @@ __device__ inline void reverse_epilogue(const RowArgs& a, int64_t p, int64_t f, 
   for (int c = 0; c < C; ++c) st_group<S, GE>(o + c * a.cstride, b[c], nv);
 }
 
+template <int P, int D, int ACT, int GE, typename M>
+__device__ inline void reverse_epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*adj)[GE]) {
+  using S = typename Prec<P>::store_t;
+  constexpr int C = Jet2Layout<D>::C;
+  const S* z = reinterpret_cast<const S*>(a.tape) + p * a.ldt + f;
+  M u[C][GE];
+#pragma unroll
+  for (int c = 0; c < C; ++c) ld_group<S, GE>(z + c * a.tstride, u[c], nv);
+  reverse_epilogue_u<P, D, ACT, GE, M>(a, p, f, nv, adj, u);
+}
+
 // Hidden layer reverse: A = A'_{l} W_l (all channels) then the transposed
 // activation chain of layer l-1 with its tape Z:  out = omega * f^T(Z, A).
 // The host passes W_l^T so the kernel still contracts X . W^T.
@@ -553,6 +564,13 @@ template <int D, int ACT> struct PlanHiddenBwd {
   template <int P, int GE, typename M>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     reverse_epilogue<P, D, ACT, GE, M>(a, p, f, nv, v);
+  }
+  // The same with the tape group already loaded (u = Z at p, f..f+GE).
+  static constexpr int TAPE_CH = C;
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue_tape(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE],
+                                              M (*u)[GE]) {
+    reverse_epilogue_u<P, D, ACT, GE, M>(a, p, f, nv, v, u);
   }
 };
 

@@ -201,6 +201,16 @@ __device__ inline int lds_k(int row, int k) {
   return k;
 }
 
+// Makes x opaque to the optimiser (kept in a VGPR, value unknown to it).
+// The asm exists in the device pass only: the host pass of a kernel body
+// does not know the AMDGPU register constraint.
+template <typename T>
+__device__ inline void opaque_vgpr(T& x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(x));
+#endif
+}
+
 // s_waitcnt vmcnt(n) with expcnt / lgkmcnt left at their maxima (vmcnt is
 // split over bits 3:0 and 15:14 of the immediate)
 constexpr int vmcnt_imm(int n) { return 0x0F70 | (n & 15) | ((n >> 4) << 14); }

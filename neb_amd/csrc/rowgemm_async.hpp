@@ -13,11 +13,30 @@
 //     tiles; the copy cursor runs ahead across tile boundaries, so the next
 //     tile's first k-steps land while the current tile's epilogue (the jet
 //     chain and stores) runs.
+//   * A wait for a load waits for every earlier copy and store of the wave
+//     (the counter is in issue order), so epilogues load nothing after the
+//     copies of the step: a bias (value channel) is held in registers from
+//     the start, and the tape of a reverse epilogue is loaded for the whole
+//     tile before the last k-step's refill, whose copies then stay in flight
+//     through the epilogue.
 // Rows past the point / feature count re-read the last row (their results
 // are never stored); k past the reduction length is zeroed in LDS on the
 // last k-step of a tile.
 #pragma once
+#include <type_traits>
+
 #include "jet_rowgemm.hpp"
+
+// Plan members the async kernel looks for: BIAS_CH0 (the epilogue adds an
+// fp32 bias to the value channel) and TAPE_CH (the epilogue reads that many
+// tape channels at its own (p, f) group and has an epilogue_tape that takes
+// them loaded).
+template <class Plan, class = void> struct PlanBiasCh0 : std::false_type {};
+template <class Plan>
+struct PlanBiasCh0<Plan, std::void_t<decltype(Plan::BIAS_CH0)>> : std::bool_constant<Plan::BIAS_CH0> {};
+template <class Plan, class = void> struct PlanTapeCh : std::integral_constant<int, 0> {};
+template <class Plan>
+struct PlanTapeCh<Plan, std::void_t<decltype(Plan::TAPE_CH)>> : std::integral_constant<int, Plan::TAPE_CH> {};
 
 // Tile shape of the async launch: the plan's own, except for the hidden
 // layers, whose 256-feature width fits one workgroup's tile (2D: 4 waves x 64
@@ -71,6 +90,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   static_assert(WPN * WPP == 4, "four waves per workgroup");
   static_assert(FB % 2 == 0, "8-feature epilogue groups pair feature blocks");
   static_assert(NS >= 2 && NS <= 4, "ring of 2..4 slots");
+  static_assert(PPW * (NS - 2) <= 63, "vmcnt range");
 
   // LDS pointers stay in address space 3 from the arrays on (a generic
   // pointer cast back to LDS would need a null check per use)
@@ -99,39 +119,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   constexpr int GE = Plan::template out_bytes<P>() == 2 ? 8 : 4;
   const int rg = PR::rowg(fr), rr = PR::rowr(fr);
 
-  // ---- copies: piece q of a slot (static), issued by wave q % 4 -----------
+  // ---- copies: this wave issues pieces q = wave + 4t (t < PPW) of every
+  // slot (a piece past NPIECE repeats an earlier one).  The per-wave plan is
+  // set once; each issue is then straight-line code of exactly PPW copies,
+  // so the compiler knows how many copies follow any load it must wait for.
   const int cr = lane >> 2, cu = lane & 3;   // row in the 16-row piece, LDS chunk slot
   const int cch = cu ^ ((cr >> 2) & 3);      // global chunk held by that slot (lds_k<S, 2>)
+  const S* csrc[PPW];
+  int64_t cld[PPW];
+  int crow[PPW], cdst[PPW];
+  bool cw[PPW];
+#pragma unroll
+  for (int t = 0; t < PPW; ++t) {
+    const int q = wave + 4 * t;
+    const int qq = q < NPIECE ? q : q - NPIECE;
+    cw[t] = qq < WPIECE;
+    if (cw[t]) {
+      const int s = qq / (TN / 16), rb = qq % (TN / 16);
+      csrc[t] = reinterpret_cast<const S*>(a.w[s]);
+      cld[t] = a.ldw[s];
+      crow[t] = 16 * rb;
+      cdst[t] = s * WT + 16 * rb * LDK;
+    } else {
+      const int c = (qq - WPIECE) / (TP / 16), rb = (qq - WPIECE) % (TP / 16);
+      csrc[t] = reinterpret_cast<const S*>(a.x[c]);
+      cld[t] = a.ldx[c];
+      crow[t] = 16 * rb;
+      cdst[t] = NW * WT + c * XT + 16 * rb * LDK;
+    }
+  }
   auto issue = [&](int64_t tp, int64_t tf, int kt, lds_S* slot) __attribute__((always_inline)) {
     const int64_t p0 = tp * TP, f0 = tf * TN, k0 = (int64_t)kt * RK;
 #pragma unroll
-    for (int q = 0; q < 4 * PPW; ++q) {
-      if ((q & 3) == wave) {
-        const int qq = q < NPIECE ? q : q - NPIECE;  // padding repeats an earlier piece
-        const S* src;
-        int64_t ld, row, nrows;
-        lds_S* dst;
-        if (qq < WPIECE) {
-          const int s = qq / (TN / 16), rb = qq % (TN / 16);
-          src = reinterpret_cast<const S*>(a.w[s]);
-          ld = a.ldw[s];
-          row = f0 + 16 * rb + cr;
-          nrows = a.nout;
-          dst = slot + s * WT + 16 * rb * LDK;
-        } else {
-          const int c = (qq - WPIECE) / (TP / 16), rb = (qq - WPIECE) % (TP / 16);
-          src = reinterpret_cast<const S*>(a.x[c]);
-          ld = a.ldx[c];
-          row = p0 + 16 * rb + cr;
-          nrows = a.npts;
-          dst = slot + NW * WT + c * XT + 16 * rb * LDK;
-        }
-        row = row < nrows ? row : nrows - 1;
-        int64_t kk = k0 + 8 * cch;
-        kk = kk + 8 <= ld ? kk : ld - 8;
-        __builtin_amdgcn_global_load_lds(src + row * ld + kk, (__attribute__((address_space(3))) void*)dst, 16, 0,
-                                         0);
-      }
+    for (int t = 0; t < PPW; ++t) {
+      const int64_t nrows = cw[t] ? a.nout : a.npts;
+      int64_t row = (cw[t] ? f0 : p0) + crow[t] + cr;
+      row = row < nrows ? row : nrows - 1;
+      int64_t kk = k0 + 8 * cch;
+      kk = kk + 8 <= cld[t] ? kk : cld[t] - 8;
+      const S* src = csrc[t] + row * cld[t] + kk;
+      lds_S* dst = slot + cdst[t];
+      __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
     }
   };
 
@@ -140,11 +168,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   // same time are npb / per_wg apart, i.e. on one XCD (a multiple of 8 in the
   // launches here), so each point block comes from HBM once and each
   // workgroup's weight slice stays in L2.
+  //
+  // Every step issues one slot of copies, also past the end of the range
+  // (the cursor then stays on the first tile past it: rows and k are clamped
+  // in issue(), and the slot it fills is never read), so each wave always
+  // has NS-2 slots issued after the one it waits for: one constant wait, and
+  // the same count of copies on every path through the loop for the
+  // compiler's own waits.  The kernel drains them before it ends.
   int64_t lt = t_beg;  // copy cursor (tile, its point / feature block, k-step)
   int64_t ltp = t_beg % npb, ltf = t_beg / npb;
   int lk = 0;
-  int nissued = 0;
   auto advance = [&]() __attribute__((always_inline)) {
+    if (lt >= t_end) return;
     if (++lk == nk) {
       lk = 0;
       ++lt;
@@ -152,12 +187,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
     }
   };
 #pragma unroll
-  for (int s = 0; s < NS - 1; ++s)
-    if (lt < t_end) {
-      issue(ltp, ltf, lk, slots[s]);
-      ++nissued;
-      advance();
-    }
+  for (int s = 0; s < NS - 1; ++s) {
+    issue(ltp, ltf, lk, slots[s]);
+    advance();
+  }
 
   acc_t acc[NA][FB][PB];
   auto zero_acc = [&]() __attribute__((always_inline)) {
@@ -170,13 +203,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
   };
   zero_acc();
 
+  // epilogue groups: lane (fr, g) holds points p0 + wp*PB*16 + 16j + fr and
+  // features f0 + wn*FB*16 + (GE == 8 ? 32q + 8g : 16q + 4g) + [0, GE)
+  constexpr int NQ = FB * 4 / GE;
+  const int g = lane >> 4;
+  auto group_p = [&](int64_t p0, int j) __attribute__((always_inline)) { return p0 + wp * PB * 16 + j * 16 + fr; };
+  auto group_f = [&](int64_t f0, int q) __attribute__((always_inline)) {
+    return f0 + wn * FB * 16 + (GE == 8 ? 32 * q + 8 * g : 16 * q + 4 * g);
+  };
+  typedef S tape_t __attribute__((ext_vector_type(GE)));
+
+  // bias of this lane's epilogue features for the first two feature blocks,
+  // in registers from the start (no load in the epilogue); more feature
+  // blocks read it in the epilogue
+  constexpr bool BIAS = PlanBiasCh0<Plan>::value;
+  M breg[BIAS ? 2 : 1][BIAS ? NQ : 1][GE];
+  const bool bias_reg = BIAS && nfb <= 2;
+  if constexpr (BIAS) {
+    const M* bsrc = reinterpret_cast<const M*>(a.bias);
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int e = 0; e < GE; ++e) {
+          const int64_t f = group_f((int64_t)h * TN, q) + e;
+          breg[h][q][e] = (bsrc && f < a.nout) ? bsrc[f] : (M)0;
+        }
+  }
+
+  // Offsets of the operand fragment reads in a slot, opaque to the
+  // compiler: with the channel stride folded into constant offsets, the reads
+  // of channels past the first lose the LDS alias scope that tells the
+  // compiler they do not touch the slot being refilled, and it then waits
+  // for every copy in flight before them.
+  int xoff[CI][PB];
+#pragma unroll
+  for (int j = 0; j < PB; ++j) {
+    const int row = wp * PB * 16 + j * 16 + fr;
+#pragma unroll
+    for (int c = 0; c < CI; ++c) {
+      xoff[c][j] = NW * WT + c * XT + row * LDK + lds_k<S, 2>(row, kg);
+      opaque_vgpr(xoff[c][j]);
+    }
+  }
+
   int64_t ct = t_beg;  // compute cursor
   int64_t ctp = t_beg % npb, ctf = t_beg / npb;
-  int ck = 0, kdone = 0;
+  int ck = 0;
   // always inlined: the accumulators must stay in registers
+  constexpr int TC = PlanTapeCh<Plan>::value;
   auto step = [&](lds_S* cur, lds_S* refill) __attribute__((always_inline)) -> bool {
     if (ct >= t_end) return false;
-    wait_copies<PPW, NS>(nissued - kdone - 1);
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(PPW * (NS - 2)));
     ring_barrier();
     if (ck == nk - 1 && kpart < RK) {  // zero k >= the reduction length
       constexpr int ROWS = NW * TN + CI * TP;
@@ -187,13 +266,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
       }
       ring_barrier();
     }
-    if (lt < t_end) {  // refill the slot the previous step used
-      issue(ltp, ltf, lk, refill);
-      ++nissued;
-      advance();
+    tape_t traw[PB][NQ][TC > 0 ? TC : 1];
+    bool tape_now = false;
+    if constexpr (TC > 0) {
+      tape_now = ck == nk - 1 && (ctp * TP + TP <= a.npts) && (ctf * TN + TN <= a.nout);
+      if (tape_now) {
+        const S* tz = reinterpret_cast<const S*>(a.tape);
+#pragma unroll
+        for (int j = 0; j < PB; ++j)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int c = 0; c < TC; ++c)
+              traw[j][q][c] = *reinterpret_cast<const tape_t*>(
+                  tz + group_p(ctp * TP, j) * a.ldt + group_f(ctf * TN, q) + c * a.tstride);
+      }
     }
+    issue(ltp, ltf, lk, refill);
+    advance();
     const lds_S* ws = cur;
-    const lds_S* xs = cur + NW * WT;
     frag_t wf[NW][FB], xf[CI][PB];
 #pragma unroll
     for (int i = 0; i < FB; ++i) {
@@ -203,12 +294,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
       for (int s = 0; s < NW; ++s) wf[s][i] = *reinterpret_cast<lds_frag*>(ws + s * WT + off);
     }
 #pragma unroll
-    for (int j = 0; j < PB; ++j) {
-      const int row = wp * PB * 16 + j * 16 + fr;
-      const int off = row * LDK + lds_k<S, 2>(row, kg);
+    for (int j = 0; j < PB; ++j)
 #pragma unroll
-      for (int c = 0; c < CI; ++c) xf[c][j] = *reinterpret_cast<lds_frag*>(xs + c * XT + off);
-    }
+      for (int c = 0; c < CI; ++c) xf[c][j] = *reinterpret_cast<lds_frag*>(cur + xoff[c][j]);
 #pragma unroll
     for (int t = 0; t < NT; ++t)
 #pragma unroll
@@ -218,26 +306,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
         for (int j = 0; j < PB; ++j)
           acc[Plan::ta[t]][i][j] = PR::mfma(wv, xf[Plan::tc[t]][j], acc[Plan::ta[t]][i][j]);
       }
-    ++kdone;
-    if (++ck == nk) {  // tile done: epilogue while the next tile's copies fly
+    if (++ck == nk) {
       const int64_t p0 = ctp * TP, f0 = ctf * TN;
-      const int g = lane >> 4;
+      auto group_v = [&](int j, int q, M (*v)[GE]) __attribute__((always_inline)) {
 #pragma unroll
-      for (int j = 0; j < PB; ++j) {
-        const int64_t p = p0 + wp * PB * 16 + j * 16 + fr;
+        for (int s = 0; s < NA; ++s)
 #pragma unroll
-        for (int q = 0; q < FB * 4 / GE; ++q) {
-          const int64_t f = f0 + wn * FB * 16 + (GE == 8 ? 32 * q + 8 * g : 16 * q + 4 * g);
-          const int64_t rem = a.nout - f;
-          if (p < a.npts && rem > 0) {
-            const int nvalid = rem >= GE ? GE : (int)rem;
-            M v[NA][GE];
+          for (int e = 0; e < GE; ++e)
+            v[s][e] = GE == 8 ? (M)acc[s][2 * q + (e >> 2)][j][e & 3] : (M)acc[s][q][j][e];
+      };
+      if constexpr (TC > 0) {
+        if (tape_now) {
 #pragma unroll
-            for (int s = 0; s < NA; ++s)
+          for (int j = 0; j < PB; ++j)
 #pragma unroll
-              for (int e = 0; e < GE; ++e)
-                v[s][e] = GE == 8 ? (M)acc[s][2 * q + (e >> 2)][j][e & 3] : (M)acc[s][q][j][e];
-            Plan::template epilogue<P, GE>(a, p, f, nvalid, v);
+            for (int q = 0; q < NQ; ++q) {
+              M v[NA][GE], u[TC][GE];
+              group_v(j, q, v);
+#pragma unroll
+              for (int c = 0; c < TC; ++c)
+#pragma unroll
+                for (int e = 0; e < GE; ++e) u[c][e] = tomath<M>(traw[j][q][c][e]);
+              Plan::template epilogue_tape<P, GE>(a, group_p(p0, j), group_f(f0, q), GE, v, u);
+            }
+        }
+      }
+      if (!tape_now) {
+#pragma unroll
+        for (int j = 0; j < PB; ++j) {
+          const int64_t p = group_p(p0, j);
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const int64_t f = group_f(f0, q);
+            const int64_t rem = a.nout - f;
+            if (p < a.npts && rem > 0) {
+              const int nvalid = rem >= GE ? GE : (int)rem;
+              M v[NA][GE];
+              group_v(j, q, v);
+              if constexpr (BIAS) {
+                if (bias_reg) {
+                  const M w1 = (M)(ctf != 0), w0 = (M)1 - w1;
+#pragma unroll
+                  for (int e = 0; e < GE; ++e) v[0][e] += w0 * breg[0][q][e] + w1 * breg[1][q][e];
+                } else {
+                  const M* bsrc = reinterpret_cast<const M*>(a.bias);
+#pragma unroll
+                  for (int e = 0; e < GE; ++e) v[0][e] += (bsrc && e < nvalid) ? bsrc[f + e] : (M)0;
+                }
+                Plan::template epilogue<P, GE, M, true>(a, p, f, nvalid, v);
+              } else {
+                Plan::template epilogue<P, GE>(a, p, f, nvalid, v);
+              }
+            }
           }
         }
       }
@@ -258,6 +378,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void r
     while (step(r0, r1) && step(r1, r0)) {
     }
   }
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));  // copies past the range land before the LDS is released
 }
 
 // Plans whose slot leaves a ring of fewer than 3 slots stay on the
