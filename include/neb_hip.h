@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 4
+#define NEB_ABI_VERSION 5
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -163,12 +163,14 @@ int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int
 int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w, double beta,
                    double* G, int64_t ldg, void* workspace, size_t ws_bytes, void* stream);
 
-/* Caps the workgroups per CU that this thread's later bf16 cross launches
- * (neb_cross on bf16 operands, neb_gram_split) may keep resident: 0 = as
- * many as fit (at most 2), 1 or 2.  No counterpart in the reference: a
- * scheduling knob for products queued on another stream beside
- * latency-bound launches (the FP64 SPD sweeps), so both fit on every CU. */
-int neb_set_cross_occupancy(int per_cu);
+/* Caps the workgroups that each of this thread's later bf16 cross launches
+ * (neb_cross on bf16 operands, neb_gram_split) keeps resident: 0 = as many
+ * as fit (two per CU), otherwise at most max_workgroups (the row splits
+ * shrink to fit; results change only by the rounding of differently split
+ * partial sums).  No counterpart in the reference: a scheduling knob for
+ * products queued on another stream beside latency-bound launches (the FP64
+ * SPD sweeps), which then find whole CUs free.  NEB_ERR_ARG if negative. */
+int neb_set_cross_occupancy(int max_workgroups);
 
 /* Column sums in FP64: out[c] = beta*out[c] + sum_r x[r][c] (bias
  * gradients, HB2P:1101 `grad_bias`, VJ2:30).  Workspace from

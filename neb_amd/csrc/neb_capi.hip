@@ -252,26 +252,9 @@ int cross_splits(int ntiles, int64_t rows, int slots = CR_MAX_SLOTS) {
   return s;
 }
 
-// Workgroups per CU that the asynchronous cross launches of the calling
-// thread may occupy (0: as many as fit, at most 2); see
-// neb_set_cross_occupancy.
-thread_local int g_cross_per_cu = 0;
-
-// CU count of the current device (0 if unknown)
-int device_cus() {
-  static int cache[16] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 0;
-  if (cache[dev] == 0) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
-      (void)hipGetLastError();
-      return 0;
-    }
-    cache[dev] = cus;
-  }
-  return cache[dev];
-}
+// Resident workgroups that an asynchronous cross launch of the calling
+// thread may use (0: as many as fit); see neb_set_cross_occupancy.
+thread_local int g_cross_max_wg = 0;
 
 // Resident workgroups of a kernel on the current device times the CU count
 // (at most 2 per CU), queried once per kernel and device into `cache`.
@@ -333,8 +316,7 @@ int launch_cross_async_as(CrossArgs& a, double scale, double beta, double* G, in
   for (int s = 0; s < a.nseg; ++s) maxrows = a.seg[s].rows > maxrows ? a.seg[s].rows : maxrows;
   static int slots_cache[16] = {0};  // one per kernel instantiation
   int slots = resident_slots(kern, lds, slots_cache, NT);
-  const int cus = device_cus();
-  if (g_cross_per_cu > 0 && cus > 0 && slots > cus * g_cross_per_cu) slots = cus * g_cross_per_cu;
+  if (g_cross_max_wg > 0 && slots > g_cross_max_wg) slots = g_cross_max_wg;
   const int fit = cross_splits(a.ntiles, maxrows, slots);
   if (fit < a.splits) a.splits = fit;
   hipLaunchKernelGGL(kern, dim3(a.ntiles * a.splits), dim3(NT), lds, st, a);
@@ -440,9 +422,9 @@ int stage_dst(int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_
 // ==========================================================================
 extern "C" {
 
-int neb_set_cross_occupancy(int per_cu) {
-  if (per_cu < 0 || per_cu > 2) return fail(NEB_ERR_ARG, "neb_set_cross_occupancy: 0, 1 or 2");
-  g_cross_per_cu = per_cu;
+int neb_set_cross_occupancy(int max_workgroups) {
+  if (max_workgroups < 0) return fail(NEB_ERR_ARG, "neb_set_cross_occupancy: max_workgroups must be >= 0");
+  g_cross_max_wg = max_workgroups;
   return NEB_OK;
 }
 

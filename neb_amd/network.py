@@ -686,6 +686,16 @@ class OutputProducts:
         self.P, self.event = P, event
 
 
+_CUS: dict = {}
+
+
+def _device_cus(dev) -> int:
+    key = str(dev)
+    if key not in _CUS:
+        _CUS[key] = torch.cuda.get_device_properties(dev).multi_processor_count
+    return _CUS[key]
+
+
 def output_products(net: EigenfluidNet, tape: Tape, fields: Fields) -> OutputProducts:
     """Queue the output-layer products [P_m; P_s] on the side stream.  They
     depend on the fields and the last hidden jets only, not on the objective,
@@ -696,9 +706,12 @@ def output_products(net: EigenfluidNet, tape: Tape, fields: Fields) -> OutputPro
         return OutputProducts(_products(net, tape, fields), None)
     main, side = torch.cuda.current_stream(dev), side_stream(dev)
     side.wait_stream(main)
-    # one cross workgroup per CU: the 64 KB of LDS it leaves free per CU is
-    # room for the objective's sweep workgroups beside it
-    H.call("neb_set_cross_occupancy", 1)
+    # the products' launches keep to half the CUs' worth of workgroups: the
+    # objective's sweep launches (one workgroup per 64 x 64 tile, 136 at
+    # K = 1024) then find whole CUs free.  With one per CU the dispatcher
+    # doubles some up, and the first sweep launch waited for the products'
+    # kernel to end (196 us instead of 29 us).
+    H.call("neb_set_cross_occupancy", max(1, _device_cus(dev) // 2))
     try:
         with torch.cuda.stream(side):
             P = _products(net, tape, fields)

@@ -56,10 +56,9 @@ def test_argument_errors_raise_without_touching_the_gpu():
 
 
 def test_cross_occupancy_knob_arguments():
-    """neb_set_cross_occupancy accepts 0, 1, 2 (host-side setting only)."""
+    """neb_set_cross_occupancy takes a workgroup cap >= 0 (host-side setting only)."""
     from neb_amd import _hip as H
     lib = H.load()
-    assert lib.neb_set_cross_occupancy(3) == 4  # NEB_ERR_ARG
-    assert lib.neb_set_cross_occupancy(-1) == 4
-    for v in (1, 2, 0):
+    assert lib.neb_set_cross_occupancy(-1) == 4  # NEB_ERR_ARG
+    for v in (1, 128, 100000, 0):
         assert lib.neb_set_cross_occupancy(v) == 0
