@@ -71,11 +71,12 @@ int launch_rowgemm(const RowArgs& a, hipStream_t st, const char* what) {
     const int64_t ntiles = ((a.npts + TP - 1) / TP) * ((a.nout + TN - 1) / TN);
     if (ntiles == 0) return NEB_OK;
     auto kern = rowgemm_async_kernel<Plan>;
+    constexpr int NTH = RowAsyncPol<Plan>::NTH;
     static int slots_cache[16] = {0};
-    const int64_t slots = resident_slots(kern, 0, slots_cache, 256);
+    const int64_t slots = resident_slots(kern, 0, slots_cache, NTH);
     const int64_t per_wg = (ntiles + slots - 1) / slots;
     const int64_t grid = (ntiles + per_wg - 1) / per_wg;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, a, per_wg);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTH), 0, st, a, per_wg);
     return hip_status(what);
   }
   const size_t lds = rowgemm_lds_bytes<P, Plan>();
