@@ -414,8 +414,31 @@ def bench_3d_eig(args):
     vals, _ = sp.generalized_eigh(res["S"], res["M"])
     torch.cuda.synchronize()
     t_eig = time.perf_counter() - t0
+    # the sorted Cholesky coefficient peel (the reference's 3D objective) on
+    # the same K = 512 Grams: FP64 on the GPU, device-side gate
+    peel = {}
+    try:
+        sp.coefficient_peel(res["M"], res["S"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pk = sp.coefficient_peel(res["M"], res["S"], deferred=True)
+        torch.cuda.synchronize()
+        peel["ms"] = (time.perf_counter() - t0) * 1e3
+        peel["gate"] = "passed" if int(pk["gate"].item()) == 0 else "failed"
+        peel["orthonormality_max_err"] = float(pk["gram_max"])
+    except FloatingPointError as e:
+        pk = None
+        peel["gate"] = f"failed: {e}"
     ref = grams("f64", "f64")  # FP64 jets, fields and Grams
     v64, _ = sp.generalized_eigh(ref["S"], ref["M"])
+    if pk is not None and peel["gate"] == "passed":
+        try:
+            p64 = sp.coefficient_peel(ref["M"], ref["S"])
+            peel["losses_rel_err_vs_fp64_path"] = float(torch.linalg.norm(pk["losses"] - p64["losses"])
+                                                        / torch.linalg.norm(p64["losses"]))
+            peel["same_order_as_fp64_path"] = bool(torch.equal(pk["order"], p64["order"]))
+        except FloatingPointError as e:
+            peel["fp64_path_gate"] = f"failed: {e}"
     gram_err = float(torch.linalg.norm(res["M"] - ref["M"]) / torch.linalg.norm(ref["M"]))
     eig_err = float(((vals - v64).abs() / v64.abs()).max())
     del ref
@@ -441,6 +464,7 @@ def bench_3d_eig(args):
         "mass_gram_rel_err_vs_fp64_path": gram_err,
         "eigenvalue_max_rel_err_vs_fp64_path": eig_err,
         "eigenvalues_min_max": [float(vals[0]), float(vals[-1])],
+        "coefficient_peel": peel,
     }
     print(json.dumps(out), flush=True)
 

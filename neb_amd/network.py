@@ -548,8 +548,11 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     dL/dM and dL/dS (symmetric, FP64 K x K).  Fuses the field adjoint
     2w Phi dL/dG (VCORE:94-95) into the output-layer reverse, then runs the
     handwritten jet reverse through every layer (HB2P:2087-2139, VJ2:77-82).
-    symmetric=True skips the symmetrisation of dM and dS for adjoints that
-    are symmetric by construction (SpectralSolver's); `products` are the
+    symmetric=True uses dM and dS as given, without symmetrising them: the
+    field cotangents applied are then 2w U dM^T and 2w D dS^T, which for
+    adjoints symmetric by construction (SpectralSolver's) is the gradient of
+    L(M, S), and for the coefficient peel's pair [Cu^T/2 | Cd^T/2] its
+    detached-prefix cotangents w U Cu, w D Cd; `products` are the
     output-layer products queued earlier by output_products (computed here
     when None).  Returns the flat gradient in the master dtype."""
     cfg = net.cfg

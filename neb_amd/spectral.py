@@ -9,6 +9,7 @@ Reference surfaces mirrored (paths relative to the reference root;
 MB = experiments/metaball/_backend/flow/cases/metaball):
   trace_objective        MB/velocity_trace_jet2.py:107-117
   grams / compute        MB/velocity/core.py:40-46, 79-105
+  coefficient_peel       MB/velocity/core.py:48-69
   generalized eigh       experiments/periodic/paper_rollout.py:170-188
   project                experiments/metaball/_backend/exports/basis_projection/
                          postprocessing/paired_projection.py:141-177
@@ -193,6 +194,110 @@ class SpectralSolver:
         return loss, dM, X
 
 
+PEEL_GATE = -2      # gate value of a failed coefficient-peel gate
+
+
+def _raise_peel_gate(value: int):
+    if value == PEEL_GATE:
+        raise FloatingPointError("Velocity rank/relative mass/orthogonality gate failed before Adam; no update")
+    _raise_gate(value)
+
+
+def coefficient_peel(M: torch.Tensor, S: torch.Tensor, ortho_tol: float = 1e-5, rel_min: float = 1e-4,
+                     deferred: bool = False) -> dict:
+    """Sorted Cholesky coefficient peel (MB/velocity/core.py:48-69) in FP64
+    on the GPU.
+
+    The modes are sorted by their raw Rayleigh quotient diag(S)/diag(M)
+    (stable order); T is the inverse upper Cholesky factor of the
+    diagonally scaled sorted mass, refined by a second Cholesky pass, so
+    that T^T M_sorted T = I; loss_r = (T^T S_sorted T)_rr.  Gates (core.py:
+    57-61): every loss finite and positive, relative mass
+    (1/T_rr)^2 / M_rr >= rel_min, max |T^T M T - I| <= ortho_tol and both
+    factorizations positive definite.  They are evaluated on the device
+    into `gate` (one int32: 0 = passed, PEEL_GATE = failed); deferred=False
+    synchronises and raises FloatingPointError, deferred=True leaves the
+    check to EigenfluidNet.adam(gate=..., on_gate=raise_peel_gate).
+
+    Adjoint (core.py:64-67): with every mode's earlier peeled modes held
+    constant, the field cotangents of sum_r loss_r are G_U = w U Cu and
+    G_D = w D Cd for M = w U^T U, S = w D^T D.  `pair` holds
+    [Cu^T / 2 | Cd^T / 2], the form backward_fields(symmetric=True)
+    consumes (it applies field cotangents 2w Phi X^T).
+
+    Returns dict(losses (sorted order), loss (their sum, 0-dim), order,
+    T (rows in the original mode order, columns in sorted order), Cu, Cd,
+    relative_mass, gram_max, raw, gate, pair)."""
+    K = M.shape[0]
+    dev = M.device
+    M = M.to(torch.float64)
+    S = S.to(torch.float64)
+    eye = _eye(K, dev)
+    raw = torch.diagonal(S) / torch.diagonal(M)
+    order = torch.argsort(raw, stable=True)
+    ms = M[order][:, order]
+    ss = S[order][:, order]
+    scale = torch.diagonal(ms).sqrt()
+    L1, info1 = torch.linalg.cholesky_ex(ms / scale[:, None] / scale[None, :])
+    T = torch.linalg.solve_triangular(L1.mT, eye, upper=True) / scale[:, None]
+    H2 = T.mT @ ms @ T
+    L2, info2 = torch.linalg.cholesky_ex(0.5 * (H2 + H2.mT))
+    T = T @ torch.linalg.solve_triangular(L2.mT, eye, upper=True)
+    error = (T.mT @ ms @ T - eye).abs().max()
+    norm = 1.0 / torch.diagonal(T)
+    relative = norm * norm / torch.diagonal(ms)
+    F = T.mT @ ss @ T
+    losses = torch.diagonal(F).clone()
+    ok = (torch.isfinite(losses).all() & (losses > 0).all() & (relative >= rel_min).all()
+          & (error <= ortho_tol) & (info1 == 0) & (info2 == 0))
+    gate = torch.where(ok, 0, PEEL_GATE).to(torch.int32).reshape(1)
+    C = -2.0 * (T * losses[None, :] + T @ torch.triu(F, 1)) / norm[None, :]
+    B = 2.0 * T / norm[None, :]
+    rows, cols = order[:, None], order[None, :]
+    Cu = torch.empty_like(C)
+    Cd = torch.empty_like(B)
+    Tr = torch.empty_like(T)
+    Cu[rows, cols] = C
+    Cd[rows, cols] = B
+    Tr[order] = T
+    pair = nw.adjoint_pair(K, dev)
+    pair[:, :K] = 0.5 * Cu.mT
+    pair[:, K:] = 0.5 * Cd.mT
+    if not deferred:
+        _raise_peel_gate(int(gate.item()))
+    return dict(losses=losses, loss=losses.sum(), order=order, T=Tr, Cu=Cu, Cd=Cd, relative_mass=relative,
+                gram_max=error, raw=raw, gate=gate, pair=pair)
+
+
+class PeelSolver:
+    """The coefficient peel as a step objective: solver(M, S) returns
+    (loss, dM, dS) like SpectralSolver, where dM, dS are the halves of the
+    peel's cotangent pair (pass them to backward_fields with
+    symmetric=True).  deferred=True keeps the gate on the device (`gate`,
+    raised by EigenfluidNet.adam through raise_gate); `last` is the peel's
+    full result dict."""
+
+    def __init__(self, ortho_tol: float = 1e-5, rel_min: float = 1e-4, deferred: bool = False):
+        self.ortho_tol, self.rel_min, self.deferred = ortho_tol, rel_min, deferred
+        self.gate = None
+        self.last = None
+        self.last_path = "coefficient-peel"
+
+    def raise_gate(self, value: int):
+        _raise_peel_gate(value)
+
+    def check(self):
+        if self.gate is not None:
+            self.raise_gate(int(self.gate.item()))
+
+    def __call__(self, M: torch.Tensor, S: torch.Tensor):
+        p = coefficient_peel(M, S, self.ortho_tol, self.rel_min, deferred=self.deferred)
+        self.gate = p["gate"]
+        self.last = p
+        K = M.shape[0]
+        return p["loss"], p["pair"][:, :K], p["pair"][:, K:]
+
+
 def generalized_eigh(S: torch.Tensor, M: torch.Tensor, rank_tol: float = 1e-12):
     """FP64 generalized symmetric eigendecomposition S x = lambda M x via
     Cholesky reduction, with the full-rank gate of
@@ -224,10 +329,15 @@ def timed(record: dict, name: str):
 
 def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
             gram_mode: str = "bf16x3", lam: float = 1e-2, ridge: float = 0.0, training: bool = True,
-            gate: str = "eig", timings: dict | None = None):
+            gate: str = "eig", timings: dict | None = None, objective: str = "trace"):
     """One spectral step (the analogue of MB/velocity/core.py:79-105
     ``compute``): jets -> lifted fields -> FP64 Grams -> spectral objective ->
-    fused reverse.  Returns dict(loss, M, S, grads, fields)."""
+    fused reverse.  objective="trace" is tr((M + ridge I)^-1 S) +
+    lam ||M - I||^2 (MB/velocity_trace_jet2.py:107-117), objective="peel" the
+    sorted Cholesky coefficient peel (core.py:48-69; its result dict is
+    returned as "peel").  Returns dict(loss, M, S, grads, fields)."""
+    if objective not in ("trace", "peel"):
+        raise ValueError(f"unknown objective {objective!r}")
     p = nw.PRECISIONS[prec]
     tt = {} if timings is None else timings
     K = net.cfg.modes
@@ -241,8 +351,14 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     # on the side stream while the FP64 objective holds this one
     prod = nw.output_products(net, tape, fields) if training else None
     with timed(tt, "spectral_objective"):
-        loss, dM, dS = spectral_objective(M, S, lam, ridge, gate=gate)
+        if objective == "peel":
+            peel = coefficient_peel(M, S)
+            loss, dM, dS = peel["loss"], peel["pair"][:, :K], peel["pair"][:, K:]
+        else:
+            loss, dM, dS = spectral_objective(M, S, lam, ridge, gate=gate)
     out = dict(loss=loss, M=M, S=S, fields=fields, timings=tt)
+    if objective == "peel":
+        out["peel"] = peel
     if training:
         with timed(tt, "analytic_backward"):
             out["grads"] = nw.backward_fields(net, tape, fields, dM, dS, weight, symmetric=True, products=prod)

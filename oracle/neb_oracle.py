@@ -325,6 +325,92 @@ def field_adjoint(Phi: np.ndarray, w, dG: np.ndarray) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------
+# next row: sorted Cholesky coefficient peel (VCORE:48-69) and its literal
+# modified-Gram-Schmidt form (VCORE:107-119)
+# --------------------------------------------------------------------------
+def coefficient_peel(M: np.ndarray, S: np.ndarray, ortho_tol: float = 1e-5, rel_min: float = 1e-4):
+    """Causal peel of the modes sorted by their raw Rayleigh quotient
+    diag(S)/diag(M) (stable order, VCORE:49-50): T = inverse upper Cholesky
+    factor of the diagonally scaled sorted mass, refined by a second
+    Cholesky pass (VCORE:51-56), so that T^T M_sorted T = I; loss_r =
+    (T^T S_sorted T)_rr.  Gates (VCORE:57-61): every loss finite and
+    positive, relative mass norm_r^2 / M_rr >= rel_min (norm = 1/diag T),
+    max |T^T M T - I| <= ortho_tol, else FloatingPointError.
+
+    The detached-prefix adjoint (VCORE:64-67): the field cotangents of
+    sum_r loss_r with every mode's earlier (peeled) modes held constant are
+    G_U = w U Cu and G_D = w D Cd for Grams M = w U^T U, S = w D^T D.
+    Returns dict(losses (sorted order), order, T (rows in original mode
+    order, columns in sorted order), Cu, Cd (original order), relative_mass,
+    gram_max, raw)."""
+    k = len(M)
+    raw = np.diag(S) / np.diag(M)
+    order = np.argsort(raw, kind="stable")
+    ms = M[np.ix_(order, order)]
+    ss = S[np.ix_(order, order)]
+    scale = np.sqrt(np.diag(ms))
+    Lc = np.linalg.cholesky(ms / scale[:, None] / scale[None, :])
+    T = np.linalg.inv(Lc.T) / scale[:, None]
+    H2 = T.T @ ms @ T
+    L2 = np.linalg.cholesky(0.5 * (H2 + H2.T))
+    T = T @ np.linalg.inv(L2.T)
+    error = float(np.max(np.abs(T.T @ ms @ T - np.eye(k))))
+    norm = 1.0 / np.diag(T)
+    relative = norm * norm / np.diag(ms)
+    F = T.T @ ss @ T
+    loss = np.diag(F).copy()
+    if not (np.all(np.isfinite(loss) & (loss > 0) & (relative >= rel_min)) and error <= ortho_tol):
+        raise FloatingPointError("rank / relative mass / orthogonality gate failed; no update")
+    C = -2.0 * (T * loss[None, :] + T @ np.triu(F, 1)) / norm[None, :]
+    B = 2.0 * T / norm[None, :]
+    Cu = np.empty_like(C)
+    Cd = np.empty_like(B)
+    Tr = np.empty_like(T)
+    Cu[np.ix_(order, order)] = C
+    Cd[np.ix_(order, order)] = B
+    Tr[order] = T
+    return dict(losses=loss, order=order, T=Tr, Cu=Cu, Cd=Cd, relative_mass=relative, gram_max=error, raw=raw)
+
+
+def literal_peel(U: np.ndarray, D: np.ndarray, w: float):
+    """Mode-by-mode form of coefficient_peel (VCORE:107-119), with quadrature
+    weight w (the reference's 1/n): modes in the stable order of
+    sum D^2 / sum U^2; each mode's velocity column is orthogonalised twice
+    against the already peeled (normalised) modes, its D column with the same
+    coefficients; loss = s/m; the cotangents of loss w.r.t. that mode's own
+    columns, earlier modes held constant, are gd = 2 w d / m and
+    gu = -2 s w u / m^2 projected twice off the earlier modes.  Returns
+    (losses, GU, GD, Q, V, order)."""
+    k = U.shape[1]
+    order = np.argsort(np.sum(D * D, 0) / np.sum(U * U, 0), kind="stable")
+    Q = np.empty_like(U)
+    V = np.empty_like(D)
+    GU = np.empty_like(U)
+    GD = np.empty_like(D)
+    losses = []
+    for r, index in enumerate(order):
+        u = U[:, index].copy()
+        d = D[:, index].copy()
+        q, v = Q[:, :r], V[:, :r]
+        for _ in range(2):
+            a = w * (q.T @ u)
+            u -= q @ a
+            d -= v @ a
+        m = w * float(u @ u)
+        s = w * float(d @ d)
+        losses.append(s / m)
+        gu = -2.0 * s * w * u / (m * m)
+        gd = 2.0 * w * d / m
+        for _ in range(2):
+            gu -= w * (q @ (q.T @ gu + v.T @ gd))
+        GU[:, index] = gu
+        GD[:, index] = gd
+        Q[:, r] = u / math.sqrt(m)
+        V[:, r] = d / math.sqrt(m)
+    return np.asarray(losses), GU, GD, Q, V, order
+
+
+# --------------------------------------------------------------------------
 # a7: FP64 generalized symmetric eigendecomposition with the rank gate
 # (experiments/periodic/paper_rollout.py:180-188)
 # --------------------------------------------------------------------------

@@ -671,3 +671,106 @@ def test_spd_inverse_rejects_indefinite_later_block():
     A[150, 150] = -1.0
     with pytest.raises(FloatingPointError, match=r"pivot 151\)"):
         sp.spd_inverse(torch.tensor(A, device=DEV))
+
+
+# ---------------------------------------------------------------------------
+# coefficient peel objective (MB/velocity/core.py:48-69, 94-95)
+# ---------------------------------------------------------------------------
+def _oracle_peel_step(onet, x, F, cfg, w):
+    x0 = o.feature_jets(x, F, cfg.lo, cfg.hi)
+    out, tape = o.forward(onet, x0, record=True)
+    n, K = len(x), cfg.modes
+    if cfg.dim == 2:
+        u, om = o.lift2d(out)
+        Pm, Ps = o.field_rows(u), om
+    else:
+        u, du = o.lift3d(out)
+        Pm, Ps = o.field_rows(u), o.field_rows(du)
+    M, S = o.gram(Pm, w), o.gram(Ps, w)
+    p = o.coefficient_peel(M, S)
+    gPm, gPs = w * Pm @ p["Cu"], w * Ps @ p["Cd"]
+    if cfg.dim == 2:
+        adj = o.lift2d_reverse(gPm.reshape(n, 2, K), gPs)
+    else:
+        adj = o.lift3d_reverse(gPm.reshape(n, 3, K), gPs.reshape(n, 3, 3, K))
+    gW, gb = o.backward(onet, tape, adj)
+    return p, M, S, gW, gb
+
+
+def test_coefficient_peel_gpu_matches_oracle():
+    rng = np.random.default_rng(31)
+    K = 96
+    U = rng.normal(size=(700, K)) @ np.diag(rng.uniform(0.3, 3.0, K))
+    D = U @ rng.normal(size=(K, K)) * 0.1 + 0.2 * rng.normal(size=(700, K))
+    M, S = o.gram(U, 1 / 350), o.gram(D, 1 / 350)
+    ref = o.coefficient_peel(M, S)
+    p = nb.coefficient_peel(torch.tensor(M, device=DEV), torch.tensor(S, device=DEV))
+    assert np.array_equal(p["order"].cpu().numpy(), ref["order"])
+    for key in ("losses", "T", "Cu", "Cd"):
+        r = rel(p[key].cpu(), ref[key]); assert r < 1e-10, (key, r)
+    assert int(p["gate"].item()) == 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_spectral_step_peel_fp64_matches_oracle(dim):
+    """Full FP64 step with the coefficient peel objective on the GPU (jets,
+    lift, Grams, peel, detached-prefix field cotangents fused into the
+    reverse) against the oracle's peel and reverse pass."""
+    F = 2
+    K = 24 if dim == 2 else 8
+    cfg, net, onet = make(dim, K, [48, 40, 32], F, master=torch.float64)
+    x = points(dim, 300, seed=7)
+    w = 1.0 / 300
+    p, M, S, gW, gb = _oracle_peel_step(onet, x, F, cfg, w)
+    res = nb.compute(net, torch.tensor(x), w, prec="f64", gram_mode="f64", objective="peel")
+    tol = 1e-9
+    r = rel(res["M"].cpu(), M); assert r < tol, r
+    assert np.array_equal(res["peel"]["order"].cpu().numpy(), p["order"])
+    r = rel(res["peel"]["losses"].cpu(), p["losses"]); assert r < tol, r
+    GW, Gb = net.split(res["grads"])
+    for a, b in zip(GW, gW):
+        r = rel(a.detach().cpu(), b); assert r < tol, r
+    for a, b in zip(Gb[:-1], gb[:-1]):
+        r = rel(a.detach().cpu(), b); assert r < tol, r
+    assert float(Gb[-1].abs().max()) == 0.0
+
+
+def test_chunked_peel_step_equals_single_pass():
+    F, K = 2, 8
+    cfg, net, _ = make(3, K, [32, 32], F, master=torch.float64)
+    x = torch.tensor(points(3, 1000, seed=8))
+    w = 1.0 / 1000
+    ref = nb.compute(net, x, w, prec="f64", gram_mode="f64", objective="peel")
+    res = nb.chunked_step(net, x, w, prec="f64", gram_mode="f64", chunk=256, solver=nb.PeelSolver())
+    r = rel(res["grads"].cpu(), ref["grads"].cpu()); assert r < 1e-11, r
+    assert abs(float(res["loss"]) - float(ref["loss"])) < 1e-11 * abs(float(ref["loss"]))
+
+
+def test_peel_reduced_precision_step():
+    """FP32 and BF16 kernels under the peel: losses and gradients against the
+    FP64 step on the same weights (2D, K = 24 <= width so M has full rank)."""
+    F, K = 2, 24
+    cfg, net, _ = make(2, K, [48, 40, 32], F, master=torch.float32)
+    x = torch.tensor(points(2, 400, seed=3))
+    w = 1.0 / 400
+    ref = nb.compute(net, x, w, prec="f64", gram_mode="f64", objective="peel")
+    for prec, gmode, tol in (("f32", "f32", 1e-3), ("bf16", "bf16x3", 2e-1)):
+        res = nb.compute(net, x, w, prec=prec, gram_mode=gmode, objective="peel")
+        r = rel(res["peel"]["losses"].cpu(), ref["peel"]["losses"].cpu()); assert r < tol, (prec, r)
+        assert torch.isfinite(res["grads"]).all()
+
+
+def test_peel_gate_skips_adam_update():
+    """A rank-deficient mass (two identical modes) fails the peel gate on the
+    device: the deferred gate makes adam skip the update and raise."""
+    cfg, net, _ = make(2, 8, [16], 2, master=torch.float32)
+    rng = np.random.default_rng(4)
+    U = rng.normal(size=(300, 8)); U[:, 5] = U[:, 2]
+    D = rng.normal(size=(300, 8)); D[:, 5] = D[:, 2]
+    solver = nb.PeelSolver(deferred=True)
+    solver(torch.tensor(o.gram(U, 0.01), device=DEV), torch.tensor(o.gram(D, 0.01), device=DEV))
+    p0 = net.params.clone()
+    g = torch.ones_like(net.params)
+    with pytest.raises(FloatingPointError, match="gate failed"):
+        net.adam(g, 1e-3, gate=solver.gate, on_gate=solver.raise_gate)
+    assert torch.equal(net.params, p0) and net.epoch == 0

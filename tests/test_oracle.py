@@ -335,3 +335,89 @@ def test_blob_domain_points():
     assert pts.shape == (1000, 3) and np.all(np.abs(pts) <= 1)
     s = sum(np.exp(-np.sum((pts - c) ** 2, 1) / (2 * 0.35 ** 2)) for c in centres)
     assert np.all(s > 0.5)
+
+
+def _peel_fields(seed=5, n=400, k=9, rank_ok=True):
+    """Random velocity-like columns U [2n, k] and derivative-like columns D."""
+    rng = np.random.default_rng(seed)
+    U = rng.normal(size=(2 * n, k)) @ np.diag(rng.uniform(0.5, 2.0, k))
+    D = U @ rng.normal(size=(k, k)) + 0.3 * rng.normal(size=(2 * n, k))
+    return U, D
+
+
+def test_coefficient_peel_equals_literal_peel():
+    """The coefficient form (VCORE:48-69) and the mode-by-mode MGS form
+    (VCORE:107-119) compute the same losses and field cotangents:
+    GU = w U Cu and GD = w D Cd (VCORE:94-95)."""
+    U, D = _peel_fields()
+    w = 1.0 / 400
+    M, S = o.gram(U, w), o.gram(D, w)
+    p = o.coefficient_peel(M, S)
+    losses, GU, GD, Q, V, order = o.literal_peel(U, D, w)
+    np.testing.assert_array_equal(p["order"], order)
+    np.testing.assert_allclose(p["losses"], losses, rtol=1e-11)
+    np.testing.assert_allclose(w * U @ p["Cu"], GU, rtol=1e-9, atol=1e-12 * np.abs(GU).max())
+    np.testing.assert_allclose(w * D @ p["Cd"], GD, rtol=1e-9, atol=1e-12 * np.abs(GD).max())
+    # the peeled modes are U T (columns in sorted order): w-orthonormal, and
+    # the same as the literal form's normalised residuals
+    np.testing.assert_allclose(U @ p["T"], Q, atol=1e-10)
+    np.testing.assert_allclose(o.gram(U @ p["T"], w), np.eye(9), atol=1e-12)
+    Ts = p["T"][p["order"]]
+    assert np.abs(np.tril(Ts, -1)).max() == 0.0       # upper triangular in sorted order
+
+
+def test_literal_peel_cotangents_match_fd_with_detached_prefix():
+    """Each mode's cotangent is the derivative of its own loss with the
+    earlier peeled modes held constant: central FD on that mode's columns."""
+    U, D = _peel_fields(seed=8, n=150, k=5)
+    w = 1.0 / 150
+    losses, GU, GD, Q, V, order = o.literal_peel(U, D, w)
+    rng = np.random.default_rng(3)
+
+    def own_loss(r, u, d):
+        q, v = Q[:, :r], V[:, :r]
+        u, d = u.copy(), d.copy()
+        for _ in range(2):
+            a = w * (q.T @ u)
+            u -= q @ a
+            d -= v @ a
+        return float(d @ d) / float(u @ u)
+
+    h = 1e-6
+    for r, index in enumerate(order):
+        du, dd = rng.normal(size=U.shape[0]), rng.normal(size=D.shape[0])
+        exact = GU[:, index] @ du + GD[:, index] @ dd
+        fd = (own_loss(r, U[:, index] + h * du, D[:, index] + h * dd)
+              - own_loss(r, U[:, index] - h * du, D[:, index] - h * dd)) / (2 * h)
+        assert abs(fd - exact) <= 1e-6 * max(1.0, abs(exact)), (r, fd, exact)
+        assert abs(own_loss(r, U[:, index], D[:, index]) - losses[r]) < 1e-12 * losses[r]
+
+
+def test_coefficient_peel_fourier_closed_form_and_gates():
+    """Orthonormal Fourier stream functions: the peel sorts the modes by
+    |k|^2 and returns losses = |k|^2 (the stiffness eigenvalues), T = I
+    permuted (verify_analytic.py:89-97 basis).  A repeated mode fails the
+    rank gate."""
+    n = 32
+    x = o.periodic_grid(n)
+    w = (2 * np.pi / n) ** 2
+    ks = [(3, 1), (1, 0), (0, 2), (2, 2), (1, 1)]
+    cols_u, cols_d, lam = [], [], []
+    for kx, ky in ks:
+        kk = kx * kx + ky * ky
+        a = 1.0 / (math.pi * math.sqrt(2.0 * kk))          # unit velocity norm
+        ph = kx * x[:, 0] + ky * x[:, 1]
+        cols_u.append(np.column_stack((a * ky * np.cos(ph), -a * kx * np.cos(ph))).reshape(-1))
+        cols_d.append(a * kk * np.sin(ph))                  # -lap psi for psi = a sin(ph)
+        lam.append(kk)
+    U, D = np.column_stack(cols_u), np.column_stack(cols_d)
+    M, S = o.gram(U, w), o.gram(D, w)
+    np.testing.assert_allclose(M, np.eye(5), atol=1e-12)
+    p = o.coefficient_peel(M, S)
+    np.testing.assert_array_equal(p["order"], np.argsort(lam, kind="stable"))
+    np.testing.assert_allclose(p["losses"], np.sort(lam), rtol=1e-12)
+    np.testing.assert_allclose(np.abs(p["T"][p["order"]]), np.eye(5), atol=1e-12)
+    Ud = np.column_stack((U, U[:, 2]))
+    Dd = np.column_stack((D, D[:, 2]))
+    with pytest.raises((FloatingPointError, np.linalg.LinAlgError)):
+        o.coefficient_peel(o.gram(Ud, w), o.gram(Dd, w))
