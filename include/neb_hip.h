@@ -37,6 +37,11 @@
  *    and epilogue math in fp32), NEB_F32 (fp32), NEB_F64 (fp64).
  *  - Leading dimensions are in elements and must keep 16-byte alignment of
  *    every row (multiples of 8 for bf16, 4 for fp32, 2 for fp64).
+ *  - Empty inputs: with zero points (n = 0, rows = 0) a call checks its
+ *    sizes and touches no point-indexed buffer, which may then be null; the
+ *    jet and field calls write nothing, the Gram / cross / column-sum
+ *    reductions write beta * G (zeros for beta = 0), like a sum over no
+ *    points.
  */
 #ifndef NEB_HIP_H
 #define NEB_HIP_H

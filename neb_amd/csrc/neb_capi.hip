@@ -436,8 +436,8 @@ int neb_feature_jets(int prec, int dim, int64_t n, const void* x, int freqs, con
                      void* out, int64_t ldf, void* stream) {
   if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
   if (n < 0 || freqs < 1 || ldf < 2 * freqs * dim) return fail(NEB_ERR_SHAPE, "neb_feature_jets: bad sizes");
+  if (n == 0) return NEB_OK;  // no points: nothing to read or write (buffers may be null)
   if (!x || !out || !lo || !hi) return fail(NEB_ERR_ARG, "neb_feature_jets: null pointer");
-  if (n == 0) return NEB_OK;
   const int64_t tot = n * ldf;
   const dim3 grid((unsigned)((tot + 255) / 256));
   const double l2 = dim == 3 ? lo[2] : 0.0, h2 = dim == 3 ? hi[2] : 1.0;
@@ -474,6 +474,7 @@ int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int6
   if (int rc = check_common(prec, dim, act)) return rc;
   if (n < 0 || k < 1 || width < 1 || ldx < k || ldw < k || ldy < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward: bad sizes");
+  if (n == 0) return NEB_OK;
   if (!x || !w || !z || !y) return fail(NEB_ERR_ARG, "neb_jet_layer_forward: null pointer");
   if (!aligned(x, ldx, prec) || !aligned(w, ldw, prec) || !aligned(y, ldy, prec) || !aligned(z, ldy, prec))
     return fail(NEB_ERR_ARG, "neb_jet_layer_forward: rows must be 16-byte aligned");
@@ -487,6 +488,7 @@ int neb_jet_output_jets(int prec, int dim, int64_t n, int64_t k, int64_t nout, c
   if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
   if (n < 0 || k < 1 || nout < 1 || ldy < k || ldw < k || ldo < nout)
     return fail(NEB_ERR_SHAPE, "neb_jet_output_jets: bad sizes");
+  if (n == 0) return NEB_OK;
   if (!y || !w || !out) return fail(NEB_ERR_ARG, "neb_jet_output_jets: null pointer");
   if (!aligned(y, ldy, prec) || !aligned(w, ldw, prec))
     return fail(NEB_ERR_ARG, "neb_jet_output_jets: rows must be 16-byte aligned");
@@ -505,6 +507,7 @@ int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, co
   if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
   if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_fields: bad sizes");
+  if (n == 0) return NEB_OK;
   if (!y || !w || !mass || !stiff) return fail(NEB_ERR_ARG, "neb_jet_output_fields: null pointer");
   if (!aligned(y, ldy, prec) || !aligned(w, ldw, prec))
     return fail(NEB_ERR_ARG, "neb_jet_output_fields: rows must be 16-byte aligned");
@@ -520,6 +523,7 @@ int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const 
   if (int rc = check_common(NEB_BF16, dim, NEB_TANH)) return rc;
   if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_fields_split: bad sizes");
+  if (n == 0) return NEB_OK;
   if (!y || !w || !mass_hi || !mass_lo || !stiff_hi || !stiff_lo)
     return fail(NEB_ERR_ARG, "neb_jet_output_fields_split: null pointer");
   if (!aligned(y, ldy, NEB_BF16) || !aligned(w, ldw, NEB_BF16) || !aligned(mass_hi, ldf, NEB_BF16) ||
@@ -537,6 +541,7 @@ int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width
   if (n < 0 || width < 1 || K < 1 || ldf < K || ldb < K || ldz < width || lda < width ||
       (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_backward: bad sizes");
+  if (n == 0) return NEB_OK;
   if (!mass || !stiff || !bm || !bs || !z || !adj) return fail(NEB_ERR_ARG, "neb_jet_output_backward: null pointer");
   const bool fok = prec == NEB_F64 ? fields_dtype == NEB_F64
                                    : (fields_dtype == NEB_F32 || (prec == NEB_BF16 && fields_dtype == NEB_BF16));
@@ -563,6 +568,7 @@ int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int
   if (int rc = check_common(prec, dim, act)) return rc;
   if (n < 0 || k < 1 || width < 1 || lda_in < width || ldw < width || ldz < k || lda_out < k)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_backward: bad sizes");
+  if (n == 0) return NEB_OK;
   if (!adj_in || !wt || !z_prev || !adj_out) return fail(NEB_ERR_ARG, "neb_jet_layer_backward: null pointer");
   if (!aligned(adj_in, lda_in, prec) || !aligned(wt, ldw, prec))
     return fail(NEB_ERR_ARG, "neb_jet_layer_backward: rows must be 16-byte aligned");
@@ -593,7 +599,7 @@ int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, co
   memset(&a, 0, sizeof(a));
   int64_t maxrows = 0;
   for (int s = 0; s < nseg; ++s) {
-    if (!x[s] || !y[s] || rows[s] < 0 || ldx[s] < m || ldy[s] < n || ldx[s] >= (1 << 26) || ldy[s] >= (1 << 26))
+    if ((rows[s] > 0 && (!x[s] || !y[s])) || rows[s] < 0 || ldx[s] < m || ldy[s] < n || ldx[s] >= (1 << 26) || ldy[s] >= (1 << 26))
       return fail(NEB_ERR_SHAPE, "neb_cross: bad segment %d", s);
     a.seg[s].x = x[s]; a.seg[s].y = y[s];
     a.seg[s].ldx = ldx[s]; a.seg[s].ldy = ldy[s];
@@ -631,7 +637,8 @@ int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t
                    double* G, int64_t ldg, void* workspace, size_t ws_bytes, void* stream) {
   if (rows < 0 || K < 1 || ld < K || ld >= (1 << 25) || ldg < K)
     return fail(NEB_ERR_SHAPE, "neb_gram_split: bad sizes (ld must be below 2^25)");
-  if (!phi_hi || !phi_lo || !G || !workspace) return fail(NEB_ERR_ARG, "neb_gram_split: null pointer");
+  if ((rows > 0 && (!phi_hi || !phi_lo)) || !G || !workspace)
+    return fail(NEB_ERR_ARG, "neb_gram_split: null pointer");
   if (!aligned(phi_hi, ld, NEB_BF16) || !aligned(phi_lo, ld, NEB_BF16))
     return fail(NEB_ERR_ARG, "neb_gram_split: rows must be 16-byte aligned");
   CrossArgs a;
@@ -656,7 +663,7 @@ size_t neb_colsum_workspace_size(int64_t rows, int64_t cols) {
 int neb_colsum(int dtype, const void* x, int64_t rows, int64_t cols, int64_t ld, double beta, double* out,
                void* workspace, size_t ws_bytes, void* stream) {
   if (rows < 0 || cols < 1 || ld < cols) return fail(NEB_ERR_SHAPE, "neb_colsum: bad sizes");
-  if (!x || !out || !workspace) return fail(NEB_ERR_ARG, "neb_colsum: null pointer");
+  if ((rows > 0 && !x) || !out || !workspace) return fail(NEB_ERR_ARG, "neb_colsum: null pointer");
   const int sp = colsum_splits(rows);
   if (ws_bytes < (size_t)sp * cols * sizeof(double)) return fail(NEB_ERR_ARG, "neb_colsum: workspace too small");
   double* part = reinterpret_cast<double*>(workspace);
@@ -670,7 +677,7 @@ int neb_colsum(int dtype, const void* x, int64_t rows, int64_t cols, int64_t ld,
 }
 
 int neb_sumsq(int dtype, const void* x, int64_t n, double* out, void* workspace, size_t ws_bytes, void* stream) {
-  if (n < 0 || !x || !out || !workspace) return fail(NEB_ERR_ARG, "neb_sumsq: bad arguments");
+  if (n < 0 || (n > 0 && !x) || !out || !workspace) return fail(NEB_ERR_ARG, "neb_sumsq: bad arguments");
   if (ws_bytes < 256 * sizeof(double)) return fail(NEB_ERR_ARG, "neb_sumsq: workspace too small");
   double* part = reinterpret_cast<double*>(workspace);
   hipStream_t st = S(stream);

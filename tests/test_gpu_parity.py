@@ -774,3 +774,28 @@ def test_peel_gate_skips_adam_update():
     with pytest.raises(FloatingPointError, match="gate failed"):
         net.adam(g, 1e-3, gate=solver.gate, on_gate=solver.raise_gate)
     assert torch.equal(net.params, p0) and net.epoch == 0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_empty_point_set(dim):
+    """Zero points: jets, fields and the reverse write nothing; the Grams and
+    parameter gradients are sums over no points (zeros), and an accumulating
+    Gram keeps beta * G."""
+    F, K = 2, 8
+    cfg, net, _ = make(dim, K, [32, 32], F, master=torch.float32)
+    x = torch.zeros((0, dim))
+    for prec, gmode in (("f32", "f32"), ("bf16", "bf16x3")):
+        p = nw.PRECISIONS[prec]
+        tape = nw.trunk_forward(net, x, p)
+        fields = nw.output_fields(net, tape, split=prec == "bf16")
+        assert fields.mass.shape[0] == 0 and fields.stiff.shape[0] == 0
+        M, S = nw.field_grams(fields, 0.5, gmode)
+        assert float(M.abs().max()) == 0.0 and float(S.abs().max()) == 0.0
+        dM = torch.eye(K, dtype=torch.float64, device=DEV)
+        g = nw.backward_fields(net, tape, fields, dM, 2.0 * dM, 0.5)
+        assert float(g.abs().max()) == 0.0
+    for mode in ("f32", "f64"):
+        dt = torch.float64 if mode == "f64" else torch.float32
+        G = torch.full((K, K), 3.0, dtype=torch.float64, device=DEV)
+        nw.gram(torch.zeros((0, K), dtype=dt, device=DEV), K, 1.0, mode, out=G, beta=1.0)
+        assert float((G - 3.0).abs().max()) == 0.0
