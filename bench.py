@@ -79,7 +79,7 @@ class StageTimer:
         return {k: [a.elapsed_time(b) for a, b in v] for k, v in self.events.items()}
 
 
-def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=None):
+def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=None, reverse_prec=None):
     import torch
     K = net.cfg.modes
     solver = solver if solver is not None else sp.SpectralSolver(1e-2, ridge, deferred=True)
@@ -102,6 +102,8 @@ def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=
             ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
             timer.events.setdefault("gram_mass_launch", []).append(list(ev))
         M, S = mark(lambda: nw.field_grams(fields, w, gram_mode, mass_events=ev), "grams")
+        if reverse_prec is not None:  # FP32 fields and Grams, BF16 reverse on the converted tape
+            tape = nw.convert_tape(tape, nw.PRECISIONS[reverse_prec])
         # output-layer products on the side stream, overlapping the FP64 objective
         prod = nw.output_products(net, tape, fields)
         loss, dM, dS = mark(lambda: solver(M, S), "objective_fp64")
@@ -517,23 +519,24 @@ def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
     lam64, _ = sp.generalized_eigh(S64, M64 + ridge * eye)
     top = WIDTHS[-1]  # rank of M: the physical eigenvalues are the top `top`; the rest are ridge-dominated
     accuracy = {}
-    for prec in ("bf16", "f32"):
+    for name, prec, gm in (("bf16", "bf16", "bf16x3"), ("f32", "f32", "f32"), ("f32+bf16x3", "f32", "bf16x3")):
         tp = nw.trunk_forward(net, x, nw.PRECISIONS[prec])
         fp = nw.output_fields(net, tp, split=(prec == "bf16"))
         del tp
-        gm = "bf16x3" if prec == "bf16" else "f32"
         Mp = nw.field_gram(fp, "mass", w, gm)
         Sp = nw.field_gram(fp, "stiff", w, gm)
         U = fp.mass[:, :K].float() + (fp.mass_lo[:, :K].float() if fp.split else 0.0)
         vel = float(torch.linalg.norm(U.double() - f64f.mass[:, :K]) / torch.linalg.norm(f64f.mass[:, :K]))
         lam, _ = sp.generalized_eigh(Sp, Mp + ridge * eye)
-        accuracy[prec] = dict(velocity_rel_l2=vel,
+        accuracy[name] = dict(velocity_rel_l2=vel,
                               mass_gram_rel_err=float(torch.linalg.norm(Mp - M64) / torch.linalg.norm(M64)),
                               stiffness_gram_rel_err=float(torch.linalg.norm(Sp - S64) / torch.linalg.norm(S64)),
                               eigenvalue_max_rel_err_top=float(((lam[-top:] - lam64[-top:]).abs()
                                                                 / lam64[-top:].abs()).max()))
         del fp, U
-    accuracy["note"] = (f"vs the FP64 path on the same grid and weights; eigenvalues of S x = lambda (M + ridge I) x, "
+    accuracy["note"] = (f"vs the FP64 path on the same grid and weights; f32+bf16x3 = FP32 jets and fields, "
+                        f"split-precision BF16 Grams (the fields and Grams of the fp32_fields_bf16_reverse step); "
+                        f"eigenvalues of S x = lambda (M + ridge I) x, "
                         f"the top {top} (= rank M; the others are set by the ridge); bf16 velocities are the hi + lo "
                         f"planes of the fp32 field values")
     return accuracy
@@ -603,6 +606,23 @@ def main():
         torch.cuda.synchronize()
         f32_rate = N * K / ((time.perf_counter() - t0) / n32)
 
+    # ---- FP32 jets / fields, split BF16 Grams, BF16 reverse -----------------
+    # (Grams at FP32 accuracy; the reverse on BF16 MFMA from the rounded tape)
+    mixed = None
+    if not args.no_f32:
+        stepm = make_step(nb, nw, sp, net, x, w, "f32", "bf16x3", ridge, reverse_prec="bf16")
+        stepm()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nm = max(1, min(3, args.steps))
+        for _ in range(nm):
+            stepm()
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - t0) / nm
+        mixed = dict(point_modes_per_s=N * K / tm, ms_per_step=tm * 1e3,
+                     vs_fp32_step=(N * K / tm) / f32_rate if f32_rate else None,
+                     accuracy="accuracy_vs_fp64['f32+bf16x3']")
+
     # ---- headline BF16 step --------------------------------------------------
     solver = sp.SpectralSolver(1e-2, ridge, deferred=True)
     step = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, solver=solver)
@@ -650,6 +670,7 @@ def main():
         "stages_ms": stages,
         "gram": gram_stats,
         "fp32_step_point_modes_per_s": f32_rate,
+        "fp32_fields_bf16_reverse_step": mixed,
         "accuracy_vs_fp64": accuracy,
         "loss": float(loss),
         "objective_path": solver.last_path,

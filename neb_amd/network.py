@@ -317,6 +317,26 @@ def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
     return tape
 
 
+def convert_tape(tape: Tape, prec: int) -> Tape:
+    """The same jet tape stored in another operand precision (one staging
+    launch per tensor, rounding to nearest): an FP32 forward's tape feeds a
+    BF16 reverse pass this way."""
+    if prec == tape.prec:
+        return tape
+    st = _STORE[prec]
+    s = _stream()
+
+    def conv(t: torch.Tensor) -> torch.Tensor:
+        out = torch.empty(t.shape, dtype=st, device=t.device)
+        rows, cols = t.shape[0] * t.shape[1], t.shape[2]
+        if rows:
+            H.call("neb_stage", _DT[t.dtype], prec, t.data_ptr(), rows, cols, cols, out.data_ptr(), cols, rows, cols,
+                   0, 1.0, s)
+        return out
+
+    return Tape(prec, tape.n, [conv(t) for t in tape.inputs], [conv(t) for t in tape.pre])
+
+
 def output_jets(net: EigenfluidNet, tape: Tape):
     """Raw output jets (value [n, nout], grad [n, nout, d], hess [n, nout, P])
     -- the analogue of the per-mode field jets of forward_stream_jet2

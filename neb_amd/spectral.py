@@ -329,13 +329,18 @@ def timed(record: dict, name: str):
 
 def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
             gram_mode: str = "bf16x3", lam: float = 1e-2, ridge: float = 0.0, training: bool = True,
-            gate: str = "eig", timings: dict | None = None, objective: str = "trace"):
+            gate: str = "eig", timings: dict | None = None, objective: str = "trace",
+            reverse_prec: str | None = None):
     """One spectral step (the analogue of MB/velocity/core.py:79-105
     ``compute``): jets -> lifted fields -> FP64 Grams -> spectral objective ->
     fused reverse.  objective="trace" is tr((M + ridge I)^-1 S) +
     lam ||M - I||^2 (MB/velocity_trace_jet2.py:107-117), objective="peel" the
     sorted Cholesky coefficient peel (core.py:48-69; its result dict is
-    returned as "peel").  Returns dict(loss, M, S, grads, fields)."""
+    returned as "peel").  reverse_prec (default: prec) runs the reverse pass
+    in another operand precision on the same fields and a converted tape:
+    prec="f32", gram_mode="bf16x3", reverse_prec="bf16" keeps the fields and
+    Grams at FP32 accuracy and runs the reverse on BF16 MFMA.  Returns
+    dict(loss, M, S, grads, fields)."""
     if objective not in ("trace", "peel"):
         raise ValueError(f"unknown objective {objective!r}")
     p = nw.PRECISIONS[prec]
@@ -347,9 +352,10 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
         fields = nw.output_fields(net, tape, split=split)
     with timed(tt, "gram_assembly"):
         M, S = nw.field_grams(fields, weight, gram_mode)
+    rtape = tape if reverse_prec is None else nw.convert_tape(tape, nw.PRECISIONS[reverse_prec])
     # the output-layer products need only the fields and the tape: they run
     # on the side stream while the FP64 objective holds this one
-    prod = nw.output_products(net, tape, fields) if training else None
+    prod = nw.output_products(net, rtape, fields) if training else None
     with timed(tt, "spectral_objective"):
         if objective == "peel":
             peel = coefficient_peel(M, S)
@@ -361,7 +367,7 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
         out["peel"] = peel
     if training:
         with timed(tt, "analytic_backward"):
-            out["grads"] = nw.backward_fields(net, tape, fields, dM, dS, weight, symmetric=True, products=prod)
+            out["grads"] = nw.backward_fields(net, rtape, fields, dM, dS, weight, symmetric=True, products=prod)
     return out
 
 

@@ -799,3 +799,35 @@ def test_empty_point_set(dim):
         G = torch.full((K, K), 3.0, dtype=torch.float64, device=DEV)
         nw.gram(torch.zeros((0, K), dtype=dt, device=DEV), K, 1.0, mode, out=G, beta=1.0)
         assert float((G - 3.0).abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fp32_fields_bf16_reverse(dim):
+    """FP32 jets and fields, split-precision Gram, BF16 reverse on the
+    converted tape: Grams at FP32 accuracy against the FP64 oracle (<= 1e-5),
+    the reverse pass against the oracle's at the BF16 tolerance."""
+    F = 2
+    K = 24 if dim == 2 else 8
+    cfg, net, onet = make(dim, K, [48, 40, 32], F, master=torch.float32)
+    x = points(dim, 300, seed=7)
+    w = 1.0 / 300
+    L, M, S, *_ = _oracle_step(onet, x, F, cfg, w, 1e-2, 0.0)
+    res = nb.compute(net, torch.tensor(x), w, prec="f32", gram_mode="bf16x3", gate="cholesky", reverse_prec="bf16")
+    r = rel(res["M"].cpu(), M); assert r < 1e-5, r
+    r = rel(res["S"].cpu(), S); assert r < 1e-5, r
+    assert torch.isfinite(res["grads"]).all()
+    rng = np.random.default_rng(21)
+    dM = rng.normal(size=(K, K)); dM = dM + dM.T
+    dS = rng.normal(size=(K, K)); dS = dS + dS.T
+    gW, gb = _oracle_reverse(onet, x, F, cfg, w, dM, dS)
+    tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS["f32"])
+    fields = nw.output_fields(net, tape)
+    rt = nw.convert_tape(tape, nw.PRECISIONS["bf16"])
+    assert rt.inputs[-1].dtype == torch.bfloat16
+    assert torch.equal(rt.pre[0], tape.pre[0].to(torch.bfloat16))
+    g = nw.backward_fields(net, rt, fields, torch.tensor(dM, device=DEV), torch.tensor(dS, device=DEV), w)
+    GW, Gb = net.split(g)
+    for a, b in zip(GW, gW):
+        r = rel(a.detach().cpu(), b); assert r < 4e-2, r
+    for a, b in zip(Gb[:-1], gb[:-1]):
+        r = rel(a.detach().cpu(), b); assert r < 4e-2, r
