@@ -519,9 +519,10 @@ def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
     lam64, _ = sp.generalized_eigh(S64, M64 + ridge * eye)
     top = WIDTHS[-1]  # rank of M: the physical eigenvalues are the top `top`; the rest are ridge-dominated
     accuracy = {}
-    for name, prec, gm in (("bf16", "bf16", "bf16x3"), ("f32", "f32", "f32"), ("f32+bf16x3", "f32", "bf16x3")):
+    for name, prec, gm in (("bf16", "bf16", "bf16x3"), ("f32", "f32", "f32"), ("f32+bf16x3", "f32", "bf16x3"),
+                           ("bf16x3", "bf16x3", "bf16x3")):
         tp = nw.trunk_forward(net, x, nw.PRECISIONS[prec])
-        fp = nw.output_fields(net, tp, split=(prec == "bf16"))
+        fp = nw.output_fields(net, tp, split=prec in ("bf16", "bf16x3"))
         del tp
         Mp = nw.field_gram(fp, "mass", w, gm)
         Sp = nw.field_gram(fp, "stiff", w, gm)
@@ -536,6 +537,8 @@ def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
         del fp, U
     accuracy["note"] = (f"vs the FP64 path on the same grid and weights; f32+bf16x3 = FP32 jets and fields, "
                         f"split-precision BF16 Grams (the fields and Grams of the fp32_fields_bf16_reverse step); "
+                        f"bf16x3 = split-precision jets and fields (bf16 hi + lo planes, three bf16 MFMA products "
+                        f"per term) and split Grams (the split_precision_step); "
                         f"eigenvalues of S x = lambda (M + ridge I) x, "
                         f"the top {top} (= rank M; the others are set by the ridge); bf16 velocities are the hi + lo "
                         f"planes of the fp32 field values")
@@ -623,6 +626,22 @@ def main():
                      vs_fp32_step=(N * K / tm) / f32_rate if f32_rate else None,
                      accuracy="accuracy_vs_fp64['f32+bf16x3']")
 
+    # ---- split-precision jets and fields, split Grams, BF16 reverse ---------
+    x3 = None
+    if not args.no_f32:
+        stepx = make_step(nb, nw, sp, net, x, w, "bf16x3", "bf16x3", ridge)
+        stepx()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nx = max(1, min(5, args.steps))
+        for _ in range(nx):
+            stepx()
+        torch.cuda.synchronize()
+        tx = (time.perf_counter() - t0) / nx
+        x3 = dict(point_modes_per_s=N * K / tx, ms_per_step=tx * 1e3,
+                  vs_fp32_step=(N * K / tx) / f32_rate if f32_rate else None,
+                  accuracy="accuracy_vs_fp64['bf16x3']")
+
     # ---- headline BF16 step --------------------------------------------------
     solver = sp.SpectralSolver(1e-2, ridge, deferred=True)
     step = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, solver=solver)
@@ -671,6 +690,7 @@ def main():
         "gram": gram_stats,
         "fp32_step_point_modes_per_s": f32_rate,
         "fp32_fields_bf16_reverse_step": mixed,
+        "split_precision_step": x3,
         "accuracy_vs_fp64": accuracy,
         "loss": float(loss),
         "objective_path": solver.last_path,

@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 5
+#define NEB_ABI_VERSION 6
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -109,6 +109,32 @@ int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, co
 int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy, const void* w,
                                 int64_t ldw, void* mass_hi, void* mass_lo, void* stiff_hi, void* stiff_lo,
                                 int64_t ldf, int layout, void* stream);
+
+/* Split-precision operands (2D jets only; dim 3 returns NEB_ERR_ARG).  A
+ * value x is held as two bf16 planes hi = bf16(x), lo = bf16(x - hi); every
+ * product of the contraction is W_lo x_hi + W_hi x_lo + W_hi x_hi on bf16
+ * MFMA with fp32 accumulation (about 16 significant bits per operand), and
+ * the results are stored as hi / lo planes again.  The jets and fields then
+ * carry FP32-level accuracy at bf16 MFMA rates; the hi planes are the bf16
+ * tape and fields of the reverse pass.
+ *
+ * neb_split_bf16: hi / lo planes of an fp32 or fp64 matrix (rows x cols,
+ * source stride lds, plane stride ldd): the feature jets and master weights
+ * enter the split path through it (the `.astype(compute_dtype)` of
+ * HB2P:1026-1030, VJ2:68, kept at ~16 bits).
+ * neb_jet_layer_forward_x3: neb_jet_layer_forward on split input jets and
+ * weights; writes z and y as hi / lo planes (HB2P:1005-1040, VJ2:20-34).
+ * neb_jet_output_fields_x3: neb_jet_output_fields_split on split last-hidden
+ * jets and output weights (VCORE:71-77, ELC:35-38). */
+int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* hi, void* lo,
+                   int64_t ldd, void* stream);
+int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t width, const void* x_hi,
+                             const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
+                             const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,
+                             int64_t ldy, void* stream);
+int neb_jet_output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo,
+                             int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
+                             void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, int layout, void* stream);
 
 /* Output-layer reverse fused with the lift transpose and the transpose of
  * the last hidden activation.  Given field matrices Phi (mass, stiff) and

@@ -424,6 +424,65 @@ template <int D, int ACT> struct PlanHiddenFwd {
   }
 };
 
+// Split-precision operands (2D): the input jets and the weights are each held
+// as two bf16 planes, x = x_hi + x_lo and W = W_hi + W_lo, and every product
+// is W_lo x_hi + W_hi x_lo + W_hi x_hi on bf16 MFMA with fp32 accumulation
+// (W_lo x_lo is below fp32 rounding), so the contraction carries ~16
+// significant bits of each operand instead of 8.  Inputs x[0..CH-1] are the
+// hi planes and x[CH..2CH-1] the lo planes of the same channels; w[0] = W_hi,
+// w[1] = W_lo.  Each base term (accumulator a, channel c, sign) becomes the
+// three terms (a, c, W_lo), (a, c + CH, W_hi), (a, c, W_hi): small products
+// first.
+
+// Hidden layer forward on split operands (2D): Z = omega*(X W^T + b) and
+// Y = f-jet(Z) as PlanHiddenFwd, each stored as hi / lo bf16 planes
+// (out0 / out0lo = Z, out1 / out1lo = Y).
+template <int ACT> struct PlanHiddenFwdSplit2 {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
+  static constexpr int C = Jet2Layout<2>::C;
+  static constexpr int CI = 2 * C, NW = 2, NA = C, NT = 3 * C;
+  static constexpr int FB = TileFor<2>::FB, PB = TileFor<2>::PB, WPN = TileFor<2>::WPN, WPP = TileFor<2>::WPP;
+  // base terms: channel c into accumulator c (CH = 6)
+  static constexpr int ta[NT] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5};
+  static constexpr int tc[NT] = {0, 6, 0, 1, 7, 1, 2, 8, 2, 3, 9, 3, 4, 10, 4, 5, 11, 5};
+  static constexpr int tw[NT] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0};
+  static constexpr bool tneg[NT] = {false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    using L = Jet2Layout<2>;
+    const M om = (M)a.omega;
+    const M* bias = reinterpret_cast<const M*>(a.bias);
+    M u[C][GE];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      const M b = (bias && e < nv) ? bias[f + e] : (M)0;
+      u[0][e] = om * (v[0][e] + b);
+#pragma unroll
+      for (int c = 1; c < C; ++c) u[c][e] = om * v[c][e];
+    }
+    const int64_t off = p * a.ldo + f;
+#pragma unroll
+    for (int c = 0; c < C; ++c) st_field<true, GE, M>(a.out0, a.out0lo, off + c * a.cstride, u[c], nv);
+    M f1[GE], f2[GE], t[GE];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) Act<ACT, M>::eval(u[0][e], t[e], f1[e], f2[e]);
+    st_field<true, GE, M>(a.out1, a.out1lo, off, t, nv);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+      for (int e = 0; e < GE; ++e) t[e] = f1[e] * u[1 + c][e];
+      st_field<true, GE, M>(a.out1, a.out1lo, off + (1 + c) * a.cstride, t, nv);
+    }
+#pragma unroll
+    for (int q = 0; q < L::P; ++q) {
+#pragma unroll
+      for (int e = 0; e < GE; ++e)
+        t[e] = f1[e] * u[3 + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
+      st_field<true, GE, M>(a.out1, a.out1lo, off + (3 + q) * a.cstride, t, nv);
+    }
+  }
+};
+
 // Output layer, raw jets: O = X W^T (+ b on the value channel), field_t out.
 template <int D> struct PlanOutJets {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::math_t); }
@@ -466,6 +525,25 @@ template <bool SPLIT> struct PlanOutFields2T {
   }
 };
 using PlanOutFields2 = PlanOutFields2T<false>;
+
+// 2D output layer + lift on split operands (as above): inputs x[0..3]
+// = hi planes of gx, gy, hxx, hyy, x[4..7] their lo planes; w[0] = W_hi,
+// w[1] = W_lo; the fields are stored as hi / lo bf16 planes.
+struct PlanOutFields2Split {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
+  using B = PlanOutFields2T<true>;
+  static constexpr int CI = 8, NW = 2, NA = 3, NT = 12;
+  static constexpr int FB = B::FB, PB = B::PB, WPN = B::WPN, WPP = B::WPP;
+  // base terms of PlanOutFields2T (CH = 4): u += gy, v -= gx, omega -= hxx, omega -= hyy
+  static constexpr int ta[NT] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 2, 2};
+  static constexpr int tc[NT] = {1, 5, 1, 0, 4, 0, 2, 6, 2, 3, 7, 3};
+  static constexpr int tw[NT] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0};
+  static constexpr bool tneg[NT] = {false, false, false, true, true, true, true, true, true, true, true, true};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    B::template epilogue<P, GE, M>(a, p, f, nv, v);
+  }
+};
 
 // 3D output layer with the curl lift (NCC:97-99, VJ2:93-101).  Vector
 // potential A has output rows 3k+c, passed as three weight slices W_c.

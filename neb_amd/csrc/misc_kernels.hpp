@@ -182,6 +182,21 @@ __global__ __launch_bounds__(256) void adam_gated_kernel(T* __restrict__ p, cons
 // dst[r][c] (dst_rows x dst_cols, zero outside the source) = src[r][c] or
 // src[c][r] when transposing; converts the master precision to the MFMA
 // operand precision.
+// Split of a matrix into bf16 planes: hi = bf16(x), lo = bf16(x - hi)
+// (the difference taken in the source precision, exact for fp32 sources).
+template <typename TS>
+__global__ __launch_bounds__(256) void split_kernel(const TS* __restrict__ src, int64_t rows, int64_t cols,
+                                                    int64_t lds, __bf16* __restrict__ hi, __bf16* __restrict__ lo,
+                                                    int64_t ldd) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const int64_t r = idx / cols, c = idx % cols;
+  const double x = todbl_any(src[r * lds + c]);
+  const __bf16 h = (__bf16)(float)x;
+  hi[r * ldd + c] = h;
+  lo[r * ldd + c] = (__bf16)(float)(x - (double)(float)h);
+}
+
 template <typename TS, typename TD>
 __global__ __launch_bounds__(256) void stage_kernel(const TS* __restrict__ src, int64_t rows, int64_t cols,
                                                     int64_t lds, TD* __restrict__ dst, int64_t ldd,

@@ -167,6 +167,44 @@ int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64
   return launch_rowgemm<P, Sx, PlanOutFields3T<SPLIT>>(a, st, what);
 }
 
+// split operands (2D): x_hi/x_lo, w_hi/w_lo planes (PlanHiddenFwdSplit2)
+template <int ACT>
+int layer_forward_x3(int64_t n, int64_t k, int64_t width, const void* x_hi, const void* x_lo, int64_t ldx,
+                     const void* w_hi, const void* w_lo, int64_t ldw, const void* bias, double omega, void* z_hi,
+                     void* z_lo, void* y_hi, void* y_lo, int64_t ldy, hipStream_t st) {
+  constexpr int C = Jet2Layout<2>::C;
+  RowArgs a = row_args();
+  for (int c = 0; c < C; ++c) {
+    a.x[c] = reinterpret_cast<const __bf16*>(x_hi) + (int64_t)c * n * ldx;
+    a.x[C + c] = reinterpret_cast<const __bf16*>(x_lo) + (int64_t)c * n * ldx;
+    a.ldx[c] = a.ldx[C + c] = ldx;
+  }
+  a.w[0] = w_hi; a.w[1] = w_lo; a.ldw[0] = a.ldw[1] = ldw;
+  a.npts = n; a.nout = width; a.k = k;
+  a.bias = bias; a.omega = omega;
+  a.out0 = z_hi; a.out0lo = z_lo; a.out1 = y_hi; a.out1lo = y_lo;
+  a.ldo = ldy; a.cstride = n * ldy;
+  return launch_rowgemm<PREC_BF16, __bf16, PlanHiddenFwdSplit2<ACT>>(a, st, "neb_jet_layer_forward_x3");
+}
+
+int output_fields_x3(int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo, int64_t ldy,
+                     const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi, void* mass_lo, void* stiff_hi,
+                     void* stiff_lo, int64_t ldf, int layout, hipStream_t st) {
+  const int64_t cs = n * ldy;
+  RowArgs a = row_args();
+  a.npts = n; a.nout = K; a.k = k;
+  a.out0 = mass_hi; a.out0lo = mass_lo; a.out1 = stiff_hi; a.out1lo = stiff_lo;
+  a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
+  const int chans[4] = {1, 2, 3, 5};  // gx, gy, hxx, hyy
+  for (int c = 0; c < 4; ++c) {
+    a.x[c] = reinterpret_cast<const __bf16*>(y_hi) + chans[c] * cs;
+    a.x[4 + c] = reinterpret_cast<const __bf16*>(y_lo) + chans[c] * cs;
+    a.ldx[c] = a.ldx[4 + c] = ldy;
+  }
+  a.w[0] = w_hi; a.w[1] = w_lo; a.ldw[0] = a.ldw[1] = ldw;
+  return launch_rowgemm<PREC_BF16, __bf16, PlanOutFields2Split>(a, st, "neb_jet_output_fields_x3");
+}
+
 template <int P, int ACT, typename F>
 int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* mass, const void* stiff, int64_t ldf,
                     int layout, const void* bm, const void* bs, int64_t ldb, const void* z, int64_t ldz,
@@ -812,6 +850,71 @@ int neb_spectral_adjoint(int64_t K, const double* C, int64_t ldc, const double* 
   hipLaunchKernelGGL(spectral_adjoint_kernel, dim3((unsigned)nt, (unsigned)nt), dim3(256), 0, S(stream), K, C, ldc,
                      M, ldm, lam, dM, ldd);
   return hip_status("neb_spectral_adjoint");
+}
+
+int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* hi, void* lo,
+                   int64_t ldd, void* stream) {
+  if (rows < 0 || cols < 0 || lds < cols || ldd < cols) return fail(NEB_ERR_SHAPE, "neb_split_bf16: bad sizes");
+  const int64_t tot = rows * cols;
+  if (tot == 0) return NEB_OK;
+  if (!src || !hi || !lo) return fail(NEB_ERR_ARG, "neb_split_bf16: null pointer");
+  if (tot / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_split_bf16: too many elements");
+  const dim3 grid((unsigned)((tot + 255) / 256));
+  __bf16* h = reinterpret_cast<__bf16*>(hi);
+  __bf16* l = reinterpret_cast<__bf16*>(lo);
+  hipStream_t st = S(stream);
+  switch (src_dtype) {
+    case NEB_F32:
+      hipLaunchKernelGGL(split_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<const float*>(src), rows, cols,
+                         lds, h, l, ldd);
+      return hip_status("neb_split_bf16");
+    case NEB_F64:
+      hipLaunchKernelGGL(split_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<const double*>(src), rows,
+                         cols, lds, h, l, ldd);
+      return hip_status("neb_split_bf16");
+  }
+  return fail(NEB_ERR_ARG, "neb_split_bf16: source dtype must be NEB_F32 or NEB_F64 (got %d)", src_dtype);
+}
+
+int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t width, const void* x_hi,
+                             const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
+                             const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,
+                             int64_t ldy, void* stream) {
+  if (int rc = check_common(NEB_BF16, dim, act)) return rc;
+  if (dim != 2) return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3: split operands are implemented for 2D jets");
+  if (n < 0 || k < 1 || width < 1 || ldx < k || ldw < k || ldy < width)
+    return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_x3: bad sizes");
+  if (n == 0) return NEB_OK;
+  if (!x_hi || !x_lo || !w_hi || !w_lo || !z_hi || !z_lo || !y_hi || !y_lo)
+    return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3: null pointer");
+  if (!aligned(x_hi, ldx, NEB_BF16) || !aligned(x_lo, ldx, NEB_BF16) || !aligned(w_hi, ldw, NEB_BF16) ||
+      !aligned(w_lo, ldw, NEB_BF16) || !aligned(z_hi, ldy, NEB_BF16) || !aligned(z_lo, ldy, NEB_BF16) ||
+      !aligned(y_hi, ldy, NEB_BF16) || !aligned(y_lo, ldy, NEB_BF16))
+    return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3: rows must be 16-byte aligned");
+  hipStream_t st = S(stream);
+  if (act == NEB_TANH)
+    return layer_forward_x3<ACT_TANH>(n, k, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi,
+                                      y_lo, ldy, st);
+  return layer_forward_x3<ACT_SIN>(n, k, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi,
+                                   y_lo, ldy, st);
+}
+
+int neb_jet_output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo,
+                             int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
+                             void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, int layout, void* stream) {
+  if (int rc = check_common(NEB_BF16, dim, NEB_TANH)) return rc;
+  if (dim != 2) return fail(NEB_ERR_ARG, "neb_jet_output_fields_x3: split operands are implemented for 2D jets");
+  if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
+    return fail(NEB_ERR_SHAPE, "neb_jet_output_fields_x3: bad sizes");
+  if (n == 0) return NEB_OK;
+  if (!y_hi || !y_lo || !w_hi || !w_lo || !mass_hi || !mass_lo || !stiff_hi || !stiff_lo)
+    return fail(NEB_ERR_ARG, "neb_jet_output_fields_x3: null pointer");
+  if (!aligned(y_hi, ldy, NEB_BF16) || !aligned(y_lo, ldy, NEB_BF16) || !aligned(w_hi, ldw, NEB_BF16) ||
+      !aligned(w_lo, ldw, NEB_BF16) || !aligned(mass_hi, ldf, NEB_BF16) || !aligned(mass_lo, ldf, NEB_BF16) ||
+      !aligned(stiff_hi, ldf, NEB_BF16) || !aligned(stiff_lo, ldf, NEB_BF16))
+    return fail(NEB_ERR_ARG, "neb_jet_output_fields_x3: rows must be 16-byte aligned");
+  return output_fields_x3(n, k, K, y_hi, y_lo, ldy, w_hi, w_lo, ldw, mass_hi, mass_lo, stiff_hi, stiff_lo, ldf,
+                          layout, S(stream));
 }
 
 int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst,

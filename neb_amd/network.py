@@ -27,7 +27,11 @@ import torch
 
 from . import _hip as H
 
-PRECISIONS = {"bf16": H.BF16, "f32": H.F32, "f64": H.F64}
+# BF16X3: host-side code of the split-precision forward (bf16 hi / lo planes
+# of every jet and weight, three bf16 MFMA products per term; 2D); its tape is
+# a bf16 tape (the hi planes) with the lo planes beside it
+BF16X3 = 3
+PRECISIONS = {"bf16": H.BF16, "f32": H.F32, "f64": H.F64, "bf16x3": BF16X3}
 _STORE = {H.BF16: torch.bfloat16, H.F32: torch.float32, H.F64: torch.float64}
 _FIELD = {H.BF16: torch.float32, H.F32: torch.float32, H.F64: torch.float64}
 _CROSS = {H.BF16: H.CROSS_BF16, H.F32: H.CROSS_F32, H.F64: H.CROSS_F64}
@@ -145,6 +149,7 @@ class EigenfluidNet:
         self.epoch = 0
         self._views()
         self._staged = {}
+        self._split_cache = None
         self.version = 0
 
     # -- parameter views -------------------------------------------------
@@ -201,6 +206,26 @@ class EigenfluidNet:
                 out["WT"].append(wt)
             out["b"].append(b.to(math_dt).contiguous())
         self._staged = {key: out}
+        return out
+
+    def staged_split(self):
+        """Master weights as bf16 hi / lo planes ([out][ld_in] each; hi is the
+        bf16 staging of staged(BF16)) and fp32 biases, for the split-precision
+        forward."""
+        if self._split_cache is not None and self._split_cache[0] == self.version:
+            return self._split_cache[1]
+        mdt = _DT[self.master_dtype]
+        s = _stream()
+        out = {"W": [], "Wlo": [], "b": []}
+        for W, b in zip(self.W, self.b):
+            fo, fi = W.shape
+            hi = torch.zeros((fo, _ld(fi)), dtype=torch.bfloat16, device=self.device)
+            lo = torch.zeros_like(hi)
+            H.call("neb_split_bf16", mdt, W.data_ptr(), fo, fi, fi, hi.data_ptr(), lo.data_ptr(), hi.shape[1], s)
+            out["W"].append(hi)
+            out["Wlo"].append(lo)
+            out["b"].append(b.to(torch.float32).contiguous())
+        self._split_cache = (self.version, out)
         return out
 
     # -- optimizer ---------------------------------------------------------
@@ -282,6 +307,8 @@ class Tape:
     n: int
     inputs: list = field(default_factory=list)   # packed layer inputs [C, n, ld] (X0, Y1, ..., Y_L)
     pre: list = field(default_factory=list)      # packed scaled pre-activations Z_l
+    inputs_lo: list | None = None   # split-precision tapes: lo planes beside the hi planes above
+    pre_lo: list | None = None
 
 
 def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
@@ -289,6 +316,8 @@ def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
     cfg = net.cfg
     if x.ndim != 2 or x.shape[1] != cfg.dim:
         raise ValueError(f"points must be [n, {cfg.dim}]")
+    if prec == BF16X3:
+        return _trunk_forward_x3(net, x)
     xdt = torch.float64 if prec == H.F64 else torch.float32
     x = x.to(device=net.device, dtype=xdt).contiguous()
     n = x.shape[0]
@@ -314,6 +343,49 @@ def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
         tape.pre.append(Z)
         tape.inputs.append(Y)
         cur, k = Y, width
+    return tape
+
+
+def _trunk_forward_x3(net: EigenfluidNet, x: torch.Tensor) -> Tape:
+    """Split-precision trunk (2D): fp32 feature jets split into bf16 hi / lo
+    planes, then every hidden layer as neb_jet_layer_forward_x3 (three bf16
+    MFMA products per term, Z and Y stored as hi / lo planes).  The returned
+    tape is the bf16 tape of the hi planes, which the reverse pass reads, with
+    the lo planes in inputs_lo / pre_lo."""
+    import ctypes
+    cfg = net.cfg
+    if cfg.dim != 2:
+        raise ValueError("the split-precision forward is implemented for 2D jets")
+    x = x.to(device=net.device, dtype=torch.float32).contiguous()
+    n = x.shape[0]
+    C = channels(cfg.dim)
+    s = _stream()
+    lo = H.dbl_array([float(v) for v in cfg.lo])
+    hi = H.dbl_array([float(v) for v in cfg.hi])
+    F = cfg.features
+    X0 = torch.empty((C, n, _ld(F)), dtype=torch.float32, device=net.device)
+    H.call("neb_feature_jets", H.F32, cfg.dim, n, x.data_ptr(), cfg.frequencies, ctypes.addressof(lo),
+           ctypes.addressof(hi), X0.data_ptr(), X0.shape[2], s)
+    Xh = torch.empty(X0.shape, dtype=torch.bfloat16, device=net.device)
+    Xl = torch.empty_like(Xh)
+    H.call("neb_split_bf16", H.F32, X0.data_ptr(), C * n, F, X0.shape[2], Xh.data_ptr(), Xl.data_ptr(), Xh.shape[2],
+           s)
+    del X0
+    stg = net.staged_split()
+    tape = Tape(H.BF16, n, [Xh], [], [Xl], [])
+    ch, cl, k = Xh, Xl, F
+    for l, width in enumerate(cfg.widths):
+        Zh = torch.empty((C, n, _ld(width)), dtype=torch.bfloat16, device=net.device)
+        Zl, Yh, Yl = torch.empty_like(Zh), torch.empty_like(Zh), torch.empty_like(Zh)
+        H.call("neb_jet_layer_forward_x3", cfg.dim, cfg.act_code, n, k, width, ch.data_ptr(), cl.data_ptr(),
+               ch.shape[2], stg["W"][l].data_ptr(), stg["Wlo"][l].data_ptr(), stg["W"][l].shape[1],
+               stg["b"][l].data_ptr(), cfg.omega(l), Zh.data_ptr(), Zl.data_ptr(), Yh.data_ptr(), Yl.data_ptr(),
+               Yh.shape[2], s)
+        tape.pre.append(Zh)
+        tape.pre_lo.append(Zl)
+        tape.inputs.append(Yh)
+        tape.inputs_lo.append(Yl)
+        ch, cl, k = Yh, Yl, width
     return tape
 
 
@@ -387,10 +459,21 @@ def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0, split: bool =
     (gram(..., lo=...)), and hi alone is the bf16 operand of the reverse."""
     cfg = net.cfg
     prec = tape.prec
-    stg = net.staged(prec)
     Y = tape.inputs[-1]
     n, k, K = tape.n, cfg.widths[-1], cfg.modes
     cm, cs = (2, 1) if cfg.dim == 2 else (3, 9)
+    if tape.inputs_lo is not None:
+        # split-precision tape: output layer + lift on split operands
+        if not split:
+            raise ValueError("a split-precision tape writes split fields (split=True)")
+        sw = net.staged_split()
+        planes = [torch.empty((r * n, _ld(K)), dtype=torch.bfloat16, device=net.device)
+                  for r in (cm, cm, cs, cs)]
+        H.call("neb_jet_output_fields_x3", cfg.dim, n, k, K, Y.data_ptr(), tape.inputs_lo[-1].data_ptr(), Y.shape[2],
+               sw["W"][-1].data_ptr(), sw["Wlo"][-1].data_ptr(), sw["W"][-1].shape[1],
+               *[t.data_ptr() for t in planes], planes[0].shape[1], layout, _stream())
+        return Fields(planes[0], planes[2], K, layout, planes[1], planes[3])
+    stg = net.staged(prec)
     if split:
         if prec != H.BF16:
             raise ValueError("split fields need a bf16 tape")
@@ -444,8 +527,9 @@ def gram(phi: torch.Tensor, K: int, w: float, mode: str = "bf16x3", out: torch.T
 
 def split_fields_for(prec: str, gram_mode: str) -> bool:
     """Whether output_fields should write split planes: the bf16 tape with
-    the split-precision Gram (results are bit-identical to fp32 fields)."""
-    return prec == "bf16" and gram_mode == "bf16x3"
+    the split-precision Gram (results are bit-identical to fp32 fields), and
+    always the split-precision forward."""
+    return (prec == "bf16" and gram_mode == "bf16x3") or prec == "bf16x3"
 
 
 def field_gram(fields: Fields, which: str, w: float, mode: str = "bf16x3", out: torch.Tensor | None = None,

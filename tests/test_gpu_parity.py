@@ -831,3 +831,81 @@ def test_fp32_fields_bf16_reverse(dim):
         r = rel(a.detach().cpu(), b); assert r < 4e-2, r
     for a, b in zip(Gb[:-1], gb[:-1]):
         r = rel(a.detach().cpu(), b); assert r < 4e-2, r
+
+
+def test_split_bf16_planes():
+    rng = np.random.default_rng(17)
+    xs = torch.tensor(rng.normal(size=(37, 53)) * 10.0 ** rng.integers(-6, 6, size=(37, 53)), device=DEV)
+    for src in (xs.float(), xs):
+        hi = torch.empty((37, 56), dtype=torch.bfloat16, device=DEV)
+        lo = torch.empty_like(hi)
+        nw.H.call("neb_split_bf16", nw._DT[src.dtype], src.data_ptr(), 37, 53, 53, hi.data_ptr(), lo.data_ptr(), 56,
+                  nw._stream())
+        assert torch.equal(hi[:, :53], src.float().to(torch.bfloat16))
+        err = (hi[:, :53].double() + lo[:, :53].double() - src.double()).abs() / src.double().abs()
+        assert float(err.max()) <= 2.0 ** -16, float(err.max())
+
+
+def test_split_precision_forward_matches_oracle():
+    """Split-precision jets (bf16 hi / lo planes, three bf16 MFMA products per
+    term): fields and Grams at FP32-level accuracy against the FP64 oracle,
+    two orders below the bf16 path; the hi planes are the bf16 tape."""
+    F, K = 2, 24
+    for act, omegas in (("tanh", None), ("sin", (1.4, 1.2, 1.1))):
+        cfg, net, onet = make(2, K, [48, 40, 32], F, act=act, master=torch.float32, omegas=omegas)
+        x = points(2, 300, seed=5)
+        w = 1.0 / 300
+        out, _ = o.forward(onet, o.feature_jets(x, F, cfg.lo, cfg.hi))
+        u, om = o.lift2d(out)
+        U64, D64 = o.field_rows(u), om
+        tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS["bf16x3"])
+        assert tape.prec == nw.PRECISIONS["bf16"] and len(tape.inputs_lo) == 4
+        f = nw.output_fields(net, tape, split=True)
+        U = (f.mass[:, :K].double() + f.mass_lo[:, :K].double()).cpu().numpy()
+        D = (f.stiff[:, :K].double() + f.stiff_lo[:, :K].double()).cpu().numpy()
+        eu, ed = rel(U, U64), rel(D, D64)
+        assert eu < 2e-5 and ed < 2e-5, (act, eu, ed)
+        tb = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS["bf16"])
+        fb = nw.output_fields(net, tb, split=True)
+        Ub = (fb.mass[:, :K].double() + fb.mass_lo[:, :K].double()).cpu().numpy()
+        assert eu < rel(Ub, U64) / 100, (eu, rel(Ub, U64))
+        # the bf16 tape of the reverse: each hi plane is within half a bf16 ulp (2^-8 relative) of hi + lo
+        Y = tape.inputs[-1].float() + tape.inputs_lo[-1].float()
+        assert float(((tape.inputs[-1].float() - Y).abs() - 2.0 ** -8 * Y.abs()).max()) <= 0.0
+        M, S = nw.field_grams(f, w, "bf16x3")
+        r = rel(M.cpu(), o.gram(U64, w)); assert r < 1e-5, r
+        r = rel(S.cpu(), o.gram(D64, w)); assert r < 1e-5, r
+
+
+def test_split_precision_step():
+    """compute(prec="bf16x3"): split-precision forward, split Grams, BF16
+    reverse on the hi-plane tape: Grams within 1e-5 of the oracle, gradients
+    against the bf16 step on the same fields' cotangents."""
+    F, K = 2, 24
+    cfg, net, onet = make(2, K, [48, 40, 32], F, master=torch.float32)
+    x = points(2, 300, seed=7)
+    w = 1.0 / 300
+    L, M, S, *_ = _oracle_step(onet, x, F, cfg, w, 1e-2, 0.0)
+    res = nb.compute(net, torch.tensor(x), w, prec="bf16x3", gram_mode="bf16x3", gate="cholesky")
+    r = rel(res["M"].cpu(), M); assert r < 1e-5, r
+    r = rel(res["S"].cpu(), S); assert r < 1e-5, r
+    # tr(M^-1 S) moves by up to cond(M) times the Grams' relative error
+    cond = np.linalg.cond(M)
+    assert abs(float(res["loss"]) - L) < 1e-5 * cond * abs(L), (float(res["loss"]), L, cond)
+    assert torch.isfinite(res["grads"]).all()
+    rng = np.random.default_rng(21)
+    dM = rng.normal(size=(K, K)); dM = dM + dM.T
+    dS = rng.normal(size=(K, K)); dS = dS + dS.T
+    gW, gb = _oracle_reverse(onet, x, F, cfg, w, dM, dS)
+    tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS["bf16x3"])
+    fields = nw.output_fields(net, tape, split=True)
+    g = nw.backward_fields(net, tape, fields, torch.tensor(dM, device=DEV), torch.tensor(dS, device=DEV), w)
+    GW, Gb = net.split(g)
+    for a, b in zip(GW, gW):
+        r = rel(a.detach().cpu(), b); assert r < 4e-2, r
+
+
+def test_split_precision_forward_rejects_3d():
+    cfg, net, _ = make(3, 8, [32], 2, master=torch.float32)
+    with pytest.raises(ValueError, match="2D"):
+        nw.trunk_forward(net, torch.tensor(points(3, 64)), nw.PRECISIONS["bf16x3"])
