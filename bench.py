@@ -48,8 +48,12 @@ def parse():
                     help="skip the Gram-mode statistics and the FP64 accuracy legs (profiler runs)")
     ap.add_argument("--kernel", default=None,
                     choices=["hidden_fwd", "hidden_bwd", "out_fields", "out_bwd", "gram", "gram_f32in", "cross_dw",
-                             "copy"],
+                             "copy", "hidden_fwd_x3", "out_fields_x3"],
                     help="run one kernel of the config-B step --steps times (for counter profiling)")
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16"],
+                    help="config B headline step: bf16x3 = split-precision jets and fields (bf16 hi + lo planes, "
+                         "three bf16 MFMA products per term; Grams within 1e-5 of the FP64 path), bf16 = plain "
+                         "BF16 jets (Grams within ~5e-3); the other one is reported beside it")
     ap.add_argument("--config", default="B", choices=["B", "D", "E"],
                     help="B: 2D 256^2 grid K=1024 (default); D: 3D blob domain, 1,599,754 points, K=512, "
                          "forward + Grams + FP64 generalized eigendecomposition; E: the same points, K=1024, "
@@ -225,6 +229,10 @@ def bench_kernel(args):
     f32in = nw.output_fields(net, tape) if args.kernel == "gram_f32in" else None
     copy_src = torch.empty(fields.mass.shape, dtype=torch.float32, device=dev) if args.kernel == "copy" else None
     copy_dst = torch.empty_like(copy_src) if copy_src is not None else None
+    tx3 = nw.trunk_forward(net, x, nw.PRECISIONS["bf16x3"]) if args.kernel.endswith("_x3") else None
+    if tx3 is not None:
+        sx3 = net.staged_split()
+        Ol, O2, O2l = (torch.empty_like(Y) for _ in range(3))
 
     def run():
         k = args.kernel
@@ -248,6 +256,12 @@ def bench_kernel(args):
             nw.cross([(Y.view(C * N, 256), Z.view(C * N, 256), C * N, 1.0)], 256, 256, nw._CROSS[p])
         elif k == "copy":
             copy_dst.copy_(copy_src)
+        elif k == "hidden_fwd_x3":
+            H.call("neb_jet_layer_forward_x3", 2, H.TANH, N, 256, 256, tx3.inputs[2].data_ptr(),
+                   tx3.inputs_lo[2].data_ptr(), 256, sx3["W"][2].data_ptr(), sx3["Wlo"][2].data_ptr(), 256,
+                   sx3["b"][2].data_ptr(), 1.0, out.data_ptr(), Ol.data_ptr(), O2.data_ptr(), O2l.data_ptr(), 256, s)
+        elif k == "out_fields_x3":
+            nw.output_fields(net, tx3, split=True)
 
     run()
     torch.cuda.synchronize()
@@ -626,29 +640,31 @@ def main():
                      vs_fp32_step=(N * K / tm) / f32_rate if f32_rate else None,
                      accuracy="accuracy_vs_fp64['f32+bf16x3']")
 
-    # ---- split-precision jets and fields, split Grams, BF16 reverse ---------
-    x3 = None
+    # ---- the other BF16 step (split-precision or plain jets) ------------------
+    head = args.precision
+    other_prec = "bf16" if head == "bf16x3" else "bf16x3"
+    other = None
     if not args.no_f32:
-        stepx = make_step(nb, nw, sp, net, x, w, "bf16x3", "bf16x3", ridge)
-        stepx()
+        stepo = make_step(nb, nw, sp, net, x, w, other_prec, "bf16x3", ridge)
+        stepo()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        nx = max(1, min(5, args.steps))
-        for _ in range(nx):
-            stepx()
+        no = max(1, min(5, args.steps))
+        for _ in range(no):
+            stepo()
         torch.cuda.synchronize()
-        tx = (time.perf_counter() - t0) / nx
-        x3 = dict(point_modes_per_s=N * K / tx, ms_per_step=tx * 1e3,
-                  vs_fp32_step=(N * K / tx) / f32_rate if f32_rate else None,
-                  accuracy="accuracy_vs_fp64['bf16x3']")
+        to = (time.perf_counter() - t0) / no
+        other = dict(point_modes_per_s=N * K / to, ms_per_step=to * 1e3,
+                     vs_fp32_step=(N * K / to) / f32_rate if f32_rate else None,
+                     accuracy=f"accuracy_vs_fp64['{other_prec}']")
 
-    # ---- headline BF16 step --------------------------------------------------
+    # ---- headline step -----------------------------------------------------
     solver = sp.SpectralSolver(1e-2, ridge, deferred=True)
-    step = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, solver=solver)
+    step = make_step(nb, nw, sp, net, x, w, head, "bf16x3", ridge, solver=solver)
     for _ in range(args.warmup):
         step()
     timer = StageTimer(torch)
-    step_t = make_step(nb, nw, sp, net, x, w, "bf16", "bf16x3", ridge, timer, solver=solver)
+    step_t = make_step(nb, nw, sp, net, x, w, head, "bf16x3", ridge, timer, solver=solver)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -673,6 +689,12 @@ def main():
     out = {
         "metric": "point*modes/s, jet fwd+bwd + Gram spectral step, 2D K=1024",
         "value": N * K / dt,
+        "precision": (f"{head}: " + ("split-precision jets and fields (bf16 hi + lo planes, three bf16 MFMA "
+                                     "products per term, fp32 accumulation), split bf16x3 Grams, FP64 objective, "
+                                     "BF16 reverse; Grams within 1e-5 of the FP64 path (accuracy_vs_fp64['bf16x3'])"
+                                     if head == "bf16x3" else
+                                     "BF16 jets and fields, split bf16x3 Grams, FP64 objective, BF16 reverse "
+                                     "(accuracy_vs_fp64['bf16'])")),
         "unit": "point*modes/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -685,12 +707,12 @@ def main():
         "data": "synthetic: 256^2 periodic grid, Xavier-normal weights seed 0",
         "config": {"workload": "2D periodic operator grid: 65,536 pts, sin/cos freqs 1..8, 4x256 tanh, K=1024, "
                                "mass+stiffness Gram, spectral loss + reverse + Adam",
-                   "points": N, "modes": K, "widths": list(WIDTHS), "ridge": ridge},
+                   "points": N, "modes": K, "widths": list(WIDTHS), "ridge": ridge, "jets": head},
         "stages_ms": stages,
         "gram": gram_stats,
         "fp32_step_point_modes_per_s": f32_rate,
         "fp32_fields_bf16_reverse_step": mixed,
-        "split_precision_step": x3,
+        ("bf16_step" if head == "bf16x3" else "split_precision_step"): other,
         "accuracy_vs_fp64": accuracy,
         "loss": float(loss),
         "objective_path": solver.last_path,

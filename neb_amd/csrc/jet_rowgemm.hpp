@@ -533,7 +533,7 @@ struct PlanOutFields2Split {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
   using B = PlanOutFields2T<true>;
   static constexpr int CI = 8, NW = 2, NA = 3, NT = 12;
-  static constexpr int FB = B::FB, PB = B::PB, WPN = B::WPN, WPP = B::WPP;
+  static constexpr int FB = B::FB, PB = 1, WPN = B::WPN, WPP = B::WPP;
   // base terms of PlanOutFields2T (CH = 4): u += gy, v -= gx, omega -= hxx, omega -= hyy
   static constexpr int ta[NT] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 2, 2};
   static constexpr int tc[NT] = {1, 5, 1, 0, 4, 0, 2, 6, 2, 3, 7, 3};
