@@ -673,17 +673,35 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     stages = {k: float(np.mean(v)) for k, v in timer.ms().items()}
 
-    # roofline: dominant launch = the split-precision mass Gram (R = 2N rows,
-    # K = 1024): algorithmic R*K*(K+1) flops (the symmetric half, multiply +
-    # add) over its average duration, timed with events on its stream in the
-    # timed steps above (cross kernel + split reduction)
+    # The split-precision mass Gram (R = 2N rows, K = 1024): algorithmic
+    # R*K*(K+1) flops (the symmetric half, multiply + add; the three split
+    # products are not credited) over its average duration, timed with events
+    # on its stream in the timed steps above (cross kernel + split reduction).
     gram_ms = stages["gram_mass_launch"]
     gram_flops = (2 * N) * K * (K + 1)
     achieved = gram_flops / (gram_ms * 1e-3)
-    roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK["bf16"] / 1e12, unit="TFLOP/s",
-                    frac=achieved / PEAK["bf16"], traffic=profile_traffic("gram_split_k1024_pmc.md"),
-                    kernel="neb_gram_split mass Gram: cross_async_kernel<2, 2, 2, 2, false> + cross_reduce_kernel, "
-                           "R=131072, K=1024", launch_ms=gram_ms, flops_per_launch=gram_flops)
+    gram_roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK["bf16"] / 1e12, unit="TFLOP/s",
+                         frac=achieved / PEAK["bf16"], traffic=profile_traffic("gram_split_k1024_pmc.md"),
+                         kernel="neb_gram_split mass Gram: cross_async_kernel<2, 2, 2, 2, false> + "
+                                "cross_reduce_kernel, R=131072, K=1024", launch_ms=gram_ms,
+                         flops_per_launch=gram_flops)
+    if head == "bf16x3":
+        # dominant launch of the split-precision step: the output layer + lift
+        # on split operands (one launch, the output_fields stage).  Per point
+        # it reads 4 jet channels x 256 (hi + lo, 4 B) and writes 3 fields x K
+        # (hi + lo, 4 B): 16 KB; its useful 4 * 2 * 256 * K = 2.1 MFLOP per
+        # point (128 flop/B) put it below the bf16 ridge: HBM-bound.
+        fl_ms = stages["output_fields"]
+        fl_bytes = N * (4 * 256 * 4 + 3 * K * 4) + 2 * K * 256 * 2
+        fl_flops = N * 4 * 2 * 256 * K
+        bw = fl_bytes / (fl_ms * 1e-3)
+        roofline = dict(bound="hbm", achieved=bw / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=bw / HBM_PEAK,
+                        traffic=profile_traffic("out_fields_x3_k1024_pmc.md"),
+                        kernel="neb_jet_output_fields_x3: rowgemm_kernel<PlanOutFields2Split>, 65,536 points, "
+                               "width 256, K=1024", launch_ms=fl_ms, bytes_per_launch=fl_bytes,
+                        useful_tflops=fl_flops / (fl_ms * 1e-3) / 1e12)
+    else:
+        roofline = gram_roofline
 
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
@@ -717,6 +735,7 @@ def main():
         "loss": float(loss),
         "objective_path": solver.last_path,
         "roofline": roofline,
+        "gram_roofline": gram_roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
