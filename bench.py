@@ -232,7 +232,7 @@ def bench_kernel(args):
     tx3 = nw.trunk_forward(net, x, nw.PRECISIONS["bf16x3"]) if args.kernel.endswith("_x3") else None
     if tx3 is not None:
         sx3 = net.staged_split()
-        Ol, O2, O2l = (torch.empty_like(Y) for _ in range(3))
+        O2, O2l = torch.empty_like(Y), torch.empty_like(Y)
 
     def run():
         k = args.kernel
@@ -259,7 +259,7 @@ def bench_kernel(args):
         elif k == "hidden_fwd_x3":
             H.call("neb_jet_layer_forward_x3", 2, H.TANH, N, 256, 256, tx3.inputs[2].data_ptr(),
                    tx3.inputs_lo[2].data_ptr(), 256, sx3["W"][2].data_ptr(), sx3["Wlo"][2].data_ptr(), 256,
-                   sx3["b"][2].data_ptr(), 1.0, out.data_ptr(), Ol.data_ptr(), O2.data_ptr(), O2l.data_ptr(), 256, s)
+                   sx3["b"][2].data_ptr(), 1.0, out.data_ptr(), None, O2.data_ptr(), O2l.data_ptr(), 256, s)
         elif k == "out_fields_x3":
             nw.output_fields(net, tx3, split=True)
 

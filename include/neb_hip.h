@@ -123,7 +123,8 @@ int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const 
  * enter the split path through it (the `.astype(compute_dtype)` of
  * HB2P:1026-1030, VJ2:68, kept at ~16 bits).
  * neb_jet_layer_forward_x3: neb_jet_layer_forward on split input jets and
- * weights; writes z and y as hi / lo planes (HB2P:1005-1040, VJ2:20-34).
+ * weights; writes y as hi / lo planes and z as its hi plane, plus its lo
+ * plane when z_lo is not null (HB2P:1005-1040, VJ2:20-34).
  * neb_jet_output_fields_x3: neb_jet_output_fields_split on split last-hidden
  * jets and output weights (VCORE:71-77, ELC:35-38). */
 int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* hi, void* lo,

@@ -435,8 +435,9 @@ template <int D, int ACT> struct PlanHiddenFwd {
 // first.
 
 // Hidden layer forward on split operands (2D): Z = omega*(X W^T + b) and
-// Y = f-jet(Z) as PlanHiddenFwd, each stored as hi / lo bf16 planes
-// (out0 / out0lo = Z, out1 / out1lo = Y).
+// Y = f-jet(Z) as PlanHiddenFwd.  Y is stored as hi / lo bf16 planes (out1 /
+// out1lo: the next layer's split input); Z as its hi plane (out0; the bf16
+// tape of the reverse pass), plus its lo plane when out0lo is set.
 template <int ACT> struct PlanHiddenFwdSplit2 {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
   static constexpr int C = Jet2Layout<2>::C;
@@ -461,8 +462,13 @@ template <int ACT> struct PlanHiddenFwdSplit2 {
       for (int c = 1; c < C; ++c) u[c][e] = om * v[c][e];
     }
     const int64_t off = p * a.ldo + f;
+    if (a.out0lo) {
 #pragma unroll
-    for (int c = 0; c < C; ++c) st_field<true, GE, M>(a.out0, a.out0lo, off + c * a.cstride, u[c], nv);
+      for (int c = 0; c < C; ++c) st_field<true, GE, M>(a.out0, a.out0lo, off + c * a.cstride, u[c], nv);
+    } else {
+#pragma unroll
+      for (int c = 0; c < C; ++c) st_group<__bf16, GE>(reinterpret_cast<__bf16*>(a.out0) + off + c * a.cstride, u[c], nv);
+    }
     M f1[GE], f2[GE], t[GE];
 #pragma unroll
     for (int e = 0; e < GE; ++e) Act<ACT, M>::eval(u[0][e], t[e], f1[e], f2[e]);

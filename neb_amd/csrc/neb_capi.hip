@@ -885,7 +885,7 @@ int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t wid
   if (n < 0 || k < 1 || width < 1 || ldx < k || ldw < k || ldy < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_x3: bad sizes");
   if (n == 0) return NEB_OK;
-  if (!x_hi || !x_lo || !w_hi || !w_lo || !z_hi || !z_lo || !y_hi || !y_lo)
+  if (!x_hi || !x_lo || !w_hi || !w_lo || !z_hi || !y_hi || !y_lo)  // z_lo may be null
     return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3: null pointer");
   if (!aligned(x_hi, ldx, NEB_BF16) || !aligned(x_lo, ldx, NEB_BF16) || !aligned(w_hi, ldw, NEB_BF16) ||
       !aligned(w_lo, ldw, NEB_BF16) || !aligned(z_hi, ldy, NEB_BF16) || !aligned(z_lo, ldy, NEB_BF16) ||

@@ -307,8 +307,8 @@ class Tape:
     n: int
     inputs: list = field(default_factory=list)   # packed layer inputs [C, n, ld] (X0, Y1, ..., Y_L)
     pre: list = field(default_factory=list)      # packed scaled pre-activations Z_l
-    inputs_lo: list | None = None   # split-precision tapes: lo planes beside the hi planes above
-    pre_lo: list | None = None
+    inputs_lo: list | None = None   # split-precision tapes: lo planes of the layer inputs
+    pre_lo: list | None = None      # (and of the pre-activations, when kept)
 
 
 def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
@@ -351,7 +351,8 @@ def _trunk_forward_x3(net: EigenfluidNet, x: torch.Tensor) -> Tape:
     planes, then every hidden layer as neb_jet_layer_forward_x3 (three bf16
     MFMA products per term, Z and Y stored as hi / lo planes).  The returned
     tape is the bf16 tape of the hi planes, which the reverse pass reads, with
-    the lo planes in inputs_lo / pre_lo."""
+    the lo planes of the layer inputs in inputs_lo (the pre-activations keep
+    their hi plane only: the reverse reads nothing else)."""
     import ctypes
     cfg = net.cfg
     if cfg.dim != 2:
@@ -372,17 +373,16 @@ def _trunk_forward_x3(net: EigenfluidNet, x: torch.Tensor) -> Tape:
            s)
     del X0
     stg = net.staged_split()
-    tape = Tape(H.BF16, n, [Xh], [], [Xl], [])
+    tape = Tape(H.BF16, n, [Xh], [], [Xl], None)
     ch, cl, k = Xh, Xl, F
     for l, width in enumerate(cfg.widths):
         Zh = torch.empty((C, n, _ld(width)), dtype=torch.bfloat16, device=net.device)
-        Zl, Yh, Yl = torch.empty_like(Zh), torch.empty_like(Zh), torch.empty_like(Zh)
+        Yh, Yl = torch.empty_like(Zh), torch.empty_like(Zh)
         H.call("neb_jet_layer_forward_x3", cfg.dim, cfg.act_code, n, k, width, ch.data_ptr(), cl.data_ptr(),
                ch.shape[2], stg["W"][l].data_ptr(), stg["Wlo"][l].data_ptr(), stg["W"][l].shape[1],
-               stg["b"][l].data_ptr(), cfg.omega(l), Zh.data_ptr(), Zl.data_ptr(), Yh.data_ptr(), Yl.data_ptr(),
+               stg["b"][l].data_ptr(), cfg.omega(l), Zh.data_ptr(), None, Yh.data_ptr(), Yl.data_ptr(),
                Yh.shape[2], s)
         tape.pre.append(Zh)
-        tape.pre_lo.append(Zl)
         tape.inputs.append(Yh)
         tape.inputs_lo.append(Yl)
         ch, cl, k = Yh, Yl, width
