@@ -341,14 +341,50 @@ def bench_3d(args):
         step("f32", "f32", s32)
         torch.cuda.synchronize()
         f32_rate = N * K / (time.perf_counter() - t0)
+    # accuracy of one chunk's fields and Grams against the FP64 path
+    accuracy = {}
+    tape64 = nw.trunk_forward(net, x[:CHUNK3D], nw.PRECISIONS["f64"])
+    f64f = nw.output_fields(net, tape64)
+    del tape64
+    G64 = {wh: nw.field_gram(f64f, wh, w, "f64") for wh in ("mass", "stiff")}
+    for pr in ("bf16", "bf16x3"):
+        tp = nw.trunk_forward(net, x[:CHUNK3D], nw.PRECISIONS[pr])
+        fp = nw.output_fields(net, tp, split=True)
+        del tp
+        U = fp.mass[:, :K].float() + fp.mass_lo[:, :K].float()
+        acc = dict(velocity_rel_l2=float(torch.linalg.norm(U.double() - f64f.mass[:, :K])
+                                         / torch.linalg.norm(f64f.mass[:, :K])))
+        del U
+        for wh, key in (("mass", "mass_gram_rel_err"), ("stiff", "grad_u_gram_rel_err")):
+            G = nw.field_gram(fp, wh, w, "bf16x3")
+            acc[key] = float(torch.linalg.norm(G - G64[wh]) / torch.linalg.norm(G64[wh]))
+        accuracy[pr] = acc
+        del fp
+    del f64f, G64
+    accuracy["note"] = "first 131,072-point chunk, vs FP64 jets, fields and Grams of the same points and weights"
+
+    head = args.precision
+    other_prec = "bf16" if head == "bf16x3" else "bf16x3"
+    other = None
+    if not args.no_f32:
+        so = sp.SpectralSolver(1e-2, ridge, deferred=True)
+        step(other_prec, "bf16x3", so)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step(other_prec, "bf16x3", so)
+        torch.cuda.synchronize()
+        to = time.perf_counter() - t0
+        other = dict(point_modes_per_s=N * K / to, ms_per_step=to * 1e3,
+                     vs_fp32_step=(N * K / to) / f32_rate if f32_rate else None,
+                     accuracy=f"accuracy_vs_fp64['{other_prec}']")
     for _ in range(max(1, min(args.warmup, 2))):
-        step("bf16", "bf16x3", solver)
+        step(head, "bf16x3", solver)
     steps = max(1, min(args.steps, 3))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stages = {}
     for _ in range(steps):
-        res, tt = step("bf16", "bf16x3", solver)
+        res, tt = step(head, "bf16x3", solver)
         for k, v in tt.items():
             stages[k] = stages.get(k, 0.0) + v / steps * 1e3
     torch.cuda.synchronize()
@@ -390,7 +426,10 @@ def bench_3d(args):
                                "potential, curl lift, mass+grad-u Gram, 131,072-pt chunks, spectral loss + "
                                "reverse + Adam", "points": N, "modes": K, "chunk": CHUNK3D, "ridge": ridge,
                    "domain_volume_estimate": vol},
-        "stages_ms": stages, "gram_one_chunk": gram_stats, "fp32_step_point_modes_per_s": f32_rate,
+        "precision": head, "stages_ms": stages, "gram_one_chunk": gram_stats,
+        "fp32_step_point_modes_per_s": f32_rate,
+        ("bf16_step" if head == "bf16x3" else "split_precision_step"): other,
+        "accuracy_vs_fp64": accuracy,
         "loss": float(res["loss"]), "objective_path": solver.last_path, "projection": projection,
     }
     print(json.dumps(out), flush=True)

@@ -110,8 +110,7 @@ int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const 
                                 int64_t ldw, void* mass_hi, void* mass_lo, void* stiff_hi, void* stiff_lo,
                                 int64_t ldf, int layout, void* stream);
 
-/* Split-precision operands (2D jets only; dim 3 returns NEB_ERR_ARG).  A
- * value x is held as two bf16 planes hi = bf16(x), lo = bf16(x - hi); every
+/* Split-precision operands (2D and 3D jets).  A value x is held as two bf16 planes hi = bf16(x), lo = bf16(x - hi); every
  * product of the contraction is W_lo x_hi + W_hi x_lo + W_hi x_hi on bf16
  * MFMA with fp32 accumulation (about 16 significant bits per operand), and
  * the results are stored as hi / lo planes again.  The jets and fields then
@@ -126,7 +125,8 @@ int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const 
  * weights; writes y as hi / lo planes and z as its hi plane, plus its lo
  * plane when z_lo is not null (HB2P:1005-1040, VJ2:20-34).
  * neb_jet_output_fields_x3: neb_jet_output_fields_split on split last-hidden
- * jets and output weights (VCORE:71-77, ELC:35-38). */
+ * jets and output weights (VCORE:71-77, ELC:35-38; 3D: NCC:97-99,
+ * VJ2:93-101, the weight rows 3k+c as in neb_jet_output_fields). */
 int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* hi, void* lo,
                    int64_t ldd, void* stream);
 int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t width, const void* x_hi,

@@ -28,7 +28,7 @@
 #pragma once
 #include "neb_common.hpp"
 
-constexpr int RG_MAXCH = 12;
+constexpr int RG_MAXCH = 20;
 constexpr int RG_MAXW = 6;
 
 struct RowArgs {
@@ -424,7 +424,7 @@ template <int D, int ACT> struct PlanHiddenFwd {
   }
 };
 
-// Split-precision operands (2D): the input jets and the weights are each held
+// Split-precision operands: the input jets and the weights are each held
 // as two bf16 planes, x = x_hi + x_lo and W = W_hi + W_lo, and every product
 // is W_lo x_hi + W_hi x_lo + W_hi x_hi on bf16 MFMA with fp32 accumulation
 // (W_lo x_lo is below fp32 rounding), so the contraction carries ~16
@@ -434,23 +434,36 @@ template <int D, int ACT> struct PlanHiddenFwd {
 // three terms (a, c, W_lo), (a, c + CH, W_hi), (a, c, W_hi): small products
 // first.
 
-// Hidden layer forward on split operands (2D): Z = omega*(X W^T + b) and
+// Term tables of the split hidden layers: channel c into accumulator c.
+template <int D> struct HiddenSplitTerms;
+template <> struct HiddenSplitTerms<2> {
+  static constexpr int ta[18] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5};
+  static constexpr int tc[18] = {0, 6, 0, 1, 7, 1, 2, 8, 2, 3, 9, 3, 4, 10, 4, 5, 11, 5};
+  static constexpr int tw[18] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0};
+  static constexpr bool tneg[18] = {false};
+};
+template <> struct HiddenSplitTerms<3> {
+  static constexpr int ta[30] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 6, 6, 6, 7, 7, 7, 8, 8, 8,
+                                 9, 9, 9};
+  static constexpr int tc[30] = {0, 10, 0, 1, 11, 1, 2, 12, 2, 3, 13, 3, 4, 14, 4, 5, 15, 5, 6, 16, 6, 7, 17, 7,
+                                 8, 18, 8, 9, 19, 9};
+  static constexpr int tw[30] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0,
+                                 1, 0, 0};
+  static constexpr bool tneg[30] = {false};
+};
+
+// Hidden layer forward on split operands: Z = omega*(X W^T + b) and
 // Y = f-jet(Z) as PlanHiddenFwd.  Y is stored as hi / lo bf16 planes (out1 /
 // out1lo: the next layer's split input); Z as its hi plane (out0; the bf16
 // tape of the reverse pass), plus its lo plane when out0lo is set.
-template <int ACT> struct PlanHiddenFwdSplit2 {
+template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
-  static constexpr int C = Jet2Layout<2>::C;
+  static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = 2 * C, NW = 2, NA = C, NT = 3 * C;
-  static constexpr int FB = TileFor<2>::FB, PB = TileFor<2>::PB, WPN = TileFor<2>::WPN, WPP = TileFor<2>::WPP;
-  // base terms: channel c into accumulator c (CH = 6)
-  static constexpr int ta[NT] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5};
-  static constexpr int tc[NT] = {0, 6, 0, 1, 7, 1, 2, 8, 2, 3, 9, 3, 4, 10, 4, 5, 11, 5};
-  static constexpr int tw[NT] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0};
-  static constexpr bool tneg[NT] = {false};
+  static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
   template <int P, int GE, typename M>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
-    using L = Jet2Layout<2>;
+    using L = Jet2Layout<D>;
     const M om = (M)a.omega;
     const M* bias = reinterpret_cast<const M*>(a.bias);
     M u[C][GE];
@@ -474,7 +487,7 @@ template <int ACT> struct PlanHiddenFwdSplit2 {
     for (int e = 0; e < GE; ++e) Act<ACT, M>::eval(u[0][e], t[e], f1[e], f2[e]);
     st_field<true, GE, M>(a.out1, a.out1lo, off, t, nv);
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < D; ++c) {
 #pragma unroll
       for (int e = 0; e < GE; ++e) t[e] = f1[e] * u[1 + c][e];
       st_field<true, GE, M>(a.out1, a.out1lo, off + (1 + c) * a.cstride, t, nv);
@@ -483,8 +496,8 @@ template <int ACT> struct PlanHiddenFwdSplit2 {
     for (int q = 0; q < L::P; ++q) {
 #pragma unroll
       for (int e = 0; e < GE; ++e)
-        t[e] = f1[e] * u[3 + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
-      st_field<true, GE, M>(a.out1, a.out1lo, off + (3 + q) * a.cstride, t, nv);
+        t[e] = f1[e] * u[1 + D + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
+      st_field<true, GE, M>(a.out1, a.out1lo, off + (1 + D + q) * a.cstride, t, nv);
     }
   }
 };
@@ -600,6 +613,54 @@ template <bool SPLIT> struct PlanOutFields3T {
   }
 };
 using PlanOutFields3 = PlanOutFields3T<false>;
+
+// 3D output layer + curl lift on split operands: inputs x[0..8] = hi planes
+// of g0..g2, h0..h5, x[9..17] their lo planes; w[0..2] = W_hi slices, w[3..5]
+// = W_lo slices; each term of PlanOutFields3T expanded as the 2D split plan.
+struct PlanOutFields3Split {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
+  using B = PlanOutFields3T<true>;
+  static constexpr int CI = 18, NW = 6, NA = 12, NT = 72;
+  static constexpr int FB = B::FB, PB = B::PB, WPN = B::WPN, WPP = B::WPP;
+  static constexpr int ta[NT] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3,
+                                 4, 4, 4, 4, 4, 4, 5, 5, 5, 5, 5, 5, 6, 6, 6, 6, 6, 6, 7, 7, 7, 7, 7, 7,
+                                 8, 8, 8, 8, 8, 8, 9, 9, 9, 9, 9, 9, 10, 10, 10, 10, 10, 10, 11, 11, 11, 11, 11, 11};
+  static constexpr int tc[NT] = {1, 10, 1, 2, 11, 2, 2, 11, 2, 0, 9, 0, 0, 9, 0, 1, 10, 1, 4, 13, 4, 5, 14, 5,
+                                 6, 15, 6, 7, 16, 7, 7, 16, 7, 8, 17, 8, 5, 14, 5, 3, 12, 3, 7, 16, 7, 4, 13, 4,
+                                 8, 17, 8, 5, 14, 5, 3, 12, 3, 4, 13, 4, 4, 13, 4, 6, 15, 6, 5, 14, 5, 7, 16, 7};
+  static constexpr int tw[NT] = {5, 2, 2, 4, 1, 1, 3, 0, 0, 5, 2, 2, 4, 1, 1, 3, 0, 0, 5, 2, 2, 4, 1, 1,
+                                 5, 2, 2, 4, 1, 1, 5, 2, 2, 4, 1, 1, 3, 0, 0, 5, 2, 2, 3, 0, 0, 5, 2, 2,
+                                 3, 0, 0, 5, 2, 2, 4, 1, 1, 3, 0, 0, 4, 1, 1, 3, 0, 0, 4, 1, 1, 3, 0, 0};
+  static constexpr bool tneg[NT] = {false, false, false, true, true, true, false, false, false, true, true, true,
+                                    false, false, false, true, true, true, false, false, false, true, true, true,
+                                    false, false, false, true, true, true, false, false, false, true, true, true,
+                                    false, false, false, true, true, true, false, false, false, true, true, true,
+                                    false, false, false, true, true, true, false, false, false, true, true, true,
+                                    false, false, false, true, true, true, false, false, false, true, true, true};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    B::template epilogue<P, GE, M>(a, p, f, nv, v);
+  }
+};
+
+// Every split plan's terms are its base plan's, each expanded in the order
+// (W_lo x_hi), (W_hi x_lo), (W_hi x_hi): checked at compile time.
+template <class Split, int NB, int CH, int NWB, class Base>
+constexpr bool split_terms_match() {
+  for (int t = 0; t < NB; ++t)
+    for (int r = 0; r < 3; ++r) {
+      const int q = 3 * t + r;
+      if (Split::ta[q] != Base::ta[t] || Split::tneg[q] != Base::tneg[t]) return false;
+      if (Split::tc[q] != Base::tc[t] + (r == 1 ? CH : 0)) return false;
+      if (Split::tw[q] != Base::tw[t] + (r == 0 ? NWB : 0)) return false;
+    }
+  return true;
+}
+struct IdentityTerms10 { IDENTITY_TERMS };
+static_assert(split_terms_match<PlanOutFields3Split, 24, 9, 3, PlanOutFields3T<true>>(), "3D split field terms");
+static_assert(split_terms_match<PlanOutFields2Split, 4, 4, 1, PlanOutFields2T<true>>(), "2D split field terms");
+static_assert(split_terms_match<HiddenSplitTerms<2>, 6, 6, 1, IdentityTerms10>(), "2D split hidden terms");
+static_assert(split_terms_match<HiddenSplitTerms<3>, 10, 10, 1, IdentityTerms10>(), "3D split hidden terms");
 
 // Shared epilogue of the reverse plans: given the adjoint of the activation
 // output (adj[C][GE]), apply the transposed jet chain with the tape Z and

@@ -167,12 +167,13 @@ int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64
   return launch_rowgemm<P, Sx, PlanOutFields3T<SPLIT>>(a, st, what);
 }
 
-// split operands (2D): x_hi/x_lo, w_hi/w_lo planes (PlanHiddenFwdSplit2)
-template <int ACT>
+// split operands: x_hi/x_lo, w_hi/w_lo planes (PlanHiddenFwdSplit)
+template <int D, int ACT>
 int layer_forward_x3(int64_t n, int64_t k, int64_t width, const void* x_hi, const void* x_lo, int64_t ldx,
                      const void* w_hi, const void* w_lo, int64_t ldw, const void* bias, double omega, void* z_hi,
                      void* z_lo, void* y_hi, void* y_lo, int64_t ldy, hipStream_t st) {
-  constexpr int C = Jet2Layout<2>::C;
+  constexpr int C = Jet2Layout<D>::C;
+  static_assert(2 * C <= RG_MAXCH, "split input planes");
   RowArgs a = row_args();
   for (int c = 0; c < C; ++c) {
     a.x[c] = reinterpret_cast<const __bf16*>(x_hi) + (int64_t)c * n * ldx;
@@ -184,25 +185,41 @@ int layer_forward_x3(int64_t n, int64_t k, int64_t width, const void* x_hi, cons
   a.bias = bias; a.omega = omega;
   a.out0 = z_hi; a.out0lo = z_lo; a.out1 = y_hi; a.out1lo = y_lo;
   a.ldo = ldy; a.cstride = n * ldy;
-  return launch_rowgemm<PREC_BF16, __bf16, PlanHiddenFwdSplit2<ACT>>(a, st, "neb_jet_layer_forward_x3");
+  return launch_rowgemm<PREC_BF16, __bf16, PlanHiddenFwdSplit<D, ACT>>(a, st, "neb_jet_layer_forward_x3");
 }
 
-int output_fields_x3(int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo, int64_t ldy,
+int output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo, int64_t ldy,
                      const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi, void* mass_lo, void* stiff_hi,
                      void* stiff_lo, int64_t ldf, int layout, hipStream_t st) {
   const int64_t cs = n * ldy;
+  const __bf16* Yh = reinterpret_cast<const __bf16*>(y_hi);
+  const __bf16* Yl = reinterpret_cast<const __bf16*>(y_lo);
   RowArgs a = row_args();
   a.npts = n; a.nout = K; a.k = k;
   a.out0 = mass_hi; a.out0lo = mass_lo; a.out1 = stiff_hi; a.out1lo = stiff_lo;
   a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
-  const int chans[4] = {1, 2, 3, 5};  // gx, gy, hxx, hyy
-  for (int c = 0; c < 4; ++c) {
-    a.x[c] = reinterpret_cast<const __bf16*>(y_hi) + chans[c] * cs;
-    a.x[4 + c] = reinterpret_cast<const __bf16*>(y_lo) + chans[c] * cs;
-    a.ldx[c] = a.ldx[4 + c] = ldy;
+  if (dim == 2) {
+    const int chans[4] = {1, 2, 3, 5};  // gx, gy, hxx, hyy
+    for (int c = 0; c < 4; ++c) {
+      a.x[c] = Yh + chans[c] * cs;
+      a.x[4 + c] = Yl + chans[c] * cs;
+      a.ldx[c] = a.ldx[4 + c] = ldy;
+    }
+    a.w[0] = w_hi; a.w[1] = w_lo; a.ldw[0] = a.ldw[1] = ldw;
+    return launch_rowgemm<PREC_BF16, __bf16, PlanOutFields2Split>(a, st, "neb_jet_output_fields_x3");
   }
-  a.w[0] = w_hi; a.w[1] = w_lo; a.ldw[0] = a.ldw[1] = ldw;
-  return launch_rowgemm<PREC_BF16, __bf16, PlanOutFields2Split>(a, st, "neb_jet_output_fields_x3");
+  // 3D: channels g0..g2, h0..h5 (1..9); weight slices W_c = rows 3k+c
+  for (int c = 0; c < 9; ++c) {
+    a.x[c] = Yh + (1 + c) * cs;
+    a.x[9 + c] = Yl + (1 + c) * cs;
+    a.ldx[c] = a.ldx[9 + c] = ldy;
+  }
+  for (int sl = 0; sl < 3; ++sl) {
+    a.w[sl] = reinterpret_cast<const __bf16*>(w_hi) + sl * ldw;
+    a.w[3 + sl] = reinterpret_cast<const __bf16*>(w_lo) + sl * ldw;
+    a.ldw[sl] = a.ldw[3 + sl] = 3 * ldw;
+  }
+  return launch_rowgemm<PREC_BF16, __bf16, PlanOutFields3Split>(a, st, "neb_jet_output_fields_x3");
 }
 
 template <int P, int ACT, typename F>
@@ -881,7 +898,6 @@ int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t wid
                              const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,
                              int64_t ldy, void* stream) {
   if (int rc = check_common(NEB_BF16, dim, act)) return rc;
-  if (dim != 2) return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3: split operands are implemented for 2D jets");
   if (n < 0 || k < 1 || width < 1 || ldx < k || ldw < k || ldy < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_x3: bad sizes");
   if (n == 0) return NEB_OK;
@@ -892,18 +908,17 @@ int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t wid
       !aligned(y_hi, ldy, NEB_BF16) || !aligned(y_lo, ldy, NEB_BF16))
     return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3: rows must be 16-byte aligned");
   hipStream_t st = S(stream);
-  if (act == NEB_TANH)
-    return layer_forward_x3<ACT_TANH>(n, k, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi,
-                                      y_lo, ldy, st);
-  return layer_forward_x3<ACT_SIN>(n, k, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi,
-                                   y_lo, ldy, st);
+#define LFX3(D, A) \
+  layer_forward_x3<D, A>(n, k, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi, y_lo, ldy, st)
+  if (dim == 2) return act == NEB_TANH ? LFX3(2, ACT_TANH) : LFX3(2, ACT_SIN);
+  return act == NEB_TANH ? LFX3(3, ACT_TANH) : LFX3(3, ACT_SIN);
+#undef LFX3
 }
 
 int neb_jet_output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo,
                              int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
                              void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, int layout, void* stream) {
   if (int rc = check_common(NEB_BF16, dim, NEB_TANH)) return rc;
-  if (dim != 2) return fail(NEB_ERR_ARG, "neb_jet_output_fields_x3: split operands are implemented for 2D jets");
   if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_fields_x3: bad sizes");
   if (n == 0) return NEB_OK;
@@ -913,7 +928,7 @@ int neb_jet_output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const voi
       !aligned(w_lo, ldw, NEB_BF16) || !aligned(mass_hi, ldf, NEB_BF16) || !aligned(mass_lo, ldf, NEB_BF16) ||
       !aligned(stiff_hi, ldf, NEB_BF16) || !aligned(stiff_lo, ldf, NEB_BF16))
     return fail(NEB_ERR_ARG, "neb_jet_output_fields_x3: rows must be 16-byte aligned");
-  return output_fields_x3(n, k, K, y_hi, y_lo, ldy, w_hi, w_lo, ldw, mass_hi, mass_lo, stiff_hi, stiff_lo, ldf,
+  return output_fields_x3(dim, n, k, K, y_hi, y_lo, ldy, w_hi, w_lo, ldw, mass_hi, mass_lo, stiff_hi, stiff_lo, ldf,
                           layout, S(stream));
 }
 

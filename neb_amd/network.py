@@ -28,7 +28,7 @@ import torch
 from . import _hip as H
 
 # BF16X3: host-side code of the split-precision forward (bf16 hi / lo planes
-# of every jet and weight, three bf16 MFMA products per term; 2D); its tape is
+# of every jet and weight, three bf16 MFMA products per term); its tape is
 # a bf16 tape (the hi planes) with the lo planes beside it
 BF16X3 = 3
 PRECISIONS = {"bf16": H.BF16, "f32": H.F32, "f64": H.F64, "bf16x3": BF16X3}
@@ -347,7 +347,7 @@ def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
 
 
 def _trunk_forward_x3(net: EigenfluidNet, x: torch.Tensor) -> Tape:
-    """Split-precision trunk (2D): fp32 feature jets split into bf16 hi / lo
+    """Split-precision trunk: fp32 feature jets split into bf16 hi / lo
     planes, then every hidden layer as neb_jet_layer_forward_x3 (three bf16
     MFMA products per term, Z and Y stored as hi / lo planes).  The returned
     tape is the bf16 tape of the hi planes, which the reverse pass reads, with
@@ -355,8 +355,6 @@ def _trunk_forward_x3(net: EigenfluidNet, x: torch.Tensor) -> Tape:
     their hi plane only: the reverse reads nothing else)."""
     import ctypes
     cfg = net.cfg
-    if cfg.dim != 2:
-        raise ValueError("the split-precision forward is implemented for 2D jets")
     x = x.to(device=net.device, dtype=torch.float32).contiguous()
     n = x.shape[0]
     C = channels(cfg.dim)

@@ -846,18 +846,24 @@ def test_split_bf16_planes():
         assert float(err.max()) <= 2.0 ** -16, float(err.max())
 
 
-def test_split_precision_forward_matches_oracle():
+@pytest.mark.parametrize("dim", [2, 3])
+def test_split_precision_forward_matches_oracle(dim):
     """Split-precision jets (bf16 hi / lo planes, three bf16 MFMA products per
     term): fields and Grams at FP32-level accuracy against the FP64 oracle,
     two orders below the bf16 path; the hi planes are the bf16 tape."""
-    F, K = 2, 24
+    F = 2
+    K = 24 if dim == 2 else 8
     for act, omegas in (("tanh", None), ("sin", (1.4, 1.2, 1.1))):
-        cfg, net, onet = make(2, K, [48, 40, 32], F, act=act, master=torch.float32, omegas=omegas)
-        x = points(2, 300, seed=5)
+        cfg, net, onet = make(dim, K, [48, 40, 32], F, act=act, master=torch.float32, omegas=omegas)
+        x = points(dim, 300, seed=5)
         w = 1.0 / 300
         out, _ = o.forward(onet, o.feature_jets(x, F, cfg.lo, cfg.hi))
-        u, om = o.lift2d(out)
-        U64, D64 = o.field_rows(u), om
+        if dim == 2:
+            u, om = o.lift2d(out)
+            U64, D64 = o.field_rows(u), om
+        else:
+            u, du = o.lift3d(out)
+            U64, D64 = o.field_rows(u), o.field_rows(du)
         tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS["bf16x3"])
         assert tape.prec == nw.PRECISIONS["bf16"] and len(tape.inputs_lo) == 4
         f = nw.output_fields(net, tape, split=True)
@@ -877,13 +883,15 @@ def test_split_precision_forward_matches_oracle():
         r = rel(S.cpu(), o.gram(D64, w)); assert r < 1e-5, r
 
 
-def test_split_precision_step():
+@pytest.mark.parametrize("dim", [2, 3])
+def test_split_precision_step(dim):
     """compute(prec="bf16x3"): split-precision forward, split Grams, BF16
     reverse on the hi-plane tape: Grams within 1e-5 of the oracle, gradients
     against the bf16 step on the same fields' cotangents."""
-    F, K = 2, 24
-    cfg, net, onet = make(2, K, [48, 40, 32], F, master=torch.float32)
-    x = points(2, 300, seed=7)
+    F = 2
+    K = 24 if dim == 2 else 8
+    cfg, net, onet = make(dim, K, [48, 40, 32], F, master=torch.float32)
+    x = points(dim, 300, seed=7)
     w = 1.0 / 300
     L, M, S, *_ = _oracle_step(onet, x, F, cfg, w, 1e-2, 0.0)
     res = nb.compute(net, torch.tensor(x), w, prec="bf16x3", gram_mode="bf16x3", gate="cholesky")
@@ -905,7 +913,3 @@ def test_split_precision_step():
         r = rel(a.detach().cpu(), b); assert r < 4e-2, r
 
 
-def test_split_precision_forward_rejects_3d():
-    cfg, net, _ = make(3, 8, [32], 2, master=torch.float32)
-    with pytest.raises(ValueError, match="2D"):
-        nw.trunk_forward(net, torch.tensor(points(3, 64)), nw.PRECISIONS["bf16x3"])
