@@ -27,6 +27,7 @@
 // evaluate_large_comparison.py.
 #pragma once
 #include "neb_common.hpp"
+#include <type_traits>
 
 constexpr int RG_MAXCH = 20;
 constexpr int RG_MAXW = 6;
@@ -189,9 +190,16 @@ constexpr size_t rowgemm_stage_elems() {
   return (size_t)LDK * (Plan::NW * TN + Plan::CI * TP);
 }
 
+// Plans that declare SINGLE_BUFFER keep one LDS stage (two workgroups per
+// CU instead of one double-buffered workgroup).
+template <class Plan, class = void> struct PlanSingleBuffer : std::false_type {};
+template <class Plan>
+struct PlanSingleBuffer<Plan, std::void_t<decltype(Plan::SINGLE_BUFFER)>> : std::bool_constant<Plan::SINGLE_BUFFER> {};
+
 template <int P, class Plan>
 constexpr bool rowgemm_double_buffered() {
-  return 2 * rowgemm_stage_elems<P, Plan>() * sizeof(typename Prec<P>::store_t) <= 96 * 1024;
+  return !PlanSingleBuffer<Plan>::value &&
+         2 * rowgemm_stage_elems<P, Plan>() * sizeof(typename Prec<P>::store_t) <= 96 * 1024;
 }
 
 template <int P, class Plan>
@@ -461,6 +469,9 @@ template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = 2 * C, NW = 2, NA = C, NT = 3 * C;
   static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
+  // 3D: a 48 KB stage (20 input planes); double-buffered it allows one
+  // workgroup per CU
+  static constexpr bool SINGLE_BUFFER = D == 3;
   template <int P, int GE, typename M>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     using L = Jet2Layout<D>;
