@@ -558,6 +558,59 @@ def gram_mode_stats(nw, fields, split, K, w):
     return gram_stats
 
 
+def fd_check_fp64(nb, sp, cfg, dev, directions=64, hs=(1e-2, 1e-3, 1e-4), seed=0):
+    """The K=1024 differentiation check at full size, on the GPU in FP64: the
+    handwritten gradient of L = tr((M + ridge I)^-1 S) + 1e-2 ||M - I||^2 over
+    65,536 uniform random points against central differences (h = 1e-3)
+    along `directions` random unit parameter directions (the reference's
+    directional check, tests/test_network_backends.py:74-90), and at h = 1e-2
+    and 1e-4 beside it: a central difference errs by O(h^2) (truncation) plus
+    (rounding noise of the loss) / h, so the error's trend with h says which
+    one limits it.  The ridge is 1e-4 * mean diag(M) at the base point (M has
+    rank <= 256 < K, and M + ridge I a large condition number)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    n = 65536
+    x = torch.tensor(rng.uniform(0.0, 2 * math.pi, (n, 2)), dtype=torch.float64, device=dev)
+    w = 4 * math.pi ** 2 / n
+    net = nb.EigenfluidNet(cfg, seed=0, device=dev, master_dtype=torch.float64)
+    t0 = time.perf_counter()
+    M0 = sp.compute(net, x, w, prec="f64", gram_mode="f64", training=False, gate="cholesky", ridge=1.0)["M"]
+    ridge = 1e-4 * float(torch.diagonal(M0).mean())
+    del M0
+    res = sp.compute(net, x, w, prec="f64", gram_mode="f64", ridge=ridge, gate="cholesky")
+    g = res["grads"].clone()
+    L = float(res["loss"])
+    del res
+    base = net.params.detach().clone()
+    errs = {h: [] for h in hs}
+    for _ in range(directions):
+        d = torch.tensor(rng.normal(size=base.numel()), dtype=torch.float64, device=dev)
+        d /= d.norm()
+        exact = float(g @ d)
+        for h in hs:
+            vals = []
+            for sgn in (1.0, -1.0):
+                with torch.no_grad():
+                    net.params.copy_(base + sgn * h * d)
+                net.touch()
+                vals.append(float(sp.compute(net, x, w, prec="f64", gram_mode="f64", ridge=ridge, training=False,
+                                             gate="cholesky")["loss"]))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            errs[h].append(abs(fd - exact) / max(abs(exact), 1e-3 * abs(L)))
+    with torch.no_grad():
+        net.params.copy_(base)
+    net.touch()
+    per_h = {}
+    for h in hs:
+        e = np.asarray(errs[h])
+        per_h[f"h={h:g}"] = dict(max_rel_err=float(e.max()), median_rel_err=float(np.median(e)),
+                                 within_1e_4=int((e <= 1e-4).sum()))
+    return dict(points=n, directions=directions, ridge=ridge, loss=L, **per_h, seconds=time.perf_counter() - t0,
+                note="|FD - <grad, d>| / max(|<grad, d>|, 1e-3 |L|), FP64 jets, fields, Grams and reverse")
+
+
 def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
     """End-to-end errors of the bf16 and fp32 paths against the FP64 path
     (FP64 jets, fields and Grams) on the same grid and weights."""
@@ -642,10 +695,11 @@ def main():
     # ---- Gram modes on the same fields: TFLOP/s and error vs FP64 ----------
     # "bf16x3-split": the step's path (pre-split bf16 planes); "bf16x3" splits
     # the fp32 fields per tile; both give the same bits.
-    gram_stats, accuracy = {}, {}
+    gram_stats, accuracy, fd_check = {}, {}, None
     if not args.step_only:
         gram_stats = gram_mode_stats(nw, fields, split, K, w)
         accuracy = accuracy_vs_fp64(nw, sp, net, x, K, w, ridge)
+        fd_check = fd_check_fp64(nb, sp, cfg, dev)
     del tape, fields, split
 
     # ---- FP32 comparison step ------------------------------------------------
@@ -771,6 +825,7 @@ def main():
         "fp32_fields_bf16_reverse_step": mixed,
         ("bf16_step" if head == "bf16x3" else "split_precision_step"): other,
         "accuracy_vs_fp64": accuracy,
+        "fd_check_fp64": fd_check,
         "loss": float(loss),
         "objective_path": solver.last_path,
         "roofline": roofline,
