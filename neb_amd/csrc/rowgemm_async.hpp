@@ -59,6 +59,11 @@ template <int D> struct WideHiddenShape {
   static constexpr int FB = D == 2 ? 4 : 2, PB = 1, WPN = 4, WPP = 2;
 };
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenFwd<D, ACT>> : WideHiddenShape<D> {};
+// 2D output layer on split operands: 8 waves on a 128 x 64 tile, three 48 KB
+// slots (four waves of the plan's own 128 x 32 tile would leave two)
+template <> struct RowAsyncShape<PlanOutFields2Split> {
+  static constexpr int FB = 4, PB = 1, WPN = 2, WPP = 4;
+};
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenBwd<D, ACT>> : WideHiddenShape<D> {};
 
 template <class Plan>
