@@ -790,7 +790,7 @@ def main():
         bw = fl_bytes / (fl_ms * 1e-3)
         roofline = dict(bound="hbm", achieved=bw / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=bw / HBM_PEAK,
                         traffic=profile_traffic("out_fields_x3_k1024_pmc.md"),
-                        kernel="neb_jet_output_fields_x3: rowgemm_kernel<PlanOutFields2Split>, 65,536 points, "
+                        kernel="neb_jet_output_fields_x3: rowgemm_async_kernel<PlanOutFields2Split>, 65,536 points, "
                                "width 256, K=1024", launch_ms=fl_ms, bytes_per_launch=fl_bytes,
                         useful_tflops=fl_flops / (fl_ms * 1e-3) / 1e12)
     else:
