@@ -472,7 +472,10 @@ template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
   // 3D: a 48 KB stage (20 input planes); double-buffered it allows one
   // workgroup per CU
   static constexpr bool SINGLE_BUFFER = D == 3;
-  template <int P, int GE, typename M>
+  // the value channel takes the bias (a.bias, fp32 [nout] or null); a caller
+  // that has added it to v[0] already passes BIAS_ADDED
+  static constexpr bool BIAS_CH0 = true;
+  template <int P, int GE, typename M, bool BIAS_ADDED = false>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     using L = Jet2Layout<D>;
     const M om = (M)a.omega;
@@ -480,7 +483,7 @@ template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
     M u[C][GE];
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
-      const M b = (bias && e < nv) ? bias[f + e] : (M)0;
+      const M b = (!BIAS_ADDED && bias && e < nv) ? bias[f + e] : (M)0;
       u[0][e] = om * (v[0][e] + b);
 #pragma unroll
       for (int c = 1; c < C; ++c) u[c][e] = om * v[c][e];

@@ -59,6 +59,11 @@ template <int D> struct WideHiddenShape {
   static constexpr int FB = D == 2 ? 4 : 2, PB = 1, WPN = 4, WPP = 2;
 };
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenFwd<D, ACT>> : WideHiddenShape<D> {};
+// 2D hidden layers on split operands: 8 waves on a 128 x 32 tile (32
+// features x 16 points each), three 40 KB slots
+template <int ACT> struct RowAsyncShape<PlanHiddenFwdSplit<2, ACT>> {
+  static constexpr int FB = 2, PB = 1, WPN = 4, WPP = 2;
+};
 // 2D output layer on split operands: 8 waves on a 128 x 64 tile, three 48 KB
 // slots (four waves of the plan's own 128 x 32 tile would leave two)
 template <> struct RowAsyncShape<PlanOutFields2Split> {
