@@ -456,13 +456,15 @@ def bench_3d_eig(args):
         return sp.chunked_step(net, x, w, prec, gmode, CHUNK3D, solver=lambda M, S: (0.0, None, None),
                                training=False)
 
+    head = args.precision
+    other_prec = "bf16" if head == "bf16x3" else "bf16x3"
     for _ in range(max(1, min(args.warmup, 2))):
-        warm = grams("bf16", "bf16x3")
+        warm = grams(head, "bf16x3")
         sp.generalized_eigh(warm["S"], warm["M"])  # first-call library set-up stays out of the timing
     del warm
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = grams("bf16", "bf16x3")
+    res = grams(head, "bf16x3")
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -496,6 +498,18 @@ def bench_3d_eig(args):
             peel["fp64_path_gate"] = f"failed: {e}"
     gram_err = float(torch.linalg.norm(res["M"] - ref["M"]) / torch.linalg.norm(ref["M"]))
     eig_err = float(((vals - v64).abs() / v64.abs()).max())
+    # the other jet precision on the same points: time and accuracy
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    oth = grams(other_prec, "bf16x3")
+    torch.cuda.synchronize()
+    t_oth = time.perf_counter() - t0
+    v_oth, _ = sp.generalized_eigh(oth["S"], oth["M"])
+    other = dict(forward_fields_grams_ms=t_oth * 1e3,
+                 mass_gram_rel_err_vs_fp64_path=float(torch.linalg.norm(oth["M"] - ref["M"])
+                                                      / torch.linalg.norm(ref["M"])),
+                 eigenvalue_max_rel_err_vs_fp64_path=float(((v_oth - v64).abs() / v64.abs()).max()))
+    del oth
     del ref
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -516,6 +530,8 @@ def bench_3d_eig(args):
         "stages_ms": {"forward_fields_grams": t_fwd * 1e3, "generalized_eigh": t_eig * 1e3},
         "gram_useful_tflops": (R_m + R_s) * K * (K + 1) / t_fwd / 1e12,
         "fp32_forward_grams_ms": t_f32 * 1e3,
+        "precision": head,
+        ("bf16_jets" if head == "bf16x3" else "split_precision_jets"): other,
         "mass_gram_rel_err_vs_fp64_path": gram_err,
         "eigenvalue_max_rel_err_vs_fp64_path": eig_err,
         "eigenvalues_min_max": [float(vals[0]), float(vals[-1])],
