@@ -64,10 +64,11 @@ template <int D, int ACT> struct RowAsyncShape<PlanHiddenFwd<D, ACT>> : WideHidd
 template <int ACT> struct RowAsyncShape<PlanHiddenFwdSplit<2, ACT>> {
   static constexpr int FB = 2, PB = 1, WPN = 4, WPP = 2;
 };
-// 2D output layer on split operands: 8 waves on a 128 x 64 tile, three 48 KB
-// slots (four waves of the plan's own 128 x 32 tile would leave two)
+// 2D output layer on split operands: 8 waves on a 256 x 32 tile (64 features
+// x 16 points each), three 48 KB slots (four waves of the plan's own 128 x 32
+// tile would leave two); each point block is read by 4 feature tiles, not 8
 template <> struct RowAsyncShape<PlanOutFields2Split> {
-  static constexpr int FB = 4, PB = 1, WPN = 2, WPP = 4;
+  static constexpr int FB = 4, PB = 1, WPN = 4, WPP = 2;
 };
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenBwd<D, ACT>> : WideHiddenShape<D> {};
 
