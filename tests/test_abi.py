@@ -62,3 +62,32 @@ def test_cross_occupancy_knob_arguments():
     assert lib.neb_set_cross_occupancy(-1) == 4  # NEB_ERR_ARG
     for v in (1, 128, 100000, 0):
         assert lib.neb_set_cross_occupancy(v) == 0
+
+
+def test_split_precision_entry_points_check_arguments_without_the_gpu():
+    H.load()
+    # unknown dimension, null lo plane with points to process, bad leading dimension
+    with pytest.raises(ValueError):
+        H.call("neb_jet_layer_forward_x3", 4, 0, 8, 8, 8, 16, 16, 8, 16, 16, 8, 0, 1.0, 16, None, 16, 16, 8, None)
+    with pytest.raises(ValueError, match="null"):
+        H.call("neb_jet_layer_forward_x3", 2, 0, 8, 8, 8, 16, None, 8, 16, 16, 8, 0, 1.0, 16, None, 16, 16, 8, None)
+    with pytest.raises(ValueError):
+        H.call("neb_jet_output_fields_x3", 3, 8, 8, 16, 16, 16, 8, 16, 16, 8, 16, 16, 16, 16, 8, 0, None)
+    with pytest.raises(ValueError):
+        H.call("neb_split_bf16", H.BF16, 16, 4, 4, 4, 16, 16, 4, None)
+    with pytest.raises(ValueError, match="null"):
+        H.call("neb_split_bf16", H.F32, 16, 4, 4, 4, None, 16, 4, None)
+    # zero points: sizes are checked, nothing is launched, null buffers are accepted
+    H.call("neb_jet_layer_forward_x3", 2, 0, 0, 8, 8, None, None, 8, None, None, 8, None, 1.0, None, None, None, None,
+           8, None)
+    H.call("neb_split_bf16", H.F32, None, 0, 4, 4, None, None, 4, None)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No CPU or PyTorch fallback: without libnebhip.so every entry point raises."""
+    import subprocess
+    import sys
+    code = ("import neb_amd._hip as H; H.LIB_PATH = '/nonexistent/libnebhip.so'; H._lib = None\n"
+            "try:\n    H.load()\nexcept RuntimeError as e:\n    print('raised', e)\n")
+    p = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert p.returncode == 0 and "raised" in p.stdout and "not built" in p.stdout, p.stdout + p.stderr
