@@ -938,3 +938,18 @@ def test_autograd_function_split_precision():
     fields = nw.output_fields(net, tape, split=True)
     ref = nw.backward_fields(net, tape, fields, gM, gS, w)
     r = rel(g.cpu(), ref.cpu()); assert r < 1e-12, r
+
+
+def test_chunked_split_precision_step_3d():
+    """Two-pass chunked step on split-precision jets (3D) against the single
+    pass: the same kernels, Grams accumulated over chunks (their FP64 sums in
+    another order)."""
+    F, K = 2, 8
+    cfg, net, _ = make(3, K, [32, 32], F, master=torch.float32)
+    x = torch.tensor(points(3, 1000, seed=8))
+    w = 1.0 / 1000
+    ref = nb.compute(net, x, w, prec="bf16x3", gram_mode="bf16x3", gate="cholesky")
+    res = nb.chunked_step(net, x, w, prec="bf16x3", gram_mode="bf16x3", chunk=256)
+    r = rel(res["M"].cpu(), ref["M"].cpu()); assert r < 1e-6, r
+    r = rel(res["S"].cpu(), ref["S"].cpu()); assert r < 1e-6, r
+    r = rel(res["grads"].cpu(), ref["grads"].cpu()); assert r < 1e-2, r
