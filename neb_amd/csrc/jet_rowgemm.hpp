@@ -190,16 +190,9 @@ constexpr size_t rowgemm_stage_elems() {
   return (size_t)LDK * (Plan::NW * TN + Plan::CI * TP);
 }
 
-// Plans that declare SINGLE_BUFFER keep one LDS stage (two workgroups per
-// CU instead of one double-buffered workgroup).
-template <class Plan, class = void> struct PlanSingleBuffer : std::false_type {};
-template <class Plan>
-struct PlanSingleBuffer<Plan, std::void_t<decltype(Plan::SINGLE_BUFFER)>> : std::bool_constant<Plan::SINGLE_BUFFER> {};
-
 template <int P, class Plan>
 constexpr bool rowgemm_double_buffered() {
-  return !PlanSingleBuffer<Plan>::value &&
-         2 * rowgemm_stage_elems<P, Plan>() * sizeof(typename Prec<P>::store_t) <= 96 * 1024;
+  return 2 * rowgemm_stage_elems<P, Plan>() * sizeof(typename Prec<P>::store_t) <= 96 * 1024;
 }
 
 template <int P, class Plan>
@@ -464,14 +457,17 @@ template <> struct HiddenSplitTerms<3> {
 // Y = f-jet(Z) as PlanHiddenFwd.  Y is stored as hi / lo bf16 planes (out1 /
 // out1lo: the next layer's split input); Z as its hi plane (out0; the bf16
 // tape of the reverse pass), plus its lo plane when out0lo is set.
+// 3D split hidden tile: 4 waves across 128 features x 16 points (each wave
+// 32 x 16), a 36 KB stage that double-buffers at two workgroups per CU
+template <int D> struct SplitHiddenTile : TileFor<D> {};
+template <> struct SplitHiddenTile<3> { static constexpr int FB = 2, PB = 1, WPN = 4, WPP = 1; };
+
 template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = 2 * C, NW = 2, NA = C, NT = 3 * C;
-  static constexpr int FB = TileFor<D>::FB, PB = TileFor<D>::PB, WPN = TileFor<D>::WPN, WPP = TileFor<D>::WPP;
-  // 3D: a 48 KB stage (20 input planes); double-buffered it allows one
-  // workgroup per CU
-  static constexpr bool SINGLE_BUFFER = D == 3;
+  using TL = SplitHiddenTile<D>;
+  static constexpr int FB = TL::FB, PB = TL::PB, WPN = TL::WPN, WPP = TL::WPP;
   // the value channel takes the bias (a.bias, fp32 [nout] or null); a caller
   // that has added it to v[0] already passes BIAS_ADDED
   static constexpr bool BIAS_CH0 = true;
