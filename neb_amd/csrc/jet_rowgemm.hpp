@@ -127,7 +127,9 @@ struct TileLoad {
 };
 
 // Group stores / loads of GE consecutive features (16-byte vectors).
-template <typename T, int GE, typename M>
+// NT: non-temporal stores (streamed past the caches: outputs no kernel of
+// the launch reads back, so they do not evict inputs other tiles reuse)
+template <typename T, int GE, typename M, bool NT = false>
 __device__ inline void st_group(T* dst, const M* v, int nvalid) {
   constexpr int PV = (int)(16 / sizeof(T)) < GE ? (int)(16 / sizeof(T)) : GE;
   typedef T pv __attribute__((ext_vector_type(PV)));
@@ -137,7 +139,10 @@ __device__ inline void st_group(T* dst, const M* v, int nvalid) {
       pv t;
 #pragma unroll
       for (int e = 0; e < PV; ++e) t[e] = cvt<T>(v[c + e]);
-      *reinterpret_cast<pv*>(dst + c) = t;
+      if constexpr (NT)
+        __builtin_nontemporal_store(t, reinterpret_cast<pv*>(dst + c));
+      else
+        *reinterpret_cast<pv*>(dst + c) = t;
     }
   } else {
 #pragma unroll
@@ -149,7 +154,7 @@ __device__ inline void st_group(T* dst, const M* v, int nvalid) {
 // Field row store at element offset `off`: the math type as is, or (SPLIT)
 // hi = bf16(x) into `out` and lo = bf16(x - hi) into `out_lo`.  x - hi is
 // exact in fp32, so hi + lo carries 16 significant bits of x.
-template <bool SPLIT, int GE, typename M>
+template <bool SPLIT, int GE, typename M, bool NT = false>
 __device__ inline void st_field(void* out, void* out_lo, int64_t off, const M* v, int nv) {
   if constexpr (SPLIT) {
     float h[GE], l[GE];
@@ -158,10 +163,10 @@ __device__ inline void st_field(void* out, void* out_lo, int64_t off, const M* v
       h[e] = (float)(__bf16)(float)v[e];
       l[e] = (float)v[e] - h[e];
     }
-    st_group<__bf16, GE>(reinterpret_cast<__bf16*>(out) + off, h, nv);
-    st_group<__bf16, GE>(reinterpret_cast<__bf16*>(out_lo) + off, l, nv);
+    st_group<__bf16, GE, float, NT>(reinterpret_cast<__bf16*>(out) + off, h, nv);
+    st_group<__bf16, GE, float, NT>(reinterpret_cast<__bf16*>(out_lo) + off, l, nv);
   } else {
-    st_group<M, GE>(reinterpret_cast<M*>(out) + off, v, nv);
+    st_group<M, GE, M, NT>(reinterpret_cast<M*>(out) + off, v, nv);
   }
 }
 
@@ -534,7 +539,7 @@ template <int D> struct PlanOutJets {
 // 2D output layer with the velocity / vorticity lift (ELC:35-38):
 //   u = psi_y, v = -psi_x  (mass fields),  omega = -(psi_xx + psi_yy).
 // Inputs: x[0]=gx, x[1]=gy, x[2]=hxx, x[3]=hyy of the last hidden jet.
-template <bool SPLIT> struct PlanOutFields2T {
+template <bool SPLIT, bool STREAM = false> struct PlanOutFields2T {
   template <int P> __host__ __device__ static constexpr int out_bytes() {
     return SPLIT ? 2 : sizeof(typename Prec<P>::math_t);
   }
@@ -548,9 +553,9 @@ template <bool SPLIT> struct PlanOutFields2T {
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
     const int64_t r0 = a.layout == 0 ? 2 * p : p;
     const int64_t r1 = a.layout == 0 ? 2 * p + 1 : a.npts + p;
-    st_field<SPLIT, GE, M>(a.out0, a.out0lo, r0 * a.ldo + f, v[0], nv);
-    st_field<SPLIT, GE, M>(a.out0, a.out0lo, r1 * a.ldo + f, v[1], nv);
-    st_field<SPLIT, GE, M>(a.out1, a.out1lo, p * a.ldo2 + f, v[2], nv);
+    st_field<SPLIT, GE, M, STREAM>(a.out0, a.out0lo, r0 * a.ldo + f, v[0], nv);
+    st_field<SPLIT, GE, M, STREAM>(a.out0, a.out0lo, r1 * a.ldo + f, v[1], nv);
+    st_field<SPLIT, GE, M, STREAM>(a.out1, a.out1lo, p * a.ldo2 + f, v[2], nv);
   }
 };
 using PlanOutFields2 = PlanOutFields2T<false>;
@@ -560,7 +565,7 @@ using PlanOutFields2 = PlanOutFields2T<false>;
 // w[1] = W_lo; the fields are stored as hi / lo bf16 planes.
 struct PlanOutFields2Split {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
-  using B = PlanOutFields2T<true>;
+  using B = PlanOutFields2T<true, true>;
   static constexpr int CI = 8, NW = 2, NA = 3, NT = 12;
   static constexpr int FB = B::FB, PB = 1, WPN = B::WPN, WPP = B::WPP;
   // base terms of PlanOutFields2T (CH = 4): u += gy, v -= gx, omega -= hxx, omega -= hyy
