@@ -157,7 +157,7 @@ int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64
     const int chans[4] = {1, 2, 3, 5};
     for (int c = 0; c < 4; ++c) { a.x[c] = Y + chans[c] * cs; a.ldx[c] = ldy; }
     a.w[0] = w; a.ldw[0] = ldw;
-    return launch_rowgemm<P, Sx, PlanOutFields2T<SPLIT>>(a, st, what);
+    return launch_rowgemm<P, Sx, PlanOutFields2T<SPLIT, SPLIT>>(a, st, what);
   }
   for (int c = 0; c < 9; ++c) { a.x[c] = Y + (1 + c) * cs; a.ldx[c] = ldy; }
   for (int s = 0; s < 3; ++s) {
