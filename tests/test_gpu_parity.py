@@ -784,10 +784,10 @@ def test_empty_point_set(dim):
     F, K = 2, 8
     cfg, net, _ = make(dim, K, [32, 32], F, master=torch.float32)
     x = torch.zeros((0, dim))
-    for prec, gmode in (("f32", "f32"), ("bf16", "bf16x3")):
+    for prec, gmode in (("f32", "f32"), ("bf16", "bf16x3"), ("bf16x3", "bf16x3")):
         p = nw.PRECISIONS[prec]
         tape = nw.trunk_forward(net, x, p)
-        fields = nw.output_fields(net, tape, split=prec == "bf16")
+        fields = nw.output_fields(net, tape, split=prec != "f32")
         assert fields.mass.shape[0] == 0 and fields.stiff.shape[0] == 0
         M, S = nw.field_grams(fields, 0.5, gmode)
         assert float(M.abs().max()) == 0.0 and float(S.abs().max()) == 0.0
