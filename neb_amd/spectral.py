@@ -44,6 +44,29 @@ def _spd_inverse_async(M: torch.Tensor, out: torch.Tensor | None = None):
     return X, info
 
 
+def _refined_inverse(Mr: torch.Tensor, out: torch.Tensor, steps: int):
+    """Enqueue the sweep inverse of M_r followed by `steps` Newton-Schulz
+    steps X <- X + X (I - M_r X) (two FP64 GEMMs each), written into `out`;
+    returns info of the sweep (see _spd_inverse_async).
+
+    The sweep inverse's forward error grows with cond(M_r) faster than a
+    backward-stable solve's: at cond(M_r) = 1.3e6 (the K = 1024 objective with
+    ridge 1e-4 mean diag M) it moves tr(X S) by 2e-9 relative, noise that a
+    central difference of the loss divides by h.  One step squares the
+    residual I - M_r X and leaves tr(X S) within 1e-12 of a Cholesky solve."""
+    if steps <= 0:
+        return _spd_inverse_async(Mr, out=out)[1]
+    X, info = _spd_inverse_async(Mr)
+    eye = _eye(Mr.shape[0], Mr.device)
+    for s in range(steps):
+        R = torch.addmm(eye, Mr, X, alpha=-1.0)
+        if s == steps - 1:
+            out.copy_(torch.addmm(X, X, R))
+        else:
+            X = torch.addmm(X, X, R)
+    return info
+
+
 def _raise_not_pd(bad: int):
     raise FloatingPointError(f"mass matrix is not positive definite (pivot {bad}); no update")
 
@@ -81,7 +104,7 @@ def _raise_gate(value: int):
 
 
 def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridge: float = 0.0,
-                       cond_limit: float = 1e12, gate: str = "eig"):
+                       cond_limit: float = 1e12, gate: str = "eig", refine: int = 1):
     """L = tr((M + ridge I)^-1 S) + lam ||M - I||_F^2 and its closed-form
     adjoints dL/dM = -(M_r^-1 S M_r^-1) + 2 lam (M - I), dL/dS = M_r^-1.
 
@@ -90,8 +113,9 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
     anything is updated.  gate="eig" checks eigenvalues (the reference's
     rule); gate="cholesky" only requires positive definiteness (every pivot
     of the FP64 inversion positive, as a Cholesky factorization requires) and
-    a finite result.  Returns (loss, dM, dS) with FP64 tensors on M's device
-    (loss 0-dim)."""
+    a finite result.  The inverse is the sweep inverse refined by `refine`
+    Newton-Schulz steps (_refined_inverse).  Returns (loss, dM, dS) with FP64
+    tensors on M's device (loss 0-dim)."""
     Mr = nw.sym_shift(M, ridge)
     if gate == "eig":
         e = torch.linalg.eigvalsh(Mr)
@@ -100,7 +124,8 @@ def spectral_objective(M: torch.Tensor, S: torch.Tensor, lam: float = 1e-2, ridg
             raise FloatingPointError("Physical mass zero/rank deficient/condition exceeds limit; no update")
     K = M.shape[0]
     pair = nw.adjoint_pair(K, M.device)      # [dL/dM | dL/dS]
-    X, info = _spd_inverse_async(Mr, out=pair[:, K:])
+    X = pair[:, K:]
+    info = _refined_inverse(Mr, X, refine)
     loss, X, Ss, g = nw.spectral_terms(M, S, X, info, lam, ridge, math.inf)
     _raise_gate(int(g.item()))
     dM = nw.mass_adjoint(X, Ss, M, lam, out=pair[:, :K])
@@ -111,7 +136,8 @@ class SpectralSolver:
     """Stateful FP64 evaluator of the spectral objective for a training loop.
 
     Step 1 (or whenever the check below fails) inverts M_r = M + ridge I
-    directly (FP64, positive-definiteness gate), as spectral_objective does.  Later steps start from the
+    directly (FP64 sweeps, positive-definiteness gate, then `refine`
+    Newton-Schulz steps), as spectral_objective does.  Later steps start from the
     previous step's inverse X and refine it with Newton-Schulz iterations
     X <- X (2I - M_r X) (two FP64 GEMMs each, quadratic convergence), which
     is valid while the residual ||I - M_r X||_F < 1; the iteration stops at a
@@ -130,8 +156,9 @@ class SpectralSolver:
     raises, or calls check() before using the results."""
 
     def __init__(self, lam: float = 1e-2, ridge: float = 0.0, cond_limit: float = 1e12, tol: float = 1e-10,
-                 max_iter: int = 6, backoff: int = 16, deferred: bool = False):
+                 max_iter: int = 6, backoff: int = 16, deferred: bool = False, refine: int = 1):
         self.lam, self.ridge, self.cond_limit, self.tol, self.max_iter = lam, ridge, cond_limit, tol, max_iter
+        self.refine = refine
         self.backoff = backoff
         self.deferred = deferred
         self.X = None
@@ -178,7 +205,8 @@ class SpectralSolver:
         info = None
         pair = nw.adjoint_pair(K, M.device)  # [dL/dM | dL/dS] side by side for the reverse
         if X is None:
-            X, info = _spd_inverse_async(Mr, out=pair[:, K:])
+            X = pair[:, K:]
+            info = _refined_inverse(Mr, X, self.refine)
             self.last_path = "spd-inverse"
         else:
             X = pair[:, K:].copy_(X)
@@ -336,7 +364,8 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     fused reverse.  objective="trace" is tr((M + ridge I)^-1 S) +
     lam ||M - I||^2 (MB/velocity_trace_jet2.py:107-117), objective="peel" the
     sorted Cholesky coefficient peel (core.py:48-69; its result dict is
-    returned as "peel").  reverse_prec (default: prec) runs the reverse pass
+    returned as "peel"; dM, dS are the field-Gram cotangents the reverse
+    applies).  reverse_prec (default: prec) runs the reverse pass
     in another operand precision on the same fields and a converted tape:
     prec="f32", gram_mode="bf16x3", reverse_prec="bf16" keeps the fields and
     Grams at FP32 accuracy and runs the reverse on BF16 MFMA.  Returns
@@ -362,7 +391,7 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
             loss, dM, dS = peel["loss"], peel["pair"][:, :K], peel["pair"][:, K:]
         else:
             loss, dM, dS = spectral_objective(M, S, lam, ridge, gate=gate)
-    out = dict(loss=loss, M=M, S=S, fields=fields, timings=tt)
+    out = dict(loss=loss, M=M, S=S, dM=dM, dS=dS, fields=fields, timings=tt)
     if objective == "peel":
         out["peel"] = peel
     if training:

@@ -120,7 +120,18 @@ def linear_jet(x: Jet, W: np.ndarray, b: np.ndarray | None) -> Jet:
     v = x.value @ W.T
     if b is not None:
         v = v + b
-    return Jet(v, np.einsum("nid,oi->nod", x.grad, W), np.einsum("nip,oi->nop", x.hess, W))
+    return Jet(v, _chan_matmul(x.grad, W.T), _chan_matmul(x.hess, W.T))
+
+
+def _chan_matmul(t: np.ndarray, A: np.ndarray) -> np.ndarray:
+    """t [n, i, c] contracted over i with A [i, o] -> [n, o, c] (each tangent
+    channel through the same matrix)."""
+    return np.swapaxes(np.swapaxes(t, 1, 2) @ A, 1, 2)
+
+
+def _packed_outer(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """sum over channels and points of a[c, n, o] b[c, n, i] -> [o, i]."""
+    return a.reshape(-1, a.shape[-1]).T @ b.reshape(-1, b.shape[-1])
 
 
 def activation_jet(kind: str, pre: Jet) -> Jet:
@@ -202,19 +213,19 @@ def backward(net: Net, tape, adj_out: Jet):
     gW = [None] * len(net.W)
     gb = [None] * len(net.W)
     a = adj_out
-    gW[-1] = np.einsum("cno,cni->oi", a.pack(), last.pack())
+    gW[-1] = _packed_outer(a.pack(), last.pack())
     gb[-1] = a.value.sum(0)
     W = net.W[-1]
-    a = Jet(a.value @ W, np.einsum("nod,oi->nid", a.grad, W), np.einsum("nop,oi->nip", a.hess, W))
+    a = Jet(a.value @ W, _chan_matmul(a.grad, W), _chan_matmul(a.hess, W))
     for l in range(net.hidden_layers - 1, -1, -1):
         src, pre = layers[l]
         a = activation_jet_reverse(net.activation, pre, a)
         om = net.omegas[l]
         a = Jet(om * a.value, om * a.grad, om * a.hess)
-        gW[l] = np.einsum("cno,cni->oi", a.pack(), src.pack())
+        gW[l] = _packed_outer(a.pack(), src.pack())
         gb[l] = a.value.sum(0)
         W = net.W[l]
-        a = Jet(a.value @ W, np.einsum("nod,oi->nid", a.grad, W), np.einsum("nop,oi->nip", a.hess, W))
+        a = Jet(a.value @ W, _chan_matmul(a.grad, W), _chan_matmul(a.hess, W))
     return gW, gb
 
 
