@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 6
+#define NEB_ABI_VERSION 7
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -194,6 +194,22 @@ int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int
  * `grams` on the split-precision path. */
 int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w, double beta,
                    double* G, int64_t ldg, void* workspace, size_t ws_bytes, void* stream);
+
+/* neb_gram_split without a workspace argument: the same kernels and the same
+ * result, bit for bit, with the split-R partial tiles held in a file-scope
+ * device array of NEB_NOALLOC_SCRATCH_BYTES (32 MiB) that each GPU the
+ * library runs on keeps until the process ends.  The call allocates nothing
+ * (no hipMalloc, hipFree or allocating library routine); its kernels write
+ * every partial tile before reading it, so nothing has to start at zero.
+ * Limit: K up to 3968 (31 column tiles of 128) for any row count; larger K is
+ * rejected with NEB_ERR_ARG.  Every call on one GPU shares that GPU's array:
+ * calls that overlap in time on one GPU (other streams, other threads) race
+ * and must be ordered by the caller.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+#define NEB_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int neb_gram_split_noalloc(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w,
+                           double beta, double* G, int64_t ldg, void* stream);
 
 /* Caps the workgroups that each of this thread's later bf16 cross launches
  * (neb_cross on bf16 operands, neb_gram_split) keeps resident: 0 = as many

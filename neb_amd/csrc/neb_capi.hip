@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -473,6 +474,70 @@ int stage_dst(int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_
   return fail(NEB_ERR_ARG, "neb_stage: unknown dst dtype %d", dst_dtype);
 }
 
+// Split Gram on caller-provided partial-tile memory (the workspace, or the
+// library's own scratch for neb_gram_split_noalloc); `cap` bytes available,
+// `elem` bytes per partial value counted against it.
+int gram_split_launch(const char* what, const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld,
+                      double w, double beta, double* G, int64_t ldg, void* partial, size_t cap, size_t elem,
+                      void* stream) {
+  CrossArgs a;
+  memset(&a, 0, sizeof(a));
+  a.seg[0].x = phi_hi; a.seg[0].y = phi_hi;
+  a.seg[0].xlo = phi_lo; a.seg[0].ylo = phi_lo;
+  a.seg[0].ldx = ld; a.seg[0].ldy = ld;
+  a.seg[0].rows = rows; a.seg[0].sign = 1.0;
+  a.nseg = 1; a.m = K; a.n = K; a.roww = nullptr; a.symmetric = 1;
+  a.ntiles = cross_tiles(K, K, 1, &a.tiles_m, &a.tiles_n);
+  a.splits = cross_splits(a.ntiles, rows);
+  const size_t need = (size_t)a.splits * a.ntiles * CR_TM * CR_TN * elem;
+  if (cap < need) return fail(NEB_ERR_ARG, "%s: %zu bytes of partial tiles needed, %zu available", what, need, cap);
+  a.partial = reinterpret_cast<double*>(partial);
+  return launch_cross_async<2>(a, w, beta, G, ldg, S(stream));
+}
+
+int gram_split_check(const char* what, const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld,
+                     const double* G, int64_t ldg) {
+  if (rows < 0 || K < 1 || ld < K || ld >= (1 << 25) || ldg < K)
+    return fail(NEB_ERR_SHAPE, "%s: bad sizes (ld must be below 2^25)", what);
+  if ((rows > 0 && (!phi_hi || !phi_lo)) || !G) return fail(NEB_ERR_ARG, "%s: null pointer", what);
+  if (!aligned(phi_hi, ld, NEB_BF16) || !aligned(phi_lo, ld, NEB_BF16))
+    return fail(NEB_ERR_ARG, "%s: rows must be 16-byte aligned", what);
+  return NEB_OK;
+}
+
+}  // namespace
+
+// Scratch of neb_gram_split_noalloc: the fp32 split-R partial tiles.  Each
+// GPU the library runs on holds its own copy from the module's load until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on the
+// host the name is a stand-in, not a device address): it uses the per-device
+// address from hipGetSymbolAddress below.
+static __device__ __align__(256) unsigned char scratch[NEB_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(NEB_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= 64) return fail(NEB_ERR_HIP, "device index %d outside 0..63", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(NEB_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NEB_OK;
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -690,25 +755,26 @@ int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int
 
 int neb_gram_split(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w, double beta,
                    double* G, int64_t ldg, void* workspace, size_t ws_bytes, void* stream) {
-  if (rows < 0 || K < 1 || ld < K || ld >= (1 << 25) || ldg < K)
-    return fail(NEB_ERR_SHAPE, "neb_gram_split: bad sizes (ld must be below 2^25)");
-  if ((rows > 0 && (!phi_hi || !phi_lo)) || !G || !workspace)
-    return fail(NEB_ERR_ARG, "neb_gram_split: null pointer");
-  if (!aligned(phi_hi, ld, NEB_BF16) || !aligned(phi_lo, ld, NEB_BF16))
-    return fail(NEB_ERR_ARG, "neb_gram_split: rows must be 16-byte aligned");
-  CrossArgs a;
-  memset(&a, 0, sizeof(a));
-  a.seg[0].x = phi_hi; a.seg[0].y = phi_hi;
-  a.seg[0].xlo = phi_lo; a.seg[0].ylo = phi_lo;
-  a.seg[0].ldx = ld; a.seg[0].ldy = ld;
-  a.seg[0].rows = rows; a.seg[0].sign = 1.0;
-  a.nseg = 1; a.m = K; a.n = K; a.roww = nullptr; a.symmetric = 1;
-  a.ntiles = cross_tiles(K, K, 1, &a.tiles_m, &a.tiles_n);
-  a.splits = cross_splits(a.ntiles, rows);
-  const size_t need = (size_t)a.splits * a.ntiles * CR_TM * CR_TN * sizeof(double);
-  if (ws_bytes < need) return fail(NEB_ERR_ARG, "neb_gram_split: workspace %zu < %zu bytes", ws_bytes, need);
-  a.partial = reinterpret_cast<double*>(workspace);
-  return launch_cross_async<2>(a, w, beta, G, ldg, S(stream));
+  if (int rc = gram_split_check("neb_gram_split", phi_hi, phi_lo, rows, K, ld, G, ldg)) return rc;
+  if (!workspace) return fail(NEB_ERR_ARG, "neb_gram_split: null workspace");
+  // the workspace is sized for FP64 partials (neb_cross_workspace_size)
+  return gram_split_launch("neb_gram_split", phi_hi, phi_lo, rows, K, ld, w, beta, G, ldg, workspace, ws_bytes,
+                           sizeof(double), stream);
+}
+
+int neb_gram_split_noalloc(const void* phi_hi, const void* phi_lo, int64_t rows, int64_t K, int64_t ld, double w,
+                           double beta, double* G, int64_t ldg, void* stream) {
+  if (int rc = gram_split_check("neb_gram_split_noalloc", phi_hi, phi_lo, rows, K, ld, G, ldg)) return rc;
+  int tm, tn;
+  const int nt = cross_tiles(K, K, 1, &tm, &tn);
+  const size_t need = (size_t)cross_splits(nt, rows) * nt * CR_TM * CR_TN * sizeof(float);
+  if (need > NEB_NOALLOC_SCRATCH_BYTES)
+    return fail(NEB_ERR_ARG, "neb_gram_split_noalloc: K=%lld needs %zu bytes of scratch, above the %zu-byte maximum",
+                (long long)K, need, (size_t)NEB_NOALLOC_SCRATCH_BYTES);
+  void* ptr = nullptr;
+  if (int rc = scratch_address(&ptr)) return rc;
+  return gram_split_launch("neb_gram_split_noalloc", phi_hi, phi_lo, rows, K, ld, w, beta, G, ldg, ptr,
+                           NEB_NOALLOC_SCRATCH_BYTES, sizeof(float), stream);
 }
 
 size_t neb_colsum_workspace_size(int64_t rows, int64_t cols) {
