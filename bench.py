@@ -44,11 +44,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host oracle baseline")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 comparison step")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D K=1024 (config E) leg of the default run")
     ap.add_argument("--step-only", action="store_true",
                     help="skip the Gram-mode statistics and the FP64 accuracy legs (profiler runs)")
     ap.add_argument("--kernel", default=None,
                     choices=["hidden_fwd", "hidden_bwd", "out_fields", "out_bwd", "gram", "gram_f32in", "cross_dw",
-                             "copy", "hidden_fwd_x3", "out_fields_x3"],
+                             "copy", "hidden_fwd_x3", "out_fields_x3", "products"],
                     help="run one kernel of the config-B step --steps times (for counter profiling)")
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16"],
                     help="config B headline step: bf16x3 = split-precision jets and fields (bf16 hi + lo planes, "
@@ -81,6 +82,111 @@ class StageTimer:
 
     def ms(self):
         return {k: [a.elapsed_time(b) for a, b in v] for k, v in self.events.items()}
+
+
+# committed rocprofv3 counter summaries (FETCH_SIZE / WRITE_SIZE per launch)
+# of the kernel classes of the 2D K=1024 step
+PROFILE_OF = {"neb_jet_layer_forward_x3 k=256": "hidden_fwd_x3_k1024_pmc.md",
+              "neb_jet_output_fields_x3": "out_fields_x3_k1024_pmc.md",
+              "neb_gram_split rows=131072": "gram_split_k1024_pmc.md",
+              "neb_jet_output_backward": "out_bwd_k1024_pmc.md",
+              "neb_jet_layer_backward k=256": "hidden_bwd_k1024_pmc.md"}
+RIDGE_FLOP_PER_BYTE = 2.5e15 / 8.0e12   # bf16 dense MFMA peak / HBM peak
+_ES = {0: 4, 1: 2, 2: 8}                 # NEB_F32, NEB_BF16, NEB_F64 element bytes
+
+
+def launch_work(name, a):
+    """(kernel class, algorithmic HBM bytes, useful flops) of one libnebhip
+    launch from its arguments (the C ABI order of include/neb_hip.h), or
+    (class, None, None) for launches without a roofline figure.  Bytes count
+    each operand read and each result written once; flops count the useful
+    products (the split-precision forms issue three per product)."""
+    if name == "neb_jet_layer_forward_x3":
+        dim, n, k, width = a[0], a[2], a[3], a[4]
+        C = 6 if dim == 2 else 10
+        return f"{name} k={k}", n * C * (k * 4 + width * 6), 2 * C * n * k * width
+    if name == "neb_jet_layer_forward":
+        prec, dim, n, k, width = a[0], a[1], a[3], a[4], a[5]
+        C, es = (6 if dim == 2 else 10), _ES[prec]
+        return f"{name} k={k}", n * C * (k + 2 * width) * es, 2 * C * n * k * width
+    if name in ("neb_jet_output_fields_x3", "neb_jet_output_fields_split", "neb_jet_output_fields"):
+        if name == "neb_jet_output_fields":
+            prec, dim, n, k, K = a[0], a[1], a[2], a[3], a[4]
+            es_in, es_out = _ES[prec], (8 if prec == 2 else 4)
+        else:
+            dim, n, k, K = a[0], a[1], a[2], a[3]
+            es_in, es_out = (4 if name.endswith("_x3") else 2), 4
+        nch, rows, terms = (4, 3, 4) if dim == 2 else (9, 12, 24)
+        return name, n * (nch * k * es_in + rows * K * es_out), 2 * terms * n * k * K
+    if name == "neb_gram_split":
+        rows, K = a[2], a[3]
+        return f"{name} rows={rows}", rows * K * 4, rows * K * (K + 1)
+    if name == "neb_gram":
+        rows, K, dt = a[2], a[3], a[5]
+        return f"{name} rows={rows}", rows * K * _ES[dt], rows * K * (K + 1)
+    if name == "neb_jet_output_backward":
+        prec, dim, n, width, K, fdt = a[0], a[1], a[3], a[4], a[5], a[9]
+        C, es = (6 if dim == 2 else 10), _ES[prec]
+        rows, terms = (3, 4) if dim == 2 else (12, 24)
+        return name, n * (rows * K * _ES[fdt] + 2 * C * width * es), 2 * terms * n * width * K
+    if name == "neb_jet_layer_backward":
+        prec, dim, n, k, width = a[0], a[1], a[3], a[4], a[5]
+        C, es = (6 if dim == 2 else 10), _ES[prec]
+        return f"{name} k={k}", n * C * (width + 2 * k) * es, 2 * C * n * k * width
+    if name == "neb_cross":
+        nseg, xd, yd, m, ncol, sym = a[1], a[8], a[9], a[10], a[11], a[13]
+        rows = sum(a[6][i] for i in range(nseg))
+        flops = rows * m * (m + 1) if sym else 2 * rows * m * ncol
+        return f"{name} {m}x{ncol}", rows * (m * _ES[xd] + ncol * _ES[yd]), flops
+    return name, None, None
+
+
+class LaunchTimer:
+    """Launch recorder for neb_amd._hip.set_launch_recorder: HIP events on
+    the launching (current) stream around every libnebhip call, with the
+    call's kernel class and algorithmic work."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.recs = []
+
+    def __call__(self, name, args):
+        e0 = self.torch.cuda.Event(enable_timing=True)
+        e0.record()
+
+        def done():
+            e1 = self.torch.cuda.Event(enable_timing=True)
+            e1.record()
+            self.recs.append((launch_work(name, args), e0, e1))
+        return done
+
+    def table(self, steps, step_ms):
+        """Per kernel class: launches and ms per step, share of the step,
+        per-launch time and roofline (bound by the class's arithmetic
+        intensity against the bf16 ridge)."""
+        out = {}
+        for (cls, nbytes, flops), e0, e1 in self.recs:
+            r = out.setdefault(cls, dict(launches=0, ms=0.0, bytes=nbytes, flops=flops))
+            r["launches"] += 1
+            r["ms"] += e0.elapsed_time(e1)
+        rows = []
+        for cls, r in out.items():
+            per = r["ms"] / r["launches"]
+            row = dict(kernel=cls, launches_per_step=r["launches"] / steps, ms_per_step=r["ms"] / steps,
+                       share_of_step=r["ms"] / steps / step_ms, ms_per_launch=per)
+            if r["bytes"]:
+                intensity = r["flops"] / r["bytes"]
+                if intensity < RIDGE_FLOP_PER_BYTE:
+                    bw = r["bytes"] / (per * 1e-3)
+                    row.update(bound="hbm", achieved=bw / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=bw / HBM_PEAK,
+                               bytes_per_launch=r["bytes"], flops_per_launch=r["flops"])
+                else:
+                    fl = r["flops"] / (per * 1e-3)
+                    row.update(bound="mfma", achieved=fl / 1e12, peak=PEAK["bf16"] / 1e12, unit="TFLOP/s",
+                               frac=fl / PEAK["bf16"], bytes_per_launch=r["bytes"], flops_per_launch=r["flops"])
+            rows.append(row)
+        rows.sort(key=lambda r: -r["ms_per_step"])
+        return rows
 
 
 def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=None, reverse_prec=None):
@@ -136,47 +242,49 @@ def profile_traffic(name):
     return None
 
 
-def cpu_baseline(seconds_target=12.0, single_seconds=8.0):
+def cpu_baseline(seconds_target=12.0, single_seconds=8.0, batch=1024):
     """FP64 NumPy oracle of the same step on a bounded sample of the grid
-    (consecutive 128-point batches), timed on this host with all of its
-    cores and again with BLAS pinned to one thread (the reference's own
-    pinning); scaled to point*modes/s."""
+    (consecutive `batch`-point batches: large enough that every BLAS call of
+    the oracle spreads over all host cores), timed on this host with all of
+    its cores and again with BLAS pinned to one thread (the reference's own
+    pinning, experiments/metaball/run.py:5-6); scaled to point*modes/s."""
     import numpy as np
     from oracle import neb_oracle as o
     pts = o.periodic_grid(N_GRID)
     sizes = [4 * FREQS] + list(WIDTHS) + [K_MODES]
     net = o.init_net(sizes, 0)
     w = (2 * math.pi / N_GRID) ** 2
-    n_s = 128
+    dM = np.eye(K_MODES) * 1e-3  # the objective's K x K algebra is size-independent; fixed cotangents
 
     def run(seconds):
         t_total, n_done = 0.0, 0
         while True:
-            x = pts[n_done % len(pts): n_done % len(pts) + n_s]
+            start = n_done % len(pts)
+            x = pts[start: start + batch]
             t0 = time.perf_counter()
             x0 = o.feature_jets(x, FREQS, (0.0, 0.0), (2 * math.pi, 2 * math.pi))
             out, tape = o.forward(net, x0, record=True)
             u, om = o.lift2d(out)
             Pm, Ps = o.field_rows(u), om
-            M, S = o.gram(Pm, w), o.gram(Ps, w)
-            dM = np.eye(K_MODES) * 1e-3  # the objective's K x K algebra is size-independent; fixed cotangents
+            o.gram(Pm, w), o.gram(Ps, w)
             gPm, gPs = o.field_adjoint(Pm, w, dM), o.field_adjoint(Ps, w, dM)
             adj = o.lift2d_reverse(gPm.reshape(len(x), 2, K_MODES), gPs)
             o.backward(net, tape, adj)
             t_total += time.perf_counter() - t0
-            n_done += n_s
+            n_done += len(x)
             if t_total >= seconds:
                 return n_done, t_total
 
-    n_all, t_all = run(seconds_target)
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count()
-    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    threads = None
+    n_all, t_all = run(seconds_target)
     single = None
     try:
-        from threadpoolctl import threadpool_limits
+        from threadpoolctl import threadpool_info, threadpool_limits
+        threads = max((d.get("num_threads", 0) for d in threadpool_info()), default=None)
         with threadpool_limits(limits=1):
             n_1, t_1 = run(single_seconds)
         single = n_1 * K_MODES / t_1
@@ -188,10 +296,11 @@ def cpu_baseline(seconds_target=12.0, single_seconds=8.0):
             model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
     except OSError:
         pass
-    return dict(value=n_all * K_MODES / t_all, unit="point*modes/s", cores=min(threads, cores), kind="port",
-                sample=f"{n_all} of the 65,536 grid points in {n_s}-point batches, full jet fwd + Grams + "
+    return dict(value=n_all * K_MODES / t_all, unit="point*modes/s", cores=threads or cores, kind="port",
+                sample=f"{n_all} of the 65,536 grid points in {batch}-point batches, full jet fwd + Grams + "
                        f"reverse at K=1024 (FP64 NumPy oracle; K x K objective excluded), {t_all:.1f} s",
-                single_thread_value=single, affinity_cores=cores, cpu_model=model)
+                single_thread_value=single, single_thread_sample_s=single_seconds, affinity_cores=cores,
+                blas_threads=threads, cpu_model=model)
 
 
 def bench_kernel(args):
@@ -262,6 +371,8 @@ def bench_kernel(args):
                    sx3["b"][2].data_ptr(), 1.0, out.data_ptr(), None, O2.data_ptr(), O2l.data_ptr(), 256, s)
         elif k == "out_fields_x3":
             nw.output_fields(net, tx3, split=True)
+        elif k == "products":
+            nw._products(net, tape, fields)
 
     run()
     torch.cuda.synchronize()
@@ -279,10 +390,16 @@ N3D = 1_599_754
 CHUNK3D = 131_072
 
 
-def bench_3d(args):
-    """3D blob domain, K=1024 vector potential: two-pass chunked step
-    (pass 1: jets + lifted fields + FP64 Grams per 131,072-point chunk;
-    FP64 objective; pass 2: per-chunk recomputation + reverse) + Adam."""
+def config_e(args, quick=False):
+    """3D blob domain, K=1024 vector potential (the 3D K=1024 target
+    workload): the two-pass chunked step (pass 1: jets + lifted fields +
+    FP64 Grams per 131,072-point chunk; FP64 objective; pass 2: per-chunk
+    recomputation + reverse) + Adam over all 1,599,754 points, in split
+    precision (headline), plain BF16 and FP32 kernels; one chunk's field and
+    Gram accuracy and the whole step's gradient against the FP64 path; and
+    the 20-target FP64 full-mass projection on the split-precision basis
+    against the same projection on the FP64 basis.  quick=True (inside the
+    default 2D run) times 2 split-precision steps and one of each other."""
     import numpy as np
     import torch
 
@@ -295,7 +412,7 @@ def bench_3d(args):
     torch.cuda.set_device(dev)
     t0 = time.perf_counter()
     pts, _, vol = syn.blob_domain(N3D, seed=0)
-    print(f"bench E: sampled {N3D} blob-domain points in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+    print(f"config E: sampled {N3D} blob-domain points in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
     x = torch.tensor(pts, dtype=torch.float32, device=dev)
     N, K = N3D, K_MODES
@@ -323,8 +440,7 @@ def bench_3d(args):
         torch.cuda.synchronize()
         t = e0.elapsed_time(e1) * 1e-3
         gram_stats[mode] = dict(tflops=(12 * CHUNK3D) * K * (K + 1) / t / 1e12, ms=t * 1e3)
-    del f0, s0, G
-    solver = sp.SpectralSolver(1e-2, ridge, deferred=True)
+    del f0, s0, G, M0
 
     def step(prec, gmode, solv):
         tt = {}
@@ -332,15 +448,35 @@ def bench_3d(args):
         net.adam(res["grads"], 1e-4, gate=solv.gate, on_gate=solv.raise_gate)
         return res, tt
 
-    f32_rate = None
-    if not args.no_f32:
-        s32 = sp.SpectralSolver(1e-2, ridge, deferred=True)
-        step("f32", "f32", s32)
+    def timed_steps(prec, gmode, warm, reps):
+        solv = sp.SpectralSolver(1e-2, ridge, deferred=True)
+        for _ in range(warm):
+            step(prec, gmode, solv)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        step("f32", "f32", s32)
+        stages = {}
+        for _ in range(reps):
+            res, tt = step(prec, gmode, solv)
+            for k, v in tt.items():
+                stages[k] = stages.get(k, 0.0) + v / reps * 1e3
         torch.cuda.synchronize()
-        f32_rate = N * K / (time.perf_counter() - t0)
+        return (time.perf_counter() - t0) / reps, stages, res, solv
+
+    base = net.params.detach().clone()
+
+    def reset():
+        with torch.no_grad():
+            net.params.copy_(base)
+            net.m.zero_()
+            net.v.zero_()
+        net.epoch = 0
+        net.touch()
+
+    f32_rate = None
+    if not args.no_f32:
+        t32, _, _, _ = timed_steps("f32", "f32", 1, 1)
+        f32_rate = N * K / t32
+        reset()
     # accuracy of one chunk's fields and Grams against the FP64 path
     accuracy = {}
     tape64 = nw.trunk_forward(net, x[:CHUNK3D], nw.PRECISIONS["f64"])
@@ -351,9 +487,8 @@ def bench_3d(args):
         tp = nw.trunk_forward(net, x[:CHUNK3D], nw.PRECISIONS[pr])
         fp = nw.output_fields(net, tp, split=True)
         del tp
-        U = fp.mass[:, :K].float() + fp.mass_lo[:, :K].float()
-        acc = dict(velocity_rel_l2=float(torch.linalg.norm(U.double() - f64f.mass[:, :K])
-                                         / torch.linalg.norm(f64f.mass[:, :K])))
+        U = nw.field_matrix(fp, "mass", torch.float64)
+        acc = dict(velocity_rel_l2=float(torch.linalg.norm(U - f64f.mass[:, :K]) / torch.linalg.norm(f64f.mass[:, :K])))
         del U
         for wh, key in (("mass", "mass_gram_rel_err"), ("stiff", "grad_u_gram_rel_err")):
             G = nw.field_gram(fp, wh, w, "bf16x3")
@@ -363,75 +498,103 @@ def bench_3d(args):
     del f64f, G64
     accuracy["note"] = "first 131,072-point chunk, vs FP64 jets, fields and Grams of the same points and weights"
 
+    # the whole step's gradient against the FP64 path (same points, weights,
+    # ridge and base parameters): the split-precision step, and its reverse
+    # alone under the FP64 path's cotangents
+    grad_err = None
+    if not args.no_f32:
+        r64 = sp.chunked_step(net, x, w, "f64", "f64", CHUNK3D, sp.SpectralSolver(1e-2, ridge))
+        g64 = r64["grads"].double()
+        L64 = r64["loss"]
+        solv64 = sp.SpectralSolver(1e-2, ridge)
+        _, dM64, dS64 = solv64(r64["M"], r64["S"])
+        del r64
+        rs = sp.chunked_step(net, x, w, "bf16x3", "bf16x3", CHUNK3D, sp.SpectralSolver(1e-2, ridge))
+        rr = sp.chunked_step(net, x, w, "bf16x3", "bf16x3", CHUNK3D, lambda M, S: (L64, dM64, dS64))
+
+        def cmp(g):
+            g = g.double()
+            return dict(rel_l2=float(torch.linalg.norm(g - g64) / torch.linalg.norm(g64)),
+                        cosine=float(torch.dot(g, g64) / (torch.linalg.norm(g) * torch.linalg.norm(g64))))
+        grad_err = dict(whole_step=cmp(rs["grads"]), reverse_only=cmp(rr["grads"]),
+                        note="split-precision chunked step vs the FP64 chunked step over all points; reverse_only "
+                             "= split-precision tape and fields under the FP64 path's cotangents")
+        del rs, rr, g64, dM64, dS64
+
     head = args.precision
     other_prec = "bf16" if head == "bf16x3" else "bf16x3"
     other = None
     if not args.no_f32:
-        so = sp.SpectralSolver(1e-2, ridge, deferred=True)
-        step(other_prec, "bf16x3", so)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        step(other_prec, "bf16x3", so)
-        torch.cuda.synchronize()
-        to = time.perf_counter() - t0
+        to, _, _, _ = timed_steps(other_prec, "bf16x3", 1, 1)
         other = dict(point_modes_per_s=N * K / to, ms_per_step=to * 1e3,
                      vs_fp32_step=(N * K / to) / f32_rate if f32_rate else None,
                      accuracy=f"accuracy_vs_fp64['{other_prec}']")
-    for _ in range(max(1, min(args.warmup, 2))):
-        step(head, "bf16x3", solver)
-    steps = max(1, min(args.steps, 3))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    stages = {}
-    for _ in range(steps):
-        res, tt = step(head, "bf16x3", solver)
-        for k, v in tt.items():
-            stages[k] = stages.get(k, 0.0) + v / steps * 1e3
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
+        reset()
+    reps = 2 if quick else max(1, min(args.steps, 3))
+    warm = 1 if quick else max(1, min(args.warmup, 2))
+    dt, stages, res, solver = timed_steps(head, "bf16x3", warm, reps)
 
-    # 20 divergence-free targets projected onto the basis by the FP64 mass
-    # solve (two streamed passes: loads, then the direct residual)
+    # 20 divergence-free targets projected onto the split-precision basis by
+    # the FP64 mass solve (two streamed passes: loads, then the direct
+    # residual), and onto the FP64 basis of the same network
     t1 = time.perf_counter()
     tgt = syn.curl_targets(pts, 20, seed=1)
     targets = torch.tensor(tgt.reshape(N * 3, 20), dtype=torch.float64, device=dev)
     del tgt
     gen_s = time.perf_counter() - t1
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    # FP64 mass of the fp32 basis of the current network (fp32 widened on
-    # load), as the projection's Gram identity needs; the K=1024 basis is
-    # rank deficient, so the solve uses the step's ridge
-    M64 = torch.zeros((K, K), dtype=torch.float64, device=dev)
-    for _, f in sp.basis_stream(net, x, "bf16", CHUNK3D):
-        nw.field_gram(f, "mass", w, "f64", out=M64, beta=1.0)
-        del f
-    torch.cuda.synchronize()
-    mass_s = time.perf_counter() - t1
-    t1 = time.perf_counter()
-    proj = sp.project_streamed(net, x, targets, M64, w, "bf16", CHUNK3D, ridge=ridge)
-    torch.cuda.synchronize()
-    proj_s = time.perf_counter() - t1
-    projection = dict(targets=20, ms=proj_s * 1e3, fp64_mass_ms=mass_s * 1e3, target_generation_host_s=gen_s,
-                      normal_residual_max=float(proj["normal_residuals"].max()),
-                      direct_relative_L2_median=float(proj["direct_relative_L2"].median()),
-                      identity_gap_max=float((proj["direct_relative_L2"] - proj["Gram_identity_errors"]).abs().max()))
-    del targets, proj, M64
-    out = {
-        "metric": "point*modes/s, jet fwd+bwd + Gram spectral step, 3D K=1024",
-        "value": N * K / dt, "unit": "point*modes/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "bf16", "data": "synthetic: 6-blob domain in [-1,1]^3 (seed 0), Xavier-normal weights seed 0",
+    projection = dict(targets=20, target_generation_host_s=gen_s, basis=head)
+    proj = {}
+    for pr in (head, "f64"):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        # FP64 mass of the projection's basis (the K=1024 basis is rank
+        # deficient, so the solve uses the step's ridge)
+        M64 = sp.fullmass_fp64(net, x, w, pr, CHUNK3D)
+        torch.cuda.synchronize()
+        mass_s = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        proj[pr] = sp.project_streamed(net, x, targets, M64, w, pr, CHUNK3D, ridge=ridge)
+        torch.cuda.synchronize()
+        if pr == head:
+            projection.update(ms=(time.perf_counter() - t1) * 1e3, fp64_mass_ms=mass_s * 1e3)
+        del M64
+    p, q = proj[head], proj["f64"]
+    projection.update(
+        normal_residual_max=float(p["normal_residuals"].max()),
+        direct_relative_L2_median=float(p["direct_relative_L2"].median()),
+        identity_gap_max=float((p["direct_relative_L2"] - p["Gram_identity_errors"]).abs().max()),
+        vs_fp64_basis=dict(
+            coefficient_rel_err=float(torch.linalg.norm(p["coefficients"] - q["coefficients"])
+                                      / torch.linalg.norm(q["coefficients"])),
+            loads_rel_err=float(torch.linalg.norm(p["loads"] - q["loads"]) / torch.linalg.norm(q["loads"])),
+            residual_max_abs_diff=float((p["direct_relative_L2"] - q["direct_relative_L2"]).abs().max()),
+            fp64_basis_residual_median=float(q["direct_relative_L2"].median()),
+            note="same network, targets, weights and ridge; the FP64 basis = FP64 jets and fields"))
+    del targets, proj, p, q
+    return {
+        "value": N * K / dt, "unit": "point*modes/s", "ms_per_step": dt * 1e3, "steps": reps, "warmup": warm,
         "config": {"workload": "3D blob domain: 1,599,754 pts, sin/cos freqs 1..6, 4x256 tanh, K=1024 vector "
                                "potential, curl lift, mass+grad-u Gram, 131,072-pt chunks, spectral loss + "
                                "reverse + Adam", "points": N, "modes": K, "chunk": CHUNK3D, "ridge": ridge,
                    "domain_volume_estimate": vol},
         "precision": head, "stages_ms": stages, "gram_one_chunk": gram_stats,
         "fp32_step_point_modes_per_s": f32_rate,
+        "vs_fp32_step": (N * K / dt) / f32_rate if f32_rate else None,
         ("bf16_step" if head == "bf16x3" else "split_precision_step"): other,
-        "accuracy_vs_fp64": accuracy,
+        "accuracy_vs_fp64": accuracy, "gradient_vs_fp64": grad_err,
         "loss": float(res["loss"]), "objective_path": solver.last_path, "projection": projection,
     }
+
+
+def bench_3d(args):
+    """--config E: the 3D K=1024 step as the headline line."""
+    e = config_e(args)
+    out = {"metric": "point*modes/s, jet fwd+bwd + Gram spectral step, 3D K=1024", "value": e.pop("value"),
+           "unit": e.pop("unit"), "n_gpus": 1, "steps": e.pop("steps"), "warmup": e.pop("warmup"),
+           "ms_per_step": e.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16", "data": "synthetic: 6-blob domain in [-1,1]^3 (seed 0), Xavier-normal weights seed 0",
+           "config": e.pop("config")}
+    out.update(e)
     print(json.dumps(out), flush=True)
 
 
@@ -667,6 +830,45 @@ def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
     return accuracy
 
 
+def gradient_errors(nw, sp, net, x, w, ridge):
+    """Gradient of every step variant against the FP64 path's on the same
+    points, weights and ridge: relative L2 error and cosine of the full
+    parameter gradient, for the whole step (the variant's own fields, Grams,
+    objective and reverse) and for its reverse alone (the variant's tape and
+    fields under the FP64 path's cotangents dL/dM, dL/dS).  The whole-step
+    error includes the objective's amplification of Gram errors (its
+    cotangent M_r^-1 S M_r^-1 moves by up to cond(M_r) ~ 1e6 times the Grams'
+    relative error); the reverse-only error is the kernels' own rounding."""
+    import torch
+    res64 = sp.compute(net, x, w, prec="f64", gram_mode="f64", ridge=ridge, gate="cholesky")
+    g64 = res64["grads"].double()
+    dM, dS = res64["dM"], res64["dS"]
+
+    def cmp(g):
+        g = g.double()
+        return dict(rel_l2=float(torch.linalg.norm(g - g64) / torch.linalg.norm(g64)),
+                    cosine=float(torch.dot(g, g64) / (torch.linalg.norm(g) * torch.linalg.norm(g64))))
+
+    out = {}
+    for name, prec, gm, rp in (("bf16x3", "bf16x3", "bf16x3", None), ("bf16", "bf16", "bf16x3", None),
+                               ("f32+bf16x3", "f32", "bf16x3", "bf16"), ("f32", "f32", "f32", None)):
+        res = sp.compute(net, x, w, prec=prec, gram_mode=gm, ridge=ridge, gate="cholesky", reverse_prec=rp)
+        whole = cmp(res["grads"])
+        del res
+        tape = nw.trunk_forward(net, x, nw.PRECISIONS[prec])
+        fields = nw.output_fields(net, tape, split=nw.split_fields_for(prec, gm))
+        if rp is not None:
+            tape = nw.convert_tape(tape, nw.PRECISIONS[rp])
+        g = nw.backward_fields(net, tape, fields, dM, dS, w, symmetric=True)
+        out[name] = dict(whole_step=whole, reverse_only=cmp(g))
+        del tape, fields, g
+    out["note"] = ("vs the FP64 path's gradient (FP64 jets, fields, Grams, objective and reverse) on the same grid, "
+                   "weights and ridge; whole_step = the variant's own step, reverse_only = the variant's tape and "
+                   "fields under the FP64 path's cotangents")
+    del res64
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -711,10 +913,11 @@ def main():
     # ---- Gram modes on the same fields: TFLOP/s and error vs FP64 ----------
     # "bf16x3-split": the step's path (pre-split bf16 planes); "bf16x3" splits
     # the fp32 fields per tile; both give the same bits.
-    gram_stats, accuracy, fd_check = {}, {}, None
+    gram_stats, accuracy, fd_check, grad_errors = {}, {}, None, None
     if not args.step_only:
         gram_stats = gram_mode_stats(nw, fields, split, K, w)
         accuracy = accuracy_vs_fp64(nw, sp, net, x, K, w, ridge)
+        grad_errors = gradient_errors(nw, sp, net, x, w, ridge)
         fd_check = fd_check_fp64(nb, sp, cfg, dev)
     del tape, fields, split
 
@@ -781,6 +984,19 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     stages = {k: float(np.mean(v)) for k, v in timer.ms().items()}
+    # per-kernel launch times: the same steps again, with HIP events on the
+    # launching stream around every libnebhip call (recording them costs
+    # ~10 % of the step, so these steps stay out of `value`)
+    launches = LaunchTimer(torch)
+    torch.cuda.synchronize()
+    nw.H.set_launch_recorder(launches)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt_rec = (time.perf_counter() - t1) / args.steps
+    nw.H.set_launch_recorder(None)
+    kernels = launches.table(args.steps, dt_rec * 1e3)
 
     # The split-precision mass Gram (R = 2N rows, K = 1024): algorithmic
     # R*K*(K+1) flops (the symmetric half, multiply + add; the three split
@@ -794,23 +1010,15 @@ def main():
                          kernel="neb_gram_split mass Gram: cross_async_kernel<2, 2, 2, 2, false> + "
                                 "cross_reduce_kernel, R=131072, K=1024", launch_ms=gram_ms,
                          flops_per_launch=gram_flops)
-    if head == "bf16x3":
-        # dominant launch of the split-precision step: the output layer + lift
-        # on split operands (one launch, the output_fields stage).  Per point
-        # it reads 4 jet channels x 256 (hi + lo, 4 B) and writes 3 fields x K
-        # (hi + lo, 4 B): 16 KB; its useful 4 * 2 * 256 * K = 2.1 MFLOP per
-        # point (128 flop/B) put it below the bf16 ridge: HBM-bound.
-        fl_ms = stages["output_fields"]
-        fl_bytes = N * (4 * 256 * 4 + 3 * K * 4) + 2 * K * 256 * 2
-        fl_flops = N * 4 * 2 * 256 * K
-        bw = fl_bytes / (fl_ms * 1e-3)
-        roofline = dict(bound="hbm", achieved=bw / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=bw / HBM_PEAK,
-                        traffic=profile_traffic("out_fields_x3_k1024_pmc.md"),
-                        kernel="neb_jet_output_fields_x3: rowgemm_async_kernel<PlanOutFields2Split>, 65,536 points, "
-                               "width 256, K=1024", launch_ms=fl_ms, bytes_per_launch=fl_bytes,
-                        useful_tflops=fl_flops / (fl_ms * 1e-3) / 1e12)
-    else:
-        roofline = gram_roofline
+    # roofline of the dominant kernel: the kernel class with the largest share
+    # of the step's time (event-timed launches of the timed steps above)
+    dom = next(r for r in kernels if "bound" in r)
+    profile = PROFILE_OF.get(dom["kernel"])
+    roofline = dict(bound=dom["bound"], achieved=dom["achieved"], peak=dom["peak"], unit=dom["unit"],
+                    frac=dom["frac"], traffic=profile_traffic(profile) if profile else None,
+                    kernel=dom["kernel"], share_of_step=dom["share_of_step"], launch_ms=dom["ms_per_launch"],
+                    bytes_per_launch=dom["bytes_per_launch"], flops_per_launch=dom["flops_per_launch"],
+                    traffic_profile=profile)
 
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
@@ -846,8 +1054,18 @@ def main():
         "objective_path": solver.last_path,
         "roofline": roofline,
         "gram_roofline": gram_roofline,
+        "kernels": kernels,
+        "kernels_note": (f"per kernel class (entry point, inner width or rows): launches, event-timed ms per "
+                         f"step and per launch over {args.steps} recorded steps ({dt_rec * 1e3:.3f} ms each), "
+                         "share of that step (side-stream kernels overlap the current stream, so shares can sum "
+                         "above 1), algorithmic bytes / useful flops per launch and the roofline they imply"),
+        "gradient_vs_fp64": grad_errors,
         "cpu_baseline": cpu,
     }
+    if not args.no_3d:
+        del net, solver, step, step_t, x
+        torch.cuda.empty_cache()
+        out["config_E"] = config_e(args, quick=True)
     print(json.dumps(out), flush=True)
 
 

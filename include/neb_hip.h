@@ -129,6 +129,12 @@ int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const 
  * VJ2:93-101, the weight rows 3k+c as in neb_jet_output_fields). */
 int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* hi, void* lo,
                    int64_t ldd, void* stream);
+
+/* The inverse of neb_split_bf16: dst = hi + lo (rows x cols, plane stride ld)
+ * as NEB_F32 (exact) or NEB_F64.  The split-precision basis enters the FP64
+ * full-mass projection loads through it (PP:141-177 reads an fp32 basis). */
+int neb_join_bf16(int dst_dtype, const void* hi, const void* lo, int64_t rows, int64_t cols, int64_t ld, void* dst,
+                  int64_t ldd, void* stream);
 int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t width, const void* x_hi,
                              const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
                              const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,

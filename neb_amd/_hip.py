@@ -34,6 +34,7 @@ SIGNATURES = {
     "neb_jet_output_fields_split": (_int, [_int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i64,
                                            _int, _vp]),
     "neb_split_bf16": (_int, [_int, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _vp]),
+    "neb_join_bf16": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp]),
     "neb_jet_layer_forward_x3": (_int, [_int, _int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _dbl,
                                         _vp, _vp, _vp, _vp, _i64, _vp]),
     "neb_jet_output_fields_x3": (_int, [_int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
@@ -95,8 +96,25 @@ def check(rc: int) -> None:
         raise _EXC.get(rc, RuntimeError)(msg)
 
 
+_recorder = None
+
+
+def set_launch_recorder(recorder) -> None:
+    """Install (or, with None, remove) a per-launch hook: recorder(name, args)
+    is called before each entry point and must return a callable that is
+    called after it (a timing harness records stream events there).  The
+    default is no hook."""
+    global _recorder
+    _recorder = recorder
+
+
 def call(name: str, *args) -> None:
+    if _recorder is None:
+        check(getattr(load(), name)(*args))
+        return
+    done = _recorder(name, args)
     check(getattr(load(), name)(*args))
+    done()
 
 
 def ptr_array(ptrs):

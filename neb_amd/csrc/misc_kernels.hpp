@@ -197,6 +197,27 @@ __global__ __launch_bounds__(256) void split_kernel(const TS* __restrict__ src, 
   lo[r * ldd + c] = (__bf16)(float)(x - (double)(float)h);
 }
 
+// Join of bf16 hi / lo planes into one fp32 or fp64 matrix: dst = hi + lo
+// (exact in fp32: lo holds the rounding residual of hi).  Each thread joins
+// eight consecutive columns of one row.
+template <typename TD>
+__global__ __launch_bounds__(256) void join_kernel(const __bf16* __restrict__ hi, const __bf16* __restrict__ lo,
+                                                   int64_t rows, int64_t cols, int64_t ld, TD* __restrict__ dst,
+                                                   int64_t ldd) {
+  const int64_t per_row = (cols + 7) / 8;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * per_row) return;
+  const int64_t r = idx / per_row, c0 = (idx % per_row) * 8;
+  const __bf16* h = hi + r * ld;
+  const __bf16* l = lo + r * ld;
+  TD* d = dst + r * ldd;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t c = c0 + q;
+    if (c < cols) d[c] = (TD)((float)h[c] + (float)l[c]);
+  }
+}
+
 template <typename TS, typename TD>
 __global__ __launch_bounds__(256) void stage_kernel(const TS* __restrict__ src, int64_t rows, int64_t cols,
                                                     int64_t lds, TD* __restrict__ dst, int64_t ldd,

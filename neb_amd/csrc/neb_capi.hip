@@ -959,6 +959,30 @@ int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, i
   return fail(NEB_ERR_ARG, "neb_split_bf16: source dtype must be NEB_F32 or NEB_F64 (got %d)", src_dtype);
 }
 
+int neb_join_bf16(int dst_dtype, const void* hi, const void* lo, int64_t rows, int64_t cols, int64_t ld, void* dst,
+                  int64_t ldd, void* stream) {
+  if (rows < 0 || cols < 0 || ld < cols || ldd < cols) return fail(NEB_ERR_SHAPE, "neb_join_bf16: bad sizes");
+  const int64_t units = rows * ((cols + 7) / 8);
+  if (units == 0) return NEB_OK;
+  if (!hi || !lo || !dst) return fail(NEB_ERR_ARG, "neb_join_bf16: null pointer");
+  if (units / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_join_bf16: too many elements");
+  const dim3 grid((unsigned)((units + 255) / 256));
+  const __bf16* h = reinterpret_cast<const __bf16*>(hi);
+  const __bf16* l = reinterpret_cast<const __bf16*>(lo);
+  hipStream_t st = S(stream);
+  switch (dst_dtype) {
+    case NEB_F32:
+      hipLaunchKernelGGL(join_kernel<float>, grid, dim3(256), 0, st, h, l, rows, cols, ld,
+                         reinterpret_cast<float*>(dst), ldd);
+      return hip_status("neb_join_bf16");
+    case NEB_F64:
+      hipLaunchKernelGGL(join_kernel<double>, grid, dim3(256), 0, st, h, l, rows, cols, ld,
+                         reinterpret_cast<double*>(dst), ldd);
+      return hip_status("neb_join_bf16");
+  }
+  return fail(NEB_ERR_ARG, "neb_join_bf16: destination dtype must be NEB_F32 or NEB_F64 (got %d)", dst_dtype);
+}
+
 int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t width, const void* x_hi,
                              const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
                              const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,

@@ -523,6 +523,20 @@ def gram(phi: torch.Tensor, K: int, w: float, mode: str = "bf16x3", out: torch.T
     return out
 
 
+def field_matrix(fields: Fields, which: str, dtype=torch.float32) -> torch.Tensor:
+    """The mass ("mass") or stiffness ("stiff") field matrix [rows, K] as one
+    fp32 (or fp64) tensor: split planes joined as hi + lo (neb_join_bf16),
+    other fields converted only when their dtype differs."""
+    hi, lo = (fields.mass, fields.mass_lo) if which == "mass" else (fields.stiff, fields.stiff_lo)
+    K = fields.K
+    if lo is None:
+        return hi[:, :K] if hi.dtype == dtype else hi[:, :K].to(dtype)
+    out = torch.empty((hi.shape[0], _ld(K)), dtype=dtype, device=hi.device)
+    H.call("neb_join_bf16", _DT[dtype], hi.data_ptr(), lo.data_ptr(), hi.shape[0], K, hi.stride(0), out.data_ptr(),
+           out.shape[1], _stream())
+    return out[:, :K]
+
+
 def split_fields_for(prec: str, gram_mode: str) -> bool:
     """Whether output_fields should write split planes: the bf16 tape with
     the split-precision Gram (results are bit-identical to fp32 fields), and

@@ -443,8 +443,10 @@ def basis_stream(net: nw.EigenfluidNet, points: torch.Tensor, prec: str = "bf16"
                  split: bool = False):
     """Yield (first point, Fields) chunk by chunk: the full-grid basis
     evaluation of experiments/metaball/evaluate.py:130-190 without the disk
-    round trip."""
+    round trip.  The split-precision forward (prec="bf16x3") always yields
+    split fields."""
     p = nw.PRECISIONS[prec]
+    split = split or prec == "bf16x3"
     for c0 in range(0, points.shape[0], chunk):
         tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
         f = nw.output_fields(net, tape, split=split)
@@ -466,6 +468,21 @@ def fullmass(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: s
     return M, ev
 
 
+def fullmass_fp64(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
+                  chunk: int = 131072) -> torch.Tensor:
+    """FP64 mass Gram of the basis that project_streamed uses for `prec`
+    (the fp32 field matrix widened to FP64 on load; the FP64 fields in the
+    FP64 mode), accumulated chunk by chunk: the mass of the projection's Gram
+    identity (PP:141-177)."""
+    K = net.cfg.modes
+    M = torch.zeros((K, K), dtype=torch.float64, device=net.device)
+    for _, f in basis_stream(net, points, prec, chunk):
+        B = nw.field_matrix(f, "mass", torch.float64 if prec == "f64" else torch.float32)
+        nw.gram(B, K, weight, "f64", out=M, beta=1.0)
+        del f, B
+    return M
+
+
 def project_streamed(net: nw.EigenfluidNet, points: torch.Tensor, targets: torch.Tensor, mass: torch.Tensor,
                      weight: float, prec: str = "bf16", chunk: int = 131072, ridge: float = 0.0,
                      normal_tol: float = 1e-8, identity_tol: float = 1e-8):
@@ -474,19 +491,23 @@ def project_streamed(net: nw.EigenfluidNet, points: torch.Tensor, targets: torch
     loads = w B^T T and the target norms (FP64 cross kernel), the FP64 solve
     (with optional ridge) is gated on the normal residual, pass 2 re-streams
     the basis for the direct residual sum w |B c - T|^2, which must agree with
-    the Gram identity.  targets: [n*comp, m] point-major rows, FP64."""
+    the Gram identity.  targets: [n*comp, m] point-major rows, FP64.  The
+    basis is the fp32 field matrix of the `prec` forward (split-precision
+    fields joined as hi + lo, FP64 fields in the FP64 mode); `mass` must be
+    the FP64 mass of that same basis (fullmass_fp64)."""
     K = net.cfg.modes
     m = targets.shape[1]
     comp = 2 if net.cfg.dim == 2 else 3
     loads = torch.zeros((K, m), dtype=torch.float64, device=net.device)
     norms2 = torch.zeros((m, m), dtype=torch.float64, device=net.device)
     for c0, f in basis_stream(net, points, prec, chunk):
-        rows = f.mass.shape[0]
+        B = nw.field_matrix(f, "mass", torch.float64 if prec == "f64" else torch.float32)
+        rows = B.shape[0]
         T = targets[comp * c0: comp * c0 + rows]
         # fp32 basis widened to fp64 on load (fp64 basis in the FP64 mode)
-        nw.cross([(f.mass, T, rows, 1.0)], K, m, nw.GRAM_MODES["f64"], scale=weight, out=loads, beta=1.0)
+        nw.cross([(B, T, rows, 1.0)], K, m, nw.GRAM_MODES["f64"], scale=weight, out=loads, beta=1.0)
         nw.cross([(T, T, rows, 1.0)], m, m, nw.GRAM_MODES["f64"], scale=weight, out=norms2, beta=1.0)
-        del f
+        del f, B
     norms = torch.diagonal(norms2).clone()
     Mr = _sym(mass) + ridge * torch.eye(K, dtype=torch.float64, device=mass.device)
     coeff = torch.linalg.solve(Mr, loads)
@@ -496,10 +517,11 @@ def project_streamed(net: nw.EigenfluidNet, points: torch.Tensor, targets: torch
     gram_square = norms - 2 * torch.sum(coeff * loads, 0) + torch.sum(coeff * (mass @ coeff), 0)
     direct = torch.zeros(m, dtype=torch.float64, device=net.device)
     for c0, f in basis_stream(net, points, prec, chunk):
-        rows = f.mass.shape[0]
-        diff = f.mass[:, :K].to(torch.float64) @ coeff - targets[comp * c0: comp * c0 + rows]
+        B = nw.field_matrix(f, "mass", torch.float64)
+        rows = B.shape[0]
+        diff = B @ coeff - targets[comp * c0: comp * c0 + rows]
         direct += weight * torch.sum(diff * diff, 0)
-        del f, diff
+        del f, diff, B
     errors = torch.sqrt(direct / norms)
     identity = torch.sqrt(torch.clamp(gram_square, min=0) / norms)
     if not bool((torch.abs(errors - identity) < identity_tol).all()):

@@ -582,6 +582,36 @@ def test_project_streamed_against_oracle():
     assert float(res["normal_residuals"].max()) < 1e-8
 
 
+def test_project_streamed_split_precision_basis():
+    """The projection on the split-precision basis (hi + lo planes joined to
+    fp32 by neb_join_bf16) against the oracle's projection on the exact
+    basis: coefficients at the basis' FP32-level accuracy, the identity and
+    normal gates passing; the join itself is exact."""
+    from neb_amd import synthetic as syn
+    F, K = 2, 12
+    cfg, net, onet = make(3, K, [32, 32], F, master=torch.float32)
+    pts, _, _ = syn.blob_domain(3000, seed=3, batch=1 << 14)
+    T = syn.curl_targets(pts, 20, seed=4).reshape(-1, 20)
+    w = 1.0 / len(pts)
+    x = torch.tensor(pts)
+    M = nb.spectral.fullmass_fp64(net, x, w, "bf16x3", chunk=1024)
+    res = nb.project_streamed(net, x, torch.tensor(T, device=DEV), M, w, prec="bf16x3", chunk=1024)
+    out, _ = o.forward(onet, o.feature_jets(pts, F, cfg.lo, cfg.hi))
+    u, _ = o.lift3d(out)
+    B = o.field_rows(u)
+    ref = o.project(o.gram(B, w), B, T, w)
+    r = rel(res["coefficients"].cpu(), ref["coefficients"]); assert r < 1e-3, r
+    r = rel(res["direct_relative_L2"].cpu(), ref["direct_relative_L2"]); assert r < 1e-5, r
+    assert float(res["normal_residuals"].max()) < 1e-8
+    # the join: hi + lo of an fp32 split, exactly
+    tape = nw.trunk_forward(net, x[:1000], nw.PRECISIONS["bf16x3"])
+    f = nw.output_fields(net, tape, split=True)
+    J = nw.field_matrix(f, "mass", torch.float32)
+    assert torch.equal(J, f.mass[:, :K].float() + f.mass_lo[:, :K].float())
+    J64 = nw.field_matrix(f, "stiff", torch.float64)
+    assert torch.equal(J64, (f.stiff[:, :K].float() + f.stiff_lo[:, :K].float()).double())
+
+
 def test_fullmass_rank_gate_rejects_rank_deficient_basis():
     """K modes from a width-4 trunk in 2D: rank(M) <= 8 < K, so the full-mass
     rank gate (experiments/metaball/run.py:108-110) raises."""
