@@ -765,7 +765,16 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
 def _products(net: EigenfluidNet, tape: Tape, fields: Fields) -> list:
     """[P_m; P_s] ([2K, width] FP64; one per output component slice in 3D):
     the field x last-hidden-jet cross products over points of the
-    output-layer gradient (VCORE:94-95 through the lift)."""
+    output-layer gradient (VCORE:94-95 through the lift).
+
+    2D: the fields are linear in the output weights W (u = Y_y W^T,
+    v = -Y_x W^T, omega = -(Y_xx + Y_yy) W^T), so
+        P_m = u^T Y_y - v^T Y_x           = W (Y_x^T Y_x + Y_y^T Y_y)
+        P_s = -omega^T (Y_xx + Y_yy)      = W (Y_xx + Y_yy)^T (Y_xx + Y_yy)
+    -- 256 x 256 Grams of the last hidden jets over the points (three cross
+    launches reading only those jets, a fifth of the bytes of the K-wide
+    field products) times the master W in FP64.  3D keeps the field
+    products: its 27 channel-pair Grams would read as many bytes."""
     cfg = net.cfg
     d, n, K = cfg.dim, tape.n, cfg.modes
     width = cfg.widths[-1]
@@ -776,10 +785,15 @@ def _products(net: EigenfluidNet, tape: Tape, fields: Fields) -> list:
     if fields.layout != 0:
         raise ValueError("the output-layer products expect point-major field matrices")
     if d == 2:
-        u, v = _rows_view(U, 0, 2), _rows_view(U, 1, 2)
+        Hm = cross([(ch(Ylast, 1), ch(Ylast, 1), n, 1.0), (ch(Ylast, 2), ch(Ylast, 2), n, 1.0)], width, width, mode)
+        Hs = cross([(ch(Ylast, 3), ch(Ylast, 3), n, 1.0), (ch(Ylast, 5), ch(Ylast, 5), n, 1.0)], width, width, mode)
+        G35 = cross([(ch(Ylast, 3), ch(Ylast, 5), n, 1.0)], width, width, mode)
+        Hs += G35
+        Hs += G35.T
+        Wd = net.W[-1].to(torch.float64)
         P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
-        cross([(u, ch(Ylast, 2), n, 1.0), (v, ch(Ylast, 1), n, -1.0)], K, width, mode, out=P[:K])
-        cross([(Dg, ch(Ylast, 3), n, -1.0), (Dg, ch(Ylast, 5), n, -1.0)], K, width, mode, out=P[K:])
+        torch.mm(Wd, Hm, out=P[:K])
+        torch.mm(Wd, Hs, out=P[K:])
         return [P]
     from .lifts import curl_terms
     comps = [_rows_view(U, a, 3) for a in range(3)]

@@ -19,10 +19,11 @@ compared with the FP64 oracle:
      two-pass chunked step against the single pass at the chunk size.
 
 Reverse pass at full size: the field cotangent of a Gram adjoint is
-2w Phi dL/dG row by row, so zeroing the field rows of every point outside a
-sample makes the full-size reverse (every tile, every workgroup's walk over
-several tiles, the full-width epilogues) equal to the oracle's reverse over
-the sampled points alone.
+2w Phi dL/dG row by row, so zeroing the field rows (and, for the 2D
+output-layer products, the last hidden jets) of every point outside a sample
+makes the full-size reverse (every tile, every workgroup's walk over several
+tiles, the full-width epilogues) equal to the oracle's reverse over the
+sampled points alone.
 """
 import math
 
@@ -117,6 +118,17 @@ def masked(f, sub, dim):
         m[idx] = t[idx]
         out.append(m)
     return nw.Fields(out[0], out[1], f.K, f.layout)
+
+
+def masked_tape(tape, sub):
+    """The tape with the last hidden jets zeroed outside the points `sub`
+    (the 2D output-layer products read them directly; every other reverse
+    term at an unsampled point already carries a zero cotangent)."""
+    Y = tape.inputs[-1]
+    m = torch.zeros_like(Y)
+    idx = torch.tensor(sub, device=Y.device)
+    m[:, idx] = Y[:, idx]
+    return nw.Tape(tape.prec, tape.n, tape.inputs[:-1] + [m], tape.pre, tape.inputs_lo, tape.pre_lo)
 
 
 def sym_cotangents(K, seed):
@@ -266,7 +278,8 @@ def test_config_B_reverse_full_size(prec):
     gW, gb = oracle_reverse(oracle_net(net), cfg, x[sub], w, dM, dS)
     tape = nw.trunk_forward(net, torch.tensor(x), nw.PRECISIONS[prec])
     f = nw.output_fields(net, tape, split=prec in ("bf16", "bf16x3"))
-    g = nw.backward_fields(net, tape, masked(f, sub, cfg.dim), torch.tensor(dM, device=DEV), torch.tensor(dS, device=DEV), w)
+    g = nw.backward_fields(net, masked_tape(tape, sub), masked(f, sub, cfg.dim), torch.tensor(dM, device=DEV),
+                           torch.tensor(dS, device=DEV), w)
     check_grads(net, g, gW, gb, REV_TOL[prec], f"config B reverse {prec}")
     del tape, f, g
     torch.cuda.empty_cache()
@@ -331,7 +344,8 @@ def test_config_E_chunk_reverse_full_size(chunk3d, prec):
     gW, gb = oracle_reverse(oracle_net(net), cfg, chunk3d[sub], w, dM, dS)
     tape = nw.trunk_forward(net, torch.tensor(chunk3d), nw.PRECISIONS[prec])
     f = nw.output_fields(net, tape, split=prec == "bf16x3")
-    g = nw.backward_fields(net, tape, masked(f, sub, cfg.dim), torch.tensor(dM, device=DEV), torch.tensor(dS, device=DEV), w)
+    g = nw.backward_fields(net, masked_tape(tape, sub), masked(f, sub, cfg.dim), torch.tensor(dM, device=DEV),
+                           torch.tensor(dS, device=DEV), w)
     check_grads(net, g, gW, gb, REV_TOL[prec], f"config E reverse {prec}")
     del tape, f, g
     torch.cuda.empty_cache()
