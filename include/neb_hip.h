@@ -290,6 +290,20 @@ size_t neb_spd_inverse_workspace_size(int64_t K);
 int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
                     size_t ws_bytes, void* stream);
 
+/* Every layer's bf16 operand forms in one launch, from the flat master
+ * parameters (fp32 or fp64): layer l's weight W_l is the row-major
+ * [rows[l]][cols[l]] block at element offset off[l]; hi[l] = bf16(W_l) as
+ * [rows][ld[l]] (columns beyond cols zero), lo[l] = bf16(W_l - hi[l]) in the
+ * same layout when lo[l] is not NULL (the rounding of neb_split_bf16), and
+ * wt[l] = bf16(W_l^T) as [cols][ldt[l]] (columns beyond rows zero) when wt[l]
+ * is not NULL.  At most 16 layers.  Replaces the per-update
+ * `.astype(compute_dtype)` copies of the weight stacks after Adam
+ * (HB2P:1026-1030, HB2P:2184-2278): one launch instead of one per form and
+ * layer. */
+int neb_stage_net(int master_dtype, const void* master, int nl, const int64_t* off, const int64_t* rows,
+                  const int64_t* cols, void* const* hi, void* const* lo, const int64_t* ld, void* const* wt,
+                  const int64_t* ldt, void* stream);
+
 /* Weight staging: dst[r][c] = scale * (transpose ? src[c][r] : src[r][c])
  * converted from src_dtype to dst_dtype, zero outside the source extent
  * (dst is drows x dcols).  Replaces the per-call `.astype(compute_dtype)`

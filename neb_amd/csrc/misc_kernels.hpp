@@ -197,6 +197,44 @@ __global__ __launch_bounds__(256) void split_kernel(const TS* __restrict__ src, 
   lo[r * ldd + c] = (__bf16)(float)(x - (double)(float)h);
 }
 
+// Every layer's bf16 operand forms in one launch (see neb_stage_net): a flat
+// index over, first, the [rows][ld] planes of all layers (hi, and lo when
+// requested), then the [cols][ldt] transposes; padding columns are zeros.
+constexpr int STAGE_NET_MAX = 16;
+struct StageNetArgs {
+  int nl;
+  int64_t off[STAGE_NET_MAX], rows[STAGE_NET_MAX], cols[STAGE_NET_MAX];
+  __bf16* hi[STAGE_NET_MAX];
+  __bf16* lo[STAGE_NET_MAX];
+  __bf16* wt[STAGE_NET_MAX];
+  int64_t ld[STAGE_NET_MAX], ldt[STAGE_NET_MAX];
+  int64_t start_w[STAGE_NET_MAX + 1], start_t[STAGE_NET_MAX + 1];
+};
+
+template <typename TS>
+__global__ __launch_bounds__(256) void stage_net_kernel(const TS* __restrict__ master, StageNetArgs a) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nw = a.start_w[a.nl];
+  if (idx < nw) {
+    int l = 0;
+    while (idx >= a.start_w[l + 1]) ++l;
+    const int64_t e = idx - a.start_w[l], r = e / a.ld[l], c = e % a.ld[l];
+    // the rounding of split_kernel: hi = bf16(x), lo = bf16(x - hi) with the
+    // difference taken in double
+    const double x = c < a.cols[l] ? (double)master[a.off[l] + r * a.cols[l] + c] : 0.0;
+    const __bf16 h = (__bf16)(float)x;
+    a.hi[l][e] = h;
+    if (a.lo[l]) a.lo[l][e] = (__bf16)(float)(x - (double)(float)h);
+    return;
+  }
+  const int64_t it = idx - nw;
+  if (it >= a.start_t[a.nl]) return;
+  int l = 0;
+  while (it >= a.start_t[l + 1]) ++l;
+  const int64_t e = it - a.start_t[l], i = e / a.ldt[l], o = e % a.ldt[l];
+  a.wt[l][e] = o < a.rows[l] ? (__bf16)(float)master[a.off[l] + o * a.cols[l] + i] : (__bf16)0.0f;
+}
+
 // Join of bf16 hi / lo planes into one fp32 or fp64 matrix: dst = hi + lo
 // (exact in fp32: lo holds the rounding residual of hi).  Each thread joins
 // eight consecutive columns of one row.

@@ -1022,6 +1022,41 @@ int neb_jet_output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const voi
                           layout, S(stream));
 }
 
+int neb_stage_net(int master_dtype, const void* master, int nl, const int64_t* off, const int64_t* rows,
+                  const int64_t* cols, void* const* hi, void* const* lo, const int64_t* ld, void* const* wt,
+                  const int64_t* ldt, void* stream) {
+  if (nl < 1 || nl > STAGE_NET_MAX) return fail(NEB_ERR_ARG, "neb_stage_net: 1..%d layers", STAGE_NET_MAX);
+  if (!master || !off || !rows || !cols || !hi || !lo || !ld || !wt || !ldt)
+    return fail(NEB_ERR_ARG, "neb_stage_net: null pointer");
+  StageNetArgs a;
+  memset(&a, 0, sizeof(a));
+  a.nl = nl;
+  for (int l = 0; l < nl; ++l) {
+    if (rows[l] < 1 || cols[l] < 1 || ld[l] < cols[l] || off[l] < 0 || !hi[l] || (wt[l] && ldt[l] < rows[l]))
+      return fail(NEB_ERR_SHAPE, "neb_stage_net: bad layer %d", l);
+    a.off[l] = off[l]; a.rows[l] = rows[l]; a.cols[l] = cols[l];
+    a.hi[l] = reinterpret_cast<__bf16*>(hi[l]);
+    a.lo[l] = reinterpret_cast<__bf16*>(lo[l]);
+    a.wt[l] = reinterpret_cast<__bf16*>(wt[l]);
+    a.ld[l] = ld[l]; a.ldt[l] = wt[l] ? ldt[l] : 1;
+    a.start_w[l + 1] = a.start_w[l] + rows[l] * ld[l];
+    a.start_t[l + 1] = a.start_t[l] + (wt[l] ? cols[l] * ldt[l] : 0);
+  }
+  const int64_t tot = a.start_w[nl] + a.start_t[nl];
+  if (tot / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_stage_net: too many elements");
+  const dim3 grid((unsigned)((tot + 255) / 256));
+  hipStream_t st = S(stream);
+  switch (master_dtype) {
+    case NEB_F32:
+      hipLaunchKernelGGL(stage_net_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<const float*>(master), a);
+      return hip_status("neb_stage_net");
+    case NEB_F64:
+      hipLaunchKernelGGL(stage_net_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<const double*>(master), a);
+      return hip_status("neb_stage_net");
+  }
+  return fail(NEB_ERR_ARG, "neb_stage_net: master dtype must be NEB_F32 or NEB_F64 (got %d)", master_dtype);
+}
+
 int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst,
               int64_t ldd, int64_t drows, int64_t dcols, int transpose, double scale, void* stream) {
   if (!src || !dst || rows < 0 || cols < 0 || drows < 0 || dcols < 0 || ldd < dcols)

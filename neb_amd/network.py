@@ -184,9 +184,38 @@ class EigenfluidNet:
         self._staged.clear()
 
     # -- staging to the MFMA operand precision ----------------------------
+    def _stage_bf16(self):
+        """Both bf16 staging caches in one launch (neb_stage_net): the split
+        planes W hi / lo [out][ld_in] of every layer (staged_split) and, for
+        staged(BF16), W = the hi planes and W^T [in][ld_out] of the hidden
+        layers."""
+        mdt = _DT[self.master_dtype]
+        L = len(self.W)
+        his, los, wts = [], [], []
+        off, rows, cols, ld, ldt = [], [], [], [], []
+        for l, W in enumerate(self.W):
+            fo, fi = W.shape
+            his.append(torch.empty((fo, _ld(fi)), dtype=torch.bfloat16, device=self.device))
+            los.append(torch.empty_like(his[-1]))
+            wts.append(torch.empty((fi, _ld(fo)), dtype=torch.bfloat16, device=self.device) if l < L - 1 else None)
+            off.append(self._offsets[l][0])
+            rows.append(fo)
+            cols.append(fi)
+            ld.append(_ld(fi))
+            ldt.append(_ld(fo) if l < L - 1 else 0)
+        ptrs = lambda ts: H.ptr_array([t.data_ptr() if t is not None else None for t in ts])  # noqa: E731
+        H.call("neb_stage_net", mdt, self.params.data_ptr(), L, H.i64_array(off), H.i64_array(rows), H.i64_array(cols),
+               ptrs(his), ptrs(los), H.i64_array(ld), ptrs(wts), H.i64_array(ldt), _stream())
+        bias = [b.to(torch.float32).contiguous() for b in self.b]
+        self._split_cache = (self.version, {"W": his, "Wlo": los, "b": bias})
+        self._staged[(H.BF16, self.version)] = {"W": his, "WT": wts[:-1], "b": bias}
+
     def staged(self, prec: int):
         key = (prec, self.version)
         if key in self._staged:
+            return self._staged[key]
+        if prec == H.BF16:
+            self._stage_bf16()
             return self._staged[key]
         st = _STORE[prec]
         mdt = _DT[self.master_dtype]
@@ -205,28 +234,16 @@ class EigenfluidNet:
                        wt.shape[1], 1, 1.0, s)
                 out["WT"].append(wt)
             out["b"].append(b.to(math_dt).contiguous())
-        self._staged = {key: out}
+        self._staged[key] = out
         return out
 
     def staged_split(self):
         """Master weights as bf16 hi / lo planes ([out][ld_in] each; hi is the
         bf16 staging of staged(BF16)) and fp32 biases, for the split-precision
         forward."""
-        if self._split_cache is not None and self._split_cache[0] == self.version:
-            return self._split_cache[1]
-        mdt = _DT[self.master_dtype]
-        s = _stream()
-        out = {"W": [], "Wlo": [], "b": []}
-        for W, b in zip(self.W, self.b):
-            fo, fi = W.shape
-            hi = torch.zeros((fo, _ld(fi)), dtype=torch.bfloat16, device=self.device)
-            lo = torch.zeros_like(hi)
-            H.call("neb_split_bf16", mdt, W.data_ptr(), fo, fi, fi, hi.data_ptr(), lo.data_ptr(), hi.shape[1], s)
-            out["W"].append(hi)
-            out["Wlo"].append(lo)
-            out["b"].append(b.to(torch.float32).contiguous())
-        self._split_cache = (self.version, out)
-        return out
+        if self._split_cache is None or self._split_cache[0] != self.version:
+            self._stage_bf16()
+        return self._split_cache[1]
 
     # -- optimizer ---------------------------------------------------------
     def adam(self, grads: torch.Tensor, lr: float, clip: float = 8.0, gate: torch.Tensor | None = None,

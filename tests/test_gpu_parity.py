@@ -876,6 +876,31 @@ def test_split_bf16_planes():
         assert float(err.max()) <= 2.0 ** -16, float(err.max())
 
 
+def test_stage_net_matches_per_layer_staging():
+    """neb_stage_net (every layer's hi / lo / transposed bf16 forms in one
+    launch) gives the bits of neb_split_bf16 and neb_stage layer by layer,
+    padding included, for fp32 and fp64 masters."""
+    for master in (torch.float32, torch.float64):
+        cfg, net, _ = make(2, 20, [48, 36], 3, master=master)
+        sp_ = net.staged_split()
+        st = net.staged(nw.PRECISIONS["bf16"])
+        s = nw._stream()
+        for l, W in enumerate(net.W):
+            fo, fi = W.shape
+            ld = sp_["W"][l].shape[1]
+            hi = torch.zeros((fo, ld), dtype=torch.bfloat16, device=DEV)
+            lo = torch.zeros_like(hi)
+            nw.H.call("neb_split_bf16", nw._DT[master], W.data_ptr(), fo, fi, fi, hi.data_ptr(), lo.data_ptr(), ld, s)
+            assert torch.equal(sp_["W"][l], hi) and torch.equal(sp_["Wlo"][l], lo), l
+            assert torch.equal(st["W"][l], hi), l
+            if l < len(net.W) - 1:
+                ldt = st["WT"][l].shape[1]
+                wt = torch.empty((fi, ldt), dtype=torch.bfloat16, device=DEV)
+                nw.H.call("neb_stage", nw._DT[master], nw.H.BF16, W.data_ptr(), fo, fi, fi, wt.data_ptr(), ldt, fi,
+                          ldt, 1, 1.0, s)
+                assert torch.equal(st["WT"][l], wt), l
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_split_precision_forward_matches_oracle(dim):
     """Split-precision jets (bf16 hi / lo planes, three bf16 MFMA products per
