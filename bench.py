@@ -116,7 +116,7 @@ def launch_work(name, a):
         else:
             dim, n, k, K = a[0], a[1], a[2], a[3]
             es_in, es_out = (4 if name.endswith("_x3") else 2), 4
-        nch, rows, terms = (4, 3, 4) if dim == 2 else (9, 12, 24)
+        nch, rows, terms = {2: (4, 3, 4), 3: (9, 12, 24), 4: (5, 5, 5)}[dim]
         return name, n * (nch * k * es_in + rows * K * es_out), 2 * terms * n * k * K
     if name == "neb_gram_split":
         rows, K = a[2], a[3]
@@ -126,8 +126,8 @@ def launch_work(name, a):
         return f"{name} rows={rows}", rows * K * _ES[dt], rows * K * (K + 1)
     if name == "neb_jet_output_backward":
         prec, dim, n, width, K, fdt = a[0], a[1], a[3], a[4], a[5], a[9]
-        C, es = (6 if dim == 2 else 10), _ES[prec]
-        rows, terms = (3, 4) if dim == 2 else (12, 24)
+        C, es = (10 if dim == 3 else 6), _ES[prec]
+        rows, terms = {2: (3, 3), 3: (12, 24), 4: (5, 5)}[dim]
         return name, n * (rows * K * _ES[fdt] + 2 * C * width * es), 2 * terms * n * width * K
     if name == "neb_jet_layer_backward":
         prec, dim, n, k, width = a[0], a[1], a[3], a[4], a[5]

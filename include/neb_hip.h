@@ -57,6 +57,12 @@ extern "C" {
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
 enum { NEB_TANH = 0, NEB_SIN = 1 };                           /* activations */
+/* lift codes of the output-layer entry points (their `dim` argument): the 2D
+ * velocity + vorticity lift, the 3D curl lift, and the 2D velocity + Hessian
+ * lift of the causal velocity peel (stiffness rows psi_xx, sqrt2 psi_xy,
+ * psi_yy, whose Gram is the Hessian Frobenius form with the pair metric
+ * (1, 2, 1) of HB2P:3022) */
+enum { NEB_LIFT_2D_VORTICITY = 2, NEB_LIFT_3D_CURL = 3, NEB_LIFT_2D_HESSIAN = 4 };
 enum { NEB_CROSS_F32 = 0, NEB_CROSS_BF16 = 1, NEB_CROSS_BF16X3 = 2, NEB_CROSS_F64 = 3 };
 
 int neb_abi_version(void);
@@ -93,8 +99,13 @@ int neb_jet_output_jets(int prec, int dim, int64_t n, int64_t k, int64_t nout, c
  * .../canonical_gauge/scripts/evaluate_large_comparison.py).  3D: w [3K][ldw]
  * with rows 3k+c (vector potential); mass = curl A as [3n][ldf], stiff =
  * grad u as [9n][ldf] rows 3a+j = d_j u_a (NCC:97-99, VJ2:93-101,
- * VCORE:71-77).  layout 0 = point-major rows (p*comp+c, the reference's
- * [N*comp, K] layout, VCORE:75-76); 1 = component-major (c*n+p). */
+ * VCORE:71-77).  dim = NEB_LIFT_2D_HESSIAN: 2D mass as above, stiff = the
+ * Hessian rows (psi_xx, sqrt2 psi_xy, psi_yy) as [3n][ldf] (the velocity
+ * Rayleigh peel's H1 seminorm, HB2P:2935-3060, metric (1, 2, 1) at
+ * HB2P:3022); the same lift codes apply to neb_jet_output_fields_split,
+ * neb_jet_output_fields_x3 and neb_jet_output_backward.  layout 0 =
+ * point-major rows (p*comp+c, the reference's [N*comp, K] layout,
+ * VCORE:75-76); 1 = component-major (c*n+p). */
 int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy,
                           const void* w, int64_t ldw, void* mass, void* stiff, int64_t ldf, int layout,
                           void* stream);

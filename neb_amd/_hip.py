@@ -17,6 +17,7 @@ ABI_VERSION = 7
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
+LIFT_2D_VORTICITY, LIFT_3D_CURL, LIFT_2D_HESSIAN = 2, 3, 4
 CROSS_F32, CROSS_BF16, CROSS_BF16X3, CROSS_F64 = 0, 1, 2, 3
 
 _i64, _dbl, _int, _vp, _sz = C.c_int64, C.c_double, C.c_int, C.c_void_p, C.c_size_t

@@ -579,6 +579,57 @@ struct PlanOutFields2Split {
   }
 };
 
+// 2D output layer with the velocity lift and the Hessian stiffness of the
+// causal velocity peel (HB2P:2935-3060): mass fields u = psi_y, v = -psi_x;
+// stiffness rows (psi_xx, sqrt2 psi_xy, psi_yy), whose Gram is the Hessian
+// Frobenius form with the pair metric (1, 2, 1) (HB2P:3022).
+// Inputs: x[0]=gx, x[1]=gy, x[2]=hxx, x[3]=hxy, x[4]=hyy.
+constexpr float SQRT2F = 1.41421356237309504880f;
+template <bool SPLIT, bool STREAM = false> struct PlanOutFields2HT {
+  template <int P> __host__ __device__ static constexpr int out_bytes() {
+    return SPLIT ? 2 : sizeof(typename Prec<P>::math_t);
+  }
+  static constexpr int CI = 5, NW = 1, NA = 5, NT = 5;
+  static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
+  static constexpr int ta[NT] = {0, 1, 2, 3, 4};
+  static constexpr int tc[NT] = {1, 0, 2, 3, 4};
+  static constexpr int tw[NT] = {0, 0, 0, 0, 0};
+  static constexpr bool tneg[NT] = {false, true, false, false, false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    const int64_t r0 = a.layout == 0 ? 2 * p : p;
+    const int64_t r1 = a.layout == 0 ? 2 * p + 1 : a.npts + p;
+    st_field<SPLIT, GE, M, STREAM>(a.out0, a.out0lo, r0 * a.ldo + f, v[0], nv);
+    st_field<SPLIT, GE, M, STREAM>(a.out0, a.out0lo, r1 * a.ldo + f, v[1], nv);
+#pragma unroll
+    for (int e = 0; e < GE; ++e) v[3][e] *= (M)(P == PREC_F64 ? 1.41421356237309504880 : SQRT2F);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int64_t row = a.layout == 0 ? 3 * p + r : r * a.npts + p;
+      st_field<SPLIT, GE, M, STREAM>(a.out1, a.out1lo, row * a.ldo2 + f, v[2 + r], nv);
+    }
+  }
+};
+
+// The same on split operands: x[0..4] hi planes of gx, gy, hxx, hxy, hyy,
+// x[5..9] their lo planes; w[0] = W_hi, w[1] = W_lo; three products per base
+// term (W_lo x_hi, W_hi x_lo, W_hi x_hi), hi / lo field planes out.
+struct PlanOutFields2HSplit {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
+  using B = PlanOutFields2HT<true, true>;
+  static constexpr int CI = 10, NW = 2, NA = 5, NT = 15;
+  static constexpr int FB = B::FB, PB = 1, WPN = B::WPN, WPP = B::WPP;
+  static constexpr int ta[NT] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4};
+  static constexpr int tc[NT] = {1, 6, 1, 0, 5, 0, 2, 7, 2, 3, 8, 3, 4, 9, 4};
+  static constexpr int tw[NT] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0};
+  static constexpr bool tneg[NT] = {false, false, false, true, true, true, false, false, false,
+                                    false, false, false, false, false, false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    B::template epilogue<P, GE, M>(a, p, f, nv, v);
+  }
+};
+
 // 3D output layer with the curl lift (NCC:97-99, VJ2:93-101).  Vector
 // potential A has output rows 3k+c, passed as three weight slices W_c.
 // Inputs x[0..2] = grad channels g0..g2, x[3..8] = Hessian pairs h0..h5.
@@ -674,6 +725,7 @@ constexpr bool split_terms_match() {
 struct IdentityTerms10 { IDENTITY_TERMS };
 static_assert(split_terms_match<PlanOutFields3Split, 24, 9, 3, PlanOutFields3T<true>>(), "3D split field terms");
 static_assert(split_terms_match<PlanOutFields2Split, 4, 4, 1, PlanOutFields2T<true>>(), "2D split field terms");
+static_assert(split_terms_match<PlanOutFields2HSplit, 5, 5, 1, PlanOutFields2HT<true>>(), "2D Hessian split terms");
 static_assert(split_terms_match<HiddenSplitTerms<2>, 6, 6, 1, IdentityTerms10>(), "2D split hidden terms");
 static_assert(split_terms_match<HiddenSplitTerms<3>, 10, 10, 1, IdentityTerms10>(), "3D split hidden terms");
 
@@ -754,6 +806,32 @@ template <int ACT> struct PlanOutBwd2 {
     for (int e = 0; e < GE; ++e) {
       adj[0][e] = (M)0; adj[1][e] = v[0][e]; adj[2][e] = v[1][e];
       adj[3][e] = v[2][e]; adj[4][e] = (M)0; adj[5][e] = v[2][e];
+    }
+    reverse_epilogue<P, 2, ACT, GE, M>(a, p, f, nv, adj);
+  }
+};
+
+// 2D output-layer reverse for the Hessian stiffness (the transpose of
+// PlanOutFields2HT): x[0]=u rows, x[1]=v rows, x[2..4] = the stiffness rows
+// (psi_xx, sqrt2 psi_xy, psi_yy); w[0] = Bm, w[1] = Bs.
+//   adj gx = -(v . Bm), adj gy = u . Bm,
+//   adj hxx = s0 . Bs, adj hxy = sqrt2 (s1 . Bs), adj hyy = s2 . Bs
+template <int ACT> struct PlanOutBwd2H {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::store_t); }
+  static constexpr int CI = 5, NW = 2, NA = 5, NT = 5;
+  static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
+  static constexpr int ta[NT] = {1, 0, 2, 3, 4};
+  static constexpr int tc[NT] = {0, 1, 2, 3, 4};
+  static constexpr int tw[NT] = {0, 0, 1, 1, 1};
+  static constexpr bool tneg[NT] = {false, true, false, false, false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    M adj[6][GE];
+    const M r2 = (M)(P == PREC_F64 ? 1.41421356237309504880 : SQRT2F);
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      adj[0][e] = (M)0; adj[1][e] = v[0][e]; adj[2][e] = v[1][e];
+      adj[3][e] = v[2][e]; adj[4][e] = r2 * v[3][e]; adj[5][e] = v[4][e];
     }
     reverse_epilogue<P, 2, ACT, GE, M>(a, p, f, nv, adj);
   }

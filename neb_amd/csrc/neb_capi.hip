@@ -106,6 +106,13 @@ int check_common(int prec, int dim, int act) {
   return NEB_OK;
 }
 
+// The lift argument of the output-layer entry points: 2 (2D velocity +
+// vorticity), 3 (3D curl), NEB_LIFT_2D_HESSIAN (2D velocity + Hessian rows).
+int check_lift(int prec, int lift, int act) {
+  if (lift == NEB_LIFT_2D_HESSIAN) return check_common(prec, 2, act);
+  return check_common(prec, lift, act);
+}
+
 // --------------------------------------------------------------------------
 template <int P, int D, int ACT>
 int layer_forward(int64_t n, int64_t k, int64_t width, const void* x, int64_t ldx, const void* w, int64_t ldw,
@@ -153,6 +160,12 @@ int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64
   a.out0 = mass; a.out1 = stiff; a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
   a.out0lo = mass_lo; a.out1lo = stiff_lo;
   const char* what = SPLIT ? "neb_jet_output_fields_split" : "neb_jet_output_fields";
+  if (dim == NEB_LIFT_2D_HESSIAN) {
+    // channels gx, gy, hxx, hxy, hyy
+    for (int c = 0; c < 5; ++c) { a.x[c] = Y + (1 + c) * cs; a.ldx[c] = ldy; }
+    a.w[0] = w; a.ldw[0] = ldw;
+    return launch_rowgemm<P, Sx, PlanOutFields2HT<SPLIT, SPLIT>>(a, st, what);
+  }
   if (dim == 2) {
     // channels: 0 v, 1 gx, 2 gy, 3 hxx, 4 hxy, 5 hyy
     const int chans[4] = {1, 2, 3, 5};
@@ -199,6 +212,15 @@ int output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi,
   a.npts = n; a.nout = K; a.k = k;
   a.out0 = mass_hi; a.out0lo = mass_lo; a.out1 = stiff_hi; a.out1lo = stiff_lo;
   a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
+  if (dim == NEB_LIFT_2D_HESSIAN) {
+    for (int c = 0; c < 5; ++c) {
+      a.x[c] = Yh + (1 + c) * cs;
+      a.x[5 + c] = Yl + (1 + c) * cs;
+      a.ldx[c] = a.ldx[5 + c] = ldy;
+    }
+    a.w[0] = w_hi; a.w[1] = w_lo; a.ldw[0] = a.ldw[1] = ldw;
+    return launch_rowgemm<PREC_BF16, __bf16, PlanOutFields2HSplit>(a, st, "neb_jet_output_fields_x3");
+  }
   if (dim == 2) {
     const int chans[4] = {1, 2, 3, 5};  // gx, gy, hxx, hyy
     for (int c = 0; c < 4; ++c) {
@@ -235,6 +257,17 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
   a.tape = z; a.ldt = ldz; a.tstride = n * ldz;
   a.omega = omega;
   a.out0 = adj; a.ldo = lda; a.cstride = n * lda;
+  if (dim == NEB_LIFT_2D_HESSIAN) {
+    a.x[0] = Mf; a.ldx[0] = layout == 0 ? 2 * ldf : ldf;
+    a.x[1] = layout == 0 ? Mf + ldf : Mf + n * ldf; a.ldx[1] = a.ldx[0];
+    for (int r = 0; r < 3; ++r) {
+      a.x[2 + r] = layout == 0 ? Sf + r * ldf : Sf + r * n * ldf;
+      a.ldx[2 + r] = layout == 0 ? 3 * ldf : ldf;
+    }
+    a.w[0] = bm; a.ldw[0] = ldb;
+    a.w[1] = bs; a.ldw[1] = ldb;
+    return launch_rowgemm<P, F, PlanOutBwd2H<ACT>>(a, st, "neb_jet_output_backward");
+  }
   if (dim == 2) {
     if (layout == 0) {
       a.x[0] = Mf; a.ldx[0] = 2 * ldf;
@@ -624,12 +657,13 @@ int neb_jet_output_jets(int prec, int dim, int64_t n, int64_t k, int64_t nout, c
 int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy,
                           const void* w, int64_t ldw, void* mass, void* stiff, int64_t ldf, int layout,
                           void* stream) {
-  if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
+  if (int rc = check_lift(prec, dim, NEB_TANH)) return rc;
   if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_fields: bad sizes");
   if (n == 0) return NEB_OK;
   if (!y || !w || !mass || !stiff) return fail(NEB_ERR_ARG, "neb_jet_output_fields: null pointer");
-  if (!aligned(y, ldy, prec) || !aligned(w, ldw, prec))
+  const int fdt = prec == NEB_F64 ? NEB_F64 : NEB_F32;
+  if (!aligned(y, ldy, prec) || !aligned(w, ldw, prec) || !aligned(mass, ldf, fdt) || !aligned(stiff, ldf, fdt))
     return fail(NEB_ERR_ARG, "neb_jet_output_fields: rows must be 16-byte aligned");
   hipStream_t st = S(stream);
   if (prec == NEB_BF16) return output_fields<PREC_BF16>(dim, n, k, K, y, ldy, w, ldw, mass, stiff, ldf, layout, st);
@@ -640,7 +674,7 @@ int neb_jet_output_fields(int prec, int dim, int64_t n, int64_t k, int64_t K, co
 int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64_t ldy, const void* w,
                                 int64_t ldw, void* mass_hi, void* mass_lo, void* stiff_hi, void* stiff_lo,
                                 int64_t ldf, int layout, void* stream) {
-  if (int rc = check_common(NEB_BF16, dim, NEB_TANH)) return rc;
+  if (int rc = check_lift(NEB_BF16, dim, NEB_TANH)) return rc;
   if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_fields_split: bad sizes");
   if (n == 0) return NEB_OK;
@@ -657,7 +691,7 @@ int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width
                             const void* stiff, int64_t ldf, int fields_dtype, int layout, const void* bm,
                             const void* bs, int64_t ldb, const void* z, int64_t ldz, double omega, void* adj,
                             int64_t lda, void* stream) {
-  if (int rc = check_common(prec, dim, act)) return rc;
+  if (int rc = check_lift(prec, dim, act)) return rc;
   if (n < 0 || width < 1 || K < 1 || ldf < K || ldb < K || ldz < width || lda < width ||
       (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_backward: bad sizes");
@@ -667,7 +701,7 @@ int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width
                                    : (fields_dtype == NEB_F32 || (prec == NEB_BF16 && fields_dtype == NEB_BF16));
   if (!fok) return fail(NEB_ERR_ARG, "neb_jet_output_backward: fields dtype %d does not fit prec %d", fields_dtype, prec);
   if (!aligned(mass, ldf, fields_dtype) || !aligned(stiff, ldf, fields_dtype) || !aligned(bm, ldb, prec) ||
-      !aligned(bs, ldb, prec))
+      !aligned(bs, ldb, prec) || !aligned(z, ldz, prec) || !aligned(adj, lda, prec))
     return fail(NEB_ERR_ARG, "neb_jet_output_backward: rows must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define OB(P, A, F) \
@@ -690,7 +724,8 @@ int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_backward: bad sizes");
   if (n == 0) return NEB_OK;
   if (!adj_in || !wt || !z_prev || !adj_out) return fail(NEB_ERR_ARG, "neb_jet_layer_backward: null pointer");
-  if (!aligned(adj_in, lda_in, prec) || !aligned(wt, ldw, prec))
+  if (!aligned(adj_in, lda_in, prec) || !aligned(wt, ldw, prec) || !aligned(z_prev, ldz, prec) ||
+      !aligned(adj_out, lda_out, prec))
     return fail(NEB_ERR_ARG, "neb_jet_layer_backward: rows must be 16-byte aligned");
 #define CALL(P, D, A) \
   layer_backward<P, D, A>(n, k, width, adj_in, lda_in, wt, ldw, z_prev, ldz, omega_prev, adj_out, lda_out, S(stream))
@@ -1008,7 +1043,7 @@ int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t wid
 int neb_jet_output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo,
                              int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
                              void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, int layout, void* stream) {
-  if (int rc = check_common(NEB_BF16, dim, NEB_TANH)) return rc;
+  if (int rc = check_lift(NEB_BF16, dim, NEB_TANH)) return rc;
   if (n < 0 || k < 1 || K < 1 || ldy < k || ldw < k || ldf < K || (layout != 0 && layout != 1))
     return fail(NEB_ERR_SHAPE, "neb_jet_output_fields_x3: bad sizes");
   if (n == 0) return NEB_OK;

@@ -61,6 +61,11 @@ class NetConfig:
     modes: int = 64
     activation: str = "tanh"
     omegas: tuple | None = None
+    # stiffness fields of the 2D lift: "vorticity" (omega = -lap psi, one row
+    # per point) or "hessian" (psi_xx, sqrt2 psi_xy, psi_yy: the H1 seminorm
+    # of the velocity with the pair metric (1, 2, 1) of the causal velocity
+    # peel, HB2P:2935-3060, :3022); 3D always uses grad u (nine rows)
+    stiffness: str = "vorticity"
 
     @property
     def features(self) -> int:
@@ -73,6 +78,24 @@ class NetConfig:
     @property
     def sizes(self) -> list[int]:
         return [self.features, *self.widths, self.ncomp * self.modes]
+
+    @property
+    def lift(self) -> int:
+        """Lift code of the output-layer entry points (their dim argument)."""
+        if self.dim == 3:
+            return H.LIFT_3D_CURL
+        if self.stiffness == "hessian":
+            return H.LIFT_2D_HESSIAN
+        if self.stiffness != "vorticity":
+            raise ValueError(f"unknown 2D stiffness {self.stiffness!r}")
+        return H.LIFT_2D_VORTICITY
+
+    @property
+    def field_rows(self) -> tuple:
+        """(mass rows, stiffness rows) per point of the lifted fields."""
+        if self.dim == 3:
+            return 3, 9
+        return (2, 3) if self.stiffness == "hessian" else (2, 1)
 
     @property
     def act_code(self) -> int:
@@ -476,7 +499,7 @@ def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0, split: bool =
     prec = tape.prec
     Y = tape.inputs[-1]
     n, k, K = tape.n, cfg.widths[-1], cfg.modes
-    cm, cs = (2, 1) if cfg.dim == 2 else (3, 9)
+    cm, cs = cfg.field_rows
     if tape.inputs_lo is not None:
         # split-precision tape: output layer + lift on split operands
         if not split:
@@ -484,7 +507,7 @@ def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0, split: bool =
         sw = net.staged_split()
         planes = [torch.empty((r * n, _ld(K)), dtype=torch.bfloat16, device=net.device)
                   for r in (cm, cm, cs, cs)]
-        H.call("neb_jet_output_fields_x3", cfg.dim, n, k, K, Y.data_ptr(), tape.inputs_lo[-1].data_ptr(), Y.shape[2],
+        H.call("neb_jet_output_fields_x3", cfg.lift, n, k, K, Y.data_ptr(), tape.inputs_lo[-1].data_ptr(), Y.shape[2],
                sw["W"][-1].data_ptr(), sw["Wlo"][-1].data_ptr(), sw["W"][-1].shape[1],
                *[t.data_ptr() for t in planes], planes[0].shape[1], layout, _stream())
         return Fields(planes[0], planes[2], K, layout, planes[1], planes[3])
@@ -494,14 +517,14 @@ def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0, split: bool =
             raise ValueError("split fields need a bf16 tape")
         planes = [torch.empty((r * n, _ld(K)), dtype=torch.bfloat16, device=net.device)
                   for r in (cm, cm, cs, cs)]
-        H.call("neb_jet_output_fields_split", cfg.dim, n, k, K, Y.data_ptr(), Y.shape[2],
+        H.call("neb_jet_output_fields_split", cfg.lift, n, k, K, Y.data_ptr(), Y.shape[2],
                stg["W"][-1].data_ptr(), stg["W"][-1].shape[1], *[t.data_ptr() for t in planes],
                planes[0].shape[1], layout, _stream())
         return Fields(planes[0], planes[2], K, layout, planes[1], planes[3])
     ft = _FIELD[prec]
     mass = torch.empty((cm * n, _ld(K)), dtype=ft, device=net.device)
     stiff = torch.empty((cs * n, _ld(K)), dtype=ft, device=net.device)
-    H.call("neb_jet_output_fields", prec, cfg.dim, n, k, K, Y.data_ptr(), Y.shape[2], stg["W"][-1].data_ptr(),
+    H.call("neb_jet_output_fields", prec, cfg.lift, n, k, K, Y.data_ptr(), Y.shape[2], stg["W"][-1].data_ptr(),
            stg["W"][-1].shape[1], mass.data_ptr(), stiff.data_ptr(), mass.shape[1], layout, _stream())
     return Fields(mass, stiff, K, layout)
 
@@ -735,7 +758,7 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
         side.wait_stream(main)
     A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
     Z = tape.pre[-1]
-    H.call("neb_jet_output_backward", prec, d, cfg.act_code, n, width, K, fields.mass.data_ptr(),
+    H.call("neb_jet_output_backward", prec, cfg.lift, cfg.act_code, n, width, K, fields.mass.data_ptr(),
            fields.stiff.data_ptr(), ldK, _DT[fields.mass.dtype], fields.layout, bm.data_ptr(), bs.data_ptr(),
            bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
     ctx = torch.cuda.stream(side) if on_gpu else nullcontext()
@@ -803,10 +826,17 @@ def _products(net: EigenfluidNet, tape: Tape, fields: Fields) -> list:
         raise ValueError("the output-layer products expect point-major field matrices")
     if d == 2:
         Hm = cross([(ch(Ylast, 1), ch(Ylast, 1), n, 1.0), (ch(Ylast, 2), ch(Ylast, 2), n, 1.0)], width, width, mode)
-        Hs = cross([(ch(Ylast, 3), ch(Ylast, 3), n, 1.0), (ch(Ylast, 5), ch(Ylast, 5), n, 1.0)], width, width, mode)
-        G35 = cross([(ch(Ylast, 3), ch(Ylast, 5), n, 1.0)], width, width, mode)
-        Hs += G35
-        Hs += G35.T
+        if cfg.stiffness == "hessian":
+            # rows (psi_xx, sqrt2 psi_xy, psi_yy): P_s = W (Y_xx^T Y_xx + 2 Y_xy^T Y_xy + Y_yy^T Y_yy)
+            Hs = cross([(ch(Ylast, 3), ch(Ylast, 3), n, 1.0), (ch(Ylast, 4), ch(Ylast, 4), n, 1.0),
+                        (ch(Ylast, 4), ch(Ylast, 4), n, 1.0), (ch(Ylast, 5), ch(Ylast, 5), n, 1.0)],
+                       width, width, mode)
+        else:
+            Hs = cross([(ch(Ylast, 3), ch(Ylast, 3), n, 1.0), (ch(Ylast, 5), ch(Ylast, 5), n, 1.0)], width, width,
+                       mode)
+            G35 = cross([(ch(Ylast, 3), ch(Ylast, 5), n, 1.0)], width, width, mode)
+            Hs += G35
+            Hs += G35.T
         Wd = net.W[-1].to(torch.float64)
         P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
         torch.mm(Wd, Hm, out=P[:K])

@@ -232,7 +232,7 @@ def _raise_peel_gate(value: int):
 
 
 def coefficient_peel(M: torch.Tensor, S: torch.Tensor, ortho_tol: float = 1e-5, rel_min: float = 1e-4,
-                     deferred: bool = False) -> dict:
+                     deferred: bool = False, order: str = "rayleigh", mass_floor: float = 0.0) -> dict:
     """Sorted Cholesky coefficient peel (MB/velocity/core.py:48-69) in FP64
     on the GPU.
 
@@ -246,6 +246,10 @@ def coefficient_peel(M: torch.Tensor, S: torch.Tensor, ortho_tol: float = 1e-5, 
     into `gate` (one int32: 0 = passed, PEEL_GATE = failed); deferred=False
     synchronises and raises FloatingPointError, deferred=True leaves the
     check to EigenfluidNet.adam(gate=..., on_gate=raise_peel_gate).
+
+    order="fixed" keeps the network order (HB2P:2999-3004 "fixed_order");
+    mass_floor adds the 2D peel's gates: every raw and residual mass above
+    it (HB2P:3137-3153).
 
     Adjoint (core.py:64-67): with every mode's earlier peeled modes held
     constant, the field cotangents of sum_r loss_r are G_U = w U Cu and
@@ -262,7 +266,12 @@ def coefficient_peel(M: torch.Tensor, S: torch.Tensor, ortho_tol: float = 1e-5, 
     S = S.to(torch.float64)
     eye = _eye(K, dev)
     raw = torch.diagonal(S) / torch.diagonal(M)
-    order = torch.argsort(raw, stable=True)
+    if order == "rayleigh":
+        order = torch.argsort(raw, stable=True)
+    elif order == "fixed":
+        order = torch.arange(K, device=dev)
+    else:
+        raise ValueError(f"unknown peel order {order!r}")
     ms = M[order][:, order]
     ss = S[order][:, order]
     scale = torch.diagonal(ms).sqrt()
@@ -277,7 +286,8 @@ def coefficient_peel(M: torch.Tensor, S: torch.Tensor, ortho_tol: float = 1e-5, 
     F = T.mT @ ss @ T
     losses = torch.diagonal(F).clone()
     ok = (torch.isfinite(losses).all() & (losses > 0).all() & (relative >= rel_min).all()
-          & (error <= ortho_tol) & (info1 == 0) & (info2 == 0))
+          & (error <= ortho_tol) & (info1 == 0) & (info2 == 0) & (norm * norm > mass_floor).all()
+          & (torch.diagonal(M) > mass_floor).all() & torch.isfinite(raw).all())
     gate = torch.where(ok, 0, PEEL_GATE).to(torch.int32).reshape(1)
     C = -2.0 * (T * losses[None, :] + T @ torch.triu(F, 1)) / norm[None, :]
     B = 2.0 * T / norm[None, :]
@@ -358,7 +368,7 @@ def timed(record: dict, name: str):
 def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
             gram_mode: str = "bf16x3", lam: float = 1e-2, ridge: float = 0.0, training: bool = True,
             gate: str = "eig", timings: dict | None = None, objective: str = "trace",
-            reverse_prec: str | None = None):
+            reverse_prec: str | None = None, peel_options: dict | None = None):
     """One spectral step (the analogue of MB/velocity/core.py:79-105
     ``compute``): jets -> lifted fields -> FP64 Grams -> spectral objective ->
     fused reverse.  objective="trace" is tr((M + ridge I)^-1 S) +
@@ -368,8 +378,9 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     applies).  reverse_prec (default: prec) runs the reverse pass
     in another operand precision on the same fields and a converted tape:
     prec="f32", gram_mode="bf16x3", reverse_prec="bf16" keeps the fields and
-    Grams at FP32 accuracy and runs the reverse on BF16 MFMA.  Returns
-    dict(loss, M, S, grads, fields)."""
+    Grams at FP32 accuracy and runs the reverse on BF16 MFMA; peel_options
+    are keyword arguments of coefficient_peel.  Returns dict(loss, M, S,
+    grads, fields)."""
     if objective not in ("trace", "peel"):
         raise ValueError(f"unknown objective {objective!r}")
     p = nw.PRECISIONS[prec]
@@ -387,7 +398,7 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     prod = nw.output_products(net, rtape, fields) if training else None
     with timed(tt, "spectral_objective"):
         if objective == "peel":
-            peel = coefficient_peel(M, S)
+            peel = coefficient_peel(M, S, **(peel_options or {}))
             loss, dM, dS = peel["loss"], peel["pair"][:, :K], peel["pair"][:, K:]
         else:
             loss, dM, dS = spectral_objective(M, S, lam, ridge, gate=gate)
@@ -397,6 +408,61 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     if training:
         with timed(tt, "analytic_backward"):
             out["grads"] = nw.backward_fields(net, rtape, fields, dM, dS, weight, symmetric=True, products=prod)
+    return out
+
+
+def velocity_peel_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16x3",
+                       training: bool = True, causal_sorting: str = "geometry_rayleigh",
+                       relative_mass_floor: float = 1e-4, overlap_tolerance: float = 1e-3,
+                       mass_floor: float = 1e-12, lr: float | None = None, clip: float = 8.0) -> dict:
+    """The 2D causal velocity Rayleigh peel step (HB2P:2935-3380
+    ``_velocity_rayleigh_peel_step``) on the shared trunk.
+
+    Velocity u = (psi_y, -psi_x); mass M = w sum |grad psi|^2 and stiffness
+    S = w sum (psi_xx^2 + 2 psi_xy^2 + psi_yy^2) as Grams of the lifted
+    fields (NetConfig(stiffness="hessian")); the two-pass modified
+    Gram-Schmidt peel in the velocity mass metric, every pass's coefficients
+    applied to the Hessian rows too, is the sorted Cholesky coefficient peel
+    of (M, S) (coefficient_peel; its equality with the literal two-pass peel
+    is pinned by the oracle), with the candidates sorted by raw S/M
+    ("geometry_rayleigh") or in network order ("fixed_order").  Gates (all on
+    the device, a failed one raises FloatingPointError before any update):
+    raw and residual masses above mass_floor, relative residual mass >=
+    relative_mass_floor, post-peel overlap <= overlap_tolerance, finite
+    values.  The gradient is that of the sum of the Rayleigh quotients with
+    the earlier (detached) modes held constant; with lr given, one gated Adam
+    update follows.  Returns the reference's diagnostics: loss = mean
+    Rayleigh quotient, mode_energies (causal order), mode_energies_by_network,
+    causal_order, min_relative_residual_mass, max_post_peel_overlap,
+    orth_error, peeled_rank, plus grads (training) and the Grams."""
+    if net.cfg.dim != 2 or net.cfg.stiffness != "hessian":
+        raise ValueError("the velocity Rayleigh peel needs a 2D net with NetConfig(stiffness='hessian')")
+    if causal_sorting not in ("fixed_order", "geometry_rayleigh"):
+        raise ValueError(f"unsupported causal_sorting {causal_sorting!r}")
+    if not 0.0 < relative_mass_floor < 1.0 or not 0.0 < overlap_tolerance < 1.0:
+        raise ValueError("relative_mass_floor and overlap_tolerance must be in (0, 1)")
+    if points.shape[0] == 0:
+        raise ValueError("velocity Rayleigh received no quadrature points")
+    opts = dict(ortho_tol=overlap_tolerance, rel_min=relative_mass_floor, mass_floor=mass_floor,
+                order="rayleigh" if causal_sorting == "geometry_rayleigh" else "fixed", deferred=True)
+    res = compute(net, points, weight, prec=prec, gram_mode="f64" if prec == "f64" else "bf16x3",
+                  training=training, objective="peel", peel_options=opts)
+    pk = res["peel"]
+    K = net.cfg.modes
+    if training and lr is not None:
+        net.adam(res["grads"], lr, clip=clip, gate=pk["gate"], on_gate=_raise_peel_gate)
+    else:
+        _raise_peel_gate(int(pk["gate"].item()))
+    ray = pk["losses"]
+    by_network = torch.empty_like(ray)
+    by_network[pk["order"]] = ray
+    out = dict(loss=float(ray.mean()), velocity_rayleigh=float(ray.mean()), mode_energies=ray,
+               mode_energies_by_network=by_network, causal_order=pk["order"], causal_sorting=causal_sorting,
+               min_relative_residual_mass=float(pk["relative_mass"].min()),
+               max_post_peel_overlap=float(pk["gram_max"]), orth_error=float(pk["gram_max"]), peeled_rank=K,
+               requested_modes=K, peeling_passes_used=2, M=res["M"], S=res["S"], peel=pk)
+    if training:
+        out["grads"] = res["grads"]
     return out
 
 

@@ -248,6 +248,79 @@ def lift2d_reverse(gu: np.ndarray, gw: np.ndarray) -> Jet:
     return Jet(np.zeros((n, K)), grad, hess)
 
 
+def lift2d_hessian(out: Jet):
+    """psi jets [n, K] -> (mass fields [n, 2, K] = (psi_y, -psi_x), Hessian
+    rows [n, 3, K] = (psi_xx, sqrt2 psi_xy, psi_yy)): the velocity mass
+    integral psi_x^2 + psi_y^2 and the H1 seminorm
+    psi_xx^2 + 2 psi_xy^2 + psi_yy^2 of the causal velocity peel
+    (HB2P:2946-2950, the pair metric (1, 2, 1) at HB2P:3022) as Grams of
+    these rows."""
+    u = np.stack((out.grad[..., 1], -out.grad[..., 0]), 1)
+    h = np.stack((out.hess[..., 0], math.sqrt(2.0) * out.hess[..., 1], out.hess[..., 2]), 1)
+    return u, h
+
+
+def lift2d_hessian_reverse(gu: np.ndarray, gh: np.ndarray) -> Jet:
+    n, _, K = gu.shape
+    grad = np.stack((-gu[:, 1], gu[:, 0]), -1)
+    hess = np.stack((gh[:, 0], math.sqrt(2.0) * gh[:, 1], gh[:, 2]), -1)
+    return Jet(np.zeros((n, K)), grad, hess)
+
+
+def velocity_rayleigh_peel(grad: np.ndarray, hess: np.ndarray, w: float, order: str = "rayleigh",
+                           passes: int = 2, mass_floor: float = 1e-12):
+    """The 2D causal velocity peel, mode by mode (HB2P:3017-3134): raw
+    gradient rows [2n, K] (psi_x, psi_y per point), raw Hessian rows [3n, K]
+    (psi_xx, psi_xy, psi_yy per point, metric (1, 2, 1)); modes in the stable
+    order of raw K/M ("rayleigh") or in network order ("fixed"); each
+    candidate's gradient column orthogonalised `passes` times against the
+    peeled (normalised) modes in the velocity mass metric, its Hessian column
+    with the same coefficients; Rayleigh quotient s / max(m, mass_floor);
+    the cotangents of each mode's quotient with the earlier modes held
+    constant, reversed through the projections (HB2P:3098-3116).  Returns
+    dict(rayleigh (causal order), mass, relative_mass, orth, order,
+    G_grad, G_hess (cotangents [2n, K] / [3n, K] in network order), raw_mass,
+    raw_stiffness)."""
+    n = grad.shape[0] // 2
+    K = grad.shape[1]
+    metric = np.tile(np.array([1.0, 2.0, 1.0]), n)
+    raw_mass = w * np.sum(grad * grad, 0)
+    raw_stiff = w * np.sum(metric[:, None] * hess * hess, 0)
+    causal = np.argsort(raw_stiff / raw_mass, kind="stable") if order == "rayleigh" else np.arange(K)
+    Qg = np.empty_like(grad)
+    Qh = np.empty_like(hess)
+    Gg = np.zeros_like(grad)
+    Gh = np.zeros_like(hess)
+    ray, mass, rel, orth = (np.empty(K) for _ in range(4))
+    for r, net in enumerate(causal):
+        rg = grad[:, net].copy()
+        rh = hess[:, net].copy()
+        pg, ph = Qg[:, :r], Qh[:, :r]
+        for _ in range(passes if r else 0):
+            inc = w * (pg.T @ rg)
+            rg -= pg @ inc
+            rh -= ph @ inc
+        m = w * float(rg @ rg)
+        st = w * float((metric * rh) @ rh)
+        safe = max(m, mass_floor)
+        ray[r], mass[r], rel[r] = st / safe, m, m / raw_mass[net]
+        ag = (-2.0 * st * w / (safe * safe)) * rg
+        ah = (2.0 * w / safe) * metric * rh
+        for _ in range(passes if r else 0):
+            ca = -(pg.T @ ag + ph.T @ ah)
+            ag = ag + w * (pg @ ca)
+        Gg[:, net] = ag
+        Gh[:, net] = ah
+        qg, qh = rg / math.sqrt(safe), rh / math.sqrt(safe)
+        Qg[:, r], Qh[:, r] = qg, qh
+        o = abs(w * float(qg @ qg) - 1.0)
+        if r:
+            o = max(o, float(np.max(np.abs(w * (pg.T @ qg)))))
+        orth[r] = o
+    return dict(rayleigh=ray, mass=mass, relative_mass=rel, orth=orth, order=causal, G_grad=Gg, G_hess=Gh,
+                raw_mass=raw_mass, raw_stiffness=raw_stiff)
+
+
 def _hid(a: int, b: int) -> int:
     return pairs(3).index(tuple(sorted((a, b))))
 

@@ -421,3 +421,108 @@ def test_coefficient_peel_fourier_closed_form_and_gates():
     Dd = np.column_stack((D, D[:, 2]))
     with pytest.raises((FloatingPointError, np.linalg.LinAlgError)):
         o.coefficient_peel(o.gram(Ud, w), o.gram(Dd, w))
+
+
+# --------------------------------------------------------------------------
+# 2D causal velocity Rayleigh peel (HB2P:2935-3380)
+# --------------------------------------------------------------------------
+def _velocity_rows(out):
+    n, K = out.value.shape
+    grad = out.grad.transpose(0, 2, 1).reshape(2 * n, K)      # rows 2p + (x, y)
+    hess = out.hess.transpose(0, 2, 1).reshape(3 * n, K)      # rows 3p + (xx, xy, yy)
+    return grad, hess
+
+
+def test_lift2d_hessian_metric_and_adjoint():
+    """The Hessian rows' Gram is the full symmetric Hessian's Frobenius Gram
+    (every entry, the xy pair twice), and the reverse is the transpose."""
+    rng = np.random.default_rng(31)
+    net = _net(2, 2, [12], 5, seed=3)
+    x = rng.uniform(0, 2 * np.pi, (40, 2))
+    out = _eval(net, x, 2, 0.0, 2 * np.pi)
+    u, h = o.lift2d_hessian(out)
+    full = np.empty((40, 5, 2, 2))
+    full[..., 0, 0], full[..., 1, 1] = out.hess[..., 0], out.hess[..., 2]
+    full[..., 0, 1] = full[..., 1, 0] = out.hess[..., 1]
+    S_full = np.einsum("nkab,nlab->kl", full, full)
+    S_rows = o.field_rows(h).T @ o.field_rows(h)
+    np.testing.assert_allclose(S_rows, S_full, rtol=1e-12)
+    M = o.field_rows(u).T @ o.field_rows(u)
+    np.testing.assert_allclose(M, np.einsum("nka,nla->kl", out.grad, out.grad), rtol=1e-12)
+    gu, gh = rng.normal(size=u.shape), rng.normal(size=h.shape)
+    adj = o.lift2d_hessian_reverse(gu, gh)
+    lhs = np.sum(u * gu) + np.sum(h * gh)
+    rhs = np.sum(out.grad * adj.grad) + np.sum(out.hess * adj.hess)
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def _velocity_fields(seed=5, n=300, K=7):
+    rng = np.random.default_rng(seed)
+    net = _net(2, 2, [16, 16], K, seed=seed)
+    x = rng.uniform(0, 2 * np.pi, (n, 2))
+    return _velocity_rows(_eval(net, x, 2, 0.0, 2 * np.pi))
+
+
+def test_velocity_peel_equals_cholesky_coefficient_peel():
+    """The literal two-pass velocity peel equals the sorted Cholesky peel of
+    its Grams (M = w G^T G, S = w H^T diag(1,2,1) H): Rayleigh quotients,
+    causal order and the detached-prefix cotangents w G Cu, w metric H Cd."""
+    grad, hess = _velocity_fields()
+    n = grad.shape[0] // 2
+    w = 1.0 / n
+    metric = np.tile([1.0, 2.0, 1.0], n)
+    lit = o.velocity_rayleigh_peel(grad, hess, w)
+    M = w * grad.T @ grad
+    S = w * hess.T @ (metric[:, None] * hess)
+    cp = o.coefficient_peel(M, S, ortho_tol=1e-3)
+    np.testing.assert_array_equal(lit["order"], cp["order"])
+    np.testing.assert_allclose(lit["rayleigh"], cp["losses"], rtol=1e-10)
+    np.testing.assert_allclose(lit["G_grad"], w * grad @ cp["Cu"], rtol=1e-8, atol=1e-12 * np.abs(lit["G_grad"]).max())
+    np.testing.assert_allclose(lit["G_hess"], w * metric[:, None] * (hess @ cp["Cd"]), rtol=1e-8,
+                               atol=1e-12 * np.abs(lit["G_hess"]).max())
+    assert lit["orth"].max() < 1e-8
+
+
+def test_velocity_peel_cotangents_match_fd_with_detached_prefix():
+    """Each mode's cotangent is the central difference of its own Rayleigh
+    quotient with the earlier peeled modes held (fixed order)."""
+    grad, hess = _velocity_fields(seed=7, K=5)
+    n = grad.shape[0] // 2
+    w = 1.0 / n
+    base = o.velocity_rayleigh_peel(grad, hess, w, order="fixed")
+    rng = np.random.default_rng(2)
+    for r in (0, 2, 4):
+        dg = rng.normal(size=2 * n)
+        dh = rng.normal(size=3 * n)
+        h = 1e-6
+        vals = []
+        for sgn in (1.0, -1.0):
+            g2, h2 = grad.copy(), hess.copy()
+            g2[:, r] += sgn * h * dg
+            h2[:, r] += sgn * h * dh
+            vals.append(o.velocity_rayleigh_peel(g2, h2, w, order="fixed")["rayleigh"][r])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        exact = base["G_grad"][:, r] @ dg + base["G_hess"][:, r] @ dh
+        assert abs(fd - exact) <= 1e-6 * max(abs(exact), 1e-8), (r, fd, exact)
+
+
+def test_velocity_peel_fourier_closed_form():
+    """Orthogonal Fourier stream functions on the periodic grid: the peel
+    leaves them as they are and the Rayleigh quotients are the sorted |k|^2
+    (grad u Frobenius / velocity mass of sin, cos (k.x))."""
+    x = o.periodic_grid(32)
+    ks = [(1, 0), (0, 2), (1, 1), (2, 1), (3, 0), (1, 2)]
+    n = len(x)
+    grad = np.zeros((2 * n, len(ks)))
+    hess = np.zeros((3 * n, len(ks)))
+    for j, (kx, ky) in enumerate(ks):
+        ph = kx * x[:, 0] + ky * x[:, 1]
+        c, s = np.cos(ph), np.sin(ph)
+        grad[0::2, j], grad[1::2, j] = kx * c, ky * c
+        hess[0::3, j], hess[1::3, j], hess[2::3, j] = -kx * kx * s, -kx * ky * s, -ky * ky * s
+    w = (2 * np.pi / 32) ** 2
+    res = o.velocity_rayleigh_peel(grad, hess, w)
+    k2 = np.array([kx * kx + ky * ky for kx, ky in ks], dtype=float)
+    np.testing.assert_allclose(res["rayleigh"], np.sort(k2), rtol=1e-12)
+    np.testing.assert_array_equal(res["order"], np.argsort(k2, kind="stable"))
+    np.testing.assert_allclose(res["relative_mass"], 1.0, rtol=1e-12)
