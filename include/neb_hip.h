@@ -301,6 +301,18 @@ size_t neb_spd_inverse_workspace_size(int64_t K);
 int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
                     size_t ws_bytes, void* stream);
 
+/* Reduced-order right-hand side of the 2D periodic rollout on the network
+ * basis (experiments/periodic/paper_rollout.py:37-41 `rhs`): with the
+ * orthonormalised basis fields gu, gv (velocity) and gw (vorticity) as
+ * [n][ld] matrices (dtype NEB_F32 or NEB_F64, K <= 8192 columns) and the
+ * coefficients a [K] (FP64):
+ *   u = gu a, v = gv a, omega = gw a (per point),
+ *   rhs = gu^T (w omega v) - gv^T (w omega u)   (FP64, [K]).
+ * Workspace: neb_galerkin_workspace_size(n, K) bytes. */
+size_t neb_galerkin_workspace_size(int64_t n, int64_t K);
+int neb_galerkin_rhs2d(int dtype, const void* gu, const void* gv, const void* gw, int64_t n, int64_t K, int64_t ld,
+                       const double* a, double w, double* rhs, void* workspace, size_t ws_bytes, void* stream);
+
 /* Every layer's bf16 operand forms in one launch, from the flat master
  * parameters (fp32 or fp64): layer l's weight W_l is the row-major
  * [rows[l]][cols[l]] block at element offset off[l]; hi[l] = bf16(W_l) as

@@ -13,6 +13,7 @@
 #include "../../include/neb_hip.h"
 #include "cross_async.hpp"
 #include "crossgemm.hpp"
+#include "galerkin.hpp"
 #include "jet_rowgemm.hpp"
 #include "misc_kernels.hpp"
 #include "rowgemm_async.hpp"
@@ -992,6 +993,43 @@ int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, i
       return hip_status("neb_split_bf16");
   }
   return fail(NEB_ERR_ARG, "neb_split_bf16: source dtype must be NEB_F32 or NEB_F64 (got %d)", src_dtype);
+}
+
+size_t neb_galerkin_workspace_size(int64_t n, int64_t K) {
+  return (size_t)(2 * n + (int64_t)colsum_splits(n) * K) * sizeof(double);
+}
+
+int neb_galerkin_rhs2d(int dtype, const void* gu, const void* gv, const void* gw, int64_t n, int64_t K, int64_t ld,
+                       const double* a, double w, double* rhs, void* workspace, size_t ws_bytes, void* stream) {
+  if (n < 1 || K < 1 || ld < K || K > 8192) return fail(NEB_ERR_SHAPE, "neb_galerkin_rhs2d: bad sizes (K <= 8192)");
+  if (!gu || !gv || !gw || !a || !rhs || !workspace) return fail(NEB_ERR_ARG, "neb_galerkin_rhs2d: null pointer");
+  if (dtype != NEB_F32 && dtype != NEB_F64) return fail(NEB_ERR_ARG, "neb_galerkin_rhs2d: fields must be F32 or F64");
+  const int sp = colsum_splits(n);
+  if (ws_bytes < (size_t)(2 * n + (int64_t)sp * K) * sizeof(double))
+    return fail(NEB_ERR_ARG, "neb_galerkin_rhs2d: workspace too small");
+  double* r1 = reinterpret_cast<double*>(workspace);
+  double* r2 = r1 + n;
+  double* part = r2 + n;
+  const int64_t chunk = (n + sp - 1) / sp;
+  const dim3 gp((unsigned)((n + 3) / 4)), gc((unsigned)((K + 255) / 256), (unsigned)sp);
+  const size_t lds = (size_t)K * sizeof(double);  // <= 64 KiB for K <= 8192
+  hipStream_t st = S(stream);
+  if (dtype == NEB_F32) {
+    const float *u = reinterpret_cast<const float*>(gu), *v = reinterpret_cast<const float*>(gv),
+                *o = reinterpret_cast<const float*>(gw);
+    hipLaunchKernelGGL(rhs2d_points_kernel<float>, gp, dim3(256), lds, st, u, v, o, ld, n, K, a, w, r1, r2);
+    if (int rc = hip_status("neb_galerkin_rhs2d")) return rc;
+    hipLaunchKernelGGL(rhs2d_cols_kernel<float>, gc, dim3(256), 0, st, u, v, ld, n, K, chunk, r1, r2, part);
+  } else {
+    const double *u = reinterpret_cast<const double*>(gu), *v = reinterpret_cast<const double*>(gv),
+                 *o = reinterpret_cast<const double*>(gw);
+    hipLaunchKernelGGL(rhs2d_points_kernel<double>, gp, dim3(256), lds, st, u, v, o, ld, n, K, a, w, r1, r2);
+    if (int rc = hip_status("neb_galerkin_rhs2d")) return rc;
+    hipLaunchKernelGGL(rhs2d_cols_kernel<double>, gc, dim3(256), 0, st, u, v, ld, n, K, chunk, r1, r2, part);
+  }
+  if (int rc = hip_status("neb_galerkin_rhs2d columns")) return rc;
+  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, part, sp, K, rhs, 0.0);
+  return hip_status("neb_galerkin_rhs2d reduce");
 }
 
 int neb_join_bf16(int dst_dtype, const void* hi, const void* lo, int64_t rows, int64_t cols, int64_t ld, void* dst,

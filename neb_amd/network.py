@@ -606,9 +606,11 @@ def field_grams(fields: Fields, w: float, mode: str = "bf16x3", M: torch.Tensor 
     return M, S
 
 
-def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, beta: float = 0.0):
+def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, beta: float = 0.0,
+          roww: torch.Tensor | None = None):
     """sum_s sign_s X_s^T Y_s over rows; segments = [(X, Y, rows, sign)] with
-    X/Y row-major 2-D tensors (views allowed)."""
+    X/Y row-major 2-D tensors (views allowed).  roww: optional FP64 weight
+    per row (one segment only)."""
     xs = [s[0] for s in segments]
     ys = [s[1] for s in segments]
     if out is None:
@@ -620,8 +622,8 @@ def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, bet
     H.call("neb_cross", mode, len(segments), H.ptr_array([t.data_ptr() for t in xs]),
            H.ptr_array([t.data_ptr() for t in ys]), H.i64_array([t.stride(0) for t in xs]),
            H.i64_array([t.stride(0) for t in ys]), H.i64_array(rows), H.dbl_array([float(s[3]) for s in segments]),
-           _DT[xs[0].dtype], _DT[ys[0].dtype], m, n, 0, 0, float(scale), float(beta), out.data_ptr(),
-           out.stride(0), ws.data_ptr(), ws.numel(), _stream())
+           _DT[xs[0].dtype], _DT[ys[0].dtype], m, n, 0 if roww is None else roww.data_ptr(), 0, float(scale),
+           float(beta), out.data_ptr(), out.stride(0), ws.data_ptr(), ws.numel(), _stream())
     return out
 
 

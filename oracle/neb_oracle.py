@@ -612,3 +612,145 @@ def fourier_potential_targets(points: np.ndarray, count: int, seed: int, kmax: i
         out[:, :, t] = np.stack((J[:, 2, 1] - J[:, 1, 2], J[:, 0, 2] - J[:, 2, 0],
                                  J[:, 1, 0] - J[:, 0, 1]), 1)
     return out
+
+
+# --------------------------------------------------------------------------
+# next row: Galerkin reduced-order operators on the network basis
+# (experiments/periodic/paper_rollout.py:37-66,166-186;
+#  experiments/metaball/operators.py:20-60,137-257)
+# --------------------------------------------------------------------------
+def periodic_reduced_basis(gu, gv, gw, w: float, rank_tol: float = 1e-12):
+    """Velocity-normalised columns, FP64 mass w(gu^T gu + gv^T gv) and
+    stiffness w gw^T gw, the full-rank gate, generalized eigh and the
+    transformed fields (paper_rollout.py:166-186).  Returns (gu, gv, gw, vals)."""
+    import scipy.linalg as la
+    norm = np.sqrt(w * np.sum(gu * gu + gv * gv, 0))
+    if not np.all(norm > 0):
+        raise FloatingPointError("zero-velocity mode")
+    gu, gv, gw = gu / norm, gv / norm, gw / norm
+    mass = w * (gu.T @ gu + gv.T @ gv)
+    stiff = w * (gw.T @ gw)
+    mh, sh = 0.5 * (mass + mass.T), 0.5 * (stiff + stiff.T)
+    ev = np.linalg.eigvalsh(mh)
+    if int(np.sum(ev > ev[-1] * rank_tol)) != len(mh):
+        raise ArithmeticError("Full-K rank gate")
+    vals, T = la.eigh(sh, mh)
+    return gu @ T, gv @ T, gw @ T, vals
+
+
+def rollout_rhs(gu, gv, gw, a, w: float):
+    """Galerkin right-hand side of the 2D vorticity-form Euler nonlinearity
+    (paper_rollout.py:37-41): u = gu a, v = gv a, omega = gw a,
+    rhs = gu^T (w omega v) - gv^T (w omega u)."""
+    u, v, om = gu @ a, gv @ a, gw @ a
+    return gu.T @ (w * om * v) - gv.T @ (w * om * u)
+
+
+def etdrk4_coefficients(linear, h: float, roots: int = 32):
+    """Exponential time-differencing RK4 coefficients of the diagonal linear
+    part `linear` at step h, by the contour mean over `roots` points
+    (Kassam & Trefethen 2005; paper_rollout.py:49-56)."""
+    e = np.exp(h * linear)
+    e2 = np.exp(0.5 * h * linear)
+    r = np.exp(1j * np.pi * (np.arange(1, roots + 1) - 0.5) / roots)
+    lr = h * linear[:, None] + r
+    q = h * np.real(np.mean((np.exp(lr / 2) - 1) / lr, -1))
+    f1 = h * np.real(np.mean((-4 - lr + np.exp(lr) * (4 - 3 * lr + lr * lr)) / lr ** 3, -1))
+    f2 = h * np.real(np.mean((2 + lr + np.exp(lr) * (-2 + lr)) / lr ** 3, -1))
+    f3 = h * np.real(np.mean((-4 - 3 * lr - lr * lr + np.exp(lr) * (4 - lr)) / lr ** 3, -1))
+    return e, e2, q, f1, f2, f3
+
+
+def etdrk4(rhs, linear, c, h: float, steps: int):
+    """`steps` ETDRK4 steps of c' = linear * c + rhs(c) (paper_rollout.py:57-64)."""
+    e, e2, q, f1, f2, f3 = etdrk4_coefficients(linear, h)
+    for _ in range(steps):
+        nv = rhs(c)
+        a = e2 * c + q * nv
+        na = rhs(a)
+        b = e2 * c + q * na
+        nb = rhs(b)
+        z = e2 * a + q * (2 * nb - nv)
+        c = e * c + f1 * nv + 2 * f2 * (na + nb) + f3 * rhs(z)
+    return c
+
+
+class SpectralGrid:
+    """Periodic n^3 spectral operators of the metaball operator derivation
+    (operators.py:20-60): Leray projection with 2/3 dealiasing, curl,
+    viscous and normal-penalty forces, FP64 FFTs over the last three axes of
+    [3, n, n, n] fields (x fastest)."""
+
+    def __init__(self, normal, chi, n: int, viscosity: float = 2.5e-4, penalty_time: float = 1e-3):
+        k = 2 * np.pi * np.fft.fftfreq(n, d=1 / n)
+        self.kz, self.ky, self.kx = k[:, None, None], k[None, :, None], k[None, None, :]
+        self.k2 = self.kx ** 2 + self.ky ** 2 + self.kz ** 2
+        self.safe = self.k2.copy()
+        self.safe[0, 0, 0] = 1
+        m = np.fft.fftfreq(n, d=1 / n)
+        ok = np.abs(m) <= n // 3
+        self.dealias = ok[:, None, None] & ok[None, :, None] & ok[None, None, :]
+        self.normal = np.asarray(normal, dtype=np.float64).transpose(3, 0, 1, 2)
+        self.chi = np.asarray(chi, dtype=np.float64)
+        self.n, self.nu, self.eta = n, viscosity, penalty_time
+
+    def fft(self, u):
+        return np.fft.fftn(u, axes=(1, 2, 3))
+
+    def real(self, v):
+        return np.fft.ifftn(v, axes=(1, 2, 3)).real
+
+    def project(self, v):
+        dot = (self.kx * v[0] + self.ky * v[1] + self.kz * v[2]) / self.safe
+        r = np.stack((v[0] - self.kx * dot, v[1] - self.ky * dot, v[2] - self.kz * dot)) * self.dealias
+        r[:, 0, 0, 0] = 0
+        return r
+
+    def curl(self, v):
+        return np.stack((1j * (self.ky * v[2] - self.kz * v[1]), 1j * (self.kz * v[0] - self.kx * v[2]),
+                         1j * (self.kx * v[1] - self.ky * v[0])))
+
+    def derived(self, u):
+        """(projected, curl, linear force) of one [3, n, n, n] field."""
+        v = self.fft(u)
+        viscous = self.project(-self.nu * self.k2 * v)
+        penalty = self.project(self.fft(-self.chi[None] / self.eta * np.sum(u * self.normal, 0)[None] * self.normal))
+        return self.real(self.project(v)), self.real(self.curl(v)), self.real(viscous + penalty)
+
+
+def derive_fields(B, flat, grid: SpectralGrid):
+    """Per mode k: the basis field B[:, :, k] at the fluid points `flat`
+    (flat indices of the n^3 grid, x fastest), zero-extended to the grid ->
+    (projected, curl, linear) back at the fluid points: [N, 9, K]
+    (operators.py:137-177)."""
+    N, _, K = B.shape
+    n = grid.n
+    out = np.empty((N, 9, K))
+    for k in range(K):
+        u = np.zeros((n ** 3, 3))
+        u[flat] = B[:, :, k]
+        u = u.reshape(n, n, n, 3).transpose(3, 0, 1, 2)
+        parts = grid.derived(u)
+        out[:, :, k] = np.concatenate([x.transpose(1, 2, 3, 0).reshape(-1, 3)[flat] for x in parts], 1)
+    return out
+
+
+def assemble_operators(B, D, dV: float):
+    """Full-mass M = dV B^T B, linear A = dV B^T L and the packed convective
+    tensor Q[i, (j<=k)] = dV (q[j,i,k] + q[k,i,j]) (j != k) or dV q[j,i,j],
+    q[j,i,k] = sum_p PB_i . (B_j x curlB_k) (operators.py:199-257; pairs in
+    numpy.triu_indices(K) order).  B: [N, 3, K]; D: [N, 9, K] = (projected,
+    curl, linear)."""
+    N, _, K = B.shape
+    b = B.reshape(N * 3, K)
+    M = dV * b.T @ b
+    A = dV * b.T @ D[:, 6:9, :].reshape(N * 3, K)
+    P, C = D[:, 0:3, :], D[:, 3:6, :]
+    q = np.zeros((K, K, K))
+    for a, bb, c, sign in ((0, 1, 2, 1), (0, 2, 1, -1), (1, 2, 0, 1), (1, 0, 2, -1), (2, 0, 1, 1), (2, 1, 0, -1)):
+        q += sign * np.einsum("pj,pi,pk->jik", B[:, bb, :], P[:, a, :], C[:, c, :], optimize=True)
+    jj, kk = np.triu_indices(K)
+    Q = q[jj, :, kk].T.copy()
+    diff = jj != kk
+    Q[:, diff] += q[kk[diff], :, jj[diff]].T
+    return M, A, dV * Q
