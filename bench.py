@@ -830,6 +830,59 @@ def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
     return accuracy
 
 
+def galerkin_bench(nw, net, x, w):
+    """The basis's consumers at the config-B size: the rollout right-hand
+    side (neb_galerkin_rhs2d) on the network's FP64 velocity / vorticity
+    fields (65,536 points x K = 1024; 5 x n x K x 8 B read per call: an
+    HBM-bound pair of passes) and the packed convective tensor of the
+    metaball operators (12 N K^3 FP64 flops) on N = 131,072 synthetic points
+    at K = 64."""
+    import torch
+
+    from neb_amd import galerkin as gk
+    dev = x.device
+    tape = nw.trunk_forward(net, x, nw.PRECISIONS["f64"])
+    f = nw.output_fields(net, tape)
+    del tape
+    K = f.K
+    U = nw.field_matrix(f, "mass", torch.float64)
+    n = U.shape[0] // 2
+    gu = U.as_strided((n, K), (2 * U.stride(0), 1)).contiguous()
+    gv = U.as_strided((n, K), (2 * U.stride(0), 1), U.storage_offset() + U.stride(0)).contiguous()
+    gw = nw.field_matrix(f, "stiff", torch.float64).contiguous()
+    del f, U
+    a = torch.randn(K, dtype=torch.float64, device=dev, generator=torch.Generator(device=dev).manual_seed(0))
+    r = gk.rollout_rhs(gu, gv, gw, a, w)
+    skew = float(torch.abs(torch.dot(a, r)) / (torch.linalg.norm(a) * torch.linalg.norm(r)))
+    torch.cuda.synchronize()
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        gk.rollout_rhs(gu, gv, gw, a, w, out=r)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 5 * n * K * 8
+    rhs = dict(points=n, modes=K, ms_per_call=ms, bytes_per_call=nbytes, achieved_GBps=nbytes / (ms * 1e-3) / 1e9,
+               frac_of_hbm=nbytes / (ms * 1e-3) / HBM_PEAK, relative_energy_skew=skew)
+    del gu, gv, gw
+    N, Kq = 131072, 64
+    g = torch.Generator(device=dev).manual_seed(1)
+    B = torch.randn((N, 3, Kq), dtype=torch.float64, device=dev, generator=g)
+    D = torch.randn((N, 9, Kq), dtype=torch.float64, device=dev, generator=g)
+    gk.assemble_operators(B[:4096], D[:4096], 1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gk.assemble_operators(B, D, 1.0 / 128 ** 3)
+    torch.cuda.synchronize()
+    tq = time.perf_counter() - t0
+    flops = 12 * N * Kq ** 3
+    quad = dict(points=N, modes=Kq, seconds=tq, tflops=flops / tq / 1e12, frac_of_fp64_peak=flops / tq / PEAK["f64"],
+                note="6 K row-weighted FP64 cross products of N x K x K each, plus M and A")
+    return dict(rollout_rhs=rhs, convective_tensor=quad)
+
+
 def gradient_errors(nw, sp, net, x, w, ridge):
     """Gradient of every step variant against the FP64 path's on the same
     points, weights and ridge: relative L2 error and cosine of the full
@@ -913,11 +966,12 @@ def main():
     # ---- Gram modes on the same fields: TFLOP/s and error vs FP64 ----------
     # "bf16x3-split": the step's path (pre-split bf16 planes); "bf16x3" splits
     # the fp32 fields per tile; both give the same bits.
-    gram_stats, accuracy, fd_check, grad_errors = {}, {}, None, None
+    gram_stats, accuracy, fd_check, grad_errors, galerkin = {}, {}, None, None, None
     if not args.step_only:
         gram_stats = gram_mode_stats(nw, fields, split, K, w)
         accuracy = accuracy_vs_fp64(nw, sp, net, x, K, w, ridge)
         grad_errors = gradient_errors(nw, sp, net, x, w, ridge)
+        galerkin = galerkin_bench(nw, net, x, w)
         fd_check = fd_check_fp64(nb, sp, cfg, dev)
     del tape, fields, split
 
@@ -1060,6 +1114,7 @@ def main():
                          "share of that step (side-stream kernels overlap the current stream, so shares can sum "
                          "above 1), algorithmic bytes / useful flops per launch and the roofline they imply"),
         "gradient_vs_fp64": grad_errors,
+        "galerkin": galerkin,
         "cpu_baseline": cpu,
     }
     if not args.no_3d:

@@ -93,6 +93,24 @@ int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int6
 int neb_jet_output_jets(int prec, int dim, int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy,
                         const void* w, int64_t ldw, const void* bias, void* out, int64_t ldo, void* stream);
 
+/* The output jets times a per-point envelope jet e (the Jet2 product rule
+ * y = e o, grad y = e grad o + o grad e, hess_ij y = e hess_ij o + o hess_ij e
+ * + e_i o_j + e_j o_i) applied in the epilogue of the output-layer GEMM:
+ * field = envelope * raw (VJ2:47-50 `product`, NCC:94-95; the periodic
+ * configurations use the identity envelope, HB2P:546-547).  env: packed
+ * [C][n] jets in the math precision (fp32, fp64 for F64), one per point,
+ * shared by all nout outputs.  Other arguments as neb_jet_output_jets.
+ * neb_envelope_reverse: its transpose (VJ2:51-55 `product_reverse`): the
+ * cotangent adj [C][n][lda] of the enveloped jets (math precision) -> the
+ * cotangent of the raw output jets, written [C][n][ldo] in the operand
+ * precision (bf16 / fp32 / fp64) of the output-layer reverse
+ * (neb_jet_layer_backward with the output weights). */
+int neb_jet_output_jets_enveloped(int prec, int dim, int64_t n, int64_t k, int64_t nout, const void* y,
+                                  int64_t ldy, const void* w, int64_t ldw, const void* bias, const void* env,
+                                  void* out, int64_t ldo, void* stream);
+int neb_envelope_reverse(int prec, int dim, int64_t n, int64_t nout, const void* env, const void* adj, int64_t lda,
+                         void* out, int64_t ldo, void* stream);
+
 /* Output layer fused with the differential lift, writing field matrices in
  * fp32 (fp64 for F64).  2D: w [K][ldw]; mass = (psi_y, -psi_x) as [2n][ldf]
  * and stiff = omega = -(psi_xx+psi_yy) as [n][ldf] (ELC:35-38 =

@@ -30,6 +30,9 @@ SIGNATURES = {
     "neb_jet_layer_forward": (_int, [_int, _int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _dbl,
                                      _vp, _vp, _i64, _vp]),
     "neb_jet_output_jets": (_int, [_int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "neb_jet_output_jets_enveloped": (_int, [_int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                                             _i64, _vp]),
+    "neb_envelope_reverse": (_int, [_int, _int, _i64, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "neb_jet_output_fields": (_int, [_int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _int,
                                      _vp]),
     "neb_jet_output_fields_split": (_int, [_int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i64,

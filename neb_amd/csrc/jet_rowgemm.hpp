@@ -50,6 +50,7 @@ struct RowArgs {
   int layout;               // fields: 0 = point-major rows, 1 = component-major
   void* out0lo;             // split fields: bf16 lo planes beside out0 / out1
   void* out1lo;
+  const void* env;          // envelope jets, math_t [C][npts] (PlanOutJetsT<D, true>)
 };
 
 template <typename T> __device__ inline T fneg(T v) { return -v; }
@@ -517,8 +518,11 @@ template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
   }
 };
 
-// Output layer, raw jets: O = X W^T (+ b on the value channel), field_t out.
-template <int D> struct PlanOutJets {
+// Output layer, raw jets: O = X W^T (+ b on the value channel), field_t out;
+// with ENV the stored jets are the product e O with the per-point envelope
+// jet e (jet_product, VJ2:47-50; NCC:94-95 field = e * raw), one envelope
+// per point shared by every output feature.
+template <int D, bool ENV = false> struct PlanOutJetsT {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::math_t); }
   static constexpr int C = Jet2Layout<D>::C;
   static constexpr int CI = C, NW = 1, NA = C, NT = C;
@@ -530,11 +534,28 @@ template <int D> struct PlanOutJets {
     if (bias)
 #pragma unroll
       for (int e = 0; e < GE; ++e) v[0][e] += e < nv ? bias[f + e] : (M)0;
+    if constexpr (ENV) {
+      using MT = typename Prec<P>::math_t;
+      const MT* env = reinterpret_cast<const MT*>(a.env) + p;
+      M ej[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) ej[c] = (M)env[c * a.npts];
+#pragma unroll
+      for (int e = 0; e < GE; ++e) {
+        M x[C], y[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) x[c] = v[c][e];
+        jet_product<D, M>(ej, x, y);
+#pragma unroll
+        for (int c = 0; c < C; ++c) v[c][e] = y[c];
+      }
+    }
     M* o = reinterpret_cast<M*>(a.out0) + p * a.ldo + f;
 #pragma unroll
     for (int c = 0; c < C; ++c) st_group<M, GE>(o + c * a.cstride, v[c], nv);
   }
 };
+template <int D> using PlanOutJets = PlanOutJetsT<D, false>;
 
 // 2D output layer with the velocity / vorticity lift (ELC:35-38):
 //   u = psi_y, v = -psi_x  (mass fields),  omega = -(psi_xx + psi_yy).

@@ -197,6 +197,28 @@ __global__ __launch_bounds__(256) void split_kernel(const TS* __restrict__ src, 
   lo[r * ldd + c] = (__bf16)(float)(x - (double)(float)h);
 }
 
+// Transpose of the envelope product (jet_product_reverse, VJ2:51-55) for
+// every (point, output) pair: adj [C][n][lda] (math precision) -> out
+// [C][n][ldo] in the operand precision of the output-layer reverse.
+template <typename SO, typename M, int D>
+__global__ __launch_bounds__(256) void envelope_reverse_kernel(const M* __restrict__ env, const M* __restrict__ adj,
+                                                               int64_t lda, int64_t n, int64_t nout,
+                                                               SO* __restrict__ out, int64_t ldo) {
+  constexpr int C = Jet2Layout<D>::C;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * nout) return;
+  const int64_t p = idx / nout, o = idx % nout;
+  M e[C], a[C], r[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    e[c] = env[c * n + p];
+    a[c] = adj[(c * n + p) * lda + o];
+  }
+  jet_product_reverse<D, M>(e, a, r);
+#pragma unroll
+  for (int c = 0; c < C; ++c) out[(c * n + p) * ldo + o] = cvt<SO>(r[c]);
+}
+
 // Every layer's bf16 operand forms in one launch (see neb_stage_net): a flat
 // index over, first, the [rows][ld] planes of all layers (hi, and lo when
 // requested), then the [cols][ldt] transposes; padding columns are zeros.

@@ -134,7 +134,7 @@ int layer_forward(int64_t n, int64_t k, int64_t width, const void* x, int64_t ld
 
 template <int P, int D>
 int output_jets(int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy, const void* w, int64_t ldw,
-                const void* bias, void* out, int64_t ldo, hipStream_t st) {
+                const void* bias, void* out, int64_t ldo, hipStream_t st, const void* env = nullptr) {
   constexpr int C = Jet2Layout<D>::C;
   using Sx = typename Prec<P>::store_t;
   RowArgs a = row_args();
@@ -146,7 +146,22 @@ int output_jets(int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy, 
   a.npts = n; a.nout = nout; a.k = k;
   a.bias = bias;
   a.out0 = out; a.ldo = ldo; a.cstride = n * ldo;
+  if (env) {
+    a.env = env;
+    return launch_rowgemm<P, Sx, PlanOutJetsT<D, true>>(a, st, "neb_jet_output_jets_enveloped");
+  }
   return launch_rowgemm<P, Sx, PlanOutJets<D>>(a, st, "neb_jet_output_jets");
+}
+
+template <typename SO, typename M, int D>
+int envelope_reverse_launch(int64_t n, int64_t nout, const void* env, const void* adj, int64_t lda, void* out,
+                            int64_t ldo, hipStream_t st) {
+  const int64_t tot = n * nout;
+  if (tot / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_envelope_reverse: too many elements");
+  hipLaunchKernelGGL((envelope_reverse_kernel<SO, M, D>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<const M*>(env), reinterpret_cast<const M*>(adj), lda, n, nout,
+                     reinterpret_cast<SO*>(out), ldo);
+  return hip_status("neb_envelope_reverse");
 }
 
 template <int P, bool SPLIT = false>
@@ -635,6 +650,44 @@ int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int6
 #define CALL(P, D, A) layer_forward<P, D, A>(n, k, width, x, ldx, w, ldw, bias, omega, z, y, ldy, S(stream))
   DISPATCH_PDA(CALL)
 #undef CALL
+}
+
+int neb_jet_output_jets_enveloped(int prec, int dim, int64_t n, int64_t k, int64_t nout, const void* y,
+                                  int64_t ldy, const void* w, int64_t ldw, const void* bias, const void* env,
+                                  void* out, int64_t ldo, void* stream) {
+  if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
+  if (n < 0 || k < 1 || nout < 1 || ldy < k || ldw < k || ldo < nout)
+    return fail(NEB_ERR_SHAPE, "neb_jet_output_jets_enveloped: bad sizes");
+  if (n == 0) return NEB_OK;
+  if (!y || !w || !out || !env) return fail(NEB_ERR_ARG, "neb_jet_output_jets_enveloped: null pointer");
+  if (!aligned(y, ldy, prec) || !aligned(w, ldw, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_output_jets_enveloped: rows must be 16-byte aligned");
+  hipStream_t st = S(stream);
+  if (prec == NEB_BF16)
+    return dim == 2 ? output_jets<PREC_BF16, 2>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st, env)
+                    : output_jets<PREC_BF16, 3>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st, env);
+  if (prec == NEB_F32)
+    return dim == 2 ? output_jets<PREC_F32, 2>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st, env)
+                    : output_jets<PREC_F32, 3>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st, env);
+  return dim == 2 ? output_jets<PREC_F64, 2>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st, env)
+                  : output_jets<PREC_F64, 3>(n, k, nout, y, ldy, w, ldw, bias, out, ldo, st, env);
+}
+
+int neb_envelope_reverse(int prec, int dim, int64_t n, int64_t nout, const void* env, const void* adj, int64_t lda,
+                         void* out, int64_t ldo, void* stream) {
+  if (int rc = check_common(prec, dim, NEB_TANH)) return rc;
+  if (n < 0 || nout < 1 || lda < nout || ldo < nout) return fail(NEB_ERR_SHAPE, "neb_envelope_reverse: bad sizes");
+  if (n == 0) return NEB_OK;
+  if (!env || !adj || !out) return fail(NEB_ERR_ARG, "neb_envelope_reverse: null pointer");
+  hipStream_t st = S(stream);
+  if (prec == NEB_BF16)
+    return dim == 2 ? envelope_reverse_launch<__bf16, float, 2>(n, nout, env, adj, lda, out, ldo, st)
+                    : envelope_reverse_launch<__bf16, float, 3>(n, nout, env, adj, lda, out, ldo, st);
+  if (prec == NEB_F32)
+    return dim == 2 ? envelope_reverse_launch<float, float, 2>(n, nout, env, adj, lda, out, ldo, st)
+                    : envelope_reverse_launch<float, float, 3>(n, nout, env, adj, lda, out, ldo, st);
+  return dim == 2 ? envelope_reverse_launch<double, double, 2>(n, nout, env, adj, lda, out, ldo, st)
+                  : envelope_reverse_launch<double, double, 3>(n, nout, env, adj, lda, out, ldo, st);
 }
 
 int neb_jet_output_jets(int prec, int dim, int64_t n, int64_t k, int64_t nout, const void* y, int64_t ldy,

@@ -143,6 +143,42 @@ template <> struct Jet2Layout<3> {
   __device__ static constexpr int pj(int p) { return p < 3 ? p : (p < 5 ? p - 2 : 2); }
 };
 
+// Second-order jet product y = e x (the envelope product rule, VJ2:47-50):
+//   y = e x,  grad y = e grad x + x grad e,
+//   hess_ij y = e hess_ij x + x hess_ij e + e_i x_j + e_j x_i.
+template <int D, typename M>
+__device__ inline void jet_product(const M* e, const M* x, M* y) {
+  using L = Jet2Layout<D>;
+  y[0] = e[0] * x[0];
+#pragma unroll
+  for (int i = 0; i < D; ++i) y[1 + i] = e[0] * x[1 + i] + e[1 + i] * x[0];
+#pragma unroll
+  for (int q = 0; q < L::P; ++q) {
+    const int i = L::pi(q), j = L::pj(q);
+    y[1 + D + q] = e[0] * x[1 + D + q] + e[1 + D + q] * x[0] + e[1 + i] * x[1 + j] + e[1 + j] * x[1 + i];
+  }
+}
+
+// Its transpose in x (VJ2:51-55): r = d<a, e x>/dx for the cotangent a of y.
+template <int D, typename M>
+__device__ inline void jet_product_reverse(const M* e, const M* a, M* r) {
+  using L = Jet2Layout<D>;
+  r[0] = e[0] * a[0];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    r[0] += e[1 + i] * a[1 + i];
+    r[1 + i] = e[0] * a[1 + i];
+  }
+#pragma unroll
+  for (int q = 0; q < L::P; ++q) {
+    const int i = L::pi(q), j = L::pj(q);
+    r[0] += e[1 + D + q] * a[1 + D + q];
+    r[1 + i] += e[1 + j] * a[1 + D + q];
+    r[1 + j] += e[1 + i] * a[1 + D + q];
+    r[1 + D + q] = e[0] * a[1 + D + q];
+  }
+}
+
 // Forward second-order chain rule for y = f(u) on one (point, feature) pair.
 // in/out: [C] packed channels (value, grads, hessian pairs).
 template <int D, int ACT, typename M>

@@ -447,10 +447,20 @@ def convert_tape(tape: Tape, prec: int) -> Tape:
     return Tape(prec, tape.n, [conv(t) for t in tape.inputs], [conv(t) for t in tape.pre])
 
 
-def output_jets(net: EigenfluidNet, tape: Tape):
-    """Raw output jets (value [n, nout], grad [n, nout, d], hess [n, nout, P])
+def envelope_jets(value: torch.Tensor, grad: torch.Tensor, hess: torch.Tensor, dtype=torch.float32) -> torch.Tensor:
+    """Pack the jet of a scalar envelope per point (value [n], grad [n, d],
+    hess [n, P] in the pair order) as the [C, n] operand of output_jets /
+    backward_jets (fp32; fp64 for FP64 tapes)."""
+    return torch.cat([value.reshape(1, -1), grad.T, hess.T], 0).to(dtype).contiguous()
+
+
+def output_jets(net: EigenfluidNet, tape: Tape, envelope: torch.Tensor | None = None):
+    """Output jets (value [n, nout], grad [n, nout, d], hess [n, nout, P])
     -- the analogue of the per-mode field jets of forward_stream_jet2
-    (HB2P:1802-1880) / VJ2.forward (VJ2:65-76)."""
+    (HB2P:1802-1880) / VJ2.forward (VJ2:65-76).  envelope: optional [C, n]
+    jet of a per-point envelope e (envelope_jets): the outputs are then the
+    products e * raw by the Jet2 product rule, applied in the output GEMM's
+    epilogue (VJ2:47-50, NCC:94-95)."""
     cfg = net.cfg
     prec = tape.prec
     stg = net.staged(prec)
@@ -459,8 +469,16 @@ def output_jets(net: EigenfluidNet, tape: Tape):
     nout = cfg.ncomp * cfg.modes
     C = channels(cfg.dim)
     out = torch.empty((C, n, _ld(nout)), dtype=_FIELD[prec], device=net.device)
-    H.call("neb_jet_output_jets", prec, cfg.dim, n, k, nout, Y.data_ptr(), Y.shape[2], stg["W"][-1].data_ptr(),
-           stg["W"][-1].shape[1], stg["b"][-1].data_ptr(), out.data_ptr(), out.shape[2], _stream())
+    if envelope is not None:
+        env = envelope.to(device=net.device, dtype=_FIELD[prec]).contiguous()
+        if env.shape != (C, n):
+            raise ValueError(f"envelope must be [{C}, {n}]")
+        H.call("neb_jet_output_jets_enveloped", prec, cfg.dim, n, k, nout, Y.data_ptr(), Y.shape[2],
+               stg["W"][-1].data_ptr(), stg["W"][-1].shape[1], stg["b"][-1].data_ptr(), env.data_ptr(),
+               out.data_ptr(), out.shape[2], _stream())
+    else:
+        H.call("neb_jet_output_jets", prec, cfg.dim, n, k, nout, Y.data_ptr(), Y.shape[2], stg["W"][-1].data_ptr(),
+               stg["W"][-1].shape[1], stg["b"][-1].data_ptr(), out.data_ptr(), out.shape[2], _stream())
     out = out[:, :, :nout]
     d = cfg.dim
     return out[0], out[1:1 + d].permute(1, 2, 0), out[1 + d:].permute(1, 2, 0)
@@ -801,6 +819,85 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
             A = An
     if on_gpu:
         main.wait_stream(side)
+    return grads
+
+
+def _output_wt(net: EigenfluidNet, prec: int) -> torch.Tensor:
+    """The output weights transposed, [width][ld(nout)] in the operand
+    precision (cached per parameter version)."""
+    key = ("out_wt", prec, net.version)
+    if key not in net._staged:
+        W = net.W[-1]
+        fo, fi = W.shape
+        wt = torch.empty((fi, _ld(fo)), dtype=_STORE[prec], device=net.device)
+        H.call("neb_stage", _DT[net.master_dtype], prec, W.data_ptr(), fo, fi, fi, wt.data_ptr(), wt.shape[1], fi,
+               wt.shape[1], 1, 1.0, _stream())
+        net._staged[key] = wt
+    return net._staged[key]
+
+
+def backward_jets(net: EigenfluidNet, tape: Tape, adj_value: torch.Tensor, adj_grad: torch.Tensor,
+                  adj_hess: torch.Tensor, envelope: torch.Tensor | None = None) -> torch.Tensor:
+    """Parameter gradient of <adj, output jets> for a cotangent of the output
+    jets (value [n, nout], grad [n, nout, d], hess [n, nout, P]): the
+    analogue of VJ2.backward (VJ2:77-82) -- the envelope product's transpose
+    (neb_envelope_reverse) when an envelope is given, the output layer's
+    weight / bias gradient (cross product over all channels, VJ2:28-30) and
+    its jet transpose fused with the last activation's
+    (neb_jet_layer_backward on the output weights), then the hidden-layer
+    reverse as in backward_fields.  Returns the flat gradient."""
+    cfg = net.cfg
+    prec = tape.prec
+    d, C, n = cfg.dim, channels(cfg.dim), tape.n
+    nout = cfg.ncomp * cfg.modes
+    width = cfg.widths[-1]
+    L = len(cfg.widths)
+    st = _STORE[prec]
+    s = _stream()
+    adj = torch.cat([adj_value.reshape(1, n, nout), adj_grad.permute(2, 0, 1), adj_hess.permute(2, 0, 1)], 0)
+    adj = adj.to(device=net.device, dtype=_FIELD[prec]).contiguous()
+    A_out = torch.empty((C, n, _ld(nout)), dtype=st, device=net.device)
+    if envelope is not None:
+        env = envelope.to(device=net.device, dtype=_FIELD[prec]).contiguous()
+        if env.shape != (C, n):
+            raise ValueError(f"envelope must be [{C}, {n}]")
+        H.call("neb_envelope_reverse", prec, d, n, nout, env.data_ptr(), adj.data_ptr(), nout, A_out.data_ptr(),
+               A_out.shape[2], s)
+    elif C * n:
+        H.call("neb_stage", _DT[adj.dtype], prec, adj.data_ptr(), C * n, nout, nout, A_out.data_ptr(),
+               A_out.shape[2], C * n, A_out.shape[2], 0, 1.0, s)
+    grads = torch.empty_like(net.params)
+    gW, gb = net.split(grads)
+    mode = _CROSS[prec]
+    Ylast = tape.inputs[-1]
+    gW[-1].copy_(cross([(A_out.view(C * n, -1), Ylast.view(C * n, -1), C * n, 1.0)], nout, width, mode))
+    ws = workspace(net.device).get(H.load().neb_colsum_workspace_size(n, nout))
+    db = torch.empty(nout, dtype=torch.float64, device=net.device)
+    H.call("neb_colsum", _DT[st], A_out[0].data_ptr(), n, nout, A_out.shape[2], 0.0, db.data_ptr(), ws.data_ptr(),
+           ws.numel(), s)
+    gb[-1].copy_(db)
+    wt = _output_wt(net, prec)
+    A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
+    H.call("neb_jet_layer_backward", prec, d, cfg.act_code, n, width, nout, A_out.data_ptr(), A_out.shape[2],
+           wt.data_ptr(), wt.shape[1], tape.pre[-1].data_ptr(), tape.pre[-1].shape[2], cfg.omega(L - 1),
+           A.data_ptr(), A.shape[2], s)
+    stg = net.staged(prec)
+    for l in range(L - 1, -1, -1):
+        X = tape.inputs[l]
+        k = X.shape[2] if l > 0 else cfg.features
+        wl = cfg.widths[l]
+        gW[l].copy_(cross([(A.view(C * n, A.shape[2]), X.view(C * n, X.shape[2]), C * n, 1.0)], wl, k, mode))
+        ws = workspace(net.device).get(H.load().neb_colsum_workspace_size(n, wl))
+        db = torch.empty(wl, dtype=torch.float64, device=net.device)
+        H.call("neb_colsum", _DT[st], A[0].data_ptr(), n, wl, A.shape[2], 0.0, db.data_ptr(), ws.data_ptr(),
+               ws.numel(), s)
+        gb[l].copy_(db)
+        if l > 0:
+            An = torch.empty((C, n, _ld(k)), dtype=st, device=net.device)
+            H.call("neb_jet_layer_backward", prec, d, cfg.act_code, n, k, wl, A.data_ptr(), A.shape[2],
+                   stg["WT"][l].data_ptr(), stg["WT"][l].shape[1], tape.pre[l - 1].data_ptr(),
+                   tape.pre[l - 1].shape[2], cfg.omega(l - 1), An.data_ptr(), An.shape[2], s)
+            A = An
     return grads
 
 

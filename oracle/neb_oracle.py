@@ -114,6 +114,30 @@ def activation_derivs(kind: str, u: np.ndarray):
     raise ValueError(f"unknown activation {kind!r}")
 
 
+def envelope_product(e: Jet, x: Jet) -> Jet:
+    """Jet2 product y = e x of a per-point scalar envelope jet e (value [n],
+    grad [n, d], hess [n, P]) with output jets x ([n, f] ...):
+    grad y = e grad x + x grad e, hess_ij y = e hess_ij x + x hess_ij e +
+    e_i x_j + e_j x_i (VJ2:47-50 `product`)."""
+    ev, eg, eh = e.value[:, None], e.grad[:, None, :], e.hess[:, None, :]
+    d = x.grad.shape[-1]
+    hess = np.stack([ev * x.hess[..., p] + eh[..., p] * x.value + eg[..., i] * x.grad[..., j]
+                     + eg[..., j] * x.grad[..., i] for p, (i, j) in enumerate(pairs(d))], -1)
+    return Jet(ev * x.value, ev[..., None] * x.grad + eg * x.value[..., None], hess)
+
+
+def envelope_product_reverse(e: Jet, a: Jet) -> Jet:
+    """Transpose of envelope_product in x (VJ2:51-55 `product_reverse`)."""
+    ev, eg, eh = e.value[:, None], e.grad[:, None, :], e.hess[:, None, :]
+    d = a.grad.shape[-1]
+    av = ev * a.value + np.sum(eg * a.grad, -1) + np.sum(eh * a.hess, -1)
+    ag = ev[..., None] * a.grad
+    for p, (i, j) in enumerate(pairs(d)):
+        ag[..., i] += eg[..., j] * a.hess[..., p]
+        ag[..., j] += eg[..., i] * a.hess[..., p]
+    return Jet(av, ag, ev[..., None] * a.hess)
+
+
 def linear_jet(x: Jet, W: np.ndarray, b: np.ndarray | None) -> Jet:
     """Packed-channel linear layer: all channels times W^T, bias on value rows
     only (HB2P:1005-1036, VJ2:26-27)."""

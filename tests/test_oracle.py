@@ -526,3 +526,52 @@ def test_velocity_peel_fourier_closed_form():
     np.testing.assert_allclose(res["rayleigh"], np.sort(k2), rtol=1e-12)
     np.testing.assert_array_equal(res["order"], np.argsort(k2, kind="stable"))
     np.testing.assert_allclose(res["relative_mass"], 1.0, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# envelope product (VJ2:47-55)
+# --------------------------------------------------------------------------
+def _gauss_envelope(x, c=0.3):
+    """e = exp(-c |x|^2) with its exact gradient and Hessian pairs."""
+    d = x.shape[1]
+    e = np.exp(-c * np.sum(x * x, 1))
+    g = -2 * c * x * e[:, None]
+    h = np.stack([(4 * c * c * x[:, i] * x[:, j] - (2 * c if i == j else 0.0)) * e for i, j in o.pairs(d)], -1)
+    return o.Jet(e, g, h)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_envelope_product_matches_spatial_fd(d):
+    """The product jets are the jets of the product function: FD of the
+    enveloped value -> gradient, of the enveloped gradient -> Hessian."""
+    rng = np.random.default_rng(40 + d)
+    net = _net(d, 2, [10], 3, seed=d)
+    lo, hi = (0.0, 2 * np.pi) if d == 2 else (-1.0, 1.0)
+    x = rng.uniform(lo + 0.3, hi - 0.3, (6, d))
+
+    def prod(xx):
+        return o.envelope_product(_gauss_envelope(xx), _eval(net, xx, 2, lo, hi))
+
+    y = prod(x)
+    eps = 1e-5
+    for a in range(d):
+        dx = np.zeros(d); dx[a] = eps
+        yp, ym = prod(x + dx), prod(x - dx)
+        np.testing.assert_allclose((yp.value - ym.value) / (2 * eps), y.grad[..., a], rtol=1e-7, atol=1e-8)
+        for p, (i, j) in enumerate(o.pairs(d)):
+            if i == a:
+                np.testing.assert_allclose((yp.grad[..., j] - ym.grad[..., j]) / (2 * eps), y.hess[..., p],
+                                           rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_envelope_product_reverse_is_transpose(d):
+    rng = np.random.default_rng(50 + d)
+    n, f, P = 11, 4, d * (d + 1) // 2
+    e = o.Jet(rng.normal(size=n), rng.normal(size=(n, d)), rng.normal(size=(n, P)))
+    x = o.Jet(rng.normal(size=(n, f)), rng.normal(size=(n, f, d)), rng.normal(size=(n, f, P)))
+    a = o.Jet(rng.normal(size=(n, f)), rng.normal(size=(n, f, d)), rng.normal(size=(n, f, P)))
+    y, r = o.envelope_product(e, x), o.envelope_product_reverse(e, a)
+    lhs = np.sum(y.value * a.value) + np.sum(y.grad * a.grad) + np.sum(y.hess * a.hess)
+    rhs = np.sum(x.value * r.value) + np.sum(x.grad * r.grad) + np.sum(x.hess * r.hess)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
