@@ -331,6 +331,19 @@ size_t neb_galerkin_workspace_size(int64_t n, int64_t K);
 int neb_galerkin_rhs2d(int dtype, const void* gu, const void* gv, const void* gw, int64_t n, int64_t K, int64_t ld,
                        const double* a, double w, double* rhs, void* workspace, size_t ws_bytes, void* stream);
 
+/* One block of B per-mode networks as a single layer of a grouped GEMM
+ * (the reference's mode blocks, HB2P:1716-1880 / NCC:84-89 `cp.matmul` over
+ * [B, out, in] stacks): from the mode-stacked master weights [K][out][in]
+ * (fp32 or fp64), modes m0 .. m0+B-1, writes the [B*out][ld] operand --
+ * diag = 0: the modes' weights stacked by rows (a first layer that reads
+ * shared input features); diag = 1: block-diagonal (each mode reads its own
+ * block of B*in inputs) -- in dst_dtype (bf16 / fp32 / fp64), columns
+ * beyond the width zero; lo: the split lo plane (bf16 only) or NULL; wt: the
+ * transpose [width][ldt] (for the reverse) or NULL. */
+int neb_stage_mode_block(int master_dtype, const void* master, int64_t K, int64_t out, int64_t in, int64_t m0,
+                         int64_t B, int diag, int dst_dtype, void* hi, void* lo, int64_t ld, void* wt, int64_t ldt,
+                         void* stream);
+
 /* Every layer's bf16 operand forms in one launch, from the flat master
  * parameters (fp32 or fp64): layer l's weight W_l is the row-major
  * [rows[l]][cols[l]] block at element offset off[l]; hi[l] = bf16(W_l) as

@@ -61,6 +61,8 @@ SIGNATURES = {
     "neb_adam_gated": (_int, [_int, _vp, _vp, _vp, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _i64, _vp, _vp, _vp]),
     "neb_galerkin_workspace_size": (_sz, [_i64, _i64]),
     "neb_galerkin_rhs2d": (_int, [_int, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _dbl, _vp, _vp, _sz, _vp]),
+    "neb_stage_mode_block": (_int, [_int, _vp, _i64, _i64, _i64, _i64, _i64, _int, _int, _vp, _vp, _i64, _vp, _i64,
+                                    _vp]),
     "neb_stage_net": (_int, [_int, _vp, _int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "neb_stage": (_int, [_int, _int, _vp, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _int, _dbl, _vp]),
     "neb_sym_shift": (_int, [_i64, _vp, _i64, _dbl, _vp, _i64, _vp]),

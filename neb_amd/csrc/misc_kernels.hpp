@@ -197,6 +197,46 @@ __global__ __launch_bounds__(256) void split_kernel(const TS* __restrict__ src, 
   lo[r * ldd + c] = (__bf16)(float)(x - (double)(float)h);
 }
 
+// Weights of one block of B per-mode networks as a single layer (the mode
+// block as a grouped GEMM): from the mode-stacked master S [K][out][in]
+// (row-major per mode), modes m0 .. m0+B-1:
+//   diag = 0 (the first layer, every mode reads the same input features):
+//     rows m*out + o, columns i           -> S[m0+m][o][i]
+//   diag = 1 (hidden and output layers, every mode reads its own inputs):
+//     rows m*out + o, columns m'*in + i   -> S[m0+m][o][i] if m == m' else 0
+// written as [B*out][ld] (columns beyond the width zero) in the destination
+// precision, optionally with the split lo plane (bf16 only) and the transpose
+// [width][ldt] (width = in or B*in).
+template <typename TS, typename TD>
+__global__ __launch_bounds__(256) void stage_modes_kernel(const TS* __restrict__ S, int64_t out, int64_t in,
+                                                          int64_t m0, int64_t B, int diag, TD* __restrict__ hi,
+                                                          TD* __restrict__ lo, int64_t ld, TD* __restrict__ wt,
+                                                          int64_t ldt) {
+  const int64_t rows = B * out, cols = diag ? B * in : in;
+  const int64_t nw = rows * ld;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  auto value = [&](int64_t r, int64_t c) -> double {
+    if (r >= rows || c >= cols) return 0.0;
+    const int64_t m = r / out, o = r % out;
+    int64_t i = c;
+    if (diag) {
+      if (c / in != m) return 0.0;
+      i = c % in;
+    }
+    return todbl_any(S[((m0 + m) * out + o) * in + i]);
+  };
+  if (idx < nw) {
+    const double x = value(idx / ld, idx % ld);
+    const TD h = cvt<TD>(x);
+    hi[idx] = h;
+    if (lo) lo[idx] = cvt<TD>(x - todbl_any(h));
+    return;
+  }
+  const int64_t it = idx - nw;
+  if (!wt || it >= cols * ldt) return;
+  wt[it] = cvt<TD>(value(it % ldt, it / ldt));
+}
+
 // Transpose of the envelope product (jet_product_reverse, VJ2:51-55) for
 // every (point, output) pair: adj [C][n][lda] (math precision) -> out
 // [C][n][ldo] in the operand precision of the output-layer reverse.

@@ -1183,6 +1183,37 @@ int neb_stage_net(int master_dtype, const void* master, int nl, const int64_t* o
   return fail(NEB_ERR_ARG, "neb_stage_net: master dtype must be NEB_F32 or NEB_F64 (got %d)", master_dtype);
 }
 
+int neb_stage_mode_block(int master_dtype, const void* master, int64_t K, int64_t out, int64_t in, int64_t m0,
+                         int64_t B, int diag, int dst_dtype, void* hi, void* lo, int64_t ld, void* wt, int64_t ldt,
+                         void* stream) {
+  if (K < 1 || out < 1 || in < 1 || B < 1 || m0 < 0 || m0 + B > K || (diag != 0 && diag != 1))
+    return fail(NEB_ERR_SHAPE, "neb_stage_mode_block: bad sizes");
+  const int64_t rows = B * out, cols = diag ? B * in : in;
+  if (ld < cols || (wt && ldt < rows)) return fail(NEB_ERR_SHAPE, "neb_stage_mode_block: bad strides");
+  if (!master || !hi) return fail(NEB_ERR_ARG, "neb_stage_mode_block: null pointer");
+  if (lo && dst_dtype != NEB_BF16) return fail(NEB_ERR_ARG, "neb_stage_mode_block: lo planes are bf16 only");
+  const int64_t tot = rows * ld + (wt ? cols * ldt : 0);
+  if (tot / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_stage_mode_block: too many elements");
+  const dim3 grid((unsigned)((tot + 255) / 256));
+  hipStream_t st = S(stream);
+#define SMB(TS, TD)                                                                                              \
+  hipLaunchKernelGGL((stage_modes_kernel<TS, TD>), grid, dim3(256), 0, st, reinterpret_cast<const TS*>(master), out, \
+                     in, m0, B, diag, reinterpret_cast<TD*>(hi), reinterpret_cast<TD*>(lo), ld,                    \
+                     reinterpret_cast<TD*>(wt), ldt);                                                             \
+  return hip_status("neb_stage_mode_block");
+  if (master_dtype == NEB_F32) {
+    if (dst_dtype == NEB_BF16) { SMB(float, __bf16) }
+    if (dst_dtype == NEB_F32) { SMB(float, float) }
+    if (dst_dtype == NEB_F64) { SMB(float, double) }
+  } else if (master_dtype == NEB_F64) {
+    if (dst_dtype == NEB_BF16) { SMB(double, __bf16) }
+    if (dst_dtype == NEB_F32) { SMB(double, float) }
+    if (dst_dtype == NEB_F64) { SMB(double, double) }
+  }
+#undef SMB
+  return fail(NEB_ERR_ARG, "neb_stage_mode_block: unsupported dtypes %d -> %d", master_dtype, dst_dtype);
+}
+
 int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64_t cols, int64_t lds, void* dst,
               int64_t ldd, int64_t drows, int64_t dcols, int transpose, double scale, void* stream) {
   if (!src || !dst || rows < 0 || cols < 0 || drows < 0 || dcols < 0 || ldd < dcols)

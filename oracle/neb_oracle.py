@@ -778,3 +778,74 @@ def assemble_operators(B, D, dV: float):
     diff = jj != kk
     Q[:, diff] += q[kk[diff], :, jj[diff]].T
     return M, A, dV * Q
+
+
+# --------------------------------------------------------------------------
+# next row: per-mode SIREN networks (HB2P:1590-1880; NCC:111-185)
+# --------------------------------------------------------------------------
+def mode_nets(Ws, bs, omega_first: float, omega_hidden: float) -> list:
+    """One sine Net per mode from mode-stacked weights W_j [K, out, in],
+    b_j [K, out] (omega_first on the first layer, omega_hidden after)."""
+    L = len(Ws) - 1
+    om = [omega_first] + [omega_hidden] * (L - 1)
+    return [Net([W[k].astype(np.float64) for W in Ws], [b[k].astype(np.float64) for b in bs], "sin", om)
+            for k in range(Ws[0].shape[0])]
+
+
+def mode_fields(nets: list, x0: Jet, dim: int, stiffness: str = "vorticity"):
+    """Field matrices of per-mode networks: column k is mode k's lifted
+    fields (point-major rows as field_rows).  Returns (U, D, tapes)."""
+    Us, Ds, tapes = [], [], []
+    for net in nets:
+        out, tape = forward(net, x0, record=True)
+        tapes.append(tape)
+        if dim == 2:
+            u, d = lift2d_hessian(out) if stiffness == "hessian" else lift2d(out)
+        else:
+            u, d = lift3d(out)
+        Us.append(field_rows(u))
+        Ds.append(field_rows(d) if d.ndim > 2 else d)
+    return np.concatenate(Us, 1), np.concatenate(Ds, 1), tapes
+
+
+def mode_reverse(nets: list, tapes: list, U: np.ndarray, D: np.ndarray, w: float, dM: np.ndarray,
+                 dS: np.ndarray, dim: int, stiffness: str = "vorticity"):
+    """Per-mode parameter gradients of <dM, w U^T U> + <dS, w D^T D>:
+    mode k's field cotangents are columns k of 2w U dM, 2w D dS, pulled back
+    through its lift and network.  Returns ([gW_j [K, out, in]], [gb_j])."""
+    n = len(tapes[0][1].value)
+    gU, gD = field_adjoint(U, w, dM), field_adjoint(D, w, dS)
+    gWs, gbs = [], []
+    for k, (net, tape) in enumerate(zip(nets, tapes)):
+        if dim == 2:
+            gu = gU[:, k].reshape(n, 2, 1)
+            if stiffness == "hessian":
+                adj = lift2d_hessian_reverse(gu, gD[:, k].reshape(n, 3, 1))
+            else:
+                adj = lift2d_reverse(gu, gD[:, k].reshape(n, 1))
+        else:
+            adj = lift3d_reverse(gU[:, k].reshape(n, 3, 1), gD[:, k].reshape(n, 3, 3, 1))
+        gW, gb = backward(net, tape, adj)
+        gWs.append(gW)
+        gbs.append(gb)
+    L = len(gWs[0])
+    return [np.stack([g[j] for g in gWs]) for j in range(L)], [np.stack([g[j] for g in gbs]) for j in range(L)]
+
+
+def mode_adam(Ws, bs, gWs, gbs, mWs, vWs, mbs, vbs, t: int, lr: float, clip: float = 8.0):
+    """Adam with a per-mode norm clip (NCC:156-171): the norm of mode k runs
+    over all of its weights and biases; mode k's gradient is scaled by
+    min(1, clip / max(norm_k, 1e-30)); nonfinite or zero norms raise."""
+    K = Ws[0].shape[0]
+    norms = np.sqrt(sum(np.sum(g.reshape(K, -1) ** 2, 1) for g in [*gWs, *gbs]))
+    if not np.all(np.isfinite(norms) & (norms > 0)):
+        raise FloatingPointError("Invalid parameter gradients; no optimizer update")
+    scales = np.minimum(1.0, clip / np.maximum(norms, 1e-30))
+    out = []
+    for p, m, v, g in zip([*Ws, *bs], [*mWs, *mbs], [*vWs, *vbs], [*gWs, *gbs]):
+        g = g * scales.reshape((K,) + (1,) * (g.ndim - 1))
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        p = p - lr * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+        out.append((p, m, v))
+    return out, norms

@@ -1,0 +1,188 @@
+"""GPU parity of the per-mode SIREN networks (neb_amd.modes: the
+reference's own architecture, HB2P:1590-1880 / NCC:111-185) against the
+oracle's per-mode networks, through the C ABI: fields of every mode in 2D
+(vorticity and Hessian stiffness) and 3D, independence of the mode-block
+size, the reverse pass, the coefficient-peel step, the per-mode clipped
+Adam and both checkpoint layouts."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import neb_oracle as o
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # CPU-only host: the gpu tier runs on the MI355X box
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+from neb_amd import modes as md  # noqa: E402
+from neb_amd import network as nw  # noqa: E402
+from neb_amd import spectral as sp  # noqa: E402
+
+DEV = "cuda"
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make(dim=2, K=24, block=8, hidden=16, layers=2, stiffness="vorticity", master=torch.float64, seed=0):
+    if dim == 2:
+        cfg = md.ModeConfig(dim=2, frequencies=2, hidden=hidden, hidden_layers=layers, modes=K, omega_first=2.0,
+                            omega_hidden=1.5, block=block, stiffness=stiffness)
+    else:
+        cfg = md.ModeConfig(dim=3, frequencies=2, lo=(-1.0,) * 3, hi=(1.0,) * 3, hidden=hidden, hidden_layers=layers,
+                            modes=K, omega_first=2.0, omega_hidden=1.5, block=block)
+    net = md.ModeNet(cfg, seed=seed, device=DEV, master_dtype=master)
+    return cfg, net
+
+
+def oracle_of(net):
+    Ws = [W.detach().cpu().double().numpy() for W in net.W]
+    bs = [b.detach().cpu().double().numpy() for b in net.b]
+    return o.mode_nets(Ws, bs, net.cfg.omega_first, net.cfg.omega_hidden)
+
+
+def points(dim, n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0, 2 * np.pi, (n, 2)) if dim == 2 else rng.uniform(-1, 1, (n, 3))
+
+
+def field_np(f, which):
+    return nw.field_matrix(f, which, torch.float64).cpu().numpy()
+
+
+def test_siren_init_streams_and_bounds():
+    """Mode i's weights come from its own stream: independent of K; every
+    layer within its SIREN bound."""
+    c8 = md.ModeConfig(modes=8, hidden=16, hidden_layers=2, frequencies=2, block=8)
+    c16 = md.ModeConfig(modes=16, hidden=16, hidden_layers=2, frequencies=2, block=8)
+    W8, b8 = md.siren_init(c8, 3)
+    W16, b16 = md.siren_init(c16, 3)
+    for j in range(3):
+        assert np.array_equal(W8[j], W16[j][:8]) and np.array_equal(b8[j], b16[j][:8])
+        fi = c8.sizes[j]
+        bound = 1.0 / fi if j == 0 else math.sqrt(6.0 / fi) / c8.omega_hidden
+        assert np.abs(W16[j]).max() <= bound and np.abs(b16[j]).max() <= bound
+        assert np.abs(W16[j]).max() > 0.9 * bound
+
+
+@pytest.mark.parametrize("dim,stiff", [(2, "vorticity"), (2, "hessian"), (3, "grad_u")])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("bf16x3", 2e-5), ("f32", 2e-5)])
+def test_mode_fields_match_oracle(dim, stiff, prec, tol):
+    cfg, net = make(dim, K=24 if dim == 2 else 16, stiffness=stiff if dim == 2 else "vorticity",
+                    master=torch.float64 if prec == "f64" else torch.float32)
+    x = points(dim, 257, 1)
+    U, D, _ = o.mode_fields(oracle_of(net), o.feature_jets(x, 2, cfg.lo, cfg.hi), dim,
+                            stiff if dim == 2 else "vorticity")
+    f = net.fields(torch.tensor(x), prec)
+    r = rel(field_np(f, "mass"), U); assert r < tol, r
+    r = rel(field_np(f, "stiff"), D); assert r < tol, r
+
+
+def test_mode_block_size_does_not_change_fields():
+    cfg, net8 = make(2, K=24, block=8)
+    _, net24 = make(2, K=24, block=24)
+    x = torch.tensor(points(2, 300, 2))
+    f8, f24 = net8.fields(x, "f64"), net24.fields(x, "f64")
+    assert rel(field_np(f8, "mass"), field_np(f24, "mass")) < 1e-14
+    assert rel(field_np(f8, "stiff"), field_np(f24, "stiff")) < 1e-14
+
+
+@pytest.mark.parametrize("dim,stiff", [(2, "vorticity"), (2, "hessian"), (3, "grad_u")])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-9), ("bf16x3", 4e-2)])
+def test_mode_reverse_matches_oracle(dim, stiff, prec, tol):
+    K = 24 if dim == 2 else 16
+    cfg, net = make(dim, K=K, stiffness=stiff if dim == 2 else "vorticity",
+                    master=torch.float64 if prec == "f64" else torch.float32)
+    x = points(dim, 300, 3)
+    w = 1.0 / 300
+    rng = np.random.default_rng(9)
+    dM = rng.normal(size=(K, K)); dM = dM + dM.T
+    dS = rng.normal(size=(K, K)); dS = dS + dS.T
+    nets = oracle_of(net)
+    st = stiff if dim == 2 else "vorticity"
+    U, D, tapes = o.mode_fields(nets, o.feature_jets(x, 2, cfg.lo, cfg.hi), dim, st)
+    gW, gb = o.mode_reverse(nets, tapes, U, D, w, dM, dS, dim, st)
+    xt = torch.tensor(x)
+    f = net.fields(xt, prec, split=prec != "f64")
+    g = net.backward_fields(xt, f, torch.tensor(dM, device=DEV), torch.tensor(dS, device=DEV), w, prec)
+    GW, Gb = net.split(g)
+    for j, (a, b) in enumerate(zip(GW, gW)):
+        r = rel(a.cpu(), b); assert r < tol, ("W", j, r)
+    for j, (a, b) in enumerate(zip(Gb[:-1], gb[:-1])):
+        r = rel(a.cpu(), b); assert r < tol, ("b", j, r)
+
+
+def test_mode_peel_step_matches_oracle():
+    """The 2D velocity peel on per-mode networks (Hessian stiffness): losses,
+    order and gradient against the oracle's Cholesky peel of the oracle's
+    Grams and the per-mode reverse of its detached-prefix cotangents."""
+    K = 16
+    cfg, net = make(2, K=K, stiffness="hessian", hidden=16, layers=2)
+    x = points(2, 400, 4)
+    w = 4 * math.pi ** 2 / 400
+    nets = oracle_of(net)
+    U, D, tapes = o.mode_fields(nets, o.feature_jets(x, 2, cfg.lo, cfg.hi), 2, "hessian")
+    M, S = o.gram(U, w), o.gram(D, w)
+    pk = o.coefficient_peel(M, S, ortho_tol=1e-3)
+    gW, gb = o.mode_reverse(nets, tapes, U, D, w, pk["Cu"] / 2, pk["Cd"] / 2, 2, "hessian")
+    res = net.step(torch.tensor(x), w, prec="f64", objective="peel",
+                   peel_options=dict(ortho_tol=1e-3, rel_min=1e-4, mass_floor=1e-12))
+    assert np.array_equal(res["peel"]["order"].cpu().numpy(), pk["order"])
+    r = rel(res["losses"].cpu(), pk["losses"]); assert r < 1e-9, r
+    GW, _ = net.split(res["grads"])
+    for j, (a, b) in enumerate(zip(GW, gW)):
+        r = rel(a.cpu(), b); assert r < 1e-8, (j, r)
+
+
+def test_mode_adam_matches_oracle_over_three_steps():
+    cfg, net = make(2, K=16, master=torch.float64)
+    rng = np.random.default_rng(10)
+    Ws = [W.cpu().numpy().copy() for W in net.W]
+    bs = [b.cpu().numpy().copy() for b in net.b]
+    mW = [np.zeros_like(W) for W in Ws]; vW = [np.zeros_like(W) for W in Ws]
+    mb = [np.zeros_like(b) for b in bs]; vb = [np.zeros_like(b) for b in bs]
+    for t in range(1, 4):
+        gW = [rng.normal(size=W.shape) * (50.0 if t == 2 else 1.0) for W in Ws]
+        gb = [rng.normal(size=b.shape) for b in bs]
+        flat = torch.zeros_like(net.params)
+        FW, Fb = net.split(flat)
+        for dst, src in zip(FW + Fb, gW + gb):
+            dst.copy_(torch.tensor(src))
+        norms = net.adam(flat, 1e-3, clip=8.0)
+        out, onorms = o.mode_adam(Ws, bs, gW, gb, mW, vW, mb, vb, t, 1e-3, clip=8.0)
+        L = len(Ws)
+        Ws, bs = [q[0] for q in out[:L]], [q[0] for q in out[L:]]
+        mW, mb = [q[1] for q in out[:L]], [q[1] for q in out[L:]]
+        vW, vb = [q[2] for q in out[:L]], [q[2] for q in out[L:]]
+        assert rel(norms.cpu(), onorms) < 1e-12
+        for a, b in zip(net.W + net.b, Ws + bs):
+            r = rel(a.cpu(), b); assert r < 1e-12, (t, r)
+    assert net.epoch == 3
+    with pytest.raises(FloatingPointError):
+        net.adam(torch.zeros_like(net.params), 1e-3)
+    assert net.epoch == 3
+
+
+def test_mode_checkpoint_layouts(tmp_path):
+    cfg, net = make(2, K=8, master=torch.float32)
+    net.adam(torch.randn_like(net.params), 1e-3)
+    path = tmp_path / "modes.npz"
+    net.save(path)
+    with np.load(path, allow_pickle=False) as z:
+        assert z["W_0"].shape == (8, cfg.hidden, cfg.features) and z["W_2"].shape == (8, 1, cfg.hidden)
+        assert int(z["epoch"]) == 1
+    _, other = make(2, K=8, master=torch.float32, seed=5)
+    other.load(path)
+    assert torch.equal(other.params, net.params) and torch.equal(other.m, net.m) and other.epoch == 1
+    arrays = net.arrays()
+    assert arrays["m003.l1.weight"].shape == (cfg.hidden, cfg.hidden)
+    assert arrays["m003.output.weight"].shape == (1, cfg.hidden)
+    assert int(arrays["m007.adam.p05.step"]) == 1
+    _, third = make(2, K=8, master=torch.float32, seed=6)
+    third.from_arrays(arrays)
+    assert torch.equal(third.params, net.params) and torch.equal(third.v, net.v) and third.epoch == 1

@@ -575,3 +575,63 @@ def test_envelope_product_reverse_is_transpose(d):
     lhs = np.sum(y.value * a.value) + np.sum(y.grad * a.grad) + np.sum(y.hess * a.hess)
     rhs = np.sum(x.value * r.value) + np.sum(x.grad * r.grad) + np.sum(x.hess * r.hess)
     assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+# --------------------------------------------------------------------------
+# per-mode networks
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,stiff", [(2, "vorticity"), (2, "hessian"), (3, "grad_u")])
+def test_mode_reverse_matches_directional_fd(dim, stiff):
+    """mode_reverse's gradient of <dM, M> + <dS, S> for per-mode sine
+    networks against central differences of the same objective along random
+    directions in every mode's parameters."""
+    rng = np.random.default_rng(60 + dim)
+    K, F = 3, 1
+    sizes = [2 * F * dim, 6, 5, 1 if dim == 2 else 3]
+    Ws = [rng.normal(size=(K, fo, fi)) * 0.5 for fi, fo in zip(sizes[:-1], sizes[1:])]
+    bs = [rng.normal(size=(K, fo)) * 0.3 for fo in sizes[1:]]
+    lo, hi = (0.0, 2 * np.pi) if dim == 2 else (-1.0, 1.0)
+    x = rng.uniform(lo, hi, (40, dim))
+    x0 = o.feature_jets(x, F, lo, hi)
+    w = 0.05
+    dM = rng.normal(size=(K, K)); dM = dM + dM.T
+    dS = rng.normal(size=(K, K)); dS = dS + dS.T
+
+    def objective(Ws_, bs_):
+        U, D, _ = o.mode_fields(o.mode_nets(Ws_, bs_, 1.3, 1.1), x0, dim, stiff)
+        return np.sum(dM * o.gram(U, w)) + np.sum(dS * o.gram(D, w))
+
+    nets = o.mode_nets(Ws, bs, 1.3, 1.1)
+    U, D, tapes = o.mode_fields(nets, x0, dim, stiff)
+    gW, gb = o.mode_reverse(nets, tapes, U, D, w, dM, dS, dim, stiff)
+    for _ in range(3):
+        dW = [rng.normal(size=W.shape) for W in Ws]
+        db = [rng.normal(size=b.shape) for b in bs]
+        h = 1e-6
+        fp = objective([W + h * a for W, a in zip(Ws, dW)], [b + h * a for b, a in zip(bs, db)])
+        fm = objective([W - h * a for W, a in zip(Ws, dW)], [b - h * a for b, a in zip(bs, db)])
+        exact = sum(np.sum(g * a) for g, a in zip(gW + gb, dW + db))
+        assert abs((fp - fm) / (2 * h) - exact) <= 1e-6 * max(1.0, abs(exact)), ((fp - fm) / (2 * h), exact)
+
+
+def test_mode_adam_per_mode_clip_and_first_step():
+    rng = np.random.default_rng(70)
+    K = 4
+    Ws = [rng.normal(size=(K, 3, 2))]
+    bs = [rng.normal(size=(K, 3))]
+    gW = [rng.normal(size=(K, 3, 2))]
+    gb = [rng.normal(size=(K, 3))]
+    gW[0][2] *= 1e3          # mode 2 beyond the clip
+    gb[0][2] *= 1e3
+    z = [np.zeros_like(Ws[0])], [np.zeros_like(bs[0])]
+    out, norms = o.mode_adam(Ws, bs, gW, gb, z[0], z[0], z[1], z[1], 1, 1e-3, clip=8.0)
+    (pW, mW, _), (pb, mb, _) = out
+    assert norms[2] > 8.0 and np.all(norms[[0, 1, 3]] < 8.0)
+    # clipped mode: first moment = 0.1 * g * clip / norm; unclipped modes: 0.1 * g
+    np.testing.assert_allclose(mW[2], 0.1 * gW[0][2] * 8.0 / norms[2], rtol=1e-12)
+    np.testing.assert_allclose(mW[0], 0.1 * gW[0][0], rtol=1e-12)
+    # first step: p - lr * g / (|g| + eps) elementwise (any positive scale cancels)
+    np.testing.assert_allclose(pW, Ws[0] - 1e-3 * gW[0] / (np.abs(gW[0]) + 1e-8 / np.repeat(
+        np.minimum(1, 8.0 / norms), 6).reshape(K, 3, 2)), rtol=1e-9)
+    with pytest.raises(FloatingPointError):
+        o.mode_adam(Ws, bs, [gW[0] * 0], [gb[0] * 0], z[0], z[0], z[1], z[1], 1, 1e-3)
