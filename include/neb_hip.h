@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 7
+#define NEB_ABI_VERSION 8
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -195,6 +195,41 @@ int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width
 int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* adj_in,
                            int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev, int64_t ldz,
                            double omega_prev, void* adj_out, int64_t lda_out, void* stream);
+
+/* Factored output-layer reverse (split-precision tapes).  The fields are
+ * linear in the output weights, Phi_a = sum_t sg_t Y_{c_t} W_{s_t}^T over the
+ * lift's terms t (channel c_t of the last hidden jets Y, output-weight slice
+ * s_t: rows 3k+s in 3D, all rows in 2D), so the field adjoint
+ * 2w Phi dL/dG^T through the lift transpose and the output layer is
+ *   adj Y_c = sum_c' Y_c' R_c'c,   R_c'c = 2w sum sg sg' W_s'^T dL/dG^T W_s
+ * with width x width blocks R formed in FP64 from the master weights: a
+ * GEMM whose inner length is the hidden width instead of K, and whose
+ * K x K contraction (where an ill-conditioned mass makes dL/dM large and
+ * the sum cancel) runs in FP64.  It replaces the same operations as
+ * neb_jet_output_backward (VCORE:94-100 gu = U Cu/n and the lift /
+ * product / linear transposes of VJ2:77-82, NCC:97-109), in three steps:
+ *
+ * neb_stack_channels: channels ch[0..nch-1] of a packed bf16 jet tensor
+ * ([C][n][ld], hi plane and optionally lo plane) side by side per point:
+ * dst[p][i*width + j] = src[ch[i]][p][j] ([n][ldd]; width a multiple of 8,
+ * nch <= 16).  The stacked last hidden jets are the operand of the GEMM
+ * below and of the channel Grams Y_c'^T Y_c (neb_gram_split) that give the
+ * output-weight gradient 2w sum sg sg' dL/dG W_s' Y_c'^T Y_c.
+ * neb_gemm_split: out[p][f] = sum_k x[p][k] w[f][k] on split operands (hi /
+ * lo bf16 planes, the three products of the split layers), fp32 result
+ * [n][ldo]: the stacked adjoint Y_stack R, with w = R^T staged as planes.
+ * neb_jet_act_reverse: the transpose of the last hidden activation on its
+ * own: adj = omega * act-jet^T(z, a), where a holds the adjoints of jet
+ * channels ch[i] as fp32 columns [i*width, (i+1)*width) of [n][lda] (other
+ * channels zero) and z / adj are the packed tape and result ([C][n][ld*],
+ * prec NEB_BF16 or NEB_F32); FUSED:9-19, HB2P:1147-1174. */
+int neb_stack_channels(const void* src_hi, const void* src_lo, int64_t n, int64_t ld, int64_t width, int nch,
+                       const int* ch, void* dst_hi, void* dst_lo, int64_t ldd, void* stream);
+int neb_gemm_split(int64_t n, int64_t k, int64_t nout, const void* x_hi, const void* x_lo, int64_t ldx,
+                   const void* w_hi, const void* w_lo, int64_t ldw, float* out, int64_t ldo, void* stream);
+int neb_jet_act_reverse(int prec, int dim, int act, int64_t n, int64_t width, const float* adj, int64_t lda, int nch,
+                        const int* ch, const void* z, int64_t ldz, double omega, void* out, int64_t ldo,
+                        void* stream);
 
 /* Split-R cross product  G = beta*G + scale * sum_s sign_s X_s^T diag(roww) Y_s
  * over rows, FP64 result [m][ldg].  Segment s: X_s [rows_s][ldx_s] with m

@@ -895,3 +895,73 @@ template <int ACT> struct PlanOutBwd3 {
     reverse_epilogue<P, 3, ACT, GE, M>(a, p, f, nv, adj);
   }
 };
+
+// Plain GEMM on split operands with an fp32 result (see neb_gemm_split):
+// out[p][f] = sum_k x[p][k] w[f][k] with x = x[0] + x[1] and w = w[0] + w[1]
+// (hi / lo bf16 planes; the three products of the split hidden layers).
+struct PlanGemmSplit {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return 4; }
+  static constexpr int CI = 2, NW = 2, NA = 1, NT = 3;
+  static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
+  static constexpr int ta[NT] = {0, 0, 0};
+  static constexpr int tc[NT] = {0, 1, 0};
+  static constexpr int tw[NT] = {1, 0, 0};
+  static constexpr bool tneg[NT] = {false, false, false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    st_group<float, GE, M>(reinterpret_cast<float*>(a.out0) + p * a.ldo + f, v[0], nv);
+  }
+};
+
+// The activation-jet transpose of a layer on its own (see
+// neb_jet_act_reverse): the adjoint of the post-activation jet channels
+// map.ch[i], given as fp32 columns [i * width, (i+1) * width) of a
+// point-major matrix (channels not in the map have a zero adjoint), to the
+// adjoint of the scaled pre-activation jet: out = omega * f^T(Z, adj),
+// packed [C][n][ldo].  Eight features per thread.
+template <int D, int ACT, typename S>
+__global__ __launch_bounds__(256) void act_reverse_stacked_kernel(const float* __restrict__ adj, int64_t lda,
+                                                                  ChanMap map, const S* __restrict__ z, int64_t ldz,
+                                                                  int64_t n, int64_t width, double omega,
+                                                                  S* __restrict__ out, int64_t ldo) {
+  constexpr int C = Jet2Layout<D>::C;
+  const int64_t per = (width + 7) / 8;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * per) return;
+  const int64_t p = idx / per, f0 = (idx % per) * 8;
+  const int nv = (int)(width - f0 < 8 ? width - f0 : 8);
+  float u[C][8], a[C][8];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    ld_group<S, 8, float>(z + ((int64_t)c * n + p) * ldz + f0, u[c], nv);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) a[c][e] = 0.0f;
+  }
+#pragma unroll
+  for (int t = 0; t < STACK_MAXCH; ++t) {
+    if (t < map.n) {
+      const int c = map.ch[t];
+      float v[8];
+      ld_group<float, 8, float>(adj + p * lda + (int64_t)t * width + f0, v, nv);
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc)
+        if (cc == c) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) a[cc][e] += v[e];
+        }
+    }
+  }
+  const float om = (float)omega;
+  float b[C][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    float uc[C], ac[C], bc[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) { uc[c] = u[c][e]; ac[c] = a[c][e]; }
+    act_jet_reverse<D, ACT, float>(uc, ac, bc);
+#pragma unroll
+    for (int c = 0; c < C; ++c) b[c][e] = om * bc[c];
+  }
+#pragma unroll
+  for (int c = 0; c < C; ++c) st_group<S, 8, float>(out + ((int64_t)c * n + p) * ldo + f0, b[c], nv);
+}

@@ -333,3 +333,27 @@ __global__ __launch_bounds__(256) void stage_kernel(const TS* __restrict__ src, 
   }
   dst[r * ldd + c] = cvt<TD>(v * scale);
 }
+
+// Channels of a packed jet tensor side by side, point-major (see
+// neb_stack_channels): dst[p][i * width + j] = src[ch[i]][p][j] for the
+// hi plane (and the lo plane when given), eight bf16 per thread.
+__global__ __launch_bounds__(256) void stack_channels_kernel(const __bf16* __restrict__ src_hi,
+                                                             const __bf16* __restrict__ src_lo, int64_t n,
+                                                             int64_t ld, int64_t width, ChanMap map,
+                                                             __bf16* __restrict__ dst_hi, __bf16* __restrict__ dst_lo,
+                                                             int64_t ldd) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const int64_t per = width / 8;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * map.n * per) return;
+  const int64_t j = (idx % per) * 8, q = idx / per;
+  const int i = (int)(q % map.n);
+  const int64_t p = q / map.n;
+  int c = map.ch[0];
+#pragma unroll
+  for (int t = 1; t < STACK_MAXCH; ++t)
+    if (i == t) c = map.ch[t];
+  const int64_t s = ((int64_t)c * n + p) * ld + j, d = p * ldd + (int64_t)i * width + j;
+  *reinterpret_cast<u32x4*>(dst_hi + d) = *reinterpret_cast<const u32x4*>(src_hi + s);
+  if (src_lo) *reinterpret_cast<u32x4*>(dst_lo + d) = *reinterpret_cast<const u32x4*>(src_lo + s);
+}

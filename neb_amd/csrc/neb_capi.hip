@@ -1227,4 +1227,86 @@ int neb_stage(int src_dtype, int dst_dtype, const void* src, int64_t rows, int64
   return fail(NEB_ERR_ARG, "neb_stage: unknown src dtype %d", src_dtype);
 }
 
+int neb_stack_channels(const void* src_hi, const void* src_lo, int64_t n, int64_t ld, int64_t width, int nch,
+                       const int* ch, void* dst_hi, void* dst_lo, int64_t ldd, void* stream) {
+  if (n < 0 || width < 8 || width % 8 || ld < width || nch < 1 || nch > STACK_MAXCH || ldd < (int64_t)nch * width)
+    return fail(NEB_ERR_SHAPE, "neb_stack_channels: bad sizes (width a multiple of 8, 1 <= nch <= %d)", STACK_MAXCH);
+  if (n == 0) return NEB_OK;
+  if (!src_hi || !dst_hi || !ch || (!src_lo) != (!dst_lo)) return fail(NEB_ERR_ARG, "neb_stack_channels: null pointer");
+  if (!aligned(src_hi, ld, NEB_BF16) || !aligned(dst_hi, ldd, NEB_BF16) || (src_lo && !aligned(src_lo, ld, NEB_BF16)) ||
+      (dst_lo && !aligned(dst_lo, ldd, NEB_BF16)))
+    return fail(NEB_ERR_ARG, "neb_stack_channels: rows must be 16-byte aligned");
+  ChanMap map;
+  memset(&map, 0, sizeof(map));
+  map.n = nch;
+  for (int i = 0; i < nch; ++i) {
+    if (ch[i] < 0 || ch[i] >= RG_MAXCH) return fail(NEB_ERR_ARG, "neb_stack_channels: channel %d out of range", ch[i]);
+    map.ch[i] = ch[i];
+  }
+  const int64_t units = n * nch * (width / 8);
+  if (units == 0) return NEB_OK;
+  if (units / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_stack_channels: too many elements");
+  hipLaunchKernelGGL(stack_channels_kernel, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, S(stream),
+                     reinterpret_cast<const __bf16*>(src_hi), reinterpret_cast<const __bf16*>(src_lo), n, ld, width,
+                     map, reinterpret_cast<__bf16*>(dst_hi), reinterpret_cast<__bf16*>(dst_lo), ldd);
+  return hip_status("neb_stack_channels");
+}
+
+int neb_jet_act_reverse(int prec, int dim, int act, int64_t n, int64_t width, const float* adj, int64_t lda, int nch,
+                        const int* ch, const void* z, int64_t ldz, double omega, void* out, int64_t ldo,
+                        void* stream) {
+  if (int rc = check_common(prec, dim, act)) return rc;
+  if (prec == NEB_F64) return fail(NEB_ERR_ARG, "neb_jet_act_reverse: tapes are BF16 or F32");
+  const int C = dim == 2 ? 6 : 10;
+  if (n < 0 || width < 1 || width % 4 || nch < 1 || nch > STACK_MAXCH || lda < (int64_t)nch * width || ldz < width ||
+      ldo < width)
+    return fail(NEB_ERR_SHAPE, "neb_jet_act_reverse: bad sizes (width a multiple of 4, 1 <= nch <= %d)", STACK_MAXCH);
+  if (n == 0) return NEB_OK;
+  if (!adj || !ch || !z || !out) return fail(NEB_ERR_ARG, "neb_jet_act_reverse: null pointer");
+  if (!aligned(adj, lda, NEB_F32) || !aligned(z, ldz, prec) || !aligned(out, ldo, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_act_reverse: rows must be 16-byte aligned");
+  ChanMap map;
+  memset(&map, 0, sizeof(map));
+  map.n = nch;
+  for (int i = 0; i < nch; ++i) {
+    if (ch[i] < 0 || ch[i] >= C) return fail(NEB_ERR_ARG, "neb_jet_act_reverse: channel %d out of range", ch[i]);
+    map.ch[i] = ch[i];
+  }
+  const int64_t units = n * ((width + 7) / 8);
+  if (units == 0) return NEB_OK;
+  if (units / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_jet_act_reverse: too many elements");
+  const dim3 grid((unsigned)((units + 255) / 256));
+  hipStream_t st = S(stream);
+#define ARS(D, A, T)                                                                                              \
+  hipLaunchKernelGGL((act_reverse_stacked_kernel<D, A, T>), grid, dim3(256), 0, st, adj, lda, map,               \
+                     reinterpret_cast<const T*>(z), ldz, n, width, omega, reinterpret_cast<T*>(out), ldo);      \
+  return hip_status("neb_jet_act_reverse");
+  if (prec == NEB_BF16) {
+    if (dim == 2) { if (act == NEB_TANH) { ARS(2, ACT_TANH, __bf16) } else { ARS(2, ACT_SIN, __bf16) } }
+    if (act == NEB_TANH) { ARS(3, ACT_TANH, __bf16) } else { ARS(3, ACT_SIN, __bf16) }
+  }
+  if (dim == 2) { if (act == NEB_TANH) { ARS(2, ACT_TANH, float) } else { ARS(2, ACT_SIN, float) } }
+  if (act == NEB_TANH) { ARS(3, ACT_TANH, float) } else { ARS(3, ACT_SIN, float) }
+#undef ARS
+}
+
+int neb_gemm_split(int64_t n, int64_t k, int64_t nout, const void* x_hi, const void* x_lo, int64_t ldx,
+                   const void* w_hi, const void* w_lo, int64_t ldw, float* out, int64_t ldo, void* stream) {
+  if (n < 0 || k < 1 || nout < 1 || ldx < k || ldw < k || ldo < nout)
+    return fail(NEB_ERR_SHAPE, "neb_gemm_split: bad sizes");
+  if (n == 0) return NEB_OK;
+  if (!x_hi || !x_lo || !w_hi || !w_lo || !out) return fail(NEB_ERR_ARG, "neb_gemm_split: null pointer");
+  if (!aligned(x_hi, ldx, NEB_BF16) || !aligned(x_lo, ldx, NEB_BF16) || !aligned(w_hi, ldw, NEB_BF16) ||
+      !aligned(w_lo, ldw, NEB_BF16) || !aligned(out, ldo, NEB_F32))
+    return fail(NEB_ERR_ARG, "neb_gemm_split: rows must be 16-byte aligned");
+  RowArgs a = row_args();
+  a.x[0] = x_hi; a.ldx[0] = ldx;
+  a.x[1] = x_lo; a.ldx[1] = ldx;
+  a.w[0] = w_hi; a.ldw[0] = ldw;
+  a.w[1] = w_lo; a.ldw[1] = ldw;
+  a.npts = n; a.nout = nout; a.k = k;
+  a.out0 = out; a.ldo = ldo;
+  return launch_rowgemm<PREC_BF16, __bf16, PlanGemmSplit>(a, S(stream), "neb_gemm_split");
+}
+
 }  // extern "C"

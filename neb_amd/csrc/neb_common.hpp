@@ -273,3 +273,11 @@ __device__ inline void ring_barrier() {
   __builtin_amdgcn_s_barrier();
   asm volatile("" ::: "memory");
 }
+
+// A list of packed jet channels (the stacked-channel kernels: stack_channels,
+// act_reverse_stacked), passed by value.
+constexpr int STACK_MAXCH = 16;
+struct ChanMap {
+  int n;
+  int ch[STACK_MAXCH];
+};

@@ -168,6 +168,11 @@ __global__ __launch_bounds__(RowAsyncPol<Plan>::NTH) __attribute__((amdgpu_waves
       cdst[t] = NW * WT + c * XT + 16 * rb * LDK;
     }
   }
+  // Copies past the reduction length re-read the row's last 16-byte chunk
+  // inside it (k rounded up to 8 <= every row stride, the weight slices of a
+  // lift included, whose stride spans several slices); those LDS columns
+  // are zeroed before use.
+  const int64_t kcap = (a.k + 7) / 8 * 8;
   auto issue = [&](int64_t tp, int64_t tf, int kt, lds_S* slot) __attribute__((always_inline)) {
     const int64_t p0 = tp * TP, f0 = tf * TN, k0 = (int64_t)kt * RK;
 #pragma unroll
@@ -176,7 +181,7 @@ __global__ __launch_bounds__(RowAsyncPol<Plan>::NTH) __attribute__((amdgpu_waves
       int64_t row = (cw[t] ? f0 : p0) + crow[t] + cr;
       row = row < nrows ? row : nrows - 1;
       int64_t kk = k0 + 8 * cch;
-      kk = kk + 8 <= cld[t] ? kk : cld[t] - 8;
+      kk = kk + 8 <= kcap ? kk : kcap - 8;
       const S* src = csrc[t] + row * cld[t] + kk;
       lds_S* dst = slot + cdst[t];
       __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);

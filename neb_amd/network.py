@@ -66,6 +66,13 @@ class NetConfig:
     # of the velocity with the pair metric (1, 2, 1) of the causal velocity
     # peel, HB2P:2935-3060, :3022); 3D always uses grad u (nine rows)
     stiffness: str = "vorticity"
+    # output-layer reverse of split-precision tapes: "factored" (the K x K
+    # contraction in FP64 on the master weights, then width-long split GEMMs
+    # on the tape's hi / lo planes; reads no fields), "fields" (the fused
+    # lift transpose over the bf16 hi-plane fields), or "auto": factored in
+    # 3D, whose ill-conditioned mass cancels the bf16 field form's terms,
+    # fields in 2D, where the field form is faster
+    split_reverse: str = "auto"
 
     @property
     def features(self) -> int:
@@ -717,6 +724,173 @@ def _rows_view(t: torch.Tensor, start_row: int, step: int) -> torch.Tensor:
 # ---------------------------------------------------------------------------
 # reverse pass
 # ---------------------------------------------------------------------------
+def lift_terms(cfg: NetConfig) -> tuple[list, list]:
+    """The lift as signed terms (field row, jet channel, output-weight slice,
+    sign) of the mass and the stiffness fields: field row a of a point is
+    sum_t sign_t Y[channel_t] W_{slice_t}^T over the terms t with row a, Y
+    the packed last hidden jets and W_s the output-weight rows 3k+s (3D) or
+    all rows (2D).  2D: u = psi_y, v = -psi_x; omega = -(psi_xx + psi_yy)
+    (ELC:35-40) or the Hessian rows (psi_xx, sqrt2 psi_xy, psi_yy)
+    (HB2P:3022); 3D: u = curl A and grad u (VJ2:93-106)."""
+    if cfg.dim == 3:
+        from .lifts import curl_terms
+        t = curl_terms()
+        return ([(a, 1 + c, s, sg) for (a, c, s, sg) in t if a < 3],
+                [(a - 3, 1 + c, s, sg) for (a, c, s, sg) in t if a >= 3])
+    mass = [(0, 2, 0, 1.0), (1, 1, 0, -1.0)]
+    if cfg.stiffness == "hessian":
+        return mass, [(0, 3, 0, 1.0), (1, 4, 0, math.sqrt(2.0)), (2, 5, 0, 1.0)]
+    return mass, [(0, 3, 0, -1.0), (0, 5, 0, -1.0)]
+
+
+class FactoredProducts:
+    """The last hidden jets of a split-precision tape stacked side by side
+    per point (hi / lo bf16 planes [n, nch * width]: the mass channels, then
+    the stiffness channels) and, per field group, the lift-weighted channel
+    Grams Ghat[s' width + x, s width + y] = sum S[c', c, s', s] Y_c'x^T Y_cy
+    over the points (FP64; _factored_ghat); `event` marks their completion on
+    the stream that queued them (output_products)."""
+
+    def __init__(self, hi, lo, chans, Gh, event=None):
+        self.hi, self.lo, self.chans, self.Gh, self.event = hi, lo, chans, Gh, event
+
+
+def factored_reverse_ok(net: EigenfluidNet, tape: Tape) -> bool:
+    """Whether the reverse of this tape runs factored (tape-only, field-free):
+    split-precision tapes with a hidden width that is a multiple of 8, per
+    NetConfig.split_reverse."""
+    mode = net.cfg.split_reverse
+    if mode not in ("auto", "factored", "fields"):
+        raise ValueError(f"unknown split_reverse {mode!r}")
+    if tape.inputs_lo is None or net.cfg.widths[-1] % 8:
+        return False
+    return net.cfg.dim == 3 if mode == "auto" else mode == "factored"
+
+
+def factored_channels(cfg: NetConfig) -> tuple[list, list]:
+    """The last hidden jet channels the mass and the stiffness fields read,
+    in stacking order."""
+    mterms, sterms = lift_terms(cfg)
+    return sorted({t[1] for t in mterms}), sorted({t[1] for t in sterms})
+
+
+def factored_products(net: EigenfluidNet, tape: Tape) -> FactoredProducts:
+    """Stack the last hidden jet channels the lift reads (neb_stack_channels)
+    and form their split-precision channel Grams (the products of the
+    output-weight gradient, unweighted)."""
+    cfg = net.cfg
+    width, n = cfg.widths[-1], tape.n
+    mch, sch = factored_channels(cfg)
+    chans = mch + sch
+    Y, Ylo = tape.inputs[-1], tape.inputs_lo[-1]
+    ld = len(chans) * width
+    hi = torch.empty((n, ld), dtype=torch.bfloat16, device=net.device)
+    lo = torch.empty_like(hi)
+    H.call("neb_stack_channels", Y.data_ptr(), Ylo.data_ptr(), n, Y.shape[2], width, len(chans), H.int_array(chans),
+           hi.data_ptr(), lo.data_ptr(), ld, _stream())
+    km = len(mch) * width
+    G = [gram(hi[:, :km], km, 1.0, "bf16x3", lo=lo[:, :km]), gram(hi[:, km:], ld - km, 1.0, "bf16x3", lo=lo[:, km:])]
+    return FactoredProducts(hi, lo, (mch, sch), _factored_ghat(cfg, G))
+
+
+_FACT_S: dict = {}
+
+
+def _factored_coeffs(cfg: NetConfig, device) -> list:
+    """Per field group (mass, stiffness): S[i', i, s', s] = sum of sg_t sg_t'
+    over the lift's term pairs (t, t') of one field row with channels
+    (chans[i'], chans[i]) and output-weight slices (s', s)  (FP64, cached)."""
+    key = (cfg.dim, cfg.stiffness, str(device))
+    if key not in _FACT_S:
+        ns = 3 if cfg.dim == 3 else 1
+        out = []
+        for terms, chans in zip(lift_terms(cfg), factored_channels(cfg)):
+            idx = {c: i for i, c in enumerate(chans)}
+            S = np.zeros((len(chans), len(chans), ns, ns))
+            for (a, c, sl, sg) in terms:
+                for (a2, c2, sl2, sg2) in terms:
+                    if a2 == a:
+                        S[idx[c2], idx[c], sl2, sl] += sg * sg2
+            out.append(torch.tensor(S.reshape(len(chans) ** 2, ns * ns), dtype=torch.float64, device=device))
+        _FACT_S[key] = out
+    return _FACT_S[key]
+
+
+def _factored_ghat(cfg: NetConfig, G: list) -> list:
+    """Ghat_s's = sum_c'c S[c', c, s', s] G[c', c] per field group from the
+    channel Grams G ([nch width]^2, FP64), as [ns width, ns width]."""
+    wd = cfg.widths[-1]
+    ns = 3 if cfg.dim == 3 else 1
+    out = []
+    for S2, Gg in zip(_factored_coeffs(cfg, G[0].device), G):
+        nch = Gg.shape[0] // wd
+        Gst = Gg.view(nch, wd, nch, wd).permute(0, 2, 1, 3).reshape(nch * nch, wd * wd)
+        out.append(torch.mm(S2.T, Gst).view(ns, ns, wd, wd).permute(0, 2, 1, 3).reshape(ns * wd, ns * wd))
+    return out
+
+
+def _factored_weights(net: EigenfluidNet, pair: torch.Tensor, w: float) -> list:
+    """The weight operands of the factored output-layer reverse, per field
+    group X (mass with dL/dM, stiffness with dL/dS), in FP64 from the master
+    output weights W_s (rows 3k+s in 3D) and the lift coefficients S
+    (_factored_coeffs):
+      Q_s's = W_s'^T dX^T W_s                       (one [ns width]^2 product)
+      R[c', c] = 2w sum_s's S[c', c, s', s] Q_s's   [width, width] blocks
+    returned transposed (R^T: the [features][k] operand of adj Y = Y_stack R)."""
+    cfg = net.cfg
+    K, wd = cfg.modes, cfg.widths[-1]
+    ns = 3 if cfg.dim == 3 else 1
+    Wcat = net.W[-1].to(torch.float64).reshape(K, ns * wd)        # Wcat[k, s wd + j] = W[ns k + s, j]
+    RTs = []
+    for S2, dX in zip(_factored_coeffs(cfg, Wcat.device), (pair[:, :K], pair[:, K:])):
+        nch = int(round(math.sqrt(S2.shape[0])))
+        Q = torch.mm(Wcat.T, torch.mm(dX.T, Wcat))                    # [s' wd + x, s wd + y]
+        Qst = Q.view(ns, wd, ns, wd).permute(0, 2, 1, 3).reshape(ns * ns, wd * wd)
+        Rq = torch.mm(S2, Qst).view(nch, nch, wd, wd)                 # [c', c, x, y]
+        RTs.append((Rq.permute(1, 3, 0, 2).reshape(nch * wd, nch * wd) * (2.0 * w)).contiguous())
+    return RTs
+
+
+def _factored_output_grad(net: EigenfluidNet, fp: FactoredProducts, pair: torch.Tensor, w: float) -> torch.Tensor:
+    """Output-weight gradient gW_s = 2w sum_X sum_s' dX W_s' Ghat^X_s's
+    ([ncomp K, width], rows ncomp k + s)."""
+    cfg = net.cfg
+    K, wd = cfg.modes, cfg.widths[-1]
+    ns = 3 if cfg.dim == 3 else 1
+    Wcat = net.W[-1].to(torch.float64).reshape(K, ns * wd)
+    gcat = torch.zeros((K, ns * wd), dtype=torch.float64, device=Wcat.device)
+    for Gh, dX in zip(fp.Gh, (pair[:, :K], pair[:, K:])):
+        gcat.addmm_(torch.mm(dX, Wcat), Gh, alpha=2.0 * w)
+    return gcat.view(K, ns, wd).reshape(ns * K, wd)
+
+
+def _factored_adjoint(net: EigenfluidNet, tape: Tape, fp: FactoredProducts, RTs: list) -> torch.Tensor:
+    """adj Z_L = omega * act-jet^T(Z_L, adj Y): adj Y of each field group is
+    the split GEMM Y_stack[:, group] R (neb_gemm_split with R^T staged as
+    hi / lo planes), then the activation transpose (neb_jet_act_reverse)."""
+    cfg = net.cfg
+    width, n = cfg.widths[-1], tape.n
+    C = channels(cfg.dim)
+    s = _stream()
+    ld = fp.hi.shape[1]
+    AY = torch.empty((n, ld), dtype=torch.float32, device=net.device)
+    c0 = 0
+    for RT in RTs:
+        kg = RT.shape[0]
+        rhi = torch.empty((kg, _ld(kg)), dtype=torch.bfloat16, device=net.device)
+        rlo = torch.empty_like(rhi)
+        H.call("neb_split_bf16", H.F64, RT.data_ptr(), kg, kg, kg, rhi.data_ptr(), rlo.data_ptr(), rhi.shape[1], s)
+        H.call("neb_gemm_split", n, kg, kg, fp.hi[:, c0:].data_ptr(), fp.lo[:, c0:].data_ptr(), ld, rhi.data_ptr(),
+               rlo.data_ptr(), rhi.shape[1], AY[:, c0:].data_ptr(), ld, s)
+        c0 += kg
+    chans = fp.chans[0] + fp.chans[1]
+    Z = tape.pre[-1]
+    A = torch.empty((C, n, _ld(width)), dtype=_STORE[tape.prec], device=net.device)
+    H.call("neb_jet_act_reverse", tape.prec, cfg.dim, cfg.act_code, n, width, AY.data_ptr(), ld, len(chans),
+           H.int_array(chans), Z.data_ptr(), Z.shape[2], cfg.omega(len(cfg.widths) - 1), A.data_ptr(), A.shape[2], s)
+    return A
+
+
 def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Tensor, dS: torch.Tensor,
                     w: float, grads: torch.Tensor | None = None, symmetric: bool = False,
                     products: "OutputProducts | None" = None) -> torch.Tensor:
@@ -730,7 +904,13 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     L(M, S), and for the coefficient peel's pair [Cu^T/2 | Cd^T/2] its
     detached-prefix cotangents w U Cu, w D Cd; `products` are the
     output-layer products queued earlier by output_products (computed here
-    when None).  Returns the flat gradient in the master dtype."""
+    when None).  Split-precision tapes with NetConfig.split_reverse
+    "factored" (the 3D default) take the factored output-layer reverse
+    (factored_products, _factored_weights, _factored_adjoint,
+    _factored_output_grad: the K x K contraction in FP64, then width-long
+    split GEMMs on the tape's hi / lo planes); it reads no fields, and
+    `fields` may then be None.
+    Returns the flat gradient in the master dtype."""
     cfg = net.cfg
     prec = tape.prec
     stg = net.staged(prec)
@@ -748,25 +928,10 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
         else:
             sym_shift(dM, out=pair[:, :K])
             sym_shift(dS, out=pair[:, K:])
-    dM, dS = pair[:, :K], pair[:, K:]
-    Wout = net.W[-1].to(torch.float64)               # [ncomp*K, width]
-    ldK = fields.mass.shape[1]
-    if d == 2:
-        Bp = torch.mm(Wout.T, pair)                  # [width, 2K] = [W^T dL/dM | W^T dL/dS]
-    else:
-        Bp = torch.cat([torch.mm(Wout[c::3].T, pair) for c in range(3)], 0)  # [3*width, 2K]
-    # 2w B in the operand precision, zero-padded to the row stride (one launch each)
-    bm = torch.empty((Bp.shape[0], _ld(K)), dtype=st, device=net.device)
-    bs = torch.empty_like(bm)
-    for half, dst in ((0, bm), (1, bs)):
-        H.call("neb_stage", H.F64, prec, Bp.data_ptr() + 8 * K * half, Bp.shape[0], K, Bp.stride(0), dst.data_ptr(),
-               dst.shape[1], dst.shape[0], dst.shape[1], 0, 2.0 * w, s)
     if grads is None:
         grads = torch.empty_like(net.params)
     gW, gb = net.split(grads)
     mode = _CROSS[prec]
-    if fields.layout != 0:
-        raise ValueError("backward_fields expects point-major field matrices")
     # The weight gradients (cross products over points) only read the fields,
     # the tape and each layer's adjoint once it exists: they run on the side
     # stream, overlapping the jet reverse on the current stream, which waits
@@ -776,20 +941,51 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     side = side_stream(net.device) if on_gpu else None
     if on_gpu:
         side.wait_stream(main)
-    A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
-    Z = tape.pre[-1]
-    H.call("neb_jet_output_backward", prec, cfg.lift, cfg.act_code, n, width, K, fields.mass.data_ptr(),
-           fields.stiff.data_ptr(), ldK, _DT[fields.mass.dtype], fields.layout, bm.data_ptr(), bs.data_ptr(),
-           bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
-    ctx = torch.cuda.stream(side) if on_gpu else nullcontext()
-    with ctx:
-        if products is None:
-            P = _products(net, tape, fields)
+    if factored_reverse_ok(net, tape):
+        fp = products if isinstance(products, FactoredProducts) else None
+        if fp is None:
+            fp = factored_products(net, tape)
+        elif on_gpu and fp.event is not None:
+            main.wait_event(fp.event)
+            for t in (fp.hi, fp.lo, *fp.Gh):
+                t.record_stream(main)
+        RTs = _factored_weights(net, pair, w)
+        A = _factored_adjoint(net, tape, fp, RTs)
+        if on_gpu:
+            side.wait_stream(main)
+            for t in (pair, *fp.Gh):     # read on the side stream after this frame lets them go
+                t.record_stream(side)
+        with (torch.cuda.stream(side) if on_gpu else nullcontext()):
+            gW[-1].copy_(_factored_output_grad(net, fp, pair, w))
+            gb[-1].zero_()
+        del fp, RTs
+    else:
+        if fields is None or fields.layout != 0:
+            raise ValueError("backward_fields expects point-major field matrices")
+        Wout = net.W[-1].to(torch.float64)               # [ncomp*K, width]
+        if d == 2:
+            Bp = torch.mm(Wout.T, pair)                  # [width, 2K] = [W^T dL/dM | W^T dL/dS]
         else:
-            if on_gpu and products.event is not None:
-                side.wait_event(products.event)
-            P = products.P
-        _output_layer_grads(d, K, P, pair, w, gW, gb)
+            Bp = torch.cat([torch.mm(Wout[c::3].T, pair) for c in range(3)], 0)  # [3*width, 2K]
+        # 2w B in the operand precision, zero-padded to the row stride (one launch each)
+        bm = torch.empty((Bp.shape[0], _ld(K)), dtype=st, device=net.device)
+        bs = torch.empty_like(bm)
+        for half, dst in ((0, bm), (1, bs)):
+            H.call("neb_stage", H.F64, prec, Bp.data_ptr() + 8 * K * half, Bp.shape[0], K, Bp.stride(0),
+                   dst.data_ptr(), dst.shape[1], dst.shape[0], dst.shape[1], 0, 2.0 * w, s)
+        A = torch.empty((C, n, _ld(width)), dtype=st, device=net.device)
+        Z = tape.pre[-1]
+        H.call("neb_jet_output_backward", prec, cfg.lift, cfg.act_code, n, width, K, fields.mass.data_ptr(),
+               fields.stiff.data_ptr(), fields.mass.shape[1], _DT[fields.mass.dtype], fields.layout, bm.data_ptr(),
+               bs.data_ptr(), bm.shape[1], Z.data_ptr(), Z.shape[2], cfg.omega(L - 1), A.data_ptr(), A.shape[2], s)
+        with (torch.cuda.stream(side) if on_gpu else nullcontext()):
+            if products is None:
+                P = _products(net, tape, fields)
+            else:
+                if on_gpu and products.event is not None:
+                    side.wait_event(products.event)
+                P = products.P
+            _output_layer_grads(d, K, P, pair, w, gW, gb)
 
     for l in range(L - 1, -1, -1):
         X = tape.inputs[l]
@@ -975,16 +1171,31 @@ def _device_cus(dev) -> int:
     return _CUS[key]
 
 
-def output_products(net: EigenfluidNet, tape: Tape, fields: Fields) -> OutputProducts:
-    """Queue the output-layer products [P_m; P_s] on the side stream.  They
-    depend on the fields and the last hidden jets only, not on the objective,
-    so they run while the FP64 objective (a chain of small, latency-bound
-    launches) holds the current stream; pass the result to backward_fields."""
-    dev = fields.mass.device
+def output_products(net: EigenfluidNet, tape: Tape, fields: Fields | None):
+    """Queue the output-layer products on the side stream: [P_m; P_s]
+    (OutputProducts), or for a split-precision tape the stacked last hidden
+    jets and their channel Grams (FactoredProducts).  They depend on the
+    fields and the last hidden jets only, not on the objective, so they run
+    while the FP64 objective (a chain of small, latency-bound launches) holds
+    the current stream; pass the result to backward_fields."""
+    dev = net.device
+    fact = factored_reverse_ok(net, tape)
     if dev.type != "cuda":
-        return OutputProducts(_products(net, tape, fields), None)
+        return factored_products(net, tape) if fact else OutputProducts(_products(net, tape, fields), None)
     main, side = torch.cuda.current_stream(dev), side_stream(dev)
     side.wait_stream(main)
+    if fact:
+        H.call("neb_set_cross_occupancy", max(1, _device_cus(dev) // 2))
+        try:
+            with torch.cuda.stream(side):
+                fp = factored_products(net, tape)
+                fp.event = torch.cuda.Event()
+                fp.event.record(side)
+        finally:
+            H.call("neb_set_cross_occupancy", 0)
+        for t in (tape.inputs[-1], tape.inputs_lo[-1]):
+            t.record_stream(side)
+        return fp
     # the products' launches keep to half the CUs' worth of workgroups: the
     # objective's sweep launches (one workgroup per 64 x 64 tile, 136 at
     # K = 1024) then find whole CUs free.  With one per CU the dispatcher

@@ -498,7 +498,8 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
         with timed(tt, "pass2_recompute_and_reverse"):
             for c0 in range(0, n, chunk):
                 tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
-                f = nw.output_fields(net, tape, split=split)
+                # the factored reverse of a split-precision tape reads no fields
+                f = None if nw.factored_reverse_ok(net, tape) else nw.output_fields(net, tape, split=split)
                 grads += nw.backward_fields(net, tape, f, dM, dS, weight, symmetric=True)
                 del tape, f
         out["grads"] = grads
