@@ -404,6 +404,25 @@ def test_adam_first_step_closed_form():
     assert torch.equal(net.params, after) and net.epoch == 1  # the device gate skipped both updates
 
 
+def test_adam_three_steps_match_oracle():
+    """Three gated Adam steps against the oracle's (NCC:156-171): the bias
+    corrections 1 - b^t for t = 2, 3 and the global-norm clip (active on the
+    second step) -- a kernel using b instead of b^t fails here."""
+    cfg, net, _ = make(2, 8, [16], 2, master=torch.float64)
+    rng = np.random.default_rng(84)
+    p = net.params.detach().cpu().numpy().copy()
+    m, v = np.zeros_like(p), np.zeros_like(p)
+    for t, (scale, clip) in enumerate(((1.0, 1e6), (10.0, 2.0), (0.3, 8.0)), start=1):
+        g = rng.normal(size=p.size) * scale
+        norm = net.adam(torch.tensor(g, device=DEV), 1e-3, clip=clip)
+        (p, m, v), = o.adam([p], [g], [m], [v], t, 1e-3, clip=clip)[0]
+        assert abs(norm - float(np.linalg.norm(g))) < 1e-9 * norm
+        assert np.allclose(net.params.cpu().numpy(), p, rtol=1e-12, atol=1e-14), t
+        assert np.allclose(net.m.cpu().numpy(), m, rtol=1e-12, atol=1e-16), t
+        assert np.allclose(net.v.cpu().numpy(), v, rtol=1e-12, atol=1e-18), t
+    assert net.epoch == 3
+
+
 def test_adam_device_gate_skips_update():
     """A failed step gate handed to adam leaves parameters, moments and the
     epoch untouched and raises through on_gate; a passing gate updates
