@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 8
+#define NEB_ABI_VERSION 9
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -195,6 +195,37 @@ int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width
 int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* adj_in,
                            int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev, int64_t ldz,
                            double omega_prev, void* adj_out, int64_t lda_out, void* stream);
+
+/* Block-diagonal hidden layers (the per-mode networks of HB2P:1590-1880,
+ * NCC:111-185: a block of B modes of width h is one layer of width B*h whose
+ * weights are zero outside the h x h diagonal blocks).  The same jets,
+ * epilogues and precisions as neb_jet_layer_forward / _x3 /
+ * neb_jet_layer_backward on a square layer (k = width), except that each
+ * tile of output features contracts only the k of the blocks it belongs to
+ * (block: h, a multiple of 8 dividing the width); the zeros outside the
+ * blocks are not read.  Replaces the per-mode batched matmuls of
+ * HB2P:1005-1036 (forward) and HB2P:1115-1144 (input adjoint).
+ * neb_cross_blocks: neb_cross restricted to an m x m output's diagonal
+ * blocks of `block` (block nesting with the 128-wide tiles: a divisor or a
+ * multiple of 128, dividing m): the tiles over those blocks are computed and
+ * stored, every other entry of G is left as it is; the per-mode weight
+ * gradients (HB2P:1087-1144 per mode).  Workspace:
+ * neb_cross_blocks_workspace_size. */
+int neb_jet_layer_forward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block, const void* x,
+                                 int64_t ldx, const void* w, int64_t ldw, const void* bias, double omega, void* z,
+                                 void* y, int64_t ldy, void* stream);
+int neb_jet_layer_forward_x3_blocks(int dim, int act, int64_t n, int64_t width, int64_t block, const void* x_hi,
+                                    const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
+                                    const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,
+                                    int64_t ldy, void* stream);
+int neb_jet_layer_backward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block,
+                                  const void* adj_in, int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev,
+                                  int64_t ldz, double omega_prev, void* adj_out, int64_t lda_out, void* stream);
+size_t neb_cross_blocks_workspace_size(int64_t m, int64_t block, int64_t rows);
+int neb_cross_blocks(int mode, int nseg, const void* const* x, const void* const* y, const int64_t* ldx,
+                     const int64_t* ldy, const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m,
+                     int64_t block, double scale, double beta, double* G, int64_t ldg, void* workspace,
+                     size_t ws_bytes, void* stream);
 
 /* Factored output-layer reverse (split-precision tapes).  The fields are
  * linear in the output weights, Phi_a = sum_t sg_t Y_{c_t} W_{s_t}^T over the

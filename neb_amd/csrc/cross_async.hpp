@@ -96,15 +96,7 @@ cross_async_kernel(CrossArgs a) {
     tile = u % a.ntiles;
   }
   int ti, tj;
-  if (a.symmetric) {
-    int t = tile, rowlen = a.tiles_n;
-    ti = 0;
-    while (t >= rowlen) { t -= rowlen; ++ti; --rowlen; }
-    tj = ti + t;
-  } else {
-    ti = tile / a.tiles_n;
-    tj = tile % a.tiles_n;
-  }
+  cross_tile_ij(a, tile, ti, tj);
   const int64_t i0 = (int64_t)ti * CR_TM, j0 = (int64_t)tj * CR_TN;
   // wave index made provably uniform, so the segment descriptors behind the
   // per-wave copy pieces are read with scalar loads (a vector load would be

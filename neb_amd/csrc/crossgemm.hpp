@@ -48,9 +48,28 @@ struct CrossArgs {
   int64_t m, n;
   const double* roww;  // optional per-row weights (single segment only)
   int symmetric;       // upper tiles only (X == Y)
+  int bdiag;           // > 0: block-diagonal tiles only, bdiag x bdiag tiles per block (m == n)
   int tiles_m, tiles_n, ntiles, splits;
   double* partial;     // [splits][ntiles][TM][TN]
 };
+
+// Output tile of work item `tile`: every tile, the upper triangle
+// (symmetric), or the tiles of the diagonal blocks (bdiag tiles a side).
+__host__ __device__ inline void cross_tile_ij(const CrossArgs& a, int tile, int& ti, int& tj) {
+  if (a.symmetric) {
+    int t = tile, rowlen = a.tiles_n;
+    ti = 0;
+    while (t >= rowlen) { t -= rowlen; ++ti; --rowlen; }
+    tj = ti + t;
+  } else if (a.bdiag > 0) {
+    const int per = a.bdiag * a.bdiag, blk = tile / per, r = tile % per;
+    ti = blk * a.bdiag + r / a.bdiag;
+    tj = blk * a.bdiag + r % a.bdiag;
+  } else {
+    ti = tile / a.tiles_n;
+    tj = tile % a.tiles_n;
+  }
+}
 
 template <int MODE> struct CrossPol;
 template <> struct CrossPol<CR_F32> {
@@ -289,15 +308,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void c
     tile = u % a.ntiles;
   }
   int ti, tj;
-  if (a.symmetric) {
-    int t = tile, rowlen = a.tiles_n;
-    ti = 0;
-    while (t >= rowlen) { t -= rowlen; ++ti; --rowlen; }
-    tj = ti + t;
-  } else {
-    ti = tile / a.tiles_n;
-    tj = tile % a.tiles_n;
-  }
+  cross_tile_ij(a, tile, ti, tj);
   const int64_t i0 = (int64_t)ti * CR_TM, j0 = (int64_t)tj * CR_TN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wi = (wave >> 1) * 64, wj = (wave & 1) * 64;
@@ -438,8 +449,8 @@ write_out:
 // reads and writes are row-contiguous.
 template <typename TP>
 __global__ __launch_bounds__(256) void cross_reduce_kernel(const TP* __restrict__ partial, int splits,
-                                                            int ntiles, int tiles_n, int symmetric, int64_t m,
-                                                            int64_t n, double scale, double beta,
+                                                            int ntiles, int tiles_n, int symmetric, int bdiag,
+                                                            int64_t m, int64_t n, double scale, double beta,
                                                             double* __restrict__ G, int64_t ldg) {
   __shared__ double tr[32][33];
   const int64_t nbj = (n + 31) / 32;
@@ -453,6 +464,9 @@ __global__ __launch_bounds__(256) void cross_reduce_kernel(const TP* __restrict_
     bi = blockIdx.x / nbj;
     bj = blockIdx.x % nbj;
   }
+  // block-diagonal tiles only: 32 x 32 blocks outside them are not computed
+  // (a block lies inside one tile: 32 divides the tile sides)
+  if (bdiag > 0 && (bi * 32 / CR_TM) / bdiag != (bj * 32 / CR_TN) / bdiag) return;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
   const int64_t stride = (int64_t)ntiles * CR_TM * CR_TN;
   // the thread's 4 rows share one loop over the splits, 8 splits at a time:
@@ -466,7 +480,9 @@ __global__ __launch_bounds__(256) void cross_reduce_kernel(const TP* __restrict_
     const int64_t i = bi * 32 + ty + 8 * q, j = bj * 32 + tx;
     ok[q] = i < m && j < n;
     const int64_t ti = ok[q] ? i / CR_TM : 0, tj = ok[q] ? j / CR_TN : 0;
-    const int64_t tile = symmetric ? ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti) : ti * tiles_n + tj;
+    const int64_t tile = symmetric ? ti * tiles_n - ti * (ti - 1) / 2 + (tj - ti)
+                         : bdiag > 0 ? (ti / bdiag) * bdiag * bdiag + (ti % bdiag) * bdiag + (tj % bdiag)
+                                     : ti * tiles_n + tj;
     p[q] = partial + tile * (CR_TM * CR_TN) + (ok[q] ? (i % CR_TM) * CR_TN + (j % CR_TN) : 0);
   }
   double val[4] = {0.0, 0.0, 0.0, 0.0};

@@ -51,6 +51,8 @@ struct RowArgs {
   void* out0lo;             // split fields: bf16 lo planes beside out0 / out1
   void* out1lo;
   const void* env;          // envelope jets, math_t [C][npts] (PlanOutJetsT<D, true>)
+  int64_t kwin;             // > 0: block-diagonal weights with kwin x kwin blocks; a
+                            // feature tile contracts only the k of its blocks
 };
 
 template <typename T> __device__ inline T fneg(T v) { return -v; }
@@ -245,6 +247,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F
     }
   }
   const int64_t p0 = pb * TP, f0 = fbk * TN;
+  // contraction window [kb, ke): all of k, or with block-diagonal weights
+  // (a.kwin) the k of the blocks that features f0 .. f0+TN-1 belong to
+  int64_t kb = 0, ke = a.k;
+  if (a.kwin > 0) {
+    kb = f0 / a.kwin * a.kwin;
+    const int64_t e = ((f0 + TN - 1) / a.kwin + 1) * a.kwin;
+    ke = e < a.k ? e : a.k;
+  }
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wn = wave % WPN, wp = wave / WPN;
@@ -270,17 +280,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F
   // point write adjacent 16-byte chunks (whole 32-byte sectors).
   constexpr int GE = Plan::template out_bytes<P>() == 2 ? 8 : 4;
   const int rg = PR::rowg(fr), rr = PR::rowr(fr);
-  const int nk = (int)((a.k + RK - 1) / RK);
+  const int nk = (int)((ke - kb + RK - 1) / RK);
 
   // the second in-flight stage only where its registers fit beside the
   // accumulators (raw operands no wider than the LDS type)
   constexpr int NVEC = TileLoad<NW, TN, RK, LDK, S, S>::NVT + TileLoad<CI, TP, RK, LDK, TIN, S>::NVT;
   constexpr bool DEEP = sizeof(S) == 2 && sizeof(TIN) == 2 && NA * FB * PB <= 24 && NVEC <= 6;
-  lw[0].fetch(a.w, a.ldw, f0, a.nout, 0, a.k);
-  lx[0].fetch(a.x, a.ldx, p0, a.npts, 0, a.k);
+  lw[0].fetch(a.w, a.ldw, f0, a.nout, kb, ke);
+  lx[0].fetch(a.x, a.ldx, p0, a.npts, kb, ke);
   if (DEEP && nk > 1) {
-    lw[1].fetch(a.w, a.ldw, f0, a.nout, RK, a.k);
-    lx[1].fetch(a.x, a.ldx, p0, a.npts, RK, a.k);
+    lw[1].fetch(a.w, a.ldw, f0, a.nout, kb + RK, ke);
+    lx[1].fetch(a.x, a.ldx, p0, a.npts, kb + RK, ke);
   }
   lw[0].commit(buf0);
   lx[0].commit(buf0 + NW * WT);
@@ -289,18 +299,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F
     const int cur_set = DEEP ? (kt & 1) : 1;
     S* cur = (DB && (kt & 1)) ? buf1 : buf0;
     if (!DEEP && kt + 1 < nk) {  // single in-flight stage: fetch kt+1 into set 0
-      const int64_t k1 = (int64_t)(kt + 1) * RK;
-      lw[0].fetch(a.w, a.ldw, f0, a.nout, k1, a.k);
-      lx[0].fetch(a.x, a.ldx, p0, a.npts, k1, a.k);
+      const int64_t k1 = kb + (int64_t)(kt + 1) * RK;
+      lw[0].fetch(a.w, a.ldw, f0, a.nout, k1, ke);
+      lx[0].fetch(a.x, a.ldx, p0, a.npts, k1, ke);
     }
     if (DEEP && kt + 2 < nk) {
-      const int64_t k2 = (int64_t)(kt + 2) * RK;
+      const int64_t k2 = kb + (int64_t)(kt + 2) * RK;
       if (cur_set == 0) {
-        lw[0].fetch(a.w, a.ldw, f0, a.nout, k2, a.k);
-        lx[0].fetch(a.x, a.ldx, p0, a.npts, k2, a.k);
+        lw[0].fetch(a.w, a.ldw, f0, a.nout, k2, ke);
+        lx[0].fetch(a.x, a.ldx, p0, a.npts, k2, ke);
       } else {
-        lw[1].fetch(a.w, a.ldw, f0, a.nout, k2, a.k);
-        lx[1].fetch(a.x, a.ldx, p0, a.npts, k2, a.k);
+        lw[1].fetch(a.w, a.ldw, f0, a.nout, k2, ke);
+        lx[1].fetch(a.x, a.ldx, p0, a.npts, k2, ke);
       }
     }
     const S* ws = cur;

@@ -72,13 +72,21 @@ int launch_rowgemm(const RowArgs& a, hipStream_t st, const char* what) {
     constexpr int TN = RowAsyncPol<Plan>::TN, TP = RowAsyncPol<Plan>::TP;
     const int64_t ntiles = ((a.npts + TP - 1) / TP) * ((a.nout + TN - 1) / TN);
     if (ntiles == 0) return NEB_OK;
+    // the persistent walk needs one contraction length for every tile: a
+    // block-diagonal window is used when the tile width and the block nest
+    // and every window lies whole inside k (else the full k, same result)
+    RowArgs b = a;
+    if (b.kwin > 0) {
+      const int64_t wl = TN >= b.kwin ? TN : b.kwin;
+      if ((TN % b.kwin != 0 && b.kwin % TN != 0) || b.nout % wl != 0 || b.k < b.nout) b.kwin = 0;
+    }
     auto kern = rowgemm_async_kernel<Plan>;
     constexpr int NTH = RowAsyncPol<Plan>::NTH;
     static int slots_cache[16] = {0};
     const int64_t slots = resident_slots(kern, 0, slots_cache, NTH);
     const int64_t per_wg = (ntiles + slots - 1) / slots;
     const int64_t grid = (ntiles + per_wg - 1) / per_wg;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTH), 0, st, a, per_wg);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NTH), 0, st, b, per_wg);
     return hip_status(what);
   }
   const size_t lds = rowgemm_lds_bytes<P, Plan>();
@@ -117,7 +125,7 @@ int check_lift(int prec, int lift, int act) {
 // --------------------------------------------------------------------------
 template <int P, int D, int ACT>
 int layer_forward(int64_t n, int64_t k, int64_t width, const void* x, int64_t ldx, const void* w, int64_t ldw,
-                  const void* bias, double omega, void* z, void* y, int64_t ldy, hipStream_t st) {
+                  const void* bias, double omega, void* z, void* y, int64_t ldy, hipStream_t st, int64_t kwin = 0) {
   constexpr int C = Jet2Layout<D>::C;
   using Sx = typename Prec<P>::store_t;
   RowArgs a = row_args();
@@ -129,6 +137,7 @@ int layer_forward(int64_t n, int64_t k, int64_t width, const void* x, int64_t ld
   a.npts = n; a.nout = width; a.k = k;
   a.bias = bias; a.omega = omega;
   a.out0 = z; a.out1 = y; a.ldo = ldy; a.cstride = n * ldy;
+  a.kwin = kwin;
   return launch_rowgemm<P, Sx, PlanHiddenFwd<D, ACT>>(a, st, "neb_jet_layer_forward");
 }
 
@@ -201,7 +210,7 @@ int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64
 template <int D, int ACT>
 int layer_forward_x3(int64_t n, int64_t k, int64_t width, const void* x_hi, const void* x_lo, int64_t ldx,
                      const void* w_hi, const void* w_lo, int64_t ldw, const void* bias, double omega, void* z_hi,
-                     void* z_lo, void* y_hi, void* y_lo, int64_t ldy, hipStream_t st) {
+                     void* z_lo, void* y_hi, void* y_lo, int64_t ldy, hipStream_t st, int64_t kwin = 0) {
   constexpr int C = Jet2Layout<D>::C;
   static_assert(2 * C <= RG_MAXCH, "split input planes");
   RowArgs a = row_args();
@@ -215,6 +224,7 @@ int layer_forward_x3(int64_t n, int64_t k, int64_t width, const void* x_hi, cons
   a.bias = bias; a.omega = omega;
   a.out0 = z_hi; a.out0lo = z_lo; a.out1 = y_hi; a.out1lo = y_lo;
   a.ldo = ldy; a.cstride = n * ldy;
+  a.kwin = kwin;
   return launch_rowgemm<PREC_BF16, __bf16, PlanHiddenFwdSplit<D, ACT>>(a, st, "neb_jet_layer_forward_x3");
 }
 
@@ -317,7 +327,7 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
 template <int P, int D, int ACT>
 int layer_backward(int64_t n, int64_t k, int64_t width, const void* adj_in, int64_t lda_in, const void* wt,
                    int64_t ldw, const void* z_prev, int64_t ldz, double omega_prev, void* adj_out,
-                   int64_t lda_out, hipStream_t st) {
+                   int64_t lda_out, hipStream_t st, int64_t kwin = 0) {
   constexpr int C = Jet2Layout<D>::C;
   using Sx = typename Prec<P>::store_t;
   RowArgs a = row_args();
@@ -330,6 +340,7 @@ int layer_backward(int64_t n, int64_t k, int64_t width, const void* adj_in, int6
   a.tape = z_prev; a.ldt = ldz; a.tstride = n * ldz;
   a.omega = omega_prev;
   a.out0 = adj_out; a.ldo = lda_out; a.cstride = n * lda_out;
+  a.kwin = kwin;
   return launch_rowgemm<P, Sx, PlanHiddenBwd<D, ACT>>(a, st, "neb_jet_layer_backward");
 }
 
@@ -340,10 +351,21 @@ int cross_tiles(int64_t m, int64_t n, int symmetric, int* tm, int* tn) {
   return symmetric ? (*tm) * (*tm + 1) / 2 : (*tm) * (*tn);
 }
 
+// Tiles of a block-diagonal cross (m x m output, diagonal blocks of `block`):
+// bdiag x bdiag tiles per block side (1 when a block fits inside a tile).
+int cross_block_tiles(int64_t m, int64_t block, int* tm, int* bdiag) {
+  *tm = (int)((m + CR_TM - 1) / CR_TM);
+  *bdiag = block >= CR_TM ? (int)(block / CR_TM) : 1;
+  return (*tm / *bdiag) * (*bdiag) * (*bdiag);
+}
+
 // Row splits of a cross product: as many as fill ONE round of resident
 // workgroups (`slots` = CUs x resident workgroups per CU, at most 2 per CU,
 // which bounds the FP64 partials), with ntiles * splits a multiple of 8 so
-// every XCD gets the same number of work items, and >= 1024 rows per split.
+// every XCD gets the same number of work items, and >= 256 rows per split
+// (a narrow product -- a few output tiles over a few thousand rows -- then
+// still fills the chip; a split's partial tile is 64 KB against >= 32 KB of
+// rows read per operand).
 // A second, partly filled round would idle most of the chip for as long as
 // the first.
 constexpr int CR_MAX_SLOTS = 512;
@@ -352,7 +374,7 @@ int cross_splits(int ntiles, int64_t rows, int slots = CR_MAX_SLOTS) {
   if (slots > CR_MAX_SLOTS) slots = CR_MAX_SLOTS;
   int s = slots / ntiles;
   while (s > 1 && (s * ntiles) % 8 != 0) --s;
-  const int64_t maxs = (rows + 1023) / 1024;
+  const int64_t maxs = (rows + 255) / 256;
   if (s > maxs) s = (int)maxs;
   if (s < 1) s = 1;
   return s;
@@ -398,7 +420,7 @@ int launch_cross(CrossArgs& a, double scale, double beta, double* G, int64_t ldg
   const int64_t nbi = (a.m + 31) / 32, nbj = (a.n + 31) / 32;
   const int64_t nblk = a.symmetric ? nbi * (nbi + 1) / 2 : nbi * nbj;
   hipLaunchKernelGGL(cross_reduce_kernel<double>, dim3((unsigned)nblk), dim3(256), 0, st, a.partial, a.splits, a.ntiles,
-                     a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
+                     a.tiles_n, a.symmetric, a.bdiag, a.m, a.n, scale, beta, G, ldg);
   return hip_status("neb_cross reduce");
 }
 
@@ -431,7 +453,7 @@ int launch_cross_async_as(CrossArgs& a, double scale, double beta, double* G, in
   const int64_t nblk = a.symmetric ? nbi * (nbi + 1) / 2 : nbi * nbj;
   hipLaunchKernelGGL(cross_reduce_kernel<float>, dim3((unsigned)nblk), dim3(256), 0, st,
                      reinterpret_cast<const float*>(a.partial), a.splits, a.ntiles,
-                     a.tiles_n, a.symmetric, a.m, a.n, scale, beta, G, ldg);
+                     a.tiles_n, a.symmetric, a.bdiag, a.m, a.n, scale, beta, G, ldg);
   return hip_status("neb_cross reduce");
 }
 
@@ -652,6 +674,40 @@ int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int6
 #undef CALL
 }
 
+int neb_jet_layer_forward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block, const void* x,
+                                 int64_t ldx, const void* w, int64_t ldw, const void* bias, double omega, void* z,
+                                 void* y, int64_t ldy, void* stream) {
+  if (int rc = check_common(prec, dim, act)) return rc;
+  if (n < 0 || width < 1 || block < 8 || block % 8 || width % block || ldx < width || ldw < width || ldy < width)
+    return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_blocks: bad sizes (block a multiple of 8 dividing the width)");
+  if (n == 0) return NEB_OK;
+  if (!x || !w || !z || !y) return fail(NEB_ERR_ARG, "neb_jet_layer_forward_blocks: null pointer");
+  if (!aligned(x, ldx, prec) || !aligned(w, ldw, prec) || !aligned(y, ldy, prec) || !aligned(z, ldy, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_layer_forward_blocks: rows must be 16-byte aligned");
+#define CALL(P, D, A) layer_forward<P, D, A>(n, width, width, x, ldx, w, ldw, bias, omega, z, y, ldy, S(stream), block)
+  DISPATCH_PDA(CALL)
+#undef CALL
+}
+
+int neb_jet_layer_backward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block,
+                                  const void* adj_in, int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev,
+                                  int64_t ldz, double omega_prev, void* adj_out, int64_t lda_out, void* stream) {
+  if (int rc = check_common(prec, dim, act)) return rc;
+  if (n < 0 || width < 1 || block < 8 || block % 8 || width % block || lda_in < width || ldw < width ||
+      ldz < width || lda_out < width)
+    return fail(NEB_ERR_SHAPE, "neb_jet_layer_backward_blocks: bad sizes (block a multiple of 8 dividing the width)");
+  if (n == 0) return NEB_OK;
+  if (!adj_in || !wt || !z_prev || !adj_out) return fail(NEB_ERR_ARG, "neb_jet_layer_backward_blocks: null pointer");
+  if (!aligned(adj_in, lda_in, prec) || !aligned(wt, ldw, prec) || !aligned(z_prev, ldz, prec) ||
+      !aligned(adj_out, lda_out, prec))
+    return fail(NEB_ERR_ARG, "neb_jet_layer_backward_blocks: rows must be 16-byte aligned");
+#define CALL(P, D, A)                                                                                            \
+  layer_backward<P, D, A>(n, width, width, adj_in, lda_in, wt, ldw, z_prev, ldz, omega_prev, adj_out, lda_out, \
+                          S(stream), block)
+  DISPATCH_PDA(CALL)
+#undef CALL
+}
+
 int neb_jet_output_jets_enveloped(int prec, int dim, int64_t n, int64_t k, int64_t nout, const void* y,
                                   int64_t ldy, const void* w, int64_t ldw, const void* bias, const void* env,
                                   void* out, int64_t ldo, void* stream) {
@@ -794,10 +850,10 @@ size_t neb_cross_workspace_size(int64_t m, int64_t n, int64_t rows, int symmetri
   return (size_t)sp * nt * CR_TM * CR_TN * sizeof(double);
 }
 
-int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, const int64_t* ldx, const int64_t* ldy,
+int cross_run(int mode, int nseg, const void* const* x, const void* const* y, const int64_t* ldx, const int64_t* ldy,
               const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m, int64_t n,
-              const double* roww, int symmetric, double scale, double beta, double* G, int64_t ldg, void* workspace,
-              size_t ws_bytes, void* stream) {
+              const double* roww, int symmetric, int64_t block, double scale, double beta, double* G, int64_t ldg,
+              void* workspace, size_t ws_bytes, void* stream) {
   if (nseg < 1 || nseg > CR_MAXSEG) return fail(NEB_ERR_ARG, "neb_cross: 1..%d segments", CR_MAXSEG);
   if (m < 1 || n < 1 || ldg < n) return fail(NEB_ERR_SHAPE, "neb_cross: bad output shape");
   if (roww && nseg != 1) return fail(NEB_ERR_ARG, "neb_cross: row weights need a single segment");
@@ -816,7 +872,12 @@ int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, co
     if (rows[s] > maxrows) maxrows = rows[s];
   }
   a.nseg = nseg; a.m = m; a.n = n; a.roww = roww; a.symmetric = symmetric;
-  a.ntiles = cross_tiles(m, n, symmetric, &a.tiles_m, &a.tiles_n);
+  if (block > 0) {
+    a.ntiles = cross_block_tiles(m, block, &a.tiles_m, &a.bdiag);
+    a.tiles_n = a.tiles_m;
+  } else {
+    a.ntiles = cross_tiles(m, n, symmetric, &a.tiles_m, &a.tiles_n);
+  }
   a.splits = cross_splits(a.ntiles, maxrows);
   const size_t need = (size_t)a.splits * a.ntiles * CR_TM * CR_TN * sizeof(double);
   if (ws_bytes < need) return fail(NEB_ERR_ARG, "neb_cross: workspace %zu < %zu bytes", ws_bytes, need);
@@ -829,6 +890,32 @@ int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, co
     case NEB_CROSS_F64: return cross_dispatch_types<CR_F64>(x_dtype, y_dtype, a, scale, beta, G, ldg, st);
   }
   return fail(NEB_ERR_ARG, "neb_cross: unknown mode %d", mode);
+}
+
+int neb_cross(int mode, int nseg, const void* const* x, const void* const* y, const int64_t* ldx, const int64_t* ldy,
+              const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m, int64_t n,
+              const double* roww, int symmetric, double scale, double beta, double* G, int64_t ldg, void* workspace,
+              size_t ws_bytes, void* stream) {
+  return cross_run(mode, nseg, x, y, ldx, ldy, rows, sign, x_dtype, y_dtype, m, n, roww, symmetric, 0, scale, beta, G,
+                   ldg, workspace, ws_bytes, stream);
+}
+
+size_t neb_cross_blocks_workspace_size(int64_t m, int64_t block, int64_t rows) {
+  int tm, bd;
+  if (m < 1 || block < 1) return 0;
+  const int nt = cross_block_tiles(m, block, &tm, &bd);
+  const int sp = cross_splits(nt, rows);
+  return (size_t)sp * nt * CR_TM * CR_TN * sizeof(double);
+}
+
+int neb_cross_blocks(int mode, int nseg, const void* const* x, const void* const* y, const int64_t* ldx,
+                     const int64_t* ldy, const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m,
+                     int64_t block, double scale, double beta, double* G, int64_t ldg, void* workspace,
+                     size_t ws_bytes, void* stream) {
+  if (block < 1 || m < 1 || m % block || (block % CR_TM != 0 && CR_TM % block != 0))
+    return fail(NEB_ERR_SHAPE, "neb_cross_blocks: the block must divide m and nest with %d", CR_TM);
+  return cross_run(mode, nseg, x, y, ldx, ldy, rows, sign, x_dtype, y_dtype, m, m, nullptr, 0, block, scale, beta, G,
+                   ldg, workspace, ws_bytes, stream);
 }
 
 int neb_gram(int mode, const void* phi, int64_t rows, int64_t K, int64_t ld, int phi_dtype, double w,
@@ -1126,6 +1213,29 @@ int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t wid
   hipStream_t st = S(stream);
 #define LFX3(D, A) \
   layer_forward_x3<D, A>(n, k, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi, y_lo, ldy, st)
+  if (dim == 2) return act == NEB_TANH ? LFX3(2, ACT_TANH) : LFX3(2, ACT_SIN);
+  return act == NEB_TANH ? LFX3(3, ACT_TANH) : LFX3(3, ACT_SIN);
+#undef LFX3
+}
+
+int neb_jet_layer_forward_x3_blocks(int dim, int act, int64_t n, int64_t width, int64_t block, const void* x_hi,
+                                    const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
+                                    const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,
+                                    int64_t ldy, void* stream) {
+  if (int rc = check_common(NEB_BF16, dim, act)) return rc;
+  if (n < 0 || width < 1 || block < 8 || block % 8 || width % block || ldx < width || ldw < width || ldy < width)
+    return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_x3_blocks: bad sizes (block a multiple of 8 dividing the width)");
+  if (n == 0) return NEB_OK;
+  if (!x_hi || !x_lo || !w_hi || !w_lo || !z_hi || !y_hi || !y_lo)  // z_lo may be null
+    return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3_blocks: null pointer");
+  if (!aligned(x_hi, ldx, NEB_BF16) || !aligned(x_lo, ldx, NEB_BF16) || !aligned(w_hi, ldw, NEB_BF16) ||
+      !aligned(w_lo, ldw, NEB_BF16) || !aligned(z_hi, ldy, NEB_BF16) || !aligned(z_lo, ldy, NEB_BF16) ||
+      !aligned(y_hi, ldy, NEB_BF16) || !aligned(y_lo, ldy, NEB_BF16))
+    return fail(NEB_ERR_ARG, "neb_jet_layer_forward_x3_blocks: rows must be 16-byte aligned");
+  hipStream_t st = S(stream);
+#define LFX3(D, A)                                                                                                  \
+  layer_forward_x3<D, A>(n, width, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi, y_lo, ldy, \
+                         st, block)
   if (dim == 2) return act == NEB_TANH ? LFX3(2, ACT_TANH) : LFX3(2, ACT_SIN);
   return act == NEB_TANH ? LFX3(3, ACT_TANH) : LFX3(3, ACT_SIN);
 #undef LFX3

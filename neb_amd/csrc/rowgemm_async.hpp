@@ -130,8 +130,12 @@ __global__ __launch_bounds__(RowAsyncPol<Plan>::NTH) __attribute__((amdgpu_waves
   const int64_t ntiles = npb * nfb;
   const int64_t t_beg = (int64_t)blockIdx.x * per_wg;
   const int64_t t_end = t_beg + per_wg < ntiles ? t_beg + per_wg : ntiles;
-  const int nk = (int)((a.k + RK - 1) / RK);
-  const int kpart = (int)(a.k - (int64_t)(nk - 1) * RK);  // valid k of the last k-step
+  // contraction length per tile: all of k, or with block-diagonal weights
+  // (a.kwin; the launcher sets it only when every tile's window is whole and
+  // of one length) max(TN, kwin) starting at the tile's first block
+  const int64_t wlen = a.kwin > 0 ? (TN >= a.kwin ? (int64_t)TN : a.kwin) : a.k;
+  const int nk = (int)((wlen + RK - 1) / RK);
+  const int kpart = (int)(wlen - (int64_t)(nk - 1) * RK);  // valid k of the last k-step
 
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wn = wave % WPN, wp = wave / WPN;
@@ -174,7 +178,8 @@ __global__ __launch_bounds__(RowAsyncPol<Plan>::NTH) __attribute__((amdgpu_waves
   // are zeroed before use.
   const int64_t kcap = (a.k + 7) / 8 * 8;
   auto issue = [&](int64_t tp, int64_t tf, int kt, lds_S* slot) __attribute__((always_inline)) {
-    const int64_t p0 = tp * TP, f0 = tf * TN, k0 = (int64_t)kt * RK;
+    const int64_t p0 = tp * TP, f0 = tf * TN;
+    const int64_t k0 = (a.kwin > 0 ? f0 / a.kwin * a.kwin : 0) + (int64_t)kt * RK;
 #pragma unroll
     for (int t = 0; t < PPW; ++t) {
       const int64_t nrows = cw[t] ? a.nout : a.npts;

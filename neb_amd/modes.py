@@ -10,16 +10,18 @@ On the GPU a block of B modes is one grouped GEMM per layer
 (neb_stage_mode_block): the first layer stacks the block's weights by rows
 (every mode reads the same features), the hidden and output layers are
 block-diagonal (each mode reads its own B*hidden-wide slice).  The block is
-then an ordinary jet trunk of width B*hidden, so the fused row-GEMM jets
-(every precision, split operands included), lifts, Grams and reverse of
-neb_amd.network run on it unchanged; the zeros off the diagonal cost MFMA
-issue slots, not HBM bytes (these layers are HBM-bound).  The field
-matrices of all blocks are written side by side into the K columns of one
-[n*comp, K] matrix; the Grams, the objective (trace, or the reference's
-coefficient / velocity peel) and the FP64 algebra are those of the shared
-trunk.  The reverse recomputes each block's tape (the reference's phase-3
-recompute, HB2P:3191-3230), applies the block's rows of the Gram cotangents
-to the full fields, and keeps the diagonal blocks of each weight gradient.
+then a jet trunk of width B*hidden on the fused row-GEMM jets (every
+precision, split operands included), lifts and Grams of neb_amd.network;
+its block-diagonal hidden layers run on the _blocks entry points, whose
+tiles contract only their own mode's inputs (the zeros off the diagonal are
+not read or multiplied), and the weight gradients of those layers are
+block-diagonal crosses.  The field matrices of all blocks are written side
+by side into the K columns of one [n*comp, K] matrix; the Grams, the
+objective (trace, or the reference's coefficient / velocity peel) and the
+FP64 algebra are those of the shared trunk.  The reverse forms the field
+cotangents of all modes once, then recomputes each block's tape (the
+reference's phase-3 recompute, HB2P:3191-3230) and takes the block's
+columns of the cotangents through its output layer, lift and hidden layers.
 
 Parameters are stored mode-stacked, W_j [K, out, in] and b_j [K, out] (the
 NCC:173-185 checkpoint layout; arrays() / from_arrays() give the 2D
@@ -201,22 +203,30 @@ class ModeNet:
         st = nw._STORE[p]
         tape = nw.Tape(p, n, [X0], [], [X0lo] if split else None, None)
         cur, cur_lo, k = X0, X0lo, cfg.features
+        h = cfg.hidden
         for l in range(cfg.hidden_layers):
-            width = B * cfg.hidden
+            width = B * h
             Z = torch.empty((C, n, nw._ld(width)), dtype=st, device=self.device)
             Y = torch.empty_like(Z)
+            # layers after the first are block-diagonal: each tile contracts its own mode's h inputs
             if split:
                 Yl = torch.empty_like(Z)
-                H.call("neb_jet_layer_forward_x3", cfg.dim, H.SIN, n, k, width, cur.data_ptr(), cur_lo.data_ptr(),
-                       cur.shape[2], stg["W"][l].data_ptr(), stg["Wlo"][l].data_ptr(), stg["W"][l].shape[1],
-                       stg["b"][l].data_ptr(), bc.omega(l), Z.data_ptr(), None, Y.data_ptr(), Yl.data_ptr(),
-                       Y.shape[2], s)
+                args = (cur.data_ptr(), cur_lo.data_ptr(), cur.shape[2], stg["W"][l].data_ptr(),
+                        stg["Wlo"][l].data_ptr(), stg["W"][l].shape[1], stg["b"][l].data_ptr(), bc.omega(l),
+                        Z.data_ptr(), None, Y.data_ptr(), Yl.data_ptr(), Y.shape[2], s)
+                if l == 0:
+                    H.call("neb_jet_layer_forward_x3", cfg.dim, H.SIN, n, k, width, *args)
+                else:
+                    H.call("neb_jet_layer_forward_x3_blocks", cfg.dim, H.SIN, n, width, h, *args)
                 tape.inputs_lo.append(Yl)
                 cur_lo = Yl
             else:
-                H.call("neb_jet_layer_forward", p, cfg.dim, H.SIN, n, k, width, cur.data_ptr(), cur.shape[2],
-                       stg["W"][l].data_ptr(), stg["W"][l].shape[1], stg["b"][l].data_ptr(), bc.omega(l),
-                       Z.data_ptr(), Y.data_ptr(), Y.shape[2], s)
+                args = (cur.data_ptr(), cur.shape[2], stg["W"][l].data_ptr(), stg["W"][l].shape[1],
+                        stg["b"][l].data_ptr(), bc.omega(l), Z.data_ptr(), Y.data_ptr(), Y.shape[2], s)
+                if l == 0:
+                    H.call("neb_jet_layer_forward", p, cfg.dim, H.SIN, n, k, width, *args)
+                else:
+                    H.call("neb_jet_layer_forward_blocks", p, cfg.dim, H.SIN, n, width, h, *args)
             tape.pre.append(Z)
             tape.inputs.append(Y)
             cur, k = Y, width
@@ -305,23 +315,37 @@ class ModeNet:
     # -- reverse -----------------------------------------------------------
     def backward_fields(self, x: torch.Tensor, fields: nw.Fields, dM: torch.Tensor, dS: torch.Tensor, weight: float,
                         prec: str = "bf16x3") -> torch.Tensor:
-        """Parameter gradient of L(M, S), M = w U^T U, S = w D^T D, for
-        symmetric FP64 dL/dM, dL/dS: block by block, recompute the block's
-        tape, apply 2w Phi dL/dG[:, block] through the block's output layer
-        and lift (the fused output-layer reverse reading all K field
-        columns), its output-weight products over the full fields, and the
-        hidden-layer reverse; keep the diagonal blocks of each weight
-        gradient (HB2P:2087-2139 backward_gradients_block_jet2)."""
+        """Parameter gradient of L(M, S), M = w U^T U, S = w D^T D, for FP64
+        dL/dM, dL/dS applied unsymmetrised (field cotangents 2w U dM^T,
+        2w D dS^T: the coefficient peel's detached-prefix pair as given).
+        The field cotangents are formed once for all modes (two GEMMs over
+        the K columns, VCORE:94-95 gu = U Cu / n); then block by block the
+        block's tape is recomputed and its columns of the cotangents go
+        through the block's output layer and lift (neb_jet_output_backward
+        with the block's B columns as the contraction), the output-weight
+        products are the cotangent columns against the block's last hidden
+        jets, and the hidden reverse keeps the diagonal blocks of each
+        weight gradient (HB2P:2087-2139 backward_gradients_block_jet2)."""
         cfg = self.cfg
         K = cfg.modes
         d, C = cfg.dim, nw.channels(cfg.dim)
         feats = self.features(x, prec)
         n = feats[0].shape[1]
-        pair = torch.cat([dM.to(torch.float64), dS.to(torch.float64)], 1)     # [K, 2K]
+        gdt = torch.float64 if prec == "f64" else torch.float32
+        # field cotangents [rows, K] (2w folded in); split planes joined first
+        gU = torch.mm(nw.field_matrix(fields, "mass", gdt), dM.to(gdt).T) * (2.0 * weight)
+        gD = torch.mm(nw.field_matrix(fields, "stiff", gdt), dS.to(gdt).T) * (2.0 * weight)
+        ldg = nw._ld(K)
+        GU = torch.zeros((gU.shape[0], ldg), dtype=gdt, device=self.device)
+        GD = torch.zeros((gD.shape[0], ldg), dtype=gdt, device=self.device)
+        GU[:, :K] = gU
+        GD[:, :K] = gD
+        del gU, gD
+        cot = nw.Fields(GU, GD, K, 0)
         grads = torch.zeros_like(self.params)
         gW, gb = self.split(grads)
         s = nw._stream()
-        h, L = cfg.hidden, cfg.hidden_layers
+        h, L, nc = cfg.hidden, cfg.hidden_layers, cfg.ncomp
         for m0, B in self._blocks():
             tape = self.block_tape(feats, m0, B, prec)
             p = tape.prec
@@ -329,92 +353,88 @@ class ModeNet:
             mode = nw._CROSS[p]
             bc = cfg.block_net_config(B)
             stg = self.block_weights(m0, B, prec)
-            # output layer of the block as [ncomp * B, B * h] (rows m * ncomp + c) in FP64
-            Wo = self.W[-1][m0:m0 + B].to(torch.float64)                    # [B, ncomp, h]
-            Wblk = torch.zeros((B * cfg.ncomp, B * h), dtype=torch.float64, device=self.device)
-            for m in range(B):
-                Wblk[m * cfg.ncomp:(m + 1) * cfg.ncomp, m * h:(m + 1) * h] = Wo[m]
-            rows = pair[m0:m0 + B]                                           # [B, 2K]
-            if d == 2:
-                Bp = torch.mm(Wblk.T, rows)                                  # [B h, 2K]
-            else:
-                Bp = torch.cat([torch.mm(Wblk[c::3].T, rows) for c in range(3)], 0)   # [3 B h, 2K]
-            bm = torch.empty((Bp.shape[0], nw._ld(K)), dtype=st, device=self.device)
-            bs = torch.empty_like(bm)
-            for half, dst in ((0, bm), (1, bs)):
-                H.call("neb_stage", H.F64, p, Bp.data_ptr() + 8 * K * half, Bp.shape[0], K, Bp.stride(0),
-                       dst.data_ptr(), dst.shape[1], dst.shape[0], dst.shape[1], 0, 2.0 * weight, s)
+            # the block's output layer per component c as [B h, B]: column m holds
+            # mode m's weights W_out[m, c, :] in rows m h .. m h + h - 1
+            Wo = self.W[-1][m0:m0 + B].to(torch.float64)                    # [B, nc, h]
+            Wt = torch.zeros((nc, B, h, B), dtype=torch.float64, device=self.device)
+            Wt.diagonal(dim1=1, dim2=3).copy_(Wo.permute(1, 2, 0))           # Wt[c, m, :, m] = Wo[m, c]
+            Wt = Wt.reshape(nc * B * h, B)
+            bm = torch.empty((Wt.shape[0], nw._ld(B)), dtype=st, device=self.device)
+            H.call("neb_stage", H.F64, p, Wt.data_ptr(), Wt.shape[0], B, B, bm.data_ptr(), bm.shape[1],
+                   bm.shape[0], bm.shape[1], 0, 1.0, s)
             width = B * h
             A = torch.empty((C, n, nw._ld(width)), dtype=st, device=self.device)
             Z = tape.pre[-1]
-            H.call("neb_jet_output_backward", p, bc.lift, H.SIN, n, width, K, fields.mass.data_ptr(),
-                   fields.stiff.data_ptr(), fields.mass.shape[1], nw._DT[fields.mass.dtype], fields.layout,
-                   bm.data_ptr(), bs.data_ptr(), bm.shape[1], Z.data_ptr(), Z.shape[2], bc.omega(L - 1),
-                   A.data_ptr(), A.shape[2], s)
-            # output weights: 2w [dM | dS][rows] [P_m; P_s] with the field products over all K columns
-            P = self._field_products(tape, fields, B)
-            for c in range(cfg.ncomp):
-                Gc = torch.mm(rows, P[c]) * (2.0 * weight)                   # [B, B h]
-                for m in range(B):
-                    gW[-1][m0 + m, c] = Gc[m, m * h:(m + 1) * h]
+            off = m0 * GU.element_size()
+            H.call("neb_jet_output_backward", p, bc.lift, H.SIN, n, width, B, GU.data_ptr() + off,
+                   GD.data_ptr() + off, ldg, nw._DT[gdt], 0, bm.data_ptr(), bm.data_ptr(), bm.shape[1],
+                   Z.data_ptr(), Z.shape[2], bc.omega(L - 1), A.data_ptr(), A.shape[2], s)
+            # output weights: the block's cotangent columns against its last hidden jets
+            P = self._field_products(tape, cot, m0, B)
+            for c in range(nc):   # gW[-1][m0 + m, c] = P[c][m, m h:(m + 1) h]
+                gW[-1][m0:m0 + B, c].copy_(P[c].view(B, B, h).diagonal(dim1=0, dim2=1).T)
             # hidden layers, diagonal blocks of each weight gradient
             for l in range(L - 1, -1, -1):
                 X = tape.inputs[l]
                 kin = X.shape[2] if l > 0 else cfg.features
-                G = nw.cross([(A.view(C * n, A.shape[2]), X.view(C * n, X.shape[2]), C * n, 1.0)], width, kin, mode)
+                # layer 0 reads the shared features (dense); later layers only their diagonal blocks
+                G = nw.cross([(A.view(C * n, A.shape[2]), X.view(C * n, X.shape[2]), C * n, 1.0)], width, kin, mode,
+                             block=h if l > 0 else None)
                 ws = nw.workspace(self.device).get(H.load().neb_colsum_workspace_size(n, width))
                 db = torch.empty(width, dtype=torch.float64, device=self.device)
                 H.call("neb_colsum", nw._DT[st], A[0].data_ptr(), n, width, A.shape[2], 0.0, db.data_ptr(),
                        ws.data_ptr(), ws.numel(), s)
                 gb[l][m0:m0 + B] = db.view(B, h)
-                for m in range(B):
-                    blk = G[m * h:(m + 1) * h, :] if l == 0 else G[m * h:(m + 1) * h, m * h:(m + 1) * h]
-                    gW[l][m0 + m] = blk
+                if l == 0:
+                    gW[l][m0:m0 + B].copy_(G.view(B, h, kin))
+                else:     # the diagonal h x h blocks: gW[l][m0 + m] = G[m h:(m+1) h, m h:(m+1) h]
+                    gW[l][m0:m0 + B].copy_(G.view(B, h, B, h).diagonal(dim1=0, dim2=2).permute(2, 0, 1))
                 if l > 0:
                     Zp = tape.pre[l - 1]
                     An = torch.empty((C, n, nw._ld(kin)), dtype=st, device=self.device)
                     WT = stg["WT"][l]
-                    H.call("neb_jet_layer_backward", p, d, H.SIN, n, kin, width, A.data_ptr(), A.shape[2],
+                    H.call("neb_jet_layer_backward_blocks", p, d, H.SIN, n, width, h, A.data_ptr(), A.shape[2],
                            WT.data_ptr(), WT.shape[1], Zp.data_ptr(), Zp.shape[2], bc.omega(l - 1), An.data_ptr(),
                            An.shape[2], s)
                     A = An
             del tape, A
         return grads
 
-    def _field_products(self, tape: nw.Tape, fields: nw.Fields, B: int) -> list:
-        """Per output component c: [P_m; P_s] ([2K, B h]): the products of
-        the full field columns with the block's last hidden jet channels that
-        each field reads (the lift's terms), over the points."""
+    def _field_products(self, tape: nw.Tape, cot: nw.Fields, m0: int, B: int) -> list:
+        """Per output component c: [B, B h], the block's columns m0 .. m0+B-1
+        of the field cotangents (mass rows of cot.mass, stiffness rows of
+        cot.stiff) against the block's last hidden jet channels that each
+        field reads (the lift's terms), summed over the points."""
         cfg = self.cfg
-        K, n = cfg.modes, tape.n
+        n = tape.n
         width = B * cfg.hidden
         mode = nw._CROSS[tape.prec]
         Y = tape.inputs[-1]
         ch = lambda c: Y[c]  # noqa: E731
-        U, Dg = fields.mass, fields.stiff
+        col = lambda t: t[:, m0:m0 + B]  # noqa: E731
+        U, Dg = cot.mass, cot.stiff
         rv = nw._rows_view
         if cfg.dim == 2:
-            P = torch.empty((2 * K, width), dtype=torch.float64, device=self.device)
             u, v = rv(U, 0, 2), rv(U, 1, 2)
-            nw.cross([(u, ch(2), n, 1.0), (v, ch(1), n, -1.0)], K, width, mode, out=P[:K])
+            P = nw.cross([(col(u), ch(2), n, 1.0), (col(v), ch(1), n, -1.0)], B, width, mode)
             if cfg.stiffness == "hessian":
-                r2 = math.sqrt(2.0)
-                nw.cross([(rv(Dg, 0, 3), ch(3), n, 1.0), (rv(Dg, 2, 3), ch(5), n, 1.0)], K, width, mode, out=P[K:])
-                P[K:] += r2 * nw.cross([(rv(Dg, 1, 3), ch(4), n, 1.0)], K, width, mode)
+                P += nw.cross([(col(rv(Dg, 0, 3)), ch(3), n, 1.0), (col(rv(Dg, 2, 3)), ch(5), n, 1.0)], B, width,
+                              mode)
+                P += math.sqrt(2.0) * nw.cross([(col(rv(Dg, 1, 3)), ch(4), n, 1.0)], B, width, mode)
             else:
-                nw.cross([(Dg, ch(3), n, -1.0), (Dg, ch(5), n, -1.0)], K, width, mode, out=P[K:])
+                P += nw.cross([(col(Dg), ch(3), n, -1.0), (col(Dg), ch(5), n, -1.0)], B, width, mode)
             return [P]
         from .lifts import curl_terms
         comps = [rv(U, a, 3) for a in range(3)]
         grads3 = [rv(Dg, c, 9) for c in range(9)]
         out = []
         for sl in range(3):
-            segm = [(comps[a], ch(1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a < 3 and w_s == sl]
-            segs = [(grads3[a - 3], ch(1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a >= 3 and w_s == sl]
-            P = torch.empty((2 * K, width), dtype=torch.float64, device=self.device)
-            nw.cross(segm, K, width, mode, out=P[:K])
-            for i in range(0, len(segs), 8):
-                nw.cross(segs[i:i + 8], K, width, mode, out=P[K:], beta=1.0 if i else 0.0)
+            segs = [(col(comps[a]), ch(1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a < 3 and w_s == sl]
+            segs += [(col(grads3[a - 3]), ch(1 + c), n, sg) for (a, c, w_s, sg) in curl_terms()
+                     if a >= 3 and w_s == sl]
+            P = nw.cross(segs[:8], B, width, mode)
+            for i in range(8, len(segs), 8):
+                nw.cross(segs[i:i + 8], B, width, mode, out=P, beta=1.0)
             out.append(P)
         return out
 

@@ -632,16 +632,29 @@ def field_grams(fields: Fields, w: float, mode: str = "bf16x3", M: torch.Tensor 
 
 
 def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, beta: float = 0.0,
-          roww: torch.Tensor | None = None):
+          roww: torch.Tensor | None = None, block: int | None = None):
     """sum_s sign_s X_s^T Y_s over rows; segments = [(X, Y, rows, sign)] with
     X/Y row-major 2-D tensors (views allowed).  roww: optional FP64 weight
-    per row (one segment only)."""
+    per row (one segment only).  block: only the diagonal block x block
+    blocks of the m x m result (neb_cross_blocks; the entries outside the
+    tiles over them are left as they are)."""
     xs = [s[0] for s in segments]
     ys = [s[1] for s in segments]
     if out is None:
         out = torch.empty((m, n), dtype=torch.float64, device=xs[0].device)
         beta = 0.0
     rows = [int(s[2]) for s in segments]
+    if block:
+        if roww is not None or m != n:
+            raise ValueError("a block-diagonal cross has a square result and no row weights")
+        need = H.load().neb_cross_blocks_workspace_size(m, block, max(rows))
+        ws = workspace(xs[0].device).get(need)
+        H.call("neb_cross_blocks", mode, len(segments), H.ptr_array([t.data_ptr() for t in xs]),
+               H.ptr_array([t.data_ptr() for t in ys]), H.i64_array([t.stride(0) for t in xs]),
+               H.i64_array([t.stride(0) for t in ys]), H.i64_array(rows),
+               H.dbl_array([float(s[3]) for s in segments]), _DT[xs[0].dtype], _DT[ys[0].dtype], m, block,
+               float(scale), float(beta), out.data_ptr(), out.stride(0), ws.data_ptr(), ws.numel(), _stream())
+        return out
     need = H.load().neb_cross_workspace_size(m, n, max(rows), 0)
     ws = workspace(xs[0].device).get(need)
     H.call("neb_cross", mode, len(segments), H.ptr_array([t.data_ptr() for t in xs]),

@@ -186,3 +186,74 @@ def test_mode_checkpoint_layouts(tmp_path):
     _, third = make(2, K=8, master=torch.float32, seed=6)
     third.from_arrays(arrays)
     assert torch.equal(third.params, net.params) and torch.equal(third.v, net.v) and third.epoch == 1
+
+
+@pytest.mark.parametrize("dim,h,B", [(3, 128, 4), (2, 32, 16), (3, 16, 8)])
+def test_block_diagonal_entry_points_equal_dense(dim, h, B):
+    """The _blocks entry points (each tile contracts its own mode block) on
+    block-diagonal weights give the dense entry points' results bit for bit
+    (the skipped k-steps only add zeros), in every precision; the
+    block-diagonal cross equals the dense cross on its diagonal blocks."""
+    from neb_amd import _hip as H
+    g = torch.Generator(device="cpu").manual_seed(7 + dim)
+    C, n, wd = nw.channels(dim), 777, B * h
+    ld = nw._ld(wd)
+    Wd = torch.zeros((wd, wd), dtype=torch.float64)
+    for m in range(B):
+        Wd[m * h:(m + 1) * h, m * h:(m + 1) * h] = torch.randn((h, h), generator=g, dtype=torch.float64) / math.sqrt(h)
+    bias = torch.randn(wd, generator=g, dtype=torch.float64)
+    X = torch.zeros((C, n, ld), dtype=torch.float64)
+    X[:, :, :wd] = torch.randn((C, n, wd), generator=g, dtype=torch.float64)
+    s = torch.cuda.current_stream().cuda_stream
+    for prec, p in (("bf16", H.BF16), ("f32", H.F32), ("f64", H.F64)):
+        st = nw._STORE[p]
+        Xd = X.to(DEV, st)
+        W = torch.zeros((wd, ld), dtype=st, device=DEV)
+        W[:, :wd] = Wd.to(DEV, st)
+        WT = torch.zeros((wd, ld), dtype=st, device=DEV)
+        WT[:, :wd] = Wd.T.to(DEV, st)
+        b = bias.to(DEV, torch.float64 if p == H.F64 else torch.float32)
+        outs = []
+        for name, extra in (("neb_jet_layer_forward", (wd, wd)), ("neb_jet_layer_forward_blocks", (wd, h))):
+            Z = torch.zeros((C, n, ld), dtype=st, device=DEV)
+            Y = torch.zeros_like(Z)
+            print(name, dim, h, B, prec, flush=True)
+            H.call(name, p, dim, H.SIN, n, *extra, Xd.data_ptr(), ld, W.data_ptr(), ld, b.data_ptr(), 1.3, Z.data_ptr(),
+                   Y.data_ptr(), ld, s)
+            outs.append((Z, Y))
+        assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1]), (prec, "forward")
+        Zp = outs[0][0]
+        back = []
+        for name, extra in (("neb_jet_layer_backward", (wd, wd)), ("neb_jet_layer_backward_blocks", (wd, h))):
+            A = torch.zeros((C, n, ld), dtype=st, device=DEV)
+            print(name, dim, h, B, prec, flush=True)
+            H.call(name, p, dim, H.SIN, n, *extra, Xd.data_ptr(), ld, WT.data_ptr(), ld, Zp.data_ptr(), ld, 0.7,
+                   A.data_ptr(), ld, s)
+            back.append(A)
+        assert torch.equal(back[0], back[1]), (prec, "backward")
+        mode = nw._CROSS[p]
+        Gd = nw.cross([(Xd.view(C * n, ld), back[0].view(C * n, ld), C * n, 1.0)], wd, wd, mode)
+        Gb = nw.cross([(Xd.view(C * n, ld), back[0].view(C * n, ld), C * n, 1.0)], wd, wd, mode, block=h)
+        tol = 1e-12 if p == H.F64 else 1e-5       # the row splits differ with the tile count
+        for m in range(B):
+            sl = slice(m * h, (m + 1) * h)
+            assert torch.allclose(Gd[sl, sl], Gb[sl, sl], rtol=tol, atol=tol * float(Gd.abs().max())), (prec, m)
+    # split operands
+    hi = X.to(torch.bfloat16)
+    lo = (X - hi.double()).to(torch.bfloat16)
+    whi = torch.zeros((wd, ld), dtype=torch.bfloat16)
+    whi[:, :wd] = Wd.to(torch.bfloat16)
+    wlo = torch.zeros_like(whi)
+    wlo[:, :wd] = (Wd - whi[:, :wd].double()).to(torch.bfloat16)
+    hi, lo, whi, wlo = hi.to(DEV), lo.to(DEV), whi.to(DEV), wlo.to(DEV)
+    b = bias.to(DEV, torch.float32)
+    outs = []
+    for name, extra in (("neb_jet_layer_forward_x3", (wd, wd)), ("neb_jet_layer_forward_x3_blocks", (wd, h))):
+        Z = torch.zeros((C, n, ld), dtype=torch.bfloat16, device=DEV)
+        Yh, Yl = torch.zeros_like(Z), torch.zeros_like(Z)
+        print(name, dim, h, B, flush=True)
+        H.call(name, dim, H.SIN, n, *extra, hi.data_ptr(), lo.data_ptr(), ld, whi.data_ptr(), wlo.data_ptr(), ld,
+               b.data_ptr(), 1.3, Z.data_ptr(), None, Yh.data_ptr(), Yl.data_ptr(), ld, s)
+        outs.append((Z, Yh, Yl))
+    for a, c in zip(*outs):
+        assert torch.equal(a, c), "split forward"
