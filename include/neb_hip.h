@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 9
+#define NEB_ABI_VERSION 10
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -226,6 +226,20 @@ int neb_cross_blocks(int mode, int nseg, const void* const* x, const void* const
                      const int64_t* ldy, const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m,
                      int64_t block, double scale, double beta, double* G, int64_t ldg, void* workspace,
                      size_t ws_bytes, void* stream);
+
+/* The output layer + lift of a block of B per-mode networks (hidden width h)
+ * on split operands: the fields of neb_jet_output_fields_x3 for the block's
+ * B modes, contracting each mode's ncomp output rows over its own h hidden
+ * features only.  y: the block's last hidden jets ([C][n][ldy], width B*h,
+ * hi / lo); w: the block-diagonal output weights ([B*ncomp][ldw], row
+ * m*ncomp + c holds mode m's component c at columns m*h .. m*h+h-1, hi /
+ * lo); the fields go to columns 0 .. B-1 of the split planes (point-major
+ * rows; offset the pointers to the block's first column).  lift: 2, 3 or
+ * NEB_LIFT_2D_HESSIAN; h a multiple of 32, B a multiple of 8.  Replaces the
+ * per-mode output layer and lift of HB2P:1802-1880 / VJ2:93-101. */
+int neb_mode_output_fields_x3(int lift, int64_t n, int64_t h, int64_t B, const void* y_hi, const void* y_lo,
+                              int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
+                              void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, void* stream);
 
 /* Factored output-layer reverse (split-precision tapes).  The fields are
  * linear in the output weights, Phi_a = sum_t sg_t Y_{c_t} W_{s_t}^T over the

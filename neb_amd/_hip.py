@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
@@ -56,6 +56,8 @@ SIGNATURES = {
     "neb_cross_blocks_workspace_size": (_sz, [_i64, _i64, _i64]),
     "neb_cross_blocks": (_int, [_int, _int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _int, _i64, _i64, _dbl, _dbl, _vp,
                                 _i64, _vp, _sz, _vp]),
+    "neb_mode_output_fields_x3": (_int, [_int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _vp, _vp,
+                                         _vp, _i64, _vp]),
     "neb_stack_channels": (_int, [_vp, _vp, _i64, _i64, _i64, _int, _vp, _vp, _vp, _i64, _vp]),
     "neb_gemm_split": (_int, [_i64, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _i64, _vp]),
     "neb_jet_act_reverse": (_int, [_int, _int, _int, _i64, _i64, _vp, _i64, _int, _vp, _vp, _i64, _dbl, _vp, _i64,

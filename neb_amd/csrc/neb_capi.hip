@@ -16,6 +16,7 @@
 #include "galerkin.hpp"
 #include "jet_rowgemm.hpp"
 #include "misc_kernels.hpp"
+#include "mode_fields.hpp"
 #include "rowgemm_async.hpp"
 #include "spd_inverse.hpp"
 #include "spectral_terms.hpp"
@@ -362,10 +363,10 @@ int cross_block_tiles(int64_t m, int64_t block, int* tm, int* bdiag) {
 // Row splits of a cross product: as many as fill ONE round of resident
 // workgroups (`slots` = CUs x resident workgroups per CU, at most 2 per CU,
 // which bounds the FP64 partials), with ntiles * splits a multiple of 8 so
-// every XCD gets the same number of work items, and >= 256 rows per split
-// (a narrow product -- a few output tiles over a few thousand rows -- then
-// still fills the chip; a split's partial tile is 64 KB against >= 32 KB of
-// rows read per operand).
+// every XCD gets the same number of work items, and >= 1024 rows per split
+// -- or >= 256 where 1024-row splits would leave most CUs idle (a narrow
+// product: a few output tiles over a few thousand rows); each split writes a
+// partial tile, so more splits than the chip needs only add partial bytes.
 // A second, partly filled round would idle most of the chip for as long as
 // the first.
 constexpr int CR_MAX_SLOTS = 512;
@@ -374,7 +375,8 @@ int cross_splits(int ntiles, int64_t rows, int slots = CR_MAX_SLOTS) {
   if (slots > CR_MAX_SLOTS) slots = CR_MAX_SLOTS;
   int s = slots / ntiles;
   while (s > 1 && (s * ntiles) % 8 != 0) --s;
-  const int64_t maxs = (rows + 255) / 256;
+  int64_t maxs = (rows + 1023) / 1024;
+  if (maxs * ntiles < 256) maxs = (rows + 255) / 256;
   if (s > maxs) s = (int)maxs;
   if (s < 1) s = 1;
   return s;
@@ -1239,6 +1241,37 @@ int neb_jet_layer_forward_x3_blocks(int dim, int act, int64_t n, int64_t width, 
   if (dim == 2) return act == NEB_TANH ? LFX3(2, ACT_TANH) : LFX3(2, ACT_SIN);
   return act == NEB_TANH ? LFX3(3, ACT_TANH) : LFX3(3, ACT_SIN);
 #undef LFX3
+}
+
+int neb_mode_output_fields_x3(int lift, int64_t n, int64_t h, int64_t B, const void* y_hi, const void* y_lo,
+                              int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
+                              void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, void* stream) {
+  if (lift != NEB_LIFT_2D_VORTICITY && lift != NEB_LIFT_3D_CURL && lift != NEB_LIFT_2D_HESSIAN)
+    return fail(NEB_ERR_ARG, "neb_mode_output_fields_x3: unknown lift %d", lift);
+  if (n < 0 || h < 32 || h % 32 || B < 8 || B % 8 || ldy < B * h || ldw < B * h || ldf < B)
+    return fail(NEB_ERR_SHAPE, "neb_mode_output_fields_x3: bad sizes (h a multiple of 32, B a multiple of 8)");
+  if (n == 0) return NEB_OK;
+  if (!y_hi || !y_lo || !w_hi || !w_lo || !mass_hi || !mass_lo || !stiff_hi || !stiff_lo)
+    return fail(NEB_ERR_ARG, "neb_mode_output_fields_x3: null pointer");
+  if (!aligned(y_hi, ldy, NEB_BF16) || !aligned(y_lo, ldy, NEB_BF16) || !aligned(w_hi, ldw, NEB_BF16) ||
+      !aligned(w_lo, ldw, NEB_BF16) || !aligned(mass_hi, ldf, NEB_BF16) || !aligned(mass_lo, ldf, NEB_BF16) ||
+      !aligned(stiff_hi, ldf, NEB_BF16) || !aligned(stiff_lo, ldf, NEB_BF16))
+    return fail(NEB_ERR_ARG, "neb_mode_output_fields_x3: rows must be 16-byte aligned");
+  const int64_t grid = ((n + 15) / 16) * (B / 8);
+  if (grid >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_mode_output_fields_x3: grid too large");
+  ModeFieldsArgs a;
+  a.y_hi = reinterpret_cast<const __bf16*>(y_hi); a.y_lo = reinterpret_cast<const __bf16*>(y_lo); a.ldy = ldy;
+  a.w_hi = reinterpret_cast<const __bf16*>(w_hi); a.w_lo = reinterpret_cast<const __bf16*>(w_lo); a.ldw = ldw;
+  a.n = n; a.h = h; a.B = B;
+  a.mass_hi = reinterpret_cast<__bf16*>(mass_hi); a.mass_lo = reinterpret_cast<__bf16*>(mass_lo);
+  a.stiff_hi = reinterpret_cast<__bf16*>(stiff_hi); a.stiff_lo = reinterpret_cast<__bf16*>(stiff_lo);
+  a.ldf = ldf;
+  hipStream_t st = S(stream);
+  const dim3 g((unsigned)grid);
+  if (lift == NEB_LIFT_3D_CURL) hipLaunchKernelGGL(mode_fields_kernel<3>, g, dim3(256), 0, st, a);
+  else if (lift == NEB_LIFT_2D_HESSIAN) hipLaunchKernelGGL(mode_fields_kernel<4>, g, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL(mode_fields_kernel<2>, g, dim3(256), 0, st, a);
+  return hip_status("neb_mode_output_fields_x3");
 }
 
 int neb_jet_output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi, const void* y_lo,

@@ -243,7 +243,12 @@ class ModeNet:
         n, k = tape.n, B * cfg.hidden
         s = nw._stream()
         col = lambda t: t.data_ptr() + m0 * t.element_size()  # noqa: E731
-        if prec == "bf16x3":
+        if prec == "bf16x3" and layout == 0 and cfg.hidden % 32 == 0:
+            # each mode's output rows contract only its own hidden features
+            H.call("neb_mode_output_fields_x3", bc.lift, n, cfg.hidden, B, Y.data_ptr(), tape.inputs_lo[-1].data_ptr(),
+                   Y.shape[2], stg["W"][-1].data_ptr(), stg["Wlo"][-1].data_ptr(), stg["W"][-1].shape[1],
+                   *[col(t) for t in planes], planes[0].shape[1], s)
+        elif prec == "bf16x3":
             H.call("neb_jet_output_fields_x3", bc.lift, n, k, B, Y.data_ptr(), tape.inputs_lo[-1].data_ptr(),
                    Y.shape[2], stg["W"][-1].data_ptr(), stg["Wlo"][-1].data_ptr(), stg["W"][-1].shape[1],
                    *[col(t) for t in planes], planes[0].shape[1], layout, s)

@@ -83,6 +83,23 @@ def test_mode_fields_match_oracle(dim, stiff, prec, tol):
     r = rel(field_np(f, "stiff"), D); assert r < tol, r
 
 
+@pytest.mark.parametrize("dim,stiff,hidden,block", [(2, "vorticity", 32, 8), (2, "hessian", 64, 16),
+                                                    (3, "grad_u", 128, 8), (3, "grad_u", 32, 16)])
+def test_mode_split_fields_kernel_matches_oracle(dim, stiff, hidden, block):
+    """Split-precision fields through neb_mode_output_fields_x3 (each mode's
+    output rows contract only its own hidden features; hidden a multiple of
+    32) against the oracle, on a ragged point count."""
+    K = 32 if dim == 2 else 16
+    cfg, net = make(dim, K=K, block=block, hidden=hidden, stiffness=stiff if dim == 2 else "vorticity",
+                    master=torch.float32)
+    x = points(dim, 1001, 11)
+    U, D, _ = o.mode_fields(oracle_of(net), o.feature_jets(x, 2, cfg.lo, cfg.hi), dim,
+                            stiff if dim == 2 else "vorticity")
+    f = net.fields(torch.tensor(x), "bf16x3")
+    r = rel(field_np(f, "mass"), U); assert r < 2e-5, r
+    r = rel(field_np(f, "stiff"), D); assert r < 2e-5, r
+
+
 def test_mode_block_size_does_not_change_fields():
     cfg, net8 = make(2, K=24, block=8)
     _, net24 = make(2, K=24, block=24)
