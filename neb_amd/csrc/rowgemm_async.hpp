@@ -416,7 +416,9 @@ __global__ __launch_bounds__(RowAsyncPol<Plan>::NTH) __attribute__((amdgpu_waves
 // better there (measured on the 2D output-layer reverse; halving its point
 // tile to fit 3 slots doubled the weight traffic per point and was slower
 // still).
+template <class Plan> struct RowAsyncTwoSlots : std::false_type {};
+template <int ACT> struct RowAsyncTwoSlots<PlanHiddenFwdSplit<3, ACT>> : std::true_type {};
 template <class Plan>
 constexpr bool row_async_ok() {
-  return RowAsyncPol<Plan>::NS >= 3;
+  return RowAsyncPol<Plan>::NS >= 3 || (RowAsyncPol<Plan>::NS == 2 && RowAsyncTwoSlots<Plan>::value);
 }
