@@ -209,8 +209,14 @@ constexpr size_t rowgemm_lds_bytes() {
          sizeof(typename Prec<P>::store_t);
 }
 
+// Waves per SIMD the register-staged kernel is compiled for: two (256
+// registers per lane) unless a plan's accumulators, operand fragments and
+// staged loads need more -- those get one wave per SIMD (512 registers)
+// instead of spilling to scratch (RowWaves specialisations after the plans).
+template <int P, class Plan> struct RowWaves { static constexpr int value = P == PREC_F64 ? 1 : 2; };
+
 template <int P, typename TIN, class Plan>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(P == PREC_F64 ? 1 : 2))) void rowgemm_kernel(RowArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<P, Plan>::value))) void rowgemm_kernel(RowArgs a) {
   using PR = Prec<P>;
   using S = typename PR::store_t;
   using M = typename PR::math_t;
@@ -975,3 +981,14 @@ __global__ __launch_bounds__(256) void act_reverse_stacked_kernel(const float* _
 #pragma unroll
   for (int c = 0; c < C; ++c) st_group<S, 8, float>(out + ((int64_t)c * n + p) * ldo + f0, b[c], nv);
 }
+
+// Plans that spilled at two waves per SIMD (measured with
+// -Rpass-analysis=kernel-resource-usage): the 3D output layer + curl and its
+// transpose, the 2D Hessian-row output layer and its transpose, and the
+// sine transposes of the output layer in reduced precision.
+template <int P> struct RowWaves<P, PlanOutFields3Split> { static constexpr int value = 1; };
+template <int P, bool S> struct RowWaves<P, PlanOutFields3T<S>> { static constexpr int value = 1; };
+template <int P, bool S, bool T> struct RowWaves<P, PlanOutFields2HT<S, T>> { static constexpr int value = 1; };
+template <int P, int A> struct RowWaves<P, PlanOutBwd2H<A>> { static constexpr int value = 1; };
+template <int P, int A> struct RowWaves<P, PlanOutBwd3<A>> { static constexpr int value = P == PREC_BF16 ? 2 : 1; };
+template <int P> struct RowWaves<P, PlanOutBwd2<ACT_SIN>> { static constexpr int value = 1; };

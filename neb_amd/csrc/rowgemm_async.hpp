@@ -416,6 +416,11 @@ __global__ __launch_bounds__(RowAsyncPol<Plan>::NTH) __attribute__((amdgpu_waves
 // better there (measured on the 2D output-layer reverse; halving its point
 // tile to fit 3 slots doubled the weight traffic per point and was slower
 // still).
+// The 3D split hidden layers (20 input planes: 36 KB slots) run on a
+// two-slot ring: one k-step in flight, like the register-staged kernel, but
+// the persistent tile walk issues the next tile's copies during an epilogue
+// (a 131,072-point 3D layer 5.0 -> 4.1 ms; per-mode blocks with 4 k-steps
+// a tile, 186 -> 167 us).
 template <class Plan> struct RowAsyncTwoSlots : std::false_type {};
 template <int ACT> struct RowAsyncTwoSlots<PlanHiddenFwdSplit<3, ACT>> : std::true_type {};
 template <class Plan>
