@@ -52,17 +52,19 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 10
+#define NEB_ABI_VERSION 11
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
 enum { NEB_TANH = 0, NEB_SIN = 1 };                           /* activations */
 /* lift codes of the output-layer entry points (their `dim` argument): the 2D
- * velocity + vorticity lift, the 3D curl lift, and the 2D velocity + Hessian
+ * velocity + vorticity lift, the 3D curl lift, the 2D velocity + Hessian
  * lift of the causal velocity peel (stiffness rows psi_xx, sqrt2 psi_xy,
  * psi_yy, whose Gram is the Hessian Frobenius form with the pair metric
- * (1, 2, 1) of HB2P:3022) */
-enum { NEB_LIFT_2D_VORTICITY = 2, NEB_LIFT_3D_CURL = 3, NEB_LIFT_2D_HESSIAN = 4 };
+ * (1, 2, 1) of HB2P:3022), and the 2D streamfunction lift of the Dirichlet
+ * Rayleigh peel (mass rows psi, stiffness rows psi_x, psi_y: M = int psi^2,
+ * K = int |grad psi|^2, HB2P:3394-3397) */
+enum { NEB_LIFT_2D_VORTICITY = 2, NEB_LIFT_3D_CURL = 3, NEB_LIFT_2D_HESSIAN = 4, NEB_LIFT_2D_STREAM = 5 };
 enum { NEB_CROSS_F32 = 0, NEB_CROSS_BF16 = 1, NEB_CROSS_BF16X3 = 2, NEB_CROSS_F64 = 3 };
 
 int neb_abi_version(void);
@@ -120,7 +122,10 @@ int neb_envelope_reverse(int prec, int dim, int64_t n, int64_t nout, const void*
  * VCORE:71-77).  dim = NEB_LIFT_2D_HESSIAN: 2D mass as above, stiff = the
  * Hessian rows (psi_xx, sqrt2 psi_xy, psi_yy) as [3n][ldf] (the velocity
  * Rayleigh peel's H1 seminorm, HB2P:2935-3060, metric (1, 2, 1) at
- * HB2P:3022); the same lift codes apply to neb_jet_output_fields_split,
+ * HB2P:3022); dim = NEB_LIFT_2D_STREAM: mass = psi as [n][ldf] (without the
+ * output bias, which the peel's centering removes), stiff = (psi_x, psi_y)
+ * as [2n][ldf] (HB2P:3461-3474); the same lift codes apply to
+ * neb_jet_output_fields_split,
  * neb_jet_output_fields_x3 and neb_jet_output_backward.  layout 0 =
  * point-major rows (p*comp+c, the reference's [N*comp, K] layout,
  * VCORE:75-76); 1 = component-major (c*n+p). */
@@ -162,6 +167,16 @@ int neb_split_bf16(int src_dtype, const void* src, int64_t rows, int64_t cols, i
 /* The inverse of neb_split_bf16: dst = hi + lo (rows x cols, plane stride ld)
  * as NEB_F32 (exact) or NEB_F64.  The split-precision basis enters the FP64
  * full-mass projection loads through it (PP:141-177 reads an fp32 basis). */
+/* Column centering x[r][c] -= mean[c] of a rows x cols matrix (stride ld):
+ * dtype NEB_F32 / NEB_F64, or NEB_BF16 -- one plane (lo null, the
+ * difference rounded to bf16) or split planes (lo not null: hi + lo - mean
+ * split again into hi and lo).  mean: FP64 [cols].  The periodic gauge of
+ * the Dirichlet streamfunction peel (HB2P:3475-3479, `raw_value -=
+ * cp.mean(raw_value, axis=0)`), applied to the field matrix the Grams and
+ * the output-layer reverse then read. */
+int neb_center_columns(int dtype, void* x, void* lo, int64_t rows, int64_t cols, int64_t ld, const double* mean,
+                       void* stream);
+
 int neb_join_bf16(int dst_dtype, const void* hi, const void* lo, int64_t rows, int64_t cols, int64_t ld, void* dst,
                   int64_t ldd, void* stream);
 int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t width, const void* x_hi,

@@ -11,9 +11,9 @@ from .network import (EigenfluidNet, Fields, NetConfig, Tape, backward_fields, b
                       output_fields, output_jets, trunk_forward)
 from .modes import ModeConfig, ModeNet
 from .spectral import (JetGram, PeelSolver, SpectralSolver, basis_stream, chunked_step, coefficient_peel, compute,
-                       velocity_peel_step,
+                       streamfunction_peel_step, velocity_peel_step,
                        fullmass, generalized_eigh, project, project_streamed, spectral_objective)
 
 __all__ = ["EigenfluidNet", "Fields", "NetConfig", "Tape", "backward_fields", "backward_jets", "envelope_jets", "channels", "cross", "gram",
            "output_fields", "output_jets", "trunk_forward", "JetGram", "PeelSolver", "SpectralSolver", "coefficient_peel", "basis_stream", "chunked_step", "compute", "fullmass", "generalized_eigh", "project", "project_streamed",
-           "spectral_objective", "velocity_peel_step", "ModeConfig", "ModeNet"]
+           "spectral_objective", "streamfunction_peel_step", "velocity_peel_step", "ModeConfig", "ModeNet"]

@@ -13,11 +13,11 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
-LIFT_2D_VORTICITY, LIFT_3D_CURL, LIFT_2D_HESSIAN = 2, 3, 4
+LIFT_2D_VORTICITY, LIFT_3D_CURL, LIFT_2D_HESSIAN, LIFT_2D_STREAM = 2, 3, 4, 5
 CROSS_F32, CROSS_BF16, CROSS_BF16X3, CROSS_F64 = 0, 1, 2, 3
 
 _i64, _dbl, _int, _vp, _sz = C.c_int64, C.c_double, C.c_int, C.c_void_p, C.c_size_t
@@ -38,6 +38,7 @@ SIGNATURES = {
     "neb_jet_output_fields_split": (_int, [_int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, _i64,
                                            _int, _vp]),
     "neb_split_bf16": (_int, [_int, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _vp]),
+    "neb_center_columns": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _vp]),
     "neb_join_bf16": (_int, [_int, _vp, _vp, _i64, _i64, _i64, _vp, _i64, _vp]),
     "neb_jet_layer_forward_x3": (_int, [_int, _int, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _dbl,
                                         _vp, _vp, _vp, _vp, _i64, _vp]),

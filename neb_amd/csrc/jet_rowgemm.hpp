@@ -667,6 +667,49 @@ struct PlanOutFields2HSplit {
   }
 };
 
+// 2D output layer with the streamfunction lift of the Dirichlet Rayleigh
+// peel (HB2P:3383-3474): mass rows psi (one per point), stiffness rows
+// (psi_x, psi_y) (two per point, like the velocity rows).  No output bias:
+// the peel removes each mode's mean over the points (the periodic gauge,
+// HB2P:3475-3479), which cancels it.
+// Inputs: x[0]=v, x[1]=gx, x[2]=gy of the last hidden jet.
+template <bool SPLIT, bool STREAM = false> struct PlanOutFields2ST {
+  template <int P> __host__ __device__ static constexpr int out_bytes() {
+    return SPLIT ? 2 : sizeof(typename Prec<P>::math_t);
+  }
+  static constexpr int CI = 3, NW = 1, NA = 3, NT = 3;
+  static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
+  static constexpr int ta[NT] = {0, 1, 2};
+  static constexpr int tc[NT] = {0, 1, 2};
+  static constexpr int tw[NT] = {0, 0, 0};
+  static constexpr bool tneg[NT] = {false, false, false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    const int64_t r0 = a.layout == 0 ? 2 * p : p;
+    const int64_t r1 = a.layout == 0 ? 2 * p + 1 : a.npts + p;
+    st_field<SPLIT, GE, M, STREAM>(a.out0, a.out0lo, p * a.ldo + f, v[0], nv);
+    st_field<SPLIT, GE, M, STREAM>(a.out1, a.out1lo, r0 * a.ldo2 + f, v[1], nv);
+    st_field<SPLIT, GE, M, STREAM>(a.out1, a.out1lo, r1 * a.ldo2 + f, v[2], nv);
+  }
+};
+
+// The same on split operands: x[0..2] hi planes of v, gx, gy, x[3..5] their
+// lo planes; w[0] = W_hi, w[1] = W_lo; hi / lo field planes out.
+struct PlanOutFields2SSplit {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return 2; }
+  using B = PlanOutFields2ST<true, true>;
+  static constexpr int CI = 6, NW = 2, NA = 3, NT = 9;
+  static constexpr int FB = B::FB, PB = 1, WPN = B::WPN, WPP = B::WPP;
+  static constexpr int ta[NT] = {0, 0, 0, 1, 1, 1, 2, 2, 2};
+  static constexpr int tc[NT] = {0, 3, 0, 1, 4, 1, 2, 5, 2};
+  static constexpr int tw[NT] = {1, 0, 0, 1, 0, 0, 1, 0, 0};
+  static constexpr bool tneg[NT] = {false, false, false, false, false, false, false, false, false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    B::template epilogue<P, GE, M>(a, p, f, nv, v);
+  }
+};
+
 // 3D output layer with the curl lift (NCC:97-99, VJ2:93-101).  Vector
 // potential A has output rows 3k+c, passed as three weight slices W_c.
 // Inputs x[0..2] = grad channels g0..g2, x[3..8] = Hessian pairs h0..h5.
@@ -763,6 +806,7 @@ struct IdentityTerms10 { IDENTITY_TERMS };
 static_assert(split_terms_match<PlanOutFields3Split, 24, 9, 3, PlanOutFields3T<true>>(), "3D split field terms");
 static_assert(split_terms_match<PlanOutFields2Split, 4, 4, 1, PlanOutFields2T<true>>(), "2D split field terms");
 static_assert(split_terms_match<PlanOutFields2HSplit, 5, 5, 1, PlanOutFields2HT<true>>(), "2D Hessian split terms");
+static_assert(split_terms_match<PlanOutFields2SSplit, 3, 3, 1, PlanOutFields2ST<true>>(), "2D stream split terms");
 static_assert(split_terms_match<HiddenSplitTerms<2>, 6, 6, 1, IdentityTerms10>(), "2D split hidden terms");
 static_assert(split_terms_match<HiddenSplitTerms<3>, 10, 10, 1, IdentityTerms10>(), "3D split hidden terms");
 
@@ -869,6 +913,30 @@ template <int ACT> struct PlanOutBwd2H {
     for (int e = 0; e < GE; ++e) {
       adj[0][e] = (M)0; adj[1][e] = v[0][e]; adj[2][e] = v[1][e];
       adj[3][e] = v[2][e]; adj[4][e] = r2 * v[3][e]; adj[5][e] = v[4][e];
+    }
+    reverse_epilogue<P, 2, ACT, GE, M>(a, p, f, nv, adj);
+  }
+};
+
+// 2D output-layer reverse for the streamfunction lift (the transpose of
+// PlanOutFields2ST): x[0] = psi rows, x[1] = psi_x rows, x[2] = psi_y rows;
+// w[0] = Bm, w[1] = Bs.
+//   adj v = psi . Bm, adj gx = sx . Bs, adj gy = sy . Bs
+template <int ACT> struct PlanOutBwd2S {
+  template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::store_t); }
+  static constexpr int CI = 3, NW = 2, NA = 3, NT = 3;
+  static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
+  static constexpr int ta[NT] = {0, 1, 2};
+  static constexpr int tc[NT] = {0, 1, 2};
+  static constexpr int tw[NT] = {0, 1, 1};
+  static constexpr bool tneg[NT] = {false, false, false};
+  template <int P, int GE, typename M>
+  __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
+    M adj[6][GE];
+#pragma unroll
+    for (int e = 0; e < GE; ++e) {
+      adj[0][e] = v[0][e]; adj[1][e] = v[1][e]; adj[2][e] = v[2][e];
+      adj[3][e] = (M)0; adj[4][e] = (M)0; adj[5][e] = (M)0;
     }
     reverse_epilogue<P, 2, ACT, GE, M>(a, p, f, nv, adj);
   }

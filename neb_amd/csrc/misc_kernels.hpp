@@ -2,6 +2,8 @@
 // jets, column sums (bias gradients), sum of squares (gradient norm), Adam,
 // and weight staging (precision conversion, transpose, zero padding).
 #pragma once
+#include <type_traits>
+
 #include "neb_common.hpp"
 
 // Periodic sin/cos feature jets (HB2P:566-589, periodic branch; HB2P:428-441).
@@ -315,6 +317,37 @@ __global__ __launch_bounds__(256) void join_kernel(const __bf16* __restrict__ hi
   for (int q = 0; q < 8; ++q) {
     const int64_t c = c0 + q;
     if (c < cols) d[c] = (TD)((float)h[c] + (float)l[c]);
+  }
+}
+
+// Column centering x[r][c] -= mean[c] (see neb_center_columns).  Split
+// planes (lo not null): hi + lo - mean in FP64, split again like
+// split_kernel.  Each thread handles eight consecutive columns of one row.
+template <typename T>
+__global__ __launch_bounds__(256) void center_kernel(T* __restrict__ x, __bf16* __restrict__ lo, int64_t rows,
+                                                     int64_t cols, int64_t ld, const double* __restrict__ mean) {
+  const int64_t per_row = (cols + 7) / 8;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * per_row) return;
+  const int64_t r = idx / per_row, c0 = (idx % per_row) * 8;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int64_t c = c0 + q;
+    if (c >= cols) break;
+    T* p = x + r * ld + c;
+    if constexpr (std::is_same<T, __bf16>::value) {
+      if (lo) {
+        __bf16* pl = lo + r * ld + c;
+        const double v = (double)(float)*p + (double)(float)*pl - mean[c];
+        const __bf16 h = (__bf16)(float)v;
+        *p = h;
+        *pl = (__bf16)(float)(v - (double)(float)h);
+      } else {
+        *p = (__bf16)(float)((double)(float)*p - mean[c]);
+      }
+    } else {
+      *p = (T)((double)*p - mean[c]);
+    }
   }
 }
 

@@ -117,9 +117,10 @@ int check_common(int prec, int dim, int act) {
 }
 
 // The lift argument of the output-layer entry points: 2 (2D velocity +
-// vorticity), 3 (3D curl), NEB_LIFT_2D_HESSIAN (2D velocity + Hessian rows).
+// vorticity), 3 (3D curl), NEB_LIFT_2D_HESSIAN (2D velocity + Hessian rows),
+// NEB_LIFT_2D_STREAM (2D streamfunction + gradient rows).
 int check_lift(int prec, int lift, int act) {
-  if (lift == NEB_LIFT_2D_HESSIAN) return check_common(prec, 2, act);
+  if (lift == NEB_LIFT_2D_HESSIAN || lift == NEB_LIFT_2D_STREAM) return check_common(prec, 2, act);
   return check_common(prec, lift, act);
 }
 
@@ -186,6 +187,12 @@ int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64
   a.out0 = mass; a.out1 = stiff; a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
   a.out0lo = mass_lo; a.out1lo = stiff_lo;
   const char* what = SPLIT ? "neb_jet_output_fields_split" : "neb_jet_output_fields";
+  if (dim == NEB_LIFT_2D_STREAM) {
+    // channels v, gx, gy
+    for (int c = 0; c < 3; ++c) { a.x[c] = Y + c * cs; a.ldx[c] = ldy; }
+    a.w[0] = w; a.ldw[0] = ldw;
+    return launch_rowgemm<P, Sx, PlanOutFields2ST<SPLIT, SPLIT>>(a, st, what);
+  }
   if (dim == NEB_LIFT_2D_HESSIAN) {
     // channels gx, gy, hxx, hxy, hyy
     for (int c = 0; c < 5; ++c) { a.x[c] = Y + (1 + c) * cs; a.ldx[c] = ldy; }
@@ -239,6 +246,15 @@ int output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi,
   a.npts = n; a.nout = K; a.k = k;
   a.out0 = mass_hi; a.out0lo = mass_lo; a.out1 = stiff_hi; a.out1lo = stiff_lo;
   a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
+  if (dim == NEB_LIFT_2D_STREAM) {
+    for (int c = 0; c < 3; ++c) {
+      a.x[c] = Yh + c * cs;
+      a.x[3 + c] = Yl + c * cs;
+      a.ldx[c] = a.ldx[3 + c] = ldy;
+    }
+    a.w[0] = w_hi; a.w[1] = w_lo; a.ldw[0] = a.ldw[1] = ldw;
+    return launch_rowgemm<PREC_BF16, __bf16, PlanOutFields2SSplit>(a, st, "neb_jet_output_fields_x3");
+  }
   if (dim == NEB_LIFT_2D_HESSIAN) {
     for (int c = 0; c < 5; ++c) {
       a.x[c] = Yh + (1 + c) * cs;
@@ -284,6 +300,14 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
   a.tape = z; a.ldt = ldz; a.tstride = n * ldz;
   a.omega = omega;
   a.out0 = adj; a.ldo = lda; a.cstride = n * lda;
+  if (dim == NEB_LIFT_2D_STREAM) {
+    a.x[0] = Mf; a.ldx[0] = ldf;
+    a.x[1] = Sf; a.ldx[1] = layout == 0 ? 2 * ldf : ldf;
+    a.x[2] = layout == 0 ? Sf + ldf : Sf + n * ldf; a.ldx[2] = a.ldx[1];
+    a.w[0] = bm; a.ldw[0] = ldb;
+    a.w[1] = bs; a.ldw[1] = ldb;
+    return launch_rowgemm<P, F, PlanOutBwd2S<ACT>>(a, st, "neb_jet_output_backward");
+  }
   if (dim == NEB_LIFT_2D_HESSIAN) {
     a.x[0] = Mf; a.ldx[0] = layout == 0 ? 2 * ldf : ldf;
     a.x[1] = layout == 0 ? Mf + ldf : Mf + n * ldf; a.ldx[1] = a.ldx[0];
@@ -1172,6 +1196,33 @@ int neb_galerkin_rhs2d(int dtype, const void* gu, const void* gv, const void* gw
   if (int rc = hip_status("neb_galerkin_rhs2d columns")) return rc;
   hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, part, sp, K, rhs, 0.0);
   return hip_status("neb_galerkin_rhs2d reduce");
+}
+
+int neb_center_columns(int dtype, void* x, void* lo, int64_t rows, int64_t cols, int64_t ld, const double* mean,
+                       void* stream) {
+  if (rows < 0 || cols < 0 || ld < cols) return fail(NEB_ERR_SHAPE, "neb_center_columns: bad sizes");
+  if (lo && dtype != NEB_BF16) return fail(NEB_ERR_ARG, "neb_center_columns: a lo plane needs dtype NEB_BF16");
+  const int64_t units = rows * ((cols + 7) / 8);
+  if (units == 0) return NEB_OK;
+  if (!x || !mean) return fail(NEB_ERR_ARG, "neb_center_columns: null pointer");
+  if (units / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_center_columns: too many elements");
+  const dim3 grid((unsigned)((units + 255) / 256));
+  hipStream_t st = S(stream);
+  switch (dtype) {
+    case NEB_F32:
+      hipLaunchKernelGGL(center_kernel<float>, grid, dim3(256), 0, st, reinterpret_cast<float*>(x), nullptr, rows,
+                         cols, ld, mean);
+      return hip_status("neb_center_columns");
+    case NEB_F64:
+      hipLaunchKernelGGL(center_kernel<double>, grid, dim3(256), 0, st, reinterpret_cast<double*>(x), nullptr, rows,
+                         cols, ld, mean);
+      return hip_status("neb_center_columns");
+    case NEB_BF16:
+      hipLaunchKernelGGL(center_kernel<__bf16>, grid, dim3(256), 0, st, reinterpret_cast<__bf16*>(x),
+                         reinterpret_cast<__bf16*>(lo), rows, cols, ld, mean);
+      return hip_status("neb_center_columns");
+  }
+  return fail(NEB_ERR_ARG, "neb_center_columns: unknown dtype %d", dtype);
 }
 
 int neb_join_bf16(int dst_dtype, const void* hi, const void* lo, int64_t rows, int64_t cols, int64_t ld, void* dst,

@@ -64,7 +64,10 @@ class NetConfig:
     # stiffness fields of the 2D lift: "vorticity" (omega = -lap psi, one row
     # per point) or "hessian" (psi_xx, sqrt2 psi_xy, psi_yy: the H1 seminorm
     # of the velocity with the pair metric (1, 2, 1) of the causal velocity
-    # peel, HB2P:2935-3060, :3022); 3D always uses grad u (nine rows)
+    # peel, HB2P:2935-3060, :3022), both with the velocity (psi_y, -psi_x) as
+    # mass fields; or "stream" (the Dirichlet streamfunction peel,
+    # HB2P:3383-3474: mass rows psi centered over the points, stiffness rows
+    # (psi_x, psi_y)); 3D always uses curl A and grad u (three and nine rows)
     stiffness: str = "vorticity"
     # output-layer reverse of split-precision tapes: "factored" (the K x K
     # contraction in FP64 on the master weights, then width-long split GEMMs
@@ -93,6 +96,8 @@ class NetConfig:
             return H.LIFT_3D_CURL
         if self.stiffness == "hessian":
             return H.LIFT_2D_HESSIAN
+        if self.stiffness == "stream":
+            return H.LIFT_2D_STREAM
         if self.stiffness != "vorticity":
             raise ValueError(f"unknown 2D stiffness {self.stiffness!r}")
         return H.LIFT_2D_VORTICITY
@@ -102,7 +107,7 @@ class NetConfig:
         """(mass rows, stiffness rows) per point of the lifted fields."""
         if self.dim == 3:
             return 3, 9
-        return (2, 3) if self.stiffness == "hessian" else (2, 1)
+        return {"hessian": (2, 3), "stream": (1, 2)}.get(self.stiffness, (2, 1))
 
     @property
     def act_code(self) -> int:
@@ -535,7 +540,7 @@ def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0, split: bool =
         H.call("neb_jet_output_fields_x3", cfg.lift, n, k, K, Y.data_ptr(), tape.inputs_lo[-1].data_ptr(), Y.shape[2],
                sw["W"][-1].data_ptr(), sw["Wlo"][-1].data_ptr(), sw["W"][-1].shape[1],
                *[t.data_ptr() for t in planes], planes[0].shape[1], layout, _stream())
-        return Fields(planes[0], planes[2], K, layout, planes[1], planes[3])
+        return _gauge(cfg, Fields(planes[0], planes[2], K, layout, planes[1], planes[3]))
     stg = net.staged(prec)
     if split:
         if prec != H.BF16:
@@ -545,13 +550,41 @@ def output_fields(net: EigenfluidNet, tape: Tape, layout: int = 0, split: bool =
         H.call("neb_jet_output_fields_split", cfg.lift, n, k, K, Y.data_ptr(), Y.shape[2],
                stg["W"][-1].data_ptr(), stg["W"][-1].shape[1], *[t.data_ptr() for t in planes],
                planes[0].shape[1], layout, _stream())
-        return Fields(planes[0], planes[2], K, layout, planes[1], planes[3])
+        return _gauge(cfg, Fields(planes[0], planes[2], K, layout, planes[1], planes[3]))
     ft = _FIELD[prec]
     mass = torch.empty((cm * n, _ld(K)), dtype=ft, device=net.device)
     stiff = torch.empty((cs * n, _ld(K)), dtype=ft, device=net.device)
     H.call("neb_jet_output_fields", prec, cfg.lift, n, k, K, Y.data_ptr(), Y.shape[2], stg["W"][-1].data_ptr(),
            stg["W"][-1].shape[1], mass.data_ptr(), stiff.data_ptr(), mass.shape[1], layout, _stream())
-    return Fields(mass, stiff, K, layout)
+    return _gauge(cfg, Fields(mass, stiff, K, layout))
+
+
+def column_mean(x: torch.Tensor, rows: int, cols: int, lo: torch.Tensor | None = None) -> torch.Tensor:
+    """FP64 column means of a [rows, ld] matrix (of hi + lo for split
+    planes) by the library's split column sums."""
+    ws = workspace(x.device).get(H.load().neb_colsum_workspace_size(rows, cols))
+    out = torch.empty(cols, dtype=torch.float64, device=x.device)
+    for beta, t in ((0.0, x), (1.0, lo)):
+        if t is not None:
+            H.call("neb_colsum", _DT[t.dtype], t.data_ptr(), rows, cols, t.stride(0), beta, out.data_ptr(),
+                   ws.data_ptr(), ws.numel(), _stream())
+    return out.div_(max(rows, 1))
+
+
+def _gauge(cfg: NetConfig, f: Fields) -> Fields:
+    """The streamfunction lift's periodic gauge: every mode's psi rows lose
+    their mean over the points (HB2P:3475-3479), in place (hi + lo split
+    again for split fields).  The Grams and the output-layer reverse then
+    read the centered rows; their field cotangents 2w Psi_c dL/dM are
+    mean-free already, which is the transpose of the centering
+    (HB2P:3579-3583).  Other lifts pass through."""
+    if cfg.dim != 2 or cfg.stiffness != "stream":
+        return f
+    n, K = f.mass.shape[0], f.K
+    mu = column_mean(f.mass, n, K, f.mass_lo)
+    H.call("neb_center_columns", _DT[f.mass.dtype], f.mass.data_ptr(),
+           f.mass_lo.data_ptr() if f.mass_lo is not None else None, n, K, f.mass.stride(0), mu.data_ptr(), _stream())
+    return f
 
 
 # ---------------------------------------------------------------------------
@@ -750,6 +783,8 @@ def lift_terms(cfg: NetConfig) -> tuple[list, list]:
         t = curl_terms()
         return ([(a, 1 + c, s, sg) for (a, c, s, sg) in t if a < 3],
                 [(a - 3, 1 + c, s, sg) for (a, c, s, sg) in t if a >= 3])
+    if cfg.stiffness == "stream":
+        return [(0, 0, 0, 1.0)], [(0, 1, 0, 1.0), (1, 2, 0, 1.0)]
     mass = [(0, 2, 0, 1.0), (1, 1, 0, -1.0)]
     if cfg.stiffness == "hessian":
         return mass, [(0, 3, 0, 1.0), (1, 4, 0, math.sqrt(2.0)), (2, 5, 0, 1.0)]
@@ -776,6 +811,12 @@ def factored_reverse_ok(net: EigenfluidNet, tape: Tape) -> bool:
     if mode not in ("auto", "factored", "fields"):
         raise ValueError(f"unknown split_reverse {mode!r}")
     if tape.inputs_lo is None or net.cfg.widths[-1] % 8:
+        return False
+    if net.cfg.dim == 2 and net.cfg.stiffness == "stream":
+        # the centered streamfunction rows need the centered value channel,
+        # which the factored algebra does not form
+        if mode == "factored":
+            raise ValueError("the streamfunction lift takes the field-form reverse (split_reverse='fields')")
         return False
     return net.cfg.dim == 3 if mode == "auto" else mode == "factored"
 
@@ -1132,6 +1173,25 @@ def _products(net: EigenfluidNet, tape: Tape, fields: Fields) -> list:
     U, Dg = fields.mass, fields.stiff
     if fields.layout != 0:
         raise ValueError("the output-layer products expect point-major field matrices")
+    if d == 2 and cfg.stiffness == "stream":
+        # psi_c = (Y_v - 1 ybar^T) W^T (centered), grad psi = (Y_x, Y_y) W^T:
+        #   P_m = psi_c^T Y_v = W (Y_v^T Y_v - n ybar ybar^T),  P_s = W (Y_x^T Y_x + Y_y^T Y_y)
+        # (the mean-free Gram cancels the square of the value channel's mean:
+        # split tapes take it from the split Gram of hi + lo, whose error stays
+        # ~1e-7 of Y_v^T Y_v rather than the bf16 hi plane's 2^-9)
+        Ylo = tape.inputs_lo[-1] if tape.inputs_lo is not None else None
+        if Ylo is not None:
+            Hm = gram(ch(Ylast, 0), width, 1.0, "bf16x3", lo=ch(Ylo, 0))
+        else:
+            Hm = cross([(ch(Ylast, 0), ch(Ylast, 0), n, 1.0)], width, width, mode)
+        yb = column_mean(ch(Ylast, 0), n, width, None if Ylo is None else ch(Ylo, 0))
+        Hm.sub_(torch.outer(yb, yb), alpha=float(n))
+        Hs = cross([(ch(Ylast, 1), ch(Ylast, 1), n, 1.0), (ch(Ylast, 2), ch(Ylast, 2), n, 1.0)], width, width, mode)
+        Wd = net.W[-1].to(torch.float64)
+        P = torch.empty((2 * K, width), dtype=torch.float64, device=U.device)
+        torch.mm(Wd, Hm, out=P[:K])
+        torch.mm(Wd, Hs, out=P[K:])
+        return [P]
     if d == 2:
         Hm = cross([(ch(Ylast, 1), ch(Ylast, 1), n, 1.0), (ch(Ylast, 2), ch(Ylast, 2), n, 1.0)], width, width, mode)
         if cfg.stiffness == "hessian":

@@ -466,6 +466,72 @@ def velocity_peel_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: floa
     return out
 
 
+def streamfunction_peel_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16x3",
+                             training: bool = True, causal_sorting: str = "geometry_rayleigh",
+                             mass_floor: float = 1e-12, lr: float | None = None, clip: float = 8.0) -> dict:
+    """The 2D Dirichlet streamfunction Rayleigh peel step (HB2P:3383-3770
+    ``_streamfunction_dirichlet_rayleigh_peel_step``, periodic trial space)
+    on the shared trunk.
+
+    Jet1 fields of the streamfunction lift (NetConfig(stiffness="stream")):
+    psi rows centered over the points (the periodic gauge, HB2P:3475-3479)
+    and the gradient rows (psi_x, psi_y); mass M = w sum psi_c^2, stiffness
+    S = w sum |grad psi|^2 as Grams of those rows.  The two-pass modified
+    Gram-Schmidt peel of the value columns in the w-weighted L2 metric, every
+    pass's coefficients applied to the gradient rows too (HB2P:3496-3560), is
+    the sorted Cholesky coefficient peel of (M, S) (coefficient_peel; its
+    equality with the literal peel is pinned by the oracle), with the
+    candidates sorted by raw S/M ("geometry_rayleigh") or in network order
+    ("fixed_order").  Gates, on the device (a failed one raises
+    FloatingPointError before any update, HB2P:3650-3684): raw and residual
+    masses above mass_floor and every quotient finite.  The gradient is that
+    of the sum of the Rayleigh quotients with the earlier (detached) modes
+    held constant, through the centering (HB2P:3579-3583); with lr given,
+    one gated Adam update follows.  Returns the reference's diagnostics
+    (HB2P:3719-3770): loss = mean Rayleigh quotient, mode_energies (causal
+    order), mode_energies_by_network, causal_order (0- and 1-based),
+    sorting_rayleigh_by_network, mean_projection_ratio, orth_error,
+    peeled_rank, plus grads (training) and the Grams."""
+    if net.cfg.dim != 2 or net.cfg.stiffness != "stream":
+        raise ValueError("the streamfunction Rayleigh peel needs a 2D net with NetConfig(stiffness='stream')")
+    if causal_sorting not in ("fixed_order", "geometry_rayleigh"):
+        raise ValueError(f"unsupported causal_sorting {causal_sorting!r}")
+    if points.shape[0] == 0:
+        raise ValueError("Dirichlet streamfunction Rayleigh received no points")
+    opts = dict(ortho_tol=math.inf, rel_min=0.0, mass_floor=mass_floor,
+                order="rayleigh" if causal_sorting == "geometry_rayleigh" else "fixed", deferred=True)
+    res = compute(net, points, weight, prec=prec, gram_mode="f64" if prec == "f64" else "bf16x3",
+                  training=training, objective="peel", peel_options=opts)
+    pk = res["peel"]
+    K = net.cfg.modes
+    if training and lr is not None:
+        net.adam(res["grads"], lr, clip=clip, gate=pk["gate"], on_gate=_raise_stream_gate)
+    else:
+        _raise_stream_gate(int(pk["gate"].item()))
+    ray = pk["losses"]
+    by_network = torch.empty_like(ray)
+    by_network[pk["order"]] = ray
+    # peeled mass norm_r^2 = relative_r * M_rr (sorted); projection ratio
+    # |coefficients|^2 / peeled mass = 1 / relative - 1 (HB2P:3546-3550)
+    proj = 1.0 / pk["relative_mass"] - 1.0
+    proj[0] = 0.0
+    out = dict(loss=float(ray.mean()), base_loss=float(ray.mean()), streamfunction_rayleigh=float(ray.mean()),
+               mean_energy=float(ray.mean()), max_energy=float(ray.max()), min_energy=float(ray.min()),
+               mean_projection_ratio=float(proj.mean()), orth_error=float(pk["gram_max"]), peeled_rank=K,
+               requested_modes=K, mode_energies=ray, mode_energies_by_network=by_network,
+               causal_sorting=causal_sorting, causal_order=pk["order"], causal_order_1based=pk["order"] + 1,
+               sorting_rayleigh_by_network=pk["raw"], spatial_jet_order_used=1, M=res["M"], S=res["S"], peel=pk)
+    if training:
+        out["grads"] = res["grads"]
+    return out
+
+
+def _raise_stream_gate(value: int):
+    if value == PEEL_GATE:
+        raise FloatingPointError("streamfunction candidate mass/finiteness gate failed before Adam; no update")
+    _raise_gate(value)
+
+
 def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
                  gram_mode: str = "bf16x3", chunk: int = 131072, solver=None, training: bool = True,
                  timings: dict | None = None):

@@ -345,6 +345,78 @@ def velocity_rayleigh_peel(grad: np.ndarray, hess: np.ndarray, w: float, order: 
                 raw_mass=raw_mass, raw_stiffness=raw_stiff)
 
 
+def lift2d_stream(out: Jet):
+    """psi jets [n, K] -> (mass rows [n, K] = psi, stiffness rows [n, 2, K] =
+    (psi_x, psi_y)): M = integral psi^2 and K = integral |grad psi|^2 of the
+    Dirichlet streamfunction peel (HB2P:3394-3397, the raw value and gradient
+    columns of :3461-3474) as Grams of these rows.  Raw: the peel removes
+    the mean."""
+    return out.value.copy(), np.stack((out.grad[..., 0], out.grad[..., 1]), 1)
+
+
+def lift2d_stream_reverse(gv: np.ndarray, gg: np.ndarray) -> Jet:
+    n, K = gv.shape
+    return Jet(gv.copy(), np.stack((gg[:, 0], gg[:, 1]), -1), np.zeros((n, K, 3)))
+
+
+def streamfunction_rayleigh_peel(value: np.ndarray, grad: np.ndarray, w: float, order: str = "rayleigh",
+                                 passes: int = 2, center: bool = True):
+    """The 2D Dirichlet streamfunction Rayleigh peel on Jet1 fields, mode by
+    mode (HB2P:3455-3583): raw values [n, K] and gradient rows [2n, K]
+    (psi_x, psi_y per point); with `center` (the periodic gauge) each value
+    column loses its mean over the points first (:3475-3479); modes in the
+    stable order of raw K/M ("rayleigh", the geometry order) or in network
+    order ("fixed"); each candidate's value column orthogonalised `passes`
+    times against the peeled (normalised) modes in the w-weighted L2 metric,
+    its gradient column with the same coefficients (:3508-3516); Rayleigh
+    quotient K/M of the residual; the cotangents of each mode's quotient with
+    the earlier modes held constant, reversed through the projections
+    (:3525-3542), then through the centering (:3579-3583).  Returns
+    dict(rayleigh, mass, stiffness, projection, orth (causal order), order,
+    G_value [n, K], G_grad [2n, K] (network order), raw_mass, raw_stiffness)."""
+    n, K = value.shape
+    v = value - value.mean(0, keepdims=True) if center else value.copy()
+    raw_mass = w * np.sum(v * v, 0)
+    raw_stiff = w * np.sum(grad * grad, 0)
+    causal = np.argsort(raw_stiff / raw_mass, kind="stable") if order == "rayleigh" else np.arange(K)
+    Qv = np.empty_like(v)
+    Qg = np.empty_like(grad)
+    Gv = np.zeros_like(v)
+    Gg = np.zeros_like(grad)
+    ray, mass, stiff, proj, orth = (np.empty(K) for _ in range(5))
+    for r, net in enumerate(causal):
+        rv = v[:, net].copy()
+        rg = grad[:, net].copy()
+        coeff = np.zeros(r)
+        pv, pg = Qv[:, :r], Qg[:, :r]
+        for _ in range(passes if r else 0):
+            inc = w * (pv.T @ rv)
+            coeff += inc
+            rv -= pv @ inc
+            rg -= pg @ inc
+        m = w * float(rv @ rv)
+        s = w * float(rg @ rg)
+        ray[r], mass[r], stiff[r] = s / m, m, s
+        av = (-2.0 * s * w / (m * m)) * rv
+        ag = (2.0 * w / m) * rg
+        for _ in range(passes if r else 0):
+            ca = -(pv.T @ av + pg.T @ ag)
+            av = av + w * (pv @ ca)
+        Gv[:, net] = av
+        Gg[:, net] = ag
+        qv, qg = rv / math.sqrt(m), rg / math.sqrt(m)
+        Qv[:, r], Qg[:, r] = qv, qg
+        o = abs(w * float(qv @ qv) - 1.0)
+        if r:
+            o = max(o, float(np.max(np.abs(w * (pv.T @ qv)))))
+        orth[r] = o
+        proj[r] = float(coeff @ coeff) / m if r else 0.0
+    if center:
+        Gv -= Gv.mean(0, keepdims=True)
+    return dict(rayleigh=ray, mass=mass, stiffness=stiff, projection=proj, orth=orth, order=causal, G_value=Gv,
+                G_grad=Gg, raw_mass=raw_mass, raw_stiffness=raw_stiff)
+
+
 def _hid(a: int, b: int) -> int:
     return pairs(3).index(tuple(sorted((a, b))))
 

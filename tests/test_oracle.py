@@ -528,6 +528,101 @@ def test_velocity_peel_fourier_closed_form():
     np.testing.assert_allclose(res["relative_mass"], 1.0, rtol=1e-12)
 
 
+def _stream_fields(seed=11, n=300, K=7):
+    rng = np.random.default_rng(seed)
+    net = _net(2, 2, [16, 16], K, seed=seed)
+    x = rng.uniform(0, 2 * np.pi, (n, 2))
+    value, g = o.lift2d_stream(_eval(net, x, 2, 0.0, 2 * np.pi))
+    return value, o.field_rows(g)
+
+
+def test_stream_lift_rows():
+    """Mass rows are psi itself, stiffness rows (psi_x, psi_y) point-major;
+    the lift's transpose is its adjoint."""
+    rng = np.random.default_rng(4)
+    net = _net(2, 2, [12], 5, seed=4)
+    out = _eval(net, rng.uniform(0, 2 * np.pi, (40, 2)), 2, 0.0, 2 * np.pi)
+    v, g = o.lift2d_stream(out)
+    np.testing.assert_array_equal(v, out.value)
+    np.testing.assert_array_equal(o.field_rows(g)[0::2], out.grad[..., 0])
+    np.testing.assert_array_equal(o.field_rows(g)[1::2], out.grad[..., 1])
+    gv, gg = rng.normal(size=v.shape), rng.normal(size=g.shape)
+    back = o.lift2d_stream_reverse(gv, gg)
+    lhs = np.sum(gv * v) + np.sum(gg * g)
+    rhs = np.sum(back.value * out.value) + np.sum(back.grad * out.grad) + np.sum(back.hess * out.hess)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_stream_peel_equals_cholesky_coefficient_peel():
+    """The literal two-pass streamfunction peel equals the sorted Cholesky
+    peel of the centered Grams (M = w Vc^T Vc, S = w G^T G): Rayleigh
+    quotients, causal order, masses and the detached-prefix cotangents
+    w Vc Cu (already mean-free) and w G Cd."""
+    value, grad = _stream_fields()
+    n = value.shape[0]
+    w = 1.0 / n
+    lit = o.streamfunction_rayleigh_peel(value, grad, w)
+    vc = value - value.mean(0)
+    M = w * vc.T @ vc
+    S = w * grad.T @ grad
+    cp = o.coefficient_peel(M, S, ortho_tol=1e-3, rel_min=0.0)
+    np.testing.assert_array_equal(lit["order"], cp["order"])
+    np.testing.assert_allclose(lit["rayleigh"], cp["losses"], rtol=1e-10)
+    norm2 = 1.0 / np.diag(cp["T"][cp["order"]]) ** 2
+    np.testing.assert_allclose(lit["mass"], norm2, rtol=1e-10)
+    np.testing.assert_allclose(lit["projection"], np.diag(M)[cp["order"]] / norm2 - 1.0, rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(lit["G_value"], w * vc @ cp["Cu"], rtol=1e-8,
+                               atol=1e-12 * np.abs(lit["G_value"]).max())
+    np.testing.assert_allclose(lit["G_grad"], w * grad @ cp["Cd"], rtol=1e-8, atol=1e-12 * np.abs(lit["G_grad"]).max())
+    assert lit["orth"].max() < 1e-8
+
+
+def test_stream_peel_cotangents_match_fd_with_detached_prefix():
+    """Each mode's cotangent (through the centering) is the central
+    difference of its own Rayleigh quotient in the raw, uncentered columns,
+    with the earlier peeled modes held (fixed order)."""
+    value, grad = _stream_fields(seed=12, K=5)
+    n = value.shape[0]
+    w = 1.0 / n
+    base = o.streamfunction_rayleigh_peel(value, grad, w, order="fixed")
+    rng = np.random.default_rng(3)
+    for r in (0, 2, 4):
+        dv = rng.normal(size=n) + 0.7          # a shift too: the centering removes it
+        dg = rng.normal(size=2 * n)
+        h = 1e-6
+        vals = []
+        for sgn in (1.0, -1.0):
+            v2, g2 = value.copy(), grad.copy()
+            v2[:, r] += sgn * h * dv
+            g2[:, r] += sgn * h * dg
+            vals.append(o.streamfunction_rayleigh_peel(v2, g2, w, order="fixed")["rayleigh"][r])
+        fd = (vals[0] - vals[1]) / (2 * h)
+        exact = base["G_value"][:, r] @ dv + base["G_grad"][:, r] @ dg
+        assert abs(fd - exact) <= 1e-6 * max(abs(exact), 1e-8), (r, fd, exact)
+
+
+def test_stream_peel_fourier_closed_form():
+    """Fourier stream functions plus constants on the periodic grid: the
+    centering removes the constants, the peel leaves the modes as they are
+    and the Rayleigh quotients are the sorted |k|^2."""
+    x = o.periodic_grid(32)
+    ks = [(2, 0), (0, 1), (1, 1), (1, 2), (3, 0), (2, 2)]
+    n = len(x)
+    value = np.zeros((n, len(ks)))
+    grad = np.zeros((2 * n, len(ks)))
+    for j, (kx, ky) in enumerate(ks):
+        ph = kx * x[:, 0] + ky * x[:, 1]
+        value[:, j] = np.sin(ph) + 0.5 * (j + 1)
+        grad[0::2, j], grad[1::2, j] = kx * np.cos(ph), ky * np.cos(ph)
+    w = (2 * np.pi / 32) ** 2
+    res = o.streamfunction_rayleigh_peel(value, grad, w)
+    k2 = np.array([kx * kx + ky * ky for kx, ky in ks], dtype=float)
+    np.testing.assert_allclose(res["rayleigh"], np.sort(k2), rtol=1e-12)
+    np.testing.assert_array_equal(res["order"], np.argsort(k2, kind="stable"))
+    np.testing.assert_allclose(res["projection"], 0.0, atol=1e-20)
+    np.testing.assert_allclose(res["mass"], 2 * np.pi ** 2, rtol=1e-12)
+
+
 # --------------------------------------------------------------------------
 # envelope product (VJ2:47-55)
 # --------------------------------------------------------------------------
