@@ -1,7 +1,7 @@
 // Output layer + lift of a block of per-mode networks on split operands
 // (the reference's K independent SIRENs, HB2P:1590-1880 / NCC:111-185; the
-// mode's output  A_m = Y_m W_m^T  and its curl / velocity lift, NCC:97-99,
-// VJ2:93-101, ELC:35-38, HB2P:3022).
+// mode's output  A_m = Y_m W_m^T  and its curl / velocity / streamfunction
+// lift, NCC:97-99, VJ2:93-101, ELC:35-38, HB2P:3022, HB2P:3461-3474).
 //
 // In a block of B modes the output weights are block-diagonal: mode m's
 // ncomp (1 in 2D, 3 in 3D) output rows read only its own h hidden
@@ -39,7 +39,8 @@ struct ModeFieldsArgs {
   int64_t ldf;
 };
 
-// LIFT: NEB_LIFT_2D_VORTICITY (2), NEB_LIFT_3D_CURL (3), NEB_LIFT_2D_HESSIAN (4)
+// LIFT: NEB_LIFT_2D_VORTICITY (2), NEB_LIFT_3D_CURL (3), NEB_LIFT_2D_HESSIAN (4),
+// NEB_LIFT_2D_STREAM (5)
 template <int LIFT> struct ModeLift;
 template <> struct ModeLift<2> {
   static constexpr int D = 2, NC = 1, CM = 2, CS = 1;
@@ -58,6 +59,14 @@ template <> struct ModeLift<4> {
     s[0] = a[3][0];                       // psi_xx, sqrt2 psi_xy, psi_yy
     s[1] = SQRT2F * a[4][0];
     s[2] = a[5][0];
+  }
+};
+template <> struct ModeLift<5> {
+  static constexpr int D = 2, NC = 1, CM = 1, CS = 2;
+  __device__ static inline void apply(const float (*a)[3], float* m, float* s) {
+    m[0] = a[0][0];                       // psi (no bias: the peel's centering removes it)
+    s[0] = a[1][0];                       // psi_x, psi_y
+    s[1] = a[2][0];
   }
 };
 template <> struct ModeLift<3> {

@@ -1297,7 +1297,8 @@ int neb_jet_layer_forward_x3_blocks(int dim, int act, int64_t n, int64_t width, 
 int neb_mode_output_fields_x3(int lift, int64_t n, int64_t h, int64_t B, const void* y_hi, const void* y_lo,
                               int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
                               void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, void* stream) {
-  if (lift != NEB_LIFT_2D_VORTICITY && lift != NEB_LIFT_3D_CURL && lift != NEB_LIFT_2D_HESSIAN)
+  if (lift != NEB_LIFT_2D_VORTICITY && lift != NEB_LIFT_3D_CURL && lift != NEB_LIFT_2D_HESSIAN &&
+      lift != NEB_LIFT_2D_STREAM)
     return fail(NEB_ERR_ARG, "neb_mode_output_fields_x3: unknown lift %d", lift);
   if (n < 0 || h < 32 || h % 32 || B < 8 || B % 8 || ldy < B * h || ldw < B * h || ldf < B)
     return fail(NEB_ERR_SHAPE, "neb_mode_output_fields_x3: bad sizes (h a multiple of 32, B a multiple of 8)");
@@ -1321,6 +1322,7 @@ int neb_mode_output_fields_x3(int lift, int64_t n, int64_t h, int64_t B, const v
   const dim3 g((unsigned)grid);
   if (lift == NEB_LIFT_3D_CURL) hipLaunchKernelGGL(mode_fields_kernel<3>, g, dim3(256), 0, st, a);
   else if (lift == NEB_LIFT_2D_HESSIAN) hipLaunchKernelGGL(mode_fields_kernel<4>, g, dim3(256), 0, st, a);
+  else if (lift == NEB_LIFT_2D_STREAM) hipLaunchKernelGGL(mode_fields_kernel<5>, g, dim3(256), 0, st, a);
   else hipLaunchKernelGGL(mode_fields_kernel<2>, g, dim3(256), 0, st, a);
   return hip_status("neb_mode_output_fields_x3");
 }

@@ -284,8 +284,11 @@ class ModeNet:
             self._block_fields_into(tape, m0, B, prec, planes)
             del tape
         if split:
-            return nw.Fields(planes[0], planes[2], K, 0, planes[1], planes[3])
-        return nw.Fields(planes[0], planes[1], K, 0)
+            f = nw.Fields(planes[0], planes[2], K, 0, planes[1], planes[3])
+        else:
+            f = nw.Fields(planes[0], planes[1], K, 0)
+        # the streamfunction lift's periodic gauge over all modes' psi rows
+        return nw._gauge(cfg.block_net_config(8), f)
 
     def output_jets(self, x: torch.Tensor, prec: str = "f32"):
         """Raw output jets of every mode (value [n, K*ncomp], grad [.., d],
@@ -419,6 +422,13 @@ class ModeNet:
         col = lambda t: t[:, m0:m0 + B]  # noqa: E731
         U, Dg = cot.mass, cot.stiff
         rv = nw._rows_view
+        if cfg.dim == 2 and cfg.stiffness == "stream":
+            # psi rows against the value channel (the cotangent columns are
+            # mean-free, so the uncentered channel gives the centered product),
+            # gradient rows against the gradient channels
+            P = nw.cross([(col(U), ch(0), n, 1.0), (col(rv(Dg, 0, 2)), ch(1), n, 1.0),
+                          (col(rv(Dg, 1, 2)), ch(2), n, 1.0)], B, width, mode)
+            return [P]
         if cfg.dim == 2:
             u, v = rv(U, 0, 2), rv(U, 1, 2)
             P = nw.cross([(col(u), ch(2), n, 1.0), (col(v), ch(1), n, -1.0)], B, width, mode)
@@ -449,7 +459,9 @@ class ModeNet:
         """Fields -> FP64 Grams -> objective -> recompute + reverse
         (VCORE:79-105 compute on the per-mode networks).  objective="peel":
         the sorted coefficient peel (VCORE:48-69; 2D with
-        stiffness="hessian" and the velocity options, HB2P:2935-3380);
+        stiffness="hessian" and the velocity options, HB2P:2935-3380, or
+        stiffness="stream" with peel_options rel_min=0, ortho_tol=inf,
+        mass_floor: the Dirichlet streamfunction peel, HB2P:3383-3770);
         "trace": tr((M + ridge I)^-1 S) + lam ||M - I||^2."""
         gram_mode = "f64" if prec == "f64" else ("f32" if prec == "f32" else "bf16x3")
         fields = self.fields(x, prec, split=prec in ("bf16", "bf16x3"))

@@ -871,6 +871,11 @@ def mode_fields(nets: list, x0: Jet, dim: int, stiffness: str = "vorticity"):
     for net in nets:
         out, tape = forward(net, x0, record=True)
         tapes.append(tape)
+        if dim == 2 and stiffness == "stream":
+            u, d = lift2d_stream(out)       # raw psi (the peel centers it)
+            Us.append(u)
+            Ds.append(field_rows(d))
+            continue
         if dim == 2:
             u, d = lift2d_hessian(out) if stiffness == "hessian" else lift2d(out)
         else:

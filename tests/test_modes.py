@@ -1,7 +1,7 @@
 """GPU parity of the per-mode SIREN networks (neb_amd.modes: the
 reference's own architecture, HB2P:1590-1880 / NCC:111-185) against the
 oracle's per-mode networks, through the C ABI: fields of every mode in 2D
-(vorticity and Hessian stiffness) and 3D, independence of the mode-block
+(vorticity, Hessian and streamfunction lifts) and 3D, independence of the mode-block
 size, the reverse pass, the coefficient-peel step, the per-mode clipped
 Adam and both checkpoint layouts."""
 import math
@@ -151,6 +151,45 @@ def test_mode_peel_step_matches_oracle():
                    peel_options=dict(ortho_tol=1e-3, rel_min=1e-4, mass_floor=1e-12))
     assert np.array_equal(res["peel"]["order"].cpu().numpy(), pk["order"])
     r = rel(res["losses"].cpu(), pk["losses"]); assert r < 1e-9, r
+    GW, _ = net.split(res["grads"])
+    for j, (a, b) in enumerate(zip(GW, gW)):
+        r = rel(a.cpu(), b); assert r < 1e-8, (j, r)
+
+
+@pytest.mark.parametrize("prec,hidden,tol", [("f64", 16, 1e-10), ("bf16x3", 32, 2e-5)])
+def test_mode_stream_fields_centered(prec, hidden, tol):
+    """Streamfunction lift of per-mode networks (bf16x3 with hidden 32 runs
+    neb_mode_output_fields_x3's per-mode contraction): psi rows centered over
+    the points, gradient rows, against the oracle."""
+    cfg, net = make(2, K=24, stiffness="stream", hidden=hidden, master=torch.float64 if prec == "f64" else torch.float32)
+    x = points(2, 301, 5)
+    U, D, _ = o.mode_fields(oracle_of(net), o.feature_jets(x, 2, cfg.lo, cfg.hi), 2, "stream")
+    f = net.fields(torch.tensor(x), prec)
+    r = rel(field_np(f, "mass"), U - U.mean(0)); assert r < tol, r
+    r = rel(field_np(f, "stiff"), D); assert r < tol, r
+
+
+def test_mode_stream_peel_step_matches_oracle():
+    """The Dirichlet streamfunction peel on per-mode networks: quotients,
+    order and gradient against the oracle's literal peel and the per-mode
+    reverse of its cotangents (through the centering)."""
+    K = 16
+    cfg, net = make(2, K=K, stiffness="stream", hidden=16, layers=2)
+    x = points(2, 400, 6)
+    n = len(x)
+    w = 4 * math.pi ** 2 / n
+    nets = oracle_of(net)
+    U, D, tapes = o.mode_fields(nets, o.feature_jets(x, 2, cfg.lo, cfg.hi), 2, "stream")
+    pk = o.streamfunction_rayleigh_peel(U, D, w)
+    gWs = []
+    for k, (onet, tape) in enumerate(zip(nets, tapes)):
+        adj = o.lift2d_stream_reverse(pk["G_value"][:, k:k + 1], pk["G_grad"][:, k].reshape(n, 2, 1))
+        gWs.append(o.backward(onet, tape, adj)[0])
+    gW = [np.stack([g[j] for g in gWs]) for j in range(len(gWs[0]))]
+    res = net.step(torch.tensor(x), w, prec="f64", objective="peel",
+                   peel_options=dict(ortho_tol=math.inf, rel_min=0.0, mass_floor=1e-12))
+    assert np.array_equal(res["peel"]["order"].cpu().numpy(), pk["order"])
+    r = rel(res["losses"].cpu(), pk["rayleigh"]); assert r < 1e-9, r
     GW, _ = net.split(res["grads"])
     for j, (a, b) in enumerate(zip(GW, gW)):
         r = rel(a.cpu(), b); assert r < 1e-8, (j, r)
