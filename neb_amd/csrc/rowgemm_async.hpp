@@ -72,6 +72,14 @@ template <> struct RowAsyncShape<PlanOutFields2Split> {
 };
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenBwd<D, ACT>> : WideHiddenShape<D> {};
 
+// First term of a plan that reads input plane c (1 << 20: none).
+template <class Plan>
+__host__ __device__ constexpr int plan_first_use(int c) {
+  for (int t = 0; t < Plan::NT; ++t)
+    if (Plan::tc[t] == c) return t;
+  return 1 << 20;
+}
+
 template <class Plan>
 struct RowAsyncPol {
   using SH = RowAsyncShape<Plan>;
@@ -323,19 +331,41 @@ __global__ __launch_bounds__(RowAsyncPol<Plan>::NTH) __attribute__((amdgpu_waves
 #pragma unroll
       for (int s = 0; s < NW; ++s) wf[s][i] = *reinterpret_cast<lds_frag*>(ws + s * WT + off);
     }
+    // The terms run in chunks of TG (>= 12 MFMAs each); an input plane is
+    // read just after the first MFMA of the chunk before the one that first
+    // uses it, so its LDS latency runs under that chunk's MFMAs.  (Read all
+    // at once, the compiler reused a few registers for the planes in turn
+    // and waited lgkmcnt(0) every two MFMAs: the wait also covers every
+    // read issued before it, so a read cannot overlap the MFMAs before its
+    // own wait unless the MFMAs come between.)
+    constexpr int TG = (12 + FB * PB - 1) / (FB * PB), NCH = (NT + TG - 1) / TG;
+    auto read_chunk = [&](int k) __attribute__((always_inline)) {
 #pragma unroll
-    for (int j = 0; j < PB; ++j)
+      for (int c = 0; c < CI; ++c)
+        if (plan_first_use<Plan>(c) / TG == k)
 #pragma unroll
-      for (int c = 0; c < CI; ++c) xf[c][j] = *reinterpret_cast<lds_frag*>(cur + xoff[c][j]);
+          for (int j = 0; j < PB; ++j) xf[c][j] = *reinterpret_cast<lds_frag*>(cur + xoff[c][j]);
+    };
+    auto term = [&](int t, int i, int j) __attribute__((always_inline)) {
+      const frag_t wv = Plan::tneg[t] ? fneg(wf[Plan::tw[t]][i]) : wf[Plan::tw[t]][i];
+      acc[Plan::ta[t]][i][j] = PR::mfma(wv, xf[Plan::tc[t]][j], acc[Plan::ta[t]][i][j]);
+    };
+    read_chunk(0);
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int k = 0; k < NCH; ++k) {
+      const int t0 = k * TG, t1 = t0 + TG < NT ? t0 + TG : NT;
+      term(t0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (k + 1 < NCH) read_chunk(k + 1);
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-      for (int i = 0; i < FB; ++i) {
-        const frag_t wv = Plan::tneg[t] ? fneg(wf[Plan::tw[t]][i]) : wf[Plan::tw[t]][i];
+      for (int t = t0; t < t1; ++t)
 #pragma unroll
-        for (int j = 0; j < PB; ++j)
-          acc[Plan::ta[t]][i][j] = PR::mfma(wv, xf[Plan::tc[t]][j], acc[Plan::ta[t]][i][j]);
-      }
+        for (int i = 0; i < FB; ++i)
+#pragma unroll
+          for (int j = 0; j < PB; ++j)
+            if (t != t0 || i != 0 || j != 0) term(t, i, j);
+    }
     if (++ck == nk) {
       const int64_t p0 = ctp * TP, f0 = ctf * TN;
       auto group_v = [&](int j, int q, M (*v)[GE]) __attribute__((always_inline)) {
