@@ -859,7 +859,10 @@ template <int D, int ACT> struct PlanHiddenBwd {
     reverse_epilogue<P, D, ACT, GE, M>(a, p, f, nv, v);
   }
   // The same with the tape group already loaded (u = Z at p, f..f+GE).
-  static constexpr int TAPE_CH = C;
+  // Only tanh holds the tile's tape in registers through the last k-step:
+  // the sin chain's epilogue needs more registers than the 48 that costs
+  // (with it the 8-wave launch spilled 119 registers in 2D, 203 in 3D)
+  static constexpr int TAPE_CH = ACT == ACT_TANH ? C : 0;
   template <int P, int GE, typename M>
   __device__ static inline void epilogue_tape(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE],
                                               M (*u)[GE]) {

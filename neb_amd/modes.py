@@ -81,6 +81,17 @@ class ModeConfig:
                             stiffness=self.stiffness)
 
 
+# Bytes of block tapes a training step keeps between its forward and its
+# reverse (HBM is 288 GB on MI355X: the 2D K = 1024 configuration's tapes
+# are 19 GB; the 3D 5 x 128 one's 240 GB are recomputed instead).
+TAPE_BUDGET = 48 << 30
+
+
+def _tape_bytes(tape: nw.Tape) -> int:
+    ts = [*tape.inputs, *tape.pre, *(tape.inputs_lo or []), *(tape.pre_lo or [])]
+    return sum(t.numel() * t.element_size() for t in ts if t is not None)
+
+
 def siren_init(cfg: ModeConfig, seed: int):
     """Mode-stacked SIREN weights (NCC:113-124): mode i draws from its own
     PCG64(seed + 7919 (i + 1)) stream, layer by layer W then b, uniform in
@@ -264,10 +275,14 @@ class ModeNet:
         K, B = self.cfg.modes, self.cfg.block
         return [(m0, min(B, K - m0)) for m0 in range(0, K, B)]
 
-    def fields(self, x: torch.Tensor, prec: str = "bf16x3", split: bool | None = None) -> nw.Fields:
+    def fields(self, x: torch.Tensor, prec: str = "bf16x3", split: bool | None = None,
+               keep_tapes: bool = False):
         """Lifted fields of all K modes, block by block into one
         [n*comp, K] matrix (split hi / lo planes for the split-precision
-        forward, or for the bf16 forward with split=True)."""
+        forward, or for the bf16 forward with split=True).  keep_tapes=True
+        returns (fields, tapes): every block's jet tape, kept for
+        backward_fields when they fit TAPE_BUDGET bytes in all (else tapes is
+        None and the reverse recomputes them block by block, VCORE:92-102)."""
         cfg = self.cfg
         K = cfg.modes
         split = (prec == "bf16x3") if split is None else (split or prec == "bf16x3")
@@ -279,16 +294,24 @@ class ModeNet:
         else:
             ft = torch.float64 if prec == "f64" else torch.float32
             planes = [torch.empty((r * n, nw._ld(K)), dtype=ft, device=self.device) for r in (cm, cs)]
-        for m0, B in self._blocks():
+        tapes = [] if keep_tapes else None
+        blocks = self._blocks()
+        for bi, (m0, B) in enumerate(blocks):
             tape = self.block_tape(feats, m0, B, prec)
             self._block_fields_into(tape, m0, B, prec, planes)
+            if tapes is not None:
+                if bi == 0 and _tape_bytes(tape) * len(blocks) > TAPE_BUDGET:
+                    tapes = None
+                else:
+                    tapes.append(tape)
             del tape
         if split:
             f = nw.Fields(planes[0], planes[2], K, 0, planes[1], planes[3])
         else:
             f = nw.Fields(planes[0], planes[1], K, 0)
         # the streamfunction lift's periodic gauge over all modes' psi rows
-        return nw._gauge(cfg.block_net_config(8), f)
+        f = nw._gauge(cfg.block_net_config(8), f)
+        return (f, tapes) if keep_tapes else f
 
     def output_jets(self, x: torch.Tensor, prec: str = "f32"):
         """Raw output jets of every mode (value [n, K*ncomp], grad [.., d],
@@ -322,7 +345,7 @@ class ModeNet:
 
     # -- reverse -----------------------------------------------------------
     def backward_fields(self, x: torch.Tensor, fields: nw.Fields, dM: torch.Tensor, dS: torch.Tensor, weight: float,
-                        prec: str = "bf16x3") -> torch.Tensor:
+                        prec: str = "bf16x3", tapes: list | None = None) -> torch.Tensor:
         """Parameter gradient of L(M, S), M = w U^T U, S = w D^T D, for FP64
         dL/dM, dL/dS applied unsymmetrised (field cotangents 2w U dM^T,
         2w D dS^T: the coefficient peel's detached-prefix pair as given).
@@ -333,7 +356,9 @@ class ModeNet:
         with the block's B columns as the contraction), the output-weight
         products are the cotangent columns against the block's last hidden
         jets, and the hidden reverse keeps the diagonal blocks of each
-        weight gradient (HB2P:2087-2139 backward_gradients_block_jet2)."""
+        weight gradient (HB2P:2087-2139 backward_gradients_block_jet2).
+        tapes: the blocks' tapes kept by fields(keep_tapes=True); without
+        them each block's tape is recomputed."""
         cfg = self.cfg
         K = cfg.modes
         d, C = cfg.dim, nw.channels(cfg.dim)
@@ -354,8 +379,8 @@ class ModeNet:
         gW, gb = self.split(grads)
         s = nw._stream()
         h, L, nc = cfg.hidden, cfg.hidden_layers, cfg.ncomp
-        for m0, B in self._blocks():
-            tape = self.block_tape(feats, m0, B, prec)
+        for bi, (m0, B) in enumerate(self._blocks()):
+            tape = tapes[bi] if tapes is not None else self.block_tape(feats, m0, B, prec)
             p = tape.prec
             st = nw._STORE[p]
             mode = nw._CROSS[p]
@@ -464,7 +489,9 @@ class ModeNet:
         mass_floor: the Dirichlet streamfunction peel, HB2P:3383-3770);
         "trace": tr((M + ridge I)^-1 S) + lam ||M - I||^2."""
         gram_mode = "f64" if prec == "f64" else ("f32" if prec == "f32" else "bf16x3")
-        fields = self.fields(x, prec, split=prec in ("bf16", "bf16x3"))
+        split = prec in ("bf16", "bf16x3")
+        fields, tapes = self.fields(x, prec, split=split, keep_tapes=True) if training \
+            else (self.fields(x, prec, split=split), None)
         M, S = nw.field_grams(fields, weight, gram_mode)
         K = self.cfg.modes
         out = dict(M=M, S=S, fields=fields)
@@ -479,7 +506,8 @@ class ModeNet:
             raise ValueError(f"unknown objective {objective!r}")
         out.update(dM=dM, dS=dS)
         if training:
-            out["grads"] = self.backward_fields(x, fields, dM, dS, weight, prec)
+            out["grads"] = self.backward_fields(x, fields, dM, dS, weight, prec, tapes=tapes)
+            out["tapes_kept"] = tapes is not None
         return out
 
     # -- optimizer ---------------------------------------------------------
