@@ -577,7 +577,7 @@ def config_e(args, quick=False):
                                "potential, curl lift, mass+grad-u Gram, 131,072-pt chunks, spectral loss + "
                                "reverse + Adam", "points": N, "modes": K, "chunk": CHUNK3D, "ridge": ridge,
                    "domain_volume_estimate": vol},
-        "precision": head, "stages_ms": stages, "gram_one_chunk": gram_stats,
+        "precision": head, "stages_ms": stages, "tapes_kept": bool(res["tapes_kept"]), "tape_gib": res["tape_bytes"] / 2**30, "gram_one_chunk": gram_stats,
         "fp32_step_point_modes_per_s": f32_rate,
         "vs_fp32_step": (N * K / dt) / f32_rate if f32_rate else None,
         ("bf16_step" if head == "bf16x3" else "split_precision_step"): other,

@@ -87,11 +87,6 @@ class ModeConfig:
 TAPE_BUDGET = 48 << 30
 
 
-def _tape_bytes(tape: nw.Tape) -> int:
-    ts = [*tape.inputs, *tape.pre, *(tape.inputs_lo or []), *(tape.pre_lo or [])]
-    return sum(t.numel() * t.element_size() for t in ts if t is not None)
-
-
 def siren_init(cfg: ModeConfig, seed: int):
     """Mode-stacked SIREN weights (NCC:113-124): mode i draws from its own
     PCG64(seed + 7919 (i + 1)) stream, layer by layer W then b, uniform in
@@ -300,7 +295,7 @@ class ModeNet:
             tape = self.block_tape(feats, m0, B, prec)
             self._block_fields_into(tape, m0, B, prec, planes)
             if tapes is not None:
-                if bi == 0 and _tape_bytes(tape) * len(blocks) > TAPE_BUDGET:
+                if bi == 0 and nw.tape_bytes(tape) * len(blocks) > TAPE_BUDGET:
                     tapes = None
                 else:
                     tapes.append(tape)

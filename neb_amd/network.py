@@ -363,6 +363,12 @@ class Tape:
     pre_lo: list | None = None      # (and of the pre-activations, when kept)
 
 
+def tape_bytes(tape: Tape) -> int:
+    """Device bytes held by a jet tape."""
+    ts = [*tape.inputs, *tape.pre, *(tape.inputs_lo or []), *(tape.pre_lo or [])]
+    return sum(t.numel() * t.element_size() for t in ts if t is not None)
+
+
 def trunk_forward(net: EigenfluidNet, x: torch.Tensor, prec: int) -> Tape:
     """Feature jets and all hidden layers; keeps the jet tape."""
     cfg = net.cfg

@@ -532,15 +532,34 @@ def _raise_stream_gate(value: int):
     _raise_gate(value)
 
 
+# Bytes of chunk tapes a training step may keep between its two passes, and
+# the device memory left free beside them for one chunk's fields and reverse
+# workspace: 13 split-precision chunks of 131,072 3D points (4 x 256 tanh
+# trunk, 10 jet channels) fit a 288 GB MI355X.
+CHUNK_TAPE_BUDGET = 192 << 30
+CHUNK_TAPE_HEADROOM = 32 << 30
+
+
+def _device_bytes_available(dev: torch.device) -> int:
+    """Free device memory plus what torch's caching allocator holds unused."""
+    free, _ = torch.cuda.mem_get_info(dev)
+    return int(free + torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev))
+
+
 def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
                  gram_mode: str = "bf16x3", chunk: int = 131072, solver=None, training: bool = True,
-                 timings: dict | None = None):
+                 timings: dict | None = None, tape_budget: int | None = None):
     """Spectral step over point chunks, memory-bounded for large point sets
     (the structure of MB/velocity/core.py:79-105: fields -> grams ->
     objective -> recompute + reverse).  Pass 1 evaluates each chunk's jets
-    and fields and accumulates the FP64 Grams; pass 2 recomputes each
-    chunk's jet tape (per-layer recomputation: no tape outlives its chunk)
-    and accumulates the parameter gradient.  Returns dict(loss, M, S, grads)."""
+    and fields and accumulates the FP64 Grams; pass 2 accumulates the
+    parameter gradient chunk by chunk.  Every chunk's jet tape is kept from
+    pass 1 for pass 2 when all of them fit `tape_budget` bytes (default
+    CHUNK_TAPE_BUDGET, capped at the available device memory less
+    CHUNK_TAPE_HEADROOM); otherwise pass 2 recomputes each chunk's tape
+    (the reference's per-chunk recomputation, VCORE:92-102: no tape outlives
+    its chunk).  Returns dict(loss, M, S, grads, tapes_kept, tape_bytes), the
+    last the bytes all chunks' tapes together take."""
     p = nw.PRECISIONS[prec]
     tt = {} if timings is None else timings
     K = net.cfg.modes
@@ -549,21 +568,37 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
     split = nw.split_fields_for(prec, gram_mode)
     M = torch.zeros((K, K), dtype=torch.float64, device=net.device)
     S = torch.zeros_like(M)
+    starts = list(range(0, n, chunk))
+    budget = CHUNK_TAPE_BUDGET if tape_budget is None else int(tape_budget)
+    if training and budget > 0 and net.device.type == "cuda":
+        budget = min(budget, _device_bytes_available(net.device) - CHUNK_TAPE_HEADROOM)
+    tapes = [] if training and budget > 0 else None
+    kept_bytes = 0
     with timed(tt, "pass1_forward_and_grams"):
-        for c0 in range(0, n, chunk):
+        for i, c0 in enumerate(starts):
             tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
             f = nw.output_fields(net, tape, split=split)
+            if i == 0:
+                kept_bytes = nw.tape_bytes(tape) * len(starts)
+            if tapes is not None:
+                if kept_bytes > budget:
+                    tapes = None
+                else:
+                    tapes.append(tape)
             del tape
             nw.field_grams(f, weight, gram_mode, M=M, S=S, beta=1.0)
             del f
     with timed(tt, "spectral_objective"):
         loss, dM, dS = solver(M, S)
-    out = dict(loss=loss, M=M, S=S, timings=tt)
+    out = dict(loss=loss, M=M, S=S, timings=tt, tapes_kept=tapes is not None, tape_bytes=kept_bytes)
     if training:
         grads = torch.zeros_like(net.params)
         with timed(tt, "pass2_recompute_and_reverse"):
-            for c0 in range(0, n, chunk):
-                tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
+            for i, c0 in enumerate(starts):
+                if tapes is not None:
+                    tape, tapes[i] = tapes[i], None     # released once its reverse is queued
+                else:
+                    tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
                 # the factored reverse of a split-precision tape reads no fields
                 f = None if nw.factored_reverse_ok(net, tape) else nw.output_fields(net, tape, split=split)
                 grads += nw.backward_fields(net, tape, f, dM, dS, weight, symmetric=True)

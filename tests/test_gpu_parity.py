@@ -572,10 +572,13 @@ def test_chunked_step_equals_single_pass():
     x = torch.tensor(points(3, 1000, seed=8))
     w = 1.0 / 1000
     ref = nb.compute(net, x, w, prec="f64", gram_mode="f64", gate="cholesky")
-    res = nb.chunked_step(net, x, w, prec="f64", gram_mode="f64", chunk=256)
-    r = rel(res["M"].cpu(), ref["M"].cpu()); assert r < 1e-13, r
-    r = rel(res["S"].cpu(), ref["S"].cpu()); assert r < 1e-13, r
-    r = rel(res["grads"].cpu(), ref["grads"].cpu()); assert r < 1e-11, r
+    # kept chunk tapes (the default budget) and per-chunk recomputation (budget 0)
+    for budget, kept in ((None, True), (0, False)):
+        res = nb.chunked_step(net, x, w, prec="f64", gram_mode="f64", chunk=256, tape_budget=budget)
+        assert res["tapes_kept"] is kept
+        r = rel(res["M"].cpu(), ref["M"].cpu()); assert r < 1e-13, r
+        r = rel(res["S"].cpu(), ref["S"].cpu()); assert r < 1e-13, r
+        r = rel(res["grads"].cpu(), ref["grads"].cpu()); assert r < 1e-11, r
 
 
 def test_project_streamed_against_oracle():
