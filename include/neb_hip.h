@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 11
+#define NEB_ABI_VERSION 12
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -413,6 +413,16 @@ int neb_spectral_adjoint(int64_t K, const double* C, int64_t ldc, const double* 
 size_t neb_spd_inverse_workspace_size(int64_t K);
 int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
                     size_t ws_bytes, void* stream);
+
+/* neb_spd_inverse that also writes T (K x K, row stride ldt) = U^-1, the
+ * inverse of the upper Cholesky factor of A (A = U^T U, T^T A T = I, T upper
+ * triangular with a positive diagonal, zeros below it), from the same
+ * sweeps: the factor the coefficient peel orthonormalises the sorted mass
+ * with.  Replaces cholesky + solve_triangular(L^T, I) of velocity/core.py:
+ * 53-56 (both passes).  A, X and T must not alias; *info as for
+ * neb_spd_inverse (T is then meaningless); same workspace size. */
+int neb_spd_inverse_chol(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, double* T, int64_t ldt,
+                         int* info, void* workspace, size_t ws_bytes, void* stream);
 
 /* Reduced-order right-hand side of the 2D periodic rollout on the network
  * basis (experiments/periodic/paper_rollout.py:37-41 `rhs`): with the

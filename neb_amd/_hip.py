@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
@@ -88,6 +88,7 @@ SIGNATURES = {
     "neb_spectral_adjoint": (_int, [_i64, _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
     "neb_spd_inverse_workspace_size": (_sz, [_i64]),
     "neb_spd_inverse": (_int, [_i64, _vp, _i64, _vp, _i64, _vp, _vp, _sz, _vp]),
+    "neb_spd_inverse_chol": (_int, [_i64, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _sz, _vp]),
 }
 
 _lib = None

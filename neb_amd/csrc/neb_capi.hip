@@ -1058,25 +1058,30 @@ int neb_adam_gated(int dtype, void* p, const void* g, void* m, void* v, int64_t 
 
 size_t neb_spd_inverse_workspace_size(int64_t K) {
   const int64_t Kp = (K + SPD_NB - 1) / SPD_NB * SPD_NB;
-  return (size_t)(2 * Kp * Kp + 2 * SPD_NB * SPD_NB) * sizeof(double);
+  return (size_t)(2 * Kp * Kp + 4 * SPD_NB * SPD_NB) * sizeof(double);
 }
 
-int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
-                    size_t ws_bytes, void* stream) {
-  if (K < 1 || lda < K || ldx < K) return fail(NEB_ERR_SHAPE, "neb_spd_inverse: bad sizes");
-  if (!A || !X || !info || !workspace) return fail(NEB_ERR_ARG, "neb_spd_inverse: null pointer");
-  if (A == X) return fail(NEB_ERR_ARG, "neb_spd_inverse: X must not alias A");
-  if (ws_bytes < neb_spd_inverse_workspace_size(K)) return fail(NEB_ERR_ARG, "neb_spd_inverse: workspace too small");
+// The sweeps of neb_spd_inverse; with T non-null they also write the inverse
+// upper Cholesky factor (spd_step_kernel).
+static int spd_sweeps(const char* name, int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, double* T,
+                      int64_t ldt, int* info, void* workspace, size_t ws_bytes, void* stream) {
+  if (K < 1 || lda < K || ldx < K || (T && ldt < K)) return fail(NEB_ERR_SHAPE, "%s: bad sizes", name);
+  if (!A || !X || !info || !workspace) return fail(NEB_ERR_ARG, "%s: null pointer", name);
+  if (A == X || A == T || X == T) return fail(NEB_ERR_ARG, "%s: A, X and T must not alias", name);
+  if (ws_bytes < neb_spd_inverse_workspace_size(K)) return fail(NEB_ERR_ARG, "%s: workspace too small", name);
   const int64_t nt = (K + SPD_NB - 1) / SPD_NB, Kp = nt * SPD_NB;
-  if (nt > 2048) return fail(NEB_ERR_SHAPE, "neb_spd_inverse: K too large");
+  if (nt > 2048) return fail(NEB_ERR_SHAPE, "%s: K too large", name);
   double* base = reinterpret_cast<double*>(workspace);
   double* buf[2] = {base, base + Kp * Kp};  // padded Kp x Kp ping-pong matrices
-  double* Pi[2] = {base + 2 * Kp * Kp, base + 2 * Kp * Kp + SPD_NB * SPD_NB};
+  double* tiles = base + 2 * Kp * Kp;       // 64 x 64 pivot inverses and T_kk^T, ping-pong
+  double* Pi[2] = {tiles, tiles + SPD_NB * SPD_NB};
+  double* Tk[2] = {tiles + 2 * SPD_NB * SPD_NB, tiles + 3 * SPD_NB * SPD_NB};
   hipStream_t st = S(stream);
   if (int rc = set_lds(spd_pivot0_kernel, spd_pivot0_lds_bytes())) return rc;
   if (int rc = set_lds(spd_step_kernel, spd_step_lds_bytes())) return rc;
-  hipLaunchKernelGGL(spd_pivot0_kernel, dim3(1), dim3(256), spd_pivot0_lds_bytes(), st, A, lda, K, Pi[0], info);
-  if (int rc = hip_status("neb_spd_inverse pivot")) return rc;
+  hipLaunchKernelGGL(spd_pivot0_kernel, dim3(1), dim3(256), spd_pivot0_lds_bytes(), st, A, lda, K, Pi[0], info,
+                     T ? Tk[0] : nullptr);
+  if (int rc = hip_status(name)) return rc;
   const double* src = A;
   int64_t lds = lda;
   for (int64_t k = 0; k < nt; ++k) {
@@ -1085,12 +1090,23 @@ int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t 
     const int64_t ldd = last ? ldx : Kp;
     hipLaunchKernelGGL(spd_step_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), spd_step_lds_bytes(), st,
                        src, lds, k == 0 ? 1 : 0, K, (int)k, (int)nt, Pi[k & 1], Pi[(k + 1) & 1], dst, ldd,
-                       last ? 1 : 0, info);
-    if (int rc = hip_status("neb_spd_inverse sweep")) return rc;
+                       last ? 1 : 0, info, Tk[k & 1], Tk[(k + 1) & 1], T, ldt);
+    if (int rc = hip_status(name)) return rc;
     src = dst;
     lds = ldd;
   }
   return NEB_OK;
+}
+
+int neb_spd_inverse(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, int* info, void* workspace,
+                    size_t ws_bytes, void* stream) {
+  return spd_sweeps("neb_spd_inverse", K, A, lda, X, ldx, nullptr, 0, info, workspace, ws_bytes, stream);
+}
+
+int neb_spd_inverse_chol(int64_t K, const double* A, int64_t lda, double* X, int64_t ldx, double* T, int64_t ldt,
+                         int* info, void* workspace, size_t ws_bytes, void* stream) {
+  if (!T) return fail(NEB_ERR_ARG, "neb_spd_inverse_chol: null pointer");
+  return spd_sweeps("neb_spd_inverse_chol", K, A, lda, X, ldx, T, ldt, info, workspace, ws_bytes, stream);
 }
 
 int neb_sym_shift(int64_t K, const double* A, int64_t lda, double shift, double* B, int64_t ldb, void* stream) {

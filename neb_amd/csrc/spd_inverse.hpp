@@ -20,6 +20,16 @@
 // identity in place and the K x K part untouched), so every tile is full.
 // The sweeps keep the matrix symmetric, so A_kj is read as A_jk^T: every
 // operand is a tile of block column k, stored [row][k] in LDS.
+//
+// The same sweeps also yield T = U^-1, the inverse of the upper Cholesky
+// factor (A = U^T U, T^T A T = I, T upper with a positive diagonal), which
+// the coefficient peel needs (velocity/core.py:53-56).  Before sweep k, block
+// column k above the diagonal holds A_<k^-1 A_<k,k and the diagonal block
+// holds the Schur complement S_k; with U_<k^T U_<k,k = A_<k,k and
+// U_kk^T U_kk = S_k, block column k of T is
+//   T_<k,k = -(A_<k^-1 A_<k,k) T_kk,   T_kk = U_kk^-1,
+// and T_kk is the upper factor of S_k^-1 = T_kk T_kk^T, which the pivot
+// workgroup factors (in reverse index order) right after inverting S_k.
 #pragma once
 #include "neb_common.hpp"
 
@@ -139,6 +149,65 @@ __device__ inline void spd_acc_store(double* M, const f64x4 (&v)[2][2], int R0, 
       for (int e = 0; e < 4; ++e) M[spd_row(R0, tr, lane, e) * SPD_LDP + spd_col(C0, tc, lane)] = v[tr][tc][e];
 }
 
+// Upper T with T T^T = B for the symmetric positive definite 64 x 64 B in
+// LDS (row stride SPD_LDP): B = V D V^T with V unit upper triangular, in
+// reverse index order (column j = 63 .. 0: d_j = b_jj, v_ij = b_ij / d_j for
+// i < j, b_im -= v_ij b_mj for i, m < j), then T = V D^1/2.  The square
+// roots wait until the end, off the column-to-column dependency chain.
+// Thread t holds the 4 x 4 blocks (t / 16, t % 16) of B (then V) in
+// registers; column j of the current B passes through LDS (cbuf, 2 x 64
+// doubles, alternating so that one barrier per column suffices).  Writes
+// T^T ([column][row], stride 64: the layout spd_fetch reads as the
+// right-hand MFMA operand RT) to TkT.  A B that is not positive definite
+// gives NaN or infinities.
+__device__ inline void spd_rev_chol(const double* Bl, double* cbuf, int t, double* __restrict__ TkT) {
+  const int bi = t >> 4, bm = t & 15;
+  double b[4][4], dv[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) b[r][c] = Bl[(4 * bi + r) * SPD_LDP + 4 * bm + c];
+  for (int jb = SPD_NB / 4 - 1; jb >= 0; --jb) {
+#pragma unroll
+    for (int jc = 3; jc >= 0; --jc) {
+      const int j = 4 * jb + jc;
+      double* col = cbuf + (jc & 1) * SPD_NB;
+      if (bm == jb) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) col[4 * bi + r] = b[r][jc];
+      }
+      __syncthreads();
+      const double dj = col[j];
+      const double inv = 1.0 / dj;
+      double vi[4], cm[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        vi[r] = col[4 * bi + r] * inv;
+        cm[r] = col[4 * bm + r];
+      }
+      if (bm == jb) {  // column j of B is done with: it becomes column j of V
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int i = 4 * bi + r;
+          b[r][jc] = i < j ? vi[r] : (i == j ? 1.0 : 0.0);
+        }
+        dv[jc] = dj;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (4 * bi + r < j && 4 * bm + c < j) b[r][c] -= vi[r] * cm[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double sd = sqrt(dv[c]);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) TkT[(4 * bm + c) * SPD_NB + 4 * bi + r] = b[r][c] * sd;
+  }
+}
+
 // Inversion of the 64 x 64 pivot block held in MFMA layout (v) by lazily
 // scaled sweeps over 4 x 4 pivot blocks.  Sweeping block b (columns
 // c = 4b .. c+3) with pivot block P (current values) is the rank-4 update
@@ -155,9 +224,10 @@ __device__ inline void spd_acc_store(double* M, const f64x4 (&v)[2][2], int R0, 
 // double-buffered: one barrier per 4 columns.  lds: SPD_PIVOT_LDS doubles,
 // not in use by the caller.  Writes Pi = P^-1 (64 x 64, row stride 64).
 // info: the 1-based index k0 + i + 1 of the first nonpositive / nonfinite
-// pivot (first writer wins).
+// pivot (first writer wins).  TkT (optional): T_kk^T, T_kk the upper factor
+// of Pi = T_kk T_kk^T (spd_rev_chol).
 __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int lane, int t, int64_t k0,
-                                      double* __restrict__ Pi, int* __restrict__ info) {
+                                      double* __restrict__ Pi, int* __restrict__ info, double* __restrict__ TkT) {
   double* rowbuf = lds;
   double* colbuf = lds + 2 * 4 * SPD_NB;
   double* pivs = colbuf + 2 * SPD_NB * 4;
@@ -246,7 +316,7 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
   __syncthreads();
   const int i = t >> 2, jg = (t & 3) * 16;
   const int I = i >> 2, ir = i & 3;
-  double li[4];
+  double li[4], pio[16];
 #pragma unroll
   for (int a = 0; a < 4; ++a) li[a] = pivs[I * 16 + 4 * ir + a];
   double tr_[16];
@@ -267,13 +337,24 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
       for (int cq = 0; cq < 4; ++cq) o[cq] = fma(tr_[4 * J + bq], pivs[Jg * 16 + 4 * bq + cq], o[cq]);
     *reinterpret_cast<f64x2*>(&Pi[i * SPD_NB + jg + 4 * J]) = f64x2{-o[0], -o[1]};
     *reinterpret_cast<f64x2*>(&Pi[i * SPD_NB + jg + 4 * J + 2]) = f64x2{-o[2], -o[3]};
+#pragma unroll
+    for (int cq = 0; cq < 4; ++cq) pio[4 * J + cq] = -o[cq];
   }
   if (t == 0 && bad != 0) atomicCAS(info, 0, bad);
+  if (TkT) {
+    __syncthreads();  // every thread is done reading M
+#pragma unroll
+    for (int q = 0; q < 16; ++q) M[i * SPD_LDP + jg + q] = pio[q];
+    __syncthreads();
+    spd_rev_chol(M, rowbuf, t, TkT);
+  }
 }
 
 // First pivot: Pi = (A[0:64, 0:64], identity beyond K)^-1; clears info.
+// TkT (optional): T_00^T (spd_pivot_regs).
 __global__ __launch_bounds__(256) void spd_pivot0_kernel(const double* __restrict__ A, int64_t lda, int64_t K,
-                                                         double* __restrict__ Pi, int* __restrict__ info) {
+                                                         double* __restrict__ Pi, int* __restrict__ info,
+                                                         double* __restrict__ TkT) {
   extern __shared__ __align__(16) double spd_smem[];
   const int t = threadIdx.x, lane = t & 63;
   const int w = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -290,7 +371,7 @@ __global__ __launch_bounds__(256) void spd_pivot0_kernel(const double* __restric
       for (int e = 0; e < 4; ++e)
         v[tr][tc][e] = spd_smem[spd_row(R0, tr, lane, e) * SPD_LDP + spd_col(C0, tc, lane)];
   __syncthreads();
-  spd_pivot_regs(v, spd_smem, w, lane, t, 0, Pi, info);
+  spd_pivot_regs(v, spd_smem, w, lane, t, 0, Pi, info, TkT);
 }
 
 // Sweep k on tile pair (ti, tj), ti <= tj (blockIdx.x counts the upper
@@ -301,10 +382,18 @@ __global__ __launch_bounds__(256) void spd_pivot0_kernel(const double* __restric
 // K x K.  The workgroup of tile (k+1, k+1) inverts its new value as the next
 // pivot (Pi_next) instead of storing it: sweep k+1 replaces that tile by
 // -Pi_next without reading it.
+//
+// With Tout (K x K, row stride ldt) the launch also writes block column k of
+// T = U^-1 (header comment): tile (i, k), i < k, = -(A_ik) T_kk from the
+// tile the workgroup reads anyway, tile (k, k) = T_kk, and the workgroups of
+// row k write the zero tiles (j, k), j > k; T_kk^T comes in TkT, and the
+// pivot workgroup leaves T_k+1,k+1^T in TkT_next.
 __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict__ src, int64_t lds, int bounded,
                                                        int64_t K, int k, int nt, const double* __restrict__ Pi,
                                                        double* __restrict__ Pi_next, double* __restrict__ dst,
-                                                       int64_t ldd, int last, int* __restrict__ info) {
+                                                       int64_t ldd, int last, int* __restrict__ info,
+                                                       const double* __restrict__ TkT, double* __restrict__ TkT_next,
+                                                       double* __restrict__ Tout, int64_t ldt) {
   // two LDS tiles (70 KB): a workgroup fits beside a 64 KB cross-GEMM
   // workgroup on a CU, so the sweeps share the GPU with the output-layer
   // products queued on another stream
@@ -335,14 +424,50 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           acc[tr][tc][e] = -Pi[spd_row(R0, tr, lane, e) * SPD_NB + spd_col(C0, tc, lane)];
+    if (Tout) {  // T_kk (row r, column c) = TkT[c][r]
+#pragma unroll 4
+      for (int u = 0; u < 16; ++u) {
+        const int q = t + 256 * u, r = q >> 6, c = q & 63;
+        if (k0 + r < K && k0 + c < K) Tout[(k0 + r) * ldt + k0 + c] = TkT[c * SPD_NB + r];
+      }
+    }
   } else if (tj == k) {  // A_ik Pi  (RT = Pi^T = Pi)
     const SpdPart pa = spd_fetch(src, lds, K, bnd, i0, k0, t);
     const SpdPart pp = spd_fetch(Pi, SPD_NB, SPD_NB, false, 0, 0, t);
+    SpdPart pt;
+    if (Tout) pt = spd_fetch(TkT, SPD_NB, SPD_NB, false, 0, 0, t);
     spd_put(L0, pa, t);
     spd_put(L1, pp, t);
     __syncthreads();
     spd_mma(L0, L1, R0, C0, lane, acc);
+    if (Tout) {  // T_ik = -(A_ik) T_kk  (RT = T_kk^T)
+      f64x4 tq[2][2];
+#pragma unroll
+      for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc) tq[tr][tc] = f64x4{0.0, 0.0, 0.0, 0.0};
+      __syncthreads();  // every wave is done reading L1
+      spd_put(L1, pt, t);
+      __syncthreads();
+      spd_mma(L0, L1, R0, C0, lane, tq);
+#pragma unroll
+      for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int64_t gi = i0 + spd_row(R0, tr, lane, e), gj = k0 + spd_col(C0, tc, lane);
+            if (gi < K && gj < K) Tout[gi * ldt + gj] = -tq[tr][tc][e];
+          }
+    }
   } else if (ti == k) {  // Pi A_kj = Pi A_jk^T
+    if (Tout) {  // T is upper triangular: zero tile (j, k)
+#pragma unroll 4
+      for (int u = 0; u < 16; ++u) {
+        const int q = t + 256 * u, r = q >> 6, c = q & 63;
+        if (j0 + r < K && k0 + c < K) Tout[(j0 + r) * ldt + k0 + c] = 0.0;
+      }
+    }
     const SpdPart pp = spd_fetch(Pi, SPD_NB, SPD_NB, false, 0, 0, t);
     const SpdPart pb = spd_fetch(src, lds, K, bnd, j0, k0, t);
     spd_put(L0, pp, t);
@@ -383,7 +508,7 @@ __global__ __launch_bounds__(256) void spd_step_kernel(const double* __restrict_
       for (int tc = 0; tc < 2; ++tc) acc[tr][tc] = aij[tr][tc] - acc[tr][tc];
     if (ti == k + 1 && tj == k + 1) {
       __syncthreads();  // the pivot inversion reuses the LDS tiles
-      spd_pivot_regs(acc, spd_smem, w, lane, t, (int64_t)(k + 1) * SPD_NB, Pi_next, info);
+      spd_pivot_regs(acc, spd_smem, w, lane, t, (int64_t)(k + 1) * SPD_NB, Pi_next, info, Tout ? TkT_next : nullptr);
       return;
     }
   }

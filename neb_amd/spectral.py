@@ -44,6 +44,37 @@ def _spd_inverse_async(M: torch.Tensor, out: torch.Tensor | None = None):
     return X, info
 
 
+def _spd_inverse_chol_async(M: torch.Tensor):
+    """Enqueue neb_spd_inverse_chol: returns (X, T, info) without
+    synchronising, X = M^-1 and T = U^-1 for the upper Cholesky factor U of
+    M (T^T M T = I, T upper triangular with a positive diagonal); info[0] != 0
+    names the first nonpositive pivot (1-based), and X, T are then
+    meaningless."""
+    K = M.shape[0]
+    A = M.to(torch.float64).contiguous()
+    X = torch.empty_like(A)
+    T = torch.empty_like(A)
+    info = torch.empty(1, dtype=torch.int32, device=A.device)
+    need = nw.H.load().neb_spd_inverse_workspace_size(K)
+    ws = nw.workspace(A.device).get(need)
+    nw.H.call("neb_spd_inverse_chol", K, A.data_ptr(), A.stride(0), X.data_ptr(), X.stride(0), T.data_ptr(),
+              T.stride(0), info.data_ptr(), ws.data_ptr(), ws.numel(), nw._stream())
+    return X, T, info
+
+
+def spd_inverse_chol(M: torch.Tensor):
+    """(M^-1, T) for a symmetric positive definite FP64 matrix on the GPU,
+    T = U^-1 the inverse of its upper Cholesky factor (M = U^T U,
+    T^T M T = I), both from the blocked Gauss-Jordan sweeps of libnebhip.
+    Raises FloatingPointError when M is not positive definite (the
+    cholesky gate of velocity/core.py:53-56)."""
+    X, T, info = _spd_inverse_chol_async(M)
+    bad = int(info.item())
+    if bad != 0:
+        _raise_not_pd(bad)
+    return _sym(X), T
+
+
 def _refined_inverse(Mr: torch.Tensor, out: torch.Tensor, steps: int):
     """Enqueue the sweep inverse of M_r followed by `steps` Newton-Schulz
     steps X <- X + X (I - M_r X) (two FP64 GEMMs each), written into `out`;
@@ -239,7 +270,8 @@ def coefficient_peel(M: torch.Tensor, S: torch.Tensor, ortho_tol: float = 1e-5, 
     The modes are sorted by their raw Rayleigh quotient diag(S)/diag(M)
     (stable order); T is the inverse upper Cholesky factor of the
     diagonally scaled sorted mass, refined by a second Cholesky pass, so
-    that T^T M_sorted T = I; loss_r = (T^T S_sorted T)_rr.  Gates (core.py:
+    that T^T M_sorted T = I (both factors from the FP64 sweeps of
+    neb_spd_inverse_chol); loss_r = (T^T S_sorted T)_rr.  Gates (core.py:
     57-61): every loss finite and positive, relative mass
     (1/T_rr)^2 / M_rr >= rel_min, max |T^T M T - I| <= ortho_tol and both
     factorizations positive definite.  They are evaluated on the device
@@ -275,18 +307,20 @@ def coefficient_peel(M: torch.Tensor, S: torch.Tensor, ortho_tol: float = 1e-5, 
     ms = M[order][:, order]
     ss = S[order][:, order]
     scale = torch.diagonal(ms).sqrt()
-    L1, info1 = torch.linalg.cholesky_ex(ms / scale[:, None] / scale[None, :])
-    T = torch.linalg.solve_triangular(L1.mT, eye, upper=True) / scale[:, None]
+    # inverse upper Cholesky factors of the scaled sorted mass and of the
+    # first pass's Gram (libnebhip sweeps; info: positive definiteness)
+    _, T1, info1 = _spd_inverse_chol_async(ms / scale[:, None] / scale[None, :])
+    T = T1 / scale[:, None]
     H2 = T.mT @ ms @ T
-    L2, info2 = torch.linalg.cholesky_ex(0.5 * (H2 + H2.mT))
-    T = T @ torch.linalg.solve_triangular(L2.mT, eye, upper=True)
+    _, T2, info2 = _spd_inverse_chol_async(0.5 * (H2 + H2.mT))
+    T = T @ T2
     error = (T.mT @ ms @ T - eye).abs().max()
     norm = 1.0 / torch.diagonal(T)
     relative = norm * norm / torch.diagonal(ms)
     F = T.mT @ ss @ T
     losses = torch.diagonal(F).clone()
     ok = (torch.isfinite(losses).all() & (losses > 0).all() & (relative >= rel_min).all()
-          & (error <= ortho_tol) & (info1 == 0) & (info2 == 0) & (norm * norm > mass_floor).all()
+          & (error <= ortho_tol) & (info1[0] == 0) & (info2[0] == 0) & (norm * norm > mass_floor).all()
           & (torch.diagonal(M) > mass_floor).all() & torch.isfinite(raw).all())
     gate = torch.where(ok, 0, PEEL_GATE).to(torch.int32).reshape(1)
     C = -2.0 * (T * losses[None, :] + T @ torch.triu(F, 1)) / norm[None, :]

@@ -57,6 +57,13 @@ def test_argument_errors_raise_without_touching_the_gpu():
     with pytest.raises(ValueError):
         H.call("neb_jet_output_backward", H.F32, 2, 0, 8, 8, 8, 16, 16, 8, H.BF16, 0, 16, 16, 8, 16, 8, 1.0, 16, 8,
                None)
+    # inverse Cholesky factor: T missing, T aliasing A, T's row stride below K
+    with pytest.raises(ValueError):
+        H.call("neb_spd_inverse_chol", 8, 16, 8, 32, 8, None, 8, 48, 64, 1 << 20, None)
+    with pytest.raises(ValueError):
+        H.call("neb_spd_inverse_chol", 8, 16, 8, 32, 8, 16, 8, 48, 64, 1 << 20, None)
+    with pytest.raises(ValueError):
+        H.call("neb_spd_inverse_chol", 8, 16, 8, 32, 8, 40, 4, 48, 64, 1 << 20, None)
     assert H.load().neb_cross_workspace_size(1024, 1024, 131072, 1) > 0
 
 

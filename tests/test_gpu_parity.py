@@ -725,6 +725,38 @@ def test_spd_inverse_rejects_indefinite_later_block():
         sp.spd_inverse(torch.tensor(A, device=DEV))
 
 
+@pytest.mark.parametrize("K", [5, 64, 70, 100, 130, 300, 1024])
+def test_spd_inverse_chol_against_lapack(K):
+    """T = U^-1 for A = U^T U from the sweeps (neb_spd_inverse_chol) against
+    LAPACK's Cholesky factor, with the identities that fix it: T upper
+    triangular with a positive diagonal and T^T A T = I; the inverse beside
+    it is bit-identical to neb_spd_inverse's."""
+    from neb_amd import spectral as sp
+    rng = np.random.default_rng(K + 7)
+    B = rng.normal(size=(K + 40, K))
+    A = B.T @ B / K + 1e-3 * np.eye(K)
+    At = torch.tensor(A, device=DEV)
+    X, T = sp.spd_inverse_chol(At)
+    X, T = X.cpu().numpy(), T.cpu().numpy()
+    tol = 1e-10 * max(1.0, np.linalg.cond(A) / 1e4)
+    assert not np.tril(T, -1).any()
+    assert (np.diag(T) > 0).all()
+    Tref = np.linalg.inv(np.linalg.cholesky(A).T)
+    r = rel(T, Tref); assert r < tol, r
+    r = np.abs(T.T @ A @ T - np.eye(K)).max(); assert r < tol, r
+    assert np.array_equal(X, sp.spd_inverse(At).cpu().numpy())
+
+
+def test_spd_inverse_chol_rejects_indefinite():
+    from neb_amd import spectral as sp
+    rng = np.random.default_rng(3)
+    B = rng.normal(size=(220, 200))
+    A = B.T @ B / 200
+    A[150, 150] = -1.0
+    with pytest.raises(FloatingPointError, match=r"pivot 151\)"):
+        sp.spd_inverse_chol(torch.tensor(A, device=DEV))
+
+
 # ---------------------------------------------------------------------------
 # coefficient peel objective (MB/velocity/core.py:48-69, 94-95)
 # ---------------------------------------------------------------------------

@@ -1,12 +1,16 @@
-"""Host logic of the coefficient peel (neb_amd.spectral.coefficient_peel,
-MB/velocity/core.py:48-69): the same torch FP64 code that runs on the GPU,
-run here on CPU tensors against the oracle."""
+"""The coefficient peel (neb_amd.spectral.coefficient_peel,
+MB/velocity/core.py:48-69) on small FP64 Grams against the oracle, and its
+gate through PeelSolver: the FP64 algebra around the inverse Cholesky
+factors of neb_spd_inverse_chol."""
 import numpy as np
 import pytest
 import torch
 
 from neb_amd import spectral as sp
 from oracle import neb_oracle as o
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
 
 
 def _grams(seed=4, n=300, k=11):
@@ -17,10 +21,11 @@ def _grams(seed=4, n=300, k=11):
     return U, D, w, o.gram(U, w), o.gram(D, w)
 
 
-def test_coefficient_peel_host_matches_oracle():
+def test_coefficient_peel_small_matches_oracle():
     U, D, w, M, S = _grams()
     ref = o.coefficient_peel(M, S)
-    p = sp.coefficient_peel(torch.tensor(M), torch.tensor(S))
+    p = {k: (v.cpu() if torch.is_tensor(v) else v)
+         for k, v in sp.coefficient_peel(torch.tensor(M, device=DEV), torch.tensor(S, device=DEV)).items()}
     np.testing.assert_array_equal(p["order"].numpy(), ref["order"])
     for key in ("losses", "T", "Cu", "Cd", "relative_mass"):
         np.testing.assert_allclose(p[key].numpy(), ref[key], rtol=1e-10, atol=1e-12 * np.abs(ref[key]).max())
@@ -31,16 +36,16 @@ def test_coefficient_peel_host_matches_oracle():
     assert int(p["gate"]) == 0
 
 
-def test_coefficient_peel_host_gate():
+def test_coefficient_peel_gate():
     U, D, w, M, S = _grams(k=6)
     Ud = np.column_stack((U, U[:, 1] * 2.0))
     Dd = np.column_stack((D, D[:, 1] * 2.0))
     Md, Sd = o.gram(Ud, w), o.gram(Dd, w)
     with pytest.raises(FloatingPointError, match="gate failed"):
-        sp.coefficient_peel(torch.tensor(Md), torch.tensor(Sd))
-    p = sp.coefficient_peel(torch.tensor(Md), torch.tensor(Sd), deferred=True)
+        sp.coefficient_peel(torch.tensor(Md, device=DEV), torch.tensor(Sd, device=DEV))
+    p = sp.coefficient_peel(torch.tensor(Md, device=DEV), torch.tensor(Sd, device=DEV), deferred=True)
     assert int(p["gate"]) == sp.PEEL_GATE
     solver = sp.PeelSolver(deferred=True)
-    solver(torch.tensor(Md), torch.tensor(Sd))
+    solver(torch.tensor(Md, device=DEV), torch.tensor(Sd, device=DEV))
     with pytest.raises(FloatingPointError, match="gate failed"):
         solver.check()
