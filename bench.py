@@ -924,7 +924,8 @@ def modes_bench(nw):
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / reps
         out[prec] = dict(ms_per_update=dt * 1e3, updates_timed=reps, point_modes_per_s=n * K / dt,
-                         mass_gram_rel_err_vs_fp64=err,
+                         mass_gram_rel_err_vs_fp64=err, tape_blocks_kept=int(res["tapes_kept"]),
+                         blocks=K // cfg.block,
                          peel_gate="passed" if int(res["peel"]["gate"].item()) == 0 else "failed")
         del net, res
         torch.cuda.empty_cache()
@@ -984,7 +985,7 @@ def modes_stream_bench(nw):
         dt = (time.perf_counter() - t0) / reps
         out[prec] = dict(ms_per_update=dt * 1e3, updates_timed=reps, ms_each=per, point_modes_per_s=n * K / dt,
                          rayleigh_rel_err_vs_fp64=err, mean_rayleigh=float(res["peel"]["losses"].mean()),
-                         tapes_kept=bool(res["tapes_kept"]), peel_gate="passed" if gate == 0 else "failed")
+                         tape_blocks_kept=int(res["tapes_kept"]), peel_gate="passed" if gate == 0 else "failed")
         del net, res
         torch.cuda.empty_cache()
     out["bf16x3"]["vs_f32"] = out["f32"]["ms_per_update"] / out["bf16x3"]["ms_per_update"]

@@ -19,9 +19,10 @@ block-diagonal crosses.  The field matrices of all blocks are written side
 by side into the K columns of one [n*comp, K] matrix; the Grams, the
 objective (trace, or the reference's coefficient / velocity peel) and the
 FP64 algebra are those of the shared trunk.  The reverse forms the field
-cotangents of all modes once, then recomputes each block's tape (the
-reference's phase-3 recompute, HB2P:3191-3230) and takes the block's
-columns of the cotangents through its output layer, lift and hidden layers.
+cotangents of all modes once, then takes each block's tape -- kept from the
+fields pass as far as HBM holds the tapes, else recomputed as in the
+reference's phase-3 recompute (HB2P:3191-3230) -- and the block's columns of
+the cotangents through its output layer, lift and hidden layers.
 
 Parameters are stored mode-stacked, W_j [K, out, in] and b_j [K, out] (the
 NCC:173-185 checkpoint layout; arrays() / from_arrays() give the 2D
@@ -82,9 +83,12 @@ class ModeConfig:
 
 
 # Bytes of block tapes a training step keeps between its forward and its
-# reverse (HBM is 288 GB on MI355X: the 2D K = 1024 configuration's tapes
-# are 19 GB; the 3D 5 x 128 one's 240 GB are recomputed instead).
-TAPE_BUDGET = 48 << 30
+# reverse, and the device memory left free beside them for the reverse of
+# one block (HBM is 288 GB on MI355X: the 2D K = 1024 configuration's tapes
+# are 19 GB; the 3D 5 x 128 one's 245 GB stay as far as they fit, the
+# remaining blocks are recomputed).
+TAPE_BUDGET = 256 << 30
+TAPE_HEADROOM = 32 << 30
 
 
 def siren_init(cfg: ModeConfig, seed: int):
@@ -275,9 +279,11 @@ class ModeNet:
         """Lifted fields of all K modes, block by block into one
         [n*comp, K] matrix (split hi / lo planes for the split-precision
         forward, or for the bf16 forward with split=True).  keep_tapes=True
-        returns (fields, tapes): every block's jet tape, kept for
-        backward_fields when they fit TAPE_BUDGET bytes in all (else tapes is
-        None and the reverse recomputes them block by block, VCORE:92-102)."""
+        returns (fields, tapes): the jet tapes of the first blocks, as many as
+        fit TAPE_BUDGET bytes and the available device memory less
+        TAPE_HEADROOM, kept for backward_fields (tapes[bi] None for a block
+        whose tape the reverse recomputes, as the reference does for every
+        block, VCORE:92-102)."""
         cfg = self.cfg
         K = cfg.modes
         split = (prec == "bf16x3") if split is None else (split or prec == "bf16x3")
@@ -290,15 +296,23 @@ class ModeNet:
             ft = torch.float64 if prec == "f64" else torch.float32
             planes = [torch.empty((r * n, nw._ld(K)), dtype=ft, device=self.device) for r in (cm, cs)]
         tapes = [] if keep_tapes else None
-        blocks = self._blocks()
-        for bi, (m0, B) in enumerate(blocks):
+        budget = TAPE_BUDGET
+        if keep_tapes and self.device.type == "cuda":
+            free, _ = torch.cuda.mem_get_info(self.device)
+            avail = free + torch.cuda.memory_reserved(self.device) - torch.cuda.memory_allocated(self.device)
+            budget = min(budget, int(avail) - TAPE_HEADROOM)
+        kept = 0
+        for m0, B in self._blocks():
             tape = self.block_tape(feats, m0, B, prec)
             self._block_fields_into(tape, m0, B, prec, planes)
             if tapes is not None:
-                if bi == 0 and nw.tape_bytes(tape) * len(blocks) > TAPE_BUDGET:
-                    tapes = None
-                else:
+                nb = nw.tape_bytes(tape)
+                if kept + nb <= budget:
                     tapes.append(tape)
+                    kept += nb
+                else:
+                    tapes.append(None)
+                    budget = kept         # later blocks are recomputed too
             del tape
         if split:
             f = nw.Fields(planes[0], planes[2], K, 0, planes[1], planes[3])
@@ -352,8 +366,9 @@ class ModeNet:
         products are the cotangent columns against the block's last hidden
         jets, and the hidden reverse keeps the diagonal blocks of each
         weight gradient (HB2P:2087-2139 backward_gradients_block_jet2).
-        tapes: the blocks' tapes kept by fields(keep_tapes=True); without
-        them each block's tape is recomputed."""
+        tapes: the blocks' tapes kept by fields(keep_tapes=True) (consumed:
+        each entry is dropped once used); a block without one has its tape
+        recomputed."""
         cfg = self.cfg
         K = cfg.modes
         d, C = cfg.dim, nw.channels(cfg.dim)
@@ -375,7 +390,9 @@ class ModeNet:
         s = nw._stream()
         h, L, nc = cfg.hidden, cfg.hidden_layers, cfg.ncomp
         for bi, (m0, B) in enumerate(self._blocks()):
-            tape = tapes[bi] if tapes is not None else self.block_tape(feats, m0, B, prec)
+            tape = tapes[bi] if tapes is not None and tapes[bi] is not None else self.block_tape(feats, m0, B, prec)
+            if tapes is not None:
+                tapes[bi] = None          # released once the block's reverse is queued
             p = tape.prec
             st = nw._STORE[p]
             mode = nw._CROSS[p]
@@ -501,8 +518,8 @@ class ModeNet:
             raise ValueError(f"unknown objective {objective!r}")
         out.update(dM=dM, dS=dS)
         if training:
+            out["tapes_kept"] = sum(t is not None for t in tapes)     # blocks whose tape the reverse reuses
             out["grads"] = self.backward_fields(x, fields, dM, dS, weight, prec, tapes=tapes)
-            out["tapes_kept"] = tapes is not None
         return out
 
     # -- optimizer ---------------------------------------------------------

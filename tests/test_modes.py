@@ -156,6 +156,29 @@ def test_mode_peel_step_matches_oracle():
         r = rel(a.cpu(), b); assert r < 1e-8, (j, r)
 
 
+@pytest.mark.parametrize("prec", ["f64", "bf16x3"])
+def test_mode_step_partially_kept_tapes(prec, monkeypatch):
+    """Block tapes kept from the fields pass (all, the first ones that fit
+    the budget, none) give the same gradient bit for bit: the reverse reads
+    a kept tape or recomputes the same one."""
+    K = 24
+    cfg, net = make(2, K=K, block=8, stiffness="hessian", hidden=16, layers=2,
+                    master=torch.float64 if prec == "f64" else torch.float32)
+    x = torch.tensor(points(2, 300, 5))
+    w = 4 * math.pi ** 2 / 300
+    opts = dict(ortho_tol=1e-3, rel_min=1e-4, mass_floor=1e-12, deferred=True)
+    one = net.block_tape(net.features(x, prec), 0, 8, prec)
+    per_block = nw.tape_bytes(one)
+    del one
+    grads = {}
+    for budget, kept in ((md.TAPE_BUDGET, 3), (2 * per_block, 2), (0, 0)):
+        monkeypatch.setattr(md, "TAPE_BUDGET", budget)
+        res = net.step(x, w, prec=prec, objective="peel", peel_options=opts)
+        assert res["tapes_kept"] == kept
+        grads[kept] = res["grads"].cpu()
+    assert torch.equal(grads[3], grads[0]) and torch.equal(grads[2], grads[0])
+
+
 @pytest.mark.parametrize("prec,hidden,tol", [("f64", 16, 1e-10), ("bf16x3", 32, 2e-5)])
 def test_mode_stream_fields_centered(prec, hidden, tol):
     """Streamfunction lift of per-mode networks (bf16x3 with hidden 32 runs
