@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 12
+#define NEB_ABI_VERSION 13
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -444,7 +444,10 @@ int neb_galerkin_rhs2d(int dtype, const void* gu, const void* gv, const void* gw
  * shared input features); diag = 1: block-diagonal (each mode reads its own
  * block of B*in inputs) -- in dst_dtype (bf16 / fp32 / fp64), columns
  * beyond the width zero; lo: the split lo plane (bf16 only) or NULL; wt: the
- * transpose [width][ldt] (for the reverse) or NULL. */
+ * transpose [width][ldt] (for the reverse) or NULL.  diag = 2: the diagonal
+ * blocks of diag = 1 only (B*out*in entries of hi / lo and of wt), every
+ * other entry left as it is -- for buffers the caller keeps zero off the
+ * diagonal blocks from one update to the next. */
 int neb_stage_mode_block(int master_dtype, const void* master, int64_t K, int64_t out, int64_t in, int64_t m0,
                          int64_t B, int diag, int dst_dtype, void* hi, void* lo, int64_t ld, void* wt, int64_t ldt,
                          void* stream);

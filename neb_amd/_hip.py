@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1

@@ -47,9 +47,15 @@ __global__ __launch_bounds__(256) void feature_kernel(const TX* __restrict__ x, 
   for (int c = 0; c < L::C; ++c) out[c * cs + p * ldf + f] = cvt<S>(ch[c]);
 }
 
-// Column sums in FP64: partial[split][col] over the split's row range.
-// Each thread sums 16-byte column vectors down a strided set of rows; row
-// lanes are then combined through LDS.
+// Column sums in FP64: partial[split][col] over the split's row range
+// (blockIdx.x) for one slab of colsum_slab_groups(cols) 16-byte column
+// groups (blockIdx.y).  Each thread sums 16-byte column vectors down a
+// strided set of rows; row lanes are then combined through LDS.
+template <typename S> __host__ __device__ inline int64_t colsum_slab_groups(int64_t cols) {
+  const int64_t ngroups = (cols + 16 / (int64_t)sizeof(S) - 1) / (16 / (int64_t)sizeof(S));
+  return ngroups < 32 ? ngroups : 32;
+}
+
 template <typename S>
 __global__ __launch_bounds__(256) void colsum_kernel(const S* __restrict__ x, int64_t ld, int64_t rows,
                                                      int64_t cols, int64_t chunk, double* __restrict__ partial) {
@@ -59,10 +65,11 @@ __global__ __launch_bounds__(256) void colsum_kernel(const S* __restrict__ x, in
   const int64_t rbeg = (int64_t)blockIdx.x * chunk;
   const int64_t rend = rbeg + chunk < rows ? rbeg + chunk : rows;
   const int64_t ngroups = (cols + VE - 1) / VE;
-  const int CG = ngroups < 256 ? (int)ngroups : 256;
+  const int CG = (int)colsum_slab_groups<S>(cols);
   const int RL = 256 / CG;
   const int cgl = threadIdx.x % CG, rl = threadIdx.x / CG;
-  for (int64_t g0 = 0; g0 < ngroups; g0 += CG) {
+  {
+    const int64_t g0 = (int64_t)blockIdx.y * CG;
     double acc[VE];
 #pragma unroll
     for (int e = 0; e < VE; ++e) acc[e] = 0.0;
@@ -103,7 +110,6 @@ __global__ __launch_bounds__(256) void colsum_kernel(const S* __restrict__ x, in
       const int64_t c = g0 * VE + t;
       if (c < cols) partial[(int64_t)blockIdx.x * cols + c] = sacc;
     }
-    __syncthreads();
   }
 }
 
@@ -217,6 +223,23 @@ __global__ __launch_bounds__(256) void stage_modes_kernel(const TS* __restrict__
   const int64_t rows = B * out, cols = diag ? B * in : in;
   const int64_t nw = rows * ld;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (diag == 2) {  // the diagonal blocks only, mode by mode (the rest of hi / lo / wt is left as it is)
+    const int64_t nd = B * out * in;
+    if (idx < nd) {
+      const int64_t m = idx / (out * in), o = (idx / in) % out, i = idx % in;
+      const double x = todbl_any(S[((m0 + m) * out + o) * in + i]);
+      const TD h = cvt<TD>(x);
+      const int64_t at = (m * out + o) * ld + m * in + i;
+      hi[at] = h;
+      if (lo) lo[at] = cvt<TD>(x - todbl_any(h));
+      return;
+    }
+    const int64_t it = idx - nd;  // wt[m in + i][m out + o], o fastest
+    if (!wt || it >= nd) return;
+    const int64_t m = it / (in * out), i = (it / out) % in, o = it % out;
+    wt[(m * in + i) * ldt + m * out + o] = cvt<TD>(todbl_any(S[((m0 + m) * out + o) * in + i]));
+    return;
+  }
   auto value = [&](int64_t r, int64_t c) -> double {
     if (r >= rows || c >= cols) return 0.0;
     const int64_t m = r / out, o = r % out;

@@ -534,8 +534,11 @@ int cross_dispatch_types(int xd, int yd, CrossArgs& a, double scale, double beta
 template <typename T> int colsum_launch(const void* x, int64_t rows, int64_t cols, int64_t ld, double beta,
                                         double* out, double* part, int splits, hipStream_t st) {
   const int64_t chunk = (rows + splits - 1) / splits;
-  hipLaunchKernelGGL(colsum_kernel<T>, dim3(splits), dim3(256), 0, st, reinterpret_cast<const T*>(x), ld, rows,
-                     cols, chunk, part);
+  const int64_t ngroups = (cols + 16 / (int64_t)sizeof(T) - 1) / (16 / (int64_t)sizeof(T));
+  const int64_t slab = colsum_slab_groups<T>(cols), slabs = (ngroups + slab - 1) / slab;
+  if (slabs > 65535) return fail(NEB_ERR_SHAPE, "neb_colsum: too many columns");
+  hipLaunchKernelGGL(colsum_kernel<T>, dim3(splits, (unsigned)slabs), dim3(256), 0, st, reinterpret_cast<const T*>(x),
+                     ld, rows, cols, chunk, part);
   if (int rc = hip_status("neb_colsum")) return rc;
   hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, st, part, splits, cols,
                      out, beta);
@@ -1398,13 +1401,13 @@ int neb_stage_net(int master_dtype, const void* master, int nl, const int64_t* o
 int neb_stage_mode_block(int master_dtype, const void* master, int64_t K, int64_t out, int64_t in, int64_t m0,
                          int64_t B, int diag, int dst_dtype, void* hi, void* lo, int64_t ld, void* wt, int64_t ldt,
                          void* stream) {
-  if (K < 1 || out < 1 || in < 1 || B < 1 || m0 < 0 || m0 + B > K || (diag != 0 && diag != 1))
+  if (K < 1 || out < 1 || in < 1 || B < 1 || m0 < 0 || m0 + B > K || diag < 0 || diag > 2)
     return fail(NEB_ERR_SHAPE, "neb_stage_mode_block: bad sizes");
   const int64_t rows = B * out, cols = diag ? B * in : in;
   if (ld < cols || (wt && ldt < rows)) return fail(NEB_ERR_SHAPE, "neb_stage_mode_block: bad strides");
   if (!master || !hi) return fail(NEB_ERR_ARG, "neb_stage_mode_block: null pointer");
   if (lo && dst_dtype != NEB_BF16) return fail(NEB_ERR_ARG, "neb_stage_mode_block: lo planes are bf16 only");
-  const int64_t tot = rows * ld + (wt ? cols * ldt : 0);
+  const int64_t tot = diag == 2 ? 2 * B * out * in : rows * ld + (wt ? cols * ldt : 0);
   if (tot / 256 >= 0x7fffffff) return fail(NEB_ERR_SHAPE, "neb_stage_mode_block: too many elements");
   const dim3 grid((unsigned)((tot + 255) / 256));
   hipStream_t st = S(stream);

@@ -127,6 +127,7 @@ class ModeNet:
         self.epoch = 0
         self.version = 0
         self._cache = {}
+        self._stage_bufs = {}
         self.W, self.b = self.split(self.params)
 
     def split(self, flat: torch.Tensor):
@@ -148,7 +149,10 @@ class ModeNet:
     def block_weights(self, m0: int, B: int, prec: str) -> dict:
         """The block's operands: W (and Wlo for split precision) per layer
         as [rows][ld] in the operand precision, WT (the transposes, for the
-        reverse) and the biases in the math precision."""
+        reverse) and the biases in the math precision.  The operand buffers
+        of a block stay allocated across updates: they start zero, and an
+        update rewrites the first layer whole and only the diagonal blocks
+        of the block-diagonal ones (neb_stage_mode_block diag = 2)."""
         key = (m0, B, prec, self.version)
         if key in self._cache:
             return self._cache[key]
@@ -157,14 +161,21 @@ class ModeNet:
         st = nw._STORE[p]
         mdt = nw._DT[self.master_dtype]
         s = nw._stream()
+        bufs = self._stage_bufs.get((m0, B, prec))
+        if bufs is None:
+            bufs = []
+            for j, W in enumerate(self.W):
+                fo, fi = W.shape[1:]
+                rows, cols = B * fo, (B * fi if j > 0 else fi)
+                hi = torch.zeros((rows, nw._ld(cols)), dtype=st, device=self.device)
+                bufs.append((hi, torch.zeros_like(hi) if split else None,
+                             torch.zeros((cols, nw._ld(rows)), dtype=st, device=self.device)))
+            self._stage_bufs[(m0, B, prec)] = bufs
         out = {"W": [], "Wlo": [], "WT": [], "b": []}
         for j, (W, b) in enumerate(zip(self.W, self.b)):
             fo, fi = W.shape[1:]
-            diag = 0 if j == 0 else 1
-            rows, cols = B * fo, (B * fi if diag else fi)
-            hi = torch.empty((rows, nw._ld(cols)), dtype=st, device=self.device)
-            lo = torch.empty_like(hi) if split else None
-            wt = torch.empty((cols, nw._ld(rows)), dtype=st, device=self.device)
+            diag = 0 if j == 0 else 2
+            hi, lo, wt = bufs[j]
             H.call("neb_stage_mode_block", mdt, W.data_ptr(), self.cfg.modes, fo, fi, m0, B, diag, p, hi.data_ptr(),
                    lo.data_ptr() if lo is not None else None, hi.shape[1], wt.data_ptr(), wt.shape[1], s)
             out["W"].append(hi)
