@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 13
+#define NEB_ABI_VERSION 14
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -255,6 +255,23 @@ int neb_cross_blocks(int mode, int nseg, const void* const* x, const void* const
 int neb_mode_output_fields_x3(int lift, int64_t n, int64_t h, int64_t B, const void* y_hi, const void* y_lo,
                               int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
                               void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, void* stream);
+
+/* The output-layer reverse of a block of per-mode networks: the result of
+ * neb_jet_output_backward with K = width / block for weights with one
+ * nonzero per row, output column f = m*block + j (of each component)
+ * reading only field column m = f / block -- mode m's output weights
+ * against mode m's field cotangents.  bm / bs hold just those nonzeros, as
+ * [ncomp][ldb] vectors (ldb >= width; entry f of component c: mode f /
+ * block's weight on hidden feature f % block), in the operand precision.
+ * One thread per point and 8 or 4 features: the lift's transpose and the
+ * activation jet chain on the tape, no contraction over the block's other
+ * modes.  block: the hidden width h, a multiple of 8 dividing the width.
+ * Replaces the per-mode output-layer reverse of HB2P:2087-2139 (`gu`, lift
+ * transpose and `linear_backward_batch_jet2` of the last layer, per mode). */
+int neb_jet_output_backward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block,
+                                   const void* mass, const void* stiff, int64_t ldf, int fields_dtype, int layout,
+                                   const void* bm, const void* bs, int64_t ldb, const void* z, int64_t ldz,
+                                   double omega, void* adj, int64_t lda, void* stream);
 
 /* Factored output-layer reverse (split-precision tapes).  The fields are
  * linear in the output weights, Phi_a = sum_t sg_t Y_{c_t} W_{s_t}^T over the

@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 13
+ABI_VERSION = 14
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
@@ -46,6 +46,8 @@ SIGNATURES = {
                                         _i64, _int, _vp]),
     "neb_jet_output_backward": (_int, [_int, _int, _int, _i64, _i64, _i64, _vp, _vp, _i64, _int, _int, _vp, _vp,
                                        _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
+    "neb_jet_output_backward_blocks": (_int, [_int, _int, _int, _i64, _i64, _i64, _vp, _vp, _i64, _int, _int, _vp,
+                                              _vp, _i64, _vp, _i64, _dbl, _vp, _i64, _vp]),
     "neb_jet_layer_backward": (_int, [_int, _int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _dbl,
                                       _vp, _i64, _vp]),
     "neb_jet_layer_forward_blocks": (_int, [_int, _int, _int, _i64, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _dbl,

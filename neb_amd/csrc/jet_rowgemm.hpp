@@ -384,6 +384,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<P,
   }
 }
 
+// The same plans when every weight row holds one nonzero, in column
+// f / kdiag (the output layer of a block of per-mode networks: output
+// column f = m kdiag + j belongs to mode m, whose weights read only mode m's
+// field cotangents), the weights given as those nonzeros, w_s[f]:
+// acc[s](p, f) = sum over the plan's terms t with ta[t] = s of
+// (-)w_tw[t][f] x_tc[t][p][f / kdiag], then the plan's epilogue.  One thread
+// per (point, group of GE features); the contraction is a few loads, the
+// epilogue (tape reads, jet chain, stores) is the bandwidth.  kdiag: a
+// multiple of GE.
+template <int P, typename TIN, class Plan>
+__global__ __launch_bounds__(256) void rowdiag_kernel(RowArgs a, int64_t kdiag) {
+  using PR = Prec<P>;
+  using S = typename PR::store_t;
+  using M = typename PR::math_t;
+  constexpr int GE = Plan::template out_bytes<P>() == 2 ? 8 : 4;
+  const int64_t ngr = (a.nout + GE - 1) / GE;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.npts * ngr) return;
+  const int64_t p = idx / ngr, f = idx % ngr * GE;
+  const int64_t k = f / kdiag;
+  const int nvalid = a.nout - f >= GE ? GE : (int)(a.nout - f);
+  M xin[Plan::CI];
+#pragma unroll
+  for (int c = 0; c < Plan::CI; ++c) xin[c] = tomath<M>(reinterpret_cast<const TIN*>(a.x[c])[p * a.ldx[c] + k]);
+  M v[Plan::NA][GE];
+#pragma unroll
+  for (int s = 0; s < Plan::NA; ++s)
+#pragma unroll
+    for (int e = 0; e < GE; ++e) v[s][e] = (M)0;
+  M wv[Plan::NW][GE];
+#pragma unroll
+  for (int s = 0; s < Plan::NW; ++s) ld_group<S, GE>(reinterpret_cast<const S*>(a.w[s]) + f, wv[s], nvalid);
+#pragma unroll
+  for (int t = 0; t < Plan::NT; ++t)
+#pragma unroll
+    for (int e = 0; e < GE; ++e)
+      v[Plan::ta[t]][e] += (Plan::tneg[t] ? -wv[Plan::tw[t]][e] : wv[Plan::tw[t]][e]) * xin[Plan::tc[t]];
+  Plan::template epilogue<P, GE>(a, p, f, nvalid, v);
+}
+
 // ---------------------------------------------------------------------------
 // Plans.  epilogue<P, GE>(a, p, f, nvalid, v): v[s][e] is accumulator set s
 // at point p, feature f+e (e < nvalid <= GE).

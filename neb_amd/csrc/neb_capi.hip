@@ -100,6 +100,18 @@ int launch_rowgemm(const RowArgs& a, hipStream_t st, const char* what) {
   return hip_status(what);
 }
 
+// rowdiag_kernel launch: one thread per (point, group of GE features)
+template <int P, typename TIN, class Plan>
+int launch_rowdiag(const RowArgs& a, int64_t kdiag, hipStream_t st, const char* what) {
+  constexpr int GE = Plan::template out_bytes<P>() == 2 ? 8 : 4;
+  if (kdiag < GE || kdiag % GE) return fail(NEB_ERR_SHAPE, "%s: block must be a multiple of %d", what, GE);
+  const int64_t tot = a.npts * ((a.nout + GE - 1) / GE);
+  if (tot == 0) return NEB_OK;
+  if ((tot + 255) / 256 > 0x7fffffff) return fail(NEB_ERR_SHAPE, "%s: grid too large", what);
+  hipLaunchKernelGGL((rowdiag_kernel<P, TIN, Plan>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, kdiag);
+  return hip_status(what);
+}
+
 template <int P> struct FieldT { using type = float; };
 template <> struct FieldT<PREC_F64> { using type = double; };
 
@@ -288,10 +300,14 @@ int output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi,
   return launch_rowgemm<PREC_BF16, __bf16, PlanOutFields3Split>(a, st, "neb_jet_output_fields_x3");
 }
 
+// kdiag > 0: the weights are mode-diagonal, given as each row's one nonzero
+// (bm / bs: [ncomp][ldb] vectors, entry f of component c the weight of
+// output column f, which reads field column f / kdiag; launch_rowdiag);
+// else a row GEMM over the K columns of [ncomp * width][ldb] weights.
 template <int P, int ACT, typename F>
 int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* mass, const void* stiff, int64_t ldf,
                     int layout, const void* bm, const void* bs, int64_t ldb, const void* z, int64_t ldz,
-                    double omega, void* adj, int64_t lda, hipStream_t st) {
+                    double omega, void* adj, int64_t lda, hipStream_t st, int64_t kdiag = 0) {
   using Sx = typename Prec<P>::store_t;
   const F* Mf = reinterpret_cast<const F*>(mass);
   const F* Sf = reinterpret_cast<const F*>(stiff);
@@ -306,7 +322,8 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
     a.x[2] = layout == 0 ? Sf + ldf : Sf + n * ldf; a.ldx[2] = a.ldx[1];
     a.w[0] = bm; a.ldw[0] = ldb;
     a.w[1] = bs; a.ldw[1] = ldb;
-    return launch_rowgemm<P, F, PlanOutBwd2S<ACT>>(a, st, "neb_jet_output_backward");
+    return kdiag > 0 ? launch_rowdiag<P, F, PlanOutBwd2S<ACT>>(a, kdiag, st, "neb_jet_output_backward_blocks")
+                     : launch_rowgemm<P, F, PlanOutBwd2S<ACT>>(a, st, "neb_jet_output_backward");
   }
   if (dim == NEB_LIFT_2D_HESSIAN) {
     a.x[0] = Mf; a.ldx[0] = layout == 0 ? 2 * ldf : ldf;
@@ -317,7 +334,8 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
     }
     a.w[0] = bm; a.ldw[0] = ldb;
     a.w[1] = bs; a.ldw[1] = ldb;
-    return launch_rowgemm<P, F, PlanOutBwd2H<ACT>>(a, st, "neb_jet_output_backward");
+    return kdiag > 0 ? launch_rowdiag<P, F, PlanOutBwd2H<ACT>>(a, kdiag, st, "neb_jet_output_backward_blocks")
+                     : launch_rowgemm<P, F, PlanOutBwd2H<ACT>>(a, st, "neb_jet_output_backward");
   }
   if (dim == 2) {
     if (layout == 0) {
@@ -330,7 +348,8 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
     a.x[2] = Sf; a.ldx[2] = ldf;
     a.w[0] = bm; a.ldw[0] = ldb;
     a.w[1] = bs; a.ldw[1] = ldb;
-    return launch_rowgemm<P, F, PlanOutBwd2<ACT>>(a, st, "neb_jet_output_backward");
+    return kdiag > 0 ? launch_rowdiag<P, F, PlanOutBwd2<ACT>>(a, kdiag, st, "neb_jet_output_backward_blocks")
+                     : launch_rowgemm<P, F, PlanOutBwd2<ACT>>(a, st, "neb_jet_output_backward");
   }
   for (int c = 0; c < 3; ++c) {
     a.x[c] = layout == 0 ? Mf + c * ldf : Mf + c * n * ldf;
@@ -340,13 +359,15 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
     a.x[3 + c] = layout == 0 ? Sf + c * ldf : Sf + c * n * ldf;
     a.ldx[3 + c] = layout == 0 ? 9 * ldf : ldf;
   }
+  const int64_t wstep = kdiag > 0 ? ldb : width * ldb;  // component s's weights
   for (int s = 0; s < 3; ++s) {
-    a.w[s] = reinterpret_cast<const Sx*>(bm) + s * width * ldb;
-    a.w[3 + s] = reinterpret_cast<const Sx*>(bs) + s * width * ldb;
+    a.w[s] = reinterpret_cast<const Sx*>(bm) + s * wstep;
+    a.w[3 + s] = reinterpret_cast<const Sx*>(bs) + s * wstep;
     a.ldw[s] = ldb;
     a.ldw[3 + s] = ldb;
   }
-  return launch_rowgemm<P, F, PlanOutBwd3<ACT>>(a, st, "neb_jet_output_backward");
+  return kdiag > 0 ? launch_rowdiag<P, F, PlanOutBwd3<ACT>>(a, kdiag, st, "neb_jet_output_backward_blocks")
+                     : launch_rowgemm<P, F, PlanOutBwd3<ACT>>(a, st, "neb_jet_output_backward");
 }
 
 template <int P, int D, int ACT>
@@ -826,10 +847,34 @@ int neb_jet_output_fields_split(int dim, int64_t n, int64_t k, int64_t K, const 
                                         mass_lo, stiff_lo);
 }
 
+static int output_backward_entry(int prec, int dim, int act, int64_t n, int64_t width, int64_t K, const void* mass,
+                                 const void* stiff, int64_t ldf, int fields_dtype, int layout, const void* bm,
+                                 const void* bs, int64_t ldb, const void* z, int64_t ldz, double omega, void* adj,
+                                 int64_t lda, void* stream, int64_t kdiag);
+
 int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width, int64_t K, const void* mass,
                             const void* stiff, int64_t ldf, int fields_dtype, int layout, const void* bm,
                             const void* bs, int64_t ldb, const void* z, int64_t ldz, double omega, void* adj,
                             int64_t lda, void* stream) {
+  return output_backward_entry(prec, dim, act, n, width, K, mass, stiff, ldf, fields_dtype, layout, bm, bs, ldb, z,
+                               ldz, omega, adj, lda, stream, 0);
+}
+
+int neb_jet_output_backward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block,
+                                   const void* mass, const void* stiff, int64_t ldf, int fields_dtype, int layout,
+                                   const void* bm, const void* bs, int64_t ldb, const void* z, int64_t ldz,
+                                   double omega, void* adj, int64_t lda, void* stream) {
+  if (block < 8 || block % 8 || width % block)
+    return fail(NEB_ERR_SHAPE, "neb_jet_output_backward_blocks: block must be a multiple of 8 dividing the width");
+  if (ldb < width) return fail(NEB_ERR_SHAPE, "neb_jet_output_backward_blocks: weight vectors shorter than the width");
+  return output_backward_entry(prec, dim, act, n, width, width / block, mass, stiff, ldf, fields_dtype, layout, bm, bs,
+                               ldb, z, ldz, omega, adj, lda, stream, block);
+}
+
+static int output_backward_entry(int prec, int dim, int act, int64_t n, int64_t width, int64_t K, const void* mass,
+                                 const void* stiff, int64_t ldf, int fields_dtype, int layout, const void* bm,
+                                 const void* bs, int64_t ldb, const void* z, int64_t ldz, double omega, void* adj,
+                                 int64_t lda, void* stream, int64_t kdiag) {
   if (int rc = check_lift(prec, dim, act)) return rc;
   if (n < 0 || width < 1 || K < 1 || ldf < K || ldb < K || ldz < width || lda < width ||
       (layout != 0 && layout != 1))
@@ -844,7 +889,7 @@ int neb_jet_output_backward(int prec, int dim, int act, int64_t n, int64_t width
     return fail(NEB_ERR_ARG, "neb_jet_output_backward: rows must be 16-byte aligned");
   hipStream_t st = S(stream);
 #define OB(P, A, F) \
-  output_backward<P, A, F>(dim, n, width, K, mass, stiff, ldf, layout, bm, bs, ldb, z, ldz, omega, adj, lda, st)
+  output_backward<P, A, F>(dim, n, width, K, mass, stiff, ldf, layout, bm, bs, ldb, z, ldz, omega, adj, lda, st, kdiag)
   if (prec == NEB_BF16) {
     if (fields_dtype == NEB_BF16)
       return act == NEB_TANH ? OB(PREC_BF16, ACT_TANH, __bf16) : OB(PREC_BF16, ACT_SIN, __bf16);

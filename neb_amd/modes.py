@@ -409,21 +409,16 @@ class ModeNet:
             mode = nw._CROSS[p]
             bc = cfg.block_net_config(B)
             stg = self.block_weights(m0, B, prec)
-            # the block's output layer per component c as [B h, B]: column m holds
-            # mode m's weights W_out[m, c, :] in rows m h .. m h + h - 1
-            Wo = self.W[-1][m0:m0 + B].to(torch.float64)                    # [B, nc, h]
-            Wt = torch.zeros((nc, B, h, B), dtype=torch.float64, device=self.device)
-            Wt.diagonal(dim1=1, dim2=3).copy_(Wo.permute(1, 2, 0))           # Wt[c, m, :, m] = Wo[m, c]
-            Wt = Wt.reshape(nc * B * h, B)
-            bm = torch.empty((Wt.shape[0], nw._ld(B)), dtype=st, device=self.device)
-            H.call("neb_stage", H.F64, p, Wt.data_ptr(), Wt.shape[0], B, B, bm.data_ptr(), bm.shape[1],
-                   bm.shape[0], bm.shape[1], 0, 1.0, s)
+            # the block's output weights per component c as a [B h] vector:
+            # entry m h + j is mode m's weight W_out[m, c, j] (the one nonzero
+            # of that row of the block-diagonal output layer)
             width = B * h
+            bm = self.W[-1][m0:m0 + B].permute(1, 0, 2).reshape(nc, width).to(st).contiguous()
             A = torch.empty((C, n, nw._ld(width)), dtype=st, device=self.device)
             Z = tape.pre[-1]
             off = m0 * GU.element_size()
-            H.call("neb_jet_output_backward", p, bc.lift, H.SIN, n, width, B, GU.data_ptr() + off,
-                   GD.data_ptr() + off, ldg, nw._DT[gdt], 0, bm.data_ptr(), bm.data_ptr(), bm.shape[1],
+            H.call("neb_jet_output_backward_blocks", p, bc.lift, H.SIN, n, width, h, GU.data_ptr() + off,
+                   GD.data_ptr() + off, ldg, nw._DT[gdt], 0, bm.data_ptr(), bm.data_ptr(), width,
                    Z.data_ptr(), Z.shape[2], bc.omega(L - 1), A.data_ptr(), A.shape[2], s)
             # output weights: the block's cotangent columns against its last hidden jets
             P = self._field_products(tape, cot, m0, B)

@@ -336,3 +336,46 @@ def test_block_diagonal_entry_points_equal_dense(dim, h, B):
         outs.append((Z, Yh, Yl))
     for a, c in zip(*outs):
         assert torch.equal(a, c), "split forward"
+
+
+@pytest.mark.parametrize("lift,dim,rm,rs,nc", [(2, 2, 2, 1, 1), (3, 3, 3, 9, 3), (4, 2, 2, 3, 1), (5, 2, 1, 2, 1)])
+def test_output_backward_blocks_equals_dense(lift, dim, rm, rs, nc):
+    """neb_jet_output_backward_blocks (one weight entry per output feature:
+    mode m's output weights against mode m's field cotangents, given as those
+    entries) gives neb_jet_output_backward's result on the same
+    mode-diagonal weights as a full [ncomp * width][K] matrix:
+    exactly the same products for FP32 / FP64, and within the dense path's
+    rounding of the FP32 cotangents to bf16 operands for BF16."""
+    from neb_amd import _hip as H
+    g = torch.Generator(device="cpu").manual_seed(11 + lift)
+    h, B, n = 32, 16, 333
+    C, wd = nw.channels(dim), B * h
+    ld, ldk = nw._ld(wd), nw._ld(B)
+    Wt = torch.zeros((nc, B, h, B), dtype=torch.float64)
+    Wt.diagonal(dim1=1, dim2=3).copy_(torch.randn((nc, h, B), generator=g, dtype=torch.float64))
+    Wt = Wt.reshape(nc * wd, B)
+    Z = torch.zeros((C, n, ld), dtype=torch.float64)
+    Z[:, :, :wd] = torch.randn((C, n, wd), generator=g, dtype=torch.float64)
+    s = torch.cuda.current_stream().cuda_stream
+    for prec, p in (("bf16", H.BF16), ("f32", H.F32), ("f64", H.F64)):
+        st = nw._STORE[p]
+        ft = torch.float64 if p == H.F64 else torch.float32
+        GU = torch.zeros((rm * n, ldk), dtype=ft, device=DEV)
+        GD = torch.zeros((rs * n, ldk), dtype=ft, device=DEV)
+        GU[:, :B] = torch.randn((rm * n, B), generator=g, dtype=torch.float64).to(DEV, ft)
+        GD[:, :B] = torch.randn((rs * n, B), generator=g, dtype=torch.float64).to(DEV, ft)
+        bm = torch.zeros((nc * wd, ldk), dtype=st, device=DEV)
+        bm[:, :B] = Wt.to(DEV, st)
+        f = torch.arange(wd)
+        vec = torch.stack([Wt[c * wd + f, f // h] for c in range(nc)]).to(DEV, st).contiguous()  # each row's nonzero
+        Zd = Z.to(DEV, st)
+        outs = []
+        for name, kk, wts, ldb in (("neb_jet_output_backward", B, bm, ldk),
+                                   ("neb_jet_output_backward_blocks", h, vec, wd)):
+            A = torch.zeros((C, n, ld), dtype=st, device=DEV)
+            print(name, lift, prec, flush=True)
+            H.call(name, p, lift, H.SIN, n, wd, kk, GU.data_ptr(), GD.data_ptr(), ldk, nw._DT[ft], 0,
+                   wts.data_ptr(), wts.data_ptr(), ldb, Zd.data_ptr(), ld, 1.7, A.data_ptr(), ld, s)
+            outs.append(A[:, :, :wd].double().cpu())
+        tol = {"bf16": 1.5e-2, "f32": 1e-6, "f64": 1e-13}[prec]
+        r = rel(outs[1], outs[0]); assert r < tol, (prec, r)
