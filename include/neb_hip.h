@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 14
+#define NEB_ABI_VERSION 15
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -65,6 +65,14 @@ enum { NEB_TANH = 0, NEB_SIN = 1 };                           /* activations */
  * Rayleigh peel (mass rows psi, stiffness rows psi_x, psi_y: M = int psi^2,
  * K = int |grad psi|^2, HB2P:3394-3397) */
 enum { NEB_LIFT_2D_VORTICITY = 2, NEB_LIFT_3D_CURL = 3, NEB_LIFT_2D_HESSIAN = 4, NEB_LIFT_2D_STREAM = 5 };
+/* the streamfunction lift on first-order tapes (value and gradient channels
+ * only, the Jet1 stream of HB2P:1716-1800): the fields of
+ * NEB_LIFT_2D_STREAM; the output-layer reverse writes a 3-channel adjoint */
+enum { NEB_LIFT_2D_STREAM_JET1 = 6 };
+/* dim code of the jet-layer entry points (neb_jet_layer_forward / _x3 /
+ * _blocks, neb_jet_layer_backward / _blocks) for 2D first-order jets:
+ * packed channels (value, d/dx, d/dy), C = 3 */
+enum { NEB_DIM_2D_JET1 = 12 };
 enum { NEB_CROSS_F32 = 0, NEB_CROSS_BF16 = 1, NEB_CROSS_BF16X3 = 2, NEB_CROSS_F64 = 3 };
 
 int neb_abi_version(void);

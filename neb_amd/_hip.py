@@ -13,11 +13,12 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 14
+ABI_VERSION = 15
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
-LIFT_2D_VORTICITY, LIFT_3D_CURL, LIFT_2D_HESSIAN, LIFT_2D_STREAM = 2, 3, 4, 5
+LIFT_2D_VORTICITY, LIFT_3D_CURL, LIFT_2D_HESSIAN, LIFT_2D_STREAM, LIFT_2D_STREAM_JET1 = 2, 3, 4, 5, 6
+DIM_2D_JET1 = 12
 CROSS_F32, CROSS_BF16, CROSS_BF16X3, CROSS_F64 = 0, 1, 2, 3
 
 _i64, _dbl, _int, _vp, _sz = C.c_int64, C.c_double, C.c_int, C.c_void_p, C.c_size_t

@@ -431,6 +431,7 @@ __global__ __launch_bounds__(256) void rowdiag_kernel(RowArgs a, int64_t kdiag) 
 template <int D> struct TileFor;
 template <> struct TileFor<2> { static constexpr int FB = 4, PB = 1, WPN = 2, WPP = 2; };
 template <> struct TileFor<3> { static constexpr int FB = 2, PB = 1, WPN = 2, WPP = 2; };
+template <> struct TileFor<JET1_2D> { static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2; };
 
 #define IDENTITY_TERMS                                             \
   static constexpr int ta[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};    \
@@ -472,7 +473,7 @@ template <int D, int ACT> struct PlanHiddenFwd {
     for (int e = 0; e < GE; ++e) Act<ACT, M>::eval(u[0][e], t[e], f1[e], f2[e]);
     st_group<S, GE>(y, t, nv);
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
+    for (int c = 0; c < L::S; ++c) {
 #pragma unroll
       for (int e = 0; e < GE; ++e) t[e] = f1[e] * u[1 + c][e];
       st_group<S, GE>(y + (1 + c) * a.cstride, t, nv);
@@ -481,8 +482,8 @@ template <int D, int ACT> struct PlanHiddenFwd {
     for (int q = 0; q < L::P; ++q) {
 #pragma unroll
       for (int e = 0; e < GE; ++e)
-        t[e] = f1[e] * u[1 + D + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
-      st_group<S, GE>(y + (1 + D + q) * a.cstride, t, nv);
+        t[e] = f1[e] * u[1 + L::S + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
+      st_group<S, GE>(y + (1 + L::S + q) * a.cstride, t, nv);
     }
   }
 };
@@ -513,6 +514,12 @@ template <> struct HiddenSplitTerms<3> {
   static constexpr int tw[30] = {1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0,
                                  1, 0, 0};
   static constexpr bool tneg[30] = {false};
+};
+template <> struct HiddenSplitTerms<JET1_2D> {
+  static constexpr int ta[9] = {0, 0, 0, 1, 1, 1, 2, 2, 2};
+  static constexpr int tc[9] = {0, 3, 0, 1, 4, 1, 2, 5, 2};
+  static constexpr int tw[9] = {1, 0, 0, 1, 0, 0, 1, 0, 0};
+  static constexpr bool tneg[9] = {false};
 };
 
 // Hidden layer forward on split operands: Z = omega*(X W^T + b) and
@@ -559,7 +566,7 @@ template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
     for (int e = 0; e < GE; ++e) Act<ACT, M>::eval(u[0][e], t[e], f1[e], f2[e]);
     st_field<true, GE, M>(a.out1, a.out1lo, off, t, nv);
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
+    for (int c = 0; c < L::S; ++c) {
 #pragma unroll
       for (int e = 0; e < GE; ++e) t[e] = f1[e] * u[1 + c][e];
       st_field<true, GE, M>(a.out1, a.out1lo, off + (1 + c) * a.cstride, t, nv);
@@ -568,8 +575,8 @@ template <int D, int ACT> struct PlanHiddenFwdSplit : HiddenSplitTerms<D> {
     for (int q = 0; q < L::P; ++q) {
 #pragma unroll
       for (int e = 0; e < GE; ++e)
-        t[e] = f1[e] * u[1 + D + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
-      st_field<true, GE, M>(a.out1, a.out1lo, off + (1 + D + q) * a.cstride, t, nv);
+        t[e] = f1[e] * u[1 + L::S + q][e] + f2[e] * u[1 + L::pi(q)][e] * u[1 + L::pj(q)][e];
+      st_field<true, GE, M>(a.out1, a.out1lo, off + (1 + L::S + q) * a.cstride, t, nv);
     }
   }
 };
@@ -965,7 +972,7 @@ template <int ACT> struct PlanOutBwd2H {
 // PlanOutFields2ST): x[0] = psi rows, x[1] = psi_x rows, x[2] = psi_y rows;
 // w[0] = Bm, w[1] = Bs.
 //   adj v = psi . Bm, adj gx = sx . Bs, adj gy = sy . Bs
-template <int ACT> struct PlanOutBwd2S {
+template <int ACT, int JD = 2> struct PlanOutBwd2S {
   template <int P> __host__ __device__ static constexpr int out_bytes() { return sizeof(typename Prec<P>::store_t); }
   static constexpr int CI = 3, NW = 2, NA = 3, NT = 3;
   static constexpr int FB = 4, PB = 2, WPN = 2, WPP = 2;
@@ -973,15 +980,19 @@ template <int ACT> struct PlanOutBwd2S {
   static constexpr int tc[NT] = {0, 1, 2};
   static constexpr int tw[NT] = {0, 1, 1};
   static constexpr bool tneg[NT] = {false, false, false};
+  // JD: the tape's jets, second order (2: Hessian adjoints zero) or first
+  // order (JET1_2D: value and gradient channels only)
   template <int P, int GE, typename M>
   __device__ static inline void epilogue(const RowArgs& a, int64_t p, int64_t f, int nv, M (*v)[GE]) {
-    M adj[6][GE];
+    constexpr int C = Jet2Layout<JD>::C;
+    M adj[C][GE];
 #pragma unroll
     for (int e = 0; e < GE; ++e) {
       adj[0][e] = v[0][e]; adj[1][e] = v[1][e]; adj[2][e] = v[2][e];
-      adj[3][e] = (M)0; adj[4][e] = (M)0; adj[5][e] = (M)0;
+#pragma unroll
+      for (int c = 3; c < C; ++c) adj[c][e] = (M)0;
     }
-    reverse_epilogue<P, 2, ACT, GE, M>(a, p, f, nv, adj);
+    reverse_epilogue<P, JD, ACT, GE, M>(a, p, f, nv, adj);
   }
 };
 

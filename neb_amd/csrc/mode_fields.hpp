@@ -22,6 +22,7 @@
 // Four waves per 16 points keep enough loads in flight: the contraction is
 // latency-bound, not MFMA-bound.
 #pragma once
+#include <type_traits>
 #include "jet_rowgemm.hpp"
 
 struct ModeFieldsArgs {
@@ -40,8 +41,12 @@ struct ModeFieldsArgs {
 };
 
 // LIFT: NEB_LIFT_2D_VORTICITY (2), NEB_LIFT_3D_CURL (3), NEB_LIFT_2D_HESSIAN (4),
-// NEB_LIFT_2D_STREAM (5)
+// NEB_LIFT_2D_STREAM (5, also for first-order tapes).  CU: the jet channels
+// the lift reads (the first CU of the tape), all of them unless set.
 template <int LIFT> struct ModeLift;
+template <class L, class = void> struct ModeLiftChannels : std::integral_constant<int, Jet2Layout<L::D>::C> {};
+template <class L>
+struct ModeLiftChannels<L, std::void_t<decltype(L::CU)>> : std::integral_constant<int, L::CU> {};
 template <> struct ModeLift<2> {
   static constexpr int D = 2, NC = 1, CM = 2, CS = 1;
   // a[c][comp]: raw output jet channel c of component comp
@@ -62,7 +67,7 @@ template <> struct ModeLift<4> {
   }
 };
 template <> struct ModeLift<5> {
-  static constexpr int D = 2, NC = 1, CM = 1, CS = 2;
+  static constexpr int D = 2, NC = 1, CM = 1, CS = 2, CU = 3;
   __device__ static inline void apply(const float (*a)[3], float* m, float* s) {
     m[0] = a[0][0];                       // psi (no bias: the peel's centering removes it)
     s[0] = a[1][0];                       // psi_x, psi_y
@@ -90,7 +95,7 @@ template <> struct ModeLift<3> {
 template <int LIFT>
 __global__ __launch_bounds__(256) void mode_fields_kernel(ModeFieldsArgs a) {
   using L = ModeLift<LIFT>;
-  constexpr int C = Jet2Layout<L::D>::C, NC = L::NC, CM = L::CM, CS = L::CS;
+  constexpr int C = ModeLiftChannels<L>::value, NC = L::NC, CM = L::CM, CS = L::CS;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int fr = lane & 15, g = lane >> 4;
   const int64_t mgroups = a.B / 8;

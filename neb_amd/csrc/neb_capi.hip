@@ -128,11 +128,20 @@ int check_common(int prec, int dim, int act) {
   return NEB_OK;
 }
 
+// The dim argument of the jet-layer entry points: 2, 3 (second-order jets)
+// or NEB_DIM_2D_JET1 (2D first-order jets, kernel code JET1_2D).
+int check_layer(int prec, int dim, int act) {
+  return check_common(prec, dim == NEB_DIM_2D_JET1 ? 2 : dim, act);
+}
+
+bool is_stream_lift(int lift) { return lift == NEB_LIFT_2D_STREAM || lift == NEB_LIFT_2D_STREAM_JET1; }
+
 // The lift argument of the output-layer entry points: 2 (2D velocity +
 // vorticity), 3 (3D curl), NEB_LIFT_2D_HESSIAN (2D velocity + Hessian rows),
-// NEB_LIFT_2D_STREAM (2D streamfunction + gradient rows).
+// NEB_LIFT_2D_STREAM / _JET1 (2D streamfunction + gradient rows, on second-
+// or first-order tapes).
 int check_lift(int prec, int lift, int act) {
-  if (lift == NEB_LIFT_2D_HESSIAN || lift == NEB_LIFT_2D_STREAM) return check_common(prec, 2, act);
+  if (lift == NEB_LIFT_2D_HESSIAN || is_stream_lift(lift)) return check_common(prec, 2, act);
   return check_common(prec, lift, act);
 }
 
@@ -199,7 +208,7 @@ int output_fields(int dim, int64_t n, int64_t k, int64_t K, const void* y, int64
   a.out0 = mass; a.out1 = stiff; a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
   a.out0lo = mass_lo; a.out1lo = stiff_lo;
   const char* what = SPLIT ? "neb_jet_output_fields_split" : "neb_jet_output_fields";
-  if (dim == NEB_LIFT_2D_STREAM) {
+  if (is_stream_lift(dim)) {
     // channels v, gx, gy
     for (int c = 0; c < 3; ++c) { a.x[c] = Y + c * cs; a.ldx[c] = ldy; }
     a.w[0] = w; a.ldw[0] = ldw;
@@ -258,7 +267,7 @@ int output_fields_x3(int dim, int64_t n, int64_t k, int64_t K, const void* y_hi,
   a.npts = n; a.nout = K; a.k = k;
   a.out0 = mass_hi; a.out0lo = mass_lo; a.out1 = stiff_hi; a.out1lo = stiff_lo;
   a.ldo = ldf; a.ldo2 = ldf; a.layout = layout;
-  if (dim == NEB_LIFT_2D_STREAM) {
+  if (is_stream_lift(dim)) {
     for (int c = 0; c < 3; ++c) {
       a.x[c] = Yh + c * cs;
       a.x[3 + c] = Yl + c * cs;
@@ -316,12 +325,16 @@ int output_backward(int dim, int64_t n, int64_t width, int64_t K, const void* ma
   a.tape = z; a.ldt = ldz; a.tstride = n * ldz;
   a.omega = omega;
   a.out0 = adj; a.ldo = lda; a.cstride = n * lda;
-  if (dim == NEB_LIFT_2D_STREAM) {
+  if (is_stream_lift(dim)) {
     a.x[0] = Mf; a.ldx[0] = ldf;
     a.x[1] = Sf; a.ldx[1] = layout == 0 ? 2 * ldf : ldf;
     a.x[2] = layout == 0 ? Sf + ldf : Sf + n * ldf; a.ldx[2] = a.ldx[1];
     a.w[0] = bm; a.ldw[0] = ldb;
     a.w[1] = bs; a.ldw[1] = ldb;
+    if (dim == NEB_LIFT_2D_STREAM_JET1)
+      return kdiag > 0
+                 ? launch_rowdiag<P, F, PlanOutBwd2S<ACT, JET1_2D>>(a, kdiag, st, "neb_jet_output_backward_blocks")
+                 : launch_rowgemm<P, F, PlanOutBwd2S<ACT, JET1_2D>>(a, st, "neb_jet_output_backward");
     return kdiag > 0 ? launch_rowdiag<P, F, PlanOutBwd2S<ACT>>(a, kdiag, st, "neb_jet_output_backward_blocks")
                      : launch_rowgemm<P, F, PlanOutBwd2S<ACT>>(a, st, "neb_jet_output_backward");
   }
@@ -697,6 +710,11 @@ int neb_feature_jets(int prec, int dim, int64_t n, const void* x, int freqs, con
 }
 
 #define DISPATCH_PDA(CALL)                                                        \
+  if (dim == NEB_DIM_2D_JET1) {                                                   \
+    if (prec == NEB_BF16) return act == NEB_TANH ? CALL(PREC_BF16, JET1_2D, ACT_TANH) : CALL(PREC_BF16, JET1_2D, ACT_SIN); \
+    if (prec == NEB_F32) return act == NEB_TANH ? CALL(PREC_F32, JET1_2D, ACT_TANH) : CALL(PREC_F32, JET1_2D, ACT_SIN);    \
+    return act == NEB_TANH ? CALL(PREC_F64, JET1_2D, ACT_TANH) : CALL(PREC_F64, JET1_2D, ACT_SIN);                         \
+  }                                                                               \
   switch (prec) {                                                                 \
     case NEB_BF16:                                                                \
       if (dim == 2) { if (act == NEB_TANH) return CALL(PREC_BF16, 2, ACT_TANH); return CALL(PREC_BF16, 2, ACT_SIN); } \
@@ -712,7 +730,7 @@ int neb_feature_jets(int prec, int dim, int64_t n, const void* x, int freqs, con
 int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* x,
                           int64_t ldx, const void* w, int64_t ldw, const void* bias, double omega, void* z, void* y,
                           int64_t ldy, void* stream) {
-  if (int rc = check_common(prec, dim, act)) return rc;
+  if (int rc = check_layer(prec, dim, act)) return rc;
   if (n < 0 || k < 1 || width < 1 || ldx < k || ldw < k || ldy < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward: bad sizes");
   if (n == 0) return NEB_OK;
@@ -727,7 +745,7 @@ int neb_jet_layer_forward(int prec, int dim, int act, int64_t n, int64_t k, int6
 int neb_jet_layer_forward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block, const void* x,
                                  int64_t ldx, const void* w, int64_t ldw, const void* bias, double omega, void* z,
                                  void* y, int64_t ldy, void* stream) {
-  if (int rc = check_common(prec, dim, act)) return rc;
+  if (int rc = check_layer(prec, dim, act)) return rc;
   if (n < 0 || width < 1 || block < 8 || block % 8 || width % block || ldx < width || ldw < width || ldy < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_blocks: bad sizes (block a multiple of 8 dividing the width)");
   if (n == 0) return NEB_OK;
@@ -742,7 +760,7 @@ int neb_jet_layer_forward_blocks(int prec, int dim, int act, int64_t n, int64_t 
 int neb_jet_layer_backward_blocks(int prec, int dim, int act, int64_t n, int64_t width, int64_t block,
                                   const void* adj_in, int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev,
                                   int64_t ldz, double omega_prev, void* adj_out, int64_t lda_out, void* stream) {
-  if (int rc = check_common(prec, dim, act)) return rc;
+  if (int rc = check_layer(prec, dim, act)) return rc;
   if (n < 0 || width < 1 || block < 8 || block % 8 || width % block || lda_in < width || ldw < width ||
       ldz < width || lda_out < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_backward_blocks: bad sizes (block a multiple of 8 dividing the width)");
@@ -903,7 +921,7 @@ static int output_backward_entry(int prec, int dim, int act, int64_t n, int64_t 
 int neb_jet_layer_backward(int prec, int dim, int act, int64_t n, int64_t k, int64_t width, const void* adj_in,
                            int64_t lda_in, const void* wt, int64_t ldw, const void* z_prev, int64_t ldz,
                            double omega_prev, void* adj_out, int64_t lda_out, void* stream) {
-  if (int rc = check_common(prec, dim, act)) return rc;
+  if (int rc = check_layer(prec, dim, act)) return rc;
   if (n < 0 || k < 1 || width < 1 || lda_in < width || ldw < width || ldz < k || lda_out < k)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_backward: bad sizes");
   if (n == 0) return NEB_OK;
@@ -1317,7 +1335,7 @@ int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t wid
                              const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
                              const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,
                              int64_t ldy, void* stream) {
-  if (int rc = check_common(NEB_BF16, dim, act)) return rc;
+  if (int rc = check_layer(NEB_BF16, dim, act)) return rc;
   if (n < 0 || k < 1 || width < 1 || ldx < k || ldw < k || ldy < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_x3: bad sizes");
   if (n == 0) return NEB_OK;
@@ -1330,6 +1348,7 @@ int neb_jet_layer_forward_x3(int dim, int act, int64_t n, int64_t k, int64_t wid
   hipStream_t st = S(stream);
 #define LFX3(D, A) \
   layer_forward_x3<D, A>(n, k, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi, y_lo, ldy, st)
+  if (dim == NEB_DIM_2D_JET1) return act == NEB_TANH ? LFX3(JET1_2D, ACT_TANH) : LFX3(JET1_2D, ACT_SIN);
   if (dim == 2) return act == NEB_TANH ? LFX3(2, ACT_TANH) : LFX3(2, ACT_SIN);
   return act == NEB_TANH ? LFX3(3, ACT_TANH) : LFX3(3, ACT_SIN);
 #undef LFX3
@@ -1339,7 +1358,7 @@ int neb_jet_layer_forward_x3_blocks(int dim, int act, int64_t n, int64_t width, 
                                     const void* x_lo, int64_t ldx, const void* w_hi, const void* w_lo, int64_t ldw,
                                     const void* bias, double omega, void* z_hi, void* z_lo, void* y_hi, void* y_lo,
                                     int64_t ldy, void* stream) {
-  if (int rc = check_common(NEB_BF16, dim, act)) return rc;
+  if (int rc = check_layer(NEB_BF16, dim, act)) return rc;
   if (n < 0 || width < 1 || block < 8 || block % 8 || width % block || ldx < width || ldw < width || ldy < width)
     return fail(NEB_ERR_SHAPE, "neb_jet_layer_forward_x3_blocks: bad sizes (block a multiple of 8 dividing the width)");
   if (n == 0) return NEB_OK;
@@ -1353,6 +1372,7 @@ int neb_jet_layer_forward_x3_blocks(int dim, int act, int64_t n, int64_t width, 
 #define LFX3(D, A)                                                                                                  \
   layer_forward_x3<D, A>(n, width, width, x_hi, x_lo, ldx, w_hi, w_lo, ldw, bias, omega, z_hi, z_lo, y_hi, y_lo, ldy, \
                          st, block)
+  if (dim == NEB_DIM_2D_JET1) return act == NEB_TANH ? LFX3(JET1_2D, ACT_TANH) : LFX3(JET1_2D, ACT_SIN);
   if (dim == 2) return act == NEB_TANH ? LFX3(2, ACT_TANH) : LFX3(2, ACT_SIN);
   return act == NEB_TANH ? LFX3(3, ACT_TANH) : LFX3(3, ACT_SIN);
 #undef LFX3
@@ -1361,8 +1381,7 @@ int neb_jet_layer_forward_x3_blocks(int dim, int act, int64_t n, int64_t width, 
 int neb_mode_output_fields_x3(int lift, int64_t n, int64_t h, int64_t B, const void* y_hi, const void* y_lo,
                               int64_t ldy, const void* w_hi, const void* w_lo, int64_t ldw, void* mass_hi,
                               void* mass_lo, void* stiff_hi, void* stiff_lo, int64_t ldf, void* stream) {
-  if (lift != NEB_LIFT_2D_VORTICITY && lift != NEB_LIFT_3D_CURL && lift != NEB_LIFT_2D_HESSIAN &&
-      lift != NEB_LIFT_2D_STREAM)
+  if (lift != NEB_LIFT_2D_VORTICITY && lift != NEB_LIFT_3D_CURL && lift != NEB_LIFT_2D_HESSIAN && !is_stream_lift(lift))
     return fail(NEB_ERR_ARG, "neb_mode_output_fields_x3: unknown lift %d", lift);
   if (n < 0 || h < 32 || h % 32 || B < 8 || B % 8 || ldy < B * h || ldw < B * h || ldf < B)
     return fail(NEB_ERR_SHAPE, "neb_mode_output_fields_x3: bad sizes (h a multiple of 32, B a multiple of 8)");
@@ -1386,7 +1405,7 @@ int neb_mode_output_fields_x3(int lift, int64_t n, int64_t h, int64_t B, const v
   const dim3 g((unsigned)grid);
   if (lift == NEB_LIFT_3D_CURL) hipLaunchKernelGGL(mode_fields_kernel<3>, g, dim3(256), 0, st, a);
   else if (lift == NEB_LIFT_2D_HESSIAN) hipLaunchKernelGGL(mode_fields_kernel<4>, g, dim3(256), 0, st, a);
-  else if (lift == NEB_LIFT_2D_STREAM) hipLaunchKernelGGL(mode_fields_kernel<5>, g, dim3(256), 0, st, a);
+  else if (is_stream_lift(lift)) hipLaunchKernelGGL(mode_fields_kernel<5>, g, dim3(256), 0, st, a);
   else hipLaunchKernelGGL(mode_fields_kernel<2>, g, dim3(256), 0, st, a);
   return hip_status("neb_mode_output_fields_x3");
 }

@@ -129,18 +129,27 @@ template <typename M> struct Act<ACT_SIN, M> {
   }
 };
 
-// Hessian pair tables in the reference order: 2D (0,0),(0,1),(1,1);
-// 3D (0,0),(0,1),(0,2),(1,1),(1,2),(2,2).
+// Jet channel layouts, keyed by the kernels' dimension code D: C packed
+// channels (value, the S spatial gradients, then P Hessian pairs in the
+// reference order: 2D (0,0),(0,1),(1,1); 3D (0,0),(0,1),(0,2),(1,1),(1,2),
+// (2,2)).  D = JET1_2D: 2D first-order jets (value and gradient only, the
+// Jet1 stream of HB2P:1716-1800).
+constexpr int JET1_2D = 12;
 template <int D> struct Jet2Layout;
 template <> struct Jet2Layout<2> {
-  static constexpr int C = 6, P = 3;
+  static constexpr int C = 6, P = 3, S = 2;
   __device__ static constexpr int pi(int p) { return p == 2 ? 1 : 0; }
   __device__ static constexpr int pj(int p) { return p == 0 ? 0 : 1; }
 };
 template <> struct Jet2Layout<3> {
-  static constexpr int C = 10, P = 6;
+  static constexpr int C = 10, P = 6, S = 3;
   __device__ static constexpr int pi(int p) { return p < 3 ? 0 : (p < 5 ? 1 : 2); }
   __device__ static constexpr int pj(int p) { return p < 3 ? p : (p < 5 ? p - 2 : 2); }
+};
+template <> struct Jet2Layout<JET1_2D> {
+  static constexpr int C = 3, P = 0, S = 2;
+  __device__ static constexpr int pi(int) { return 0; }
+  __device__ static constexpr int pj(int) { return 0; }
 };
 
 // Second-order jet product y = e x (the envelope product rule, VJ2:47-50):
@@ -184,14 +193,15 @@ __device__ inline void jet_product_reverse(const M* e, const M* a, M* r) {
 template <int D, int ACT, typename M>
 __device__ inline void act_jet_forward(const M* u, M* y) {
   using L = Jet2Layout<D>;
+  constexpr int S = L::S;
   M f0, f1, f2;
   Act<ACT, M>::eval(u[0], f0, f1, f2);
   y[0] = f0;
 #pragma unroll
-  for (int a = 0; a < D; ++a) y[1 + a] = f1 * u[1 + a];
+  for (int a = 0; a < S; ++a) y[1 + a] = f1 * u[1 + a];
 #pragma unroll
   for (int p = 0; p < L::P; ++p)
-    y[1 + D + p] = f1 * u[1 + D + p] + f2 * u[1 + L::pi(p)] * u[1 + L::pj(p)];
+    y[1 + S + p] = f1 * u[1 + S + p] + f2 * u[1 + L::pi(p)] * u[1 + L::pj(p)];
 }
 
 // Transpose of act_jet_forward w.r.t. the pre-activation jet u given the
@@ -199,22 +209,23 @@ __device__ inline void act_jet_forward(const M* u, M* y) {
 template <int D, int ACT, typename M>
 __device__ inline void act_jet_reverse(const M* u, const M* a, M* b) {
   using L = Jet2Layout<D>;
+  constexpr int S = L::S;
   M f1, f2, f3;
   Act<ACT, M>::eval3(u[0], f1, f2, f3);
   M bv = a[0] * f1;
 #pragma unroll
-  for (int c = 0; c < D; ++c) {
+  for (int c = 0; c < S; ++c) {
     bv += f2 * a[1 + c] * u[1 + c];
     b[1 + c] = f1 * a[1 + c];
   }
 #pragma unroll
   for (int p = 0; p < L::P; ++p) {
     const int i = L::pi(p), j = L::pj(p);
-    const M ah = a[1 + D + p];
-    bv += ah * (f2 * u[1 + D + p] + f3 * u[1 + i] * u[1 + j]);
+    const M ah = a[1 + S + p];
+    bv += ah * (f2 * u[1 + S + p] + f3 * u[1 + i] * u[1 + j]);
     b[1 + i] += f2 * ah * u[1 + j];
     b[1 + j] += f2 * ah * u[1 + i];
-    b[1 + D + p] = f1 * ah;
+    b[1 + S + p] = f1 * ah;
   }
   b[0] = bv;
 }

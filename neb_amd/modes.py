@@ -27,9 +27,11 @@ the cotangents through its output layer, lift and hidden layers.
 Parameters are stored mode-stacked, W_j [K, out, in] and b_j [K, out] (the
 NCC:173-185 checkpoint layout; arrays() / from_arrays() give the 2D
 HardcodedModel key layout of HB2P:1596-1622, 1987-2033).  Adam clips each
-mode's gradient by its own norm (NCC:156-171).  first-order jets
-(forward_stream_jet1) run on the second-order kernels, the Hessian channels
-computed and dropped.
+mode's gradient by its own norm (NCC:156-171).  With the 2D streamfunction
+lift the blocks run on first-order jets (value and gradient channels, the
+Jet1 stream of HB2P:1716-1800: the jet kernels' NEB_DIM_2D_JET1 code), half
+the channels of the second-order tapes; output_jets evaluates second-order
+jets, and forward_stream_jet1 drops their Hessian channels.
 """
 from __future__ import annotations
 
@@ -72,6 +74,25 @@ class ModeConfig:
 
     def omega(self, l: int) -> float:
         return self.omega_first if l == 0 else self.omega_hidden
+
+    @property
+    def jet_order(self) -> int:
+        """1 for the 2D streamfunction lift, whose fields and reverse read
+        only the value and gradient channels (the reference evaluates it on
+        the Jet1 stream, HB2P:1716-1800); 2 otherwise."""
+        return 1 if self.dim == 2 and self.stiffness == "stream" else 2
+
+    def jet_channels(self, order: int | None = None) -> int:
+        return 1 + self.dim if (order or self.jet_order) == 1 else nw.channels(self.dim)
+
+    def jet_dim(self, order: int | None = None) -> int:
+        """The dim code of the jet-layer kernels for these jets."""
+        return H.DIM_2D_JET1 if (order or self.jet_order) == 1 else self.dim
+
+    def lift_code(self, order: int | None = None) -> int:
+        """The lift code of the output-layer kernels for these jets."""
+        lift = self.block_net_config(8).lift
+        return H.LIFT_2D_STREAM_JET1 if (order or self.jet_order) == 1 else lift
 
     def block_net_config(self, B: int) -> nw.NetConfig:
         """The trunk configuration one block of B modes presents to the jet
@@ -186,21 +207,25 @@ class ModeNet:
         return out
 
     # -- forward -----------------------------------------------------------
-    def features(self, x: torch.Tensor, prec: str):
+    def features(self, x: torch.Tensor, prec: str, order: int | None = None):
         """Feature jets shared by every mode: (hi, lo) planes for split
-        precision, else (X0, None)."""
+        precision, else (X0, None); first-order jets (order 1, the default
+        for the streamfunction lift) are the first 1 + dim channels."""
         import ctypes
         cfg = self.cfg
         split = prec == "bf16x3"
         p = H.F32 if split else nw.PRECISIONS[prec]
         xdt = torch.float64 if p == H.F64 else torch.float32
         x = x.to(device=self.device, dtype=xdt).contiguous()
-        n, C, F = x.shape[0], nw.channels(cfg.dim), cfg.features
+        n, F = x.shape[0], cfg.features
         lo = H.dbl_array([float(v) for v in cfg.lo])
         hi = H.dbl_array([float(v) for v in cfg.hi])
-        X0 = torch.empty((C, n, nw._ld(F)), dtype=nw._STORE[p] if not split else torch.float32, device=self.device)
+        X0 = torch.empty((nw.channels(cfg.dim), n, nw._ld(F)), dtype=nw._STORE[p] if not split else torch.float32,
+                         device=self.device)
         H.call("neb_feature_jets", p, cfg.dim, n, x.data_ptr(), cfg.frequencies, ctypes.addressof(lo),
                ctypes.addressof(hi), X0.data_ptr(), X0.shape[2], nw._stream())
+        C = cfg.jet_channels(order)
+        X0 = X0[:C]
         if not split:
             return X0, None
         Xh = torch.empty(X0.shape, dtype=torch.bfloat16, device=self.device)
@@ -211,13 +236,14 @@ class ModeNet:
 
     def block_tape(self, feats, m0: int, B: int, prec: str) -> nw.Tape:
         """Hidden jets of modes m0 .. m0+B-1 (the block as one trunk of width
-        B*hidden)."""
+        B*hidden), of the order of the feature jets `feats`."""
         cfg = self.cfg
         bc = cfg.block_net_config(B)
         stg = self.block_weights(m0, B, prec)
         X0, X0lo = feats
         n = X0.shape[1]
-        C = nw.channels(cfg.dim)
+        C = X0.shape[0]
+        kd = H.DIM_2D_JET1 if C == 1 + cfg.dim else cfg.dim
         s = nw._stream()
         split = prec == "bf16x3"
         p = H.BF16 if split else nw.PRECISIONS[prec]
@@ -236,18 +262,18 @@ class ModeNet:
                         stg["Wlo"][l].data_ptr(), stg["W"][l].shape[1], stg["b"][l].data_ptr(), bc.omega(l),
                         Z.data_ptr(), None, Y.data_ptr(), Yl.data_ptr(), Y.shape[2], s)
                 if l == 0:
-                    H.call("neb_jet_layer_forward_x3", cfg.dim, H.SIN, n, k, width, *args)
+                    H.call("neb_jet_layer_forward_x3", kd, H.SIN, n, k, width, *args)
                 else:
-                    H.call("neb_jet_layer_forward_x3_blocks", cfg.dim, H.SIN, n, width, h, *args)
+                    H.call("neb_jet_layer_forward_x3_blocks", kd, H.SIN, n, width, h, *args)
                 tape.inputs_lo.append(Yl)
                 cur_lo = Yl
             else:
                 args = (cur.data_ptr(), cur.shape[2], stg["W"][l].data_ptr(), stg["W"][l].shape[1],
                         stg["b"][l].data_ptr(), bc.omega(l), Z.data_ptr(), Y.data_ptr(), Y.shape[2], s)
                 if l == 0:
-                    H.call("neb_jet_layer_forward", p, cfg.dim, H.SIN, n, k, width, *args)
+                    H.call("neb_jet_layer_forward", p, kd, H.SIN, n, k, width, *args)
                 else:
-                    H.call("neb_jet_layer_forward_blocks", p, cfg.dim, H.SIN, n, width, h, *args)
+                    H.call("neb_jet_layer_forward_blocks", p, kd, H.SIN, n, width, h, *args)
             tape.pre.append(Z)
             tape.inputs.append(Y)
             cur, k = Y, width
@@ -340,7 +366,7 @@ class ModeNet:
         cfg = self.cfg
         if prec == "bf16x3":
             raise ValueError("raw output jets come from the bf16 / f32 / f64 forward")
-        feats = self.features(x, prec)
+        feats = self.features(x, prec, order=2)
         n = feats[0].shape[1]
         nout = cfg.modes * cfg.ncomp
         C = nw.channels(cfg.dim)
@@ -359,7 +385,7 @@ class ModeNet:
 
     def forward_stream_jet1(self, x: torch.Tensor, prec: str = "f32"):
         """First-order jets of every mode (value, grad): the Jet1 stream of
-        HB2P:1716-1800, evaluated on the second-order kernels."""
+        HB2P:1716-1800 (from output_jets, the Hessian channels dropped)."""
         v, g, _ = self.output_jets(x, prec)
         return v, g
 
@@ -382,7 +408,7 @@ class ModeNet:
         recomputed."""
         cfg = self.cfg
         K = cfg.modes
-        d, C = cfg.dim, nw.channels(cfg.dim)
+        d, C = cfg.jet_dim(), cfg.jet_channels()
         feats = self.features(x, prec)
         n = feats[0].shape[1]
         gdt = torch.float64 if prec == "f64" else torch.float32
@@ -417,7 +443,7 @@ class ModeNet:
             A = torch.empty((C, n, nw._ld(width)), dtype=st, device=self.device)
             Z = tape.pre[-1]
             off = m0 * GU.element_size()
-            H.call("neb_jet_output_backward_blocks", p, bc.lift, H.SIN, n, width, h, GU.data_ptr() + off,
+            H.call("neb_jet_output_backward_blocks", p, cfg.lift_code(), H.SIN, n, width, h, GU.data_ptr() + off,
                    GD.data_ptr() + off, ldg, nw._DT[gdt], 0, bm.data_ptr(), bm.data_ptr(), width,
                    Z.data_ptr(), Z.shape[2], bc.omega(L - 1), A.data_ptr(), A.shape[2], s)
             # output weights: the block's cotangent columns against its last hidden jets

@@ -46,7 +46,7 @@ def test_argument_errors_raise_without_touching_the_gpu():
     # split fields: bad dimension, split Gram: null lo plane, reverse: fields
     # dtype that does not fit the precision
     with pytest.raises(ValueError):
-        H.call("neb_jet_output_fields_split", 6, 8, 8, 8, 16, 8, 16, 8, 16, 16, 16, 16, 8, 0, None)
+        H.call("neb_jet_output_fields_split", 7, 8, 8, 8, 16, 8, 16, 8, 16, 16, 16, 16, 8, 0, None)
     # centering: a lo plane needs bf16 planes, a row stride below the width
     with pytest.raises(ValueError):
         H.call("neb_center_columns", H.F32, 16, 16, 8, 8, 8, 16, None)

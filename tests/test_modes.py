@@ -379,3 +379,73 @@ def test_output_backward_blocks_equals_dense(lift, dim, rm, rs, nc):
             outs.append(A[:, :, :wd].double().cpu())
         tol = {"bf16": 1.5e-2, "f32": 1e-6, "f64": 1e-13}[prec]
         r = rel(outs[1], outs[0]); assert r < tol, (prec, r)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "f32", "f64"])
+def test_first_order_layers_equal_second_order_channels(prec):
+    """The jet layers on 2D first-order jets (dim code NEB_DIM_2D_JET1:
+    value and gradient channels) reproduce the value and gradient channels of
+    the second-order layers: the forward (dense and block-diagonal, and the
+    split-precision one) never mixes the Hessian channels into them, and the
+    reverse with zero Hessian adjoints adds only zeros."""
+    from neb_amd import _hip as H
+    g = torch.Generator(device="cpu").manual_seed(5)
+    p = {"bf16": H.BF16, "f32": H.F32, "f64": H.F64}[prec]
+    st = nw._STORE[p]
+    n, k, h, B = 513, 64, 32, 4
+    wd = B * h
+    ld = nw._ld(max(k, wd))
+    s = torch.cuda.current_stream().cuda_stream
+    X = torch.zeros((6, n, ld), dtype=torch.float64)
+    X[:, :, :k] = torch.randn((6, n, k), generator=g, dtype=torch.float64)
+    W = torch.zeros((wd, ld), dtype=torch.float64)
+    W[:, :k] = torch.randn((wd, k), generator=g, dtype=torch.float64) / math.sqrt(k)
+    b = torch.randn(wd, generator=g, dtype=torch.float64).to(DEV, torch.float64 if p == H.F64 else torch.float32)
+    Xd, Wd = X.to(DEV, st), W.to(DEV, st)
+    outs = {}
+    for dim, C in ((2, 6), (H.DIM_2D_JET1, 3)):
+        Z = torch.zeros((C, n, ld), dtype=st, device=DEV)
+        Y = torch.zeros_like(Z)
+        H.call("neb_jet_layer_forward", p, dim, H.SIN, n, k, wd, Xd.data_ptr(), ld, Wd.data_ptr(), ld, b.data_ptr(),
+               1.3, Z.data_ptr(), Y.data_ptr(), ld, s)
+        outs[dim] = (Z, Y)
+    for i in range(2):
+        assert torch.equal(outs[2][i][:3], outs[H.DIM_2D_JET1][i]), (prec, "forward", i)
+    # block-diagonal hidden layer on those jets, and its reverse
+    Wb = torch.zeros((wd, ld), dtype=torch.float64)
+    for m in range(B):
+        Wb[m * h:(m + 1) * h, m * h:(m + 1) * h] = torch.randn((h, h), generator=g, dtype=torch.float64) / math.sqrt(h)
+    Wbd, WbT = Wb.to(DEV, st), torch.zeros((wd, ld), dtype=torch.float64)
+    WbT[:, :wd] = Wb[:, :wd].T
+    WbT = WbT.to(DEV, st)
+    A = torch.zeros((6, n, ld), dtype=torch.float64)
+    A[:3, :, :wd] = torch.randn((3, n, wd), generator=g, dtype=torch.float64)
+    Ad = A.to(DEV, st)
+    res = {}
+    for dim, C in ((2, 6), (H.DIM_2D_JET1, 3)):
+        Y0 = outs[dim][1]
+        Z = torch.zeros((C, n, ld), dtype=st, device=DEV)
+        Y = torch.zeros_like(Z)
+        H.call("neb_jet_layer_forward_blocks", p, dim, H.SIN, n, wd, h, Y0.data_ptr(), ld, Wbd.data_ptr(), ld,
+               b.data_ptr(), 0.9, Z.data_ptr(), Y.data_ptr(), ld, s)
+        An = torch.zeros((C, n, ld), dtype=st, device=DEV)
+        H.call("neb_jet_layer_backward_blocks", p, dim, H.SIN, n, wd, h, Ad[:C].contiguous().data_ptr(), ld,
+               WbT.data_ptr(), ld, outs[dim][0].data_ptr(), ld, 1.3, An.data_ptr(), ld, s)
+        res[dim] = (Z, Y, An)
+    for i, name in enumerate(("Z", "Y", "reverse")):
+        a, c = res[2][i][:3].double().cpu(), res[H.DIM_2D_JET1][i].double().cpu()
+        r = rel(c, a); assert r < (1e-14 if p == H.F64 else 1e-6), (prec, name, r)
+    if p == H.BF16:   # the split-precision forward
+        hi, lo = X.to(torch.bfloat16), (X - X.to(torch.bfloat16).double()).to(torch.bfloat16)
+        whi, wlo = W.to(torch.bfloat16), (W - W.to(torch.bfloat16).double()).to(torch.bfloat16)
+        hi, lo, whi, wlo = hi.to(DEV), lo.to(DEV), whi.to(DEV), wlo.to(DEV)
+        ys = {}
+        for dim, C in ((2, 6), (H.DIM_2D_JET1, 3)):
+            Z = torch.zeros((C, n, ld), dtype=torch.bfloat16, device=DEV)
+            Yh, Yl = torch.zeros_like(Z), torch.zeros_like(Z)
+            H.call("neb_jet_layer_forward_x3", dim, H.SIN, n, k, wd, hi[:C].contiguous().data_ptr(),
+                   lo[:C].contiguous().data_ptr(), ld, whi.data_ptr(), wlo.data_ptr(), ld, b.data_ptr(), 1.3,
+                   Z.data_ptr(), None, Yh.data_ptr(), Yl.data_ptr(), ld, s)
+            ys[dim] = (Z, Yh, Yl)
+        for i in range(3):
+            assert torch.equal(ys[2][i][:3], ys[H.DIM_2D_JET1][i]), ("x3", i)
