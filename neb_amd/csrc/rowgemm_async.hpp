@@ -56,12 +56,16 @@ struct RowAsyncShape {
   static constexpr int FB = Plan::FB, PB = Plan::PB, WPN = Plan::WPN, WPP = Plan::WPP;
 };
 template <int D> struct WideHiddenShape {
-  static constexpr int FB = D == 2 ? 4 : 2, PB = 1, WPN = 4, WPP = 2;
+  static constexpr int FB = D == 3 ? 2 : 4, PB = 1, WPN = 4, WPP = 2;
 };
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenFwd<D, ACT>> : WideHiddenShape<D> {};
 // 2D hidden layers on split operands: 8 waves on a 128 x 32 tile (32
 // features x 16 points each), three 40 KB slots
 template <int ACT> struct RowAsyncShape<PlanHiddenFwdSplit<2, ACT>> {
+  static constexpr int FB = 2, PB = 1, WPN = 4, WPP = 2;
+};
+// the same tile on first-order jets (six input planes): four 28 KB slots
+template <int ACT> struct RowAsyncShape<PlanHiddenFwdSplit<JET1_2D, ACT>> {
   static constexpr int FB = 2, PB = 1, WPN = 4, WPP = 2;
 };
 // 2D output layer on split operands: 8 waves on a 256 x 32 tile (64 features
