@@ -56,7 +56,7 @@ struct RowAsyncShape {
   static constexpr int FB = Plan::FB, PB = Plan::PB, WPN = Plan::WPN, WPP = Plan::WPP;
 };
 template <int D> struct WideHiddenShape {
-  static constexpr int FB = D == 3 ? 2 : 4, PB = 1, WPN = 4, WPP = 2;
+  static constexpr int FB = D == 2 ? 4 : 2, PB = 1, WPN = 4, WPP = 2;
 };
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenFwd<D, ACT>> : WideHiddenShape<D> {};
 // 2D hidden layers on split operands: 8 waves on a 128 x 32 tile (32
@@ -64,9 +64,16 @@ template <int D, int ACT> struct RowAsyncShape<PlanHiddenFwd<D, ACT>> : WideHidd
 template <int ACT> struct RowAsyncShape<PlanHiddenFwdSplit<2, ACT>> {
   static constexpr int FB = 2, PB = 1, WPN = 4, WPP = 2;
 };
-// the same tile on first-order jets (six input planes): four 28 KB slots
+// on first-order jets (six input planes): 8 waves on a 128 x 64 tile (32
+// features x 32 points each), three 40 KB slots (per-mode 2D blocks: a
+// 4,096-wide layer over 4,096 points 172 -> 155 us against 128 x 32 tiles)
 template <int ACT> struct RowAsyncShape<PlanHiddenFwdSplit<JET1_2D, ACT>> {
-  static constexpr int FB = 2, PB = 1, WPN = 4, WPP = 2;
+  static constexpr int FB = 2, PB = 2, WPN = 4, WPP = 2;
+};
+// and the first-order reverse on 4-wave 64 x 32 tiles, four 10 KB slots
+// (15.9 -> 15.6 ms per 2D per-mode step against the 8-wave 128 x 32 tile)
+template <int ACT> struct RowAsyncShape<PlanHiddenBwd<JET1_2D, ACT>> {
+  static constexpr int FB = 2, PB = 1, WPN = 2, WPP = 2;
 };
 // 2D output layer on split operands: 8 waves on a 256 x 32 tile (64 features
 // x 16 points each), three 48 KB slots (four waves of the plan's own 128 x 32
