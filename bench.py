@@ -55,10 +55,12 @@ def parse():
                     help="config B headline step: bf16x3 = split-precision jets and fields (bf16 hi + lo planes, "
                          "three bf16 MFMA products per term; Grams within 1e-5 of the FP64 path), bf16 = plain "
                          "BF16 jets (Grams within ~5e-3); the other one is reported beside it")
-    ap.add_argument("--config", default="B", choices=["B", "D", "E"],
+    ap.add_argument("--config", default="B", choices=["B", "D", "E", "modes2d", "modes3d"],
                     help="B: 2D 256^2 grid K=1024 (default); D: 3D blob domain, 1,599,754 points, K=512, "
                          "forward + Grams + FP64 generalized eigendecomposition; E: the same points, K=1024, "
-                         "training step + 20-target full-mass projection")
+                         "training step + 20-target full-mass projection; modes2d / modes3d: only the "
+                         "per-mode SIREN legs of the default run (the reference's 2D periodic potential and 3D "
+                         "architecture updates), one JSON line each (profiler runs)")
     return ap.parse_args()
 
 
@@ -883,7 +885,7 @@ def galerkin_bench(nw, net, x, w):
     return dict(rollout_rhs=rhs, convective_tensor=quad)
 
 
-def modes_bench(nw):
+def modes_bench(nw, with_f32=True):
     """The reference's own architecture at its published K = 1024 shape: K
     per-mode sine networks (hidden 128 x 5, omegas 40 / 36), 3D vector
     potential with second-order jets, 6,144 blob-domain points, one update =
@@ -911,7 +913,7 @@ def modes_bench(nw):
     G64 = nw.field_grams(ref.fields(x, "f64"), w, "f64")[0]
     del ref
     torch.cuda.empty_cache()
-    for prec, reps in (("bf16x3", 2), ("f32", 1)):
+    for prec, reps in (("bf16x3", 2), ("f32", 1))[:2 if with_f32 else 1]:
         net = md.ModeNet(cfg, seed=0, device=dev, master_dtype=torch.float32)
         res = net.step(x, w, prec=prec, objective="peel", peel_options=opts)     # warm-up: staging, workspaces
         err = float(torch.linalg.norm(res["M"] - G64) / torch.linalg.norm(G64))
@@ -929,7 +931,8 @@ def modes_bench(nw):
                          peel_gate="passed" if int(res["peel"]["gate"].item()) == 0 else "failed")
         del net, res
         torch.cuda.empty_cache()
-    out["bf16x3"]["vs_f32"] = out["f32"]["ms_per_update"] / out["bf16x3"]["ms_per_update"]
+    if with_f32:
+        out["bf16x3"]["vs_f32"] = out["f32"]["ms_per_update"] / out["bf16x3"]["ms_per_update"]
     out["reference_published_s_per_update"] = 2.698 + 8.952 + 0.266
     out["note"] = ("K=1024 per-mode SIRENs 5x128, 3D Jet2, 6,144 points, mode blocks of 16 as grouped GEMMs (each "
                    "tile contracting its own mode's inputs); the published reference figure is another GPU "
@@ -937,7 +940,7 @@ def modes_bench(nw):
     return out
 
 
-def modes_stream_bench(nw):
+def modes_stream_bench(nw, with_f32=True):
     """The reference's 2D periodic potential configuration
     (experiments/periodic/configs/periodic-potential-k1024-seed0.json): K =
     1024 per-mode sine networks (hidden 32 x 4, sin/cos frequencies 1..6 = 24
@@ -967,7 +970,7 @@ def modes_stream_bench(nw):
     ray64 = r64["peel"]["losses"]
     del ref, r64
     torch.cuda.empty_cache()
-    for prec, reps in (("bf16x3", 5), ("f32", 3)):
+    for prec, reps in (("bf16x3", 5), ("f32", 3))[:2 if with_f32 else 1]:
         net = md.ModeNet(cfg, seed=0, device=dev, master_dtype=torch.float32)
         res = net.step(x, w, prec=prec, objective="peel", peel_options=opts)     # warm-up: staging, workspaces
         err = float(torch.linalg.norm(res["peel"]["losses"] - ray64) / torch.linalg.norm(ray64))
@@ -988,7 +991,8 @@ def modes_stream_bench(nw):
                          tape_blocks_kept=int(res["tapes_kept"]), peel_gate="passed" if gate == 0 else "failed")
         del net, res
         torch.cuda.empty_cache()
-    out["bf16x3"]["vs_f32"] = out["f32"]["ms_per_update"] / out["bf16x3"]["ms_per_update"]
+    if with_f32:
+        out["bf16x3"]["vs_f32"] = out["f32"]["ms_per_update"] / out["bf16x3"]["ms_per_update"]
     out["note"] = ("periodic-potential-k1024 shape: K=1024 per-mode SIRENs 4x32, 4,096 points, Jet1 streamfunction "
                    "fields (computed on the second-order kernels), Dirichlet streamfunction peel + per-mode Adam, "
                    "mode blocks of 128; the reference publishes no time for this configuration")
@@ -1045,6 +1049,11 @@ def main():
         return
     if args.config == "D":
         bench_3d_eig(args)
+        return
+    if args.config in ("modes2d", "modes3d"):
+        from neb_amd import network as nw
+        leg = (modes_stream_bench if args.config == "modes2d" else modes_bench)(nw, with_f32=not args.no_f32)
+        print(json.dumps({"config": args.config, **leg}), flush=True)
         return
     if args.kernel:
         bench_kernel(args)
