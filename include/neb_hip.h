@@ -52,7 +52,7 @@
 extern "C" {
 #endif
 
-#define NEB_ABI_VERSION 15
+#define NEB_ABI_VERSION 16
 
 enum { NEB_OK = 0, NEB_ERR_SHAPE = 1, NEB_ERR_NONFINITE = 2, NEB_ERR_HIP = 3, NEB_ERR_ARG = 4 };
 enum { NEB_F32 = 0, NEB_BF16 = 1, NEB_F64 = 2 };             /* prec / dtype codes */
@@ -249,6 +249,22 @@ int neb_cross_blocks(int mode, int nseg, const void* const* x, const void* const
                      const int64_t* ldy, const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t m,
                      int64_t block, double scale, double beta, double* G, int64_t ldg, void* workspace,
                      size_t ws_bytes, void* stream);
+
+/* Mode-diagonal crosses: out[m][j] = beta*out[m][j] + scale * sum_s sign_s
+ * sum_r x_s[r][m] y_s[r][m*h + j], m < B, j < h (out: FP64 [B][h], row
+ * stride h) -- the diagonal 1 x h blocks of neb_cross's B x B*h result,
+ * i.e. each mode's output-weight products (a block's field-cotangent column
+ * m against its own h last-hidden features), without the products against
+ * the block's other modes.  x_s: [rows_s][ldx_s] (F32 / F64, ldx_s >= B);
+ * y_s: [rows_s][ldy_s] (BF16 / F32 / F64, 16-byte aligned rows, ldy_s >=
+ * B*h); h a multiple of 8; up to 16 segments.  FP64 accumulation in a fixed
+ * order (deterministic).  Replaces the per-mode `cp.matmul(gu.T, y)` output
+ * weight gradients of HB2P:2087-2139.  Workspace:
+ * neb_cross_modes_workspace_size(B, h, max rows). */
+size_t neb_cross_modes_workspace_size(int64_t B, int64_t h, int64_t rows);
+int neb_cross_modes(int nseg, const void* const* x, const void* const* y, const int64_t* ldx, const int64_t* ldy,
+                    const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t B, int64_t h,
+                    double scale, double beta, double* out, void* workspace, size_t ws_bytes, void* stream);
 
 /* The output layer + lift of a block of B per-mode networks (hidden width h)
  * on split operands: the fields of neb_jet_output_fields_x3 for the block's

@@ -13,7 +13,7 @@ import ctypes as C
 import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libnebhip.so")
-ABI_VERSION = 15
+ABI_VERSION = 16
 
 F32, BF16, F64 = 0, 1, 2
 TANH, SIN = 0, 1
@@ -73,6 +73,9 @@ SIGNATURES = {
     "neb_gram_split": (_int, [_vp, _vp, _i64, _i64, _i64, _dbl, _dbl, _vp, _i64, _vp, _sz, _vp]),
     "neb_gram_split_noalloc": (_int, [_vp, _vp, _i64, _i64, _i64, _dbl, _dbl, _vp, _i64, _vp]),
     "neb_set_cross_occupancy": (_int, [_int]),
+    "neb_cross_modes_workspace_size": (_sz, [_i64, _i64, _i64]),
+    "neb_cross_modes": (_int, [_int, _vp, _vp, _vp, _vp, _vp, _vp, _int, _int, _i64, _i64, _dbl, _dbl, _vp, _vp, _sz,
+                               _vp]),
     "neb_colsum_workspace_size": (_sz, [_i64, _i64]),
     "neb_colsum": (_int, [_int, _vp, _i64, _i64, _i64, _dbl, _vp, _vp, _sz, _vp]),
     "neb_sumsq": (_int, [_int, _vp, _i64, _vp, _vp, _sz, _vp]),

@@ -113,6 +113,64 @@ __global__ __launch_bounds__(256) void colsum_kernel(const S* __restrict__ x, in
   }
 }
 
+// Mode-diagonal crosses: out[m][j] = sum_s sign_s sum_r x_s[r][m] y_s[r][m h + j]
+// (the diagonal 1 x h blocks of sum_s sign_s X_s^T Y_s for a [B, B h] result:
+// each mode's output-weight products against its own hidden features).
+// Thread = 8 consecutive columns (one mode's) over the rows of one chunk
+// (blockIdx.x), FP64 accumulation; partial[chunk][B h] is then summed over
+// the chunks in fixed order (sum_splits_kernel).
+constexpr int MC_MAXSEG = 16;
+struct ModeCrossArgs {
+  const void* x[MC_MAXSEG];
+  const void* y[MC_MAXSEG];
+  int64_t ldx[MC_MAXSEG], ldy[MC_MAXSEG], rows[MC_MAXSEG];
+  double sign[MC_MAXSEG];
+  int nseg;
+  int64_t B, h, chunk;
+  double scale;
+  double* partial;
+};
+
+// 8 consecutive elements (16-byte aligned) as FP64
+template <typename T> __device__ inline void ld8_f64(const T* src, double* v) {
+  constexpr int PV = 16 / sizeof(T);
+  typedef T pv __attribute__((ext_vector_type(PV)));
+#pragma unroll
+  for (int c = 0; c < 8; c += PV) {
+    const pv t = *reinterpret_cast<const pv*>(src + c);
+#pragma unroll
+    for (int e = 0; e < PV; ++e) v[c + e] = todbl_any(t[e]);
+  }
+}
+
+template <typename TX, typename TY>
+__global__ __launch_bounds__(256) void mode_cross_kernel(ModeCrossArgs a) {
+  const int64_t ncol = a.B * a.h;
+  const int64_t g = (int64_t)blockIdx.y * blockDim.x + threadIdx.x;
+  if (g * 8 >= ncol) return;
+  const int64_t c0 = g * 8, m = c0 / a.h;
+  const int64_t r0 = (int64_t)blockIdx.x * a.chunk;
+  double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int s = 0; s < a.nseg; ++s) {
+    const TX* xs = reinterpret_cast<const TX*>(a.x[s]);
+    const TY* ys = reinterpret_cast<const TY*>(a.y[s]);
+    const int64_t r1 = r0 + a.chunk < a.rows[s] ? r0 + a.chunk : a.rows[s];
+    double sa[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t r = r0; r < r1; ++r) {
+      const double xv = todbl_any(xs[r * a.ldx[s] + m]);
+      double yv[8];
+      ld8_f64<TY>(ys + r * a.ldy[s] + c0, yv);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) sa[e] = fma(xv, yv[e], sa[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) acc[e] = fma(a.sign[s], sa[e], acc[e]);
+  }
+  double* o = a.partial + (int64_t)blockIdx.x * ncol + c0;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) o[e] = a.scale * acc[e];
+}
+
 // out[c] = sum_s partial[s][c] in fixed order
 __global__ __launch_bounds__(256) void sum_splits_kernel(const double* __restrict__ partial, int splits,
                                                          int64_t cols, double* __restrict__ out, double beta) {

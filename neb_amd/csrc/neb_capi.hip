@@ -9,6 +9,7 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <utility>
 
 #include "../../include/neb_hip.h"
 #include "cross_async.hpp"
@@ -672,10 +673,71 @@ int scratch_address(void** out) {
   return NEB_OK;
 }
 
+// Row chunks of neb_cross_modes: about 1,024 workgroups over the chunks and
+// the column groups, at least 8 rows each.  Returns (chunks, rows per chunk).
+std::pair<int64_t, int64_t> mode_cross_chunks(int64_t B, int64_t h, int64_t rows) {
+  const int64_t gy = (B * h / 8 + 255) / 256;
+  const int64_t want = gy >= 1024 ? 1 : 1024 / gy;
+  int64_t chunk = (rows + want - 1) / want;
+  if (chunk < 8) chunk = 8;
+  const int64_t n = rows > 0 ? (rows + chunk - 1) / chunk : 1;
+  return {n, chunk};
+}
+
 }  // namespace
 
 // ==========================================================================
 extern "C" {
+
+size_t neb_cross_modes_workspace_size(int64_t B, int64_t h, int64_t rows) {
+  return (size_t)mode_cross_chunks(B, h, rows).first * (size_t)(B * h) * sizeof(double);
+}
+
+int neb_cross_modes(int nseg, const void* const* x, const void* const* y, const int64_t* ldx, const int64_t* ldy,
+                    const int64_t* rows, const double* sign, int x_dtype, int y_dtype, int64_t B, int64_t h,
+                    double scale, double beta, double* out, void* workspace, size_t ws_bytes, void* stream) {
+  if (nseg < 1 || nseg > MC_MAXSEG) return fail(NEB_ERR_ARG, "neb_cross_modes: 1..%d segments", MC_MAXSEG);
+  if (B < 1 || h < 8 || h % 8) return fail(NEB_ERR_SHAPE, "neb_cross_modes: h must be a positive multiple of 8");
+  if (!x || !y || !ldx || !ldy || !rows || !sign || !out || !workspace)
+    return fail(NEB_ERR_ARG, "neb_cross_modes: null pointer");
+  const bool xok = x_dtype == NEB_F32 || x_dtype == NEB_F64;
+  const bool yok = y_dtype == NEB_BF16 || y_dtype == NEB_F32 || y_dtype == NEB_F64;
+  if (!xok || !yok) return fail(NEB_ERR_ARG, "neb_cross_modes: x dtype F32 / F64, y dtype BF16 / F32 / F64");
+  ModeCrossArgs a;
+  memset(&a, 0, sizeof(a));
+  int64_t rmax = 0;
+  for (int s = 0; s < nseg; ++s) {
+    if (rows[s] < 0 || ldx[s] < B || ldy[s] < B * h) return fail(NEB_ERR_SHAPE, "neb_cross_modes: segment %d sizes", s);
+    if (rows[s] > 0 && (!x[s] || !y[s] || !aligned(y[s], ldy[s], y_dtype)))
+      return fail(NEB_ERR_ARG, "neb_cross_modes: segment %d: null or unaligned (16-byte rows of y)", s);
+    a.x[s] = x[s]; a.y[s] = y[s]; a.ldx[s] = ldx[s]; a.ldy[s] = ldy[s]; a.rows[s] = rows[s]; a.sign[s] = sign[s];
+    rmax = rows[s] > rmax ? rows[s] : rmax;
+  }
+  const auto ch = mode_cross_chunks(B, h, rmax);
+  if (ws_bytes < (size_t)ch.first * (size_t)(B * h) * sizeof(double))
+    return fail(NEB_ERR_ARG, "neb_cross_modes: workspace too small");
+  a.nseg = nseg; a.B = B; a.h = h; a.chunk = ch.second; a.scale = scale;
+  a.partial = reinterpret_cast<double*>(workspace);
+  hipStream_t st = S(stream);
+  const int64_t gy = (B * h / 8 + 255) / 256;
+  if (gy > 65535) return fail(NEB_ERR_SHAPE, "neb_cross_modes: too many columns");
+  const dim3 grid((unsigned)ch.first, (unsigned)gy);
+#define MCK(TX, TY) hipLaunchKernelGGL((mode_cross_kernel<TX, TY>), grid, dim3(256), 0, st, a)
+  if (x_dtype == NEB_F32) {
+    if (y_dtype == NEB_BF16) MCK(float, __bf16);
+    else if (y_dtype == NEB_F32) MCK(float, float);
+    else MCK(float, double);
+  } else {
+    if (y_dtype == NEB_BF16) MCK(double, __bf16);
+    else if (y_dtype == NEB_F32) MCK(double, float);
+    else MCK(double, double);
+  }
+#undef MCK
+  if (int rc = hip_status("neb_cross_modes")) return rc;
+  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((B * h + 255) / 256)), dim3(256), 0, st, a.partial,
+                     (int)ch.first, B * h, out, beta);
+  return hip_status("neb_cross_modes reduce");
+}
 
 int neb_set_cross_occupancy(int max_workgroups) {
   if (max_workgroups < 0) return fail(NEB_ERR_ARG, "neb_set_cross_occupancy: max_workgroups must be >= 0");

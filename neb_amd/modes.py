@@ -448,8 +448,8 @@ class ModeNet:
                    Z.data_ptr(), Z.shape[2], bc.omega(L - 1), A.data_ptr(), A.shape[2], s)
             # output weights: the block's cotangent columns against its last hidden jets
             P = self._field_products(tape, cot, m0, B)
-            for c in range(nc):   # gW[-1][m0 + m, c] = P[c][m, m h:(m + 1) h]
-                gW[-1][m0:m0 + B, c].copy_(P[c].view(B, B, h).diagonal(dim1=0, dim2=1).T)
+            for c in range(nc):   # gW[-1][m0 + m, c] = P[c][m]
+                gW[-1][m0:m0 + B, c].copy_(P[c])
             # hidden layers, diagonal blocks of each weight gradient
             for l in range(L - 1, -1, -1):
                 X = tape.inputs[l]
@@ -478,36 +478,41 @@ class ModeNet:
         return grads
 
     def _field_products(self, tape: nw.Tape, cot: nw.Fields, m0: int, B: int) -> list:
-        """Per output component c: [B, B h], the block's columns m0 .. m0+B-1
-        of the field cotangents (mass rows of cot.mass, stiffness rows of
-        cot.stiff) against the block's last hidden jet channels that each
-        field reads (the lift's terms), summed over the points."""
+        """Per output component c: [B, h], mode m0+m's output-weight
+        products -- the block's field-cotangent column m (mass rows of
+        cot.mass, stiffness rows of cot.stiff) against mode m's own h last
+        hidden jet features of the channels each field reads (the lift's
+        terms), summed over the points (neb_cross_modes: no products against
+        the block's other modes)."""
         cfg = self.cfg
-        n = tape.n
-        width = B * cfg.hidden
-        mode = nw._CROSS[tape.prec]
+        n, h = tape.n, cfg.hidden
         Y = tape.inputs[-1]
         ch = lambda c: Y[c]  # noqa: E731
         col = lambda t: t[:, m0:m0 + B]  # noqa: E731
         U, Dg = cot.mass, cot.stiff
         rv = nw._rows_view
+
+        def run(segs):
+            P = nw.cross_modes(segs[:16], B, h)
+            for i in range(16, len(segs), 16):
+                nw.cross_modes(segs[i:i + 16], B, h, out=P, beta=1.0)
+            return P
+
         if cfg.dim == 2 and cfg.stiffness == "stream":
             # psi rows against the value channel (the cotangent columns are
             # mean-free, so the uncentered channel gives the centered product),
             # gradient rows against the gradient channels
-            P = nw.cross([(col(U), ch(0), n, 1.0), (col(rv(Dg, 0, 2)), ch(1), n, 1.0),
-                          (col(rv(Dg, 1, 2)), ch(2), n, 1.0)], B, width, mode)
-            return [P]
+            return [run([(col(U), ch(0), n, 1.0), (col(rv(Dg, 0, 2)), ch(1), n, 1.0),
+                         (col(rv(Dg, 1, 2)), ch(2), n, 1.0)])]
         if cfg.dim == 2:
             u, v = rv(U, 0, 2), rv(U, 1, 2)
-            P = nw.cross([(col(u), ch(2), n, 1.0), (col(v), ch(1), n, -1.0)], B, width, mode)
+            segs = [(col(u), ch(2), n, 1.0), (col(v), ch(1), n, -1.0)]
             if cfg.stiffness == "hessian":
-                P += nw.cross([(col(rv(Dg, 0, 3)), ch(3), n, 1.0), (col(rv(Dg, 2, 3)), ch(5), n, 1.0)], B, width,
-                              mode)
-                P += math.sqrt(2.0) * nw.cross([(col(rv(Dg, 1, 3)), ch(4), n, 1.0)], B, width, mode)
+                segs += [(col(rv(Dg, 0, 3)), ch(3), n, 1.0), (col(rv(Dg, 2, 3)), ch(5), n, 1.0),
+                         (col(rv(Dg, 1, 3)), ch(4), n, math.sqrt(2.0))]
             else:
-                P += nw.cross([(col(Dg), ch(3), n, -1.0), (col(Dg), ch(5), n, -1.0)], B, width, mode)
-            return [P]
+                segs += [(col(Dg), ch(3), n, -1.0), (col(Dg), ch(5), n, -1.0)]
+            return [run(segs)]
         from .lifts import curl_terms
         comps = [rv(U, a, 3) for a in range(3)]
         grads3 = [rv(Dg, c, 9) for c in range(9)]
@@ -516,10 +521,7 @@ class ModeNet:
             segs = [(col(comps[a]), ch(1 + c), n, sg) for (a, c, w_s, sg) in curl_terms() if a < 3 and w_s == sl]
             segs += [(col(grads3[a - 3]), ch(1 + c), n, sg) for (a, c, w_s, sg) in curl_terms()
                      if a >= 3 and w_s == sl]
-            P = nw.cross(segs[:8], B, width, mode)
-            for i in range(8, len(segs), 8):
-                nw.cross(segs[i:i + 8], B, width, mode, out=P, beta=1.0)
-            out.append(P)
+            out.append(run(segs))
         return out
 
     # -- the reference's step ---------------------------------------------

@@ -704,6 +704,27 @@ def cross(segments, m: int, n: int, mode: int, scale: float = 1.0, out=None, bet
     return out
 
 
+def cross_modes(segments, B: int, h: int, scale: float = 1.0, out=None, beta: float = 0.0):
+    """Mode-diagonal crosses (neb_cross_modes): out[m, j] = beta out[m, j] +
+    scale sum_s sign_s sum_r X_s[r, m] Y_s[r, m h + j], FP64 [B, h];
+    segments = [(X, Y, rows, sign)] as for cross (up to 16 per call)."""
+    xs = [s[0] for s in segments]
+    ys = [s[1] for s in segments]
+    if out is None:
+        out = torch.empty((B, h), dtype=torch.float64, device=xs[0].device)
+        beta = 0.0
+    if not out.is_contiguous() or tuple(out.shape) != (B, h) or out.dtype != torch.float64:
+        raise ValueError("cross_modes writes a contiguous FP64 [B, h] result")
+    rows = [int(s[2]) for s in segments]
+    ws = workspace(xs[0].device).get(H.load().neb_cross_modes_workspace_size(B, h, max(rows)))
+    H.call("neb_cross_modes", len(segments), H.ptr_array([t.data_ptr() for t in xs]),
+           H.ptr_array([t.data_ptr() for t in ys]), H.i64_array([t.stride(0) for t in xs]),
+           H.i64_array([t.stride(0) for t in ys]), H.i64_array(rows), H.dbl_array([float(s[3]) for s in segments]),
+           _DT[xs[0].dtype], _DT[ys[0].dtype], B, h, float(scale), float(beta), out.data_ptr(), ws.data_ptr(),
+           ws.numel(), _stream())
+    return out
+
+
 def _f64_rows(A: torch.Tensor) -> torch.Tensor:
     """A as an FP64 matrix with unit column stride (no copy when it is one)."""
     A = A.to(torch.float64)

@@ -449,3 +449,30 @@ def test_first_order_layers_equal_second_order_channels(prec):
             ys[dim] = (Z, Yh, Yl)
         for i in range(3):
             assert torch.equal(ys[2][i][:3], ys[H.DIM_2D_JET1][i]), ("x3", i)
+
+
+@pytest.mark.parametrize("xd,yd", [(torch.float32, torch.bfloat16), (torch.float32, torch.float32),
+                                   (torch.float64, torch.float64)])
+def test_cross_modes_matches_fp64_products(xd, yd):
+    """neb_cross_modes: out[m, j] = scale sum_s sign_s sum_r X_s[r, m] Y_s[r, m h + j]
+    against the FP64 products of the same stored values, on ragged row
+    counts, strided column views and accumulation into out (beta)."""
+    g = torch.Generator(device="cpu").manual_seed(3)
+    B, h, ld = 12, 16, 200
+    segs, ref = [], torch.zeros((B, h), dtype=torch.float64)
+    for s, rows in enumerate((777, 500, 777)):
+        Xf = torch.randn((rows, 40), generator=g, dtype=torch.float64)
+        Yf = torch.randn((rows, ld), generator=g, dtype=torch.float64)
+        X = Xf.to(DEV, xd)[:, 5:5 + B]                       # a column view (row stride 40)
+        Y = Yf.to(DEV, yd)
+        sign = (-1.0) ** s * (1.0 + s)
+        segs.append((X, Y, rows, sign))
+        Xs, Ys = X.double().cpu(), Y.double().cpu()
+        for m in range(B):
+            ref[m] += sign * (Xs[:, m:m + 1] * Ys[:, m * h:(m + 1) * h]).sum(0)
+    out = nw.cross_modes(segs, B, h, scale=0.5)
+    r = rel(out.cpu(), 0.5 * ref); assert r < 1e-12, r
+    nw.cross_modes(segs[:1], B, h, out=out, beta=2.0)
+    Xs, Ys = segs[0][0].double().cpu(), segs[0][1].double().cpu()
+    first = torch.stack([(Xs[:, m:m + 1] * Ys[:, m * h:(m + 1) * h]).sum(0) for m in range(B)])
+    r = rel(out.cpu(), ref + first); assert r < 1e-12, r
