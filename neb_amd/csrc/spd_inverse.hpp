@@ -28,8 +28,8 @@
 // holds the Schur complement S_k; with U_<k^T U_<k,k = A_<k,k and
 // U_kk^T U_kk = S_k, block column k of T is
 //   T_<k,k = -(A_<k^-1 A_<k,k) T_kk,   T_kk = U_kk^-1,
-// and T_kk is the upper factor of S_k^-1 = T_kk T_kk^T, which the pivot
-// workgroup factors (in reverse index order) right after inverting S_k.
+// and the pivot workgroup forms T_kk beside the inverse of S_k from the same
+// 4-column sweep (spd_pivot_regs).
 #pragma once
 #include "neb_common.hpp"
 
@@ -149,65 +149,6 @@ __device__ inline void spd_acc_store(double* M, const f64x4 (&v)[2][2], int R0, 
       for (int e = 0; e < 4; ++e) M[spd_row(R0, tr, lane, e) * SPD_LDP + spd_col(C0, tc, lane)] = v[tr][tc][e];
 }
 
-// Upper T with T T^T = B for the symmetric positive definite 64 x 64 B in
-// LDS (row stride SPD_LDP): B = V D V^T with V unit upper triangular, in
-// reverse index order (column j = 63 .. 0: d_j = b_jj, v_ij = b_ij / d_j for
-// i < j, b_im -= v_ij b_mj for i, m < j), then T = V D^1/2.  The square
-// roots wait until the end, off the column-to-column dependency chain.
-// Thread t holds the 4 x 4 blocks (t / 16, t % 16) of B (then V) in
-// registers; column j of the current B passes through LDS (cbuf, 2 x 64
-// doubles, alternating so that one barrier per column suffices).  Writes
-// T^T ([column][row], stride 64: the layout spd_fetch reads as the
-// right-hand MFMA operand RT) to TkT.  A B that is not positive definite
-// gives NaN or infinities.
-__device__ inline void spd_rev_chol(const double* Bl, double* cbuf, int t, double* __restrict__ TkT) {
-  const int bi = t >> 4, bm = t & 15;
-  double b[4][4], dv[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) b[r][c] = Bl[(4 * bi + r) * SPD_LDP + 4 * bm + c];
-  for (int jb = SPD_NB / 4 - 1; jb >= 0; --jb) {
-#pragma unroll
-    for (int jc = 3; jc >= 0; --jc) {
-      const int j = 4 * jb + jc;
-      double* col = cbuf + (jc & 1) * SPD_NB;
-      if (bm == jb) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) col[4 * bi + r] = b[r][jc];
-      }
-      __syncthreads();
-      const double dj = col[j];
-      const double inv = 1.0 / dj;
-      double vi[4], cm[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        vi[r] = col[4 * bi + r] * inv;
-        cm[r] = col[4 * bm + r];
-      }
-      if (bm == jb) {  // column j of B is done with: it becomes column j of V
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = 4 * bi + r;
-          b[r][jc] = i < j ? vi[r] : (i == j ? 1.0 : 0.0);
-        }
-        dv[jc] = dj;
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          if (4 * bi + r < j && 4 * bm + c < j) b[r][c] -= vi[r] * cm[c];
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double sd = sqrt(dv[c]);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) TkT[(4 * bm + c) * SPD_NB + 4 * bi + r] = b[r][c] * sd;
-  }
-}
-
 // Inversion of the 64 x 64 pivot block held in MFMA layout (v) by lazily
 // scaled sweeps over 4 x 4 pivot blocks.  Sweeping block b (columns
 // c = 4b .. c+3) with pivot block P (current values) is the rank-4 update
@@ -224,8 +165,16 @@ __device__ inline void spd_rev_chol(const double* Bl, double* cbuf, int t, doubl
 // double-buffered: one barrier per 4 columns.  lds: SPD_PIVOT_LDS doubles,
 // not in use by the caller.  Writes Pi = P^-1 (64 x 64, row stride 64).
 // info: the 1-based index k0 + i + 1 of the first nonpositive / nonfinite
-// pivot (first writer wins).  TkT (optional): T_kk^T, T_kk the upper factor
-// of Pi = T_kk T_kk^T (spd_rev_chol).
+// pivot (first writer wins).
+//
+// TkT (optional): T_kk^T for T_kk = U^-1, U^T U = the pivot block.  By the
+// identity of the header comment at 4-column granularity, column block b of
+// T_kk is [-(S_<b^-1 S_<b,b) T_bb ; T_bb] with T_bb the inverse upper
+// Cholesky factor of the 4 x 4 pivot P_b; with the pending row factors the
+// swept block (a, b) is P_a^-1 v_(a,b), so T_(a,b) = -P_a^-1 v_(a,b) T_bb.
+// Each step leaves its column block (the lazy v_(i,b), P_b on the diagonal)
+// in M; after the last step the 16 T_bb are factored side by side and the
+// blocks formed, off the sweep's dependency chain.
 __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int lane, int t, int64_t k0,
                                       double* __restrict__ Pi, int* __restrict__ info, double* __restrict__ TkT) {
   double* rowbuf = lds;
@@ -288,6 +237,10 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
         const int i = R0 + 16 * tr + r16;
         const f64x2 a01 = *reinterpret_cast<const f64x2*>(&cb[i * 4]);
         const f64x2 a23 = *reinterpret_cast<const f64x2*>(&cb[i * 4 + 2]);
+        if (TkT && g == 0 && (w & 1) == 0) {  // waves 0 and 2 cover the 64 rows: snapshot of column block b
+          *reinterpret_cast<f64x2*>(&M[i * SPD_LDP + c]) = a01;
+          *reinterpret_cast<f64x2*>(&M[i * SPD_LDP + c + 2]) = a23;
+        }
         const double xv = a01[0] * qc[0] + a01[1] * qc[1] + a23[0] * qc[2] + a23[1] * qc[3];
         x[tr] = xv * ((i >> 2) == b ? 0.0 : 1.0);
       }
@@ -310,13 +263,73 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
       }
     }
   }
+  if (TkT) {
+    __syncthreads();  // every column snapshot is in M
+    double* tb = rowbuf;  // T_bb of the 16 pivots, [b][4][4]
+    if (t < 16) {         // U^T U = P_b (upper U), T_bb = U^-1
+      const double* pb = M + 4 * t * SPD_LDP + 4 * t;
+      const double p00 = pb[0], p01 = pb[1], p02 = pb[2], p03 = pb[3];
+      const double p11 = pb[SPD_LDP + 1], p12 = pb[SPD_LDP + 2], p13 = pb[SPD_LDP + 3];
+      const double p22 = pb[2 * SPD_LDP + 2], p23 = pb[2 * SPD_LDP + 3], p33 = pb[3 * SPD_LDP + 3];
+      const double u00 = sqrt(p00), i0 = 1.0 / u00;
+      const double u01 = p01 * i0, u02 = p02 * i0, u03 = p03 * i0;
+      const double u11 = sqrt(p11 - u01 * u01), i1 = 1.0 / u11;
+      const double u12 = (p12 - u01 * u02) * i1, u13 = (p13 - u01 * u03) * i1;
+      const double u22 = sqrt(p22 - u02 * u02 - u12 * u12), i2 = 1.0 / u22;
+      const double u23 = (p23 - u02 * u03 - u12 * u13) * i2;
+      const double i3 = 1.0 / sqrt(p33 - u03 * u03 - u13 * u13 - u23 * u23);
+      const double t23 = -u23 * i3 * i2;
+      const double t12 = -u12 * i2 * i1, t13 = -(u12 * t23 + u13 * i3) * i1;
+      const double t01 = -u01 * i1 * i0, t02 = -(u01 * t12 + u02 * i2) * i0;
+      const double t03 = -(u01 * t13 + u02 * t23 + u03 * i3) * i0;
+      double* o = tb + 16 * t;
+      o[0] = i0;  o[1] = t01; o[2] = t02;  o[3] = t03;
+      o[4] = 0.0; o[5] = i1;  o[6] = t12;  o[7] = t13;
+      o[8] = 0.0; o[9] = 0.0; o[10] = i2;  o[11] = t23;
+      o[12] = 0.0; o[13] = 0.0; o[14] = 0.0; o[15] = i3;
+    }
+    __syncthreads();
+    // thread t: row i = t / 4 of T_kk, columns jg .. jg+15 (column blocks jb .. jb+3)
+    const int i = t >> 2, a = i >> 2, ir = i & 3, jg = (t & 3) * 16;
+    double pa[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) pa[q] = pivs[a * 16 + 4 * ir + q];  // row ir of P_a^-1
+#pragma unroll
+    for (int J = 0; J < 4; ++J) {
+      const int b = (jg >> 2) + J;
+      double tv[4];
+      if (b < a) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tv[q] = 0.0;
+      } else if (b == a) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tv[q] = tb[16 * b + 4 * ir + q];
+      } else {  // -(P_a^-1 v_(a,b) T_bb) row ir
+        double hv[4] = {0.0, 0.0, 0.0, 0.0};  // row ir of P_a^-1 v_(a,b)
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) hv[u] = fma(pa[s2], M[(4 * a + s2) * SPD_LDP + 4 * b + u], hv[u]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double acc = 0.0;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc = fma(hv[u], tb[16 * b + 4 * u + q], acc);
+          tv[q] = -acc;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) TkT[(4 * b + q) * SPD_NB + i] = tv[q];
+    }
+    __syncthreads();  // M is reused below
+  }
   // pending factors: T = diag(P^-1) v (rows) through LDS, then T diag(P^-1)
   // (columns) in registers; Pi = -T
   spd_acc_store(M, v, R0, C0, lane);
   __syncthreads();
   const int i = t >> 2, jg = (t & 3) * 16;
   const int I = i >> 2, ir = i & 3;
-  double li[4], pio[16];
+  double li[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) li[a] = pivs[I * 16 + 4 * ir + a];
   double tr_[16];
@@ -337,17 +350,8 @@ __device__ inline void spd_pivot_regs(f64x4 (&v)[2][2], double* lds, int w, int 
       for (int cq = 0; cq < 4; ++cq) o[cq] = fma(tr_[4 * J + bq], pivs[Jg * 16 + 4 * bq + cq], o[cq]);
     *reinterpret_cast<f64x2*>(&Pi[i * SPD_NB + jg + 4 * J]) = f64x2{-o[0], -o[1]};
     *reinterpret_cast<f64x2*>(&Pi[i * SPD_NB + jg + 4 * J + 2]) = f64x2{-o[2], -o[3]};
-#pragma unroll
-    for (int cq = 0; cq < 4; ++cq) pio[4 * J + cq] = -o[cq];
   }
   if (t == 0 && bad != 0) atomicCAS(info, 0, bad);
-  if (TkT) {
-    __syncthreads();  // every thread is done reading M
-#pragma unroll
-    for (int q = 0; q < 16; ++q) M[i * SPD_LDP + jg + q] = pio[q];
-    __syncthreads();
-    spd_rev_chol(M, rowbuf, t, TkT);
-  }
 }
 
 // First pivot: Pi = (A[0:64, 0:64], identity beyond K)^-1; clears info.
