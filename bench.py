@@ -444,9 +444,13 @@ def config_e(args, quick=False):
         gram_stats[mode] = dict(tflops=(12 * CHUNK3D) * K * (K + 1) / t / 1e12, ms=t * 1e3)
     del f0, s0, G, M0
 
+    def grams_for(prec):
+        # split-precision tapes: the field Grams from the lift-weighted channel Grams
+        return "factored" if prec == "bf16x3" else "fields"
+
     def step(prec, gmode, solv):
         tt = {}
-        res = sp.chunked_step(net, x, w, prec, gmode, CHUNK3D, solv, True, tt)
+        res = sp.chunked_step(net, x, w, prec, gmode, CHUNK3D, solv, True, tt, grams=grams_for(prec))
         net.adam(res["grads"], 1e-4, gate=solv.gate, on_gate=solv.raise_gate)
         return res, tt
 
@@ -510,15 +514,23 @@ def config_e(args, quick=False):
         L64 = r64["loss"]
         solv64 = sp.SpectralSolver(1e-2, ridge)
         _, dM64, dS64 = solv64(r64["M"], r64["S"])
+        M64, S64 = r64["M"], r64["S"]
         del r64
-        rs = sp.chunked_step(net, x, w, "bf16x3", "bf16x3", CHUNK3D, sp.SpectralSolver(1e-2, ridge))
-        rr = sp.chunked_step(net, x, w, "bf16x3", "bf16x3", CHUNK3D, lambda M, S: (L64, dM64, dS64))
+        rs = sp.chunked_step(net, x, w, "bf16x3", "bf16x3", CHUNK3D, sp.SpectralSolver(1e-2, ridge),
+                             grams=grams_for("bf16x3"))
+        rr = sp.chunked_step(net, x, w, "bf16x3", "bf16x3", CHUNK3D, lambda M, S: (L64, dM64, dS64),
+                             grams=grams_for("bf16x3"))
+        step_grams = dict(mass_gram_rel_err=float(torch.linalg.norm(rs["M"] - M64) / torch.linalg.norm(M64)),
+                          grad_u_gram_rel_err=float(torch.linalg.norm(rs["S"] - S64) / torch.linalg.norm(S64)),
+                          note="the split-precision step's Grams over all points (lift-weighted channel Grams) vs "
+                               "the FP64 chunked step's")
+        del M64, S64
 
         def cmp(g):
             g = g.double()
             return dict(rel_l2=float(torch.linalg.norm(g - g64) / torch.linalg.norm(g64)),
                         cosine=float(torch.dot(g, g64) / (torch.linalg.norm(g) * torch.linalg.norm(g64))))
-        grad_err = dict(whole_step=cmp(rs["grads"]), reverse_only=cmp(rr["grads"]),
+        grad_err = dict(whole_step=cmp(rs["grads"]), reverse_only=cmp(rr["grads"]), step_grams=step_grams,
                         note="split-precision chunked step vs the FP64 chunked step over all points; reverse_only "
                              "= split-precision tape and fields under the FP64 path's cotangents")
         del rs, rr, g64, dM64, dS64
@@ -619,7 +631,7 @@ def bench_3d_eig(args):
 
     def grams(prec, gmode):
         return sp.chunked_step(net, x, w, prec, gmode, CHUNK3D, solver=lambda M, S: (0.0, None, None),
-                               training=False)
+                               training=False, grams="factored" if prec == "bf16x3" else "fields")
 
     head = args.precision
     other_prec = "bf16" if head == "bf16x3" else "bf16x3"

@@ -855,10 +855,10 @@ def factored_channels(cfg: NetConfig) -> tuple[list, list]:
     return sorted({t[1] for t in mterms}), sorted({t[1] for t in sterms})
 
 
-def factored_products(net: EigenfluidNet, tape: Tape) -> FactoredProducts:
+def factored_products(net: EigenfluidNet, tape: Tape, grams: bool = True) -> FactoredProducts:
     """Stack the last hidden jet channels the lift reads (neb_stack_channels)
     and form their split-precision channel Grams (the products of the
-    output-weight gradient, unweighted)."""
+    output-weight gradient, unweighted); grams=False stacks only (Gh None)."""
     cfg = net.cfg
     width, n = cfg.widths[-1], tape.n
     mch, sch = factored_channels(cfg)
@@ -869,9 +869,36 @@ def factored_products(net: EigenfluidNet, tape: Tape) -> FactoredProducts:
     lo = torch.empty_like(hi)
     H.call("neb_stack_channels", Y.data_ptr(), Ylo.data_ptr(), n, Y.shape[2], width, len(chans), H.int_array(chans),
            hi.data_ptr(), lo.data_ptr(), ld, _stream())
+    if not grams:
+        return FactoredProducts(hi, lo, (mch, sch), None)
     km = len(mch) * width
     G = [gram(hi[:, :km], km, 1.0, "bf16x3", lo=lo[:, :km]), gram(hi[:, km:], ld - km, 1.0, "bf16x3", lo=lo[:, km:])]
     return FactoredProducts(hi, lo, (mch, sch), _factored_ghat(cfg, G))
+
+
+def factored_field_grams(net: EigenfluidNet, Gh: list, w: float, M: torch.Tensor | None = None,
+                         S: torch.Tensor | None = None):
+    """The field Grams from lift-weighted channel Grams (FactoredProducts.Gh,
+    summed over any point chunks): every mass and stiffness row of the
+    factored lifts is sum_t sg_t Y_c(t) W_s(t)^T over the lift's terms
+    (derivatives of the outputs, so no output bias), hence
+      M = w Wcat Ghat_mass Wcat^T,   S = w Wcat Ghat_stiff Wcat^T
+    with Wcat[k, s width + j] = W_out[ns k + s, j] (FP64; the K x K algebra
+    of velocity/core.py:40-46 on [ns width]^2 products instead of [n comp, K]
+    field matrices).  Writes into M / S when given; returns (M, S)."""
+    cfg = net.cfg
+    K, wd = cfg.modes, cfg.widths[-1]
+    ns = 3 if cfg.dim == 3 else 1
+    Wcat = net.W[-1].to(torch.float64).reshape(K, ns * wd)
+    out = []
+    for G, dst in zip(Gh, (M, S)):
+        R = torch.mm(torch.mm(Wcat, G), Wcat.T)
+        R = 0.5 * (R + R.T)
+        if dst is None:
+            out.append(R.mul_(w))
+        else:
+            out.append(dst.copy_(R).mul_(w))
+    return out[0], out[1]
 
 
 _FACT_S: dict = {}
@@ -1028,16 +1055,19 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
             fp = factored_products(net, tape)
         elif on_gpu and fp.event is not None:
             main.wait_event(fp.event)
-            for t in (fp.hi, fp.lo, *fp.Gh):
+            for t in (fp.hi, fp.lo, *(fp.Gh or [])):
                 t.record_stream(main)
         RTs = _factored_weights(net, pair, w)
         A = _factored_adjoint(net, tape, fp, RTs)
         if on_gpu:
             side.wait_stream(main)
-            for t in (pair, *fp.Gh):     # read on the side stream after this frame lets them go
+            for t in (pair, *(fp.Gh or [])):     # read on the side stream after this frame lets them go
                 t.record_stream(side)
         with (torch.cuda.stream(side) if on_gpu else nullcontext()):
-            gW[-1].copy_(_factored_output_grad(net, fp, pair, w))
+            if fp.Gh is None:      # stacked channels only: the caller adds the output-weight gradient
+                gW[-1].zero_()
+            else:
+                gW[-1].copy_(_factored_output_grad(net, fp, pair, w))
             gb[-1].zero_()
         del fp, RTs
     else:

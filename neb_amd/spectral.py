@@ -582,7 +582,7 @@ def _device_bytes_available(dev: torch.device) -> int:
 
 def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
                  gram_mode: str = "bf16x3", chunk: int = 131072, solver=None, training: bool = True,
-                 timings: dict | None = None, tape_budget: int | None = None):
+                 timings: dict | None = None, tape_budget: int | None = None, grams: str = "fields"):
     """Spectral step over point chunks, memory-bounded for large point sets
     (the structure of MB/velocity/core.py:79-105: fields -> grams ->
     objective -> recompute + reverse).  Pass 1 evaluates each chunk's jets
@@ -593,7 +593,15 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
     CHUNK_TAPE_HEADROOM); otherwise pass 2 recomputes each chunk's tape
     (the reference's per-chunk recomputation, VCORE:92-102: no tape outlives
     its chunk).  Returns dict(loss, M, S, grads, tapes_kept, tape_bytes), the
-    last the bytes all chunks' tapes together take."""
+    last the bytes all chunks' tapes together take.
+
+    grams="factored" (split-precision tapes of a factored lift, the 3D curl
+    lift): pass 1 forms no fields; it stacks each chunk's last hidden jet
+    channels and accumulates their lift-weighted channel Grams, and the field
+    Grams come from those once (nw.factored_field_grams: [3 width]^2
+    products instead of [n comp, K] field matrices); pass 2 stacks the
+    channels for the factored reverse without forming their Grams again and
+    adds the output-weight gradient of the accumulated ones once."""
     p = nw.PRECISIONS[prec]
     tt = {} if timings is None else timings
     K = net.cfg.modes
@@ -608,10 +616,25 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
         budget = min(budget, _device_bytes_available(net.device) - CHUNK_TAPE_HEADROOM)
     tapes = [] if training and budget > 0 else None
     kept_bytes = 0
+    if grams not in ("fields", "factored"):
+        raise ValueError(f"unknown grams {grams!r}")
+    factored = grams == "factored"
+    Gh = None
     with timed(tt, "pass1_forward_and_grams"):
         for i, c0 in enumerate(starts):
             tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
-            f = nw.output_fields(net, tape, split=split)
+            if factored:
+                if not nw.factored_reverse_ok(net, tape):
+                    raise ValueError("factored Grams need split-precision tapes of a factored lift")
+                fp = nw.factored_products(net, tape)
+                if Gh is None:
+                    Gh = fp.Gh
+                else:
+                    for acc, g in zip(Gh, fp.Gh):
+                        acc.add_(g)
+                del fp
+            else:
+                f = nw.output_fields(net, tape, split=split)
             if i == 0:
                 kept_bytes = nw.tape_bytes(tape) * len(starts)
             if tapes is not None:
@@ -620,8 +643,11 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
                 else:
                     tapes.append(tape)
             del tape
-            nw.field_grams(f, weight, gram_mode, M=M, S=S, beta=1.0)
-            del f
+            if not factored:
+                nw.field_grams(f, weight, gram_mode, M=M, S=S, beta=1.0)
+                del f
+        if factored:
+            nw.factored_field_grams(net, Gh, weight, M=M, S=S)
     with timed(tt, "spectral_objective"):
         loss, dM, dS = solver(M, S)
     out = dict(loss=loss, M=M, S=S, timings=tt, tapes_kept=tapes is not None, tape_bytes=kept_bytes)
@@ -635,8 +661,16 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
                     tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
                 # the factored reverse of a split-precision tape reads no fields
                 f = None if nw.factored_reverse_ok(net, tape) else nw.output_fields(net, tape, split=split)
-                grads += nw.backward_fields(net, tape, f, dM, dS, weight, symmetric=True)
-                del tape, f
+                prod = nw.factored_products(net, tape, grams=False) if factored else None
+                grads += nw.backward_fields(net, tape, f, dM, dS, weight, symmetric=True, products=prod)
+                del tape, f, prod
+            if factored:   # the output-weight gradient of all chunks' channel Grams
+                gW, _ = net.split(grads)
+                pair = nw._as_pair(dM, dS)
+                if pair is None:
+                    pair = torch.cat((dM, dS), 1)
+                gW[-1] += nw._factored_output_grad(net, nw.FactoredProducts(None, None, None, Gh), pair,
+                                                   weight).to(grads.dtype)
         out["grads"] = grads
     return out
 

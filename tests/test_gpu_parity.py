@@ -1062,3 +1062,28 @@ def test_chunked_split_precision_step_3d():
     r = rel(res["M"].cpu(), ref["M"].cpu()); assert r < 1e-6, r
     r = rel(res["S"].cpu(), ref["S"].cpu()); assert r < 1e-6, r
     r = rel(res["grads"].cpu(), ref["grads"].cpu()); assert r < 1e-2, r
+
+
+def test_chunked_step_factored_grams():
+    """Split-precision 3D chunked step with grams="factored" (no field
+    matrices: lift-weighted channel Grams accumulated over the chunks, then
+    M = w Wcat Ghat Wcat^T) against the same step with the field Grams and
+    the FP64 chunked step: Grams within 3e-5 of FP64, gradient within the
+    split step's own error of the field-Gram step."""
+    F, K = 2, 48
+    cfg, net, _ = make(3, K, [64, 64], F, master=torch.float32)
+    x = torch.tensor(points(3, 3000, seed=21))
+    w = 1.0 / 3000
+    same = nb.SpectralSolver(1e-2, 1e-3)
+    r64 = nb.chunked_step(net, x, w, prec="f64", gram_mode="f64", chunk=1024, solver=nb.SpectralSolver(1e-2, 1e-3))
+    rf = nb.chunked_step(net, x, w, prec="bf16x3", gram_mode="bf16x3", chunk=1024, solver=same)
+    rg = nb.chunked_step(net, x, w, prec="bf16x3", gram_mode="bf16x3", chunk=1024,
+                         solver=nb.SpectralSolver(1e-2, 1e-3), grams="factored")
+    for key in ("M", "S"):
+        r = rel(rg[key].cpu(), r64[key].cpu()); assert r < 3e-5, (key, r)
+        r = rel(rg[key].cpu(), rf[key].cpu()); assert r < 3e-5, (key, r)
+    r = rel(rg["grads"].cpu(), rf["grads"].cpu()); assert r < 2e-3, r
+    g64 = r64["grads"].cpu().double()
+    r = rel(rg["grads"].cpu(), g64); assert r < 2e-2, r
+    with pytest.raises(ValueError, match="factored"):
+        nb.chunked_step(net, x, w, prec="f32", gram_mode="f32", chunk=1024, grams="factored")
