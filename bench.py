@@ -191,7 +191,12 @@ class LaunchTimer:
         return rows
 
 
-def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=None, reverse_prec=None):
+def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=None, reverse_prec=None,
+              grams="fields"):
+    """One training step of config B.  grams="factored" (split-precision
+    jets): no field matrices -- stacked last-hidden channels, their split
+    channel Grams, the field Grams assembled from them in FP64, and the
+    factored reverse on the same stacked channels."""
     import torch
     K = net.cfg.modes
     solver = solver if solver is not None else sp.SpectralSolver(1e-2, ridge, deferred=True)
@@ -203,6 +208,15 @@ def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=
         r = fn()
         timer.stop(name)
         return r
+
+    def step_factored():
+        tape = mark(lambda: nw.trunk_forward(net, x, nw.PRECISIONS[prec]), "trunk_forward")
+        fp = mark(lambda: nw.factored_products(net, tape), "channel_grams")
+        M, S = mark(lambda: nw.factored_field_grams(net, fp.Gh, w), "field_grams")
+        loss, dM, dS = mark(lambda: solver(M, S), "objective_fp64")
+        g = mark(lambda: nw.backward_fields(net, tape, None, dM, dS, w, symmetric=True, products=fp), "reverse")
+        mark(lambda: net.adam(g, 1e-4, gate=solver.gate, on_gate=solver.raise_gate), "adam")
+        return loss
 
     def step():
         p = nw.PRECISIONS[prec]
@@ -225,7 +239,7 @@ def make_step(nb, nw, sp, net, x, w, prec, gram_mode, ridge, timer=None, solver=
         mark(lambda: net.adam(g, 1e-4, gate=solver.gate, on_gate=solver.raise_gate), "adam")
         return loss
 
-    return step
+    return step_factored if grams == "factored" else step
 
 
 def profile_traffic(name):
@@ -834,10 +848,21 @@ def accuracy_vs_fp64(nw, sp, net, x, K, w, ridge):
                               eigenvalue_max_rel_err_top=float(((lam[-top:] - lam64[-top:]).abs()
                                                                 / lam64[-top:].abs()).max()))
         del fp, U
+    # the headline step's Grams: split jets, channel Grams, FP64 assembly
+    tp = nw.trunk_forward(net, x, nw.PRECISIONS["bf16x3"])
+    prod = nw.factored_products(net, tp)
+    Mp, Sp = nw.factored_field_grams(net, prod.Gh, w)
+    del tp, prod
+    lam, _ = sp.generalized_eigh(Sp, Mp + ridge * eye)
+    accuracy["bf16x3_factored_grams"] = dict(
+        mass_gram_rel_err=float(torch.linalg.norm(Mp - M64) / torch.linalg.norm(M64)),
+        stiffness_gram_rel_err=float(torch.linalg.norm(Sp - S64) / torch.linalg.norm(S64)),
+        eigenvalue_max_rel_err_top=float(((lam[-top:] - lam64[-top:]).abs() / lam64[-top:].abs()).max()))
     accuracy["note"] = (f"vs the FP64 path on the same grid and weights; f32+bf16x3 = FP32 jets and fields, "
                         f"split-precision BF16 Grams (the fields and Grams of the fp32_fields_bf16_reverse step); "
                         f"bf16x3 = split-precision jets and fields (bf16 hi + lo planes, three bf16 MFMA products "
-                        f"per term) and split Grams (the split_precision_step); "
+                        f"per term) and split Grams; bf16x3_factored_grams = the headline step's Grams "
+                        f"(split jets, split channel Grams of the stacked last-hidden jets, FP64 assembly); "
                         f"eigenvalues of S x = lambda (M + ridge I) x, "
                         f"the top {top} (= rank M; the others are set by the ridge); bf16 velocities are the hi + lo "
                         f"planes of the fp32 field values")
@@ -1031,21 +1056,29 @@ def gradient_errors(nw, sp, net, x, w, ridge):
                     cosine=float(torch.dot(g, g64) / (torch.linalg.norm(g) * torch.linalg.norm(g64))))
 
     out = {}
-    for name, prec, gm, rp in (("bf16x3", "bf16x3", "bf16x3", None), ("bf16", "bf16", "bf16x3", None),
-                               ("f32+bf16x3", "f32", "bf16x3", "bf16"), ("f32", "f32", "f32", None)):
-        res = sp.compute(net, x, w, prec=prec, gram_mode=gm, ridge=ridge, gate="cholesky", reverse_prec=rp)
+    for name, prec, gm, rp, grams in (("bf16x3", "bf16x3", "bf16x3", None, "factored"),
+                                      ("bf16x3_field_grams", "bf16x3", "bf16x3", None, "fields"),
+                                      ("bf16", "bf16", "bf16x3", None, "fields"),
+                                      ("f32+bf16x3", "f32", "bf16x3", "bf16", "fields"),
+                                      ("f32", "f32", "f32", None, "fields")):
+        res = sp.compute(net, x, w, prec=prec, gram_mode=gm, ridge=ridge, gate="cholesky", reverse_prec=rp,
+                         grams=grams)
         whole = cmp(res["grads"])
         del res
         tape = nw.trunk_forward(net, x, nw.PRECISIONS[prec])
-        fields = nw.output_fields(net, tape, split=nw.split_fields_for(prec, gm))
+        if grams == "factored":
+            fields, prod = None, nw.factored_products(net, tape)
+        else:
+            fields, prod = nw.output_fields(net, tape, split=nw.split_fields_for(prec, gm)), None
         if rp is not None:
             tape = nw.convert_tape(tape, nw.PRECISIONS[rp])
-        g = nw.backward_fields(net, tape, fields, dM, dS, w, symmetric=True)
+        g = nw.backward_fields(net, tape, fields, dM, dS, w, symmetric=True, products=prod)
         out[name] = dict(whole_step=whole, reverse_only=cmp(g))
-        del tape, fields, g
+        del tape, fields, prod, g
     out["note"] = ("vs the FP64 path's gradient (FP64 jets, fields, Grams, objective and reverse) on the same grid, "
                    "weights and ridge; whole_step = the variant's own step, reverse_only = the variant's tape and "
-                   "fields under the FP64 path's cotangents")
+                   "fields (channel Grams for bf16x3, the headline step) under the FP64 path's cotangents; "
+                   "bf16x3_field_grams = split jets with field matrices, split field Grams and the fused reverse")
     del res64
     return out
 
@@ -1146,7 +1179,8 @@ def main():
     other_prec = "bf16" if head == "bf16x3" else "bf16x3"
     other = None
     if not args.no_f32:
-        stepo = make_step(nb, nw, sp, net, x, w, other_prec, "bf16x3", ridge)
+        stepo = make_step(nb, nw, sp, net, x, w, other_prec, "bf16x3", ridge,
+                          grams="factored" if other_prec == "bf16x3" else "fields")
         stepo()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -1161,11 +1195,12 @@ def main():
 
     # ---- headline step -----------------------------------------------------
     solver = sp.SpectralSolver(1e-2, ridge, deferred=True)
-    step = make_step(nb, nw, sp, net, x, w, head, "bf16x3", ridge, solver=solver)
+    head_grams = "factored" if head == "bf16x3" else "fields"
+    step = make_step(nb, nw, sp, net, x, w, head, "bf16x3", ridge, solver=solver, grams=head_grams)
     for _ in range(args.warmup):
         step()
     timer = StageTimer(torch)
-    step_t = make_step(nb, nw, sp, net, x, w, head, "bf16x3", ridge, timer, solver=solver)
+    step_t = make_step(nb, nw, sp, net, x, w, head, "bf16x3", ridge, timer, solver=solver, grams=head_grams)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -1187,18 +1222,27 @@ def main():
     nw.H.set_launch_recorder(None)
     kernels = launches.table(args.steps, dt_rec * 1e3)
 
-    # The split-precision mass Gram (R = 2N rows, K = 1024): algorithmic
-    # R*K*(K+1) flops (the symmetric half, multiply + add; the three split
-    # products are not credited) over its average duration, timed with events
-    # on its stream in the timed steps above (cross kernel + split reduction).
-    gram_ms = stages["gram_mass_launch"]
-    gram_flops = (2 * N) * K * (K + 1)
+    # The step's split-precision Gram launches (cross kernel + split
+    # reduction): algorithmic R*K*(K+1) flops (the symmetric half, multiply +
+    # add; the three split products are not credited) over their event-timed
+    # average duration -- the channel Grams (R = N points, K = 2 x 256 jet
+    # channels) on the factored path, the mass Gram (R = 2N rows, K = 1024)
+    # on the field path.
+    if "gram_mass_launch" in stages:
+        gram_ms = stages["gram_mass_launch"]
+        gram_flops = (2 * N) * K * (K + 1)
+        gram_kernel, gram_profile = ("neb_gram_split mass Gram: cross_async_kernel<2, 2, 2, 2, false> + "
+                                     "cross_reduce_kernel, R=131072, K=1024"), "gram_split_k1024_pmc.md"
+    else:
+        grow = max((r for r in kernels if r["kernel"].startswith("neb_gram_split")), key=lambda r: r["ms_per_step"])
+        gram_ms, gram_flops = grow["ms_per_launch"], grow["flops_per_launch"]
+        gram_kernel = (f"{grow['kernel']} channel Gram (K = 512 stacked jet channels): cross_async_kernel + "
+                       f"cross_reduce_kernel")
+        gram_profile = "gram_split_k512_r65536_pmc.md"
     achieved = gram_flops / (gram_ms * 1e-3)
     gram_roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK["bf16"] / 1e12, unit="TFLOP/s",
-                         frac=achieved / PEAK["bf16"], traffic=profile_traffic("gram_split_k1024_pmc.md"),
-                         kernel="neb_gram_split mass Gram: cross_async_kernel<2, 2, 2, 2, false> + "
-                                "cross_reduce_kernel, R=131072, K=1024", launch_ms=gram_ms,
-                         flops_per_launch=gram_flops)
+                         frac=achieved / PEAK["bf16"], traffic=profile_traffic(gram_profile),
+                         kernel=gram_kernel, launch_ms=gram_ms, flops_per_launch=gram_flops)
     # roofline of the dominant kernel: the kernel class with the largest share
     # of the step's time (event-timed launches of the timed steps above)
     dom = next(r for r in kernels if "bound" in r)
@@ -1213,9 +1257,12 @@ def main():
     out = {
         "metric": "point*modes/s, jet fwd+bwd + Gram spectral step, 2D K=1024",
         "value": N * K / dt,
-        "precision": (f"{head}: " + ("split-precision jets and fields (bf16 hi + lo planes, three bf16 MFMA "
-                                     "products per term, fp32 accumulation), split bf16x3 Grams, FP64 objective, "
-                                     "BF16 reverse; Grams within 1e-5 of the FP64 path (accuracy_vs_fp64['bf16x3'])"
+        "precision": (f"{head}: " + ("split-precision jets (bf16 hi + lo planes, three bf16 MFMA products per "
+                                     "term, fp32 accumulation); the field Grams from split channel Grams of the "
+                                     "stacked last-hidden jets, assembled in FP64 (no field matrices), FP64 "
+                                     "objective, factored reverse (FP64 output-layer algebra, split GEMM on the "
+                                     "tape); Grams within 1e-5 of the FP64 path "
+                                     "(accuracy_vs_fp64['bf16x3_factored_grams'])"
                                      if head == "bf16x3" else
                                      "BF16 jets and fields, split bf16x3 Grams, FP64 objective, BF16 reverse "
                                      "(accuracy_vs_fp64['bf16'])")),

@@ -848,6 +848,18 @@ def factored_reverse_ok(net: EigenfluidNet, tape: Tape) -> bool:
     return net.cfg.dim == 3 if mode == "auto" else mode == "factored"
 
 
+def factored_grams_ok(net: EigenfluidNet, tape: Tape) -> bool:
+    """Whether the field Grams of this tape can come from channel Grams
+    (factored_field_grams) and its reverse run factored with them: a
+    split-precision tape, a hidden width that is a multiple of 8 and a lift
+    whose rows are bias-free combinations of jet channels (every lift but the
+    streamfunction lift, whose psi rows carry the output bias and are
+    centered)."""
+    if tape.inputs_lo is None or net.cfg.widths[-1] % 8:
+        return False
+    return not (net.cfg.dim == 2 and net.cfg.stiffness == "stream")
+
+
 def factored_channels(cfg: NetConfig) -> tuple[list, list]:
     """The last hidden jet channels the mass and the stiffness fields read,
     in stacking order."""
@@ -1049,7 +1061,9 @@ def backward_fields(net: EigenfluidNet, tape: Tape, fields: Fields, dM: torch.Te
     side = side_stream(net.device) if on_gpu else None
     if on_gpu:
         side.wait_stream(main)
-    if factored_reverse_ok(net, tape):
+    # factored products handed in (the factored field Grams' stacked channels)
+    # select the factored reverse whatever NetConfig.split_reverse says
+    if isinstance(products, FactoredProducts) or factored_reverse_ok(net, tape):
         fp = products if isinstance(products, FactoredProducts) else None
         if fp is None:
             fp = factored_products(net, tape)

@@ -402,7 +402,7 @@ def timed(record: dict, name: str):
 def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: str = "bf16",
             gram_mode: str = "bf16x3", lam: float = 1e-2, ridge: float = 0.0, training: bool = True,
             gate: str = "eig", timings: dict | None = None, objective: str = "trace",
-            reverse_prec: str | None = None, peel_options: dict | None = None):
+            reverse_prec: str | None = None, peel_options: dict | None = None, grams: str = "fields"):
     """One spectral step (the analogue of MB/velocity/core.py:79-105
     ``compute``): jets -> lifted fields -> FP64 Grams -> spectral objective ->
     fused reverse.  objective="trace" is tr((M + ridge I)^-1 S) +
@@ -413,23 +413,40 @@ def compute(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, prec: st
     in another operand precision on the same fields and a converted tape:
     prec="f32", gram_mode="bf16x3", reverse_prec="bf16" keeps the fields and
     Grams at FP32 accuracy and runs the reverse on BF16 MFMA; peel_options
-    are keyword arguments of coefficient_peel.  Returns dict(loss, M, S,
+    are keyword arguments of coefficient_peel.  grams="factored" (split-
+    precision tapes, every lift but the streamfunction one) forms no fields:
+    the Grams come from the lift-weighted channel Grams of the last hidden
+    jets (nw.factored_field_grams) and the reverse is the factored one on the
+    same stacked channels ("fields" is then None).  Returns dict(loss, M, S,
     grads, fields)."""
     if objective not in ("trace", "peel"):
         raise ValueError(f"unknown objective {objective!r}")
+    if grams not in ("fields", "factored"):
+        raise ValueError(f"unknown grams {grams!r}")
     p = nw.PRECISIONS[prec]
     tt = {} if timings is None else timings
     K = net.cfg.modes
     split = nw.split_fields_for(prec, gram_mode)
+    factored = grams == "factored"
     with timed(tt, "neural_forward_analytic_derivatives"):
         tape = nw.trunk_forward(net, points, p)
-        fields = nw.output_fields(net, tape, split=split)
+        if factored:
+            if reverse_prec is not None or not nw.factored_grams_ok(net, tape):
+                raise ValueError("factored Grams need split-precision tapes of a factored lift (no reverse_prec)")
+            fields = None
+        else:
+            fields = nw.output_fields(net, tape, split=split)
     with timed(tt, "gram_assembly"):
-        M, S = nw.field_grams(fields, weight, gram_mode)
+        if factored:
+            prod = nw.factored_products(net, tape)
+            M, S = nw.factored_field_grams(net, prod.Gh, weight)
+        else:
+            M, S = nw.field_grams(fields, weight, gram_mode)
     rtape = tape if reverse_prec is None else nw.convert_tape(tape, nw.PRECISIONS[reverse_prec])
     # the output-layer products need only the fields and the tape: they run
     # on the side stream while the FP64 objective holds this one
-    prod = nw.output_products(net, rtape, fields) if training else None
+    if not factored:
+        prod = nw.output_products(net, rtape, fields) if training else None
     with timed(tt, "spectral_objective"):
         if objective == "peel":
             peel = coefficient_peel(M, S, **(peel_options or {}))
@@ -624,7 +641,7 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
         for i, c0 in enumerate(starts):
             tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
             if factored:
-                if not nw.factored_reverse_ok(net, tape):
+                if not nw.factored_grams_ok(net, tape):
                     raise ValueError("factored Grams need split-precision tapes of a factored lift")
                 fp = nw.factored_products(net, tape)
                 if Gh is None:
@@ -660,7 +677,7 @@ def chunked_step(net: nw.EigenfluidNet, points: torch.Tensor, weight: float, pre
                 else:
                     tape = nw.trunk_forward(net, points[c0:c0 + chunk], p)
                 # the factored reverse of a split-precision tape reads no fields
-                f = None if nw.factored_reverse_ok(net, tape) else nw.output_fields(net, tape, split=split)
+                f = None if factored or nw.factored_reverse_ok(net, tape) else nw.output_fields(net, tape, split=split)
                 prod = nw.factored_products(net, tape, grams=False) if factored else None
                 grads += nw.backward_fields(net, tape, f, dM, dS, weight, symmetric=True, products=prod)
                 del tape, f, prod
