@@ -10,7 +10,8 @@ compared with the FP64 oracle:
      fields on a stratified 2,048-point subset against the oracle, the
      Grams against the FP64 Gram of the same fields, the FP64 generalized
      eigendecomposition against LAPACK on identical FP64 Grams, and the
-     reverse pass at full size against the oracle (below).
+     reverse pass at full size against the oracle (below); the whole
+     headline step (factored Grams) against the FP64 path.
   C  2D differentiation step: tests/test_config_c.py.
   D/E  3D blob domain, 4x256 tanh, K=512 / 1024 vector potential (3K
      outputs), one full 131,072-point chunk of the 1,599,754-point set --
@@ -297,6 +298,38 @@ def test_config_B_reverse_full_size(prec):
     check_grads(net, g, gW, gb, REV_TOL_FACTORED if fact else REV_TOL[prec],
                 f"config B reverse {prec}{' factored' if fact else ''}")
     del tape, f, g
+    torch.cuda.empty_cache()
+
+
+def test_config_B_headline_step_factored_grams_vs_fp64():
+    """The headline step at full size (65,536-point grid, 4x256 tanh, K=1024,
+    split-precision jets, field Grams from the lift-weighted channel Grams,
+    factored reverse) against the FP64 path on the same grid, weights and
+    ridge: Grams within 1e-5 (the split-precision Gram bound), loss within
+    1e-4, whole-step gradient within 5e-3 relative L2 (measured 1.5e-3; the
+    objective amplifies the Grams' 7e-6 by up to cond(M + ridge I))."""
+    cfg = cfg_2d((256,) * 4, 1024)
+    net = nb.EigenfluidNet(cfg, seed=0, device=DEV, master_dtype=torch.float32)
+    x = torch.tensor(syn.periodic_grid(256), dtype=torch.float32, device=DEV)
+    w = syn.grid_weight(256)
+
+    def trel(a, b):
+        return float(torch.linalg.norm(a.double() - b.double()) / torch.linalg.norm(b.double()))
+
+    M0 = nb.compute(net, x, w, prec="f64", gram_mode="f64", training=False, gate="cholesky", ridge=1.0)["M"]
+    ridge = 1e-4 * float(torch.diagonal(M0).mean())
+    del M0
+    ref = nb.compute(net, x, w, prec="f64", gram_mode="f64", ridge=ridge, gate="cholesky")
+    res = nb.compute(net, x, w, prec="bf16x3", gram_mode="bf16x3", ridge=ridge, gate="cholesky", grams="factored")
+    assert trel(res["M"], ref["M"]) < 1e-5 and trel(res["S"], ref["S"]) < 1e-5
+    assert abs(float(res["loss"]) - float(ref["loss"])) < 1e-4 * abs(float(ref["loss"]))
+    g, g64 = res["grads"].double(), ref["grads"].double()
+    err = float(torch.linalg.norm(g - g64) / torch.linalg.norm(g64))
+    assert err < 5e-3, err
+    # the same step through the Grams of the K-column fields agrees with it
+    fld = nb.compute(net, x, w, prec="bf16x3", gram_mode="bf16x3", ridge=ridge, gate="cholesky")
+    assert trel(fld["M"], res["M"]) < 2e-5
+    del ref, res, fld
     torch.cuda.empty_cache()
 
 
