@@ -171,22 +171,39 @@ __global__ __launch_bounds__(256) void mode_cross_kernel(ModeCrossArgs a) {
   for (int e = 0; e < 8; ++e) o[e] = a.scale * acc[e];
 }
 
-// out[c] = sum_s partial[s][c] in fixed order
+// out[c] = beta out[c] + sum_s partial[s][c] in a fixed order: SS_COLS
+// columns per workgroup, SS_LANES split lanes per column, lane l summing
+// splits l, l + SS_LANES, ... in order (8 loads in flight), then the lanes
+// combined in lane order through LDS.  The order depends only on `splits`.
+constexpr int SS_COLS = 16, SS_LANES = 16;
+inline dim3 sum_splits_grid(int64_t cols) { return dim3((unsigned)((cols + SS_COLS - 1) / SS_COLS)); }
+
 __global__ __launch_bounds__(256) void sum_splits_kernel(const double* __restrict__ partial, int splits,
                                                          int64_t cols, double* __restrict__ out, double beta) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+  static_assert(SS_COLS * SS_LANES == 256, "one thread per (column, lane)");
+  __shared__ double red[SS_LANES][SS_COLS];
+  const int cl = threadIdx.x % SS_COLS, lane = threadIdx.x / SS_COLS;
+  const int64_t c = (int64_t)blockIdx.x * SS_COLS + cl;
   double s = 0.0;
-  int q = 0;
-  for (; q + 8 <= splits; q += 8) {  // 8 independent loads in flight, added in order
-    double t[8];
+  if (c < cols) {
+    int q = lane;
+    for (; q + 7 * SS_LANES < splits; q += 8 * SS_LANES) {
+      double t[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) t[u] = partial[(int64_t)(q + u) * cols + c];
+      for (int u = 0; u < 8; ++u) t[u] = partial[(int64_t)(q + u * SS_LANES) * cols + c];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) s += t[u];
+      for (int u = 0; u < 8; ++u) s += t[u];
+    }
+    for (; q < splits; q += SS_LANES) s += partial[(int64_t)q * cols + c];
   }
-  for (; q < splits; ++q) s += partial[(int64_t)q * cols + c];
-  out[c] = (beta == 0.0 ? 0.0 : beta * out[c]) + s;
+  red[lane][cl] = s;
+  __syncthreads();
+  if (lane == 0 && c < cols) {
+    double t = red[0][cl];
+#pragma unroll
+    for (int l = 1; l < SS_LANES; ++l) t += red[l][cl];
+    out[c] = (beta == 0.0 ? 0.0 : beta * out[c]) + t;
+  }
 }
 
 // Sum of squares in FP64 (gradient norm for the Adam clip, NCC:157).

@@ -575,13 +575,13 @@ template <typename T> int colsum_launch(const void* x, int64_t rows, int64_t col
   hipLaunchKernelGGL(colsum_kernel<T>, dim3(splits, (unsigned)slabs), dim3(256), 0, st, reinterpret_cast<const T*>(x),
                      ld, rows, cols, chunk, part);
   if (int rc = hip_status("neb_colsum")) return rc;
-  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, st, part, splits, cols,
+  hipLaunchKernelGGL(sum_splits_kernel, sum_splits_grid(cols), dim3(256), 0, st, part, splits, cols,
                      out, beta);
   return hip_status("neb_colsum reduce");
 }
 
 int colsum_splits(int64_t rows) {
-  int64_t s = (rows + 255) / 256;
+  int64_t s = (rows + 127) / 128;
   if (s > 1024) s = 1024;
   if (s < 1) s = 1;
   return (int)s;
@@ -734,7 +734,7 @@ int neb_cross_modes(int nseg, const void* const* x, const void* const* y, const 
   }
 #undef MCK
   if (int rc = hip_status("neb_cross_modes")) return rc;
-  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((B * h + 255) / 256)), dim3(256), 0, st, a.partial,
+  hipLaunchKernelGGL(sum_splits_kernel, sum_splits_grid(B * h), dim3(256), 0, st, a.partial,
                      (int)ch.first, B * h, out, beta);
   return hip_status("neb_cross_modes reduce");
 }
@@ -1139,7 +1139,7 @@ int neb_sumsq(int dtype, const void* x, int64_t n, double* out, void* workspace,
     default: return fail(NEB_ERR_ARG, "neb_sumsq: unknown dtype %d", dtype);
   }
   if (int rc = hip_status("neb_sumsq")) return rc;
-  hipLaunchKernelGGL(sum_splits_kernel, dim3(1), dim3(256), 0, st, part, 256, (int64_t)1, out, 0.0);
+  hipLaunchKernelGGL(sum_splits_kernel, sum_splits_grid(1), dim3(256), 0, st, part, 256, (int64_t)1, out, 0.0);
   return hip_status("neb_sumsq reduce");
 }
 
@@ -1338,7 +1338,7 @@ int neb_galerkin_rhs2d(int dtype, const void* gu, const void* gv, const void* gw
     hipLaunchKernelGGL(rhs2d_cols_kernel<double>, gc, dim3(256), 0, st, u, v, ld, n, K, chunk, r1, r2, part);
   }
   if (int rc = hip_status("neb_galerkin_rhs2d columns")) return rc;
-  hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, st, part, sp, K, rhs, 0.0);
+  hipLaunchKernelGGL(sum_splits_kernel, sum_splits_grid(K), dim3(256), 0, st, part, sp, K, rhs, 0.0);
   return hip_status("neb_galerkin_rhs2d reduce");
 }
 
