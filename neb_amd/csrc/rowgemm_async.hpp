@@ -82,6 +82,12 @@ template <> struct RowAsyncShape<PlanOutFields2Split> {
   static constexpr int FB = 4, PB = 1, WPN = 4, WPP = 2;
 };
 template <int D, int ACT> struct RowAsyncShape<PlanHiddenBwd<D, ACT>> : WideHiddenShape<D> {};
+// split GEMM of the factored reverse (65,536 x 512 x 512 per field group):
+// 8 waves on a 256 x 64 tile, three 40 KB slots (0.210 -> 0.200 ms against
+// the plan's 4-wave 128 x 64 tile; 128 x 128 and 256 x 32 tiles were slower)
+template <> struct RowAsyncShape<PlanGemmSplit> {
+  static constexpr int FB = 4, PB = 2, WPN = 4, WPP = 2;
+};
 
 // First term of a plan that reads input plane c (1 << 20: none).
 template <class Plan>
