@@ -1,8 +1,8 @@
 """Pins for the FP64 oracle: closed forms, finite differences, library routines.
 
 Each test is chosen so that a plausible slip in the oracle (a dropped term, a
-wrong sign or index, a transposed operand) fails it.  The reference has no
-stored arrays: its known-answer tests are restated here --
+wrong sign or index, a transposed operand) fails it.  The reference stores
+no arrays for this path: its known-answer tests are restated here --
 experiments/2d/_backend/periodic_core/vortex_merger/canonical_gauge/scripts/
 verify_analytic.py:48-66,89-97 (Fourier basis Gram = I), tests/
 test_metaball_operators.py:22 (blocked Gram = dense), experiments/2d/
